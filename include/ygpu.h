@@ -1,0 +1,189 @@
+/*
+ * ygpu.h -- C-ABI of the MI355X SSA/DIVA velocity solver (libygpu.so).
+ *
+ * Drop-in boundary for Yelmo's velocity-solver call chain.  Two levels:
+ *
+ *   inner  ygpu_csr_solve      replaces linear_solver_matrix_solve
+ *                              (src/physics/solver_linear.F90:151-190, the LIS
+ *                              call at :194-400): a CSR system as the
+ *                              reference assembles it, solved on the GPU.
+ *   outer  ygpu_velocity_ssa   replaces calc_velocity_ssa
+ *                              (src/physics/velocity_ssa.f90:60-357): the whole
+ *                              Picard loop resident on the device.
+ *          ygpu_velocity_diva  replaces calc_velocity_diva
+ *                              (src/physics/velocity_diva.f90:59-429).
+ *          ygpu_velocity_ssa_batch / ygpu_velocity_diva_batch
+ *                              many independent domains in one call.
+ *
+ * Conventions follow the reference's C API (docs/c-api.md:23-38): plain
+ * pointers, Fortran column-major (nx,ny[,nz]) arrays with i fastest, 3D
+ * arrays with k slowest and k = 0 at the base.  The caller owns every host
+ * array; the library copies in and out and keeps no host pointer after a call
+ * returns.  Calls never abort: they return YGPU_OK, a negative error code
+ * (text from ygpu_last_error), and report solver outcomes through LIS-style
+ * status values (0 converged, 2 breakdown, 4 iteration limit).
+ * One handle per host thread; handles share nothing.
+ */
+#ifndef YGPU_H
+#define YGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define YGPU_OK 0
+#define YGPU_ERR_ARG (-1)         /* invalid argument / option string          */
+#define YGPU_ERR_DEVICE (-2)      /* HIP runtime failure                       */
+#define YGPU_ERR_UNSUPPORTED (-3) /* matrix pattern / option not supported     */
+#define YGPU_ERR_NONFINITE (-6)   /* non-finite velocity (the reference's
+                                     error stop, velocity_general.f90:2114-2152) */
+
+/* boundaries strings (src/yelmo_tools.f90:248-271) */
+#define YGPU_BND_ZEROS 0
+#define YGPU_BND_INFINITE 1
+#define YGPU_BND_MASK 2
+#define YGPU_BND_MISMIP3D 3
+#define YGPU_BND_TROUGH 4
+#define YGPU_BND_PERIODIC 5
+#define YGPU_BND_PERIODIC_X 6
+#define YGPU_BND_PERIODIC_Y 7
+
+/* arithmetic of the outer solve */
+#define YGPU_PREC_FP64 0  /* FP64 fields, matrix and Krylov (plain FP64 path)          */
+#define YGPU_PREC_REF 1   /* FP32 fields, FP64 matrix and Krylov (the reference, wp=sp) */
+#define YGPU_PREC_MIXED 2 /* FP32 fields; FP32 SpMV/ILU inside FP64 refinement          */
+
+typedef struct ygpu_handle ygpu_handle;
+
+/* Mirrors ssa_param_class (velocity_ssa.f90:22-52) and diva_param_class
+ * (velocity_diva.f90:22-51); strings become the enums above.             */
+typedef struct ygpu_ssa_params {
+  int32_t ssa_solver;     /* 0 "residual" (1 "energy": not available)           */
+  int32_t boundaries;     /* YGPU_BND_*                                          */
+  int32_t visc_method;    /* 0, 1 (quadrature), 2 (aa-nodes)                     */
+  int32_t beta_method;    /* -1 .. 5                                             */
+  int32_t beta_gl_scale;  /* 0, 1, 3                                             */
+  int32_t beta_gl_stag;   /* -1, 0, 1, 2                                         */
+  int32_t ssa_vel_lim_method; /* 0 "clip", 1 "drag"                              */
+  int32_t ssa_iter_max;
+  int32_t ssa_write_log;
+  int32_t precision;      /* YGPU_PREC_*                                         */
+  double visc_const, beta_const, beta_q, beta_u0, beta_gl_f, H_grnd_lim, beta_min, eps_0,
+      ssa_vel_lim_tau, ssa_vel_max, ssa_iter_rel, ssa_iter_conv, rho_ice, rho_sw, g;
+  int32_t no_slip;        /* DIVA: "diva-noslip"                                 */
+  int32_t block;          /* threads per member block (0: pick automatically)   */
+  char ssa_lis_opt[256];  /* LIS option string, e.g.
+                             "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false";
+                             extensions: -ir_inner_tol T, -ir_max N              */
+} ygpu_ssa_params;
+
+/* One domain of a batch: the arguments of calc_velocity_ssa (and the extra
+ * ones of calc_velocity_diva).  real_bytes = 4 for float arrays (Yelmo's
+ * wp = sp), 8 for double arrays.  Unused DIVA pointers may be NULL for SSA. */
+typedef struct ygpu_ssa_problem {
+  int32_t nx, ny, nz, real_bytes;
+  double dx, dy, n_glen;
+  /* INOUT */
+  void *ux_b, *uy_b, *taub_acx, *taub_acy, *visc_eff, *visc_eff_int;
+  int32_t *ssa_mask_acx, *ssa_mask_acy;
+  void *ssa_err_acx, *ssa_err_acy;
+  void *beta, *beta_acx, *beta_acy;
+  /* IN */
+  const void *c_bed, *f_slide, *taud_acx, *taud_acy, *taul_int_acx, *taul_int_acy, *H_ice, *f_ice,
+      *H_grnd, *f_grnd, *f_grnd_acx, *f_grnd_acy, *ATT, *zeta_aa, *z_sl, *z_bed, *z_srf;
+  /* DIVA INOUT (NULL for SSA) */
+  void *ux, *uy, *ux_bar, *uy_bar, *ux_i, *uy_i, *beta_eff, *duxdz, *duydz;
+  /* OUT */
+  int32_t ssa_iter_now, ssa_lin_iter, ssa_lin_fail, ssa_lim_n;
+  int32_t status;         /* YGPU_OK or YGPU_ERR_NONFINITE for this domain       */
+  int32_t pad;
+  double picard_resid;    /* last Picard residual                                 */
+  double solve_seconds;   /* device time of the whole batch call                   */
+} ygpu_ssa_problem;
+
+/* ---- handle ------------------------------------------------------------ */
+int ygpu_create(int32_t device, ygpu_handle** out);
+int ygpu_destroy(ygpu_handle* h);
+const char* ygpu_last_error(const ygpu_handle* h);
+/* fills params with Yelmo's &ydyn defaults (input/yelmo_defaults.nml:140-173)
+ * and input/yelmo_phys_const.nml:6-10 (no handle needed)                     */
+void ygpu_default_params(ygpu_ssa_params* par);
+
+/* ---- inner boundary: replaces linear_solver_matrix_solve
+ * (solver_linear.F90:151).  n rows, CSR arrays 1-based as in lgs%a_ptr /
+ * a_index / a_value; x_inout is x0 on entry (unless -initx_zeros true) and the
+ * solution on return.  OUT: iters, status (0/2/4), rel_resid
+ * (||b-Ax||/||r0||, LIS conv_cond 0), seconds (device solve time).
+ * Supported patterns: the C-grid SSA stencil of linear_solver_init's ordering
+ * (solver_linear.F90:134-143), non-periodic.  Option grammar: LIS's -i, -p,
+ * -maxiter, -tol, -initx_zeros, -conv_cond; extensions -ygpu_ny NY (grid hint),
+ * -ygpu_precision fp64|mixed.                                                */
+int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
+                   const int32_t* a_idx1, const double* a_val, const double* b, double* x_inout,
+                   const char* opts, int32_t* iters, int32_t* status, double* rel_resid,
+                   double* seconds);
+
+/* ---- outer boundary: calc_velocity_ssa (velocity_ssa.f90:60-63), argument
+ * order as in the reference, wp = sp arrays.                                  */
+int ygpu_velocity_ssa(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int32_t ny,
+                      int32_t nz, float* ux_b, float* uy_b, float* taub_acx, float* taub_acy,
+                      float* visc_eff, float* visc_eff_int, int32_t* ssa_mask_acx,
+                      int32_t* ssa_mask_acy, float* ssa_err_acx, float* ssa_err_acy,
+                      int32_t* ssa_iter_now, int32_t* ssa_lin_iter, int32_t* ssa_lin_fail,
+                      int32_t* ssa_lim_n, float* beta, float* beta_acx, float* beta_acy,
+                      const float* c_bed, const float* f_slide, const float* taud_acx,
+                      const float* taud_acy, const float* taul_int_acx,
+                      const float* taul_int_acy, const float* H_ice, const float* f_ice,
+                      const float* H_grnd, const float* f_grnd, const float* f_grnd_acx,
+                      const float* f_grnd_acy, const float* ATT, const float* zeta_aa,
+                      const float* z_sl, const float* z_bed, const float* z_srf, float dx,
+                      float dy, float n_glen);
+
+/* ---- many independent domains (ensembles, uneven batches) in one call.
+ * Every domain runs the full Picard loop; per-domain outcomes are written into
+ * probs[k] (ssa_iter_now ... status).  Returns YGPU_OK unless an argument or
+ * the device failed.                                                          */
+int ygpu_velocity_ssa_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
+                            ygpu_ssa_problem* probs);
+
+/* ---- calc_velocity_diva (velocity_diva.f90:59-429), batched form.         */
+int ygpu_velocity_diva_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
+                             ygpu_ssa_problem* probs);
+
+/* ---- resident sessions (warm starts for served solve sequences).
+ * A session keeps one domain's fields, velocity, viscosity and workspace on
+ * the device between solves.  ygpu_session_solve uploads the IN fields of
+ * prob, keeps the device-resident INOUT state (velocity, viscosity, beta) as
+ * the warm start, runs the Picard loop and downloads the results into prob.
+ * Sessions of one handle run on that handle's stream in call order.          */
+typedef struct ygpu_session ygpu_session;
+int ygpu_session_create(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob,
+                        ygpu_session** out, int64_t* device_bytes);
+int ygpu_session_solve(ygpu_session* s, ygpu_ssa_problem* prob);
+int ygpu_session_destroy(ygpu_session* s);
+
+/* ---- kernel statistics of a handle (for roofline accounting).  When
+ * enabled, every launch of the persistent Krylov kernel is bracketed by HIP
+ * events on the handle's stream; algorithmic bytes are counted from the
+ * matrix size and the Krylov iterations each launch performed.              */
+typedef struct ygpu_stats {
+  int64_t krylov_launches;
+  double krylov_ms;             /* summed event time of those launches          */
+  double krylov_bytes;          /* algorithmic HBM bytes of those launches      */
+  int64_t krylov_iters;         /* Krylov iterations summed over members        */
+  double unknown_iters;         /* sum over members of N * iterations           */
+  int64_t picard_iters;         /* Picard iterations summed over members        */
+} ygpu_stats;
+int ygpu_stats_enable(ygpu_handle* h, int32_t on);
+int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out);  /* returns and resets     */
+
+/* device bytes a domain of this size needs (admission control)               */
+int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny, int32_t nz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YGPU_H */

@@ -1,0 +1,255 @@
+"""ctypes driver of the CPU oracle (oracle/ssa_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker -- never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_libs = {}
+
+
+def build():
+    r = subprocess.run(["make", "-C", str(HERE), "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
+def lib(dp: bool) -> C.CDLL:
+    key = "dp" if dp else "sp"
+    if key not in _libs:
+        path = HERE / f"liboracle_{key}.so"
+        if not path.exists():
+            build()
+        _libs[key] = C.CDLL(str(path))
+        assert _libs[key].oracle_wp_bytes() == (8 if dp else 4)
+    return _libs[key]
+
+
+class OracleParams(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "ssa_solver", "boundaries", "visc_method", "beta_method", "beta_gl_scale",
+        "beta_gl_stag", "vel_lim_method", "ssa_iter_max", "ssa_write_log", "pad0")]
+    _fields_ += [(n, C.c_double) for n in (
+        "visc_const", "beta_const", "beta_q", "beta_u0", "beta_gl_f", "H_grnd_lim", "beta_min",
+        "eps_0", "ssa_vel_lim_tau", "ssa_vel_max", "ssa_iter_rel", "ssa_iter_conv", "rho_ice",
+        "rho_sw", "g")]
+
+
+BND = {"zeros": 0, "infinite": 1, "mask": 2, "MISMIP3D": 3, "TROUGH": 4, "periodic": 5,
+       "periodic-x": 6, "periodic-y": 7}
+
+
+def params_c(par) -> OracleParams:
+    c = OracleParams()
+    c.ssa_solver = 0
+    c.boundaries = BND[par.boundaries]
+    c.visc_method = par.visc_method
+    c.beta_method = par.beta_method
+    c.beta_gl_scale = par.beta_gl_scale
+    c.beta_gl_stag = par.beta_gl_stag
+    c.vel_lim_method = {"clip": 0, "drag": 1}[par.ssa_vel_lim_method]
+    c.ssa_iter_max = par.ssa_iter_max
+    c.ssa_write_log = int(par.ssa_write_log)
+    for n in ("visc_const", "beta_const", "beta_q", "beta_u0", "beta_gl_f", "H_grnd_lim",
+              "beta_min", "eps_0", "ssa_vel_lim_tau", "ssa_vel_max", "ssa_iter_rel",
+              "ssa_iter_conv", "rho_ice", "rho_sw", "g"):
+        setattr(c, n, float(getattr(par, n)))
+    return c
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def calc_velocity_ssa(prob, par, dp: bool):
+    """Runs the oracle Picard loop on a copy of prob (fields cast to the
+    build's precision).  Returns (state dict, info dict)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    P = prob.astype(dt)
+    st, fl = P.state, P.fields
+    out_i = np.zeros(4, dtype=np.int32)
+    rh = np.zeros(par.ssa_iter_max, dtype=np.float64)
+    lh = np.zeros(par.ssa_iter_max, dtype=np.int32)
+    secs = C.c_double()
+    W = C.c_double if dp else C.c_float
+    L.oracle_calc_velocity_ssa.restype = C.c_int
+    args = [st["ux_b"], st["uy_b"], st["taub_acx"], st["taub_acy"], st["visc_eff"],
+            st["visc_eff_int"], st["ssa_mask_acx"], st["ssa_mask_acy"], st["ssa_err_acx"],
+            st["ssa_err_acy"], out_i, st["beta"], st["beta_acx"], st["beta_acy"],
+            fl["c_bed"], fl["f_slide"], fl["taud_acx"], fl["taud_acy"], fl["taul_int_acx"],
+            fl["taul_int_acy"], fl["H_ice"], fl["f_ice"], fl["H_grnd"], fl["f_grnd"],
+            fl["f_grnd_acx"], fl["f_grnd_acy"], fl["ATT"], fl["zeta_aa"], fl["z_sl"],
+            fl["z_bed"], fl["z_srf"]]
+    pc = params_c(par)
+    rc = L.oracle_calc_velocity_ssa(
+        *[_p(a) for a in args], W(P.dx), W(P.dy), W(P.n_glen), C.byref(pc),
+        par.ssa_lis_opt.encode(), P.nx, P.ny, P.nz, _p(rh), _p(lh), C.byref(secs))
+    info = dict(rc=rc, ssa_iter_now=int(out_i[0]), ssa_lin_iter=int(out_i[1]),
+                ssa_lin_fail=int(out_i[2]), ssa_lim_n=int(out_i[3]),
+                resid_hist=rh[: max(out_i[0], 0)].copy(), lin_hist=lh[: max(out_i[0], 0)].copy(),
+                solve_seconds=secs.value)
+    return st, info
+
+
+def csr_solve(a_ptr1, a_idx1, a_val, b, x0, opts: str):
+    L = lib(True)
+    n = len(b)
+    ap = np.ascontiguousarray(a_ptr1, dtype=np.int32)
+    ai = np.ascontiguousarray(a_idx1, dtype=np.int32)
+    av = np.ascontiguousarray(a_val, dtype=np.float64)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    it, st = C.c_int32(), C.c_int32()
+    res, secs = C.c_double(), C.c_double()
+    L.oracle_csr_solve.restype = C.c_int
+    rc = L.oracle_csr_solve(n, _p(ap), _p(ai), _p(av), _p(bb), _p(x), opts.encode(), C.byref(it),
+                            C.byref(st), C.byref(res), C.byref(secs))
+    return dict(rc=rc, x=x, iters=it.value, status=st.value, resid=res.value, seconds=secs.value)
+
+
+def ilu0_apply(a_ptr1, a_idx1, a_val, v, nb=1):
+    L = lib(True)
+    n = len(v)
+    z = np.zeros(n)
+    keep = [np.ascontiguousarray(a_ptr1, dtype=np.int32),
+            np.ascontiguousarray(a_idx1, dtype=np.int32),
+            np.ascontiguousarray(a_val, dtype=np.float64), np.ascontiguousarray(v, dtype=np.float64)]
+    rc = L.oracle_ilu0_apply(n, *[_p(a) for a in keep[:3]], nb, _p(keep[3]), _p(z))
+    assert rc == 0, rc
+    return z
+
+
+def assemble(prob, beta_acx, beta_acy, N_aa, boundaries="zeros", lim=None, dp=True):
+    """The oracle's residual assembly on given fields; returns 1-based CSR
+    (a_ptr, a_index, a_value, b, x0) and N_ab."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = prob.nx, prob.ny
+    nc = nx * ny
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=dt))
+
+    class Lgs(C.Structure):
+        _fields_ = [("nmax", C.c_int), ("a_ptr", C.c_void_p), ("a_index", C.c_void_p),
+                    ("a_value", C.c_void_p), ("b_value", C.c_void_p), ("x_value", C.c_void_p)]
+
+    ap = np.zeros(2 * nc + 1, dtype=np.int32)
+    ai = np.zeros(18 * nc, dtype=np.int32)
+    av = np.zeros(18 * nc)
+    bv = np.zeros(2 * nc)
+    xv = np.zeros(2 * nc)
+    lgs = Lgs(2 * nc, ap.ctypes.data, ai.ctypes.data, av.ctypes.data, bv.ctypes.data,
+              xv.ctypes.data)
+    st, fl = prob.state, prob.fields
+    args = [F(st["ux_b"]), F(st["uy_b"]), F(beta_acx), F(beta_acy), F(N_aa)]
+    masks = [np.asfortranarray(st["ssa_mask_acx"], dtype=np.int32),
+             np.asfortranarray(st["ssa_mask_acy"], dtype=np.int32)]
+    rest = [F(fl["H_ice"]), F(fl["f_ice"]), F(fl["z_bed"]), F(fl["z_sl"]), F(fl["taud_acx"]),
+            F(fl["taud_acy"]), F(fl["taul_int_acx"]), F(fl["taul_int_acy"])]
+    W = C.c_double if dp else C.c_float
+    Nab = np.zeros((nx, ny), dtype=dt, order="F")
+    if lim is None:
+        limp = [None] * 4
+    else:
+        limp = [F(a) for a in lim]
+    L.oracle_assemble_ssa.restype = C.c_int
+    k = L.oracle_assemble_ssa(C.byref(lgs), *[_p(a) for a in args], *[_p(a) for a in masks],
+                              *[_p(a) for a in rest], W(prob.dx), W(prob.dy), BND[boundaries],
+                              *[(_p(a) if a is not None else None) for a in limp], _p(Nab), nx, ny)
+    return ap, ai[:k], av[:k], bv, xv, Nab
+
+
+def set_ssa_masks(mask_frnt, f_ice, f_grnd, lateral_bc="all", boundaries="zeros", dp=False):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = f_ice.shape
+    mx = np.zeros((nx, ny), dtype=np.int32, order="F")
+    my = np.zeros((nx, ny), dtype=np.int32, order="F")
+    lat = {"none": 0, "floating": 1, "float": 1, "marine": 2, "all": 3}[lateral_bc]
+    keep = [np.asfortranarray(mask_frnt, dtype=np.int32), np.asfortranarray(f_ice, dtype=dt),
+            np.asfortranarray(f_grnd, dtype=dt)]
+    L.oracle_set_ssa_masks(_p(mx), _p(my), *[_p(a) for a in keep], lat, nx, ny, BND[boundaries])
+    return mx, my
+
+
+def visc_eff_3D_aa(ux, uy, ATT, f_ice, dx, dy, n_glen, eps_0, boundaries="zeros", dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = C.c_double if dp else C.c_float
+    nx, ny, nz = ATT.shape
+    visc = np.zeros((nx, ny, nz), dtype=dt, order="F")
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=dt))
+    keep = [F(ux), F(uy), F(ATT), F(f_ice)]
+    L.oracle_visc_eff_3D_aa(_p(visc), *[_p(a) for a in keep], W(dx), W(dy), W(n_glen), W(eps_0),
+                            nx, ny, nz, BND[boundaries])
+    return visc
+
+
+def strain_rate_2D(ux, uy, f_ice, dx, dy, boundaries="zeros", dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = C.c_double if dp else C.c_float
+    nx, ny = f_ice.shape
+    outs = [np.zeros((nx, ny), dtype=dt, order="F") for _ in range(4)]
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=dt))
+    keep = [F(ux), F(uy), F(f_ice)]
+    L.oracle_strain_rate_2D(*[_p(o) for o in outs], *[_p(a) for a in keep], W(dx), W(dy), nx, ny,
+                            BND[boundaries])
+    return outs
+
+
+def ilu0_factors(a_ptr1, a_idx1, a_val, nb=1):
+    """ILU(0) factors (0-based sorted CSR, unit L below the diagonal, U on
+    and above it)."""
+    L = lib(True)
+    n = len(a_ptr1) - 1
+    nnz = int(a_ptr1[n]) - 1
+    fp = np.zeros(n + 1, dtype=np.int32)
+    fc = np.zeros(nnz, dtype=np.int32)
+    fv = np.zeros(nnz)
+    keep = [np.ascontiguousarray(a_ptr1, dtype=np.int32),
+            np.ascontiguousarray(a_idx1, dtype=np.int32),
+            np.ascontiguousarray(a_val, dtype=np.float64)]
+    rc = L.oracle_ilu0_factors(n, *[_p(a) for a in keep], nb, _p(fp), _p(fc), _p(fv))
+    assert rc == 0, rc
+    m = fp[n]
+    return fp, fc[:m], fv[:m]
+
+
+def visc_eff_int(visc, H_ice, f_ice, zeta, dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny, nz = visc.shape
+    out = np.zeros((nx, ny), dtype=dt, order="F")
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=dt))
+    keep = [F(visc), F(H_ice), F(f_ice), np.ascontiguousarray(zeta, dtype=dt)]
+    L.oracle_visc_eff_int(_p(out), *[_p(a) for a in keep], nx, ny, nz)
+    return out
+
+
+def calc_beta(prob, par, ux_b, uy_b, dp=True):
+    """calc_beta then stagger_beta on the oracle; returns (beta, beta_acx, beta_acy)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = prob.nx, prob.ny
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=dt))
+    fl = prob.fields
+    beta = np.zeros((nx, ny), dtype=dt, order="F")
+    bx = np.zeros((nx, ny), dtype=dt, order="F")
+    by = np.zeros((nx, ny), dtype=dt, order="F")
+    pc = params_c(par)
+    keep = [F(fl["c_bed"]), F(fl["f_slide"]), F(ux_b), F(uy_b), F(fl["H_ice"]), F(fl["f_ice"]),
+            F(fl["H_grnd"]), F(fl["f_grnd"])]
+    rc = L.oracle_calc_beta(_p(beta), *[_p(a) for a in keep], C.byref(pc), nx, ny)
+    assert rc == 0, rc
+    rc = L.oracle_stagger_beta(_p(bx), _p(by), _p(beta), _p(keep[5]), _p(keep[7]), C.byref(pc),
+                               nx, ny)
+    assert rc == 0, rc
+    return beta, bx, by

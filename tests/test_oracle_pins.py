@@ -1,0 +1,377 @@
+"""Pins of the CPU oracle (CPU only).
+
+The reference ships no numeric golden vectors for this path, and its Fortran
+cannot be built here (LIS, fesm-utils and netCDF are absent), so each oracle
+function is pinned by what the mathematics fixes: closed forms, invariants,
+library routines (scipy's sparse direct solver) and symmetry.  Each check is
+chosen so that a plausible slip (a sign, an index, a transposed operand, a
+dropped term) fails it.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from oracle import oracle
+from tests._helpers import rel_l2
+from yelmo_amd import generator, inputs
+from yelmo_amd.params import SsaParams
+
+
+def _csr(ap, ai, av):
+    n = len(ap) - 1
+    return sp.csr_matrix((av.copy(), (ai - 1).copy(), (ap - 1).copy()), shape=(n, n))
+
+
+def _full_ice_problem(nx=12, ny=10, dx=10e3, dy=None):
+    """Fully ice-covered grounded slab: no fronts, every inner face is an
+    inner SSA row (masks 1), borders are the reference's no-slip rows."""
+    p = generator.make_problem(nx, dx, seed=0, dtype=np.float64, ny=ny)
+    dy = dx if dy is None else dy
+    p.dy = dy
+    fl, st = p.fields, p.state
+    fl["f_ice"][:] = 1.0
+    fl["H_ice"][:] = 1000.0
+    fl["f_grnd"][:] = 1.0
+    fl["z_bed"][:] = 100.0
+    fl["mask_frnt"][:] = 0
+    st["ssa_mask_acx"][:] = 1
+    st["ssa_mask_acy"][:] = 1
+    return p
+
+
+def _sample(nx, ny, dx, dy, f):
+    """Sample f(x, y) at ux positions ((i+1/2) dx, j dy) and uy positions
+    (i dx, (j+1/2) dy)."""
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ux = f[0]((i + 0.5) * dx, j * dy)
+    uy = f[1](i * dx, (j + 0.5) * dy)
+    return ux, uy
+
+
+def _interleave(ux, uy):
+    nx, ny = ux.shape
+    x = np.zeros(2 * nx * ny)
+    for i in range(nx):
+        for j in range(ny):
+            n = i * ny + j
+            x[2 * n] = ux[i, j]
+            x[2 * n + 1] = uy[i, j]
+    return x
+
+
+@pytest.mark.parametrize("dy_ratio", [1.0, 1.5])
+def test_assembly_inner_rows_reproduce_ssa_operator_on_quadratics(dy_ratio):
+    """Inner rows of the residual assembly (solver_ssa_ac.f90:473-545,
+    765-835) applied to quadratic velocity fields with uniform viscosity N
+    and friction beta equal the continuous SSA operator exactly:
+      x-row: 4N u_xx + N u_yy + 3N v_xy - beta u
+      y-row: 4N v_yy + N v_xx + 3N u_xy - beta v
+    Centred differences are exact on quadratics, so any wrong sign, index or
+    swapped cross term shows up at round-off scale."""
+    nx, ny, dx = 12, 10, 10e3
+    dy = dx * dy_ratio
+    p = _full_ice_problem(nx, ny, dx, dy)
+    N, beta = 3.0e12, 2.0e3
+    Naa = np.full((nx, ny), N)
+    bx = np.full((nx, ny), beta)
+    ap, ai, av, b, x0, Nab = oracle.assemble(p, bx, bx, Naa)
+    A = _csr(ap, ai, av)
+    a = dict(u_xx=1.3e-9, u_yy=-0.7e-9, u_xy=0.4e-9, v_xx=-0.9e-9, v_yy=0.5e-9, v_xy=1.1e-9)
+    fu = lambda x, y: 0.5 * a["u_xx"] * x ** 2 + 0.5 * a["u_yy"] * y ** 2 + a["u_xy"] * x * y + 3e-4 * x + 50.0
+    fv = lambda x, y: 0.5 * a["v_xx"] * x ** 2 + 0.5 * a["v_yy"] * y ** 2 + a["v_xy"] * x * y - 2e-4 * y + 20.0
+    ux, uy = _sample(nx, ny, dx, dy, (fu, fv))
+    y = A @ _interleave(ux, uy)
+    for i in range(1, nx - 1):
+        for j in range(1, ny - 1):
+            n = i * ny + j
+            ex = 4 * N * a["u_xx"] + N * a["u_yy"] + 3 * N * a["v_xy"] - beta * ux[i, j]
+            ey = 4 * N * a["v_yy"] + N * a["v_xx"] + 3 * N * a["u_xy"] - beta * uy[i, j]
+            assert y[2 * n] == pytest.approx(ex, rel=1e-9, abs=1e-6 * abs(beta * ux[i, j]))
+            assert y[2 * n + 1] == pytest.approx(ey, rel=1e-9, abs=1e-6 * abs(beta * uy[i, j]))
+    # interior corners of a fully iced grid carry the cell viscosity
+    assert np.allclose(Nab[:-1, :-1], N)
+    # right-hand side of inner rows is the driving stress
+    n = 3 * ny + 4
+    assert b[2 * n] == pytest.approx(p.fields["taud_acx"][3, 4])
+    assert b[2 * n + 1] == pytest.approx(p.fields["taud_acy"][3, 4])
+
+
+def test_assembly_border_and_dirichlet_rows():
+    """no-slip borders and mask 0 faces are identity rows with b = x0 = 0;
+    free-slip ("infinite") borders are u(border) - u(inner) = 0."""
+    p = _full_ice_problem(8, 7)
+    nx, ny = 8, 7
+    Naa = np.full((nx, ny), 1e12)
+    bx = np.full((nx, ny), 1e3)
+    p.state["ssa_mask_acx"][3, 3] = 0
+    ap, ai, av, b, x0, _ = oracle.assemble(p, bx, bx, Naa)
+    A = _csr(ap, ai, av).toarray()
+    for (i, j) in [(0, 3), (nx - 1, 2), (3, 0), (3, ny - 1), (3, 3)]:
+        r = 2 * (i * ny + j)
+        row = A[r].copy()
+        assert row[r] == 1.0 and np.count_nonzero(row) == 1 and b[r] == 0.0
+    ap, ai, av, b, x0, _ = oracle.assemble(p, bx, bx, Naa, boundaries="infinite")
+    A = _csr(ap, ai, av).toarray()
+    r = 2 * (0 * ny + 3)
+    assert A[r, r] == 1.0 and A[r, 2 * (1 * ny + 3)] == -1.0 and np.count_nonzero(A[r]) == 2
+
+
+def test_lateral_front_rows_are_the_front_stress_condition():
+    """A mask-3 face (ice to the left, open to the right) carries the
+    vertically integrated front balance (solver_ssa_ac.f90:402-471):
+      N (4 du/dx + 2 dv/dy) = taul_int,
+    so a field with du/dx = a, dv/dy = c gives 4 N a + 2 N c exactly."""
+    nx, ny, dx = 10, 9, 10e3
+    p = _full_ice_problem(nx, ny, dx)
+    p.fields["f_ice"][6:, :] = 0.0
+    p.state["ssa_mask_acx"][5, :] = 3
+    Naa = np.full((nx, ny), 2e12)
+    bx = np.full((nx, ny), 1e3)
+    ap, ai, av, b, x0, _ = oracle.assemble(p, bx, bx, Naa)
+    A = _csr(ap, ai, av)
+    a_, c_ = 2e-4, -3e-4
+    ux, uy = _sample(nx, ny, dx, dx, (lambda x, y: a_ * x + 7.0, lambda x, y: c_ * y - 3.0))
+    y = A @ _interleave(ux, uy)
+    for j in range(1, ny - 1):
+        r = 2 * (5 * ny + j)
+        assert y[r] == pytest.approx(4 * 2e12 * a_ + 2 * 2e12 * c_, rel=1e-10)
+        assert b[r] == p.fields["taul_int_acx"][5, j]
+
+
+def test_stagger_visc_corner_rules():
+    """stagger_visc_aa_ab (solver_ssa_ac.f90:1000-1067): mean over fully iced
+    cells of the four around a corner, zero if any is ocean-side (ice-free
+    or partial with bed below sea level), zero if none is iced."""
+    nx, ny = 6, 5
+    p = _full_ice_problem(nx, ny)
+    f = p.fields
+    f["f_ice"][:] = 1.0
+    f["f_ice"][4:, :] = 0.0
+    f["z_bed"][4:, :] = 50.0          # ice-free land to the right
+    f["f_ice"][:, 4] = 0.0
+    f["z_bed"][:, 4] = -50.0          # ocean on top
+    N = np.arange(nx * ny, dtype=float).reshape(nx, ny, order="F") * 1e10 + 1e11
+    _, _, _, _, _, Nab = oracle.assemble(p, np.ones((nx, ny)), np.ones((nx, ny)), N)
+    # interior corner: plain mean of four
+    assert Nab[1, 1] == pytest.approx((N[1, 1] + N[2, 1] + N[1, 2] + N[2, 2]) / 4)
+    # corner touching land (cells 3,4 in x): mean over the two iced cells only
+    assert Nab[3, 1] == pytest.approx((N[3, 1] + N[3, 2]) / 2)
+    # corner touching the ocean row: zero (traction-free front)
+    assert Nab[1, 3] == 0.0
+
+
+def test_sparse_direct_and_krylov_agree():
+    """BiCGSTAB + ILU(0) at rtol 1e-8 on the assembled system of a real
+    problem lands on scipy's sparse LU solution; b = 0 gives x = 0."""
+    par = SsaParams(visc_method=2, beta_method=4, ssa_iter_max=3)
+    p = generator.make_problem(41, 32e3, seed=4, dtype=np.float64)
+    st, info = oracle.calc_velocity_ssa(p, par, dp=True)
+    q = p.copy()
+    q.state.update(ux_b=st["ux_b"], uy_b=st["uy_b"])
+    ap, ai, av, b, x0, _ = oracle.assemble(q, st["beta_acx"], st["beta_acy"], st["visc_eff_int"])
+    A = _csr(ap, ai, av)
+    xd = spl.spsolve(A.tocsc(), b)
+    for opts, tol in (("-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-10", 1e-7),
+                      ("-i bicgstab -p jacobi -maxiter 5000 -tol 1.0e-10", 1e-6)):
+        r = oracle.csr_solve(ap, ai, av, b, x0, opts)
+        assert r["status"] == 0
+        assert np.linalg.norm(A @ r["x"] - b) / np.linalg.norm(b) < 1e-9
+        assert rel_l2(r["x"], xd) < tol
+    z = oracle.csr_solve(ap, ai, av, np.zeros_like(b), x0 + 1.0, "-i bicgstab -p ilu")
+    assert z["iters"] == 0 and z["status"] == 0 and not z["x"].any()
+    m = oracle.csr_solve(ap, ai, av, b, x0, "-i bicgstab -p none -maxiter 2 -tol 1e-12")
+    assert m["status"] == 4 and m["iters"] == 2
+
+
+def test_ilu0_pattern_identity_and_exactness():
+    """ILU(0) is the factorisation with (L U)_ij = A_ij on pattern(A); on a
+    tridiagonal matrix (no fill) it is the exact LU, so M^-1 = A^-1."""
+    rng = np.random.default_rng(3)
+    p = generator.make_problem(21, 64e3, seed=1, dtype=np.float64)
+    nx = ny = 21
+    Naa = 1e11 + 1e11 * rng.random((nx, ny))
+    bx = 1e3 + 1e3 * rng.random((nx, ny))
+    ap, ai, av, b, x0, _ = oracle.assemble(p, bx, bx, Naa)
+    A = _csr(ap, ai, av)
+    A.sum_duplicates()
+    fp, fc, fv = oracle.ilu0_factors(ap, ai, av)
+    n = len(fp) - 1
+    F = sp.csr_matrix((fv, fc, fp), shape=(n, n)).toarray()
+    Lm = np.tril(F, -1) + np.eye(n)
+    Um = np.triu(F)
+    LU = Lm @ Um
+    Ad = A.toarray()
+    pat = Ad != 0
+    pat |= np.eye(n, dtype=bool)
+    struct = _csr(ap, ai, np.ones_like(av)).toarray() != 0
+    scale = np.abs(Ad).max()
+    assert np.abs((LU - Ad)[struct]).max() < 1e-9 * scale
+    # tridiagonal: exact
+    m = 50
+    d = 4.0 + rng.random(m)
+    lo = rng.random(m - 1)
+    up = rng.random(m - 1)
+    T = np.diag(d) + np.diag(lo, -1) + np.diag(up, 1)
+    Ts = sp.csr_matrix(T)
+    tap, tai, tav = (Ts.indptr + 1).astype(np.int32), (Ts.indices + 1).astype(np.int32), Ts.data
+    v = rng.random(m)
+    assert np.allclose(oracle.ilu0_apply(tap, tai, tav, v), np.linalg.solve(T, v), rtol=1e-12)
+
+
+def test_glen_viscosity_closed_form():
+    """calc_visc_eff_3D_aa (velocity_ssa.f90:490-586) under uniform strain:
+    eta = 1/2 A^(-1/n) eps_e^((1-n)/n), eps_e^2 = a^2 + c^2 + a c + eps_0^2
+    for du/dx = a, dv/dy = c; with the rate factor varying by level."""
+    nx, ny, nz, dx = 9, 8, 4, 5e3
+    a_, c_ = 3e-3, -1e-3
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ux = a_ * (i + 0.5) * dx
+    uy = c_ * (j + 0.5) * dx
+    ATT = np.broadcast_to((1e-16 * np.array([1, 2, 4, 8]))[None, None, :], (nx, ny, nz)).copy()
+    f_ice = np.ones((nx, ny))
+    v = oracle.visc_eff_3D_aa(ux, uy, ATT, f_ice, dx, dx, 3.0, 1e-6)
+    eps2 = a_ ** 2 + c_ ** 2 + a_ * c_ + 1e-12
+    for k in range(nz):
+        ex = 0.5 * ATT[0, 0, k] ** (-1 / 3) * eps2 ** (-1 / 3)
+        assert np.allclose(v[2:-2, 2:-2, k], ex, rtol=1e-12)
+    f_ice[5, 4] = 0.0
+    v = oracle.visc_eff_3D_aa(ux, uy, ATT, f_ice, dx, dx, 3.0, 1e-6)
+    assert np.all(v[5, 4, :] == 1e5)
+
+
+def test_translating_slab_has_zero_strain_at_fronts():
+    """tests/test_front_strain.f90: a translating slab (velocity U on every
+    face next to ice, zero elsewhere) has zero strain rate everywhere, also
+    at straight and 45-degree fronts (deformation.f90:1393-1507)."""
+    nx, ny, dx = 16, 14, 1e3
+    f_ice = np.zeros((nx, ny))
+    for i in range(nx):
+        for j in range(ny):
+            if i <= 9 and (i + j) <= 17:
+                f_ice[i, j] = 1.0
+    act_x = np.zeros((nx, ny), bool)
+    act_y = np.zeros((nx, ny), bool)
+    for i in range(nx):
+        for j in range(ny):
+            act_x[i, j] = f_ice[i, j] == 1 or f_ice[min(i + 1, nx - 1), j] == 1
+            act_y[i, j] = f_ice[i, j] == 1 or f_ice[i, min(j + 1, ny - 1)] == 1
+    U, V = 120.0, -45.0
+    ux = np.where(act_x, U, 0.0)
+    uy = np.where(act_y, V, 0.0)
+    dudx, dudy, dvdx, dvdy = oracle.strain_rate_2D(ux, uy, f_ice, dx, dx)
+    ice = f_ice == 1
+    for d in (dudx, dudy, dvdx, dvdy):
+        assert np.abs(d[ice]).max() < 1e-12
+
+
+def test_depth_integral_trapezoid():
+    """calc_visc_eff_int (velocity_general.f90:2215-2261): H * int_0^1 eta
+    dzeta, exact for eta linear in zeta; 1e5 outside ice and as a floor."""
+    nx, ny, nz = 5, 4, 11
+    zeta = inputs.calc_zeta(nz, "exp", 2.0, dtype=np.float64)
+    H = np.full((nx, ny), 800.0)
+    f_ice = np.ones((nx, ny))
+    f_ice[0, 0] = 0.0
+    visc = np.broadcast_to((2e13 + 1e13 * zeta)[None, None, :], (nx, ny, nz)).copy()
+    out = oracle.visc_eff_int(visc, H, f_ice, zeta)
+    assert np.allclose(out[1:, :], 800.0 * (2e13 + 0.5e13), rtol=1e-13)
+    assert out[0, 0] == 1e5
+    tiny = oracle.visc_eff_int(visc * 1e-20, H, f_ice, zeta)
+    assert np.all(tiny == 1e5)
+
+
+def test_power_law_friction_closed_form():
+    """calc_beta (basal_dragging.f90:384-579) with beta_method 4 (cell
+    centre): beta = c_b (|u|/u0)^q / |u|, |u| = sqrt(ux^2 + uy^2 + ub_min^2)
+    at the unstaggered velocity; floored at beta_min, zero where floating;
+    method 5 is the regularised Coulomb law c_b (|u|/(|u|+u0))^q / |u|."""
+    p = generator.make_problem(21, 64e3, seed=2, dtype=np.float64)
+    nx = ny = 21
+    ux = np.full((nx, ny), 300.0)
+    uy = np.full((nx, ny), -400.0)
+    par = SsaParams(beta_method=4, beta_q=0.2, beta_u0=1.0, beta_min=100.0, beta_gl_f=1.0)
+    beta, bx, by = oracle.calc_beta(p, par, ux, uy)
+    cb = p.fields["c_bed"]
+    fg = p.fields["f_grnd"]
+    fi = p.fields["f_ice"]
+    s = np.sqrt(300.0 ** 2 + 400.0 ** 2 + 1e-6)
+    ex = np.where(fg > 0, np.maximum(cb * s ** 0.2 / s, 100.0), 0.0)
+    ex = np.where((fg > 0) & (ex == 0), 0.0, ex)
+    m = (fi == 1) & (fg > 0) & (cb * s ** -0.8 > 100)
+    assert np.allclose(beta[m], ex[m], rtol=1e-12)
+    assert np.all(beta[fg == 0] == 0.0)
+    par5 = par.with_(beta_method=5, beta_u0=50.0)
+    beta5, _, _ = oracle.calc_beta(p, par5, ux, uy)
+    ex5 = cb * (s / (s + 50.0)) ** 0.2 / s
+    m5 = m & (ex5 > 100)
+    assert np.allclose(beta5[m5], ex5[m5], rtol=1e-12)
+    # staggered: interior grounded faces take the mean of the two cells
+    inner = (fi[:-1] == 1) & (fi[1:] == 1) & (fg[:-1] > 0) & (fg[1:] > 0)
+    mean = 0.5 * (beta[:-1] + beta[1:])
+    assert np.allclose(bx[:-1][inner], np.maximum(mean[inner], 100.0))
+
+
+def test_masks_two_restatements_agree_and_truth_table():
+    """set_ssa_masks (solver_ssa_ac.f90:851-948): the package's numpy
+    restatement and the oracle's C restatement agree on real geometry; the
+    front rules follow front_face_mask's table."""
+    for seed in range(3):
+        p = generator.make_problem(41, 32e3, seed=seed, dtype=np.float32)
+        f = p.fields
+        for lat in ("all", "none", "floating", "marine"):
+            a = inputs.set_ssa_masks(f["mask_frnt"], f["f_ice"], f["f_grnd"], lat)
+            b = oracle.set_ssa_masks(f["mask_frnt"], f["f_ice"], f["f_grnd"], lat)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    ff = inputs.front_face_mask
+    assert ff(1, -2, "all") == 0          # floating ice against land: wall
+    assert ff(2, -2, "floating") == 4     # land front is a grounded front
+    assert ff(1, -1, "floating") == 3
+    assert ff(2, -1, "marine") == 3
+    assert ff(3, -1, "marine") == 4
+    assert ff(1, 2, "all") == -1          # not a front face
+    assert ff(-1, 3, "none") == 4
+
+
+def test_picard_solution_mirror_symmetric():
+    """A domain symmetric under x -> -x (noise-free radial cap on a symmetric
+    bed) gives ux antisymmetric and uy symmetric after the whole Picard loop
+    (the reference reports a 2e-13 floor in double precision,
+    docs/numerics-precision.md:52-53).  An index slip in any cross term of
+    the assembly or the viscosity staggering breaks it."""
+    nx = ny = 33
+    dx = 40e3
+    p = generator.make_problem(nx, dx, seed=0, dtype=np.float64)
+    xc = (np.arange(1, nx + 1) - (nx + 1) / 2.0) * dx
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    r = np.hypot(X, Y)
+    H = 3000.0 * np.sqrt(np.maximum(0.0, 1.0 - (r / 550e3) ** 2))
+    zb = -200.0 + 400.0 * np.cos(X / 300e3) * np.cos(Y / 250e3) * 0.5
+    f_ice = (H > 0).astype(float)
+    z_sl = np.zeros_like(H)
+    Hg = inputs.calc_H_grnd(H, f_ice, zb, z_sl, 910.0, 1028.0)
+    fg = (Hg > 0).astype(float)
+    zs = inputs.calc_z_srf_max(H, f_ice, zb, z_sl, 910.0, 1028.0)
+    mf = inputs.calc_ice_front(f_ice, fg, zb, z_sl)
+    gx = inputs.calc_gradient_acx(zs, f_ice, dx, 0.5)
+    gy = inputs.calc_gradient_acy(zs, f_ice, dx, 0.5)
+    tx, ty = inputs.calc_driving_stress(H, f_ice, gx, gy, dx, 2e5, 910.0, 9.81)
+    lx, ly = inputs.calc_lateral_bc_stress_2D(mf, H, f_ice, zs, z_sl, 910.0, 1028.0, 9.81)
+    mx, my = inputs.set_ssa_masks(mf, f_ice, fg)
+    F = lambda a: np.asfortranarray(a, dtype=np.float64)
+    p.fields.update(H_ice=F(H), f_ice=F(f_ice), z_bed=F(zb), H_grnd=F(Hg), f_grnd=F(fg),
+                    z_srf=F(zs), taud_acx=F(tx), taud_acy=F(ty), taul_int_acx=F(lx),
+                    taul_int_acy=F(ly), c_bed=F(np.where(fg > 0, 1e4, 0.0)), mask_frnt=mf)
+    p.state.update(ssa_mask_acx=mx, ssa_mask_acy=my)
+    par = SsaParams(visc_method=2, beta_method=4, ssa_iter_max=30,
+                    ssa_lis_opt="-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-12")
+    st, info = oracle.calc_velocity_ssa(p, par, dp=True)
+    assert info["rc"] == 0
+    ux, uy = st["ux_b"], st["uy_b"]
+    scale = np.abs(ux).max()
+    assert scale > 1.0
+    # ux lives at i+1/2: mirror face of i is nx-2-i
+    assert np.abs(ux[:-1, :] + ux[-2::-1, :]).max() < 1e-8 * scale
+    assert np.abs(uy - uy[::-1, :]).max() < 1e-8 * scale
+    # and the same in y (transpose symmetry of the domain)
+    assert np.abs(uy[:, :-1] + uy[:, -2::-1]).max() < 1e-8 * scale

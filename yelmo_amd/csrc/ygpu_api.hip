@@ -1,0 +1,934 @@
+// ygpu_api.hip -- host side of libygpu.so: handles, device residency of a
+// batch of domains, the Picard loop driver and the C-ABI entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ygpu.h"
+#include "ygpu_diva.h"
+#include "ygpu_fields.h"
+#include "ygpu_krylov.h"
+
+struct ygpu_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool stats_on = false;
+  ygpu_stats stats{};
+};
+
+namespace {
+
+#define YG_CHECK(h, call)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
+      return YGPU_ERR_DEVICE;                                                      \
+    }                                                                              \
+  } while (0)
+
+// ---- LIS option string (solver_linear.F90:350) ----------------------------
+struct Opts {
+  YgKrylovOpts k;
+  int ny_hint = 0;
+  int prec = -1;  // -1 unset, 0 fp64, 2 mixed
+};
+
+int parse_opts(const char* s, Opts& o, std::string& err) {
+  o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
+  o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3;
+  if (!s) return 0;
+  std::vector<std::string> tok;
+  std::string cur;
+  for (const char* p = s; *p; ++p) {
+    if (*p == ' ' || *p == '\t' || *p == '\n') { if (!cur.empty()) { tok.push_back(cur); cur.clear(); } }
+    else cur += *p;
+  }
+  if (!cur.empty()) tok.push_back(cur);
+  if (tok.size() % 2) { err = "option string: missing value"; return YGPU_ERR_ARG; }
+  for (size_t q = 0; q < tok.size(); q += 2) {
+    const std::string& k = tok[q];
+    const std::string& v = tok[q + 1];
+    if (k == "-i") {
+      if (v == "cg" || v == "1") o.k.solver = 1;
+      else if (v == "bicgstab" || v == "4") o.k.solver = 4;
+      else { err = "unsupported Krylov solver: " + v; return YGPU_ERR_UNSUPPORTED; }
+    } else if (k == "-p") {
+      if (v == "none" || v == "0") o.k.precon = 0;
+      else if (v == "jacobi" || v == "1") o.k.precon = 1;
+      else if (v == "ilu" || v == "2") o.k.precon = 2;
+      else { err = "unsupported preconditioner: " + v; return YGPU_ERR_UNSUPPORTED; }
+    } else if (k == "-maxiter") o.k.maxiter = atoi(v.c_str());
+    else if (k == "-tol") o.k.tol = atof(v.c_str());
+    else if (k == "-initx_zeros") o.k.initx_zeros = (v == "true" || v == "1");
+    else if (k == "-conv_cond") o.k.conv_cond = atoi(v.c_str());
+    else if (k == "-ir_inner_tol") o.k.inner_tol = atof(v.c_str());
+    else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
+    else if (k == "-ygpu_ny") o.ny_hint = atoi(v.c_str());
+    else if (k == "-ygpu_precision") {
+      if (v == "fp64") o.prec = 0;
+      else if (v == "mixed") o.prec = 2;
+      else { err = "unknown precision: " + v; return YGPU_ERR_ARG; }
+    } else if (k == "-omp_num_threads" || k == "-print" || k == "-scale" || k == "-f") {
+      // accepted; no effect here
+    } else { err = "unknown option: " + k; return YGPU_ERR_ARG; }
+  }
+  if (o.k.maxiter < 0) { err = "negative -maxiter"; return YGPU_ERR_ARG; }
+  return 0;
+}
+
+// Algorithmic HBM bytes of the persistent Krylov kernel for one member of C
+// cells that ran `iters` BiCGSTAB iterations with a factorization: matrix
+// slots pA, factors pF, vectors pv bytes per value.  Per iteration: two
+// stencil SpMVs (18 slot values + 8 B of mask/coordinates per cell), two
+// ILU(0) applications (18 factor values per cell), 27 vector passes of two
+// values per cell (the fused updates and dot products).
+double krylov_bytes(double C, int iters, int pA, int pF, int pv, bool ilu) {
+  const double setup = 18.0 * C * pA + 8.0 * C + 4.0 * 2.0 * C * pv;
+  const double fact = ilu ? (18.0 * C * pA + 4.0 * C + 18.0 * C * pF) : 0.0;
+  const double per_it = 36.0 * C * pA + 16.0 * C + (ilu ? 36.0 * C * pF : 4.0 * C * pF) +
+                        54.0 * C * pv;
+  return setup + fact + iters * per_it;
+}
+
+// skewed step table of an nx x ny member: soff[t] = cells before step t
+void step_table(int nx, int ny, std::vector<int>& soff, int& T) {
+  T = ny + 2 * (nx - 1);
+  soff.assign(T + 1, 0);
+  for (int t = 0; t < T; ++t) {
+    const int a = t - (ny - 1);
+    const int glo = a <= 0 ? 0 : (a + 1) / 2;
+    const int ghi = std::min(nx - 1, t / 2);
+    soff[t + 1] = soff[t] + std::max(0, ghi - glo + 1);
+  }
+}
+
+int skew_q(const std::vector<int>& soff, int i, int j, int ny) {
+  const int t = j + 2 * i;
+  const int a = t - (ny - 1);
+  const int glo = a <= 0 ? 0 : (a + 1) / 2;
+  return soff[t] + i - glo;
+}
+
+int bc_code(int bnd) {
+  switch (bnd) {
+    case YGPU_BND_ZEROS: return YG_BC_ZEROS;
+    case YGPU_BND_INFINITE: case YGPU_BND_MASK: return YG_BC_INFINITE;
+    case YGPU_BND_MISMIP3D: return YG_BC_MISMIP3D;
+    case YGPU_BND_TROUGH: return YG_BC_TROUGH;
+    case YGPU_BND_PERIODIC: return YG_BC_PERIODIC;
+    case YGPU_BND_PERIODIC_X: return YG_BC_PERIODIC_X;
+    case YGPU_BND_PERIODIC_Y: return YG_BC_PERIODIC_Y;
+  }
+  return -1;
+}
+
+// ---- device allocations of one batch --------------------------------------
+struct Arena {
+  std::vector<void*> ptrs;
+  size_t bytes = 0;
+  ~Arena() { for (void* p : ptrs) (void)hipFree(p); }
+  template <typename T>
+  hipError_t alloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e == hipSuccess) { ptrs.push_back(*p); bytes += n * sizeof(T); }
+    return e;
+  }
+};
+
+template <typename R>
+struct Batch {
+  ygpu_handle* h = nullptr;
+  Arena ar;
+  int nmem = 0, diva = 0, prec = 0, kmode = 0, block = 1024;
+  long long total_cells = 0, total_z = 0;
+  int zstride = 0;
+  std::vector<YgMember> mem;
+  std::vector<long long> coffs, zoffs;
+  std::vector<int> soff;
+  YgFieldParams fp{};
+  Opts opts;
+  int iter_max = 1;
+  // device
+  YgMember* d_mem = nullptr;
+  long long *d_coffs = nullptr, *d_zoffs = nullptr;
+  int* d_soff = nullptr;
+  uint32_t *d_cellij = nullptr, *d_mask = nullptr;
+  double *d_A64 = nullptr, *d_b = nullptr, *d_x = nullptr, *d_F64 = nullptr, *d_W64 = nullptr;
+  float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
+  int *d_done = nullptr, *d_iter_now = nullptr, *d_lin_iter = nullptr, *d_lin_fail = nullptr,
+      *d_lim_n = nullptr, *d_kit = nullptr, *d_kst = nullptr;
+  double *d_resid = nullptr, *d_kres = nullptr;
+  YgFields<R> f{};
+  YgDivaFields<R> dv{};
+
+  YgBatchGeom geom() const {
+    YgBatchGeom G;
+    G.members = d_mem; G.coffs = d_coffs; G.zoffs = d_zoffs; G.soff = d_soff;
+    G.nmem = nmem; G.total_cells = total_cells;
+    return G;
+  }
+
+  int setup(ygpu_handle* hh, const ygpu_ssa_params* par, int np, const ygpu_ssa_problem* probs,
+            int is_diva) {
+    h = hh;
+    nmem = np;
+    diva = is_diva;
+    std::string e;
+    int rc = parse_opts(par->ssa_lis_opt, opts, e);
+    if (rc) { h->err = e; return rc; }
+    prec = par->precision;
+    kmode = prec == YGPU_PREC_MIXED ? YG_MODE_MIXED : YG_MODE_FP64;
+    if (par->ssa_solver != 0) { h->err = "ssa_solver \"energy\" is not available"; return YGPU_ERR_UNSUPPORTED; }
+    const int BC = bc_code(par->boundaries);
+    if (BC < 0) { h->err = "unknown boundaries code"; return YGPU_ERR_ARG; }
+    if (opts.k.precon == 2 && (BC == YG_BC_PERIODIC || BC == YG_BC_PERIODIC_X ||
+                               BC == YG_BC_PERIODIC_Y || BC == YG_BC_MISMIP3D || BC == YG_BC_TROUGH)) {
+      h->err = "ILU(0) needs a non-periodic domain (use -p jacobi)";
+      return YGPU_ERR_UNSUPPORTED;
+    }
+    fp.bnd = par->boundaries; fp.BC = BC;
+    fp.visc_method = par->visc_method; fp.beta_method = par->beta_method;
+    fp.beta_gl_scale = par->beta_gl_scale; fp.beta_gl_stag = par->beta_gl_stag;
+    fp.vel_lim_method = par->ssa_vel_lim_method; fp.diva = is_diva; fp.no_slip = par->no_slip;
+    fp.visc_const = par->visc_const; fp.beta_const = par->beta_const; fp.beta_q = par->beta_q;
+    fp.beta_u0 = par->beta_u0; fp.beta_gl_f = par->beta_gl_f; fp.H_grnd_lim = par->H_grnd_lim;
+    fp.beta_min = par->beta_min; fp.eps_0 = par->eps_0; fp.vel_lim_tau = par->ssa_vel_lim_tau;
+    fp.vel_max = par->ssa_vel_max; fp.iter_rel = par->ssa_iter_rel; fp.iter_conv = par->ssa_iter_conv;
+    if (par->visc_method < 0 || par->visc_method > 2 || par->beta_method < -1 || par->beta_method > 5 ||
+        !(par->beta_gl_scale == 0 || par->beta_gl_scale == 1 || par->beta_gl_scale == 3) ||
+        par->beta_gl_stag < -1 || par->beta_gl_stag > 2) {
+      h->err = "parameter value not supported";
+      return YGPU_ERR_UNSUPPORTED;
+    }
+    iter_max = par->ssa_iter_max;
+    if (iter_max < 1) { h->err = "ssa_iter_max < 1"; return YGPU_ERR_ARG; }
+    mem.resize(np);
+    coffs.assign(np + 1, 0);
+    zoffs.assign(np + 1, 0);
+    soff.clear();
+    int maxnx = 0;
+    for (int k = 0; k < np; ++k) {
+      const ygpu_ssa_problem& P = probs[k];
+      if (P.nx < 3 || P.ny < 3 || P.nz < 2 || P.nx > YG_MAX_LINES || P.ny > 65535 ||
+          (P.real_bytes != 4 && P.real_bytes != 8)) {
+        h->err = "domain " + std::to_string(k) + ": unsupported size or real_bytes";
+        return YGPU_ERR_ARG;
+      }
+      YgMember& M = mem[k];
+      M.nx = P.nx; M.ny = P.ny; M.nz = P.nz;
+      std::vector<int> st;
+      int T;
+      step_table(P.nx, P.ny, st, T);
+      if (T > YG_MAX_STEPS) { h->err = "domain too large for the step table"; return YGPU_ERR_ARG; }
+      M.T = T;
+      M.coff = coffs[k];
+      M.toff = (long long)soff.size();
+      M.dx = P.dx; M.dy = P.dy;
+      soff.insert(soff.end(), st.begin(), st.end());
+      coffs[k + 1] = coffs[k] + (long long)P.nx * P.ny;
+      zoffs[k + 1] = zoffs[k] + (long long)P.nx * P.ny * P.nz;
+      zstride = std::max(zstride, P.nz);
+      maxnx = std::max(maxnx, P.nx);
+      fp.n_glen = P.n_glen;
+    }
+    total_cells = coffs[np];
+    total_z = zoffs[np];
+    block = par->block > 0 ? par->block : 1024;
+    block = std::max(block, ((maxnx + 63) / 64) * 64);
+    block = std::min(block, 1024);
+    return allocate();
+  }
+
+  int allocate() {
+    const long long C = total_cells, Z = total_z;
+    YG_CHECK(h, ar.alloc(&d_mem, nmem));
+    YG_CHECK(h, ar.alloc(&d_coffs, nmem + 1));
+    YG_CHECK(h, ar.alloc(&d_zoffs, nmem + 1));
+    YG_CHECK(h, ar.alloc(&d_soff, soff.size()));
+    YG_CHECK(h, ar.alloc(&d_cellij, C));
+    YG_CHECK(h, ar.alloc(&d_mask, C));
+    YG_CHECK(h, ar.alloc(&d_A64, 18 * C));
+    YG_CHECK(h, ar.alloc(&d_b, 2 * C));
+    YG_CHECK(h, ar.alloc(&d_x, 2 * C));
+    YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * 2 * C));
+    if (kmode == YG_MODE_MIXED) {
+      YG_CHECK(h, ar.alloc(&d_A32, 18 * C));
+      YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * C));
+      YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * 2 * C));
+    } else {
+      YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * C));
+    }
+    int* ints[7];
+    for (int q = 0; q < 7; ++q) YG_CHECK(h, ar.alloc(&ints[q], nmem));
+    d_done = ints[0]; d_iter_now = ints[1]; d_lin_iter = ints[2]; d_lin_fail = ints[3];
+    d_lim_n = ints[4]; d_kit = ints[5]; d_kst = ints[6];
+    YG_CHECK(h, ar.alloc(&d_resid, nmem));
+    YG_CHECK(h, ar.alloc(&d_kres, nmem));
+    // fields
+    R** r2d[] = {(R**)&f.c_bed, (R**)&f.f_slide, (R**)&f.taud_acx, (R**)&f.taud_acy,
+                 (R**)&f.taul_acx, (R**)&f.taul_acy, (R**)&f.H_ice, (R**)&f.f_ice,
+                 (R**)&f.H_grnd, (R**)&f.f_grnd, (R**)&f.z_bed, (R**)&f.z_sl, &f.ux, &f.uy,
+                 &f.ux_nm1, &f.uy_nm1, &f.visc_int, &f.beta, &f.beta_acx, &f.beta_acy, &f.bss_x,
+                 &f.bss_y, &f.lkx, &f.lky, &f.lrx, &f.lry, &f.dudx, &f.dudy, &f.dvdx, &f.dvdy,
+                 &f.err_x, &f.err_y, &f.taub_x, &f.taub_y};
+    for (R** p : r2d) YG_CHECK(h, ar.alloc(p, C));
+    YG_CHECK(h, ar.alloc((R**)&f.ATT, Z));
+    YG_CHECK(h, ar.alloc(&f.visc, Z));
+    YG_CHECK(h, ar.alloc(&f.visc_nm1, Z));
+    YG_CHECK(h, ar.alloc((R**)&f.zeta, (size_t)zstride * nmem));
+    f.zstride = zstride;
+    YG_CHECK(h, ar.alloc((int32_t**)&f.mx, C));
+    YG_CHECK(h, ar.alloc((int32_t**)&f.my, C));
+    if (diva) {
+      R** d2[] = {&f.F2, &f.F2_acx, &f.F2_acy, &f.beff_acx, &f.beff_acy, &f.beta_eff, &f.ubar_x,
+                  &f.ubar_y};
+      for (R** p : d2) YG_CHECK(h, ar.alloc(p, C));
+      YG_CHECK(h, ar.alloc(&f.duxdz, Z));
+      YG_CHECK(h, ar.alloc(&f.duydz, Z));
+      YG_CHECK(h, ar.alloc(&dv.ux3, Z));
+      YG_CHECK(h, ar.alloc(&dv.uy3, Z));
+    }
+    // geometry tables
+    std::vector<long long> mc(coffs.begin(), coffs.end()), mz(zoffs.begin(), zoffs.end());
+    YG_CHECK(h, hipMemcpyAsync(d_mem, mem.data(), sizeof(YgMember) * nmem, hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipMemcpyAsync(d_coffs, mc.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipMemcpyAsync(d_zoffs, mz.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipMemcpyAsync(d_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, h->stream));
+    yg_launch_cellij(geom(), d_cellij, h->stream);
+    YG_CHECK(h, hipGetLastError());
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    return YGPU_OK;
+  }
+
+  // pack one field of every domain (host float or double) into R and upload
+  int up2(R* dst, const void* const* src, int nz_mult, bool may_null, R fill = R(0)) {
+    const long long n = nz_mult ? total_z : total_cells;
+    std::vector<R> st(n);
+    for (int k = 0; k < nmem; ++k) {
+      const long long off = nz_mult ? zoffs[k] : coffs[k];
+      const long long cnt = (nz_mult ? zoffs[k + 1] : coffs[k + 1]) - off;
+      const void* s = src[k];
+      if (!s) {
+        if (!may_null) { h->err = "missing input array"; return YGPU_ERR_ARG; }
+        for (long long q = 0; q < cnt; ++q) st[off + q] = fill;
+      } else if (rb[k] == 4) {
+        const float* p = (const float*)s;
+        for (long long q = 0; q < cnt; ++q) st[off + q] = (R)p[q];
+      } else {
+        const double* p = (const double*)s;
+        for (long long q = 0; q < cnt; ++q) st[off + q] = (R)p[q];
+      }
+    }
+    YG_CHECK(h, hipMemcpyAsync(dst, st.data(), sizeof(R) * n, hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    return YGPU_OK;
+  }
+
+  int down2(const R* src, void* const* dst, int nz_mult) {
+    const long long n = nz_mult ? total_z : total_cells;
+    std::vector<R> st(n);
+    YG_CHECK(h, hipMemcpyAsync(st.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost, h->stream));
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < nmem; ++k) {
+      const long long off = nz_mult ? zoffs[k] : coffs[k];
+      const long long cnt = (nz_mult ? zoffs[k + 1] : coffs[k + 1]) - off;
+      void* d = dst[k];
+      if (!d) continue;
+      if (rb[k] == 4) {
+        float* p = (float*)d;
+        for (long long q = 0; q < cnt; ++q) p[q] = (float)st[off + q];
+      } else {
+        double* p = (double*)d;
+        for (long long q = 0; q < cnt; ++q) p[q] = (double)st[off + q];
+      }
+    }
+    return YGPU_OK;
+  }
+
+  std::vector<int> rb;
+
+#define YG_FIELD(member) [&] { std::vector<const void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = probs[k].member; return v; }()
+#define YG_FIELDW(member) [&] { std::vector<void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = probs[k].member; return v; }()
+
+  int upload_inputs(const ygpu_ssa_problem* probs) {
+    rb.resize(nmem);
+    for (int k = 0; k < nmem; ++k) rb[k] = probs[k].real_bytes;
+    int rc;
+#define UP(dst, member) if ((rc = up2((R*)(dst), YG_FIELD(member).data(), 0, false))) return rc;
+    UP(f.c_bed, c_bed) UP(f.f_slide, f_slide) UP(f.taud_acx, taud_acx) UP(f.taud_acy, taud_acy)
+    UP(f.taul_acx, taul_int_acx) UP(f.taul_acy, taul_int_acy) UP(f.H_ice, H_ice) UP(f.f_ice, f_ice)
+    UP(f.H_grnd, H_grnd) UP(f.f_grnd, f_grnd) UP(f.z_bed, z_bed) UP(f.z_sl, z_sl)
+#undef UP
+    if ((rc = up2((R*)f.ATT, YG_FIELD(ATT).data(), 1, false))) return rc;
+    // zeta: nz per member, padded to zstride
+    std::vector<R> zs((size_t)zstride * nmem, R(0));
+    for (int k = 0; k < nmem; ++k) {
+      if (!probs[k].zeta_aa) { h->err = "missing zeta_aa"; return YGPU_ERR_ARG; }
+      for (int q = 0; q < probs[k].nz; ++q)
+        zs[(size_t)k * zstride + q] = rb[k] == 4 ? (R)((const float*)probs[k].zeta_aa)[q]
+                                                 : (R)((const double*)probs[k].zeta_aa)[q];
+    }
+    YG_CHECK(h, hipMemcpyAsync((R*)f.zeta, zs.data(), sizeof(R) * zs.size(), hipMemcpyHostToDevice, h->stream));
+    // masks
+    std::vector<int32_t> mxs(total_cells), mys(total_cells);
+    for (int k = 0; k < nmem; ++k) {
+      if (!probs[k].ssa_mask_acx || !probs[k].ssa_mask_acy) { h->err = "missing ssa masks"; return YGPU_ERR_ARG; }
+      memcpy(&mxs[coffs[k]], probs[k].ssa_mask_acx, sizeof(int32_t) * (coffs[k + 1] - coffs[k]));
+      memcpy(&mys[coffs[k]], probs[k].ssa_mask_acy, sizeof(int32_t) * (coffs[k + 1] - coffs[k]));
+    }
+    YG_CHECK(h, hipMemcpyAsync((int32_t*)f.mx, mxs.data(), sizeof(int32_t) * total_cells, hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipMemcpyAsync((int32_t*)f.my, mys.data(), sizeof(int32_t) * total_cells, hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    return YGPU_OK;
+  }
+
+  int upload_state(const ygpu_ssa_problem* probs) {
+    int rc;
+    if (diva) {
+      if ((rc = up2(f.ubar_x, YG_FIELD(ux_bar).data(), 0, false))) return rc;
+      if ((rc = up2(f.ubar_y, YG_FIELD(uy_bar).data(), 0, false))) return rc;
+      if ((rc = up2(f.duxdz, YG_FIELD(duxdz).data(), 1, true))) return rc;
+      if ((rc = up2(f.duydz, YG_FIELD(duydz).data(), 1, true))) return rc;
+    }
+    if ((rc = up2(f.ux, YG_FIELD(ux_b).data(), 0, false))) return rc;
+    if ((rc = up2(f.uy, YG_FIELD(uy_b).data(), 0, false))) return rc;
+    if ((rc = up2(f.taub_x, YG_FIELD(taub_acx).data(), 0, true))) return rc;
+    if ((rc = up2(f.taub_y, YG_FIELD(taub_acy).data(), 0, true))) return rc;
+    if ((rc = up2(f.visc, YG_FIELD(visc_eff).data(), 1, false))) return rc;
+    if ((rc = up2(f.beta, YG_FIELD(beta).data(), 0, true))) return rc;
+    if ((rc = up2(f.beta_acx, YG_FIELD(beta_acx).data(), 0, true))) return rc;
+    if ((rc = up2(f.beta_acy, YG_FIELD(beta_acy).data(), 0, true))) return rc;
+    return YGPU_OK;
+  }
+
+  int download(ygpu_ssa_problem* probs) {
+    int rc;
+    if (diva) {
+      if ((rc = down2(f.ubar_x, YG_FIELDW(ux_bar).data(), 0))) return rc;
+      if ((rc = down2(f.ubar_y, YG_FIELDW(uy_bar).data(), 0))) return rc;
+      if ((rc = down2(f.duxdz, YG_FIELDW(duxdz).data(), 1))) return rc;
+      if ((rc = down2(f.duydz, YG_FIELDW(duydz).data(), 1))) return rc;
+      if ((rc = down2(f.beta_eff, YG_FIELDW(beta_eff).data(), 0))) return rc;
+      if ((rc = down2(dv.ux3, YG_FIELDW(ux).data(), 1))) return rc;
+      if ((rc = down2(dv.uy3, YG_FIELDW(uy).data(), 1))) return rc;
+      if ((rc = diva_down_interior(probs))) return rc;
+    }
+    if ((rc = down2(f.ux, YG_FIELDW(ux_b).data(), 0))) return rc;
+    if ((rc = down2(f.uy, YG_FIELDW(uy_b).data(), 0))) return rc;
+    if ((rc = down2(f.taub_x, YG_FIELDW(taub_acx).data(), 0))) return rc;
+    if ((rc = down2(f.taub_y, YG_FIELDW(taub_acy).data(), 0))) return rc;
+    if ((rc = down2(f.visc, YG_FIELDW(visc_eff).data(), 1))) return rc;
+    if ((rc = down2(f.visc_int, YG_FIELDW(visc_eff_int).data(), 0))) return rc;
+    if ((rc = down2(f.err_x, YG_FIELDW(ssa_err_acx).data(), 0))) return rc;
+    if ((rc = down2(f.err_y, YG_FIELDW(ssa_err_acy).data(), 0))) return rc;
+    if ((rc = down2(f.beta, YG_FIELDW(beta).data(), 0))) return rc;
+    if ((rc = down2(f.beta_acx, YG_FIELDW(beta_acx).data(), 0))) return rc;
+    if ((rc = down2(f.beta_acy, YG_FIELDW(beta_acy).data(), 0))) return rc;
+    std::vector<int> it(nmem), li(nmem), lf(nmem), ln(nmem), dn(nmem);
+    std::vector<double> rs(nmem);
+    YG_CHECK(h, hipMemcpy(it.data(), d_iter_now, sizeof(int) * nmem, hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(li.data(), d_lin_iter, sizeof(int) * nmem, hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(lf.data(), d_lin_fail, sizeof(int) * nmem, hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(ln.data(), d_lim_n, sizeof(int) * nmem, hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(dn.data(), d_done, sizeof(int) * nmem, hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(rs.data(), d_resid, sizeof(double) * nmem, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nmem; ++k) {
+      probs[k].ssa_iter_now = it[k];
+      probs[k].ssa_lin_iter = li[k];
+      probs[k].ssa_lin_fail = lf[k];
+      probs[k].ssa_lim_n = ln[k];
+      probs[k].picard_resid = rs[k];
+      probs[k].status = dn[k] < 0 ? YGPU_ERR_NONFINITE : YGPU_OK;
+    }
+    return YGPU_OK;
+  }
+
+  int diva_down_interior(ygpu_ssa_problem* probs);
+
+  // the Picard loop (velocity_ssa.f90:172-334 / velocity_diva.f90:195-404)
+  int run(double* seconds) {
+    hipStream_t st = h->stream;
+    YG_CHECK(h, hipMemsetAsync(d_done, 0, sizeof(int) * nmem, st));
+    YG_CHECK(h, hipMemsetAsync(d_lin_iter, 0, sizeof(int) * nmem, st));
+    YG_CHECK(h, hipMemsetAsync(d_lin_fail, 0, sizeof(int) * nmem, st));
+    YG_CHECK(h, hipMemsetAsync(d_iter_now, 0, sizeof(int) * nmem, st));
+    YG_CHECK(h, hipMemsetAsync(d_resid, 0, sizeof(double) * nmem, st));
+    hipEvent_t e0, e1;
+    YG_CHECK(h, hipEventCreate(&e0));
+    YG_CHECK(h, hipEventCreate(&e1));
+    YG_CHECK(h, hipEventRecord(e0, st));
+    const YgBatchGeom G = geom();
+    yg_launch_init<R>(G, f, fp, st);
+    if (diva) yg_launch_diva_init<R>(G, f, dv, fp, st);
+    YgPicardState ps{d_iter_now, d_done, d_resid, d_lin_iter, d_lin_fail, d_lim_n};
+    YgKrylovArgs ka{};
+    ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask; ka.cellij = d_cellij;
+    ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.b = d_b; ka.x = d_x;
+    ka.W64 = d_W64; ka.W32 = d_W32; ka.total_cells = total_cells;
+    ka.wrapx = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_X);
+    ka.wrapy = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_Y || fp.BC == YG_BC_MISMIP3D ||
+                fp.BC == YG_BC_TROUGH);
+    ka.skip = d_done; ka.out_iters = d_kit; ka.out_status = d_kst; ka.out_resid = d_kres;
+    ka.opts = opts.k;
+    std::vector<int> done(nmem), kit(nmem);
+    hipEvent_t k0 = nullptr, k1 = nullptr;
+    if (h->stats_on) {
+      YG_CHECK(h, hipEventCreate(&k0));
+      YG_CHECK(h, hipEventCreate(&k1));
+    }
+    for (int iter = 1; iter <= iter_max; ++iter) {
+      if (diva) yg_launch_diva_pre<R>(G, f, dv, fp, d_done, st);
+      yg_launch_fields_pre<R>(G, f, fp, d_done, iter == 1, st);
+      if (diva) yg_launch_diva_mid<R>(G, f, dv, fp, d_done, st);
+      yg_launch_assemble<R>(G, f, fp, d_done, d_A64, kmode == YG_MODE_MIXED ? d_A32 : nullptr,
+                            d_b, d_x, d_mask, d_cellij, st);
+      std::vector<int> active;
+      if (h->stats_on) {
+        for (int k = 0; k < nmem; ++k) if (!done[k]) active.push_back(k);
+        YG_CHECK(h, hipEventRecord(k0, st));
+      }
+      yg_launch_krylov(ka, kmode, nmem, block, st);
+      if (h->stats_on) YG_CHECK(h, hipEventRecord(k1, st));
+      yg_launch_epilogue<R>(G, f, fp, d_x, d_kit, d_kst, iter, iter_max, ps, st);
+      if (diva) yg_launch_diva_post_iter<R>(G, f, fp, d_iter_now, iter, st);
+      YG_CHECK(h, hipGetLastError());
+      YG_CHECK(h, hipMemcpyAsync(done.data(), d_done, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
+      if (h->stats_on)
+        YG_CHECK(h, hipMemcpyAsync(kit.data(), d_kit, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
+      YG_CHECK(h, hipStreamSynchronize(st));
+      if (h->stats_on) {
+        float kms = 0.f;
+        YG_CHECK(h, hipEventElapsedTime(&kms, k0, k1));
+        h->stats.krylov_launches += 1;
+        h->stats.krylov_ms += kms;
+        const bool mixed = kmode == YG_MODE_MIXED;
+        for (int k : active) {
+          const double C = (double)mem[k].nx * mem[k].ny;
+          const int it = kit[k];
+          h->stats.krylov_bytes += mixed ? krylov_bytes(C, it, 4, 4, 4, opts.k.precon == 2) +
+                                               4.0 * (18.0 * C * 8 + 8.0 * C + 6.0 * 2.0 * C * 8)
+                                         : krylov_bytes(C, it, 8, 8, 8, opts.k.precon == 2);
+          h->stats.krylov_iters += it;
+          h->stats.unknown_iters += 2.0 * C * it;
+          h->stats.picard_iters += 1;
+        }
+      }
+      bool all = true;
+      for (int k = 0; k < nmem; ++k) all &= done[k] != 0;
+      if (all) break;
+    }
+    if (k0) (void)hipEventDestroy(k0);
+    if (k1) (void)hipEventDestroy(k1);
+    yg_launch_finish<R>(G, f, fp, ps, st);
+    if (diva) yg_launch_diva_finish<R>(G, f, dv, fp, st);
+    YG_CHECK(h, hipGetLastError());
+    YG_CHECK(h, hipEventRecord(e1, st));
+    YG_CHECK(h, hipEventSynchronize(e1));
+    float ms = 0.f;
+    YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (seconds) *seconds = ms * 1e-3;
+    return YGPU_OK;
+  }
+};
+
+template <typename R>
+int Batch<R>::diva_down_interior(ygpu_ssa_problem* probs) {
+  // ux_i = ux - ux_b for every level (velocity_diva.f90:419-423), on the host
+  for (int k = 0; k < nmem; ++k) {
+    ygpu_ssa_problem& P = probs[k];
+    if (!P.ux_i || !P.uy_i || !P.ux || !P.uy) continue;
+    (void)P;
+  }
+  return YGPU_OK;
+}
+
+template <typename R>
+int run_batch(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_problem* probs, int diva) {
+  Batch<R> B;
+  int rc = B.setup(h, par, np, probs, diva);
+  if (rc) return rc;
+  if ((rc = B.upload_inputs(probs))) return rc;
+  if ((rc = B.upload_state(probs))) return rc;
+  double secs = 0.0;
+  if ((rc = B.run(&secs))) return rc;
+  if ((rc = B.download(probs))) return rc;
+  if (diva) {
+    // interior velocity of each level: 3D velocity minus basal velocity
+    for (int k = 0; k < np; ++k) {
+      ygpu_ssa_problem& P = probs[k];
+      if (!P.ux_i || !P.uy_i || !P.ux || !P.uy || !P.ux_b || !P.uy_b) continue;
+      const long long nc = (long long)P.nx * P.ny;
+      for (int z = 0; z < P.nz; ++z)
+        for (long long c = 0; c < nc; ++c) {
+          if (P.real_bytes == 4) {
+            ((float*)P.ux_i)[c + z * nc] = ((float*)P.ux)[c + z * nc] - ((float*)P.ux_b)[c];
+            ((float*)P.uy_i)[c + z * nc] = ((float*)P.uy)[c + z * nc] - ((float*)P.uy_b)[c];
+          } else {
+            ((double*)P.ux_i)[c + z * nc] = ((double*)P.ux)[c + z * nc] - ((double*)P.ux_b)[c];
+            ((double*)P.uy_i)[c + z * nc] = ((double*)P.uy)[c + z * nc] - ((double*)P.uy_b)[c];
+          }
+        }
+    }
+  }
+  for (int k = 0; k < np; ++k) probs[k].solve_seconds = secs;
+  return YGPU_OK;
+}
+
+int run_any(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_problem* probs, int diva) {
+  if (!h || !par || np < 1 || !probs) return YGPU_ERR_ARG;
+  h->err.clear();
+  if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  if (par->precision == YGPU_PREC_FP64) return run_batch<double>(h, par, np, probs, diva);
+  if (par->precision == YGPU_PREC_REF || par->precision == YGPU_PREC_MIXED)
+    return run_batch<float>(h, par, np, probs, diva);
+  h->err = "unknown precision";
+  return YGPU_ERR_ARG;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+extern "C" {
+
+int ygpu_create(int32_t device, ygpu_handle** out) {
+  if (!out) return YGPU_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return YGPU_ERR_DEVICE;
+  if (device < 0 || device >= n) return YGPU_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return YGPU_ERR_DEVICE;
+  ygpu_handle* h = new ygpu_handle();
+  h->device = device;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return YGPU_ERR_DEVICE;
+  }
+  *out = h;
+  return YGPU_OK;
+}
+
+int ygpu_destroy(ygpu_handle* h) {
+  if (!h) return YGPU_ERR_ARG;
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return YGPU_OK;
+}
+
+const char* ygpu_last_error(const ygpu_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+void ygpu_default_params(ygpu_ssa_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->ssa_solver = 0;
+  p->boundaries = YGPU_BND_ZEROS;
+  p->visc_method = 1;
+  p->beta_method = 1;
+  p->beta_gl_scale = 0;
+  p->beta_gl_stag = 1;
+  p->ssa_vel_lim_method = 1;
+  p->ssa_iter_max = 20;
+  p->ssa_write_log = 0;
+  p->precision = YGPU_PREC_REF;
+  p->visc_const = 1e7; p->beta_const = 1e3; p->beta_q = 1.0; p->beta_u0 = 100.0;
+  p->beta_gl_f = 1.0; p->H_grnd_lim = 500.0; p->beta_min = 100.0; p->eps_0 = 1e-6;
+  p->ssa_vel_lim_tau = 1e5; p->ssa_vel_max = 1e4; p->ssa_iter_rel = 0.7; p->ssa_iter_conv = 1e-2;
+  p->rho_ice = 910.0; p->rho_sw = 1028.0; p->g = 9.81;
+  p->no_slip = 0;
+  p->block = 0;
+  strncpy(p->ssa_lis_opt, "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false",
+          sizeof(p->ssa_lis_opt) - 1);
+}
+
+int ygpu_velocity_ssa_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
+                            ygpu_ssa_problem* probs) {
+  return run_any(h, par, nprob, probs, 0);
+}
+
+int ygpu_velocity_diva_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
+                             ygpu_ssa_problem* probs) {
+  return run_any(h, par, nprob, probs, 1);
+}
+
+int ygpu_velocity_ssa(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int32_t ny,
+                      int32_t nz, float* ux_b, float* uy_b, float* taub_acx, float* taub_acy,
+                      float* visc_eff, float* visc_eff_int, int32_t* ssa_mask_acx,
+                      int32_t* ssa_mask_acy, float* ssa_err_acx, float* ssa_err_acy,
+                      int32_t* ssa_iter_now, int32_t* ssa_lin_iter, int32_t* ssa_lin_fail,
+                      int32_t* ssa_lim_n, float* beta, float* beta_acx, float* beta_acy,
+                      const float* c_bed, const float* f_slide, const float* taud_acx,
+                      const float* taud_acy, const float* taul_int_acx,
+                      const float* taul_int_acy, const float* H_ice, const float* f_ice,
+                      const float* H_grnd, const float* f_grnd, const float* f_grnd_acx,
+                      const float* f_grnd_acy, const float* ATT, const float* zeta_aa,
+                      const float* z_sl, const float* z_bed, const float* z_srf, float dx,
+                      float dy, float n_glen) {
+  ygpu_ssa_problem P;
+  memset(&P, 0, sizeof(P));
+  P.nx = nx; P.ny = ny; P.nz = nz; P.real_bytes = 4;
+  P.dx = dx; P.dy = dy; P.n_glen = n_glen;
+  P.ux_b = ux_b; P.uy_b = uy_b; P.taub_acx = taub_acx; P.taub_acy = taub_acy;
+  P.visc_eff = visc_eff; P.visc_eff_int = visc_eff_int;
+  P.ssa_mask_acx = ssa_mask_acx; P.ssa_mask_acy = ssa_mask_acy;
+  P.ssa_err_acx = ssa_err_acx; P.ssa_err_acy = ssa_err_acy;
+  P.beta = beta; P.beta_acx = beta_acx; P.beta_acy = beta_acy;
+  P.c_bed = c_bed; P.f_slide = f_slide; P.taud_acx = taud_acx; P.taud_acy = taud_acy;
+  P.taul_int_acx = taul_int_acx; P.taul_int_acy = taul_int_acy; P.H_ice = H_ice; P.f_ice = f_ice;
+  P.H_grnd = H_grnd; P.f_grnd = f_grnd; P.f_grnd_acx = f_grnd_acx; P.f_grnd_acy = f_grnd_acy;
+  P.ATT = ATT; P.zeta_aa = zeta_aa; P.z_sl = z_sl; P.z_bed = z_bed; P.z_srf = z_srf;
+  int rc = run_any(h, par, 1, &P, 0);
+  if (rc) return rc;
+  if (ssa_iter_now) *ssa_iter_now = P.ssa_iter_now;
+  if (ssa_lin_iter) *ssa_lin_iter = P.ssa_lin_iter;
+  if (ssa_lin_fail) *ssa_lin_fail = P.ssa_lin_fail;
+  if (ssa_lim_n) *ssa_lim_n = P.ssa_lim_n;
+  return P.status;
+}
+
+// ---- inner boundary -------------------------------------------------------
+int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
+                   const int32_t* a_idx1, const double* a_val, const double* b, double* x_inout,
+                   const char* opts, int32_t* iters, int32_t* status, double* rel_resid,
+                   double* seconds) {
+  if (!h) return YGPU_ERR_ARG;
+  h->err.clear();
+  if (n <= 0 || (n & 1) || !a_ptr1 || !a_idx1 || !a_val || !b || !x_inout || !iters || !status ||
+      !rel_resid) {
+    h->err = "invalid arguments (n must be even: two unknowns per cell)";
+    return YGPU_ERR_ARG;
+  }
+  if (a_ptr1[0] != 1 || a_ptr1[n] - 1 != nnz || nnz < 0) {
+    h->err = "a_ptr must be 1-based with a_ptr(n+1)-1 == nnz";
+    return YGPU_ERR_ARG;
+  }
+  Opts o;
+  std::string e;
+  int rc = parse_opts(opts, o, e);
+  if (rc) { h->err = e; return rc; }
+  if (seconds) *seconds = 0.0;
+  // b == 0 shortcut (solver_linear.F90:162-174)
+  bool allzero = true;
+  for (int q = 0; q < n; ++q) if (b[q] != 0.0) { allzero = false; break; }
+  if (allzero) {
+    for (int q = 0; q < n; ++q) x_inout[q] = 0.0;
+    *iters = 0; *status = 0; *rel_resid = 0.0;
+    return YGPU_OK;
+  }
+  const int cells = n / 2;
+  // grid: ny from the widest coupling (uy(i+1,j) is 2 ny + 1 columns right of ux(i,j))
+  int ny = o.ny_hint;
+  if (ny <= 0) {
+    long long maxd = 0;
+    for (int r = 0; r < n; ++r)
+      for (int k = a_ptr1[r] - 1; k < a_ptr1[r + 1] - 1; ++k) maxd = std::max<long long>(maxd, (long long)(a_idx1[k] - 1) - r);
+    ny = maxd >= 2 ? (int)(maxd / 2) : cells;
+  }
+  if (ny <= 0 || cells % ny) { h->err = "cannot infer the grid of this CSR pattern (give -ygpu_ny)"; return YGPU_ERR_UNSUPPORTED; }
+  const int nx = cells / ny;
+  if (nx > YG_MAX_LINES || ny > 65535) { h->err = "grid too large"; return YGPU_ERR_UNSUPPORTED; }
+  std::vector<int> soff;
+  int T;
+  step_table(nx, ny, soff, T);
+  if (T > YG_MAX_STEPS) { h->err = "grid too large for the step table"; return YGPU_ERR_UNSUPPORTED; }
+  // CSR -> stencil slots in skewed order
+  std::vector<double> A((size_t)18 * cells, 0.0), bs(n), xs(n);
+  std::vector<uint32_t> mask(cells, 0u), cellij(cells);
+  static const int SU[2][9][3] = {
+      {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {1, 0, 1}, {1, -1, 1}, {0, -1, 1}},
+      {{0, 0, 1}, {0, 1, 1}, {0, -1, 1}, {1, 0, 1}, {-1, 0, 1}, {0, 0, 0}, {0, 1, 0}, {-1, 1, 0}, {-1, 0, 0}}};
+  for (int r = 0; r < n; ++r) {
+    const int cell = r / 2, comp = r & 1;
+    const int i = cell / ny, j = cell % ny;
+    const int q = skew_q(soff, i, j, ny);
+    cellij[q] = ((uint32_t)i << 16) | (uint32_t)j;
+    bs[2 * q + comp] = b[r];
+    xs[2 * q + comp] = x_inout[r];
+    for (int k = a_ptr1[r] - 1; k < a_ptr1[r + 1] - 1; ++k) {
+      const int col = a_idx1[k] - 1;
+      if (col < 0 || col >= n) { h->err = "column index out of range"; return YGPU_ERR_ARG; }
+      const int c2 = col / 2, comp2 = col & 1;
+      const int di = c2 / ny - i, dj = c2 % ny - j;
+      int s = -1;
+      for (int t = 0; t < 9; ++t)
+        if (SU[comp][t][0] == di && SU[comp][t][1] == dj && SU[comp][t][2] == comp2) { s = t; break; }
+      if (s < 0) { h->err = "entry outside the C-grid SSA stencil"; return YGPU_ERR_UNSUPPORTED; }
+      A[((size_t)s * cells + q) * 2 + comp] += a_val[k];
+      mask[q] |= 1u << (16 * comp + s);
+    }
+    if (!((mask[q] >> (16 * comp)) & 1u) && o.k.precon != 0) {
+      h->err = "row without a diagonal entry";
+      return YGPU_ERR_UNSUPPORTED;
+    }
+  }
+  const int kmode = o.prec == 2 ? YG_MODE_MIXED : YG_MODE_FP64;
+  if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  Arena ar;
+  YgMember M;
+  M.nx = nx; M.ny = ny; M.nz = 1; M.T = T; M.coff = 0; M.toff = 0; M.dx = M.dy = 1.0;
+  YgKrylovArgs ka{};
+  YgMember* d_mem; int* d_soff; uint32_t *d_mask, *d_cellij; double *d_A64, *d_b, *d_x, *d_F64, *d_W64;
+  float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
+  int *d_it, *d_st; double* d_res;
+  YG_CHECK(h, ar.alloc(&d_mem, 1)); YG_CHECK(h, ar.alloc(&d_soff, soff.size()));
+  YG_CHECK(h, ar.alloc(&d_mask, cells)); YG_CHECK(h, ar.alloc(&d_cellij, cells));
+  YG_CHECK(h, ar.alloc(&d_A64, A.size())); YG_CHECK(h, ar.alloc(&d_b, n)); YG_CHECK(h, ar.alloc(&d_x, n));
+  YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * cells)); YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * n));
+  YG_CHECK(h, ar.alloc(&d_it, 1)); YG_CHECK(h, ar.alloc(&d_st, 1)); YG_CHECK(h, ar.alloc(&d_res, 1));
+  hipStream_t st = h->stream;
+  YG_CHECK(h, hipMemcpyAsync(d_mem, &M, sizeof(M), hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_mask, mask.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_cellij, cellij.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_A64, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_b, bs.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_x, xs.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  std::vector<float> A32;
+  if (kmode == YG_MODE_MIXED) {
+    A32.resize(A.size());
+    for (size_t q = 0; q < A.size(); ++q) A32[q] = (float)A[q];
+    YG_CHECK(h, ar.alloc(&d_A32, A32.size())); YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * cells));
+    YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * n));
+    YG_CHECK(h, hipMemcpyAsync(d_A32, A32.data(), sizeof(float) * A32.size(), hipMemcpyHostToDevice, st));
+  }
+  ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask; ka.cellij = d_cellij;
+  ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.b = d_b; ka.x = d_x;
+  ka.W64 = d_W64; ka.W32 = d_W32; ka.total_cells = cells; ka.wrapx = 0; ka.wrapy = 0;
+  ka.skip = nullptr; ka.out_iters = d_it; ka.out_status = d_st; ka.out_resid = d_res;
+  ka.opts = o.k;
+  hipEvent_t e0, e1;
+  YG_CHECK(h, hipEventCreate(&e0)); YG_CHECK(h, hipEventCreate(&e1));
+  YG_CHECK(h, hipEventRecord(e0, st));
+  const int block = std::min(1024, std::max(256, ((nx + 63) / 64) * 64));
+  yg_launch_krylov(ka, kmode, 1, block, st);
+  YG_CHECK(h, hipGetLastError());
+  YG_CHECK(h, hipEventRecord(e1, st));
+  YG_CHECK(h, hipEventSynchronize(e1));
+  float ms = 0.f;
+  YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  if (seconds) *seconds = ms * 1e-3;
+  YG_CHECK(h, hipMemcpy(xs.data(), d_x, sizeof(double) * n, hipMemcpyDeviceToHost));
+  int it = 0, stt = 0; double res = 0.0;
+  YG_CHECK(h, hipMemcpy(&it, d_it, sizeof(int), hipMemcpyDeviceToHost));
+  YG_CHECK(h, hipMemcpy(&stt, d_st, sizeof(int), hipMemcpyDeviceToHost));
+  YG_CHECK(h, hipMemcpy(&res, d_res, sizeof(double), hipMemcpyDeviceToHost));
+  for (int r = 0; r < n; ++r) {
+    const int cell = r / 2, comp = r & 1;
+    x_inout[r] = xs[2 * skew_q(soff, cell / ny, cell % ny, ny) + comp];
+  }
+  *iters = it; *status = stt; *rel_resid = res;
+  return YGPU_OK;
+}
+
+}  // extern "C"
+
+int ygpu_stats_enable(ygpu_handle* h, int32_t on) {
+  if (!h) return YGPU_ERR_ARG;
+  h->stats_on = on != 0;
+  h->stats = ygpu_stats{};
+  return YGPU_OK;
+}
+
+int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out) {
+  if (!h || !out) return YGPU_ERR_ARG;
+  *out = h->stats;
+  h->stats = ygpu_stats{};
+  return YGPU_OK;
+}
+
+// ---- sessions and admission ------------------------------------------------
+struct YgSessionBase {
+  virtual ~YgSessionBase() {}
+  virtual int solve(ygpu_ssa_problem* P) = 0;
+  int64_t bytes = 0;
+};
+
+template <typename R>
+struct YgSessionT : YgSessionBase {
+  Batch<R> B;
+  bool first = true;
+  int solve(ygpu_ssa_problem* P) override {
+    int rc;
+    if ((rc = B.upload_inputs(P))) return rc;
+    // the device-resident velocity, viscosity and friction are the warm start
+    if (first && (rc = B.upload_state(P))) return rc;
+    double secs = 0.0;
+    if ((rc = B.run(&secs))) return rc;
+    if ((rc = B.download(P))) return rc;
+    P->solve_seconds = secs;
+    first = false;
+    return YGPU_OK;
+  }
+};
+
+struct ygpu_session {
+  ygpu_handle* h;
+  YgSessionBase* s;
+};
+
+extern "C" {
+
+int ygpu_session_create(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob,
+                        ygpu_session** out, int64_t* device_bytes) {
+  if (!h || !par || !prob || !out) return YGPU_ERR_ARG;
+  *out = nullptr;
+  h->err.clear();
+  if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  YgSessionBase* sb = nullptr;
+  int rc;
+  if (par->precision == YGPU_PREC_FP64) {
+    auto* t = new YgSessionT<double>();
+    rc = t->B.setup(h, par, 1, prob, 0);
+    t->bytes = (int64_t)t->B.ar.bytes;
+    sb = t;
+  } else {
+    auto* t = new YgSessionT<float>();
+    rc = t->B.setup(h, par, 1, prob, 0);
+    t->bytes = (int64_t)t->B.ar.bytes;
+    sb = t;
+  }
+  if (rc) { delete sb; return rc; }
+  if (device_bytes) *device_bytes = sb->bytes;
+  *out = new ygpu_session{h, sb};
+  return YGPU_OK;
+}
+
+int ygpu_session_solve(ygpu_session* s, ygpu_ssa_problem* prob) {
+  if (!s || !prob) return YGPU_ERR_ARG;
+  s->h->err.clear();
+  if (hipSetDevice(s->h->device) != hipSuccess) { s->h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  return s->s->solve(prob);
+}
+
+int ygpu_session_destroy(ygpu_session* s) {
+  if (!s) return YGPU_ERR_ARG;
+  delete s->s;
+  delete s;
+  return YGPU_OK;
+}
+
+int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny, int32_t nz) {
+  if (!par) return -1;
+  const int64_t C = (int64_t)nx * ny, Z = C * nz;
+  const int64_t rb = par->precision == YGPU_PREC_FP64 ? 8 : 4;
+  int64_t b = C * (4 + 4 + 18 * 8 + 2 * 8 + 2 * 8 + YG_NW64 * 2 * 8 + 8);
+  if (par->precision == YGPU_PREC_MIXED) b += C * (18 * 4 + YG_NF * 4 + YG_NW32 * 2 * 4);
+  else b += C * YG_NF * 8;
+  b += rb * (C * 34 + Z * 3 + 2 * Z);
+  return b;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// ygpu_fields.h -- per-Picard-iteration field kernels of the SSA solve.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ygpu_common.h"
+
+// BND_* codes (yelmo_tools.f90:14-20)
+#define YG_BC_ZEROS 0
+#define YG_BC_INFINITE 1
+#define YG_BC_MISMIP3D 2
+#define YG_BC_TROUGH 3
+#define YG_BC_PERIODIC 4
+#define YG_BC_PERIODIC_X 5
+#define YG_BC_PERIODIC_Y 6
+
+// boundaries strings, numbered as YGPU_BND_* in include/ygpu.h
+#define YG_B_ZEROS 0
+#define YG_B_INFINITE 1
+#define YG_B_MASK 2
+#define YG_B_MISMIP3D 3
+#define YG_B_TROUGH 4
+#define YG_B_PERIODIC 5
+#define YG_B_PERIODIC_X 6
+#define YG_B_PERIODIC_Y 7
+
+struct YgFieldParams {
+  int bnd;             // YG_B_*
+  int BC;              // YG_BC_*
+  int visc_method, beta_method, beta_gl_scale, beta_gl_stag, vel_lim_method;
+  int diva;            // 1: DIVA viscosity / beta_eff
+  int no_slip;         // DIVA no-slip variant
+  double visc_const, beta_const, beta_q, beta_u0, beta_gl_f, H_grnd_lim, beta_min, eps_0,
+      vel_lim_tau, vel_max, iter_rel, iter_conv, n_glen;
+};
+
+// Device arrays of one batch, natural (Fortran) cell order, type R.
+template <typename R>
+struct YgFields {
+  // inputs (2D unless noted)
+  const R *c_bed, *f_slide, *taud_acx, *taud_acy, *taul_acx, *taul_acy, *H_ice, *f_ice, *H_grnd,
+      *f_grnd, *z_bed, *z_sl;
+  const R* ATT;      // 3D, per member at zoff
+  const R* zeta;     // nz per member at member index * zmax
+  int zstride;
+  const int32_t *mx, *my;
+  // state
+  R *ux, *uy, *ux_nm1, *uy_nm1;
+  R *visc, *visc_nm1;   // 3D
+  R *visc_int;
+  R *beta, *beta_acx, *beta_acy, *bss_x, *bss_y;  // bss: friction used by the matrix
+  R *lkx, *lky, *lrx, *lry;                       // limit drag
+  R *dudx, *dudy, *dvdx, *dvdy;                   // strain rates
+  R *err_x, *err_y, *taub_x, *taub_y;
+  // DIVA
+  R *duxdz, *duydz;           // 3D
+  R *F2, *F2_acx, *F2_acy;
+  R *beff_acx, *beff_acy, *beta_eff;
+  R *ubar_x, *ubar_y;         // DIVA depth-averaged velocity (the solved field)
+};
+
+struct YgBatchGeom {
+  const YgMember* members;
+  const long long* coffs;     // member cell offsets, nmem+1 entries (device)
+  const long long* zoffs;     // member 3D offsets
+  const int* soff;            // step tables
+  int nmem;
+  long long total_cells;
+};
+
+template <typename R>
+void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                          const int* skip, int first_iter, hipStream_t st);
+
+template <typename R>
+void yg_launch_assemble(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                        const int* skip, double* A64, float* A32, double* b, double* x0,
+                        uint32_t* mask, uint32_t* cellij, hipStream_t st);
+
+struct YgPicardState {
+  int* iter_now;      // per member
+  int* done;          // per member: 0 running, 1 converged, 2 iteration limit, -1 non-finite
+  double* resid;      // per member Picard residual of the last iteration
+  int* lin_iter;      // per member sums
+  int* lin_fail;
+  int* lim_n;
+};
+
+template <typename R>
+void yg_launch_epilogue(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                        const double* x, const int* lin_iters, const int* lin_status, int iter,
+                        int iter_max, YgPicardState ps, hipStream_t st);
+
+template <typename R>
+void yg_launch_finish(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                      YgPicardState ps, hipStream_t st);
+
+template <typename R>
+void yg_launch_init(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                    hipStream_t st);
+
+// natural <-> skewed layout helpers
+void yg_launch_cellij(const YgBatchGeom& G, uint32_t* cellij, hipStream_t st);

@@ -1,0 +1,992 @@
+// ygpu_fields.hip -- grid-wide field kernels of one Picard iteration.
+//
+// Every kernel runs over all cells of all members of a batch (thread per
+// cell, member found from the packed cell offsets), so the whole GPU works on
+// these phases even when the Krylov solve owns one CU per member.  Arrays are
+// in the reference's Fortran order (i fastest) inside each member.  The
+// arithmetic follows the reference routines cited at each kernel; the order
+// of operations is kept so FP64 results agree with the FP64 oracle to
+// rounding.
+#include "ygpu_fields.h"
+#include "ygpu_krylov.h"
+
+namespace {
+
+#define I2(i, j) ((i) + nx * (j))
+
+template <typename R> YG_DEV R ypow(R a, R b);
+template <> YG_DEV float ypow<float>(float a, float b) { return powf(a, b); }
+template <> YG_DEV double ypow<double>(double a, double b) { return pow(a, b); }
+template <typename R> YG_DEV R ysqrt(R a);
+template <> YG_DEV float ysqrt<float>(float a) { return sqrtf(a); }
+template <> YG_DEV double ysqrt<double>(double a) { return sqrt(a); }
+template <typename R> YG_DEV R yexp(R a);
+template <> YG_DEV float yexp<float>(float a) { return expf(a); }
+template <> YG_DEV double yexp<double>(double a) { return exp(a); }
+template <typename R> YG_DEV R ylog(R a);
+template <> YG_DEV float ylog<float>(float a) { return logf(a); }
+template <> YG_DEV double ylog<double>(double a) { return log(a); }
+template <typename R> YG_DEV R yabs(R a) { return a < R(0) ? -a : a; }
+
+// default-real literal of the reference (single precision), converted to R
+#define RS(x) ((R)(float)(x))
+
+YG_DEV int find_member(const long long* coffs, int nmem, long long gc) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (coffs[mid] <= gc) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// get_neighbor_indices_bc_codes (yelmo_tools.f90:118-201), 0-based
+YG_DEV void nbr(int i, int j, int nx, int ny, int BC, int& im1, int& ip1, int& jm1, int& jp1) {
+  const bool wx = BC == YG_BC_PERIODIC || BC == YG_BC_PERIODIC_X;
+  const bool wy = BC == YG_BC_MISMIP3D || BC == YG_BC_TROUGH || BC == YG_BC_PERIODIC ||
+                  BC == YG_BC_PERIODIC_Y;
+  if (wx) { im1 = i == 0 ? nx - 1 : i - 1; ip1 = i == nx - 1 ? 0 : i + 1; }
+  else { im1 = i > 0 ? i - 1 : 0; ip1 = i < nx - 1 ? i + 1 : nx - 1; }
+  if (wy) { jm1 = j == 0 ? ny - 1 : j - 1; jp1 = j == ny - 1 ? 0 : j + 1; }
+  else { jm1 = j > 0 ? j - 1 : 0; jp1 = j < ny - 1 ? j + 1 : ny - 1; }
+}
+
+struct CellCtx {
+  int m, nx, ny, nz, i, j;
+  long long c0;   // member cell offset
+  long long z0;   // member 3D offset
+  double dx, dy;
+};
+
+YG_DEV bool cell_of(const YgBatchGeom& G, long long gc, const int* skip, CellCtx& cc) {
+  if (gc >= G.total_cells) return false;
+  const int m = find_member(G.coffs, G.nmem, gc);
+  if (skip && skip[m]) return false;
+  const YgMember& mb = G.members[m];
+  cc.m = m; cc.nx = mb.nx; cc.ny = mb.ny; cc.nz = mb.nz;
+  cc.c0 = G.coffs[m]; cc.z0 = G.zoffs[m];
+  const int loc = (int)(gc - cc.c0);
+  cc.i = loc % mb.nx; cc.j = loc / mb.nx;
+  cc.dx = mb.dx; cc.dy = mb.dy;
+  return true;
+}
+
+// ---- strain rates (deformation.f90:1316-1507) ---------------------------
+template <typename R>
+YG_DEV R deriv_active(R vm, R v0, R vp, bool am, bool ap, R h) {
+  if (am && ap) return (vp - vm) / (R(2) * h);
+  if (ap) return (vp - v0) / h;
+  if (am) return (v0 - vm) / h;
+  return R(0);
+}
+
+template <typename R>
+YG_DEV bool act_x(const R* fi, int nx, int ny, int BC, int i, int j) {
+  int im1, ip1, jm1, jp1;
+  nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+  return fi[I2(i, j)] == R(1) || fi[I2(ip1, j)] == R(1);
+}
+template <typename R>
+YG_DEV bool act_y(const R* fi, int nx, int ny, int BC, int i, int j) {
+  int im1, ip1, jm1, jp1;
+  nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+  return fi[I2(i, j)] == R(1) || fi[I2(i, jp1)] == R(1);
+}
+
+template <typename R>
+__global__ void k_strain(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
+                         int first_iter) {
+  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  CellCtx cc;
+  if (!cell_of(G, gc, skip, cc)) return;
+  const int nx = cc.nx, ny = cc.ny, i = cc.i, j = cc.j, BC = p.BC;
+  const R* ux = (p.diva ? f.ubar_x : f.ux) + cc.c0;
+  const R* uy = (p.diva ? f.ubar_y : f.uy) + cc.c0;
+  const R* fi = f.f_ice + cc.c0;
+  const R dx = (R)cc.dx, dy = (R)cc.dy;
+  int im1, ip1, jm1, jp1, im2, ip2, jm2, jp2, d1, d2, d3;
+  nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+  const int c = I2(i, j);
+  R dudx = (ux[I2(ip1, j)] - ux[I2(im1, j)]) / (R(2) * dx);
+  const R dudy = deriv_active<R>(ux[I2(i, jm1)], ux[c], ux[I2(i, jp1)], act_x(fi, nx, ny, BC, i, jm1),
+                                 act_x(fi, nx, ny, BC, i, jp1), dy);
+  const R dvdx = deriv_active<R>(uy[I2(im1, j)], uy[c], uy[I2(ip1, j)], act_y(fi, nx, ny, BC, im1, j),
+                                 act_y(fi, nx, ny, BC, ip1, j), dx);
+  R dvdy = (uy[I2(i, jp1)] - uy[I2(i, jm1)]) / (R(2) * dy);
+  if (fi[c] == R(1) && fi[I2(ip1, j)] < R(1)) {
+    nbr(im1, j, nx, ny, BC, im2, d1, d2, d3);
+    if (fi[I2(im1, j)] == R(1) && im2 != im1)
+      dudx = (R(1) * ux[I2(im2, j)] - R(4) * ux[I2(im1, j)] + R(3) * ux[c]) / (R(2) * dx);
+    else
+      dudx = (ux[c] - ux[I2(im1, j)]) / dx;
+  } else if (fi[c] < R(1) && fi[I2(ip1, j)] == R(1)) {
+    nbr(ip1, j, nx, ny, BC, d1, ip2, d2, d3);
+    if (ip2 != ip1 && fi[I2(ip2, j)] == R(1))
+      dudx = -(R(1) * ux[I2(ip2, j)] - R(4) * ux[I2(ip1, j)] + R(3) * ux[c]) / (R(2) * dx);
+    else
+      dudx = (ux[I2(ip1, j)] - ux[c]) / dx;
+  }
+  if (fi[c] == R(1) && fi[I2(i, jp1)] < R(1)) {
+    nbr(i, jm1, nx, ny, BC, d1, d2, jm2, d3);
+    if (fi[I2(i, jm1)] == R(1) && jm2 != jm1)
+      dvdy = (R(1) * uy[I2(i, jm2)] - R(4) * uy[I2(i, jm1)] + R(3) * uy[c]) / (R(2) * dy);
+    else
+      dvdy = (uy[c] - uy[I2(i, jm1)]) / dy;
+  } else if (fi[c] < R(1) && fi[I2(i, jp1)] == R(1)) {
+    nbr(i, jp1, nx, ny, BC, d1, d2, d3, jp2);
+    if (jp2 != jp1 && fi[I2(i, jp2)] == R(1))
+      dvdy = -(R(1) * uy[I2(i, jp2)] - R(4) * uy[I2(i, jp1)] + R(3) * uy[c]) / (R(2) * dy);
+    else
+      dvdy = (uy[I2(i, jp1)] - uy[c]) / dy;
+  }
+  const long long g = cc.c0 + c;
+  f.dudx[g] = dudx; f.dudy[g] = dudy; f.dvdx[g] = dvdx; f.dvdy[g] = dvdy;
+  // state of the previous Picard iteration (velocity_ssa.f90:175-193)
+  f.ux_nm1[g] = ux[c];
+  f.uy_nm1[g] = uy[c];
+}
+
+// ---- 2x2 Gauss quadrature in a cell ----------------------------------------
+// Node values are bilinear interpolations of the cell-corner values at
+// xi, eta = +-1/sqrt(3); corner values are means over the active points of
+// the field's own grid (fesm-utils gaussian_quadrature, used at
+// velocity_ssa.f90:454-471).  Corners and nodes ordered SW, SE, NE, NW.
+template <typename R>
+YG_DEV void corners_to_nodes(const R c[4], R n[4]) {
+  const double A = 0.57735026918962576451;
+  const double xs[4] = {-1, 1, 1, -1}, ys[4] = {-1, -1, 1, 1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double xi = xs[q] * A, eta = ys[q] * A;
+    const double w0 = 0.25 * (1 - xi) * (1 - eta), w1 = 0.25 * (1 + xi) * (1 - eta);
+    const double w2 = 0.25 * (1 + xi) * (1 + eta), w3 = 0.25 * (1 - xi) * (1 + eta);
+    n[q] = (R)(w0 * (double)c[0] + w1 * (double)c[1] + w2 * (double)c[2] + w3 * (double)c[3]);
+  }
+}
+
+template <typename R, int K>
+YG_DEV R mean_act(const R (&v)[K], const bool (&a)[K]) {
+  int cnt = 0;
+  R s = R(0);
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    if (a[q]) { s += v[q]; cnt++; }
+  return cnt > 0 ? s / (R)cnt : R(0);
+}
+
+// kind 0 aa (act: flags or all), 1 acx, 2 acy
+template <typename R, typename ACT>
+YG_DEV void gq_nodes(int kind, R n[4], const R* v, ACT act, int nx, int i, int j, int im1,
+                     int ip1, int jm1, int jp1) {
+  R c[4];
+  if (kind == 0) {
+    const int cs[4][4] = {{I2(im1, jm1), I2(i, jm1), I2(im1, j), I2(i, j)},
+                          {I2(i, jm1), I2(ip1, jm1), I2(i, j), I2(ip1, j)},
+                          {I2(i, j), I2(ip1, j), I2(i, jp1), I2(ip1, jp1)},
+                          {I2(im1, j), I2(i, j), I2(im1, jp1), I2(i, jp1)}};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      R vv[4];
+      bool aa[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) { vv[k] = v[cs[q][k]]; aa[k] = act(cs[q][k]); }
+      c[q] = mean_act<R, 4>(vv, aa);
+    }
+  } else {
+    int cs[4][2];
+    if (kind == 1) {
+      cs[0][0] = I2(im1, jm1); cs[0][1] = I2(im1, j);
+      cs[1][0] = I2(i, jm1);   cs[1][1] = I2(i, j);
+      cs[2][0] = I2(i, j);     cs[2][1] = I2(i, jp1);
+      cs[3][0] = I2(im1, j);   cs[3][1] = I2(im1, jp1);
+    } else {
+      cs[0][0] = I2(im1, jm1); cs[0][1] = I2(i, jm1);
+      cs[1][0] = I2(i, jm1);   cs[1][1] = I2(ip1, jm1);
+      cs[2][0] = I2(i, j);     cs[2][1] = I2(ip1, j);
+      cs[3][0] = I2(im1, j);   cs[3][1] = I2(i, j);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      R vv[2];
+      bool aa[2];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) { vv[k] = v[cs[q][k]]; aa[k] = act(cs[q][k]); }
+      c[q] = mean_act<R, 2>(vv, aa);
+    }
+  }
+  corners_to_nodes<R>(c, n);
+}
+
+// ---- effective viscosity, relaxation and depth integral -----------------
+// calc_visc_eff_3D_nodes / _aa (velocity_ssa.f90:359-586; DIVA forms
+// velocity_diva.f90:573-822 add 1/4 (du/dz)^2 + 1/4 (dv/dz)^2),
+// picard_relax_visc (velocity_general.f90:2188-2213), calc_visc_eff_int
+// (velocity_general.f90:2215-2261).
+template <typename R>
+__global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  CellCtx cc;
+  if (!cell_of(G, gc, skip, cc)) return;
+  const int nx = cc.nx, ny = cc.ny, nz = cc.nz, i = cc.i, j = cc.j, BC = p.BC;
+  const long long nc = (long long)nx * ny;
+  const int c = I2(i, j);
+  const R* fi = f.f_ice + cc.c0;
+  const R visc_min = R(1e5);
+  R* visc = f.visc + cc.z0;
+  R* vnm1 = f.visc_nm1 + cc.z0;
+  const R* ATT = f.ATT + cc.z0;
+  const R* zeta = f.zeta + (long long)cc.m * f.zstride;
+  const R rel = (R)p.iter_rel;
+  const R n_glen = (R)p.n_glen;
+  // keep the previous iterate
+  for (int k = 0; k < nz; ++k) vnm1[c + k * nc] = visc[c + k * nc];
+  if (p.visc_method == 0) {
+    for (int k = 0; k < nz; ++k) visc[c + k * nc] = (R)p.visc_const;
+  } else if (fi[c] != R(1)) {
+    for (int k = 0; k < nz; ++k) visc[c + k * nc] = visc_min;
+  }
+  if (p.visc_method != 0 && fi[c] == R(1)) {
+    const R eps_0 = (R)p.eps_0, eps_0_sq = eps_0 * eps_0;
+    int im1, ip1, jm1, jp1;
+    nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+    const R dx = (R)cc.dx, dy = (R)cc.dy;
+    if (p.visc_method == 1) {
+      const R p1 = (RS(1.0) - n_glen) / (RS(2.0) * n_glen);
+      const R p2 = -RS(1.0) / n_glen;
+      const R* dudx = f.dudx + cc.c0;
+      const R* dudy = f.dudy + cc.c0;
+      const R* dvdx = f.dvdx + cc.c0;
+      const R* dvdy = f.dvdy + cc.c0;
+      auto ax = [&](int qq) { return act_x<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
+      auto ay = [&](int qq) { return act_y<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
+      auto all = [&](int) { return true; };
+      R dudxn[4], dudyn[4], dvdxn[4], dvdyn[4], eps_sq_n[4];
+      gq_nodes<R>(1, dudxn, dudx, ax, nx, i, j, im1, ip1, jm1, jp1);
+      gq_nodes<R>(1, dudyn, dudy, ax, nx, i, j, im1, ip1, jm1, jp1);
+      gq_nodes<R>(2, dvdxn, dvdx, ay, nx, i, j, im1, ip1, jm1, jp1);
+      gq_nodes<R>(2, dvdyn, dvdy, ay, nx, i, j, im1, ip1, jm1, jp1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const R s = dudyn[q] + dvdxn[q];
+        eps_sq_n[q] = dudxn[q] * dudxn[q] + dvdyn[q] * dvdyn[q] + dudxn[q] * dvdyn[q] +
+                      RS(0.25) * (s * s);
+        if (!p.diva) eps_sq_n[q] = eps_sq_n[q] + eps_0_sq;
+      }
+      for (int k = 0; k < nz; ++k) {
+        R ATTn[4];
+        gq_nodes<R>(0, ATTn, ATT + k * nc, all, nx, i, j, im1, ip1, jm1, jp1);
+        R e[4];
+        if (p.diva) {
+          R dzx[4], dzy[4];
+          // DIVA: vertical shear at the nodes from the acx/acy shear fields
+          gq_nodes<R>(1, dzx, f.duxdz + cc.z0 + k * nc, ax, nx, i, j, im1, ip1, jm1, jp1);
+          gq_nodes<R>(2, dzy, f.duydz + cc.z0 + k * nc, ay, nx, i, j, im1, ip1, jm1, jp1);
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            e[q] = eps_sq_n[q] + R(0.25) * (dzx[q] * dzx[q]) + R(0.25) * (dzy[q] * dzy[q]) + eps_0_sq;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) e[q] = eps_sq_n[q];
+        }
+        R sum = R(0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum += (RS(0.5) * ypow<R>(e[q], p1) * ypow<R>(ATTn[q], p2)) * R(1);
+        visc[c + k * nc] = sum / R(4);
+      }
+    } else {
+      const R p1 = (R(1) - n_glen) / (R(2) * n_glen);
+      const R p2 = -R(1) / n_glen;
+      const R* ux = (p.diva ? f.ubar_x : f.ux) + cc.c0;
+      const R* uy = (p.diva ? f.ubar_y : f.uy) + cc.c0;
+      const R dudx_aa = (ux[c] - ux[I2(im1, j)]) / dx;
+      const R dvdy_aa = (uy[c] - uy[I2(i, jm1)]) / dy;
+      const R dudy_aa_1 = (ux[I2(i, jp1)] - ux[I2(i, jm1)]) / (R(2) * dy);
+      const R dudy_aa_2 = (ux[I2(im1, jp1)] - ux[I2(im1, jm1)]) / (R(2) * dy);
+      const R dudy_aa = R(0.5) * (dudy_aa_1 + dudy_aa_2);
+      const R dvdx_aa_1 = (uy[I2(ip1, j)] - uy[I2(im1, j)]) / (R(2) * dx);
+      const R dvdx_aa_2 = (uy[I2(ip1, jm1)] - uy[I2(im1, jm1)]) / (R(2) * dx);
+      const R dvdx_aa = R(0.5) * (dvdx_aa_1 + dvdx_aa_2);
+      const R s = dudy_aa + dvdx_aa;
+      const R eps_sq_h = dudx_aa * dudx_aa + dvdy_aa * dvdy_aa + dudx_aa * dvdy_aa +
+                         R(0.25) * (s * s);
+      const R eps_sq_aa = eps_sq_h + eps_0_sq;
+      for (int k = 0; k < nz; ++k) {
+        R e = eps_sq_aa;
+        if (p.diva) {
+          // DIVA aa form: shear of the cell's own faces, averaged to the centre
+          const R* dzx = f.duxdz + cc.z0 + k * nc;
+          const R* dzy = f.duydz + cc.z0 + k * nc;
+          const R ddx = R(0.5) * (dzx[c] + dzx[I2(im1, j)]);
+          const R ddy = R(0.5) * (dzy[c] + dzy[I2(i, jm1)]);
+          e = eps_sq_h + R(0.25) * (ddx * ddx) + R(0.25) * (ddy * ddy) + eps_0_sq;
+        }
+        visc[c + k * nc] = R(0.5) * ypow<R>(e, p1) * ypow<R>(ATT[c + k * nc], p2);
+      }
+    }
+  }
+  if (p.visc_method != 0 || p.diva) {
+    for (int k = 0; k < nz; ++k) {
+      const long long q = c + k * nc;
+      visc[q] = yexp<R>((RS(1.0) - rel) * ylog<R>(vnm1[q]) + rel * ylog<R>(visc[q]));
+    }
+  }
+  // depth integral (trapezoid from the base, yelmo_tools.f90:1092-1122)
+  R vi;
+  if (fi[c] == R(1)) {
+    R s = R(0);
+    for (int k = 1; k < nz; ++k) {
+      const R mid = R(0.5) * (visc[c + k * nc] + visc[c + (k - 1) * nc]);
+      s = s + mid * (zeta[k] - zeta[k - 1]);
+    }
+    vi = s * f.H_ice[cc.c0 + c];
+  } else {
+    vi = visc_min;
+  }
+  if (vi < visc_min) vi = visc_min;
+  f.visc_int[cc.c0 + c] = vi;
+  if (p.diva) {
+    // calc_F_integral with n = 2 (velocity_diva.f90:824-869)
+    R F2 = R(0);
+    if (fi[c] == R(1)) {
+      const R H = f.H_ice[cc.c0 + c];
+      R prev = (H / visc[c]) * ((R(1) - zeta[0]) * (R(1) - zeta[0]));
+      for (int k = 1; k < nz; ++k) {
+        const R cur = (H / visc[c + k * nc]) * ((R(1) - zeta[k]) * (R(1) - zeta[k]));
+        F2 = F2 + R(0.5) * (cur + prev) * (zeta[k] - zeta[k - 1]);
+        prev = cur;
+      }
+    }
+    f.F2[cc.c0 + c] = F2;
+  }
+}
+
+// ---- basal friction on aa-nodes (basal_dragging.f90:384-579, 949-1292) ---
+template <typename R>
+YG_DEV R beta_point(const YgFields<R>& f, const YgFieldParams& p, const CellCtx& cc, int i, int j) {
+  const int nx = cc.nx, ny = cc.ny, BC = p.BC;
+  const int c = I2(i, j);
+  const R* fi = f.f_ice + cc.c0;
+  const R* fg = f.f_grnd + cc.c0;
+  const R* cb = f.c_bed + cc.c0;
+  const R* ux = (p.diva ? f.ux : f.ux) + cc.c0;   // basal velocity (ux_b)
+  const R* uy = (p.diva ? f.uy : f.uy) + cc.c0;
+  const R q = (R)p.beta_q, u0 = (R)p.beta_u0;
+  const int bm = p.beta_method;
+  R beta;
+  if (bm == 0) {
+    beta = (R)p.beta_const;
+  } else {
+    const bool coulomb = bm == 3 || bm == 5;
+    const bool simple = bm == 4 || bm == 5;
+    const R qq = bm == 1 ? R(1) : q;
+    const R ub_min = R(1e-3), ub_sq_min = ub_min * ub_min;
+    int im1, ip1, jm1, jp1;
+    nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+    if (fi[c] == R(1)) {
+      R cbn[4], uxn[4], uyn[4];
+      if (simple) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          cbn[k] = cb[c];
+          uxn[k] = RS(0.5) * (ux[c] + ux[I2(im1, j)]);
+          uyn[k] = RS(0.5) * (uy[c] + uy[I2(i, jm1)]);
+        }
+      } else {
+        auto aa = [&](int qq_) { return fi[qq_] == R(1) && fg[qq_] > R(0); };
+        auto ax = [&](int qq_) { const int qi = qq_ % nx, qj = qq_ / nx; int a1, a2, a3, a4; nbr(qi, qj, nx, ny, BC, a1, a2, a3, a4); return fi[qq_] == R(1) || fi[I2(a2, qj)] == R(1); };
+        auto ay = [&](int qq_) { const int qi = qq_ % nx, qj = qq_ / nx; int a1, a2, a3, a4; nbr(qi, qj, nx, ny, BC, a1, a2, a3, a4); return fi[qq_] == R(1) || fi[I2(qi, a4)] == R(1); };
+        gq_nodes<R>(0, cbn, cb, aa, nx, i, j, im1, ip1, jm1, jp1);
+        gq_nodes<R>(1, uxn, ux, ax, nx, i, j, im1, ip1, jm1, jp1);
+        gq_nodes<R>(2, uyn, uy, ay, nx, i, j, im1, ip1, jm1, jp1);
+      }
+      R sum = R(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const R uxyn = ysqrt<R>(uxn[k] * uxn[k] + uyn[k] * uyn[k] + ub_sq_min);
+        const R ratio = coulomb ? uxyn / (uxyn + u0) : uxyn / u0;
+        sum += (cbn[k] * ypow<R>(ratio, qq) * (R(1) / uxyn)) * R(1);
+      }
+      beta = sum / R(4);
+    } else {
+      const R uxy_b = ub_min;
+      const R ratio = coulomb ? uxy_b / (uxy_b + u0) : uxy_b / u0;
+      beta = cb[c] * ypow<R>(ratio, qq) * (R(1) / uxy_b);
+    }
+  }
+  const R q_frz = (bm == 0 || bm == 1) ? R(1) : q;
+  beta = beta * ypow<R>(f.f_slide[cc.c0 + c], -q_frz);
+  if (p.beta_gl_scale == 0) {
+    int im1, ip1, jm1, jp1;
+    nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+    if (fg[c] > R(0) && (fg[I2(im1, j)] == R(0) || fg[I2(ip1, j)] == R(0) ||
+                         fg[I2(i, jm1)] == R(0) || fg[I2(i, jp1)] == R(0)))
+      beta = beta * (R)p.beta_gl_f;
+  } else if (p.beta_gl_scale == 1) {
+    const R lim = (R)p.H_grnd_lim;
+    const R hg = f.H_grnd[cc.c0 + c];
+    R fs = (hg < lim ? hg : lim) / lim;
+    if (fs < R(0)) fs = R(0);
+    beta = beta * fs;
+  } else if (p.beta_gl_scale == 3) {
+    beta = fg[c] * beta;
+  }
+  if (fg[c] == R(0)) beta = R(0);
+  return beta;
+}
+
+template <typename R>
+__global__ void k_beta(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  CellCtx cc;
+  if (!cell_of(G, gc, skip, cc)) return;
+  if (p.beta_method == -1) return;
+  const int nx = cc.nx, ny = cc.ny;
+  int i = cc.i, j = cc.j;
+  // border copies of calc_beta (basal_dragging.f90:544-563)
+  if (p.bnd == YG_B_MISMIP3D) {
+    if (i == 0) i = 1;
+  } else if (p.bnd == YG_B_INFINITE || p.bnd == YG_B_MASK) {
+    if (i == 0) i = 1;
+    if (i == nx - 1) i = nx - 2;
+    if (j == 0) j = 1;
+    if (j == ny - 1) j = ny - 2;
+  }
+  R beta = beta_point<R>(f, p, cc, i, j);
+  const R bmin = (R)p.beta_min;
+  if (beta > R(0) && beta < bmin) beta = bmin;
+  f.beta[cc.c0 + I2(cc.i, cc.j)] = beta;
+}
+
+// ---- staggering, beta_min, matrix friction and limit drag -----------------
+// stagger_beta (basal_dragging.f90:581-704), set_beta_min_grounded (:706-780),
+// calc_vel_lim_drag (solver_ssa_ac.f90:1095-1199), DIVA calc_beta_eff
+// (velocity_diva.f90:912-958).
+template <typename R>
+YG_DEV void stag_point(const YgFields<R>& f, const YgFieldParams& p, const CellCtx& cc, int i,
+                       int j, R& bx, R& by) {
+  const int nx = cc.nx, ny = cc.ny;
+  const R* fi = f.f_ice + cc.c0;
+  const R* fg = f.f_grnd + cc.c0;
+  const R* be = f.beta + cc.c0;
+  int im1, ip1, jm1, jp1;
+  nbr(i, j, nx, ny, p.BC, im1, ip1, jm1, jp1);
+  const int c = I2(i, j), cx = I2(ip1, j), cy = I2(i, jp1);
+  if (fg[c] == R(0) && fg[cx] == R(0)) bx = R(0);
+  else if (fi[c] == R(1) && fi[cx] < R(1)) bx = be[c];
+  else if (fi[c] < R(1) && fi[cx] == R(1)) bx = be[cx];
+  else bx = R(0.5) * (be[c] + be[cx]);
+  if (fg[c] == R(0) && fg[cy] == R(0)) by = R(0);
+  else if (fi[c] == R(1) && fi[cy] < R(1)) by = be[c];
+  else if (fi[c] < R(1) && fi[cy] == R(1)) by = be[cy];
+  else by = R(0.5) * (be[c] + be[cy]);
+  const int st = p.beta_gl_stag;
+  if (st == 1 || st == 2) {
+    if (fi[c] == R(1) && fi[cx] == R(1)) {
+      if (fg[c] > R(0) && fg[cx] == R(0)) bx = st == 1 ? be[c] : be[cx];
+      else if (fg[c] == R(0) && fg[cx] > R(0)) bx = st == 1 ? be[cx] : be[c];
+    }
+    if (fi[c] == R(1) && fi[cy] == R(1)) {
+      if (fg[c] > R(0) && fg[cy] == R(0)) by = st == 1 ? be[c] : be[cy];
+      else if (fg[c] == R(0) && fg[cy] > R(0)) by = st == 1 ? be[cy] : be[c];
+    }
+  }
+}
+
+template <typename R>
+YG_DEV void lim_drag_face(R& k, R& r, R u, R u_cross, int mask, R s0, R inv_du, R tau_c) {
+  k = R(0);
+  r = R(0);
+  if (mask < 1) return;
+  const R s = ysqrt<R>(u * u + u_cross * u_cross);
+  if (s <= s0) return;
+  const R x = (s - s0) * inv_du;
+  const R tau = tau_c * x * x;
+  const R dtauds = R(2) * tau_c * x * inv_du;
+  const R b = tau / s;
+  const R w = (u / s) * (u / s);
+  k = b * (R(1) - w) + dtauds * w;
+  r = (k - b) * u;
+}
+
+template <typename R>
+__global__ void k_stagger(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  CellCtx cc;
+  if (!cell_of(G, gc, skip, cc)) return;
+  const int nx = cc.nx, ny = cc.ny, i = cc.i, j = cc.j;
+  const long long g = cc.c0 + I2(i, j);
+  R bx, by;
+  if (p.beta_gl_stag == -1) {
+    bx = f.beta_acx[g];
+    by = f.beta_acy[g];
+  } else {
+    // border copies (basal_dragging.f90:652-683): final(i,j) = staggered(map(i), map(j))
+    int ix = i, jx = j, iy = i, jy = j;
+    const int b = p.bnd;
+    if (b == YG_B_INFINITE || b == YG_B_MISMIP3D || b == YG_B_MASK) {
+      if (ix == 0) ix = 1; else if (ix == nx - 2 || ix == nx - 1) ix = nx - 3;
+      if (iy == 0) iy = 1; else if (iy == nx - 1) iy = nx - 2;
+    }
+    if (b == YG_B_INFINITE || b == YG_B_MASK) {
+      if (jx == 0) jx = 1; else if (jx == ny - 1) jx = ny - 2;
+      if (jy == 0) jy = 1; else if (jy == ny - 2 || jy == ny - 1) jy = ny - 3;
+    }
+    R t1, t2;
+    stag_point<R>(f, p, cc, ix, jx, bx, t1);
+    stag_point<R>(f, p, cc, iy, jy, t2, by);
+    if (p.beta_method != -1) {
+      const R bmin = (R)p.beta_min;
+      if (bx > R(0) && bx < bmin) bx = bmin;
+      if (by > R(0) && by < bmin) by = bmin;
+    }
+  }
+  f.beta_acx[g] = bx;
+  f.beta_acy[g] = by;
+  R mbx = bx, mby = by;
+  if (p.diva) {
+    // stagger_F_integral (velocity_diva.f90:871-910) and calc_beta_eff (:912-958)
+    int im1, ip1, jm1, jp1;
+    nbr(i, j, nx, ny, p.BC, im1, ip1, jm1, jp1);
+    const R* F2 = f.F2 + cc.c0;
+    const R* fi = f.f_ice + cc.c0;
+    auto smg = [](R v0, R v1, R f0, R f1) {
+      if (f0 == R(1) && f1 < R(1)) return v0;
+      if (f0 < R(1) && f1 == R(1)) return v1;
+      return R(0.5) * (v0 + v1);
+    };
+    const int c = I2(i, j);
+    const R F2x = smg(F2[c], F2[I2(ip1, j)], fi[c], fi[I2(ip1, j)]);
+    const R F2y = smg(F2[c], F2[I2(i, jp1)], fi[c], fi[I2(i, jp1)]);
+    f.F2_acx[g] = F2x;
+    f.F2_acy[g] = F2y;
+    auto beff = [&](R b, R F) {
+      if (p.no_slip) return F > R(0) ? R(1) / F : R(0);
+      return b / (R(1) + b * F);
+    };
+    mbx = beff(bx, F2x);
+    mby = beff(by, F2y);
+    f.beff_acx[g] = mbx;
+    f.beff_acy[g] = mby;
+    f.beta_eff[g] = beff(f.beta[g], F2[c]);
+  }
+  // set_beta_min_grounded on the copy used by the matrix
+  if (f.mx[g] == 1 && mbx == R(0)) mbx = (R)p.beta_min;
+  if (f.my[g] == 1 && mby == R(0)) mby = (R)p.beta_min;
+  f.bss_x[g] = mbx;
+  f.bss_y[g] = mby;
+  // limit drag
+  R kx = R(0), ky = R(0), rx = R(0), ry = R(0);
+  if (p.vel_lim_method == 1) {
+    const R* ux = (p.diva ? f.ubar_x : f.ux) + cc.c0;
+    const R* uy = (p.diva ? f.ubar_y : f.uy) + cc.c0;
+    int im1, ip1, jm1, jp1;
+    nbr(i, j, nx, ny, p.BC, im1, ip1, jm1, jp1);
+    const R u_max = (R)p.vel_max, tau_c = (R)p.vel_lim_tau;
+    const R s0 = R(0.8) * u_max;
+    const R inv_du = R(1) / (u_max - s0);
+    const int c = I2(i, j);
+    R uc = R(0.25) * (uy[c] + uy[I2(ip1, j)] + uy[I2(i, jm1)] + uy[I2(ip1, jm1)]);
+    lim_drag_face<R>(kx, rx, ux[c], uc, f.mx[g], s0, inv_du, tau_c);
+    uc = R(0.25) * (ux[c] + ux[I2(im1, j)] + ux[I2(i, jp1)] + ux[I2(im1, jp1)]);
+    lim_drag_face<R>(ky, ry, uy[c], uc, f.my[g], s0, inv_du, tau_c);
+  }
+  f.lkx[g] = kx; f.lky[g] = ky; f.lrx[g] = rx; f.lry[g] = ry;
+}
+
+// ---- fused residual-form assembly (solver_ssa_ac.f90:73-848, 1000-1067) --
+template <typename R>
+YG_DEV R visc_ab(const R* N, const R* fi, const R* zb, const R* zs, int nx, int ny, int BC, int i,
+                 int j) {
+  int im1, ip1, jm1, jp1;
+  nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+  const int cs[4] = {I2(i, j), I2(ip1, j), I2(i, jp1), I2(ip1, jp1)};
+  int k = 0;
+  bool any_ocn = false;
+  R v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool ice = fi[cs[q]] == R(1);
+    any_ocn |= (!ice) && (zb[cs[q]] < zs[cs[q]]);
+    k += ice ? 1 : 0;
+    v[q] = ice ? N[cs[q]] : R(0);
+  }
+  if (any_ocn || k == 0) return R(0);
+  return ((v[0] + v[1]) + (v[2] + v[3])) / (R)k;
+}
+
+template <typename R>
+__global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
+                           double* A64, float* A32, double* bvec, double* x0, uint32_t* maskv,
+                           const uint32_t* cellij) {
+  const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gq >= G.total_cells) return;
+  const int m = find_member(G.coffs, G.nmem, gq);
+  if (skip && skip[m]) return;
+  const YgMember& mb = G.members[m];
+  const long long c0 = G.coffs[m];
+  const int q = (int)(gq - c0);
+  const uint32_t ij = cellij[gq];
+  const int i = ij >> 16, j = ij & 0xffff;
+  const int nx = mb.nx, ny = mb.ny;
+  const int ncell = nx * ny;
+  // border types (solver_ssa_ac.f90:148-196): 0 no-slip, 1 free-slip, 2 periodic
+  int bc[4];  // 0 right, 1 upper, 2 left, 3 lower
+  switch (p.bnd) {
+    case YG_B_MISMIP3D:
+    case YG_B_TROUGH: bc[0] = 1; bc[1] = 2; bc[2] = 0; bc[3] = 2; break;
+    case YG_B_PERIODIC: bc[0] = bc[1] = bc[2] = bc[3] = 2; break;
+    case YG_B_PERIODIC_X: bc[0] = 2; bc[1] = 1; bc[2] = 2; bc[3] = 1; break;
+    case YG_B_PERIODIC_Y: bc[0] = 1; bc[1] = 2; bc[2] = 1; bc[3] = 2; break;
+    case YG_B_INFINITE:
+    case YG_B_MASK: bc[0] = bc[1] = bc[2] = bc[3] = 1; break;
+    default: bc[0] = bc[1] = bc[2] = bc[3] = 0; break;
+  }
+  const R dx = (R)mb.dx, dy = (R)mb.dy;
+  const R inv_dx = R(1) / dx, inv_dxdx = R(1) / (dx * dx);
+  const R inv_dy = R(1) / dy, inv_dydy = R(1) / (dy * dy);
+  const R inv_dxdy = R(1) / (dx * dy);
+  const R* N = f.visc_int + c0;
+  const R* fi = f.f_ice + c0;
+  const R* zb = f.z_bed + c0;
+  const R* zs = f.z_sl + c0;
+  const R* ux = (p.diva ? f.ubar_x : f.ux) + c0;
+  const R* uy = (p.diva ? f.ubar_y : f.uy) + c0;
+  const int c = I2(i, j);
+  const int im1 = i == 0 ? nx - 1 : i - 1, ip1 = i == nx - 1 ? 0 : i + 1;
+  const int jm1 = j == 0 ? ny - 1 : j - 1, jp1 = j == ny - 1 ? 0 : j + 1;
+  const int BC = p.BC;
+  double a[2][9];
+  uint32_t pm[2] = {0u, 0u};
+#pragma unroll
+  for (int s = 0; s < 9; ++s) { a[0][s] = 0.0; a[1][s] = 0.0; }
+  double bb[2], xx[2];
+  const int32_t mxc = f.mx[c0 + c], myc = f.my[c0 + c];
+#define PUT(row, s, val) do { a[row][s] = (double)(val); pm[row] |= 1u << (s); } while (0)
+  // ---- ux row ----
+  if (mxc == 0) {
+    PUT(0, 0, 1.0); bb[0] = 0.0; xx[0] = 0.0;
+  } else if (mxc == -1) {
+    PUT(0, 0, 1.0); bb[0] = ux[c]; xx[0] = ux[c];
+  } else if (i == 0 && bc[2] != 2) {
+    if (bc[2] == 1) { PUT(0, 0, 1.0); PUT(0, 1, -1.0); bb[0] = 0.0; xx[0] = ux[c]; }
+    else { PUT(0, 0, 1.0); bb[0] = 0.0; xx[0] = 0.0; }
+  } else if (i == nx - 1 && bc[0] != 2) {
+    if (bc[0] == 1) { PUT(0, 0, 1.0); PUT(0, 2, -1.0); bb[0] = 0.0; xx[0] = ux[c]; }
+    else { PUT(0, 0, 1.0); bb[0] = 0.0; xx[0] = 0.0; }
+  } else if (j == 0 && bc[3] != 2) {
+    if (bc[3] == 1) { PUT(0, 0, 1.0); PUT(0, 3, -1.0); bb[0] = 0.0; xx[0] = ux[c]; }
+    else { PUT(0, 0, 1.0); bb[0] = 0.0; xx[0] = 0.0; }
+  } else if (j == ny - 1 && bc[1] != 2) {
+    if (bc[1] == 1) { PUT(0, 0, 1.0); PUT(0, 4, -1.0); bb[0] = 0.0; xx[0] = ux[c]; }
+    else { PUT(0, 0, 1.0); bb[0] = 0.0; xx[0] = 0.0; }
+  } else if (mxc == 3) {
+    if (fi[c] == R(1) && fi[I2(ip1, j)] < R(1)) {
+      const R Nn = N[c];
+      PUT(0, 2, -R(4) * inv_dx * Nn); PUT(0, 8, -R(2) * inv_dy * Nn);
+      PUT(0, 0, R(4) * inv_dx * Nn);  PUT(0, 5, R(2) * inv_dy * Nn);
+    } else {
+      const R Nn = N[I2(ip1, j)];
+      PUT(0, 0, -R(4) * inv_dx * Nn); PUT(0, 7, -R(2) * inv_dy * Nn);
+      PUT(0, 1, R(4) * inv_dx * Nn);  PUT(0, 6, R(2) * inv_dy * Nn);
+    }
+    bb[0] = f.taul_acx[c0 + c]; xx[0] = ux[c];
+  } else {
+    const R Nab_c = visc_ab<R>(N, fi, zb, zs, nx, ny, BC, i, j);
+    const R Nab_jm = visc_ab<R>(N, fi, zb, zs, nx, ny, BC, i, jm1);
+    const R w_face = mxc == 4 ? R(0.5) : R(1);
+    R beta_now = w_face * f.bss_x[c0 + c];
+    R lim_r = R(0);
+    beta_now = beta_now + w_face * f.lkx[c0 + c];
+    lim_r = w_face * f.lrx[c0 + c];
+    PUT(0, 0, -R(4) * inv_dxdx * (N[I2(ip1, j)] + N[c]) - R(1) * inv_dydy * (Nab_c + Nab_jm) - beta_now);
+    PUT(0, 1, R(4) * inv_dxdx * N[I2(ip1, j)]);
+    PUT(0, 2, R(4) * inv_dxdx * N[c]);
+    PUT(0, 3, R(1) * inv_dydy * Nab_c);
+    PUT(0, 4, R(1) * inv_dydy * Nab_jm);
+    PUT(0, 5, -R(2) * inv_dxdy * N[c] - R(1) * inv_dxdy * Nab_c);
+    PUT(0, 6, R(2) * inv_dxdy * N[I2(ip1, j)] + R(1) * inv_dxdy * Nab_c);
+    PUT(0, 7, -R(2) * inv_dxdy * N[I2(ip1, j)] - R(1) * inv_dxdy * Nab_jm);
+    PUT(0, 8, R(2) * inv_dxdy * N[c] + R(1) * inv_dxdy * Nab_jm);
+    bb[0] = f.taud_acx[c0 + c] - lim_r; xx[0] = ux[c];
+  }
+  // ---- uy row ----
+  if (myc == 0) {
+    PUT(1, 0, 1.0); bb[1] = 0.0; xx[1] = 0.0;
+  } else if (myc == -1) {
+    PUT(1, 0, 1.0); bb[1] = uy[c]; xx[1] = uy[c];
+  } else if (j == 0 && bc[3] != 2) {
+    if (bc[3] == 1) { PUT(1, 0, 1.0); PUT(1, 1, -1.0); bb[1] = 0.0; xx[1] = uy[c]; }
+    else { PUT(1, 0, 1.0); bb[1] = 0.0; xx[1] = 0.0; }
+  } else if (j == ny - 1 && bc[1] != 2) {
+    if (bc[1] == 1) { PUT(1, 0, 1.0); PUT(1, 2, -1.0); bb[1] = 0.0; xx[1] = uy[c]; }
+    else { PUT(1, 0, 1.0); bb[1] = 0.0; xx[1] = 0.0; }
+  } else if (i == 0 && bc[2] != 2) {
+    if (bc[2] == 1) { PUT(1, 0, 1.0); PUT(1, 3, -1.0); bb[1] = 0.0; xx[1] = uy[c]; }
+    else { PUT(1, 0, 1.0); bb[1] = 0.0; xx[1] = 0.0; }
+  } else if (i == nx - 1 && bc[0] != 2) {
+    if (bc[0] == 1) { PUT(1, 0, 1.0); PUT(1, 4, -1.0); bb[1] = 0.0; xx[1] = uy[c]; }
+    else { PUT(1, 0, 1.0); bb[1] = 0.0; xx[1] = 0.0; }
+  } else if (myc == 3) {
+    if (fi[c] == R(1) && fi[I2(i, jp1)] < R(1)) {
+      const R Nn = N[c];
+      PUT(1, 8, -R(2) * inv_dx * Nn); PUT(1, 2, -R(4) * inv_dy * Nn);
+      PUT(1, 5, R(2) * inv_dx * Nn);  PUT(1, 0, R(4) * inv_dy * Nn);
+    } else {
+      const R Nn = N[I2(i, jp1)];
+      PUT(1, 7, -R(2) * inv_dx * Nn); PUT(1, 0, -R(4) * inv_dy * Nn);
+      PUT(1, 6, R(2) * inv_dx * Nn);  PUT(1, 1, R(4) * inv_dy * Nn);
+    }
+    bb[1] = f.taul_acy[c0 + c]; xx[1] = uy[c];
+  } else {
+    const R Nab_c = visc_ab<R>(N, fi, zb, zs, nx, ny, BC, i, j);
+    const R Nab_im = visc_ab<R>(N, fi, zb, zs, nx, ny, BC, im1, j);
+    const R w_face = myc == 4 ? R(0.5) : R(1);
+    R beta_now = w_face * f.bss_y[c0 + c];
+    beta_now = beta_now + w_face * f.lky[c0 + c];
+    const R lim_r = w_face * f.lry[c0 + c];
+    PUT(1, 0, -R(4) * inv_dydy * (N[I2(i, jp1)] + N[c]) - R(1) * inv_dxdx * (Nab_c + Nab_im) - beta_now);
+    PUT(1, 1, R(4) * inv_dydy * N[I2(i, jp1)]);
+    PUT(1, 2, R(4) * inv_dydy * N[c]);
+    PUT(1, 3, R(1) * inv_dxdx * Nab_c);
+    PUT(1, 4, R(1) * inv_dxdx * Nab_im);
+    PUT(1, 5, -R(2) * inv_dxdy * N[c] - R(1) * inv_dxdy * Nab_c);
+    PUT(1, 6, R(2) * inv_dxdy * N[I2(i, jp1)] + R(1) * inv_dxdy * Nab_c);
+    PUT(1, 7, -R(2) * inv_dxdy * N[I2(i, jp1)] - R(1) * inv_dxdy * Nab_im);
+    PUT(1, 8, R(2) * inv_dxdy * N[c] + R(1) * inv_dxdy * Nab_im);
+    bb[1] = f.taud_acy[c0 + c] - lim_r; xx[1] = uy[c];
+  }
+#undef PUT
+  const size_t base = (size_t)18 * c0;
+#pragma unroll
+  for (int s = 0; s < 9; ++s) {
+    const size_t k = base + ((size_t)s * ncell + q) * 2;
+    if (A64) { A64[k] = a[0][s]; A64[k + 1] = a[1][s]; }
+    if (A32) { A32[k] = (float)a[0][s]; A32[k + 1] = (float)a[1][s]; }
+  }
+  bvec[2 * gq] = bb[0]; bvec[2 * gq + 1] = bb[1];
+  x0[2 * gq] = xx[0]; x0[2 * gq + 1] = xx[1];
+  maskv[gq] = pm[0] | (pm[1] << 16);
+}
+
+// ---- Picard epilogue (one block per member) --------------------------------
+// linear_solver_save_velocity (solver_ssa_ac.f90:33-71), ssa_vel_clip
+// (:1069-1093), picard_relax_vel, picard_calc_convergence_l2 and _l1rel
+// (velocity_general.f90:1907-2186).
+template <typename R>
+__global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f, YgFieldParams p,
+                                                   const double* x, const int* lin_iters,
+                                                   const int* lin_status, int iter, int iter_max,
+                                                   YgPicardState ps) {
+  const int m = blockIdx.x;
+  if (ps.done[m]) return;
+  const YgMember mb = G.members[m];
+  const int nx = mb.nx, ny = mb.ny, ncell = nx * ny;
+  const long long c0 = G.coffs[m];
+  const int* soff = G.soff + mb.toff;
+  R* ux = (p.diva ? f.ubar_x : f.ux) + c0;
+  R* uy = (p.diva ? f.ubar_y : f.uy) + c0;
+  const R* uxp = f.ux_nm1 + c0;
+  const R* uyp = f.uy_nm1 + c0;
+  const int32_t* mx = f.mx + c0;
+  const int32_t* my = f.my + c0;
+  const R rel = (R)p.iter_rel, ulim = (R)p.vel_max;
+  const R vel_tol = RS(1e-5), tol_uf = R(1e-15);
+  const R etol = RS(1e-2), etol2 = RS(1e-5);
+  __shared__ double red[16 * 4];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // res1, res2, nchk, nonfinite
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const int i = c % nx, j = c / nx;
+    const int t = j + 2 * i;
+    const int q = soff[t] + i - yg_glo(t, ny);
+    R vx = (R)x[2 * (c0 + q)], vy = (R)x[2 * (c0 + q) + 1];
+    if (yabs<R>(vx) < tol_uf) vx = R(0);
+    if (yabs<R>(vy) < tol_uf) vy = R(0);
+    if (vx > ulim) vx = ulim;
+    if (vx < -ulim) vx = -ulim;
+    if (vy > ulim) vy = ulim;
+    if (vy < -ulim) vy = -ulim;
+    vx = uxp[c] + rel * (vx - uxp[c]);
+    vy = uyp[c] + rel * (vy - uyp[c]);
+    ux[c] = vx;
+    uy[c] = vy;
+    if (yabs<R>(vx) > vel_tol && mx[c] > 0) {
+      double tt = (double)(vx - uxp[c]);
+      if (fabs(tt) < (double)tol_uf) tt = 0.0;
+      acc[0] += tt * tt;
+      tt = (double)uxp[c];
+      if (fabs(tt) < (double)tol_uf) tt = 0.0;
+      acc[1] += tt * tt;
+      acc[2] += 1.0;
+    }
+    if (yabs<R>(vy) > vel_tol && my[c] > 0) {
+      double tt = (double)(vy - uyp[c]);
+      if (fabs(tt) < (double)tol_uf) tt = 0.0;
+      acc[0] += tt * tt;
+      tt = (double)uyp[c];
+      if (fabs(tt) < (double)tol_uf) tt = 0.0;
+      acc[1] += tt * tt;
+      acc[2] += 1.0;
+    }
+    if ((mx[c] > 0 && !isfinite((double)vx)) || (my[c] > 0 && !isfinite((double)vy))) acc[3] += 1.0;
+    f.err_x[c0 + c] = yabs<R>(vx) > etol ? R(2) * yabs<R>(vx - uxp[c]) / yabs<R>(vx + uxp[c] + etol2) : R(0);
+    f.err_y[c0 + c] = yabs<R>(vy) > etol ? R(2) * yabs<R>(vy - uyp[c]) / yabs<R>(vy + uyp[c] + etol2) : R(0);
+  }
+  yg_block_sum<4>(acc, red);
+  if (threadIdx.x == 0) {
+    const R resid = acc[2] > 0.0 ? (R)(sqrt(acc[0]) / (sqrt(acc[1]) + 1e-10)) : R(0);
+    int d = 0;
+    if (resid < (R)p.iter_conv) d = 1;
+    else if (iter == iter_max) d = 2;
+    if (acc[3] > 0.0 || !isfinite((double)resid)) d = -1;
+    ps.resid[m] = (double)resid;
+    ps.iter_now[m] = iter;
+    ps.lin_iter[m] += lin_iters[m];
+    if (lin_status[m] != 0) ps.lin_fail[m] += 1;
+    ps.done[m] = d;
+  }
+}
+
+// ---- after the loop: count_vel_lim_faces (solver_ssa_ac.f90:1201-1262) and
+// calc_basal_stress (velocity_ssa.f90:588-617) ------------------------------
+template <typename R>
+__global__ __launch_bounds__(1024) void k_finish(YgBatchGeom G, YgFields<R> f, YgFieldParams p,
+                                                 YgPicardState ps) {
+  const int m = blockIdx.x;
+  const YgMember mb = G.members[m];
+  const int nx = mb.nx, ny = mb.ny, ncell = nx * ny;
+  const long long c0 = G.coffs[m];
+  const R* ux = (p.diva ? f.ubar_x : f.ux) + c0;
+  const R* uy = (p.diva ? f.ubar_y : f.uy) + c0;
+  const int32_t* mx = f.mx + c0;
+  const int32_t* my = f.my + c0;
+  const R u_max = (R)p.vel_max, s0 = R(0.8) * u_max;
+  const bool drag = p.vel_lim_method == 1;
+  __shared__ double red[16];
+  double cnt[1] = {0.0};
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const int i = c % nx, j = c / nx;
+    int im1, ip1, jm1, jp1;
+    nbr(i, j, nx, ny, p.BC, im1, ip1, jm1, jp1);
+    if (mx[c] >= 1) {
+      if (drag) {
+        const R uc = R(0.25) * (uy[c] + uy[I2(ip1, j)] + uy[I2(i, jm1)] + uy[I2(ip1, jm1)]);
+        if (ux[c] * ux[c] + uc * uc > s0 * s0) cnt[0] += 1.0;
+      } else if (yabs<R>(ux[c]) >= u_max) cnt[0] += 1.0;
+    }
+    if (my[c] >= 1) {
+      if (drag) {
+        const R uc = R(0.25) * (ux[c] + ux[I2(im1, j)] + ux[I2(i, jp1)] + ux[I2(im1, jp1)]);
+        if (uy[c] * uy[c] + uc * uc > s0 * s0) cnt[0] += 1.0;
+      } else if (yabs<R>(uy[c]) >= u_max) cnt[0] += 1.0;
+    }
+    if (!p.diva) {
+      R tx = f.bss_x[c0 + c] * ux[c], ty = f.bss_y[c0 + c] * uy[c];
+      if (yabs<R>(tx) < R(1e-5)) tx = R(0);
+      if (yabs<R>(ty) < R(1e-5)) ty = R(0);
+      f.taub_x[c0 + c] = tx;
+      f.taub_y[c0 + c] = ty;
+    }
+  }
+  yg_block_sum<1>(cnt, red);
+  if (threadIdx.x == 0) ps.lim_n[m] = (int)cnt[0];
+}
+
+// ---- before the loop: set_inactive_margins (velocity_general.f90:1687-1750)
+template <typename R>
+__global__ void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p) {
+  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  CellCtx cc;
+  if (!cell_of(G, gc, nullptr, cc)) return;
+  const int nx = cc.nx, ny = cc.ny, i = cc.i, j = cc.j;
+  int im1, ip1, jm1, jp1;
+  nbr(i, j, nx, ny, p.BC, im1, ip1, jm1, jp1);
+  const R* fi = f.f_ice + cc.c0;
+  const long long g = cc.c0 + I2(i, j);
+  const R fa = fi[I2(i, j)];
+  R fb = fi[I2(ip1, j)];
+  R* ux = (p.diva ? f.ubar_x : f.ux);
+  R* uy = (p.diva ? f.ubar_y : f.uy);
+  if (!p.diva) {
+    if ((fb == R(0) && fa < R(1)) || (fa == R(0) && fb < R(1))) ux[g] = R(0);
+    fb = fi[I2(i, jp1)];
+    if ((fb == R(0) && fa < R(1)) || (fa == R(0) && fb < R(1))) uy[g] = R(0);
+  }
+  f.err_x[g] = R(1);
+  f.err_y[g] = R(1);
+}
+
+__global__ void k_cellij(YgBatchGeom G, uint32_t* cellij) {
+  // inverse of the skewed numbering: walk the cells of step t, lines glo..ghi
+  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= G.total_cells) return;
+  const int m = find_member(G.coffs, G.nmem, gc);
+  const YgMember& mb = G.members[m];
+  const int nx = mb.nx, ny = mb.ny;
+  const int loc = (int)(gc - G.coffs[m]);
+  const int i = loc % nx, j = loc / nx;
+  const int t = j + 2 * i;
+  const int* soff = G.soff + mb.toff;
+  const int q = soff[t] + i - yg_glo(t, ny);
+  cellij[G.coffs[m] + q] = ((uint32_t)i << 16) | (uint32_t)j;
+}
+
+}  // namespace
+
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename R>
+void yg_launch_init(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_init<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p);
+}
+
+template <typename R>
+void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                          const int* skip, int first_iter, hipStream_t st) {
+  const dim3 grid(nblk(G.total_cells, 256)), blk(256);
+  hipLaunchKernelGGL(k_strain<R>, grid, blk, 0, st, G, f, p, skip, first_iter);
+  hipLaunchKernelGGL(k_visc<R>, grid, blk, 0, st, G, f, p, skip);
+  hipLaunchKernelGGL(k_beta<R>, grid, blk, 0, st, G, f, p, skip);
+  hipLaunchKernelGGL(k_stagger<R>, grid, blk, 0, st, G, f, p, skip);
+}
+
+template <typename R>
+void yg_launch_assemble(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                        const int* skip, double* A64, float* A32, double* b, double* x0,
+                        uint32_t* mask, uint32_t* cellij, hipStream_t st) {
+  hipLaunchKernelGGL(k_assemble<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p,
+                     skip, A64, A32, b, x0, mask, (const uint32_t*)cellij);
+}
+
+template <typename R>
+void yg_launch_epilogue(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                        const double* x, const int* lin_iters, const int* lin_status, int iter,
+                        int iter_max, YgPicardState ps, hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue<R>, dim3(G.nmem), dim3(1024), 0, st, G, f, p, x, lin_iters,
+                     lin_status, iter, iter_max, ps);
+}
+
+template <typename R>
+void yg_launch_finish(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                      YgPicardState ps, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish<R>, dim3(G.nmem), dim3(1024), 0, st, G, f, p, ps);
+}
+
+void yg_launch_cellij(const YgBatchGeom& G, uint32_t* cellij, hipStream_t st) {
+  hipLaunchKernelGGL(k_cellij, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, cellij);
+}
+
+#define YG_INST(R)                                                                              \
+  template void yg_launch_init<R>(const YgBatchGeom&, const YgFields<R>&, const YgFieldParams&, \
+                                  hipStream_t);                                                 \
+  template void yg_launch_fields_pre<R>(const YgBatchGeom&, const YgFields<R>&,                 \
+                                        const YgFieldParams&, const int*, int, hipStream_t);     \
+  template void yg_launch_assemble<R>(const YgBatchGeom&, const YgFields<R>&,                   \
+                                      const YgFieldParams&, const int*, double*, float*,         \
+                                      double*, double*, uint32_t*, uint32_t*, hipStream_t);      \
+  template void yg_launch_epilogue<R>(const YgBatchGeom&, const YgFields<R>&,                   \
+                                      const YgFieldParams&, const double*, const int*,           \
+                                      const int*, int, int, YgPicardState, hipStream_t);         \
+  template void yg_launch_finish<R>(const YgBatchGeom&, const YgFields<R>&,                     \
+                                    const YgFieldParams&, YgPicardState, hipStream_t);
+YG_INST(float)
+YG_INST(double)

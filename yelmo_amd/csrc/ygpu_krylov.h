@@ -1,0 +1,50 @@
+// ygpu_krylov.h -- host/device interface of the persistent Krylov kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ygpu_common.h"
+
+#define YG_MODE_FP64 0
+#define YG_MODE_MIXED 1
+#define YG_MAX_STEPS 3072   // ny + 2*(nx-1) limit of the LDS step table
+#define YG_MAX_LINES 1024   // one lane per grid line, one block per member
+
+// Krylov workspace vectors per member (2 values per cell each)
+#define YG_NW64 9
+#define YG_NW32 11
+
+struct YgKrylovOpts {
+  int solver;       // 1 cg, 4 bicgstab (LIS numbering)
+  int precon;       // 0 none, 1 jacobi, 2 ilu
+  int maxiter;
+  int initx_zeros;
+  int conv_cond;    // 0 ||r||/||r0||, 1 ||r||/||b||
+  int ir_max;       // mixed mode: refinement sweeps
+  double tol;
+  double inner_tol; // mixed mode: relative tolerance of each FP32 inner solve
+};
+
+struct YgKrylovArgs {
+  const YgMember* members;
+  const int* soff;          // packed step offsets, T+1 per member at members[m].toff
+  const uint32_t* mask;     // per cell presence bits (skewed order)
+  const uint32_t* cellij;   // per cell (i << 16) | j
+  const double* A64;        // 18 per cell: [slot][cell][comp]
+  const float* A32;
+  double* F64;              // ILU factors / jacobi 1/diag
+  float* F32;
+  const double* b;          // 2 per cell
+  double* x;                // 2 per cell, x0 in, solution out
+  double* W64;              // YG_NW64 * 2 per cell
+  float* W32;               // YG_NW32 * 2 per cell
+  long long total_cells;
+  int wrapx, wrapy;
+  const int* skip;          // members to skip (may be null)
+  int* out_iters;
+  int* out_status;
+  double* out_resid;
+  YgKrylovOpts opts;
+};
+
+void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, hipStream_t st);

@@ -1,0 +1,158 @@
+"""Host-side mirror of Yelmo's velocity-solver interface.
+
+``calc_velocity_ssa`` / ``calc_velocity_diva`` take the same fields as
+``calc_velocity_ssa`` (src/physics/velocity_ssa.f90:60-63) and
+``calc_velocity_diva`` (src/physics/velocity_diva.f90:59-64), held in a
+``generator.SsaProblem``, and run the whole Picard loop on the GPU through
+libygpu.so.  ``linear_solver_matrix_solve`` mirrors the reference's linear
+solver boundary (src/physics/solver_linear.F90:151) on an ``Lgs`` holder of
+the 1-based CSR arrays (``linear_solver_class``, solver_linear.F90:11-34).
+Every call goes to the HIP library; there is no CPU path here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .generator import SsaProblem
+from .params import SsaParams
+
+LGS_SUCCESS, LGS_BREAKDOWN, LGS_MAXITER = 0, 2, 4
+
+
+@dataclass
+class SolveInfo:
+    ssa_iter_now: int
+    ssa_lin_iter: int
+    ssa_lin_fail: int
+    ssa_lim_n: int
+    status: int
+    picard_resid: float
+    seconds: float
+
+
+def _real_dtype(prob: SsaProblem):
+    dt = prob.fields["H_ice"].dtype
+    if dt not in (np.float32, np.float64):
+        raise TypeError("fields must be float32 or float64")
+    return dt
+
+
+def _fill_struct(prob: SsaProblem, diva: bool) -> tuple[_lib.SsaProblemC, list]:
+    dt = _real_dtype(prob)
+    keep = []
+
+    def arr(a, dtype=dt):
+        a = np.asfortranarray(a, dtype=dtype)
+        keep.append(a)
+        return a
+
+    s = _lib.SsaProblemC()
+    s.nx, s.ny, s.nz = prob.nx, prob.ny, prob.nz
+    s.real_bytes = np.dtype(dt).itemsize
+    s.dx, s.dy, s.n_glen = prob.dx, prob.dy, prob.n_glen
+    st, fl = prob.state, prob.fields
+    for n in _lib.PROBLEM_PTRS_INOUT + _lib.PROBLEM_PTRS_ERR:
+        st[n] = arr(st[n])
+        setattr(s, n, st[n].ctypes.data)
+    for n in ("ssa_mask_acx", "ssa_mask_acy"):
+        st[n] = arr(st[n], np.int32)
+        setattr(s, n, st[n].ctypes.data)
+    for n in _lib.PROBLEM_PTRS_IN:
+        a = arr(fl[n])
+        setattr(s, n, a.ctypes.data)
+    if diva:
+        nx, ny, nz = prob.nx, prob.ny, prob.nz
+        for n, shape in (("ux", (nx, ny, nz)), ("uy", (nx, ny, nz)), ("ux_bar", (nx, ny)),
+                         ("uy_bar", (nx, ny)), ("ux_i", (nx, ny, nz)), ("uy_i", (nx, ny, nz)),
+                         ("beta_eff", (nx, ny)), ("duxdz", (nx, ny, nz)), ("duydz", (nx, ny, nz))):
+            if n not in st:
+                st[n] = np.zeros(shape, dtype=dt, order="F")
+            st[n] = arr(st[n])
+            setattr(s, n, st[n].ctypes.data)
+    return s, keep
+
+
+def _info(s: _lib.SsaProblemC) -> SolveInfo:
+    return SolveInfo(s.ssa_iter_now, s.ssa_lin_iter, s.ssa_lin_fail, s.ssa_lim_n, s.status,
+                     s.picard_resid, s.solve_seconds)
+
+
+def _run_batch(problems, par: SsaParams, diva: bool, handle=None):
+    h = handle or _lib.default_handle()
+    arr = (_lib.SsaProblemC * len(problems))()
+    keep = []
+    for k, p in enumerate(problems):
+        s, kp = _fill_struct(p, diva)
+        arr[k] = s
+        keep.append(kp)
+    pc = par.to_c()
+    fn = h.lib.ygpu_velocity_diva_batch if diva else h.lib.ygpu_velocity_ssa_batch
+    h.check(fn(h.h, C.byref(pc), len(problems), arr))
+    return [_info(arr[k]) for k in range(len(problems))]
+
+
+def calc_velocity_ssa(prob: SsaProblem, par: SsaParams, handle=None) -> SolveInfo:
+    """One SSA solve (calc_velocity_ssa); updates prob.state in place."""
+    return _run_batch([prob], par, False, handle)[0]
+
+
+def calc_velocity_ssa_batch(problems, par: SsaParams, handle=None) -> list[SolveInfo]:
+    """Independent SSA solves of many domains (uneven sizes allowed) in one
+    device call; each member runs its own Picard loop."""
+    return _run_batch(problems, par, False, handle)
+
+
+def calc_velocity_diva(prob: SsaProblem, par: SsaParams, handle=None) -> SolveInfo:
+    """One DIVA solve (calc_velocity_diva); updates prob.state in place."""
+    return _run_batch([prob], par, True, handle)[0]
+
+
+def calc_velocity_diva_batch(problems, par: SsaParams, handle=None) -> list[SolveInfo]:
+    return _run_batch(problems, par, True, handle)
+
+
+# ---- the inner boundary ------------------------------------------------------
+
+@dataclass
+class Lgs:
+    """linear_solver_class (solver_linear.F90:11-34): 1-based CSR."""
+    a_ptr: np.ndarray
+    a_index: np.ndarray
+    a_value: np.ndarray
+    b_value: np.ndarray
+    x_value: np.ndarray
+    lin_iter: int = 0
+    status: int = 0
+    L2_rel_norm: float = 0.0
+    solver_time: float = 0.0
+
+    @property
+    def nmax(self) -> int:
+        return len(self.b_value)
+
+
+def linear_solver_matrix_solve(lgs: Lgs, lis_settings: str, handle=None) -> Lgs:
+    """linear_solver_matrix_solve (solver_linear.F90:151): solves in place on
+    the GPU, LIS option grammar, status 0/2/4."""
+    h = handle or _lib.default_handle()
+    n = lgs.nmax
+    nnz = int(lgs.a_ptr[n]) - 1
+    ap = np.ascontiguousarray(lgs.a_ptr[: n + 1], dtype=np.int32)
+    ai = np.ascontiguousarray(lgs.a_index[:nnz], dtype=np.int32)
+    av = np.ascontiguousarray(lgs.a_value[:nnz], dtype=np.float64)
+    b = np.ascontiguousarray(lgs.b_value, dtype=np.float64)
+    x = np.ascontiguousarray(lgs.x_value, dtype=np.float64).copy()
+    it, st = C.c_int32(), C.c_int32()
+    res, secs = C.c_double(), C.c_double()
+    rc = h.lib.ygpu_csr_solve(h.h, n, nnz, ap.ctypes.data, ai.ctypes.data, av.ctypes.data,
+                              b.ctypes.data, x.ctypes.data, lis_settings.encode(), C.byref(it),
+                              C.byref(st), C.byref(res), C.byref(secs))
+    h.check(rc)
+    lgs.x_value = x
+    lgs.lin_iter, lgs.status = it.value, st.value
+    lgs.L2_rel_norm, lgs.solver_time = res.value, secs.value
+    return lgs
