@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: SSA velocity solves per second on one MI355X.
+
+Workload (one "step"): the 64-member ensemble on 161 x 161 grids (dx = 8 km,
+51,842 unknowns each; every member has its own seeded friction field and a
+rate factor drawn log-uniformly in [1e-17, 1e-16]), each member solved from
+rest through the full Picard loop (BiCGSTAB + ILU(0) at rtol 1e-8, Picard
+tolerance 1e-4, at most 50 iterations) in one batched call of the
+mixed-precision path (FP32 stencil SpMV and ILU(0) inside FP64 iterative
+refinement).  Inputs are resident in HBM before the timed region; each step
+restores the initial state on the device and solves all members again.
+
+The line also reports the plain FP64 GPU path on the same ensemble (batched,
+and one member at a time), the per-member relative L2 velocity error of the
+mixed path against it, the Krylov kernel's HBM roofline, and the CPU oracle
+timed on this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--side", type=int, default=161)
+    ap.add_argument("--cpu-sample", type=int, default=2,
+                    help="ensemble members solved by the CPU oracle for the baseline")
+    ap.add_argument("--single-sample", type=int, default=4,
+                    help="members solved one at a time on the FP64 GPU path")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the ensemble members are independent and a "
+              f"multi-GPU run would only split them (--gpus {args.gpus} not supported)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from yelmo_amd import _lib, generator
+    from yelmo_amd.params import SsaParams
+    from yelmo_amd.velocity import ResidentBatch
+
+    t_gen = time.perf_counter()
+    base = generator.workload_ensemble(args.members, args.side, seed=0, dtype=np.float32)
+    print(f"# generated {args.members} x {args.side}^2 in {time.perf_counter() - t_gen:.1f}s",
+          file=sys.stderr, flush=True)
+    par_mixed = SsaParams(precision="mixed")
+    par_fp64 = SsaParams(precision="fp64")
+    h = _lib.default_handle()
+
+    # ---- plain FP64 GPU path, batched: the error reference --------------------
+    probs64 = [p.astype(np.float64) for p in base]
+    rb64 = ResidentBatch(probs64, par_fp64, handle=h)
+    rb64.run()                      # warm-up
+    _sync()
+    t0 = time.perf_counter()
+    rb64.run()
+    _sync()
+    t_fp64_batch = time.perf_counter() - t0
+    info64 = rb64.download()
+    ref = [np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]) for p in probs64]
+    rb64.close()
+    print(f"# fp64 batched: {t_fp64_batch:.3f}s", file=sys.stderr, flush=True)
+
+    # ---- plain FP64 GPU path, one member at a time ------------------------------
+    ns = min(args.single_sample, args.members)
+    singles = []
+    for k in range(ns):
+        rb = ResidentBatch([base[k].astype(np.float64)], par_fp64, handle=h)
+        rb.run()
+        _sync()
+        t0 = time.perf_counter()
+        rb.run()
+        _sync()
+        singles.append(time.perf_counter() - t0)
+        rb.close()
+    t_fp64_single = float(np.mean(singles))
+    print(f"# fp64 single: {t_fp64_single:.3f}s per solve", file=sys.stderr, flush=True)
+
+    # ---- mixed precision, batched: the measured value ---------------------------
+    probs = [p.copy() for p in base]
+    rb = ResidentBatch(probs, par_mixed, handle=h)
+    for _ in range(args.warmup):
+        rb.run()
+    h.stats_enable(True)
+    step_times = []
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        rb.run()
+        step_times.append(time.perf_counter() - ts)
+    _sync()
+    elapsed = time.perf_counter() - t0
+    st = h.stats()
+    h.stats_enable(False)
+    info = rb.download()
+    errs = []
+    for k, p in enumerate(probs):
+        v = np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]).astype(np.float64)
+        errs.append(rel_l2(v, ref[k]))
+    rb.close()
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = args.members * args.steps / elapsed
+    fp64_batch_sps = args.members / t_fp64_batch
+    fp64_single_sps = 1.0 / t_fp64_single
+
+    launches = max(st["krylov_launches"], 1)
+    avg_ms = st["krylov_ms"] / launches
+    bytes_per_launch = st["krylov_bytes"] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    # ---- CPU baseline: the oracle (FP32 fields + FP64 BiCGSTAB/ILU(0), the
+    # reference's arithmetic), one thread, on a bounded sample of members -------
+    cpu = None
+    if not args.no_cpu and args.cpu_sample > 0:
+        from oracle import oracle
+        par_ref = SsaParams(precision="ref")
+        t0 = time.perf_counter()
+        n_cpu = min(args.cpu_sample, args.members)
+        for k in range(n_cpu):
+            oracle.calc_velocity_ssa(base[k], par_ref, dp=False)
+        tc = time.perf_counter() - t0
+        cpu = {"value": n_cpu / tc, "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} members of the {args.members} x {args.side}^2 ensemble, "
+                         f"full Picard solve each, C oracle (FP32 fields, FP64 BiCGSTAB+ILU(0)), "
+                         f"1 thread, {tc:.1f}s"}
+
+    line = {
+        "metric": "SSA solves/s at <=1e-6 rel vel err vs plain FP64; SpMV HBM GB/s; p99 latency",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ensemble {args.members} x {args.side}x{args.side} SSA (dx 8 km, "
+                        f"{2 * args.side * args.side} unknowns each), mixed FP32/FP64, batched",
+            "members": args.members, "side": args.side,
+            "solver": par_mixed.ssa_lis_opt, "picard_tol": par_mixed.ssa_iter_conv,
+        },
+        "max_rel_l2_vs_fp64": max(errs),
+        "error_ok": max(errs) <= 1e-6,
+        "fp64_batched_solves_per_s": fp64_batch_sps,
+        "fp64_single_solves_per_s": fp64_single_sps,
+        "speedup_vs_fp64_single": value / fp64_single_sps,
+        "speedup_vs_fp64_batched": value / fp64_batch_sps,
+        "step_latency_ms": {"p50": 1e3 * float(np.percentile(step_times, 50)),
+                            "p99": 1e3 * float(np.percentile(step_times, 99))},
+        "picard_iters_mean": float(np.mean([i.ssa_iter_now for i in info])),
+        "krylov_iters_per_solve": st["krylov_iters"] / max(args.members * args.steps, 1),
+        "unknown_iters_per_s": st["unknown_iters"] / elapsed,
+        "roofline": {"bound": "hbm", "kernel": "yg_krylov_kernel<1>", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "bytes_per_launch": bytes_per_launch, "launches": launches},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,6 +153,20 @@ int ygpu_velocity_ssa_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t 
 int ygpu_velocity_diva_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
                              ygpu_ssa_problem* probs);
 
+/* ---- resident batches: upload once, solve many times.  ygpu_batch_run
+ * solves every domain from the device-resident state (reset != 0: from the
+ * state the batch was created with, restored device-to-device), with no
+ * host traffic besides one flag read per Picard iteration; seconds is the
+ * device time of the call.  ygpu_batch_download copies the results into
+ * probs (the same array layout as at creation).                             */
+typedef struct ygpu_batch ygpu_batch;
+int ygpu_batch_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
+                      const ygpu_ssa_problem* probs, int32_t diva, ygpu_batch** out,
+                      int64_t* device_bytes);
+int ygpu_batch_run(ygpu_batch* b, int32_t reset, double* seconds);
+int ygpu_batch_download(ygpu_batch* b, ygpu_ssa_problem* probs);
+int ygpu_batch_destroy(ygpu_batch* b);
+
 /* ---- resident sessions (warm starts for served solve sequences).
  * A session keeps one domain's fields, velocity, viscosity and workspace on
  * the device between solves.  ygpu_session_solve uploads the IN fields of

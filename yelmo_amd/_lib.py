@@ -61,7 +61,8 @@ class SsaProblemC(C.Structure):
 # every symbol include/ygpu.h declares
 EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_default_params",
            "ygpu_csr_solve", "ygpu_velocity_ssa", "ygpu_velocity_ssa_batch",
-           "ygpu_velocity_diva_batch", "ygpu_session_create", "ygpu_session_solve",
+           "ygpu_velocity_diva_batch", "ygpu_batch_create", "ygpu_batch_run",
+           "ygpu_batch_download", "ygpu_batch_destroy", "ygpu_session_create", "ygpu_session_solve",
            "ygpu_session_destroy", "ygpu_stats_enable", "ygpu_stats_get",
            "ygpu_footprint_bytes")
 
@@ -105,6 +106,15 @@ def load() -> C.CDLL:
     lib.ygpu_session_solve.restype = C.c_int
     lib.ygpu_session_destroy.argtypes = [PP]
     lib.ygpu_session_destroy.restype = C.c_int
+    lib.ygpu_batch_create.argtypes = [PP, C.POINTER(SsaParamsC), I, C.POINTER(SsaProblemC), I,
+                                      C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
+    lib.ygpu_batch_create.restype = C.c_int
+    lib.ygpu_batch_run.argtypes = [PP, I, C.POINTER(D)]
+    lib.ygpu_batch_run.restype = C.c_int
+    lib.ygpu_batch_download.argtypes = [PP, C.POINTER(SsaProblemC)]
+    lib.ygpu_batch_download.restype = C.c_int
+    lib.ygpu_batch_destroy.argtypes = [PP]
+    lib.ygpu_batch_destroy.restype = C.c_int
     lib.ygpu_stats_enable.argtypes = [PP, I]
     lib.ygpu_stats_enable.restype = C.c_int
     lib.ygpu_stats_get.argtypes = [PP, C.POINTER(StatsC)]

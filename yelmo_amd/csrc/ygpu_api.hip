@@ -456,6 +456,38 @@ struct Batch {
 
   int diva_down_interior(ygpu_ssa_problem* probs);
 
+  // device copy of the initial INOUT state, so a resident batch can be
+  // solved again from the same start without touching the host
+  std::vector<std::pair<R*, R*>> backup;  // (live, saved)
+  int save_initial() {
+    std::vector<std::pair<R*, long long>> st = {
+        {f.ux, total_cells}, {f.uy, total_cells}, {f.visc, total_z}, {f.beta, total_cells},
+        {f.beta_acx, total_cells}, {f.beta_acy, total_cells}, {f.taub_x, total_cells},
+        {f.taub_y, total_cells}};
+    if (diva) {
+      st.push_back({f.ubar_x, total_cells});
+      st.push_back({f.ubar_y, total_cells});
+      st.push_back({f.duxdz, total_z});
+      st.push_back({f.duydz, total_z});
+    }
+    for (auto& e : st) {
+      R* saved;
+      YG_CHECK(h, ar.alloc(&saved, e.second));
+      YG_CHECK(h, hipMemcpyAsync(saved, e.first, sizeof(R) * e.second, hipMemcpyDeviceToDevice, h->stream));
+      backup.push_back({e.first, saved});
+      sizes.push_back(e.second);
+    }
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    return YGPU_OK;
+  }
+  std::vector<long long> sizes;
+  int restore_initial() {
+    for (size_t q = 0; q < backup.size(); ++q)
+      YG_CHECK(h, hipMemcpyAsync(backup[q].first, backup[q].second, sizeof(R) * sizes[q],
+                                 hipMemcpyDeviceToDevice, h->stream));
+    return YGPU_OK;
+  }
+
   // the Picard loop (velocity_ssa.f90:172-334 / velocity_diva.f90:195-404)
   int run(double* seconds) {
     hipStream_t st = h->stream;
@@ -849,6 +881,30 @@ int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out) {
   return YGPU_OK;
 }
 
+// ---- resident batches --------------------------------------------------------
+struct YgBatchBase {
+  virtual ~YgBatchBase() {}
+  virtual int run(int reset, double* secs) = 0;
+  virtual int download(ygpu_ssa_problem* probs) = 0;
+  int64_t bytes = 0;
+};
+
+template <typename R>
+struct YgBatchT : YgBatchBase {
+  Batch<R> B;
+  int run(int reset, double* secs) override {
+    int rc;
+    if (reset && (rc = B.restore_initial())) return rc;
+    return B.run(secs);
+  }
+  int download(ygpu_ssa_problem* probs) override { return B.download(probs); }
+};
+
+struct ygpu_batch {
+  ygpu_handle* h;
+  YgBatchBase* b;
+};
+
 // ---- sessions and admission ------------------------------------------------
 struct YgSessionBase {
   virtual ~YgSessionBase() {}
@@ -911,6 +967,51 @@ int ygpu_session_solve(ygpu_session* s, ygpu_ssa_problem* prob) {
   s->h->err.clear();
   if (hipSetDevice(s->h->device) != hipSuccess) { s->h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   return s->s->solve(prob);
+}
+
+int ygpu_batch_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
+                      const ygpu_ssa_problem* probs, int32_t diva, ygpu_batch** out,
+                      int64_t* device_bytes) {
+  if (!h || !par || nprob < 1 || !probs || !out) return YGPU_ERR_ARG;
+  *out = nullptr;
+  h->err.clear();
+  if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  YgBatchBase* bb = nullptr;
+  int rc;
+  auto make = [&](auto* t) {
+    int r = t->B.setup(h, par, nprob, probs, diva ? 1 : 0);
+    if (!r) r = t->B.upload_inputs(probs);
+    if (!r) r = t->B.upload_state(probs);
+    if (!r) r = t->B.save_initial();
+    t->bytes = (int64_t)t->B.ar.bytes;
+    bb = t;
+    return r;
+  };
+  if (par->precision == YGPU_PREC_FP64) rc = make(new YgBatchT<double>());
+  else rc = make(new YgBatchT<float>());
+  if (rc) { delete bb; return rc; }
+  if (device_bytes) *device_bytes = bb->bytes;
+  *out = new ygpu_batch{h, bb};
+  return YGPU_OK;
+}
+
+int ygpu_batch_run(ygpu_batch* b, int32_t reset, double* seconds) {
+  if (!b) return YGPU_ERR_ARG;
+  b->h->err.clear();
+  if (hipSetDevice(b->h->device) != hipSuccess) { b->h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  return b->b->run(reset, seconds);
+}
+
+int ygpu_batch_download(ygpu_batch* b, ygpu_ssa_problem* probs) {
+  if (!b || !probs) return YGPU_ERR_ARG;
+  return b->b->download(probs);
+}
+
+int ygpu_batch_destroy(ygpu_batch* b) {
+  if (!b) return YGPU_ERR_ARG;
+  delete b->b;
+  delete b;
+  return YGPU_OK;
 }
 
 int ygpu_session_destroy(ygpu_session* s) {

@@ -156,3 +156,48 @@ def linear_solver_matrix_solve(lgs: Lgs, lis_settings: str, handle=None) -> Lgs:
     lgs.lin_iter, lgs.status = it.value, st.value
     lgs.L2_rel_norm, lgs.solver_time = res.value, secs.value
     return lgs
+
+
+class ResidentBatch:
+    """A batch of domains kept on the device (ygpu_batch_*): inputs are
+    uploaded once; ``run()`` solves all domains from the resident initial
+    state and returns the device seconds; ``download()`` writes the results
+    back into the problems' state arrays."""
+
+    def __init__(self, problems, par: SsaParams, diva: bool = False, handle=None):
+        self.h = handle or _lib.default_handle()
+        self.problems = problems
+        self.arr = (_lib.SsaProblemC * len(problems))()
+        self._keep = []
+        for k, p in enumerate(problems):
+            s, kp = _fill_struct(p, diva)
+            self.arr[k] = s
+            self._keep.append(kp)
+        self.pc = par.to_c()
+        b = C.c_void_p()
+        nbytes = C.c_int64()
+        self.h.check(self.h.lib.ygpu_batch_create(self.h.h, C.byref(self.pc), len(problems),
+                                                  self.arr, int(diva), C.byref(b),
+                                                  C.byref(nbytes)))
+        self.b = b
+        self.device_bytes = nbytes.value
+
+    def run(self, reset: bool = True) -> float:
+        secs = C.c_double()
+        self.h.check(self.h.lib.ygpu_batch_run(self.b, int(reset), C.byref(secs)))
+        return secs.value
+
+    def download(self) -> list[SolveInfo]:
+        self.h.check(self.h.lib.ygpu_batch_download(self.b, self.arr))
+        return [_info(self.arr[k]) for k in range(len(self.problems))]
+
+    def close(self):
+        if self.b:
+            self.h.lib.ygpu_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
