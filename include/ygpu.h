@@ -54,7 +54,7 @@ extern "C" {
 /* arithmetic of the outer solve */
 #define YGPU_PREC_FP64 0  /* FP64 fields, matrix and Krylov (plain FP64 path)          */
 #define YGPU_PREC_REF 1   /* FP32 fields, FP64 matrix and Krylov (the reference, wp=sp) */
-#define YGPU_PREC_MIXED 2 /* FP32 fields; FP32 SpMV/ILU inside FP64 refinement          */
+#define YGPU_PREC_MIXED 2 /* FP64 fields and residual; FP32 SpMV/ILU inside FP64 refinement */
 
 typedef struct ygpu_handle ygpu_handle;
 

@@ -111,12 +111,6 @@ void step_table(int nx, int ny, std::vector<int>& soff, int& T) {
   }
 }
 
-int skew_q(const std::vector<int>& soff, int i, int j, int ny) {
-  const int t = j + 2 * i;
-  const int a = t - (ny - 1);
-  const int glo = a <= 0 ? 0 : (a + 1) / 2;
-  return soff[t] + i - glo;
-}
 
 int bc_code(int bnd) {
   switch (bnd) {
@@ -163,7 +157,7 @@ struct Batch {
   YgMember* d_mem = nullptr;
   long long *d_coffs = nullptr, *d_zoffs = nullptr;
   int* d_soff = nullptr;
-  uint32_t *d_cellij = nullptr, *d_mask = nullptr;
+  uint32_t* d_mask = nullptr;
   double *d_A64 = nullptr, *d_b = nullptr, *d_x = nullptr, *d_F64 = nullptr, *d_W64 = nullptr;
   float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
   int *d_done = nullptr, *d_iter_now = nullptr, *d_lin_iter = nullptr, *d_lin_fail = nullptr,
@@ -232,6 +226,8 @@ struct Batch {
       step_table(P.nx, P.ny, st, T);
       if (T > YG_MAX_STEPS) { h->err = "domain too large for the step table"; return YGPU_ERR_ARG; }
       M.T = T;
+      M.maxcnt = 0;
+      for (int t = 0; t < T; ++t) M.maxcnt = std::max(M.maxcnt, st[t + 1] - st[t]);
       M.coff = coffs[k];
       M.toff = (long long)soff.size();
       M.dx = P.dx; M.dy = P.dy;
@@ -256,7 +252,6 @@ struct Batch {
     YG_CHECK(h, ar.alloc(&d_coffs, nmem + 1));
     YG_CHECK(h, ar.alloc(&d_zoffs, nmem + 1));
     YG_CHECK(h, ar.alloc(&d_soff, soff.size()));
-    YG_CHECK(h, ar.alloc(&d_cellij, C));
     YG_CHECK(h, ar.alloc(&d_mask, C));
     YG_CHECK(h, ar.alloc(&d_A64, 18 * C));
     YG_CHECK(h, ar.alloc(&d_b, 2 * C));
@@ -305,8 +300,6 @@ struct Batch {
     YG_CHECK(h, hipMemcpyAsync(d_coffs, mc.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
     YG_CHECK(h, hipMemcpyAsync(d_zoffs, mz.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
     YG_CHECK(h, hipMemcpyAsync(d_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, h->stream));
-    yg_launch_cellij(geom(), d_cellij, h->stream);
-    YG_CHECK(h, hipGetLastError());
     YG_CHECK(h, hipStreamSynchronize(h->stream));
     return YGPU_OK;
   }
@@ -505,7 +498,7 @@ struct Batch {
     if (diva) yg_launch_diva_init<R>(G, f, dv, fp, st);
     YgPicardState ps{d_iter_now, d_done, d_resid, d_lin_iter, d_lin_fail, d_lim_n};
     YgKrylovArgs ka{};
-    ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask; ka.cellij = d_cellij;
+    ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;
     ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.b = d_b; ka.x = d_x;
     ka.W64 = d_W64; ka.W32 = d_W32; ka.total_cells = total_cells;
     ka.wrapx = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_X);
@@ -513,6 +506,13 @@ struct Batch {
                 fp.BC == YG_BC_TROUGH);
     ka.skip = d_done; ka.out_iters = d_kit; ka.out_status = d_kst; ka.out_resid = d_kres;
     ka.opts = opts.k;
+    long long* d_tm = nullptr;
+    const bool phase = getenv("YGPU_PHASE_TIMERS") != nullptr;
+    if (phase) {
+      YG_CHECK(h, hipMalloc(&d_tm, sizeof(long long) * 8 * nmem));
+      YG_CHECK(h, hipMemsetAsync(d_tm, 0, sizeof(long long) * 8 * nmem, st));
+      ka.timers = d_tm;
+    }
     std::vector<int> done(nmem), kit(nmem);
     hipEvent_t k0 = nullptr, k1 = nullptr;
     if (h->stats_on) {
@@ -524,7 +524,7 @@ struct Batch {
       yg_launch_fields_pre<R>(G, f, fp, d_done, iter == 1, st);
       if (diva) yg_launch_diva_mid<R>(G, f, dv, fp, d_done, st);
       yg_launch_assemble<R>(G, f, fp, d_done, d_A64, kmode == YG_MODE_MIXED ? d_A32 : nullptr,
-                            d_b, d_x, d_mask, d_cellij, st);
+                            d_b, d_x, d_mask, st);
       std::vector<int> active;
       if (h->stats_on) {
         for (int k = 0; k < nmem; ++k) if (!done[k]) active.push_back(k);
@@ -562,6 +562,17 @@ struct Batch {
     }
     if (k0) (void)hipEventDestroy(k0);
     if (k1) (void)hipEventDestroy(k1);
+    if (phase) {
+      std::vector<long long> tmh(8 * nmem);
+      YG_CHECK(h, hipMemcpy(tmh.data(), d_tm, sizeof(long long) * 8 * nmem, hipMemcpyDeviceToHost));
+      (void)hipFree(d_tm);
+      int rate = 0;
+      (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, h->device);
+      const double us = rate > 0 ? 1e3 / rate : 1e-2;  // rate in kHz
+      fprintf(stderr, "ygpu phases member0 (ms): factor %.3f sweepL %.3f sweepU %.3f spmv %.3f total %.3f\n",
+              tmh[0] * us * 1e-3, tmh[1] * us * 1e-3, tmh[2] * us * 1e-3, tmh[3] * us * 1e-3,
+              tmh[7] * us * 1e-3);
+    }
     yg_launch_finish<R>(G, f, fp, ps, st);
     if (diva) yg_launch_diva_finish<R>(G, f, dv, fp, st);
     YG_CHECK(h, hipGetLastError());
@@ -623,9 +634,10 @@ int run_any(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_problem
   if (!h || !par || np < 1 || !probs) return YGPU_ERR_ARG;
   h->err.clear();
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
-  if (par->precision == YGPU_PREC_FP64) return run_batch<double>(h, par, np, probs, diva);
-  if (par->precision == YGPU_PREC_REF || par->precision == YGPU_PREC_MIXED)
-    return run_batch<float>(h, par, np, probs, diva);
+  // fields are FP64 except in the reference mode (the reference's wp = sp)
+  if (par->precision == YGPU_PREC_FP64 || par->precision == YGPU_PREC_MIXED)
+    return run_batch<double>(h, par, np, probs, diva);
+  if (par->precision == YGPU_PREC_REF) return run_batch<float>(h, par, np, probs, diva);
   h->err = "unknown precision";
   return YGPU_ERR_ARG;
 }
@@ -777,17 +789,17 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   if (T > YG_MAX_STEPS) { h->err = "grid too large for the step table"; return YGPU_ERR_UNSUPPORTED; }
   // CSR -> stencil slots in skewed order
   std::vector<double> A((size_t)18 * cells, 0.0), bs(n), xs(n);
-  std::vector<uint32_t> mask(cells, 0u), cellij(cells);
+  std::vector<uint32_t> mask(cells, 0u);
   static const int SU[2][9][3] = {
       {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {1, 0, 1}, {1, -1, 1}, {0, -1, 1}},
       {{0, 0, 1}, {0, 1, 1}, {0, -1, 1}, {1, 0, 1}, {-1, 0, 1}, {0, 0, 0}, {0, 1, 0}, {-1, 1, 0}, {-1, 0, 0}}};
   for (int r = 0; r < n; ++r) {
     const int cell = r / 2, comp = r & 1;
     const int i = cell / ny, j = cell % ny;
-    const int q = skew_q(soff, i, j, ny);
-    cellij[q] = ((uint32_t)i << 16) | (uint32_t)j;
-    bs[2 * q + comp] = b[r];
-    xs[2 * q + comp] = x_inout[r];
+    const int q = cell;   // vectors and the stencil matrix keep the row order
+    (void)i; (void)j;
+    bs[r] = b[r];
+    xs[r] = x_inout[r];
     for (int k = a_ptr1[r] - 1; k < a_ptr1[r + 1] - 1; ++k) {
       const int col = a_idx1[k] - 1;
       if (col < 0 || col >= n) { h->err = "column index out of range"; return YGPU_ERR_ARG; }
@@ -810,12 +822,14 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   Arena ar;
   YgMember M;
   M.nx = nx; M.ny = ny; M.nz = 1; M.T = T; M.coff = 0; M.toff = 0; M.dx = M.dy = 1.0;
+  M.maxcnt = 0;
+  for (int t = 0; t < T; ++t) M.maxcnt = std::max(M.maxcnt, soff[t + 1] - soff[t]);
   YgKrylovArgs ka{};
-  YgMember* d_mem; int* d_soff; uint32_t *d_mask, *d_cellij; double *d_A64, *d_b, *d_x, *d_F64, *d_W64;
+  YgMember* d_mem; int* d_soff; uint32_t* d_mask; double *d_A64, *d_b, *d_x, *d_F64, *d_W64;
   float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
   int *d_it, *d_st; double* d_res;
   YG_CHECK(h, ar.alloc(&d_mem, 1)); YG_CHECK(h, ar.alloc(&d_soff, soff.size()));
-  YG_CHECK(h, ar.alloc(&d_mask, cells)); YG_CHECK(h, ar.alloc(&d_cellij, cells));
+  YG_CHECK(h, ar.alloc(&d_mask, cells));
   YG_CHECK(h, ar.alloc(&d_A64, A.size())); YG_CHECK(h, ar.alloc(&d_b, n)); YG_CHECK(h, ar.alloc(&d_x, n));
   YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * cells)); YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * n));
   YG_CHECK(h, ar.alloc(&d_it, 1)); YG_CHECK(h, ar.alloc(&d_st, 1)); YG_CHECK(h, ar.alloc(&d_res, 1));
@@ -823,7 +837,6 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   YG_CHECK(h, hipMemcpyAsync(d_mem, &M, sizeof(M), hipMemcpyHostToDevice, st));
   YG_CHECK(h, hipMemcpyAsync(d_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, st));
   YG_CHECK(h, hipMemcpyAsync(d_mask, mask.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, st));
-  YG_CHECK(h, hipMemcpyAsync(d_cellij, cellij.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, st));
   YG_CHECK(h, hipMemcpyAsync(d_A64, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, st));
   YG_CHECK(h, hipMemcpyAsync(d_b, bs.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
   YG_CHECK(h, hipMemcpyAsync(d_x, xs.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
@@ -835,7 +848,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
     YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * n));
     YG_CHECK(h, hipMemcpyAsync(d_A32, A32.data(), sizeof(float) * A32.size(), hipMemcpyHostToDevice, st));
   }
-  ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask; ka.cellij = d_cellij;
+  ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;
   ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.b = d_b; ka.x = d_x;
   ka.W64 = d_W64; ka.W32 = d_W32; ka.total_cells = cells; ka.wrapx = 0; ka.wrapy = 0;
   ka.skip = nullptr; ka.out_iters = d_it; ka.out_status = d_st; ka.out_resid = d_res;
@@ -857,10 +870,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   YG_CHECK(h, hipMemcpy(&it, d_it, sizeof(int), hipMemcpyDeviceToHost));
   YG_CHECK(h, hipMemcpy(&stt, d_st, sizeof(int), hipMemcpyDeviceToHost));
   YG_CHECK(h, hipMemcpy(&res, d_res, sizeof(double), hipMemcpyDeviceToHost));
-  for (int r = 0; r < n; ++r) {
-    const int cell = r / 2, comp = r & 1;
-    x_inout[r] = xs[2 * skew_q(soff, cell / ny, cell % ny, ny) + comp];
-  }
+  for (int r = 0; r < n; ++r) x_inout[r] = xs[r];
   *iters = it; *status = stt; *rel_resid = res;
   return YGPU_OK;
 }
@@ -945,7 +955,7 @@ int ygpu_session_create(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_s
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   YgSessionBase* sb = nullptr;
   int rc;
-  if (par->precision == YGPU_PREC_FP64) {
+  if (par->precision != YGPU_PREC_REF) {
     auto* t = new YgSessionT<double>();
     rc = t->B.setup(h, par, 1, prob, 0);
     t->bytes = (int64_t)t->B.ar.bytes;
@@ -987,7 +997,7 @@ int ygpu_batch_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
     bb = t;
     return r;
   };
-  if (par->precision == YGPU_PREC_FP64) rc = make(new YgBatchT<double>());
+  if (par->precision != YGPU_PREC_REF) rc = make(new YgBatchT<double>());
   else rc = make(new YgBatchT<float>());
   if (rc) { delete bb; return rc; }
   if (device_bytes) *device_bytes = bb->bytes;
@@ -1024,7 +1034,7 @@ int ygpu_session_destroy(ygpu_session* s) {
 int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny, int32_t nz) {
   if (!par) return -1;
   const int64_t C = (int64_t)nx * ny, Z = C * nz;
-  const int64_t rb = par->precision == YGPU_PREC_FP64 ? 8 : 4;
+  const int64_t rb = par->precision != YGPU_PREC_REF ? 8 : 4;
   int64_t b = C * (4 + 4 + 18 * 8 + 2 * 8 + 2 * 8 + YG_NW64 * 2 * 8 + 8);
   if (par->precision == YGPU_PREC_MIXED) b += C * (18 * 4 + YG_NF * 4 + YG_NW32 * 2 * 4);
   else b += C * YG_NF * 8;

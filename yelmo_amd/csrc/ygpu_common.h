@@ -45,6 +45,7 @@ __constant__ static const int8_t kSlotUy[9][3] = {
 struct YgMember {
   int nx, ny, nz;
   int T;             // wavefront steps = ny + 2*(nx-1)
+  int maxcnt;        // most cells in one step
   long long coff;    // first cell of this member in packed per-cell arrays
   long long toff;    // first entry of this member in the packed step tables
   double dx, dy;

@@ -75,7 +75,7 @@ void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFi
 template <typename R>
 void yg_launch_assemble(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
                         const int* skip, double* A64, float* A32, double* b, double* x0,
-                        uint32_t* mask, uint32_t* cellij, hipStream_t st);
+                        uint32_t* mask, hipStream_t st);
 
 struct YgPicardState {
   int* iter_now;      // per member
@@ -99,5 +99,3 @@ template <typename R>
 void yg_launch_init(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
                     hipStream_t st);
 
-// natural <-> skewed layout helpers
-void yg_launch_cellij(const YgBatchGeom& G, uint32_t* cellij, hipStream_t st);

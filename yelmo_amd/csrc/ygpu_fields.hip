@@ -616,8 +616,8 @@ YG_DEV R visc_ab(const R* N, const R* fi, const R* zb, const R* zs, int nx, int 
 
 template <typename R>
 __global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
-                           double* A64, float* A32, double* bvec, double* x0, uint32_t* maskv,
-                           const uint32_t* cellij) {
+                           double* A64, float* A32, double* bvec, double* x0, uint32_t* maskv) {
+  // one thread per unknown pair in the reference's row order: n = i*ny + j
   const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gq >= G.total_cells) return;
   const int m = find_member(G.coffs, G.nmem, gq);
@@ -625,9 +625,8 @@ __global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const 
   const YgMember& mb = G.members[m];
   const long long c0 = G.coffs[m];
   const int q = (int)(gq - c0);
-  const uint32_t ij = cellij[gq];
-  const int i = ij >> 16, j = ij & 0xffff;
   const int nx = mb.nx, ny = mb.ny;
+  const int i = q / ny, j = q - i * ny;
   const int ncell = nx * ny;
   // border types (solver_ssa_ac.f90:148-196): 0 no-slip, 1 free-slip, 2 periodic
   int bc[4];  // 0 right, 1 upper, 2 left, 3 lower
@@ -782,7 +781,6 @@ __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f,
   const YgMember mb = G.members[m];
   const int nx = mb.nx, ny = mb.ny, ncell = nx * ny;
   const long long c0 = G.coffs[m];
-  const int* soff = G.soff + mb.toff;
   R* ux = (p.diva ? f.ubar_x : f.ux) + c0;
   R* uy = (p.diva ? f.ubar_y : f.uy) + c0;
   const R* uxp = f.ux_nm1 + c0;
@@ -796,9 +794,8 @@ __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f,
   double acc[4] = {0.0, 0.0, 0.0, 0.0};  // res1, res2, nchk, nonfinite
   for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
     const int i = c % nx, j = c / nx;
-    const int t = j + 2 * i;
-    const int q = soff[t] + i - yg_glo(t, ny);
-    R vx = (R)x[2 * (c0 + q)], vy = (R)x[2 * (c0 + q) + 1];
+    const long long n = c0 + (long long)i * ny + j;
+    R vx = (R)x[2 * n], vy = (R)x[2 * n + 1];
     if (yabs<R>(vx) < tol_uf) vx = R(0);
     if (yabs<R>(vy) < tol_uf) vy = R(0);
     if (vx > ulim) vx = ulim;
@@ -915,21 +912,6 @@ __global__ void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p) {
   f.err_y[g] = R(1);
 }
 
-__global__ void k_cellij(YgBatchGeom G, uint32_t* cellij) {
-  // inverse of the skewed numbering: walk the cells of step t, lines glo..ghi
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gc >= G.total_cells) return;
-  const int m = find_member(G.coffs, G.nmem, gc);
-  const YgMember& mb = G.members[m];
-  const int nx = mb.nx, ny = mb.ny;
-  const int loc = (int)(gc - G.coffs[m]);
-  const int i = loc % nx, j = loc / nx;
-  const int t = j + 2 * i;
-  const int* soff = G.soff + mb.toff;
-  const int q = soff[t] + i - yg_glo(t, ny);
-  cellij[G.coffs[m] + q] = ((uint32_t)i << 16) | (uint32_t)j;
-}
-
 }  // namespace
 
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -952,9 +934,9 @@ void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFi
 template <typename R>
 void yg_launch_assemble(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
                         const int* skip, double* A64, float* A32, double* b, double* x0,
-                        uint32_t* mask, uint32_t* cellij, hipStream_t st) {
+                        uint32_t* mask, hipStream_t st) {
   hipLaunchKernelGGL(k_assemble<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p,
-                     skip, A64, A32, b, x0, mask, (const uint32_t*)cellij);
+                     skip, A64, A32, b, x0, mask);
 }
 
 template <typename R>
@@ -971,9 +953,6 @@ void yg_launch_finish(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldP
   hipLaunchKernelGGL(k_finish<R>, dim3(G.nmem), dim3(1024), 0, st, G, f, p, ps);
 }
 
-void yg_launch_cellij(const YgBatchGeom& G, uint32_t* cellij, hipStream_t st) {
-  hipLaunchKernelGGL(k_cellij, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, cellij);
-}
 
 #define YG_INST(R)                                                                              \
   template void yg_launch_init<R>(const YgBatchGeom&, const YgFields<R>&, const YgFieldParams&, \
@@ -982,7 +961,7 @@ void yg_launch_cellij(const YgBatchGeom& G, uint32_t* cellij, hipStream_t st) {
                                         const YgFieldParams&, const int*, int, hipStream_t);     \
   template void yg_launch_assemble<R>(const YgBatchGeom&, const YgFields<R>&,                   \
                                       const YgFieldParams&, const int*, double*, float*,         \
-                                      double*, double*, uint32_t*, uint32_t*, hipStream_t);      \
+                                      double*, double*, uint32_t*, hipStream_t);                 \
   template void yg_launch_epilogue<R>(const YgBatchGeom&, const YgFields<R>&,                   \
                                       const YgFieldParams&, const double*, const int*,           \
                                       const int*, int, int, YgPicardState, hipStream_t);         \

@@ -28,13 +28,12 @@ struct YgKrylovOpts {
 struct YgKrylovArgs {
   const YgMember* members;
   const int* soff;          // packed step offsets, T+1 per member at members[m].toff
-  const uint32_t* mask;     // per cell presence bits (skewed order)
-  const uint32_t* cellij;   // per cell (i << 16) | j
-  const double* A64;        // 18 per cell: [slot][cell][comp]
+  const uint32_t* mask;     // per natural cell (n = i*ny + j) presence bits
+  const double* A64;        // 18 per cell: [slot][natural cell][comp]
   const float* A32;
-  double* F64;              // ILU factors / jacobi 1/diag
+  double* F64;              // ILU factors (skewed wavefront order) / jacobi 1/diag
   float* F32;
-  const double* b;          // 2 per cell
+  const double* b;          // 2 per cell, the reference's row order 2n, 2n+1
   double* x;                // 2 per cell, x0 in, solution out
   double* W64;              // YG_NW64 * 2 per cell
   float* W32;               // YG_NW32 * 2 per cell
@@ -44,6 +43,7 @@ struct YgKrylovArgs {
   int* out_iters;
   int* out_status;
   double* out_resid;
+  long long* timers;        // optional: per member 8 phase totals (wall clock ticks)
   YgKrylovOpts opts;
 };
 
