@@ -11,9 +11,11 @@ refinement).  Inputs are resident in HBM before the timed region; each step
 restores the initial state on the device and solves all members again.
 
 The line also reports the plain FP64 GPU path on the same ensemble (batched,
-and one member at a time), the per-member relative L2 velocity error of the
-mixed path against it, the Krylov kernel's HBM roofline, and the CPU oracle
-timed on this host's cores on a bounded sample of the same workload.
+and one member at a time), the per-member relative L2 and max velocity error
+of the mixed path against the FP64 path (with the linear solves converged to
+rtol 1e-12, and at the reference's rtol 1e-8), the Krylov kernel's HBM
+roofline, and the CPU oracle timed on this host's cores on a bounded sample
+of the same workload.
 """
 from __future__ import annotations
 
@@ -40,42 +42,37 @@ def _sync():
         pass
 
 
+def _finite(v):
+    """NaN (a part that was not run) becomes null, so the line stays JSON."""
+    if isinstance(v, dict):
+        return {k: _finite(x) for k, x in v.items()}
+    if isinstance(v, float) and v != v:
+        return None
+    return v
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--members", type=int, default=64)
-    ap.add_argument("--side", type=int, default=161)
-    ap.add_argument("--cpu-sample", type=int, default=2,
-                    help="ensemble members solved by the CPU oracle for the baseline")
-    ap.add_argument("--single-sample", type=int, default=4,
-                    help="members solved one at a time on the FP64 GPU path")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU: the ensemble members are independent and a "
-              f"multi-GPU run would only split them (--gpus {args.gpus} not supported)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    from yelmo_amd import _lib, generator
-    from yelmo_amd.params import SsaParams
+def fp64_paths(args, base, par_fp64, h, vel):
+    """The FP64 GPU path on the ensemble: the converged error reference, the
+    batched run at the reference's solver settings and the single-member
+    rate."""
     from yelmo_amd.velocity import ResidentBatch
 
-    t_gen = time.perf_counter()
-    base = generator.workload_ensemble(args.members, args.side, seed=0, dtype=np.float32)
-    print(f"# generated {args.members} x {args.side}^2 in {time.perf_counter() - t_gen:.1f}s",
-          file=sys.stderr, flush=True)
-    par_mixed = SsaParams(precision="mixed")
-    par_fp64 = SsaParams(precision="fp64")
-    h = _lib.default_handle()
+    # ---- error reference: the FP64 path with the linear solves converged to
+    # rtol 1e-12, i.e. the Picard solution without the 1e-8 solver tolerance ----
+    par_conv = par_fp64.with_(ssa_lis_opt=par_fp64.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
+    probs_c = [p.astype(np.float64) for p in base]
+    rbc = ResidentBatch(probs_c, par_conv, handle=h)
+    rbc.run()
+    rbc.download()
+    rbc.close()
+    ref = [vel(p) for p in probs_c]
+    del probs_c
 
-    # ---- plain FP64 GPU path, batched: the error reference --------------------
+    # ---- plain FP64 GPU path (reference solver settings), batched ---------------
     probs64 = [p.astype(np.float64) for p in base]
     rb64 = ResidentBatch(probs64, par_fp64, handle=h)
     rb64.run()                      # warm-up
@@ -85,8 +82,10 @@ def main():
     _sync()
     t_fp64_batch = time.perf_counter() - t0
     info64 = rb64.download()
-    ref = [np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]) for p in probs64]
+    plain = [vel(p) for p in probs64]
     rb64.close()
+    del probs64
+    err64 = [rel_l2(plain[k], ref[k]) for k in range(args.members)]
     print(f"# fp64 batched: {t_fp64_batch:.3f}s", file=sys.stderr, flush=True)
 
     # ---- plain FP64 GPU path, one member at a time ------------------------------
@@ -103,6 +102,66 @@ def main():
         rb.close()
     t_fp64_single = float(np.mean(singles))
     print(f"# fp64 single: {t_fp64_single:.3f}s per solve", file=sys.stderr, flush=True)
+
+    return ref, plain, err64, t_fp64_batch, t_fp64_single
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--side", type=int, default=161)
+    ap.add_argument("--cpu-sample", type=int, default=2,
+                    help="ensemble members solved by the CPU oracle for the baseline")
+    ap.add_argument("--single-sample", type=int, default=4,
+                    help="members solved one at a time on the FP64 GPU path")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time the mixed batched solve only (for counter runs): no FP64 "
+                         "reference, no CPU baseline, error fields null")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark, run a plain device copy of MIB MiB "
+                         "five times (the reference kernel for scaling HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the ensemble members are independent and a "
+              f"multi-GPU run would only split them (--gpus {args.gpus} not supported)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_calibrate:
+        import torch
+        n = args.copy_calibrate * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibrate_bytes_read": 4 * n, "copy_calibrate_bytes_written": 4 * n}))
+        return
+
+    from yelmo_amd import _lib, generator
+    from yelmo_amd.params import SsaParams
+    from yelmo_amd.velocity import ResidentBatch
+
+    t_gen = time.perf_counter()
+    base = generator.workload_ensemble(args.members, args.side, seed=0, dtype=np.float32)
+    print(f"# generated {args.members} x {args.side}^2 in {time.perf_counter() - t_gen:.1f}s",
+          file=sys.stderr, flush=True)
+    par_mixed = SsaParams(precision="mixed")
+    par_fp64 = SsaParams(precision="fp64")
+    h = _lib.default_handle()
+
+    def vel(p):
+        return np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]).astype(np.float64)
+
+    ref = plain = None
+    err64 = [float("nan")]
+    t_fp64_batch = t_fp64_single = float("nan")
+    if not args.mixed_only:
+        ref, plain, err64, t_fp64_batch, t_fp64_single = fp64_paths(args, base, par_fp64, h, vel)
 
     # ---- mixed precision, batched: the measured value ---------------------------
     probs = [p.copy() for p in base]
@@ -122,10 +181,14 @@ def main():
     st = h.stats()
     h.stats_enable(False)
     info = rb.download()
-    errs = []
-    for k, p in enumerate(probs):
-        v = np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]).astype(np.float64)
-        errs.append(rel_l2(v, ref[k]))
+    errs, errs_plain, maxabs = [float("nan")], [float("nan")], [float("nan")]
+    if ref is not None:
+        errs, errs_plain, maxabs = [], [], []
+        for k, p in enumerate(probs):
+            v = vel(p)
+            errs.append(rel_l2(v, ref[k]))
+            errs_plain.append(rel_l2(v, plain[k]))
+            maxabs.append(float(np.max(np.abs(v - ref[k]))))
     rb.close()
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.members * args.steps / elapsed
@@ -140,7 +203,7 @@ def main():
     # ---- CPU baseline: the oracle (FP32 fields + FP64 BiCGSTAB/ILU(0), the
     # reference's arithmetic), one thread, on a bounded sample of members -------
     cpu = None
-    if not args.no_cpu and args.cpu_sample > 0:
+    if not args.no_cpu and not args.mixed_only and args.cpu_sample > 0:
         from oracle import oracle
         par_ref = SsaParams(precision="ref")
         t0 = time.perf_counter()
@@ -172,7 +235,14 @@ def main():
             "members": args.members, "side": args.side,
             "solver": par_mixed.ssa_lis_opt, "picard_tol": par_mixed.ssa_iter_conv,
         },
+        # per-member velocity error of the mixed path against the FP64 path
+        # with converged linear solves (rtol 1e-12), and against the FP64
+        # path at the reference's rtol 1e-8, whose own distance from the
+        # converged solution is given beside it
         "max_rel_l2_vs_fp64": max(errs),
+        "max_abs_vel_err_m_per_yr": max(maxabs),
+        "max_rel_l2_vs_fp64_rtol1e-8": max(errs_plain),
+        "fp64_rtol1e-8_max_rel_l2_vs_converged": max(err64),
         "error_ok": max(errs) <= 1e-6,
         "fp64_batched_solves_per_s": fp64_batch_sps,
         "fp64_single_solves_per_s": fp64_single_sps,
@@ -189,7 +259,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "launches": launches},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(_finite(line)), flush=True)
 
 
 if __name__ == "__main__":
