@@ -14,7 +14,7 @@
  * the reference tree).  Arrays are Fortran column-major (nx,ny[,nz]) with i
  * fastest: a(i,j,k) -> a[i + nx*(j + ny*k)], indices 0-based here.
  *
- * Parity status (see DESIGN.md for the pinning checks):
+ * Parity status (the pinning checks are the tests in tests/test_oracle_pins.py):
  *   - assembly, masks, strain rates, aa-viscosity, power/Coulomb beta at the
  *     cell centre (beta_method 4/5), staggering, limit drag, Picard norms:
  *     pinned by closed forms / invariants / manufactured solutions in tests/.
