@@ -55,7 +55,7 @@ def test_ssa_ref_mode_matches_fp32_oracle(n, dx, vm, bm):
     assert err <= 1e-5
 
 
-@pytest.mark.parametrize("n,dx,vm,bm", CASES[:2])
+@pytest.mark.parametrize("n,dx,vm,bm", CASES)
 def test_ssa_mixed_matches_fp64_oracle(n, dx, vm, bm):
     print(f"case mixed n={n} visc={vm} beta={bm}", flush=True)
     par = SsaParams(visc_method=vm, beta_method=bm, precision="mixed")
@@ -81,10 +81,18 @@ def test_csr_solve_matches_oracle():
     ap, ai, av, b, x0, _ = oracle.assemble(q, np.asfortranarray(st["beta_acx"]),
                                            np.asfortranarray(st["beta_acy"]),
                                            np.asfortranarray(st["visc_eff_int"]))
-    for opts in ("-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false",
-                 "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros true",
-                 "-i bicgstab -p jacobi -maxiter 3000 -tol 1.0e-8 -initx_zeros false"):
-        ref = oracle.csr_solve(ap, ai, av, b, x0, opts)
+    base = "-i bicgstab -p {p} -maxiter {m} -tol 1.0e-8 -initx_zeros {z}"
+    cases = [
+        (base.format(p="ilu", m=1000, z="false"), None),
+        (base.format(p="ilu", m=1000, z="true"), None),
+        (base.format(p="jacobi", m=3000, z="false"), None),
+        # block-Jacobi ILU(0) over 3 line blocks of 27 lines: the oracle's
+        # ILU(0) over 3 contiguous row blocks is the same preconditioner
+        (base.format(p="ilu", m=1000, z="false") + " -ygpu_ilu_blocks 3",
+         base.format(p="ilu", m=1000, z="false") + " -ilu_blocks 3"),
+    ]
+    for opts, oopts in cases:
+        ref = oracle.csr_solve(ap, ai, av, b, x0, oopts or opts)
         lgs = velocity.Lgs(ap, ai, av, b, x0.copy())
         velocity.linear_solver_matrix_solve(lgs, opts)
         err = rel_l2(lgs.x_value, ref["x"])
@@ -93,6 +101,21 @@ def test_csr_solve_matches_oracle():
         assert lgs.status == ref["status"] == 0
         assert abs(lgs.lin_iter - ref["iters"]) <= 1
         assert err <= 1e-8
+    # mixed precision (FP32 block ILU(0) inside FP64 refinement) meets the
+    # same FP64 residual criterion.  Its solution is compared with the
+    # oracle's solve converged to rtol 1e-12 (the oracle's own rtol 1e-8
+    # solution differs from that by the solver tolerance times the
+    # condition number, about 1e-6 here).
+    opts = base.format(p="ilu", m=1000, z="false")
+    exact = oracle.csr_solve(ap, ai, av, b, x0, opts.replace("1.0e-8", "1.0e-12"))
+    ref = oracle.csr_solve(ap, ai, av, b, x0, opts)
+    lgs = velocity.Lgs(ap, ai, av, b, x0.copy())
+    velocity.linear_solver_matrix_solve(lgs, opts + " -ygpu_precision mixed")
+    err = rel_l2(lgs.x_value, exact["x"])
+    print("mixed iters", lgs.lin_iter, "resid", lgs.L2_rel_norm, "err", err,
+          "oracle rtol 1e-8 err", rel_l2(ref["x"], exact["x"]))
+    assert lgs.status == 0 and lgs.L2_rel_norm <= 1e-8
+    assert err <= 1e-6
 
 
 def test_batch_alone_equals_in_mix():

@@ -44,7 +44,7 @@ struct Opts {
 
 int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
-  o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3;
+  o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1;
   if (!s) return 0;
   std::vector<std::string> tok;
   std::string cur;
@@ -72,6 +72,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k == "-conv_cond") o.k.conv_cond = atoi(v.c_str());
     else if (k == "-ir_inner_tol") o.k.inner_tol = atof(v.c_str());
     else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
+    else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
     else if (k == "-ygpu_ny") o.ny_hint = atoi(v.c_str());
     else if (k == "-ygpu_precision") {
       if (v == "fp64") o.prec = 0;
@@ -145,7 +146,7 @@ struct Batch {
   ygpu_handle* h = nullptr;
   Arena ar;
   int nmem = 0, diva = 0, prec = 0, kmode = 0, block = 1024;
-  long long total_cells = 0, total_z = 0;
+  long long total_cells = 0, total_z = 0, total_v = 0;
   int zstride = 0;
   std::vector<YgMember> mem;
   std::vector<long long> coffs, zoffs;
@@ -228,10 +229,17 @@ struct Batch {
       M.T = T;
       M.maxcnt = 0;
       for (int t = 0; t < T; ++t) M.maxcnt = std::max(M.maxcnt, st[t + 1] - st[t]);
+      if (M.maxcnt > YG_MAX_STEP_CELLS) {
+        h->err = "domain " + std::to_string(k) + ": a wavefront step holds more cells than the LDS stage";
+        return YGPU_ERR_ARG;
+      }
       M.coff = coffs[k];
       M.toff = (long long)soff.size();
       M.dx = P.dx; M.dy = P.dy;
       soff.insert(soff.end(), st.begin(), st.end());
+      M.vlen = (int)(((2LL * P.nx * P.ny + 3) / 4) * 4);
+      M.woff = total_v;
+      total_v += M.vlen;
       coffs[k + 1] = coffs[k] + (long long)P.nx * P.ny;
       zoffs[k + 1] = zoffs[k] + (long long)P.nx * P.ny * P.nz;
       zstride = std::max(zstride, P.nz);
@@ -240,7 +248,7 @@ struct Batch {
     }
     total_cells = coffs[np];
     total_z = zoffs[np];
-    block = par->block > 0 ? par->block : 1024;
+    block = par->block > 0 ? par->block : (maxnx <= 512 ? 512 : 1024);
     block = std::max(block, ((maxnx + 63) / 64) * 64);
     block = std::min(block, 1024);
     return allocate();
@@ -256,11 +264,11 @@ struct Batch {
     YG_CHECK(h, ar.alloc(&d_A64, 18 * C));
     YG_CHECK(h, ar.alloc(&d_b, 2 * C));
     YG_CHECK(h, ar.alloc(&d_x, 2 * C));
-    YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * 2 * C));
+    YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * total_v));
     if (kmode == YG_MODE_MIXED) {
       YG_CHECK(h, ar.alloc(&d_A32, 18 * C));
       YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * C));
-      YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * 2 * C));
+      YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * total_v));
     } else {
       YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * C));
     }
@@ -822,8 +830,14 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   Arena ar;
   YgMember M;
   M.nx = nx; M.ny = ny; M.nz = 1; M.T = T; M.coff = 0; M.toff = 0; M.dx = M.dy = 1.0;
+  M.woff = 0;
+  M.vlen = (int)(((2LL * cells + 3) / 4) * 4);
   M.maxcnt = 0;
   for (int t = 0; t < T; ++t) M.maxcnt = std::max(M.maxcnt, soff[t + 1] - soff[t]);
+  if (M.maxcnt > YG_MAX_STEP_CELLS) {
+    h->err = "a wavefront step holds more cells than the LDS stage";
+    return YGPU_ERR_UNSUPPORTED;
+  }
   YgKrylovArgs ka{};
   YgMember* d_mem; int* d_soff; uint32_t* d_mask; double *d_A64, *d_b, *d_x, *d_F64, *d_W64;
   float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
@@ -831,7 +845,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   YG_CHECK(h, ar.alloc(&d_mem, 1)); YG_CHECK(h, ar.alloc(&d_soff, soff.size()));
   YG_CHECK(h, ar.alloc(&d_mask, cells));
   YG_CHECK(h, ar.alloc(&d_A64, A.size())); YG_CHECK(h, ar.alloc(&d_b, n)); YG_CHECK(h, ar.alloc(&d_x, n));
-  YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * cells)); YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * n));
+  YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * cells)); YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * M.vlen));
   YG_CHECK(h, ar.alloc(&d_it, 1)); YG_CHECK(h, ar.alloc(&d_st, 1)); YG_CHECK(h, ar.alloc(&d_res, 1));
   hipStream_t st = h->stream;
   YG_CHECK(h, hipMemcpyAsync(d_mem, &M, sizeof(M), hipMemcpyHostToDevice, st));
@@ -845,7 +859,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
     A32.resize(A.size());
     for (size_t q = 0; q < A.size(); ++q) A32[q] = (float)A[q];
     YG_CHECK(h, ar.alloc(&d_A32, A32.size())); YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * cells));
-    YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * n));
+    YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * M.vlen));
     YG_CHECK(h, hipMemcpyAsync(d_A32, A32.data(), sizeof(float) * A32.size(), hipMemcpyHostToDevice, st));
   }
   ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;
@@ -856,7 +870,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   hipEvent_t e0, e1;
   YG_CHECK(h, hipEventCreate(&e0)); YG_CHECK(h, hipEventCreate(&e1));
   YG_CHECK(h, hipEventRecord(e0, st));
-  const int block = std::min(1024, std::max(256, ((nx + 63) / 64) * 64));
+  const int block = nx <= 512 ? 512 : std::min(1024, ((nx + 63) / 64) * 64);
   yg_launch_krylov(ka, kmode, 1, block, st);
   YG_CHECK(h, hipGetLastError());
   YG_CHECK(h, hipEventRecord(e1, st));

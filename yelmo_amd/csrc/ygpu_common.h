@@ -34,12 +34,19 @@ __constant__ static const int8_t kSlotUy[9][3] = {
     {0, 0, 1}, {0, 1, 1}, {0, -1, 1}, {1, 0, 1}, {-1, 0, 1},
     {0, 0, 0}, {0, 1, 0}, {-1, 1, 0}, {-1, 0, 0}};
 
-// ILU(0) factor storage per cell (SoA over the skewed cell index):
-//   L ux row : slots 2,4,8            -> F[0..2]
-//   L uy row : slots 8,4,7,2,5        -> F[3..7]
-//   U ux row : slots 5,3,7,1,6 + 1/d  -> F[8..13]
-//   U uy row : slots 6,1,3 + 1/d      -> F[14..17]
-#define YG_NF 18
+// ILU(0) factor values of a cell:
+//   L ux row : slots 2,4,8            -> F0..F2
+//   L uy row : slots 8,4,7,2,5        -> F3..F7
+//   U ux row : slots 5,3,7,1,6 + 1/d  -> F8..F13
+//   U uy row : slots 6,1,3 + 1/d      -> F14..F17
+// Two storage layouts, YG_NF values per cell:
+//   coupled (exact ILU(0) over more than 64 lines): SoA, F[v*ncell + q] over
+//     the member's skewed cell index q;
+//   streamed (line blocks of at most 64 lines, one per wave): AoS in
+//     block-skewed order, L part [q][8] (F0..F7) then U part [q][12]
+//     (F8..F17 and two pad values), so that one lane's cell is 2 (3)
+//     16-byte chunks and one step of a block is a contiguous range.
+#define YG_NF 20
 
 // One member (one independent system) of a batch.
 struct YgMember {
@@ -48,6 +55,8 @@ struct YgMember {
   int maxcnt;        // most cells in one step
   long long coff;    // first cell of this member in packed per-cell arrays
   long long toff;    // first entry of this member in the packed step tables
+  long long woff;    // first value of this member's work vectors (multiple of 4)
+  int vlen;          // length of one work vector: 2*ncell rounded up to a multiple of 4
   double dx, dy;
 };
 

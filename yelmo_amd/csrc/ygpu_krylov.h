@@ -9,6 +9,10 @@
 #define YG_MODE_MIXED 1
 #define YG_MAX_STEPS 3072   // ny + 2*(nx-1) limit of the LDS step table
 #define YG_MAX_LINES 1024   // one lane per grid line, one block per member
+// Most cells of one wavefront step (min(nx, ceil(ny/2))): one step of the
+// FP64 factorization stage (19 doubles per cell, two buffers) must fit the
+// kernel's dynamic LDS.
+#define YG_MAX_STEP_CELLS 400
 
 // Krylov workspace vectors per member (2 values per cell each)
 #define YG_NW64 9
@@ -21,6 +25,9 @@ struct YgKrylovOpts {
   int initx_zeros;
   int conv_cond;    // 0 ||r||/||r0||, 1 ||r||/||b||
   int ir_max;       // mixed mode: refinement sweeps
+  int ilu_blocks;   // 0 exact ILU(0); n > 0 block-Jacobi ILU(0) over n line blocks
+                    // (at least one per 64 lines); -1 automatic: exact in the
+                    // FP64 mode, one block per 64 lines in the mixed mode
   double tol;
   double inner_tol; // mixed mode: relative tolerance of each FP32 inner solve
 };

@@ -1,0 +1,1305 @@
+// ygpu_krylov_impl.h -- persistent per-system Krylov solve on the stencil matrix.
+//
+// Included once per block size by ygpu_krylov.hip (512 threads) and
+// ygpu_krylov_1024.hip (1024 threads, grids of more than 512 lines): each
+// translation unit holds its own copies of the device functions, so their
+// register budget follows the one kernel that calls them (256 VGPRs per
+// lane at 512 threads, 128 at 1024).
+//
+// One workgroup owns one system of a batch for the whole linear solve:
+// ILU(0) factorization, preconditioned BiCGSTAB (or CG) and, in the mixed
+// precision mode, the FP64 iterative-refinement loop around an FP32 inner
+// solve.  Nothing is launched per Krylov step and no host synchronisation
+// happens inside a solve; every member of a batch runs concurrently on its
+// own CU.  The semantics follow the LIS call behind
+// linear_solver_matrix_solve (solver_linear.F90:151-400): b == 0 gives x = 0,
+// status 0 converged / 2 breakdown / 4 iteration limit, the reported residual
+// is ||b - A x|| relative to ||r0|| (LIS conv_cond 0) or ||b|| (conv_cond 1).
+#pragma once
+#include "ygpu_common.h"
+#include "ygpu_krylov.h"
+
+// Wavefront phases are separate device functions: each gets its own register
+// allocation instead of sharing the pressure of the whole inlined solver.
+#define YG_NOINLINE __device__ __attribute__((noinline))
+
+namespace {
+
+// --------------------------------------------------------------------------
+// per-block context
+//
+// Grid lines are mapped onto the waves of the block for the wavefront
+// recurrences (ILU(0) factorization and the triangular sweeps).  Wave w owns
+// lines [w*lpw, w*lpw + lpw), lane l line w*lpw + l.  Two schemes:
+//   coupled  (exact ILU(0)): lpw = 64; the first lane of a wave reads the
+//            last lane of the wave before it through an LDS ring, one block
+//            barrier per wavefront step;
+//   blocked  (block-Jacobi ILU(0)): the lines are split into nb line blocks
+//            of lpw <= 64 lines, one per wave; couplings between blocks are
+//            dropped from the factorization, so waves never wait for each
+//            other inside a sweep.
+struct Ctx {
+  const YgMember* mb;
+  int nx, ny, T;
+  int ncell;
+  int vlen;                   // stride of the work vectors (values, multiple of 4)
+  int wrapx, wrapy;
+  const uint32_t* mask;       // per natural cell: presence bits of its two rows
+  double* red;                // LDS reduction scratch
+  int maxcnt;                 // most cells in one wavefront step
+  int line;                   // grid line of this thread in the recurrences, -1 if none
+  int coupled;                // waves exchange boundary values every step
+  int nwl;                    // waves that own lines
+  int lpw;                    // lines per wave (the line block of a wave)
+  long long* tm;              // phase timers (thread 0), may be null
+};
+
+// optional phase timing: 0 factor, 1 sweep L, 2 sweep U, 3 spmv, 4 vectors
+struct PhaseTimer {
+  const Ctx& c;
+  int k;
+  long long t0;
+  YG_DEV PhaseTimer(const Ctx& cc, int kk) : c(cc), k(kk), t0(0) {
+    if (c.tm && threadIdx.x == 0) t0 = wall_clock64();
+  }
+  YG_DEV ~PhaseTimer() {
+    if (c.tm && threadIdx.x == 0) c.tm[k] += wall_clock64() - t0;
+  }
+};
+
+// The stage buffers live in the kernel's dynamic LDS.  They are always
+// addressed from this symbol (never through a stored pointer) so that the
+// compiler emits LDS instructions for them.
+extern __shared__ __align__(16) unsigned char yg_dyn_lds[];
+
+// Element k of the dynamic LDS viewed as T.  Stage buffers are addressed by
+// integer offsets through this accessor (never through a pointer held in a
+// variable), which keeps every access an LDS instruction: a generic pointer
+// would make the compiler emit flat accesses, whose completion waits also
+// cover every outstanding global load.
+template <typename T>
+YG_DEV T& lds(int k) { return reinterpret_cast<T*>(yg_dyn_lds)[k]; }
+
+// The block's copy of its member's step table and the ring through which
+// neighbouring waves pass boundary values, addressed by symbol for the same
+// reason.
+__shared__ int yg_soff[YG_MAX_STEPS + 1];
+__shared__ double yg_ring[16][3][10];
+
+// Global-memory views of pointers handed to the (non-inlined) wavefront
+// phases: the compiler then emits global (not flat) accesses, whose
+// completion counter it can track per instruction, so loads issued ahead
+// stay in flight.
+#define YG_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+YG_DEV const YG_GLOBAL T* gc(const T* p) { return (const YG_GLOBAL T*)p; }
+template <typename T>
+YG_DEV YG_GLOBAL T* gm(T* p) { return (YG_GLOBAL T*)p; }
+
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+// y = A x for the two rows of natural cell n = i*ny + j (the reference's
+// unknown numbering: rows 2n, 2n+1).  Absent stencil slots hold exact zeros
+// (the assembly and the CSR conversion zero-fill them), so every slot is
+// multiplied through; a neighbour index past a non-periodic border is
+// clamped into the domain, where it meets a zero coefficient.  The 18
+// coefficients arrive as 9 (ux,uy) pairs and x as the pairs of the 7 cells
+// of the stencil.
+template <typename TA, typename TV>
+YG_DEV void spmv_cell(const Ctx& c, const TA* __restrict__ A, const TV* __restrict__ x, int n,
+                      double& yx, double& yy) {
+  using A2 = typename Vec2<TA>::type;
+  using V2 = typename Vec2<TV>::type;
+  const int ny = c.ny, i = n / ny, j = n - i * ny;
+  int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1;
+  if (c.wrapx) { ip = ip == c.nx ? 0 : ip; im = im < 0 ? c.nx - 1 : im; }
+  else { ip = min(ip, c.nx - 1); im = max(im, 0); }
+  if (c.wrapy) { jp = jp == ny ? 0 : jp; jm = jm < 0 ? ny - 1 : jm; }
+  else { jp = min(jp, ny - 1); jm = max(jm, 0); }
+  const V2* __restrict__ X = reinterpret_cast<const V2*>(x);
+  const A2* __restrict__ M = reinterpret_cast<const A2*>(A);
+  const size_t nc = c.ncell;
+  const V2 x0 = X[n], xip = X[ip * ny + j], xim = X[im * ny + j], xjp = X[i * ny + jp],
+           xjm = X[i * ny + jm], xipjm = X[ip * ny + jm], ximjp = X[im * ny + jp];
+  A2 a[9];
+#pragma unroll
+  for (int s = 0; s < 9; ++s) a[s] = M[s * nc + n];
+  // ux row: slots ux(i,j) ux(i+1,j) ux(i-1,j) ux(i,j+1) ux(i,j-1)
+  //              uy(i,j) uy(i+1,j) uy(i+1,j-1) uy(i,j-1)
+  yx = (double)a[0].x * x0.x + (double)a[1].x * xip.x + (double)a[2].x * xim.x +
+       (double)a[3].x * xjp.x + (double)a[4].x * xjm.x + (double)a[5].x * x0.y +
+       (double)a[6].x * xip.y + (double)a[7].x * xipjm.y + (double)a[8].x * xjm.y;
+  // uy row: slots uy(i,j) uy(i,j+1) uy(i,j-1) uy(i+1,j) uy(i-1,j)
+  //              ux(i,j) ux(i,j+1) ux(i-1,j+1) ux(i-1,j)
+  yy = (double)a[0].y * x0.y + (double)a[1].y * xjp.y + (double)a[2].y * xjm.y +
+       (double)a[3].y * xip.y + (double)a[4].y * xim.y + (double)a[5].y * x0.x +
+       (double)a[6].y * xjp.x + (double)a[7].y * ximjp.x + (double)a[8].y * xim.x;
+}
+
+// first line of step t that holds a cell: max(0, ceil((t - ny + 1) / 2))
+YG_DEV int glo_of(int t, int ny) { const int a = t - ny + 2; return a > 0 ? a >> 1 : 0; }
+
+// sum over s < t of floor(s / 2)
+YG_DEV int half_sum(int t) {
+  if (t <= 0) return 0;
+  const int m = t >> 1;
+  return (t & 1) ? m * m : m * (m - 1);
+}
+
+// Cells before step t of the wavefront over L lines of ny cells: the sum
+// over s < t of min(L-1, floor(s/2)) - max(0, ceil((s-ny+1)/2)) + 1, in
+// closed form (the step table of an L x ny grid, without a table).
+YG_DEV int blk_soff(int t, int L, int ny) {
+  const int a = t <= 2 * L ? half_sum(t) : half_sum(2 * L) + (t - 2 * L) * (L - 1);
+  const int b = t > ny ? half_sum(t - ny) + (t - ny) : 0;
+  return t + a - b;
+}
+
+// Cell of a wavefront chunk in stage slot tid: the chunk's steps t0..t0+C-1
+// hold at most C*maxcnt cells; slot tid takes step offset tid / maxcnt and
+// line offset tid % maxcnt (thread g serves slots g and g + blockDim).  Returns false if that slot is
+// past the step's last cell.  Gives the skewed index q, the natural index
+// nat and the stage slot l = dt*maxcnt + k of the cell.
+YG_DEV bool chunk_cell(const Ctx& c, int t0, int t1, int tid, int& q, int& nat, int& l) {
+  const int dt = tid / c.maxcnt, k = tid - dt * c.maxcnt;
+  const int t = t0 + dt;
+  if (t >= t1) return false;
+  const int cnt = yg_soff[t + 1] - yg_soff[t];
+  if (k >= cnt) return false;
+  const int i = glo_of(t, c.ny) + k, j = t - 2 * i;
+  q = yg_soff[t] + k;
+  nat = i * c.ny + j;
+  l = tid;
+  return true;
+}
+
+// stage slot of this thread's line at step t of the chunk starting at t0,
+// or -1 if its line has no cell at step t
+YG_DEV int step_slot(const Ctx& c, int t, int t0) {
+  const int j = t - 2 * c.line;
+  if (c.line < 0 || j < 0 || j >= c.ny) return -1;
+  return (t - t0) * c.maxcnt + c.line - glo_of(t, c.ny);
+}
+
+// ---- LDS staging of the wavefront sweeps ------------------------------------
+// A wavefront sweep touches, at step t, the contiguous cells [soff[t],
+// soff[t+1]) of the skewed order.  The steps are walked in chunks whose
+// inputs fit one LDS buffer; while the lanes run the recurrence of one chunk
+// out of LDS, every thread of the block already has the next chunk's global
+// loads in flight (in registers), so the per-step cost is the recurrence and
+// not a memory round trip.  Stage slot of a cell: (step - t0)*maxcnt + its
+// line's offset in the step, computed without a table lookup.
+
+#define YG_DYN_LDS (120 * 1024)
+
+// Barrier of one wavefront step: waits for this wave's LDS traffic (the ring
+// of boundary values, the staged outputs) and nothing else, so global loads
+// and stores stay in flight across it.  The memory clobber keeps the
+// compiler from moving LDS accesses across it.
+YG_DEV void step_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+
+template <typename T> YG_DEV T mask_to_stage(uint32_t m);
+template <> YG_DEV float mask_to_stage<float>(uint32_t m) { return __uint_as_float(m); }
+template <> YG_DEV double mask_to_stage<double>(uint32_t m) { return (double)m; }
+template <typename T> YG_DEV uint32_t stage_to_mask(T v);
+template <> YG_DEV uint32_t stage_to_mask<float>(float v) { return __float_as_uint(v); }
+template <> YG_DEV uint32_t stage_to_mask<double>(double v) { return (uint32_t)v; }
+
+// Neighbour lanes exchange values by DPP wave shifts (a VALU operation).
+YG_DEV float dpp_shr1(float v) {   // lane i <- lane i-1, lane 0 <- 0
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+YG_DEV float dpp_shl1(float v) {   // lane i <- lane i+1, lane 63 <- 0
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+YG_DEV double dpp_shr1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+YG_DEV double dpp_shl1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// ---- ILU(0) factorization as a wavefront over grid lines -----------------
+// The line of a thread handles cell (i, j = t - 2 i) at step t.  The U rows
+// of the two most recent cells of each lane stay in registers; a lane reads
+// the line before it through wave shifts, and across waves (coupled scheme)
+// through an LDS ring.
+struct URow {   // U part of the two rows of one cell
+  double x5, x3, x7, x1, x6, xd;  // ux row: slots 5,3,7,1,6, 1/diag
+  double y6, y1, y3, yd;          // uy row: slots 6,1,3, 1/diag
+};
+
+YG_DEV void urow_zero(URow& u) {
+  u.x5 = u.x3 = u.x7 = u.x1 = u.x6 = u.xd = 0.0;
+  u.y6 = u.y1 = u.y3 = u.yd = 0.0;
+}
+
+YG_DEV double shup(double v) { return dpp_shr1(v); }
+
+// TA is both the matrix and the factor type (FP64 path: double; the FP32
+// inner solve of the mixed path: float)
+template <typename TA>
+YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restrict__ F,
+                        int* bad) {
+  PhaseTimer ptm(c, 0);
+  const YG_GLOBAL TA* __restrict__ Ag = gc(A);
+  YG_GLOBAL TA* __restrict__ Fg = gm(F);
+  const int g = threadIdx.x, lane = g & 63, w = g >> 6;
+  const bool ring_in = c.coupled && w > 0 && lane == 0;
+  const bool ring_out = c.coupled && w + 1 < c.nwl && lane == 63;
+  const size_t nc = c.ncell;
+  constexpr int NV = 19;  // 18 slot values + the presence mask
+  const int capc = min(2 * (int)blockDim.x, (int)(YG_DYN_LDS / sizeof(TA)) / (2 * NV));
+  const int C = max(1, capc / c.maxcnt);
+  const int stride = C * c.maxcnt;
+  auto stage = [&](int sb, int t0) {   // the chunk starting at t0 -> stage at offset sb
+    for (int r = 0; r < 2; ++r) {
+      int q, nat, l;
+      if (chunk_cell(c, t0, min(c.T, t0 + C), g + r * (int)blockDim.x, q, nat, l)) {
+#pragma unroll
+        for (int v = 0; v < 18; ++v)
+          lds<TA>(sb + v * stride + l) = Ag[((size_t)(v >> 1) * nc + nat) * 2 + (v & 1)];
+        lds<TA>(sb + 18 * stride + l) = mask_to_stage<TA>(gc(c.mask)[nat]);
+      }
+    }
+  };
+  URow o1, o2;  // own cells at steps t-1 and t-2
+  urow_zero(o1);
+  urow_zero(o2);
+  const int nch = (c.T + C - 1) / C;
+  stage(0, 0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int t0 = ch * C, t1 = min(c.T, t0 + C);
+    const int sb = (ch & 1) * NV * stride;
+    for (int t = t0; t < t1; ++t) {
+      // P = line i-1 cell j (its step t-2), Q = line i-1 cell j+1 (its step t-1)
+      URow P, Q;
+      P.x5 = shup(o2.x5); P.x3 = shup(o2.x3); P.x7 = shup(o2.x7); P.x1 = shup(o2.x1);
+      P.x6 = shup(o2.x6); P.xd = shup(o2.xd); P.y6 = shup(o2.y6); P.y3 = shup(o2.y3);
+      P.yd = shup(o2.yd); P.y1 = 0.0;
+      Q.x7 = shup(o1.x7); Q.x1 = shup(o1.x1); Q.x6 = shup(o1.x6); Q.xd = shup(o1.xd);
+      Q.x5 = Q.x3 = Q.y6 = Q.y1 = Q.y3 = Q.yd = 0.0;
+      if (ring_in) {
+        const double* rp = yg_ring[w - 1][(t + 1) % 3];  // step t-2
+        const double* rq = yg_ring[w - 1][(t + 2) % 3];  // step t-1
+        P.x5 = rp[0]; P.x3 = rp[1]; P.x7 = rp[2]; P.x1 = rp[3]; P.x6 = rp[4]; P.xd = rp[5];
+        P.y6 = rp[6]; P.y1 = rp[7]; P.y3 = rp[8]; P.yd = rp[9];
+        Q.x5 = rq[0]; Q.x3 = rq[1]; Q.x7 = rq[2]; Q.x1 = rq[3]; Q.x6 = rq[4]; Q.xd = rq[5];
+        Q.y6 = rq[6]; Q.y1 = rq[7]; Q.y3 = rq[8]; Q.yd = rq[9];
+      }
+      const int l = step_slot(c, t, t0);
+      URow X;
+      urow_zero(X);
+      if (l >= 0) {
+        const int q = yg_soff[t] + c.line - glo_of(t, c.ny);
+        const uint32_t m = stage_to_mask<TA>(lds<TA>(sb + 18 * stride + l));
+        double a[9], b[9];
+#pragma unroll
+        for (int s = 0; s < 9; ++s) {
+          a[s] = (double)lds<TA>(sb + (2 * s) * stride + l);
+          b[s] = (double)lds<TA>(sb + (2 * s + 1) * stride + l);
+        }
+        const uint32_t mx = m & 0x1ffu, my = (m >> 16) & 0x1ffu;
+#define PX(s) ((mx >> (s)) & 1u)
+#define PY(s) ((my >> (s)) & 1u)
+        const URow& O = o1;
+        double lv;
+        // ux row, lower entries in column order: ux(i-1,j), ux(i,j-1), uy(i,j-1)
+        lv = a[2] * P.xd; a[2] = lv;
+        if (PX(8)) a[8] -= lv * P.x7;
+        if (PX(0)) a[0] -= lv * P.x1;
+        if (PX(5)) a[5] -= lv * P.x6;
+        lv = a[4] * O.xd; a[4] = lv;
+        if (PX(8)) a[8] -= lv * O.x5;
+        if (PX(0)) a[0] -= lv * O.x3;
+        if (PX(7)) a[7] -= lv * O.x6;
+        lv = a[8] * O.yd; a[8] = lv;
+        if (PX(0)) a[0] -= lv * O.y6;
+        if (PX(5)) a[5] -= lv * O.y1;
+        if (PX(7)) a[7] -= lv * O.y3;
+        X.xd = 1.0 / a[0];
+        X.x5 = a[5]; X.x3 = a[3]; X.x7 = a[7]; X.x1 = a[1]; X.x6 = a[6];
+        // uy row, lower entries: ux(i-1,j), uy(i-1,j), ux(i-1,j+1), uy(i,j-1), ux(i,j)
+        lv = b[8] * P.xd; b[8] = lv;
+        if (PY(4)) b[4] -= lv * P.x5;
+        if (PY(7)) b[7] -= lv * P.x3;
+        if (PY(2)) b[2] -= lv * P.x7;
+        if (PY(5)) b[5] -= lv * P.x1;
+        if (PY(0)) b[0] -= lv * P.x6;
+        lv = b[4] * P.yd; b[4] = lv;
+        if (PY(7)) b[7] -= lv * P.y6;
+        if (PY(0)) b[0] -= lv * P.y3;
+        lv = b[7] * Q.xd; b[7] = lv;
+        if (PY(0)) b[0] -= lv * Q.x7;
+        if (PY(6)) b[6] -= lv * Q.x1;
+        if (PY(1)) b[1] -= lv * Q.x6;
+        lv = b[2] * O.yd; b[2] = lv;
+        if (PY(5)) b[5] -= lv * O.y6;
+        if (PY(0)) b[0] -= lv * O.y1;
+        lv = b[5] * X.xd; b[5] = lv;
+        if (PY(0)) b[0] -= lv * X.x5;
+        if (PY(6)) b[6] -= lv * X.x3;
+        if (PY(3)) b[3] -= lv * X.x6;
+        X.yd = 1.0 / b[0];
+        X.y6 = b[6]; X.y1 = b[1]; X.y3 = b[3];
+#undef PX
+#undef PY
+        if (!(a[0] != 0.0) || !(b[0] != 0.0)) *bad = 1;
+        if (c.coupled) {
+          Fg[0 * nc + q] = (TA)a[2]; Fg[1 * nc + q] = (TA)a[4]; Fg[2 * nc + q] = (TA)a[8];
+          Fg[3 * nc + q] = (TA)b[8]; Fg[4 * nc + q] = (TA)b[4]; Fg[5 * nc + q] = (TA)b[7];
+          Fg[6 * nc + q] = (TA)b[2]; Fg[7 * nc + q] = (TA)b[5];
+          Fg[8 * nc + q] = (TA)X.x5; Fg[9 * nc + q] = (TA)X.x3; Fg[10 * nc + q] = (TA)X.x7;
+          Fg[11 * nc + q] = (TA)X.x1; Fg[12 * nc + q] = (TA)X.x6; Fg[13 * nc + q] = (TA)X.xd;
+          Fg[14 * nc + q] = (TA)X.y6; Fg[15 * nc + q] = (TA)X.y1; Fg[16 * nc + q] = (TA)X.y3;
+          Fg[17 * nc + q] = (TA)X.yd;
+        } else {
+          // block-skewed AoS: block of this wave, its local step and slot
+          const int i0 = w * c.lpw, k = c.line - i0, Lb = min(c.lpw, c.nx - i0);
+          const int j = t - 2 * c.line, tb = j + 2 * k;
+          const size_t qb = (size_t)i0 * c.ny + blk_soff(tb, Lb, c.ny) + k - glo_of(tb, c.ny);
+          YG_GLOBAL TA* fl = Fg + 8 * qb;
+          fl[0] = (TA)a[2]; fl[1] = (TA)a[4]; fl[2] = (TA)a[8]; fl[3] = (TA)b[8];
+          fl[4] = (TA)b[4]; fl[5] = (TA)b[7]; fl[6] = (TA)b[2]; fl[7] = (TA)b[5];
+          YG_GLOBAL TA* fu = Fg + 8 * nc + 12 * qb;
+          fu[0] = (TA)X.x5; fu[1] = (TA)X.x3; fu[2] = (TA)X.x7; fu[3] = (TA)X.x1;
+          fu[4] = (TA)X.x6; fu[5] = (TA)X.xd; fu[6] = (TA)X.y6; fu[7] = (TA)X.y1;
+          fu[8] = (TA)X.y3; fu[9] = (TA)X.yd; fu[10] = (TA)0; fu[11] = (TA)0;
+        }
+      }
+      o2 = o1;
+      o1 = X;
+      if (c.coupled) {
+        if (ring_out) {
+          double* rr = yg_ring[w][t % 3];
+          rr[0] = X.x5; rr[1] = X.x3; rr[2] = X.x7; rr[3] = X.x1; rr[4] = X.x6; rr[5] = X.xd;
+          rr[6] = X.y6; rr[7] = X.y1; rr[8] = X.y3; rr[9] = X.yd;
+        }
+        step_barrier();
+      }
+    }
+    // the factorization runs once per linear solve: the next chunk is loaded
+    // after the steps, which keeps its 19 values per cell out of the registers
+    // that the recurrence needs
+    __syncthreads();
+    if (ch + 1 < nch) stage(((ch + 1) & 1) * NV * stride, t1);
+    __syncthreads();
+  }
+}
+
+// ---- triangular sweeps ---------------------------------------------------
+// Forward: y = L^{-1} rhs; backward: z = U^{-1} y.  T is the arithmetic type
+// of the sweep (the factor type).  The steps are walked in chunks of C
+// steps.  Per chunk: every thread loads one cell's factor slots and right-
+// hand side into registers for the next chunk while the lanes run the
+// recurrence of the current chunk entirely in LDS (inputs staged, outputs
+// collected); the chunk's outputs then leave in one coalesced store.  Inside
+// a chunk each lane reads the next step's slots from LDS before it runs the
+// current step, so the LDS latency overlaps the recurrence.  No global
+// memory operation sits on the per-step critical path.
+
+template <typename T, int NV>
+YG_DEV int sweep_cap_cells() {
+  const int c = (int)(YG_DYN_LDS / sizeof(T)) / (2 * NV + 2);
+  return c < 2 * (int)blockDim.x ? c : 2 * (int)blockDim.x;
+}
+
+template <typename T>
+YG_NOINLINE void sweep_L(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
+                    T* __restrict__ y) {
+  PhaseTimer ptm(c, 1);
+  const YG_GLOBAL T* __restrict__ Fg = gc(F);
+  const YG_GLOBAL T* __restrict__ rg = gc(rhs);
+  YG_GLOBAL T* __restrict__ yg = gm(y);
+  constexpr int NV = 10;  // L slots of both rows + the right-hand side
+  const int g = threadIdx.x, lane = g & 63, w = g >> 6;
+  const bool ring_in = c.coupled && w > 0 && lane == 0;
+  const bool ring_out = c.coupled && w + 1 < c.nwl && lane == 63;
+  const size_t nc = c.ncell;
+  const int capc = sweep_cap_cells<T, NV>();
+  const int C = max(1, capc / c.maxcnt);
+  const int stride = C * c.maxcnt;
+  const int ob = 2 * stride * NV;   // output slots follow the two stage buffers
+  T reg[2][NV];
+  int lreg[2];
+  auto load = [&](int t0) {   // this thread's cells of the chunk at step t0 -> registers
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      int q, nat, l;
+      lreg[r] = -1;
+      if (chunk_cell(c, t0, min(c.T, t0 + C), g + r * (int)blockDim.x, q, nat, l)) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) reg[r][v] = Fg[(size_t)v * nc + q];
+        reg[r][8] = rg[2 * nat];
+        reg[r][9] = rg[2 * nat + 1];
+        lreg[r] = l;
+      }
+    }
+  };
+  auto stash = [&](int sb) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (lreg[r] >= 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lds<T>(sb + v * stride + lreg[r]) = reg[r][v];
+      }
+    }
+  };
+  auto scatter = [&](YG_GLOBAL T* out, int t0, int t1) {   // staged outputs -> natural order
+    for (int r = 0; r < 2; ++r) {
+      int q, nat, l;
+      if (chunk_cell(c, t0, t1, g + r * (int)blockDim.x, q, nat, l)) {
+        out[2 * nat] = lds<T>(ob + 2 * l);
+        out[2 * nat + 1] = lds<T>(ob + 2 * l + 1);
+      }
+    }
+  };
+  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+  load(0);
+  stash(0);
+  __syncthreads();
+  const int nch = (c.T + C - 1) / C;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int t0 = ch * C, t1 = min(c.T, t0 + C);
+    if (ch + 1 < nch) load(t1);
+    const int sb = (ch & 1) * NV * stride;
+    T bc[NV];
+    int lc = step_slot(c, t0, t0);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) bc[v] = lc >= 0 ? lds<T>(sb + v * stride + lc) : (T)0;
+    for (int t = t0; t < t1; ++t) {
+      const int ln = t + 1 < t1 ? step_slot(c, t + 1, t0) : -1;
+      T bn[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) bn[v] = ln >= 0 ? lds<T>(sb + v * stride + ln) : (T)0;
+      T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
+      if (ring_in) {
+        n1x = (T)yg_ring[w - 1][(t + 2) % 3][0];
+        n2x = (T)yg_ring[w - 1][(t + 1) % 3][0];
+        n2y = (T)yg_ring[w - 1][(t + 1) % 3][1];
+      }
+      T yx = 0, yy = 0;
+      if (lc >= 0) {
+        yx = bc[8] - bc[0] * n2x - bc[1] * o1x - bc[2] * o1y;
+        yy = bc[9] - bc[3] * n2x - bc[4] * n2y - bc[5] * n1x - bc[6] * o1y - bc[7] * yx;
+        lds<T>(ob + 2 * lc) = yx;
+        lds<T>(ob + 2 * lc + 1) = yy;
+      }
+      o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
+      if (c.coupled) {
+        if (ring_out) {
+          yg_ring[w][t % 3][0] = (double)yx;
+          yg_ring[w][t % 3][1] = (double)yy;
+        }
+        step_barrier();
+      }
+      lc = ln;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) bc[v] = bn[v];
+    }
+    __syncthreads();
+    if (ch + 1 < nch) stash(((ch + 1) & 1) * NV * stride);
+    scatter(yg, t0, t1);
+    __syncthreads();
+  }
+}
+
+template <typename T>
+YG_NOINLINE void sweep_U(const Ctx& c, const T* __restrict__ F, const T* __restrict__ y,
+                    T* __restrict__ z) {
+  PhaseTimer ptm(c, 2);
+  const YG_GLOBAL T* __restrict__ Fg = gc(F);
+  const YG_GLOBAL T* __restrict__ yg = gc(y);
+  YG_GLOBAL T* __restrict__ zg = gm(z);
+  constexpr int NV = 12;  // U slots and 1/diag of both rows + y
+  const int g = threadIdx.x, lane = g & 63, w = g >> 6;
+  const bool ring_in = c.coupled && w + 1 < c.nwl && lane == 63;
+  const bool ring_out = c.coupled && w > 0 && lane == 0;
+  const size_t nc = c.ncell;
+  const int capc = sweep_cap_cells<T, NV>();
+  const int C = max(1, capc / c.maxcnt);
+  const int stride = C * c.maxcnt;
+  const int ob = 2 * stride * NV;   // output slots follow the two stage buffers
+  T reg[2][NV];
+  int lreg[2];
+  const int nch = (c.T + C - 1) / C;
+  auto load = [&](int ch) {   // this thread's cells of chunk ch -> registers
+    const int t0 = ch * C;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      int q, nat, l;
+      lreg[r] = -1;
+      if (chunk_cell(c, t0, min(c.T, t0 + C), g + r * (int)blockDim.x, q, nat, l)) {
+#pragma unroll
+        for (int v = 0; v < 10; ++v) reg[r][v] = Fg[(size_t)(8 + v) * nc + q];
+        reg[r][10] = yg[2 * nat];
+        reg[r][11] = yg[2 * nat + 1];
+        lreg[r] = l;
+      }
+    }
+  };
+  auto stash = [&](int sb) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (lreg[r] >= 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lds<T>(sb + v * stride + lreg[r]) = reg[r][v];
+      }
+    }
+  };
+  auto scatter = [&](YG_GLOBAL T* out, int t0, int t1) {   // staged outputs -> natural order
+    for (int r = 0; r < 2; ++r) {
+      int q, nat, l;
+      if (chunk_cell(c, t0, t1, g + r * (int)blockDim.x, q, nat, l)) {
+        out[2 * nat] = lds<T>(ob + 2 * l);
+        out[2 * nat + 1] = lds<T>(ob + 2 * l + 1);
+      }
+    }
+  };
+  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;  // own cells j+1 (step u-1), j+2 (u-2)
+  load(nch - 1);
+  stash(0);
+  __syncthreads();
+  for (int k = 0; k < nch; ++k) {
+    const int ch = nch - 1 - k;
+    const int t0 = ch * C, t1 = min(c.T, t0 + C);
+    if (ch > 0) load(ch - 1);
+    const int sb = (k & 1) * NV * stride;
+    T bc[NV];
+    int lc = step_slot(c, t1 - 1, t0);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) bc[v] = lc >= 0 ? lds<T>(sb + v * stride + lc) : (T)0;
+    for (int t = t1 - 1; t >= t0; --t) {
+      const int u = c.T - 1 - t;
+      const int ln = t - 1 >= t0 ? step_slot(c, t - 1, t0) : -1;
+      T bn[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) bn[v] = ln >= 0 ? lds<T>(sb + v * stride + ln) : (T)0;
+      // line i+1: cell j-1 (its step u-1) and cell j (its step u-2)
+      T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
+      if (ring_in) {
+        n1y = (T)yg_ring[w + 1][(u + 2) % 3][1];
+        n2x = (T)yg_ring[w + 1][(u + 1) % 3][0];
+        n2y = (T)yg_ring[w + 1][(u + 1) % 3][1];
+      }
+      T zx = 0, zy = 0;
+      if (lc >= 0) {
+        zy = (bc[11] - bc[6] * o1x - bc[7] * o1y - bc[8] * n2y) * bc[9];
+        zx = (bc[10] - bc[0] * zy - bc[1] * o1x - bc[2] * n1y - bc[3] * n2x - bc[4] * n2y) *
+             bc[5];
+        lds<T>(ob + 2 * lc) = zx;
+        lds<T>(ob + 2 * lc + 1) = zy;
+      }
+      o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
+      if (c.coupled) {
+        if (ring_out) {
+          yg_ring[w][u % 3][0] = (double)zx;
+          yg_ring[w][u % 3][1] = (double)zy;
+        }
+        step_barrier();
+      }
+      lc = ln;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) bc[v] = bn[v];
+    }
+    __syncthreads();
+    if (ch > 0) stash(((k + 1) & 1) * NV * stride);
+    scatter(zg, t0, t1);
+    __syncthreads();
+  }
+}
+
+// ---- streamed triangular sweeps (line blocks of at most 64 lines) ---------
+// Each wave sweeps its own line block over the block's own wavefront
+// (T_b = ny + 2(L-1) steps) and never waits for another wave.  At each
+// step a lane loads its cell's factor values as 16-byte chunks of the
+// block-skewed AoS layout (one step of a block is one contiguous range, so
+// the loads of a wave are coalesced) into a ring of D register slots filled
+// D steps ahead of use, so the memory latency overlaps D steps of the
+// recurrence.  The forward sweep writes y in block-skewed order (read back
+// coalesced by the backward sweep); the right-hand side and the result are
+// read and written in the natural row order of the Krylov vectors.
+
+// ---- bounds-checked buffer access ------------------------------------------
+// The streamed sweeps read and write through buffer resources (base, byte
+// range): a lane without a cell at a step passes an offset past the range,
+// which the hardware turns into a zero load or a dropped store.  Every step
+// then issues the same memory instructions on every lane with no branch
+// around them, so the compiler counts the loads in flight exactly and the
+// loads issued D steps ahead are waited for only when their step comes.
+#define YG_RSRC_WORD3 0x00020000   // raw buffer, 32-bit data format (gfx9)
+#define YG_OOB 0x80000000u
+
+// A buffer resource must sit in scalar registers: the base address and the
+// range are made wave-uniform first (pointer arguments of a called device
+// function arrive in vector registers).
+template <typename T>
+YG_DEV __amdgpu_buffer_rsrc_t yg_rsrc(const T* base, size_t count) {
+  const unsigned long long a = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  void* u = (void*)(((unsigned long long)hi << 32) | lo);
+  const int bytes = __builtin_amdgcn_readfirstlane((int)(count * sizeof(T)));
+  return __builtin_amdgcn_make_buffer_rsrc(u, (short)0, bytes, YG_RSRC_WORD3);
+}
+
+template <typename T> struct Pair16;   // T pair (8 or 16 bytes) through buffer loads
+template <> struct Pair16<float> {
+  static YG_DEV float2 load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+  static YG_DEV void store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, off, 0, 0);
+  }
+};
+template <> struct Pair16<double> {
+  static YG_DEV double2 load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
+  static YG_DEV void store(double2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, 0, 0);
+  }
+};
+
+// N values of T from consecutive 16-byte chunks at byte offset off
+template <int N>
+YG_DEV void load_vals(__amdgpu_buffer_rsrc_t r, unsigned off, float (&v)[N]) {
+#pragma unroll
+  for (int e = 0; e < N / 4; ++e) {
+    const float4 x = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * e, 0, 0));
+    v[4 * e] = x.x; v[4 * e + 1] = x.y; v[4 * e + 2] = x.z; v[4 * e + 3] = x.w;
+  }
+}
+template <int N>
+YG_DEV void load_vals(__amdgpu_buffer_rsrc_t r, unsigned off, double (&v)[N]) {
+#pragma unroll
+  for (int e = 0; e < N / 2; ++e) {
+    const double2 x = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * e, 0, 0));
+    v[2 * e] = x.x; v[2 * e + 1] = x.y;
+  }
+}
+
+// Geometry of one wave's line block in the streamed sweeps, all scalars.
+// Per step t the lanes with cells are glo(t) <= lane <= min(L-1, t/2); a
+// lane's slot in the block-skewed order is blk_soff(t) - glo(t) + lane.
+struct BlockGeom {
+  int L, Tb, ny;
+};
+struct StepQ {
+  int lo, hi, off;   // lanes lo..hi have cells; slot of a lane = off + lane
+};
+YG_DEV StepQ step_q(const BlockGeom& G, int t) {
+  StepQ r;
+  r.lo = glo_of(t, G.ny);
+  r.hi = (t >= 0 && t < G.Tb) ? min(G.L - 1, t >> 1) : -1;
+  r.off = blk_soff(t, G.L, G.ny) - r.lo;
+  return r;
+}
+// byte offset of a lane's cell at a step, or out of range if it has none
+YG_DEV unsigned cell_off(const StepQ& q, int lane, int lane_base, unsigned bytes) {
+  const unsigned o = (unsigned)(lane_base + q.off) * bytes;
+  return (lane >= q.lo && lane <= q.hi) ? o : YG_OOB;
+}
+
+YG_DEV int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <typename T>
+YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
+                             T* __restrict__ ys) {
+  PhaseTimer ptm(c, 1);
+  constexpr int D = sizeof(T) == 4 ? 8 : 4;   // register slots (steps in flight)
+  constexpr unsigned SF = 8 * sizeof(T), SV = 2 * sizeof(T);   // bytes per cell
+  using V2 = typename Vec2<T>::type;
+  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
+  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  const int nc = uni(c.ncell);
+  if (w >= nwl) return;
+  const int i0 = w * lpw, L = min(lpw, nx - i0);
+  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F, (size_t)8 * nc);
+  const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
+  const int lane_q = i0 * ny + lane;               // + off(t): block-skewed slot
+  const int lane_n = (i0 + lane) * ny - 2 * lane;  // + t: natural cell
+  T f[D][8];
+  V2 r[D];
+  // the natural cell of a lane at step t is lane_n + t, the same for every
+  // lane with a cell at t: its offset uses the step table's lane range too
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    const StepQ q = step_q(G, s);
+    load_vals<8>(rF, cell_off(q, lane, lane_q, SF), f[s]);
+    StepQ qn = q;
+    qn.off = s;
+    r[s] = Pair16<T>::load(rR, cell_off(qn, lane, lane_n, SV));
+  }
+  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+  for (int t0 = 0; t0 < G.Tb; t0 += D) {
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      const int t = t0 + s;
+      const StepQ q = step_q(G, t);
+      const bool h = lane >= q.lo && lane <= q.hi;
+      const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
+      T yx = r[s].x - f[s][0] * n2x - f[s][1] * o1x - f[s][2] * o1y;
+      T yy = r[s].y - f[s][3] * n2x - f[s][4] * n2y - f[s][5] * n1x - f[s][6] * o1y -
+             f[s][7] * yx;
+      yx = h ? yx : (T)0;
+      yy = h ? yy : (T)0;
+      V2 o;
+      o.x = yx;
+      o.y = yy;
+      Pair16<T>::store(o, rY, cell_off(q, lane, lane_q, SV));
+      o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
+      const int tn = t + D;
+      const StepQ qn = step_q(G, tn);
+      load_vals<8>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+      StepQ qr = qn;
+      qr.off = tn;
+      r[s] = Pair16<T>::load(rR, cell_off(qr, lane, lane_n, SV));
+    }
+  }
+}
+
+template <typename T>
+YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ ys,
+                             T* __restrict__ z) {
+  PhaseTimer ptm(c, 2);
+  constexpr int D = sizeof(T) == 4 ? 8 : 4;
+  constexpr unsigned SF = 12 * sizeof(T), SV = 2 * sizeof(T);
+  using V2 = typename Vec2<T>::type;
+  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
+  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  const int nc = uni(c.ncell);
+  if (w >= nwl) return;
+  const int i0 = w * lpw, L = min(lpw, nx - i0);
+  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + (size_t)8 * nc, (size_t)12 * nc);
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
+  const int lane_q = i0 * ny + lane;
+  const int lane_n = (i0 + lane) * ny - 2 * lane;
+  const int Tl = G.Tb - 1;   // backward position u is step Tl - u
+  T f[D][12];
+  V2 yv[D];
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    const StepQ q = step_q(G, Tl - s);
+    load_vals<12>(rF, cell_off(q, lane, lane_q, SF), f[s]);
+    yv[s] = Pair16<T>::load(rY, cell_off(q, lane, lane_q, SV));
+  }
+  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;  // own cells j+1, j+2
+  for (int u0 = 0; u0 < G.Tb; u0 += D) {
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      const int t = Tl - (u0 + s);
+      const StepQ q = step_q(G, t);
+      const bool h = lane >= q.lo && lane <= q.hi;
+      // line i+1: cell j-1 (one step back) and cell j (two steps back)
+      const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
+      T zy = (yv[s].y - f[s][6] * o1x - f[s][7] * o1y - f[s][8] * n2y) * f[s][9];
+      T zx = (yv[s].x - f[s][0] * zy - f[s][1] * o1x - f[s][2] * n1y - f[s][3] * n2x -
+              f[s][4] * n2y) * f[s][5];
+      zx = h ? zx : (T)0;
+      zy = h ? zy : (T)0;
+      V2 o;
+      o.x = zx;
+      o.y = zy;
+      StepQ qz = q;
+      qz.off = t;
+      Pair16<T>::store(o, rZ, cell_off(qz, lane, lane_n, SV));
+      o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
+      const StepQ qn = step_q(G, t - D);
+      load_vals<12>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+      yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
+    }
+  }
+}
+
+// ---- preconditioner --------------------------------------------------------
+template <typename T>
+struct Precon {
+  int kind;        // 0 none, 1 jacobi, 2 ilu
+  const T* F;      // ILU factors, or 1/diag pairs for jacobi
+};
+
+// out = M^-1 in; tmp is scratch of the same size
+template <typename T>
+YG_DEV void precon_apply(const Ctx& c, const Precon<T>& P, const T* in, T* tmp, T* out) {
+  if (P.kind == 2 && c.coupled) {
+    sweep_L<T>(c, P.F, in, tmp);
+    sweep_U<T>(c, P.F, tmp, out);
+  } else if (P.kind == 2) {
+    sweep_L_str<T>(c, P.F, in, tmp);
+    __syncthreads();
+    sweep_U_str<T>(c, P.F, tmp, out);
+    __syncthreads();
+  } else {
+    for (int q = threadIdx.x; q < 2 * c.ncell; q += blockDim.x)
+      out[q] = P.kind == 1 ? in[q] * P.F[q] : in[q];
+    __syncthreads();
+  }
+}
+
+// ---- BiCGSTAB / CG ------------------------------------------------------
+struct SolveOut {
+  int iters, status;
+  double resid;
+};
+
+// y = A x for every cell; post(q) runs after cell q's result is stored (the
+// dots fused into the SpMV).  Each thread takes four cells per trip so that
+// the loads of all four are in flight together.
+template <typename TA, typename TV, typename F>
+YG_DEV void spmv_all(const Ctx& c, const TA* __restrict__ A, const TV* __restrict__ x,
+                     TV* __restrict__ y, F&& post) {
+  PhaseTimer ptm(c, 3);
+  using V2 = typename Vec2<TV>::type;
+  const int B = blockDim.x;
+  V2* __restrict__ Y = reinterpret_cast<V2*>(y);
+  for (int q0 = threadIdx.x; q0 < c.ncell; q0 += 4 * B) {
+    double ax[4], ay[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u * B;
+      ax[u] = ay[u] = 0.0;
+      if (q < c.ncell) spmv_cell(c, A, x, q, ax[u], ay[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u * B;
+      if (q < c.ncell) {
+        V2 o;
+        o.x = (TV)ax[u];
+        o.y = (TV)ay[u];
+        Y[q] = o;
+        post(q);
+      }
+    }
+  }
+}
+
+// A 16-byte chunk of a vector (4 floats or 2 doubles).
+template <typename T>
+struct alignas(16) C16 {
+  T v[16 / sizeof(T)];
+};
+
+// Element-wise pass over n values: out[o][k] = f(in[0][k], ..., in[NI-1][k]).
+// The vectors are 16-byte aligned; thread g takes chunks g, g + B, ...,
+// four per trip (two for passes of four or more inputs) with every load of
+// a trip issued before its arithmetic;
+// the tail (fewer values than one chunk) goes value by value.  Arithmetic
+// in double: f(const double (&xi)[NI], double (&xo)[NO]); dots accumulate
+// in f's captures.
+template <typename T, int NI, int NO, typename F>
+YG_DEV void vmap(int n, const T* const (&in)[NI], T* const (&out)[NO], F&& f) {
+  constexpr int E = 16 / sizeof(T), U = NI >= 4 ? 2 : 4;
+  const int nch = n / E, B = blockDim.x;
+  for (int k0 = threadIdx.x; k0 < nch; k0 += U * B) {
+    C16<T> ci[U][NI];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * B;
+      if (k < nch) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) ci[u][i] = reinterpret_cast<const C16<T>*>(in[i])[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * B;
+      if (k < nch) {
+        C16<T> co[NO > 0 ? NO : 1];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          double xi[NI], xo[NO > 0 ? NO : 1];
+#pragma unroll
+          for (int i = 0; i < NI; ++i) xi[i] = (double)ci[u][i].v[e];
+          f(xi, xo);
+#pragma unroll
+          for (int o = 0; o < NO; ++o) co[o].v[e] = (T)xo[o];
+        }
+#pragma unroll
+        for (int o = 0; o < NO; ++o) reinterpret_cast<C16<T>*>(out[o])[k] = co[o];
+      }
+    }
+  }
+  for (int q = nch * E + threadIdx.x; q < n; q += B) {
+    double xi[NI], xo[NO > 0 ? NO : 1];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) xi[i] = (double)in[i][q];
+    f(xi, xo);
+#pragma unroll
+    for (int o = 0; o < NO; ++o) out[o][q] = (T)xo[o];
+  }
+}
+
+template <typename TA, typename TV>
+YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x,
+                         TV* W, const YgKrylovOpts& o) {
+  const int n2 = 2 * c.ncell;
+  const size_t vl = c.vlen;   // vector stride in W (16-byte aligned)
+  TV* __restrict__ r = W;
+  TV* __restrict__ rt = W + 1 * vl;
+  TV* __restrict__ p = W + 2 * vl;
+  TV* __restrict__ v = W + 3 * vl;
+  TV* __restrict__ ph = W + 4 * vl;
+  TV* __restrict__ s = W + 5 * vl;
+  TV* __restrict__ sh = W + 6 * vl;
+  TV* __restrict__ tt = W + 7 * vl;
+  TV* __restrict__ ytmp = W + 8 * vl;
+  SolveOut out{0, 4, 0.0};
+  double red[3];
+  // r = b - A x, rt = r
+  red[0] = red[1] = red[2] = 0.0;
+  spmv_all(c, A, x, tt, [&](int q) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const double rq = (double)b[2 * q + e] - (double)tt[2 * q + e];
+      r[2 * q + e] = (TV)rq;
+      rt[2 * q + e] = (TV)rq;
+      red[0] += (double)r[2 * q + e] * (double)r[2 * q + e];
+      red[1] += (double)b[2 * q + e] * (double)b[2 * q + e];
+    }
+  });
+  yg_block_sum<3>(red, c.red);
+  const double nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : sqrt(red[0]);
+  if (nrm0 == 0.0) { out.status = 0; out.resid = 0.0; return out; }
+  const double bnrm = 1.0 / nrm0;
+  out.resid = sqrt(red[0]) * bnrm;
+  if (out.resid <= o.tol) { out.status = 0; return out; }
+  double rho = red[0], rho_old = 1.0, alpha = 1.0, omega = 1.0;  // (rt, r) = (r, r) at start
+  for (int it = 1; it <= o.maxiter; ++it) {
+    if (rho == 0.0) { out.status = 2; out.iters = it; return out; }
+    // p = r + beta (p - omega v); phat = M^-1 p
+    if (it == 1) {
+      vmap<TV, 1, 1>(n2, {r}, {p}, [&](const double (&xi)[1], double (&xo)[1]) { xo[0] = xi[0]; });
+    } else {
+      const double beta = (rho / rho_old) * (alpha / omega);
+      vmap<TV, 3, 1>(n2, {r, p, v}, {p}, [&](const double (&xi)[3], double (&xo)[1]) {
+        xo[0] = xi[0] + beta * (xi[1] - omega * xi[2]);
+      });
+    }
+    __syncthreads();
+    precon_apply<TV>(c, P, p, ytmp, ph);
+    // v = A phat, (rt, v)
+    red[0] = 0.0;
+    spmv_all(c, A, ph, v, [&](int q) {
+      red[0] += (double)rt[2 * q] * (double)v[2 * q] + (double)rt[2 * q + 1] * (double)v[2 * q + 1];
+    });
+    yg_block_sum<1>(*reinterpret_cast<double(*)[1]>(red), c.red);
+    if (red[0] == 0.0) { out.status = 2; out.iters = it; return out; }
+    alpha = rho / red[0];
+    // s = r - alpha v and ||s||: converged on s ends the iteration early
+    red[0] = 0.0;
+    vmap<TV, 2, 1>(n2, {r, v}, {s}, [&](const double (&xi)[2], double (&xo)[1]) {
+      const double sq = (double)(TV)(xi[0] - alpha * xi[1]);
+      xo[0] = sq;
+      red[0] += sq * sq;
+    });
+    yg_block_sum<1>(*reinterpret_cast<double(*)[1]>(red), c.red);
+    const double sn = sqrt(red[0]) * bnrm;
+    if (sn <= o.tol) {
+      vmap<TV, 2, 1>(n2, {x, ph}, {x}, [&](const double (&xi)[2], double (&xo)[1]) {
+        xo[0] = xi[0] + alpha * xi[1];
+      });
+      __syncthreads();
+      out.resid = sn; out.iters = it; out.status = 0;
+      return out;
+    }
+    precon_apply<TV>(c, P, s, ytmp, sh);
+    // t = A shat, (t,t), (t,s)
+    red[0] = red[1] = 0.0;
+    spmv_all(c, A, sh, tt, [&](int q) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const double tq = (double)tt[2 * q + e];
+        red[0] += tq * tq;
+        red[1] += tq * (double)s[2 * q + e];
+      }
+    });
+    yg_block_sum<2>(*reinterpret_cast<double(*)[2]>(red), c.red);
+    if (red[0] == 0.0) { out.status = 2; out.iters = it; return out; }
+    omega = red[1] / red[0];
+    if (omega == 0.0) { out.status = 2; out.iters = it; return out; }
+    // x += alpha phat + omega shat; r = s - omega t; (r,r), (rt,r)
+    red[0] = red[1] = 0.0;
+    vmap<TV, 6, 2>(n2, {x, ph, sh, s, tt, rt}, {x, r},
+                   [&](const double (&xi)[6], double (&xo)[2]) {
+      xo[0] = xi[0] + alpha * xi[1] + omega * xi[2];
+      const double rq = (double)(TV)(xi[3] - omega * xi[4]);
+      xo[1] = rq;
+      red[0] += rq * rq;
+      red[1] += xi[5] * rq;
+    });
+    yg_block_sum<2>(*reinterpret_cast<double(*)[2]>(red), c.red);
+    out.resid = sqrt(red[0]) * bnrm;
+    out.iters = it;
+    if (out.resid <= o.tol) { out.status = 0; return out; }
+    rho_old = rho;
+    rho = red[1];
+  }
+  out.status = 4;
+  return out;
+}
+
+template <typename TA, typename TV>
+YG_DEV SolveOut cg(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x, TV* W,
+                   const YgKrylovOpts& o) {
+  const int n2 = 2 * c.ncell;
+  const size_t vl = c.vlen;
+  TV* r = W;
+  TV* z = W + 1 * vl;
+  TV* p = W + 2 * vl;
+  TV* qv = W + 3 * vl;
+  TV* ytmp = W + 8 * vl;
+  SolveOut out{0, 4, 0.0};
+  double red[2] = {0.0, 0.0};
+  spmv_all(c, A, x, qv, [&](int q) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      r[2 * q + e] = (TV)((double)b[2 * q + e] - (double)qv[2 * q + e]);
+      red[0] += (double)r[2 * q + e] * (double)r[2 * q + e];
+      red[1] += (double)b[2 * q + e] * (double)b[2 * q + e];
+    }
+  });
+  yg_block_sum<2>(red, c.red);
+  const double nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : sqrt(red[0]);
+  if (nrm0 == 0.0) { out.status = 0; return out; }
+  const double bnrm = 1.0 / nrm0;
+  out.resid = sqrt(red[0]) * bnrm;
+  if (out.resid <= o.tol) { out.status = 0; return out; }
+  double rho_old = 0.0;
+  for (int it = 1; it <= o.maxiter; ++it) {
+    precon_apply<TV>(c, P, r, ytmp, z);
+    double rr[1] = {0.0};
+    for (int q = threadIdx.x; q < n2; q += blockDim.x) rr[0] += (double)r[q] * (double)z[q];
+    yg_block_sum<1>(rr, c.red);
+    const double rho = rr[0];
+    if (rho == 0.0) { out.status = 2; out.iters = it; return out; }
+    const double beta = it == 1 ? 0.0 : rho / rho_old;
+    for (int q = threadIdx.x; q < n2; q += blockDim.x)
+      p[q] = it == 1 ? z[q] : (TV)((double)z[q] + beta * (double)p[q]);
+    __syncthreads();
+    double pq[1] = {0.0};
+    spmv_all(c, A, p, qv, [&](int q) {
+      pq[0] += (double)p[2 * q] * (double)qv[2 * q] + (double)p[2 * q + 1] * (double)qv[2 * q + 1];
+    });
+    yg_block_sum<1>(pq, c.red);
+    if (pq[0] == 0.0) { out.status = 2; out.iters = it; return out; }
+    const double alpha = rho / pq[0];
+    double rn[1] = {0.0};
+    for (int q = threadIdx.x; q < n2; q += blockDim.x) {
+      x[q] = (TV)((double)x[q] + alpha * (double)p[q]);
+      r[q] = (TV)((double)r[q] - alpha * (double)qv[q]);
+      rn[0] += (double)r[q] * (double)r[q];
+    }
+    yg_block_sum<1>(rn, c.red);
+    out.resid = sqrt(rn[0]) * bnrm;
+    out.iters = it;
+    if (out.resid <= o.tol) { out.status = 0; return out; }
+    rho_old = rho;
+  }
+  out.status = 4;
+  return out;
+}
+
+template <typename T>
+YG_DEV void build_precon(const Ctx& c, const T* A, T* F, Precon<T>& P, int kind, int* bad) {
+  P.kind = kind;
+  P.F = F;
+  if (kind == 2) {
+    ilu0_factor<T>(c, A, F, bad);
+  } else if (kind == 1) {
+    for (int q = threadIdx.x; q < c.ncell; q += blockDim.x) {
+      F[2 * q] = (T)(1.0 / (double)A[((size_t)0 * c.ncell + q) * 2 + 0]);
+      F[2 * q + 1] = (T)(1.0 / (double)A[((size_t)0 * c.ncell + q) * 2 + 1]);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+YG_DEV SolveOut krylov(const Ctx& c, const T* A, const Precon<T>& P, const T* b, T* x, T* W,
+                       const YgKrylovOpts& o) {
+  return o.solver == 1 ? cg<T, T>(c, A, P, b, x, W, o)
+                       : bicgstab<T, T>(c, A, P, b, x, W, o);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The kernel: one block per member.  Modes:
+//   YG_MODE_FP64  : FP64 matrix, factors, vectors (the plain FP64 path)
+//   YG_MODE_MIXED : FP64 residual b - A x and correction accumulation,
+//                   FP32 matrix copy, factors and inner Krylov solve
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
+  const int m = blockIdx.x;
+  if (a.skip && a.skip[m]) return;
+  const YgMember mb = a.members[m];
+  __shared__ double s_red[16 * 4];
+  __shared__ int s_bad;
+  const int ncell = mb.nx * mb.ny;
+  for (int k = threadIdx.x; k <= mb.T; k += blockDim.x) yg_soff[k] = a.soff[mb.toff + k];
+  if (threadIdx.x == 0) s_bad = 0;
+  __syncthreads();
+  Ctx c;
+  c.mb = &a.members[m];
+  c.nx = mb.nx; c.ny = mb.ny; c.T = mb.T; c.ncell = ncell; c.vlen = mb.vlen;
+  c.wrapx = a.wrapx; c.wrapy = a.wrapy;
+  c.mask = a.mask + mb.coff;
+  c.red = s_red;
+  c.maxcnt = mb.maxcnt;
+  {
+    // line-to-wave mapping of the recurrences (exact or block ILU(0))
+    const int g = threadIdx.x, lane = g & 63, w = g >> 6;
+    const int wmax = (int)(blockDim.x >> 6);
+    int nb = a.opts.ilu_blocks;
+    if (nb < 0) nb = MODE == YG_MODE_MIXED ? (mb.nx + 63) / 64 : 0;
+    if (nb == 0) {
+      c.coupled = mb.nx > 64;
+      c.nwl = (mb.nx + 63) / 64;
+      c.lpw = 64;
+      c.line = g < mb.nx ? g : -1;
+    } else {
+      nb = max(nb, (mb.nx + 63) / 64);
+      nb = min(nb, min(wmax, mb.nx));
+      const int lpw = (mb.nx + nb - 1) / nb;
+      c.coupled = 0;
+      c.lpw = lpw;
+      c.nwl = (mb.nx + lpw - 1) / lpw;
+      const int i = w * lpw + lane;
+      c.line = (w < c.nwl && lane < lpw && i < mb.nx) ? i : -1;
+    }
+  }
+  __shared__ long long s_tm[8];
+  if (threadIdx.x < 8) s_tm[threadIdx.x] = 0;
+  c.tm = a.timers ? s_tm : nullptr;
+  const long long t_kernel0 = wall_clock64();
+  const size_t c2 = 2 * (size_t)mb.coff;
+  const double* b64 = a.b + c2;
+  double* x64 = a.x + c2;
+  // b == 0 shortcut (solver_linear.F90:162-174)
+  {
+    double bm[1] = {0.0};
+    for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) bm[0] += b64[q] != 0.0 ? 1.0 : 0.0;
+    yg_block_sum<1>(bm, s_red);
+    if (bm[0] == 0.0) {
+      for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] = 0.0;
+      if (threadIdx.x == 0) { a.out_iters[m] = 0; a.out_status[m] = 0; a.out_resid[m] = 0.0; }
+      return;
+    }
+  }
+  YgKrylovOpts o = a.opts;
+  if (MODE == YG_MODE_FP64) {
+    const double* A = a.A64 + (size_t)18 * mb.coff;
+    double* F = a.F64 + (size_t)YG_NF * mb.coff;
+    double* W = a.W64 + (size_t)YG_NW64 * mb.woff;
+    if (o.initx_zeros) {
+      for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] = 0.0;
+      __syncthreads();
+    }
+    Precon<double> P;
+    build_precon<double>(c, A, F, P, o.precon, &s_bad);
+    SolveOut so = krylov<double>(c, A, P, b64, x64, W, o);
+    if (s_bad && so.status == 0) so.status = 2;
+    if (threadIdx.x == 0) { a.out_iters[m] = so.iters; a.out_status[m] = so.status; a.out_resid[m] = so.resid; }
+    if (a.timers && threadIdx.x == 0) {
+      s_tm[7] = wall_clock64() - t_kernel0;
+      for (int k = 0; k < 8; ++k) a.timers[8 * m + k] += s_tm[k];
+    }
+  } else {
+    // FP64 iterative refinement around an FP32 solve
+    const double* A64 = a.A64 + (size_t)18 * mb.coff;
+    const float* A32 = a.A32 + (size_t)18 * mb.coff;
+    float* F = a.F32 + (size_t)YG_NF * mb.coff;
+    float* W = a.W32 + (size_t)YG_NW32 * mb.woff;  // 9 inner Krylov vectors ...
+    float* d = W + (size_t)9 * mb.vlen;             // ... the correction ...
+    float* rin = W + (size_t)10 * mb.vlen;          // ... and the scaled residual
+    double* r64 = a.W64 + (size_t)YG_NW64 * mb.woff; // FP64 residual
+    if (o.initx_zeros) {
+      for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] = 0.0;
+      __syncthreads();
+    }
+    Precon<float> P;
+    build_precon<float>(c, A32, F, P, o.precon, &s_bad);
+    YgKrylovOpts oi = o;
+    oi.tol = o.inner_tol;
+    oi.conv_cond = 0;
+    oi.initx_zeros = 1;
+    double nrm0 = -1.0, res = 1.0;
+    int iters = 0, status = 4;
+    for (int ir = 0; ir < o.ir_max; ++ir) {
+      double red[2] = {0.0, 0.0};
+      for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
+        double ax, ay;
+        spmv_cell(c, A64, x64, q, ax, ay);
+        const double rx = b64[2 * q] - ax, ry = b64[2 * q + 1] - ay;
+        r64[2 * q] = rx; r64[2 * q + 1] = ry;
+        red[0] += rx * rx + ry * ry;
+        red[1] += b64[2 * q] * b64[2 * q] + b64[2 * q + 1] * b64[2 * q + 1];
+      }
+      yg_block_sum<2>(red, s_red);
+      const double rn = sqrt(red[0]);
+      if (nrm0 < 0.0) nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : rn;
+      if (nrm0 == 0.0) { res = 0.0; status = 0; break; }
+      res = rn / nrm0;
+      if (res <= o.tol) { status = 0; break; }
+      if (iters >= o.maxiter) { status = 4; break; }
+      const double sc = rn > 0.0 ? 1.0 / rn : 1.0;
+      for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) {
+        rin[q] = (float)(r64[q] * sc);
+        d[q] = 0.0f;
+      }
+      __syncthreads();
+      oi.maxiter = o.maxiter - iters;
+      SolveOut so = krylov<float>(c, A32, P, rin, d, W, oi);
+      iters += so.iters;
+      for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] += rn * (double)d[q];
+      __syncthreads();
+      if (so.status == 2 && so.iters <= 1) { status = 2; break; }
+    }
+    if (s_bad && status == 0) status = 2;
+    if (threadIdx.x == 0) { a.out_iters[m] = iters; a.out_status[m] = status; a.out_resid[m] = res; }
+    if (a.timers && threadIdx.x == 0) {
+      s_tm[7] = wall_clock64() - t_kernel0;
+      for (int k = 0; k < 8; ++k) a.timers[8 * m + k] += s_tm[k];
+    }
+  }
+}
+
+// Launch of this translation unit's block size (set attributes once).
+template <int NT>
+void yg_launch_krylov_nt(const YgKrylovArgs& a, int mode, int nmembers, int block,
+                         hipStream_t st) {
+  static bool attr_set[2] = {false, false};
+  if (mode == YG_MODE_FP64) {
+    if (!attr_set[0]) {
+      (void)hipFuncSetAttribute((const void*)yg_krylov_kernel<YG_MODE_FP64, NT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
+      attr_set[0] = true;
+    }
+    hipLaunchKernelGGL((yg_krylov_kernel<YG_MODE_FP64, NT>), dim3(nmembers), dim3(block),
+                       YG_DYN_LDS, st, a);
+  } else {
+    if (!attr_set[1]) {
+      (void)hipFuncSetAttribute((const void*)yg_krylov_kernel<YG_MODE_MIXED, NT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
+      attr_set[1] = true;
+    }
+    hipLaunchKernelGGL((yg_krylov_kernel<YG_MODE_MIXED, NT>), dim3(nmembers), dim3(block),
+                       YG_DYN_LDS, st, a);
+  }
+}
