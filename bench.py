@@ -253,7 +253,7 @@ def main():
         "picard_iters_mean": float(np.mean([i.ssa_iter_now for i in info])),
         "krylov_iters_per_solve": st["krylov_iters"] / max(args.members * args.steps, 1),
         "unknown_iters_per_s": st["unknown_iters"] / elapsed,
-        "roofline": {"bound": "hbm", "kernel": "yg_krylov_kernel<1>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "yg_krylov_kernel<1, 512>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,
                      "bytes_per_launch": bytes_per_launch, "launches": launches},

@@ -261,12 +261,12 @@ struct Batch {
     YG_CHECK(h, ar.alloc(&d_zoffs, nmem + 1));
     YG_CHECK(h, ar.alloc(&d_soff, soff.size()));
     YG_CHECK(h, ar.alloc(&d_mask, C));
-    YG_CHECK(h, ar.alloc(&d_A64, 18 * C));
+    YG_CHECK(h, ar.alloc(&d_A64, (size_t)YG_NA * C));
     YG_CHECK(h, ar.alloc(&d_b, 2 * C));
     YG_CHECK(h, ar.alloc(&d_x, 2 * C));
     YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * total_v));
     if (kmode == YG_MODE_MIXED) {
-      YG_CHECK(h, ar.alloc(&d_A32, 18 * C));
+      YG_CHECK(h, ar.alloc(&d_A32, (size_t)YG_NA * C));
       YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * C));
       YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * total_v));
     } else {
@@ -795,8 +795,8 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   int T;
   step_table(nx, ny, soff, T);
   if (T > YG_MAX_STEPS) { h->err = "grid too large for the step table"; return YGPU_ERR_UNSUPPORTED; }
-  // CSR -> stencil slots in skewed order
-  std::vector<double> A((size_t)18 * cells, 0.0), bs(n), xs(n);
+  // CSR -> stencil matrix records (one per cell, in the row order)
+  std::vector<double> A((size_t)YG_NA * cells, 0.0), bs(n), xs(n);
   std::vector<uint32_t> mask(cells, 0u);
   static const int SU[2][9][3] = {
       {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {1, 0, 1}, {1, -1, 1}, {0, -1, 1}},
@@ -817,7 +817,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
       for (int t = 0; t < 9; ++t)
         if (SU[comp][t][0] == di && SU[comp][t][1] == dj && SU[comp][t][2] == comp2) { s = t; break; }
       if (s < 0) { h->err = "entry outside the C-grid SSA stencil"; return YGPU_ERR_UNSUPPORTED; }
-      A[((size_t)s * cells + q) * 2 + comp] += a_val[k];
+      A[(size_t)YG_NA * q + 2 * s + comp] += a_val[k];
       mask[q] |= 1u << (16 * comp + s);
     }
     if (!((mask[q] >> (16 * comp)) & 1u) && o.k.precon != 0) {
@@ -1049,8 +1049,8 @@ int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny,
   if (!par) return -1;
   const int64_t C = (int64_t)nx * ny, Z = C * nz;
   const int64_t rb = par->precision != YGPU_PREC_REF ? 8 : 4;
-  int64_t b = C * (4 + 4 + 18 * 8 + 2 * 8 + 2 * 8 + YG_NW64 * 2 * 8 + 8);
-  if (par->precision == YGPU_PREC_MIXED) b += C * (18 * 4 + YG_NF * 4 + YG_NW32 * 2 * 4);
+  int64_t b = C * (4 + 4 + YG_NA * 8 + 2 * 8 + 2 * 8 + YG_NW64 * 2 * 8 + 8);
+  if (par->precision == YGPU_PREC_MIXED) b += C * (YG_NA * 4 + YG_NF * 4 + YG_NW32 * 2 * 4);
   else b += C * YG_NF * 8;
   b += rb * (C * 34 + Z * 3 + 2 * Z);
   return b;

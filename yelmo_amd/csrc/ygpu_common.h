@@ -26,6 +26,13 @@
 // Lateral-front and free-slip rows use subsets of these slots.
 #define YG_NSLOT 9
 
+// The stencil matrix on the device: one record of YG_NA values per cell,
+// value 2*s + comp = slot s of the cell's ux (comp 0) or uy (comp 1) row,
+// then two zero pad values.  Absent slots hold exact zeros.  One record is
+// five (FP32) or ten (FP64) 16-byte chunks, so a cell's row pair arrives in
+// a few wide loads from one address.
+#define YG_NA 20
+
 // (di, dj, comp) of each slot; comp 0 = ux, 1 = uy
 __constant__ static const int8_t kSlotUx[9][3] = {
     {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},

@@ -755,13 +755,14 @@ __global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const 
     bb[1] = f.taud_acy[c0 + c] - lim_r; xx[1] = uy[c];
   }
 #undef PUT
-  const size_t base = (size_t)18 * c0;
+  const size_t base = (size_t)YG_NA * gq;
 #pragma unroll
   for (int s = 0; s < 9; ++s) {
-    const size_t k = base + ((size_t)s * ncell + q) * 2;
-    if (A64) { A64[k] = a[0][s]; A64[k + 1] = a[1][s]; }
-    if (A32) { A32[k] = (float)a[0][s]; A32[k + 1] = (float)a[1][s]; }
+    if (A64) { A64[base + 2 * s] = a[0][s]; A64[base + 2 * s + 1] = a[1][s]; }
+    if (A32) { A32[base + 2 * s] = (float)a[0][s]; A32[base + 2 * s + 1] = (float)a[1][s]; }
   }
+  if (A64) { A64[base + 18] = 0.0; A64[base + 19] = 0.0; }
+  if (A32) { A32[base + 18] = 0.0f; A32[base + 19] = 0.0f; }
   bvec[2 * gq] = bb[0]; bvec[2 * gq + 1] = bb[1];
   x0[2 * gq] = xx[0]; x0[2 * gq + 1] = xx[1];
   maskv[gq] = pm[0] | (pm[1] << 16);

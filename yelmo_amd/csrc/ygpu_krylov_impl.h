@@ -45,7 +45,6 @@ struct Ctx {
   int vlen;                   // stride of the work vectors (values, multiple of 4)
   int wrapx, wrapy;
   const uint32_t* mask;       // per natural cell: presence bits of its two rows
-  double* red;                // LDS reduction scratch
   int maxcnt;                 // most cells in one wavefront step
   int line;                   // grid line of this thread in the recurrences, -1 if none
   int coupled;                // waves exchange boundary values every step
@@ -85,6 +84,7 @@ YG_DEV T& lds(int k) { return reinterpret_cast<T*>(yg_dyn_lds)[k]; }
 // reason.
 __shared__ int yg_soff[YG_MAX_STEPS + 1];
 __shared__ double yg_ring[16][3][10];
+__shared__ double yg_red[16 * 4];   // block reductions
 
 // Global-memory views of pointers handed to the (non-inlined) wavefront
 // phases: the compiler then emits global (not flat) accesses, whose
@@ -99,44 +99,6 @@ YG_DEV YG_GLOBAL T* gm(T* p) { return (YG_GLOBAL T*)p; }
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
-
-// y = A x for the two rows of natural cell n = i*ny + j (the reference's
-// unknown numbering: rows 2n, 2n+1).  Absent stencil slots hold exact zeros
-// (the assembly and the CSR conversion zero-fill them), so every slot is
-// multiplied through; a neighbour index past a non-periodic border is
-// clamped into the domain, where it meets a zero coefficient.  The 18
-// coefficients arrive as 9 (ux,uy) pairs and x as the pairs of the 7 cells
-// of the stencil.
-template <typename TA, typename TV>
-YG_DEV void spmv_cell(const Ctx& c, const TA* __restrict__ A, const TV* __restrict__ x, int n,
-                      double& yx, double& yy) {
-  using A2 = typename Vec2<TA>::type;
-  using V2 = typename Vec2<TV>::type;
-  const int ny = c.ny, i = n / ny, j = n - i * ny;
-  int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1;
-  if (c.wrapx) { ip = ip == c.nx ? 0 : ip; im = im < 0 ? c.nx - 1 : im; }
-  else { ip = min(ip, c.nx - 1); im = max(im, 0); }
-  if (c.wrapy) { jp = jp == ny ? 0 : jp; jm = jm < 0 ? ny - 1 : jm; }
-  else { jp = min(jp, ny - 1); jm = max(jm, 0); }
-  const V2* __restrict__ X = reinterpret_cast<const V2*>(x);
-  const A2* __restrict__ M = reinterpret_cast<const A2*>(A);
-  const size_t nc = c.ncell;
-  const V2 x0 = X[n], xip = X[ip * ny + j], xim = X[im * ny + j], xjp = X[i * ny + jp],
-           xjm = X[i * ny + jm], xipjm = X[ip * ny + jm], ximjp = X[im * ny + jp];
-  A2 a[9];
-#pragma unroll
-  for (int s = 0; s < 9; ++s) a[s] = M[s * nc + n];
-  // ux row: slots ux(i,j) ux(i+1,j) ux(i-1,j) ux(i,j+1) ux(i,j-1)
-  //              uy(i,j) uy(i+1,j) uy(i+1,j-1) uy(i,j-1)
-  yx = (double)a[0].x * x0.x + (double)a[1].x * xip.x + (double)a[2].x * xim.x +
-       (double)a[3].x * xjp.x + (double)a[4].x * xjm.x + (double)a[5].x * x0.y +
-       (double)a[6].x * xip.y + (double)a[7].x * xipjm.y + (double)a[8].x * xjm.y;
-  // uy row: slots uy(i,j) uy(i,j+1) uy(i,j-1) uy(i+1,j) uy(i-1,j)
-  //              ux(i,j) ux(i,j+1) ux(i-1,j+1) ux(i-1,j)
-  yy = (double)a[0].y * x0.y + (double)a[1].y * xjp.y + (double)a[2].y * xjm.y +
-       (double)a[3].y * xip.y + (double)a[4].y * xim.y + (double)a[5].y * x0.x +
-       (double)a[6].y * xjp.x + (double)a[7].y * ximjp.x + (double)a[8].y * xim.x;
-}
 
 // first line of step t that holds a cell: max(0, ceil((t - ny + 1) / 2))
 YG_DEV int glo_of(int t, int ny) { const int a = t - ny + 2; return a > 0 ? a >> 1 : 0; }
@@ -267,7 +229,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
       if (chunk_cell(c, t0, min(c.T, t0 + C), g + r * (int)blockDim.x, q, nat, l)) {
 #pragma unroll
         for (int v = 0; v < 18; ++v)
-          lds<TA>(sb + v * stride + l) = Ag[((size_t)(v >> 1) * nc + nat) * 2 + (v & 1)];
+          lds<TA>(sb + v * stride + l) = Ag[(size_t)nat * YG_NA + v];
         lds<TA>(sb + 18 * stride + l) = mask_to_stage<TA>(gc(c.mask)[nat]);
       }
     }
@@ -696,18 +658,46 @@ struct BlockGeom {
   int L, Tb, ny;
 };
 struct StepQ {
-  int lo, hi, off;   // lanes lo..hi have cells; slot of a lane = off + lane
+  int t, lo, hi, off;   // lanes lo..hi have cells at step t; slot of a lane = off + lane
 };
 YG_DEV StepQ step_q(const BlockGeom& G, int t) {
   StepQ r;
+  r.t = t;
   r.lo = glo_of(t, G.ny);
   r.hi = (t >= 0 && t < G.Tb) ? min(G.L - 1, t >> 1) : -1;
   r.off = blk_soff(t, G.L, G.ny) - r.lo;
   return r;
 }
+// The next step (t + 1) from this one, with a few scalar operations:
+// the slot offset advances by the cells of step t less the change of lo.
+// Exact while t + 1 lies inside [0, Tb); outside it no lane has a cell.
+YG_DEV void step_next(const BlockGeom& G, StepQ& q) {
+  const int t1 = q.t + 1;
+  const int lo1 = max(0, (t1 - G.ny + 2) >> 1);
+  q.off += q.hi + 1 - lo1;
+  q.hi = t1 < G.Tb ? min(G.L - 1, t1 >> 1) : -1;
+  q.lo = lo1;
+  q.t = t1;
+}
+// The previous step (t - 1), for the backward sweep.  Exact while t - 1 lies
+// inside [0, Tb).
+YG_DEV void step_prev(const BlockGeom& G, StepQ& q) {
+  const int t1 = q.t - 1;
+  const int lo1 = max(0, (t1 - G.ny + 2) >> 1);
+  const int hi1 = (t1 >= 0 && t1 < G.Tb) ? min(G.L - 1, t1 >> 1) : -1;
+  q.off -= hi1 + 1 - q.lo;
+  q.hi = hi1;
+  q.lo = lo1;
+  q.t = t1;
+}
 // byte offset of a lane's cell at a step, or out of range if it has none
 YG_DEV unsigned cell_off(const StepQ& q, int lane, int lane_base, unsigned bytes) {
   const unsigned o = (unsigned)(lane_base + q.off) * bytes;
+  return (lane >= q.lo && lane <= q.hi) ? o : YG_OOB;
+}
+// natural-order byte offset (lane_n + t) of a lane's cell at the step of q
+YG_DEV unsigned nat_off(const StepQ& q, int lane, int lane_n, unsigned bytes) {
+  const unsigned o = (unsigned)(lane_n + q.t) * bytes;
   return (lane >= q.lo && lane <= q.hi) ? o : YG_OOB;
 }
 
@@ -717,7 +707,7 @@ template <typename T>
 YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
                              T* __restrict__ ys) {
   PhaseTimer ptm(c, 1);
-  constexpr int D = sizeof(T) == 4 ? 8 : 4;   // register slots (steps in flight)
+  constexpr int D = sizeof(T) == 4 ? 12 : 6;   // register slots (steps in flight)
   constexpr unsigned SF = 8 * sizeof(T), SV = 2 * sizeof(T);   // bytes per cell
   using V2 = typename Vec2<T>::type;
   const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
@@ -729,27 +719,23 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F, (size_t)8 * nc);
   const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
-  const int lane_q = i0 * ny + lane;               // + off(t): block-skewed slot
+  const int lane_q = i0 * ny + lane;               // + off: block-skewed slot
   const int lane_n = (i0 + lane) * ny - 2 * lane;  // + t: natural cell
   T f[D][8];
   V2 r[D];
-  // the natural cell of a lane at step t is lane_n + t, the same for every
-  // lane with a cell at t: its offset uses the step table's lane range too
+  StepQ qc = step_q(G, 0), qn = step_q(G, 0);   // the step computed, the step loaded
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    const StepQ q = step_q(G, s);
-    load_vals<8>(rF, cell_off(q, lane, lane_q, SF), f[s]);
-    StepQ qn = q;
-    qn.off = s;
-    r[s] = Pair16<T>::load(rR, cell_off(qn, lane, lane_n, SV));
+    load_vals<8>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+    r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
+    step_next(G, qn);
+    __builtin_amdgcn_sched_barrier(0);   // slots issued in order (see the loop)
   }
   T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
   for (int t0 = 0; t0 < G.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      const int t = t0 + s;
-      const StepQ q = step_q(G, t);
-      const bool h = lane >= q.lo && lane <= q.hi;
+      const bool h = lane >= qc.lo && lane <= qc.hi;
       const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
       T yx = r[s].x - f[s][0] * n2x - f[s][1] * o1x - f[s][2] * o1y;
       T yy = r[s].y - f[s][3] * n2x - f[s][4] * n2y - f[s][5] * n1x - f[s][6] * o1y -
@@ -759,14 +745,16 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
       V2 o;
       o.x = yx;
       o.y = yy;
-      Pair16<T>::store(o, rY, cell_off(q, lane, lane_q, SV));
+      Pair16<T>::store(o, rY, cell_off(qc, lane, lane_q, SV));
       o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
-      const int tn = t + D;
-      const StepQ qn = step_q(G, tn);
+      step_next(G, qc);
       load_vals<8>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
-      StepQ qr = qn;
-      qr.off = tn;
-      r[s] = Pair16<T>::load(rR, cell_off(qr, lane, lane_n, SV));
+      r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
+      step_next(G, qn);
+      // keep the steps in program order: slot s is refilled only after its
+      // use, and no later step's loads move ahead of it, so the wait before
+      // each step covers exactly that step's loads
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -775,7 +763,7 @@ template <typename T>
 YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ ys,
                              T* __restrict__ z) {
   PhaseTimer ptm(c, 2);
-  constexpr int D = sizeof(T) == 4 ? 8 : 4;
+  constexpr int D = sizeof(T) == 4 ? 10 : 6;
   constexpr unsigned SF = 12 * sizeof(T), SV = 2 * sizeof(T);
   using V2 = typename Vec2<T>::type;
   const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
@@ -789,22 +777,21 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
   const int lane_q = i0 * ny + lane;
   const int lane_n = (i0 + lane) * ny - 2 * lane;
-  const int Tl = G.Tb - 1;   // backward position u is step Tl - u
   T f[D][12];
   V2 yv[D];
+  StepQ qc = step_q(G, G.Tb - 1), qn = step_q(G, G.Tb - 1);
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    const StepQ q = step_q(G, Tl - s);
-    load_vals<12>(rF, cell_off(q, lane, lane_q, SF), f[s]);
-    yv[s] = Pair16<T>::load(rY, cell_off(q, lane, lane_q, SV));
+    load_vals<12>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+    yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
+    step_prev(G, qn);
+    __builtin_amdgcn_sched_barrier(0);
   }
   T o1x = 0, o1y = 0, o2x = 0, o2y = 0;  // own cells j+1, j+2
   for (int u0 = 0; u0 < G.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      const int t = Tl - (u0 + s);
-      const StepQ q = step_q(G, t);
-      const bool h = lane >= q.lo && lane <= q.hi;
+      const bool h = lane >= qc.lo && lane <= qc.hi;
       // line i+1: cell j-1 (one step back) and cell j (two steps back)
       const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
       T zy = (yv[s].y - f[s][6] * o1x - f[s][7] * o1y - f[s][8] * n2y) * f[s][9];
@@ -815,13 +802,13 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
       V2 o;
       o.x = zx;
       o.y = zy;
-      StepQ qz = q;
-      qz.off = t;
-      Pair16<T>::store(o, rZ, cell_off(qz, lane, lane_n, SV));
+      Pair16<T>::store(o, rZ, nat_off(qc, lane, lane_n, SV));
       o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
-      const StepQ qn = step_q(G, t - D);
+      step_prev(G, qc);
       load_vals<12>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
       yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
+      step_prev(G, qn);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -857,36 +844,118 @@ struct SolveOut {
   double resid;
 };
 
-// y = A x for every cell; post(q) runs after cell q's result is stored (the
-// dots fused into the SpMV).  Each thread takes four cells per trip so that
-// the loads of all four are in flight together.
-template <typename TA, typename TV, typename F>
-YG_DEV void spmv_all(const Ctx& c, const TA* __restrict__ A, const TV* __restrict__ x,
-                     TV* __restrict__ y, F&& post) {
+// ---- stencil SpMV ------------------------------------------------------
+// y = A x for the two rows of natural cell n = i*ny + j (the reference's
+// unknown numbering: rows 2n, 2n+1).  The cell's matrix record arrives as
+// 16-byte chunks from one offset; x is read as the (ux,uy) pairs of the 7
+// cells of the stencil through a bounds-checked buffer: a neighbour index
+// past either end of the member reads 0, and one that steps across a
+// non-periodic border into another line meets a zero coefficient (absent
+// slots hold exact zeros).  Periodic dimensions wrap explicitly.
+// wrap: bit 0 periodic in x, bit 1 periodic in y (wave-uniform)
+template <typename TA, typename TV>
+YG_DEV void spmv_cell(int nx, int ny, int wrap, __amdgpu_buffer_rsrc_t rA,
+                      __amdgpu_buffer_rsrc_t rX, int n, double& yx, double& yy) {
+  using V2 = typename Vec2<TV>::type;
+  constexpr unsigned SV = 2 * sizeof(TV), SA = YG_NA * sizeof(TA);
+  int nip = n + ny, nim = n - ny, njp = n + 1, njm = n - 1, nipjm = n + ny - 1, nimjp = n - ny + 1;
+  if (wrap) {
+    const int i = n / ny, j = n - i * ny;
+    const int ip = (wrap & 1) ? (i + 1 == nx ? 0 : i + 1) : i + 1;
+    const int im = (wrap & 1) ? (i == 0 ? nx - 1 : i - 1) : i - 1;
+    const int jp = (wrap & 2) ? (j + 1 == ny ? 0 : j + 1) : j + 1;
+    const int jm = (wrap & 2) ? (j == 0 ? ny - 1 : j - 1) : j - 1;
+    nip = ip * ny + j; nim = im * ny + j; njp = i * ny + jp; njm = i * ny + jm;
+    nipjm = ip * ny + jm; nimjp = im * ny + jp;
+  }
+  auto ldx = [&](int k) { return Pair16<TV>::load(rX, (unsigned)k * SV); };
+  const V2 x0 = ldx(n), xip = ldx(nip), xim = ldx(nim), xjp = ldx(njp), xjm = ldx(njm),
+           xipjm = ldx(nipjm), ximjp = ldx(nimjp);
+  TA a[sizeof(TA) == 4 ? 20 : 18];   // the record (FP64: its 18 values, not the pad)
+  load_vals<sizeof(TA) == 4 ? 20 : 18>(rA, (unsigned)n * SA, a);
+  // products summed in the vector type (FP32 in the inner solve of the
+  // mixed mode, FP64 otherwise)
+  // ux row: slots ux(i,j) ux(i+1,j) ux(i-1,j) ux(i,j+1) ux(i,j-1)
+  //              uy(i,j) uy(i+1,j) uy(i+1,j-1) uy(i,j-1)
+  const TV sx = (TV)a[0] * x0.x + (TV)a[2] * xip.x + (TV)a[4] * xim.x + (TV)a[6] * xjp.x +
+                (TV)a[8] * xjm.x + (TV)a[10] * x0.y + (TV)a[12] * xip.y + (TV)a[14] * xipjm.y +
+                (TV)a[16] * xjm.y;
+  // uy row: slots uy(i,j) uy(i,j+1) uy(i,j-1) uy(i+1,j) uy(i-1,j)
+  //              ux(i,j) ux(i,j+1) ux(i-1,j+1) ux(i-1,j)
+  const TV sy = (TV)a[1] * x0.y + (TV)a[3] * xjp.y + (TV)a[5] * xjm.y + (TV)a[7] * xip.y +
+                (TV)a[9] * xim.y + (TV)a[11] * x0.x + (TV)a[13] * xjp.x + (TV)a[15] * ximjp.x +
+                (TV)a[17] * xim.x;
+  yx = (double)sx;
+  yy = (double)sy;
+}
+
+// What a SpMV computes beside y = A x, with its block-reduced sums:
+//   YG_SPMV_RES : r = u - y (u = b), w2 = r when given;  sums (r,r), (u,u)
+//   YG_SPMV_DOT : sum (u, y)
+//   YG_SPMV_DOT2: sums (y, y), (y, u)
+#define YG_SPMV_RES 0
+#define YG_SPMV_DOT 1
+#define YG_SPMV_DOT2 2
+struct Sums2 {
+  double s0, s1;
+};
+
+// One SpMV over the member, four cells per thread per trip (the loads of all
+// four in flight together), with the fused dots.  A device function of its
+// own (not inlined) so that its addresses and resources get their own
+// register allocation.
+template <int MODE, typename TA, typename TV>
+YG_NOINLINE Sums2 spmv_dots(const Ctx& c, const TA* A, const TV* x, TV* y, const TV* u, TV* w1,
+                            TV* w2) {
   PhaseTimer ptm(c, 3);
   using V2 = typename Vec2<TV>::type;
-  const int B = blockDim.x;
-  V2* __restrict__ Y = reinterpret_cast<V2*>(y);
-  for (int q0 = threadIdx.x; q0 < c.ncell; q0 += 4 * B) {
+  const int nc = uni(c.ncell), B = blockDim.x;
+  const int nx = uni(c.nx), ny = uni(c.ny), wrap = uni((c.wrapx ? 1 : 0) | (c.wrapy ? 2 : 0));
+  const __amdgpu_buffer_rsrc_t rA = yg_rsrc(A, (size_t)YG_NA * nc);
+  const __amdgpu_buffer_rsrc_t rX = yg_rsrc(x, (size_t)2 * nc);
+  constexpr unsigned SV = 2 * sizeof(TV);
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(y, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rU = yg_rsrc(u, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rW1 = yg_rsrc(w1, w1 ? (size_t)2 * nc : 0);
+  const __amdgpu_buffer_rsrc_t rW2 = yg_rsrc(w2, w2 ? (size_t)2 * nc : 0);   // range 0: stores dropped
+  double s0 = 0.0, s1 = 0.0;
+  // Four cells per thread per trip, with no branch around the memory
+  // operations: a cell index past the member reads zeros (matrix record and
+  // u out of range) and its stores are dropped, so every load of the trip
+  // is issued before the first is waited for.
+  for (int q0 = threadIdx.x; q0 < nc; q0 += 4 * B) {
     double ax[4], ay[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = q0 + u * B;
-      ax[u] = ay[u] = 0.0;
-      if (q < c.ncell) spmv_cell(c, A, x, q, ax[u], ay[u]);
-    }
+    for (int k = 0; k < 4; ++k) spmv_cell<TA, TV>(nx, ny, wrap, rA, rX, q0 + k * B, ax[k], ay[k]);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = q0 + u * B;
-      if (q < c.ncell) {
-        V2 o;
-        o.x = (TV)ax[u];
-        o.y = (TV)ay[u];
-        Y[q] = o;
-        post(q);
+    for (int k = 0; k < 4; ++k) {
+      const int q = q0 + k * B;
+      const unsigned off = q < nc ? (unsigned)q * SV : YG_OOB;
+      V2 o;
+      o.x = (TV)ax[k];
+      o.y = (TV)ay[k];
+      Pair16<TV>::store(o, rY, off);
+      const double yx = (double)o.x, yy = (double)o.y;
+      const V2 uq = Pair16<TV>::load(rU, off);
+      if (MODE == YG_SPMV_RES) {
+        V2 r;
+        r.x = (TV)((double)uq.x - yx);
+        r.y = (TV)((double)uq.y - yy);
+        Pair16<TV>::store(r, rW1, off);
+        Pair16<TV>::store(r, rW2, off);
+        s0 += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+        s1 += (double)uq.x * (double)uq.x + (double)uq.y * (double)uq.y;
+      } else if (MODE == YG_SPMV_DOT) {
+        s0 += (double)uq.x * yx + (double)uq.y * yy;
+      } else {
+        s0 += yx * yx + yy * yy;
+        s1 += yx * (double)uq.x + yy * (double)uq.y;
       }
     }
   }
+  double sums[2] = {s0, s1};
+  yg_block_sum<2>(sums, yg_red);
+  return Sums2{sums[0], sums[1]};
 }
 
 // A 16-byte chunk of a vector (4 floats or 2 doubles).
@@ -961,19 +1030,12 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
   TV* __restrict__ ytmp = W + 8 * vl;
   SolveOut out{0, 4, 0.0};
   double red[3];
-  // r = b - A x, rt = r
-  red[0] = red[1] = red[2] = 0.0;
-  spmv_all(c, A, x, tt, [&](int q) {
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const double rq = (double)b[2 * q + e] - (double)tt[2 * q + e];
-      r[2 * q + e] = (TV)rq;
-      rt[2 * q + e] = (TV)rq;
-      red[0] += (double)r[2 * q + e] * (double)r[2 * q + e];
-      red[1] += (double)b[2 * q + e] * (double)b[2 * q + e];
-    }
-  });
-  yg_block_sum<3>(red, c.red);
+  // r = b - A x, rt = r, (r,r), (b,b)
+  {
+    const Sums2 z = spmv_dots<YG_SPMV_RES, TA, TV>(c, A, x, tt, b, r, rt);
+    red[0] = z.s0;
+    red[1] = z.s1;
+  }
   const double nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : sqrt(red[0]);
   if (nrm0 == 0.0) { out.status = 0; out.resid = 0.0; return out; }
   const double bnrm = 1.0 / nrm0;
@@ -994,11 +1056,7 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     __syncthreads();
     precon_apply<TV>(c, P, p, ytmp, ph);
     // v = A phat, (rt, v)
-    red[0] = 0.0;
-    spmv_all(c, A, ph, v, [&](int q) {
-      red[0] += (double)rt[2 * q] * (double)v[2 * q] + (double)rt[2 * q + 1] * (double)v[2 * q + 1];
-    });
-    yg_block_sum<1>(*reinterpret_cast<double(*)[1]>(red), c.red);
+    red[0] = spmv_dots<YG_SPMV_DOT, TA, TV>(c, A, ph, v, rt, nullptr, nullptr).s0;
     if (red[0] == 0.0) { out.status = 2; out.iters = it; return out; }
     alpha = rho / red[0];
     // s = r - alpha v and ||s||: converged on s ends the iteration early
@@ -1008,7 +1066,7 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
       xo[0] = sq;
       red[0] += sq * sq;
     });
-    yg_block_sum<1>(*reinterpret_cast<double(*)[1]>(red), c.red);
+    yg_block_sum<1>(*reinterpret_cast<double(*)[1]>(red), yg_red);
     const double sn = sqrt(red[0]) * bnrm;
     if (sn <= o.tol) {
       vmap<TV, 2, 1>(n2, {x, ph}, {x}, [&](const double (&xi)[2], double (&xo)[1]) {
@@ -1020,16 +1078,11 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     }
     precon_apply<TV>(c, P, s, ytmp, sh);
     // t = A shat, (t,t), (t,s)
-    red[0] = red[1] = 0.0;
-    spmv_all(c, A, sh, tt, [&](int q) {
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const double tq = (double)tt[2 * q + e];
-        red[0] += tq * tq;
-        red[1] += tq * (double)s[2 * q + e];
-      }
-    });
-    yg_block_sum<2>(*reinterpret_cast<double(*)[2]>(red), c.red);
+    {
+      const Sums2 z = spmv_dots<YG_SPMV_DOT2, TA, TV>(c, A, sh, tt, s, nullptr, nullptr);
+      red[0] = z.s0;
+      red[1] = z.s1;
+    }
     if (red[0] == 0.0) { out.status = 2; out.iters = it; return out; }
     omega = red[1] / red[0];
     if (omega == 0.0) { out.status = 2; out.iters = it; return out; }
@@ -1043,7 +1096,7 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
       red[0] += rq * rq;
       red[1] += xi[5] * rq;
     });
-    yg_block_sum<2>(*reinterpret_cast<double(*)[2]>(red), c.red);
+    yg_block_sum<2>(*reinterpret_cast<double(*)[2]>(red), yg_red);
     out.resid = sqrt(red[0]) * bnrm;
     out.iters = it;
     if (out.resid <= o.tol) { out.status = 0; return out; }
@@ -1065,16 +1118,12 @@ YG_DEV SolveOut cg(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, 
   TV* qv = W + 3 * vl;
   TV* ytmp = W + 8 * vl;
   SolveOut out{0, 4, 0.0};
-  double red[2] = {0.0, 0.0};
-  spmv_all(c, A, x, qv, [&](int q) {
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      r[2 * q + e] = (TV)((double)b[2 * q + e] - (double)qv[2 * q + e]);
-      red[0] += (double)r[2 * q + e] * (double)r[2 * q + e];
-      red[1] += (double)b[2 * q + e] * (double)b[2 * q + e];
-    }
-  });
-  yg_block_sum<2>(red, c.red);
+  double red[2];
+  {
+    const Sums2 z = spmv_dots<YG_SPMV_RES, TA, TV>(c, A, x, qv, b, r, nullptr);
+    red[0] = z.s0;
+    red[1] = z.s1;
+  }
   const double nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : sqrt(red[0]);
   if (nrm0 == 0.0) { out.status = 0; return out; }
   const double bnrm = 1.0 / nrm0;
@@ -1085,18 +1134,15 @@ YG_DEV SolveOut cg(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, 
     precon_apply<TV>(c, P, r, ytmp, z);
     double rr[1] = {0.0};
     for (int q = threadIdx.x; q < n2; q += blockDim.x) rr[0] += (double)r[q] * (double)z[q];
-    yg_block_sum<1>(rr, c.red);
+    yg_block_sum<1>(rr, yg_red);
     const double rho = rr[0];
     if (rho == 0.0) { out.status = 2; out.iters = it; return out; }
     const double beta = it == 1 ? 0.0 : rho / rho_old;
     for (int q = threadIdx.x; q < n2; q += blockDim.x)
       p[q] = it == 1 ? z[q] : (TV)((double)z[q] + beta * (double)p[q]);
     __syncthreads();
-    double pq[1] = {0.0};
-    spmv_all(c, A, p, qv, [&](int q) {
-      pq[0] += (double)p[2 * q] * (double)qv[2 * q] + (double)p[2 * q + 1] * (double)qv[2 * q + 1];
-    });
-    yg_block_sum<1>(pq, c.red);
+    double pq[1];
+    pq[0] = spmv_dots<YG_SPMV_DOT, TA, TV>(c, A, p, qv, p, nullptr, nullptr).s0;
     if (pq[0] == 0.0) { out.status = 2; out.iters = it; return out; }
     const double alpha = rho / pq[0];
     double rn[1] = {0.0};
@@ -1105,7 +1151,7 @@ YG_DEV SolveOut cg(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, 
       r[q] = (TV)((double)r[q] - alpha * (double)qv[q]);
       rn[0] += (double)r[q] * (double)r[q];
     }
-    yg_block_sum<1>(rn, c.red);
+    yg_block_sum<1>(rn, yg_red);
     out.resid = sqrt(rn[0]) * bnrm;
     out.iters = it;
     if (out.resid <= o.tol) { out.status = 0; return out; }
@@ -1123,8 +1169,8 @@ YG_DEV void build_precon(const Ctx& c, const T* A, T* F, Precon<T>& P, int kind,
     ilu0_factor<T>(c, A, F, bad);
   } else if (kind == 1) {
     for (int q = threadIdx.x; q < c.ncell; q += blockDim.x) {
-      F[2 * q] = (T)(1.0 / (double)A[((size_t)0 * c.ncell + q) * 2 + 0]);
-      F[2 * q + 1] = (T)(1.0 / (double)A[((size_t)0 * c.ncell + q) * 2 + 1]);
+      F[2 * q] = (T)(1.0 / (double)A[(size_t)q * YG_NA + 0]);
+      F[2 * q + 1] = (T)(1.0 / (double)A[(size_t)q * YG_NA + 1]);
     }
     __syncthreads();
   }
@@ -1149,7 +1195,6 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   const int m = blockIdx.x;
   if (a.skip && a.skip[m]) return;
   const YgMember mb = a.members[m];
-  __shared__ double s_red[16 * 4];
   __shared__ int s_bad;
   const int ncell = mb.nx * mb.ny;
   for (int k = threadIdx.x; k <= mb.T; k += blockDim.x) yg_soff[k] = a.soff[mb.toff + k];
@@ -1160,7 +1205,6 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   c.nx = mb.nx; c.ny = mb.ny; c.T = mb.T; c.ncell = ncell; c.vlen = mb.vlen;
   c.wrapx = a.wrapx; c.wrapy = a.wrapy;
   c.mask = a.mask + mb.coff;
-  c.red = s_red;
   c.maxcnt = mb.maxcnt;
   {
     // line-to-wave mapping of the recurrences (exact or block ILU(0))
@@ -1195,7 +1239,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   {
     double bm[1] = {0.0};
     for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) bm[0] += b64[q] != 0.0 ? 1.0 : 0.0;
-    yg_block_sum<1>(bm, s_red);
+    yg_block_sum<1>(bm, yg_red);
     if (bm[0] == 0.0) {
       for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] = 0.0;
       if (threadIdx.x == 0) { a.out_iters[m] = 0; a.out_status[m] = 0; a.out_resid[m] = 0.0; }
@@ -1204,7 +1248,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   }
   YgKrylovOpts o = a.opts;
   if (MODE == YG_MODE_FP64) {
-    const double* A = a.A64 + (size_t)18 * mb.coff;
+    const double* A = a.A64 + (size_t)YG_NA * mb.coff;
     double* F = a.F64 + (size_t)YG_NF * mb.coff;
     double* W = a.W64 + (size_t)YG_NW64 * mb.woff;
     if (o.initx_zeros) {
@@ -1222,13 +1266,14 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     }
   } else {
     // FP64 iterative refinement around an FP32 solve
-    const double* A64 = a.A64 + (size_t)18 * mb.coff;
-    const float* A32 = a.A32 + (size_t)18 * mb.coff;
+    const double* A64 = a.A64 + (size_t)YG_NA * mb.coff;
+    const float* A32 = a.A32 + (size_t)YG_NA * mb.coff;
     float* F = a.F32 + (size_t)YG_NF * mb.coff;
     float* W = a.W32 + (size_t)YG_NW32 * mb.woff;  // 9 inner Krylov vectors ...
     float* d = W + (size_t)9 * mb.vlen;             // ... the correction ...
     float* rin = W + (size_t)10 * mb.vlen;          // ... and the scaled residual
-    double* r64 = a.W64 + (size_t)YG_NW64 * mb.woff; // FP64 residual
+    double* r64 = a.W64 + (size_t)YG_NW64 * mb.woff; // FP64 residual ...
+    double* y64 = r64 + mb.vlen;                      // ... and A x
     if (o.initx_zeros) {
       for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] = 0.0;
       __syncthreads();
@@ -1242,16 +1287,12 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     double nrm0 = -1.0, res = 1.0;
     int iters = 0, status = 4;
     for (int ir = 0; ir < o.ir_max; ++ir) {
-      double red[2] = {0.0, 0.0};
-      for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
-        double ax, ay;
-        spmv_cell(c, A64, x64, q, ax, ay);
-        const double rx = b64[2 * q] - ax, ry = b64[2 * q + 1] - ay;
-        r64[2 * q] = rx; r64[2 * q + 1] = ry;
-        red[0] += rx * rx + ry * ry;
-        red[1] += b64[2 * q] * b64[2 * q] + b64[2 * q + 1] * b64[2 * q + 1];
+      double red[2];
+      {
+        const Sums2 z = spmv_dots<YG_SPMV_RES, double, double>(c, A64, x64, y64, b64, r64, nullptr);
+        red[0] = z.s0;
+        red[1] = z.s1;
       }
-      yg_block_sum<2>(red, s_red);
       const double rn = sqrt(red[0]);
       if (nrm0 < 0.0) nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : rn;
       if (nrm0 == 0.0) { res = 0.0; status = 0; break; }
