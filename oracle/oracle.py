@@ -99,6 +99,92 @@ def calc_velocity_ssa(prob, par, dp: bool):
     return st, info
 
 
+DIVA_STATE_3D = ("ux", "uy", "ux_i", "uy_i", "duxdz", "duydz")
+DIVA_STATE_2D = ("ux_bar", "uy_bar", "beta_eff")
+
+
+def diva_state(prob):
+    """Adds the DIVA-only state arrays (zero) that prob does not hold yet."""
+    nx, ny, nz = prob.nx, prob.ny, prob.nz
+    dt = prob.fields["H_ice"].dtype
+    for n in DIVA_STATE_3D:
+        if n not in prob.state:
+            prob.state[n] = np.zeros((nx, ny, nz), dtype=dt, order="F")
+    for n in DIVA_STATE_2D:
+        if n not in prob.state:
+            prob.state[n] = np.zeros((nx, ny), dtype=dt, order="F")
+    return prob
+
+
+def calc_velocity_diva(prob, par, dp: bool):
+    """The oracle's DIVA Picard loop (calc_velocity_diva) on a copy of prob.
+    Returns (state dict, info dict)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    P = diva_state(prob.astype(dt))
+    st, fl = P.state, P.fields
+    for n in DIVA_STATE_3D + DIVA_STATE_2D:
+        st[n] = np.asfortranarray(st[n], dtype=dt)
+    out_i = np.zeros(4, dtype=np.int32)
+    rh = np.zeros(par.ssa_iter_max, dtype=np.float64)
+    lh = np.zeros(par.ssa_iter_max, dtype=np.int32)
+    secs = C.c_double()
+    W = C.c_double if dp else C.c_float
+    L.oracle_calc_velocity_diva.restype = C.c_int
+    args = [st["ux"], st["uy"], st["ux_bar"], st["uy_bar"], st["ux_b"], st["uy_b"], st["ux_i"],
+            st["uy_i"], st["taub_acx"], st["taub_acy"], st["beta"], st["beta_acx"],
+            st["beta_acy"], st["beta_eff"], st["visc_eff"], st["visc_eff_int"], st["duxdz"],
+            st["duydz"], st["ssa_mask_acx"], st["ssa_mask_acy"], st["ssa_err_acx"],
+            st["ssa_err_acy"], out_i, fl["c_bed"], fl["f_slide"], fl["taud_acx"], fl["taud_acy"],
+            fl["taul_int_acx"], fl["taul_int_acy"], fl["H_ice"], fl["f_ice"], fl["H_grnd"],
+            fl["f_grnd"], fl["f_grnd_acx"], fl["f_grnd_acy"], fl["ATT"], fl["zeta_aa"],
+            fl["z_sl"], fl["z_bed"], fl["z_srf"]]
+    pc = params_c(par)
+    rc = L.oracle_calc_velocity_diva(
+        *[_p(a) for a in args], W(P.dx), W(P.dy), W(P.n_glen), C.byref(pc), int(par.no_slip),
+        par.ssa_lis_opt.encode(), P.nx, P.ny, P.nz, _p(rh), _p(lh), C.byref(secs))
+    info = dict(rc=rc, ssa_iter_now=int(out_i[0]), ssa_lin_iter=int(out_i[1]),
+                ssa_lin_fail=int(out_i[2]), ssa_lim_n=int(out_i[3]),
+                resid_hist=rh[: max(out_i[0], 0)].copy(), lin_hist=lh[: max(out_i[0], 0)].copy(),
+                solve_seconds=secs.value)
+    return st, info
+
+
+def F_integral(visc, H_ice, f_ice, zeta, dp=True):
+    """calc_F_integral with n = 2 on given fields."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny, nz = visc.shape
+    keep = [np.asfortranarray(a, dtype=dt) for a in (visc, H_ice, f_ice, zeta)]
+    F = np.zeros((nx, ny), dtype=dt, order="F")
+    L.oracle_F_integral(_p(F), *[_p(a) for a in keep], nx, ny, nz)
+    return F
+
+
+def beta_eff(beta, F2, no_slip=False, dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    keep = [np.asfortranarray(beta, dtype=dt), np.asfortranarray(F2, dtype=dt)]
+    out = np.zeros_like(keep[0])
+    L.oracle_beta_eff(_p(out), _p(keep[0]), _p(keep[1]), int(no_slip), out.size)
+    return out
+
+
+def vel_horizontal_3D(ux_b, uy_b, taub_acx, taub_acy, visc, H_ice, f_ice, zeta,
+                      boundaries="zeros", dp=True):
+    """calc_vel_horizontal_3D on given fields: (ux, uy) of shape (nx, ny, nz)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny, nz = visc.shape
+    keep = [np.asfortranarray(a, dtype=dt) for a in
+            (ux_b, uy_b, taub_acx, taub_acy, visc, H_ice, f_ice, zeta)]
+    ux = np.zeros((nx, ny, nz), dtype=dt, order="F")
+    uy = np.zeros((nx, ny, nz), dtype=dt, order="F")
+    L.oracle_vel_horizontal_3D(_p(ux), _p(uy), *[_p(a) for a in keep], nx, ny, nz,
+                               BND[boundaries])
+    return ux, uy
+
+
 def csr_solve(a_ptr1, a_idx1, a_val, b, x0, opts: str):
     L = lib(True)
     n = len(b)

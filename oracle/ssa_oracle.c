@@ -1534,5 +1534,8 @@ out:
     return rc;
 }
 
+/* the DIVA solve (calc_velocity_diva) and its helpers */
+#include "diva_oracle.inc"
+
 /* size of wp, so the Python driver can check which build it loaded */
 int oracle_wp_bytes(void) { return (int)sizeof(wp); }

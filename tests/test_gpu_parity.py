@@ -134,3 +134,50 @@ def test_batch_alone_equals_in_mix():
     for a, m in zip(alone, mix):
         assert np.array_equal(a.state["ux_b"], m.state["ux_b"])
         assert np.array_equal(a.state["uy_b"], m.state["uy_b"])
+
+
+# ---- DIVA (calc_velocity_diva) ---------------------------------------------
+
+DIVA_CASES = [(41, 32e3, 2, 4, False), (41, 32e3, 1, 2, False), (41, 32e3, 2, 4, True)]
+
+
+def _diva_errs(st, ref):
+    vb = lambda s: np.concatenate([s["ux_bar"].ravel(), s["uy_bar"].ravel()])
+    v3 = lambda s: np.concatenate([s["ux"].ravel(), s["uy"].ravel()])
+    ub = lambda s: np.concatenate([s["ux_b"].ravel(), s["uy_b"].ravel()])
+    eb = rel_l2(vb(st), vb(ref))
+    e3 = rel_l2(v3(st), v3(ref))
+    # u_b is ubar - tau_b F2: relative to the depth-averaged scale
+    e_b = float(np.linalg.norm(ub(st) - ub(ref)) / np.linalg.norm(vb(ref)))
+    return eb, e3, e_b
+
+
+@pytest.mark.parametrize("n,dx,vm,bm,no_slip", DIVA_CASES)
+def test_diva_fp64_matches_fp64_oracle(n, dx, vm, bm, no_slip):
+    print(f"case diva fp64 n={n} visc={vm} beta={bm} no_slip={no_slip}", flush=True)
+    par = SsaParams(visc_method=vm, beta_method=bm, precision="fp64", no_slip=no_slip)
+    p = _problem(n, dx)
+    ref_st, ref = oracle.calc_velocity_diva(p, par, dp=True)
+    assert ref["rc"] == 0
+    g = oracle.diva_state(p.copy())
+    info = velocity.calc_velocity_diva(g, par)
+    eb, e3, e_b = _diva_errs(g.state, ref_st)
+    print("iters gpu", info.ssa_iter_now, "oracle", ref["ssa_iter_now"], "ubar", eb, "3D", e3,
+          "u_b", e_b)
+    assert info.status == 0
+    assert abs(info.ssa_iter_now - ref["ssa_iter_now"]) <= 1
+    assert eb <= 1e-8 and e3 <= 1e-8 and e_b <= 1e-8
+    assert rel_l2(g.state["beta_eff"], ref_st["beta_eff"]) <= 1e-7
+
+
+def test_diva_mixed_matches_fp64_oracle():
+    print("case diva mixed", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    p = _problem(41, 32e3)
+    ref_st, ref = oracle.calc_velocity_diva(p, par.with_(precision="fp64"), dp=True)
+    g = oracle.diva_state(p.copy())
+    info = velocity.calc_velocity_diva(g, par)
+    eb, e3, e_b = _diva_errs(g.state, ref_st)
+    print("iters gpu", info.ssa_iter_now, "oracle", ref["ssa_iter_now"], "ubar", eb, "3D", e3)
+    assert info.status == 0
+    assert eb <= 1e-6 and e3 <= 1e-6

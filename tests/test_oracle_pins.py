@@ -333,12 +333,10 @@ def test_masks_two_restatements_agree_and_truth_table():
     assert ff(-1, 3, "none") == 4
 
 
-def test_picard_solution_mirror_symmetric():
-    """A domain symmetric under x -> -x (noise-free radial cap on a symmetric
-    bed) gives ux antisymmetric and uy symmetric after the whole Picard loop
-    (the reference reports a 2e-13 floor in double precision,
-    docs/numerics-precision.md:52-53).  An index slip in any cross term of
-    the assembly or the viscosity staggering breaks it."""
+def _mirror_domain():
+    """A 33 x 33 domain symmetric under x -> -x and y -> -x (noise-free
+    radial cap on a symmetric bed), with its inputs derived by the input-side
+    routines."""
     nx = ny = 33
     dx = 40e3
     p = generator.make_problem(nx, dx, seed=0, dtype=np.float64)
@@ -363,6 +361,16 @@ def test_picard_solution_mirror_symmetric():
                     z_srf=F(zs), taud_acx=F(tx), taud_acy=F(ty), taul_int_acx=F(lx),
                     taul_int_acy=F(ly), c_bed=F(np.where(fg > 0, 1e4, 0.0)), mask_frnt=mf)
     p.state.update(ssa_mask_acx=mx, ssa_mask_acy=my)
+    return p
+
+
+def test_picard_solution_mirror_symmetric():
+    """A domain symmetric under x -> -x (noise-free radial cap on a symmetric
+    bed) gives ux antisymmetric and uy symmetric after the whole Picard loop
+    (the reference reports a 2e-13 floor in double precision,
+    docs/numerics-precision.md:52-53).  An index slip in any cross term of
+    the assembly or the viscosity staggering breaks it."""
+    p = _mirror_domain()
     par = SsaParams(visc_method=2, beta_method=4, ssa_iter_max=30,
                     ssa_lis_opt="-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-12")
     st, info = oracle.calc_velocity_ssa(p, par, dp=True)
@@ -375,3 +383,83 @@ def test_picard_solution_mirror_symmetric():
     assert np.abs(uy - uy[::-1, :]).max() < 1e-8 * scale
     # and the same in y (transpose symmetry of the domain)
     assert np.abs(uy[:, :-1] + uy[:, -2::-1]).max() < 1e-8 * scale
+
+
+# ---- DIVA pieces (velocity_diva.f90:431-1050) ------------------------------
+
+def _slab(nx=6, ny=5, nz=401, H=800.0, eta=3e6):
+    zeta = np.linspace(0.0, 1.0, nz)
+    visc = np.full((nx, ny, nz), eta, order="F")
+    Hf = np.full((nx, ny), H, order="F")
+    fi = np.ones((nx, ny), order="F")
+    return zeta, visc, Hf, fi
+
+
+def test_diva_F2_integral_closed_form():
+    """F2 = int_0^1 (H/eta)(1-zeta)^2 dzeta = H/(3 eta) for a uniform slab; the
+    trapezoid rule on 401 levels is within 1e-5 of it (a wrong power of
+    (1-zeta) or a missing H fails by O(1)).  Ice-free cells give 0."""
+    zeta, visc, H, fi = _slab()
+    fi[0, 0] = 0.0
+    F2 = oracle.F_integral(visc, H, fi, zeta)
+    exact = 800.0 / (3 * 3e6)
+    assert F2[0, 0] == 0.0
+    assert np.allclose(F2[1:, :], exact, rtol=1e-5)
+
+
+def test_diva_beta_eff_reproduces_basal_stress_identity():
+    """With beta_eff = beta/(1+beta F2), tau_b = beta_eff ubar and
+    u_b = ubar - tau_b F2 satisfy tau_b = beta u_b (the DIVA sliding law,
+    L19); no-slip gives beta_eff = 1/F2 (u_b = 0, tau_b = ubar/F2)."""
+    rng = np.random.default_rng(3)
+    beta = np.asfortranarray(10 ** rng.uniform(1, 5, (7, 4)))
+    F2 = np.asfortranarray(10 ** rng.uniform(-6, -3, (7, 4)))
+    ubar = rng.uniform(-500, 500, (7, 4))
+    be = oracle.beta_eff(beta, F2)
+    taub = be * ubar
+    ub = ubar - taub * F2
+    assert np.allclose(taub, beta * ub, rtol=1e-12)
+    be0 = oracle.beta_eff(beta, F2, no_slip=True)
+    assert np.allclose(be0 * F2, 1.0, rtol=1e-14)
+
+
+def test_diva_3d_velocity_depth_average_and_surface():
+    """calc_vel_horizontal_3D on a uniform slab: u(zeta) = u_b + tau_b (H/eta)
+    (zeta - zeta^2/2), so u_s = u_b + tau_b H/(2 eta) and the depth average is
+    u_b + tau_b F2 (F2 = H/(3 eta)) -- the velocity the DIVA solve returns as
+    ubar."""
+    zeta, visc, H, fi = _slab()
+    nx, ny = H.shape
+    ub = np.full((nx, ny), 12.0, order="F")
+    vb = np.full((nx, ny), -4.0, order="F")
+    tbx = np.full((nx, ny), 7.0e4, order="F")
+    tby = np.full((nx, ny), -2.0e4, order="F")
+    ux, uy = oracle.vel_horizontal_3D(ub, vb, tbx, tby, visc, H, fi, zeta)
+    k = 800.0 / 3e6
+    assert np.allclose(ux[:, :, 0], 12.0)
+    assert np.allclose(ux[:, :, -1], 12.0 + 7.0e4 * k / 2, rtol=1e-5)
+    assert np.allclose(uy[:, :, -1], -4.0 - 2.0e4 * k / 2, rtol=1e-5)
+    mean = np.trapezoid(ux, zeta, axis=2)
+    F2 = oracle.F_integral(visc, H, fi, zeta)
+    assert np.allclose(mean, 12.0 + 7.0e4 * F2, rtol=1e-5)
+
+
+def test_diva_solution_mirror_symmetric():
+    """The whole DIVA Picard loop on the mirror-symmetric domain: ux_bar and
+    the 3D surface velocity antisymmetric in x, uy_bar symmetric (the
+    reference quotes a 2e-13 symmetry floor for DIVA in double precision,
+    docs/numerics-precision.md:52-53).  An index slip in the vertical shear
+    or the F-integral staggering breaks it."""
+    p = _mirror_domain()
+    par = SsaParams(visc_method=2, beta_method=4, ssa_iter_max=30,
+                    ssa_lis_opt="-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-12")
+    st, info = oracle.calc_velocity_diva(p, par, dp=True)
+    assert info["rc"] == 0
+    ux, uy, us = st["ux_bar"], st["uy_bar"], st["ux"][:, :, -1]
+    scale = np.abs(ux).max()
+    assert scale > 1.0
+    assert np.abs(ux[:-1, :] + ux[-2::-1, :]).max() < 1e-8 * scale
+    assert np.abs(uy - uy[::-1, :]).max() < 1e-8 * scale
+    assert np.abs(us[:-1, :] + us[-2::-1, :]).max() < 1e-8 * np.abs(us).max()
+    # the DIVA sliding law holds on the result: tau_b = beta_eff ubar
+    assert np.abs(st["ux_b"]).max() < scale
