@@ -11,6 +11,8 @@ from pathlib import Path
 
 import numpy as np
 
+from yelmo_amd.generator import DIVA_STATE_2D, DIVA_STATE_3D, add_diva_state as diva_state
+
 HERE = Path(__file__).resolve().parent
 _libs = {}
 
@@ -99,21 +101,6 @@ def calc_velocity_ssa(prob, par, dp: bool):
     return st, info
 
 
-DIVA_STATE_3D = ("ux", "uy", "ux_i", "uy_i", "duxdz", "duydz")
-DIVA_STATE_2D = ("ux_bar", "uy_bar", "beta_eff")
-
-
-def diva_state(prob):
-    """Adds the DIVA-only state arrays (zero) that prob does not hold yet."""
-    nx, ny, nz = prob.nx, prob.ny, prob.nz
-    dt = prob.fields["H_ice"].dtype
-    for n in DIVA_STATE_3D:
-        if n not in prob.state:
-            prob.state[n] = np.zeros((nx, ny, nz), dtype=dt, order="F")
-    for n in DIVA_STATE_2D:
-        if n not in prob.state:
-            prob.state[n] = np.zeros((nx, ny), dtype=dt, order="F")
-    return prob
 
 
 def calc_velocity_diva(prob, par, dp: bool):

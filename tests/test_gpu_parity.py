@@ -3,7 +3,10 @@
 Tolerances (velocity, relative L2 over ux and uy):
   FP64 GPU path vs FP64 oracle          1e-8, Picard iterations equal +-1
   reference mode (FP32 fields) vs FP32 oracle   1e-5 (FP32 rounding of fields)
-  mixed precision vs FP64 oracle        1e-6
+  mixed precision vs FP64 oracle        1e-6, the oracle's linear solves
+                                        converged to rtol 1e-12 (at rtol 1e-8
+                                        the oracle's Picard solution itself
+                                        moves by about 1e-6)
 """
 import numpy as np
 import pytest
@@ -16,6 +19,11 @@ from yelmo_amd.params import SsaParams
 pytestmark = pytest.mark.gpu
 
 CASES = [(41, 32e3, 2, 4), (41, 32e3, 1, 2), (81, 16e3, 1, 2)]
+
+
+def _converged(par):
+    """The FP64 settings with the linear solves converged to rtol 1e-12."""
+    return par.with_(precision="fp64", ssa_lis_opt=par.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
 
 
 def _problem(n, dx, dtype=np.float64, seed=0):
@@ -60,7 +68,7 @@ def test_ssa_mixed_matches_fp64_oracle(n, dx, vm, bm):
     print(f"case mixed n={n} visc={vm} beta={bm}", flush=True)
     par = SsaParams(visc_method=vm, beta_method=bm, precision="mixed")
     p = _problem(n, dx, dtype=np.float32)
-    ref_st, ref = oracle.calc_velocity_ssa(p.astype(np.float64), par.with_(precision="fp64"), dp=True)
+    ref_st, ref = oracle.calc_velocity_ssa(p.astype(np.float64), _converged(par), dp=True)
     g = p.copy()
     info = velocity.calc_velocity_ssa(g, par)
     err = vel_rel_l2(g.state, ref_st)
@@ -174,7 +182,7 @@ def test_diva_mixed_matches_fp64_oracle():
     print("case diva mixed", flush=True)
     par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
     p = _problem(41, 32e3)
-    ref_st, ref = oracle.calc_velocity_diva(p, par.with_(precision="fp64"), dp=True)
+    ref_st, ref = oracle.calc_velocity_diva(p, _converged(par), dp=True)
     g = oracle.diva_state(p.copy())
     info = velocity.calc_velocity_diva(g, par)
     eb, e3, e_b = _diva_errs(g.state, ref_st)

@@ -44,7 +44,7 @@ struct Opts {
 
 int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
-  o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1;
+  o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1; o.k.ir_target = 0.0;
   if (!s) return 0;
   std::vector<std::string> tok;
   std::string cur;
@@ -71,6 +71,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k == "-initx_zeros") o.k.initx_zeros = (v == "true" || v == "1");
     else if (k == "-conv_cond") o.k.conv_cond = atoi(v.c_str());
     else if (k == "-ir_inner_tol") o.k.inner_tol = atof(v.c_str());
+    else if (k == "-ygpu_ir_tol") o.k.ir_target = atof(v.c_str());
     else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
     else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
     else if (k == "-ygpu_ny") o.ny_hint = atoi(v.c_str());

@@ -30,6 +30,7 @@ struct YgKrylovOpts {
                     // FP64 mode, one block per 64 lines in the mixed mode
   double tol;
   double inner_tol; // mixed mode: relative tolerance of each FP32 inner solve
+  double ir_target; // mixed mode: residual the refinement stops at (0: tol / 10)
 };
 
 struct YgKrylovArgs {

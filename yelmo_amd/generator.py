@@ -163,6 +163,25 @@ def workload_ensemble(members: int = 64, side: int = 161, seed: int = 0, dtype=n
                          dtype=dtype, nz=nz) for m in range(members)]
 
 
+DIVA_STATE_3D = ("ux", "uy", "ux_i", "uy_i", "duxdz", "duydz")
+DIVA_STATE_2D = ("ux_bar", "uy_bar", "beta_eff")
+
+
+def add_diva_state(prob: SsaProblem) -> SsaProblem:
+    """Adds the INOUT arrays that calc_velocity_diva holds beyond the SSA ones
+    (3D velocity and shear, depth-averaged velocity, beta_eff), zero, where
+    prob does not hold them yet."""
+    nx, ny, nz = prob.nx, prob.ny, prob.nz
+    dt = prob.fields["H_ice"].dtype
+    for n in DIVA_STATE_3D:
+        if n not in prob.state:
+            prob.state[n] = np.zeros((nx, ny, nz), dtype=dt, order="F")
+    for n in DIVA_STATE_2D:
+        if n not in prob.state:
+            prob.state[n] = np.zeros((nx, ny), dtype=dt, order="F")
+    return prob
+
+
 DIVA_SIDES = (96, 128, 192, 256, 384)
 
 
@@ -174,5 +193,6 @@ def workload_diva_uneven(domains: int = 48, seed: int = 0, dtype=np.float32, nz:
     if sides is None:
         sides = rng.choice(DIVA_SIDES, size=domains)
     A = lambda z: 1e-16 * np.exp(-3.0 * (1.0 - z))
-    return [make_problem(int(s), 1536e3 / int(s), seed=seed + 31 * k, A=A, nz=nz, dtype=dtype)
+    return [add_diva_state(make_problem(int(s), 1536e3 / int(s), seed=seed + 31 * k, A=A, nz=nz,
+                                        dtype=dtype))
             for k, s in enumerate(sides)]
