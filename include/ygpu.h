@@ -194,6 +194,36 @@ typedef struct ygpu_stats {
 int ygpu_stats_enable(ygpu_handle* h, int32_t on);
 int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out);  /* returns and resets     */
 
+/* ---- served solve sequences with continuous batching (no counterpart in
+ * the reference, which solves one domain per call; these are the submit /
+ * poll entry points of a request loop).  A server holds max_slots member
+ * slots and one device arena sized by hbm_budget_bytes (domains up to
+ * max_side x max_side, up to nz_max levels).  ygpu_submit copies one solve's
+ * arrays and queues it under a caller-chosen sequence id: the first solve of
+ * a sequence carries its INOUT state; solves flagged YGPU_SEQ_CONTINUE read
+ * only the IN fields (and masks) and start from the previous solve's
+ * device-resident velocity, viscosity and friction; YGPU_SEQ_LAST releases
+ * the sequence's slot once that solve finishes.  ygpu_server_step admits
+ * queued solves as slots and arena cells free up (by device footprint) and
+ * runs one Picard iteration of every admitted solve in shared launches;
+ * finished solves are downloaded at once.  ygpu_poll returns 1 and the next
+ * finished ticket (0 when none); ygpu_fetch copies that solve's results into
+ * out (same grid; solve_seconds = time from admission to completion).  The
+ * host arrays passed to ygpu_submit may be reused as soon as it returns.
+ * One server per handle stream; calls on one server are not thread-safe.   */
+#define YGPU_SEQ_CONTINUE 1
+#define YGPU_SEQ_LAST 2
+typedef struct ygpu_server ygpu_server;
+int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva, int32_t max_slots,
+                       int32_t max_side, int32_t nz_max, int64_t hbm_budget_bytes,
+                       ygpu_server** out, int64_t* device_bytes);
+int ygpu_submit(ygpu_server* s, const ygpu_ssa_problem* prob, int64_t seq, int32_t flags,
+                int64_t* ticket);
+int ygpu_server_step(ygpu_server* s, int32_t* n_running, int32_t* n_queued);
+int ygpu_poll(ygpu_server* s, int64_t* ticket);
+int ygpu_fetch(ygpu_server* s, int64_t ticket, ygpu_ssa_problem* out);
+int ygpu_server_destroy(ygpu_server* s);
+
 /* device bytes a domain of this size needs (admission control)               */
 int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny, int32_t nz);
 

@@ -64,7 +64,11 @@ EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_default_param
            "ygpu_velocity_diva_batch", "ygpu_batch_create", "ygpu_batch_run",
            "ygpu_batch_download", "ygpu_batch_destroy", "ygpu_session_create", "ygpu_session_solve",
            "ygpu_session_destroy", "ygpu_stats_enable", "ygpu_stats_get",
-           "ygpu_footprint_bytes")
+           "ygpu_server_create", "ygpu_submit", "ygpu_server_step", "ygpu_poll", "ygpu_fetch",
+           "ygpu_server_destroy", "ygpu_footprint_bytes")
+
+YGPU_SEQ_CONTINUE = 1
+YGPU_SEQ_LAST = 2
 
 
 class StatsC(C.Structure):
@@ -119,6 +123,19 @@ def load() -> C.CDLL:
     lib.ygpu_stats_enable.restype = C.c_int
     lib.ygpu_stats_get.argtypes = [PP, C.POINTER(StatsC)]
     lib.ygpu_stats_get.restype = C.c_int
+    lib.ygpu_server_create.argtypes = [PP, C.POINTER(SsaParamsC), I, I, I, I, C.c_int64,
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
+    lib.ygpu_server_create.restype = C.c_int
+    lib.ygpu_submit.argtypes = [PP, C.POINTER(SsaProblemC), C.c_int64, I, C.POINTER(C.c_int64)]
+    lib.ygpu_submit.restype = C.c_int
+    lib.ygpu_server_step.argtypes = [PP, C.POINTER(I), C.POINTER(I)]
+    lib.ygpu_server_step.restype = C.c_int
+    lib.ygpu_poll.argtypes = [PP, C.POINTER(C.c_int64)]
+    lib.ygpu_poll.restype = C.c_int
+    lib.ygpu_fetch.argtypes = [PP, C.c_int64, C.POINTER(SsaProblemC)]
+    lib.ygpu_fetch.restype = C.c_int
+    lib.ygpu_server_destroy.argtypes = [PP]
+    lib.ygpu_server_destroy.restype = C.c_int
     lib.ygpu_footprint_bytes.argtypes = [C.POINTER(SsaParamsC), I, I, I]
     lib.ygpu_footprint_bytes.restype = C.c_int64
     _lib = lib

@@ -1,46 +1,8 @@
-// ygpu_api.hip -- host side of libygpu.so: handles, device residency of a
-// batch of domains, the Picard loop driver and the C-ABI entry points.
-#include <hip/hip_runtime.h>
+// ygpu_api.hip -- host side of libygpu.so: option parsing, the one-shot,
+// batch and session entry points of the C-ABI and the CSR solve.
+#include "ygpu_host.h"
 
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "../../include/ygpu.h"
-#include "ygpu_diva.h"
-#include "ygpu_fields.h"
-#include "ygpu_krylov.h"
-
-struct ygpu_handle {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  bool stats_on = false;
-  ygpu_stats stats{};
-};
-
-namespace {
-
-#define YG_CHECK(h, call)                                                          \
-  do {                                                                             \
-    hipError_t e_ = (call);                                                        \
-    if (e_ != hipSuccess) {                                                        \
-      (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
-      return YGPU_ERR_DEVICE;                                                      \
-    }                                                                              \
-  } while (0)
-
-// ---- LIS option string (solver_linear.F90:350) ----------------------------
-struct Opts {
-  YgKrylovOpts k;
-  int ny_hint = 0;
-  int prec = -1;  // -1 unset, 0 fp64, 2 mixed
-};
+namespace yg {
 
 int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
@@ -127,485 +89,11 @@ int bc_code(int bnd) {
   return -1;
 }
 
-// ---- device allocations of one batch --------------------------------------
-struct Arena {
-  std::vector<void*> ptrs;
-  size_t bytes = 0;
-  ~Arena() { for (void* p : ptrs) (void)hipFree(p); }
-  template <typename T>
-  hipError_t alloc(T** p, size_t n) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-    if (e == hipSuccess) { ptrs.push_back(*p); bytes += n * sizeof(T); }
-    return e;
-  }
-};
+}  // namespace yg
 
-template <typename R>
-struct Batch {
-  ygpu_handle* h = nullptr;
-  Arena ar;
-  int nmem = 0, diva = 0, prec = 0, kmode = 0, block = 1024;
-  long long total_cells = 0, total_z = 0, total_v = 0;
-  int zstride = 0;
-  std::vector<YgMember> mem;
-  std::vector<long long> coffs, zoffs;
-  std::vector<int> soff;
-  YgFieldParams fp{};
-  Opts opts;
-  int iter_max = 1;
-  // device
-  YgMember* d_mem = nullptr;
-  long long *d_coffs = nullptr, *d_zoffs = nullptr;
-  int* d_soff = nullptr;
-  uint32_t* d_mask = nullptr;
-  double *d_A64 = nullptr, *d_b = nullptr, *d_x = nullptr, *d_F64 = nullptr, *d_W64 = nullptr;
-  float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
-  int *d_done = nullptr, *d_iter_now = nullptr, *d_lin_iter = nullptr, *d_lin_fail = nullptr,
-      *d_lim_n = nullptr, *d_kit = nullptr, *d_kst = nullptr;
-  double *d_resid = nullptr, *d_kres = nullptr;
-  YgFields<R> f{};
-  YgDivaFields<R> dv{};
+using namespace yg;
 
-  YgBatchGeom geom() const {
-    YgBatchGeom G;
-    G.members = d_mem; G.coffs = d_coffs; G.zoffs = d_zoffs; G.soff = d_soff;
-    G.nmem = nmem; G.total_cells = total_cells;
-    return G;
-  }
-
-  int setup(ygpu_handle* hh, const ygpu_ssa_params* par, int np, const ygpu_ssa_problem* probs,
-            int is_diva) {
-    h = hh;
-    nmem = np;
-    diva = is_diva;
-    std::string e;
-    int rc = parse_opts(par->ssa_lis_opt, opts, e);
-    if (rc) { h->err = e; return rc; }
-    prec = par->precision;
-    kmode = prec == YGPU_PREC_MIXED ? YG_MODE_MIXED : YG_MODE_FP64;
-    if (par->ssa_solver != 0) { h->err = "ssa_solver \"energy\" is not available"; return YGPU_ERR_UNSUPPORTED; }
-    const int BC = bc_code(par->boundaries);
-    if (BC < 0) { h->err = "unknown boundaries code"; return YGPU_ERR_ARG; }
-    if (opts.k.precon == 2 && (BC == YG_BC_PERIODIC || BC == YG_BC_PERIODIC_X ||
-                               BC == YG_BC_PERIODIC_Y || BC == YG_BC_MISMIP3D || BC == YG_BC_TROUGH)) {
-      h->err = "ILU(0) needs a non-periodic domain (use -p jacobi)";
-      return YGPU_ERR_UNSUPPORTED;
-    }
-    fp.bnd = par->boundaries; fp.BC = BC;
-    fp.visc_method = par->visc_method; fp.beta_method = par->beta_method;
-    fp.beta_gl_scale = par->beta_gl_scale; fp.beta_gl_stag = par->beta_gl_stag;
-    fp.vel_lim_method = par->ssa_vel_lim_method; fp.diva = is_diva; fp.no_slip = par->no_slip;
-    fp.visc_const = par->visc_const; fp.beta_const = par->beta_const; fp.beta_q = par->beta_q;
-    fp.beta_u0 = par->beta_u0; fp.beta_gl_f = par->beta_gl_f; fp.H_grnd_lim = par->H_grnd_lim;
-    fp.beta_min = par->beta_min; fp.eps_0 = par->eps_0; fp.vel_lim_tau = par->ssa_vel_lim_tau;
-    fp.vel_max = par->ssa_vel_max; fp.iter_rel = par->ssa_iter_rel; fp.iter_conv = par->ssa_iter_conv;
-    if (par->visc_method < 0 || par->visc_method > 2 || par->beta_method < -1 || par->beta_method > 5 ||
-        !(par->beta_gl_scale == 0 || par->beta_gl_scale == 1 || par->beta_gl_scale == 3) ||
-        par->beta_gl_stag < -1 || par->beta_gl_stag > 2) {
-      h->err = "parameter value not supported";
-      return YGPU_ERR_UNSUPPORTED;
-    }
-    iter_max = par->ssa_iter_max;
-    if (iter_max < 1) { h->err = "ssa_iter_max < 1"; return YGPU_ERR_ARG; }
-    mem.resize(np);
-    coffs.assign(np + 1, 0);
-    zoffs.assign(np + 1, 0);
-    soff.clear();
-    int maxnx = 0;
-    for (int k = 0; k < np; ++k) {
-      const ygpu_ssa_problem& P = probs[k];
-      if (P.nx < 3 || P.ny < 3 || P.nz < 2 || P.nx > YG_MAX_LINES || P.ny > 65535 ||
-          (P.real_bytes != 4 && P.real_bytes != 8)) {
-        h->err = "domain " + std::to_string(k) + ": unsupported size or real_bytes";
-        return YGPU_ERR_ARG;
-      }
-      YgMember& M = mem[k];
-      M.nx = P.nx; M.ny = P.ny; M.nz = P.nz;
-      std::vector<int> st;
-      int T;
-      step_table(P.nx, P.ny, st, T);
-      if (T > YG_MAX_STEPS) { h->err = "domain too large for the step table"; return YGPU_ERR_ARG; }
-      M.T = T;
-      M.maxcnt = 0;
-      for (int t = 0; t < T; ++t) M.maxcnt = std::max(M.maxcnt, st[t + 1] - st[t]);
-      if (M.maxcnt > YG_MAX_STEP_CELLS) {
-        h->err = "domain " + std::to_string(k) + ": a wavefront step holds more cells than the LDS stage";
-        return YGPU_ERR_ARG;
-      }
-      M.coff = coffs[k];
-      M.toff = (long long)soff.size();
-      M.dx = P.dx; M.dy = P.dy;
-      soff.insert(soff.end(), st.begin(), st.end());
-      M.vlen = (int)(((2LL * P.nx * P.ny + 3) / 4) * 4);
-      M.woff = total_v;
-      total_v += M.vlen;
-      coffs[k + 1] = coffs[k] + (long long)P.nx * P.ny;
-      zoffs[k + 1] = zoffs[k] + (long long)P.nx * P.ny * P.nz;
-      zstride = std::max(zstride, P.nz);
-      maxnx = std::max(maxnx, P.nx);
-      fp.n_glen = P.n_glen;
-    }
-    total_cells = coffs[np];
-    total_z = zoffs[np];
-    block = par->block > 0 ? par->block : (maxnx <= 512 ? 512 : 1024);
-    block = std::max(block, ((maxnx + 63) / 64) * 64);
-    block = std::min(block, 1024);
-    return allocate();
-  }
-
-  int allocate() {
-    const long long C = total_cells, Z = total_z;
-    YG_CHECK(h, ar.alloc(&d_mem, nmem));
-    YG_CHECK(h, ar.alloc(&d_coffs, nmem + 1));
-    YG_CHECK(h, ar.alloc(&d_zoffs, nmem + 1));
-    YG_CHECK(h, ar.alloc(&d_soff, soff.size()));
-    YG_CHECK(h, ar.alloc(&d_mask, C));
-    YG_CHECK(h, ar.alloc(&d_A64, (size_t)YG_NA * C));
-    YG_CHECK(h, ar.alloc(&d_b, 2 * C));
-    YG_CHECK(h, ar.alloc(&d_x, 2 * C));
-    YG_CHECK(h, ar.alloc(&d_W64, (size_t)YG_NW64 * total_v));
-    if (kmode == YG_MODE_MIXED) {
-      YG_CHECK(h, ar.alloc(&d_A32, (size_t)YG_NA * C));
-      YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * C));
-      YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * total_v));
-    } else {
-      YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * C));
-    }
-    int* ints[7];
-    for (int q = 0; q < 7; ++q) YG_CHECK(h, ar.alloc(&ints[q], nmem));
-    d_done = ints[0]; d_iter_now = ints[1]; d_lin_iter = ints[2]; d_lin_fail = ints[3];
-    d_lim_n = ints[4]; d_kit = ints[5]; d_kst = ints[6];
-    YG_CHECK(h, ar.alloc(&d_resid, nmem));
-    YG_CHECK(h, ar.alloc(&d_kres, nmem));
-    // fields
-    R** r2d[] = {(R**)&f.c_bed, (R**)&f.f_slide, (R**)&f.taud_acx, (R**)&f.taud_acy,
-                 (R**)&f.taul_acx, (R**)&f.taul_acy, (R**)&f.H_ice, (R**)&f.f_ice,
-                 (R**)&f.H_grnd, (R**)&f.f_grnd, (R**)&f.z_bed, (R**)&f.z_sl, &f.ux, &f.uy,
-                 &f.ux_nm1, &f.uy_nm1, &f.visc_int, &f.beta, &f.beta_acx, &f.beta_acy, &f.bss_x,
-                 &f.bss_y, &f.lkx, &f.lky, &f.lrx, &f.lry, &f.dudx, &f.dudy, &f.dvdx, &f.dvdy,
-                 &f.err_x, &f.err_y, &f.taub_x, &f.taub_y};
-    for (R** p : r2d) YG_CHECK(h, ar.alloc(p, C));
-    YG_CHECK(h, ar.alloc((R**)&f.ATT, Z));
-    YG_CHECK(h, ar.alloc(&f.visc, Z));
-    YG_CHECK(h, ar.alloc(&f.visc_nm1, Z));
-    YG_CHECK(h, ar.alloc((R**)&f.zeta, (size_t)zstride * nmem));
-    f.zstride = zstride;
-    YG_CHECK(h, ar.alloc((int32_t**)&f.mx, C));
-    YG_CHECK(h, ar.alloc((int32_t**)&f.my, C));
-    if (diva) {
-      R** d2[] = {&f.F2, &f.F2_acx, &f.F2_acy, &f.beff_acx, &f.beff_acy, &f.beta_eff, &f.ubar_x,
-                  &f.ubar_y};
-      for (R** p : d2) YG_CHECK(h, ar.alloc(p, C));
-      YG_CHECK(h, ar.alloc(&f.duxdz, Z));
-      YG_CHECK(h, ar.alloc(&f.duydz, Z));
-      YG_CHECK(h, ar.alloc(&dv.ux3, Z));
-      YG_CHECK(h, ar.alloc(&dv.uy3, Z));
-    }
-    // geometry tables
-    std::vector<long long> mc(coffs.begin(), coffs.end()), mz(zoffs.begin(), zoffs.end());
-    YG_CHECK(h, hipMemcpyAsync(d_mem, mem.data(), sizeof(YgMember) * nmem, hipMemcpyHostToDevice, h->stream));
-    YG_CHECK(h, hipMemcpyAsync(d_coffs, mc.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
-    YG_CHECK(h, hipMemcpyAsync(d_zoffs, mz.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
-    YG_CHECK(h, hipMemcpyAsync(d_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, h->stream));
-    YG_CHECK(h, hipStreamSynchronize(h->stream));
-    return YGPU_OK;
-  }
-
-  // pack one field of every domain (host float or double) into R and upload
-  int up2(R* dst, const void* const* src, int nz_mult, bool may_null, R fill = R(0)) {
-    const long long n = nz_mult ? total_z : total_cells;
-    std::vector<R> st(n);
-    for (int k = 0; k < nmem; ++k) {
-      const long long off = nz_mult ? zoffs[k] : coffs[k];
-      const long long cnt = (nz_mult ? zoffs[k + 1] : coffs[k + 1]) - off;
-      const void* s = src[k];
-      if (!s) {
-        if (!may_null) { h->err = "missing input array"; return YGPU_ERR_ARG; }
-        for (long long q = 0; q < cnt; ++q) st[off + q] = fill;
-      } else if (rb[k] == 4) {
-        const float* p = (const float*)s;
-        for (long long q = 0; q < cnt; ++q) st[off + q] = (R)p[q];
-      } else {
-        const double* p = (const double*)s;
-        for (long long q = 0; q < cnt; ++q) st[off + q] = (R)p[q];
-      }
-    }
-    YG_CHECK(h, hipMemcpyAsync(dst, st.data(), sizeof(R) * n, hipMemcpyHostToDevice, h->stream));
-    YG_CHECK(h, hipStreamSynchronize(h->stream));
-    return YGPU_OK;
-  }
-
-  int down2(const R* src, void* const* dst, int nz_mult) {
-    const long long n = nz_mult ? total_z : total_cells;
-    std::vector<R> st(n);
-    YG_CHECK(h, hipMemcpyAsync(st.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost, h->stream));
-    YG_CHECK(h, hipStreamSynchronize(h->stream));
-    for (int k = 0; k < nmem; ++k) {
-      const long long off = nz_mult ? zoffs[k] : coffs[k];
-      const long long cnt = (nz_mult ? zoffs[k + 1] : coffs[k + 1]) - off;
-      void* d = dst[k];
-      if (!d) continue;
-      if (rb[k] == 4) {
-        float* p = (float*)d;
-        for (long long q = 0; q < cnt; ++q) p[q] = (float)st[off + q];
-      } else {
-        double* p = (double*)d;
-        for (long long q = 0; q < cnt; ++q) p[q] = (double)st[off + q];
-      }
-    }
-    return YGPU_OK;
-  }
-
-  std::vector<int> rb;
-
-#define YG_FIELD(member) [&] { std::vector<const void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = probs[k].member; return v; }()
-#define YG_FIELDW(member) [&] { std::vector<void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = probs[k].member; return v; }()
-
-  int upload_inputs(const ygpu_ssa_problem* probs) {
-    rb.resize(nmem);
-    for (int k = 0; k < nmem; ++k) rb[k] = probs[k].real_bytes;
-    int rc;
-#define UP(dst, member) if ((rc = up2((R*)(dst), YG_FIELD(member).data(), 0, false))) return rc;
-    UP(f.c_bed, c_bed) UP(f.f_slide, f_slide) UP(f.taud_acx, taud_acx) UP(f.taud_acy, taud_acy)
-    UP(f.taul_acx, taul_int_acx) UP(f.taul_acy, taul_int_acy) UP(f.H_ice, H_ice) UP(f.f_ice, f_ice)
-    UP(f.H_grnd, H_grnd) UP(f.f_grnd, f_grnd) UP(f.z_bed, z_bed) UP(f.z_sl, z_sl)
-#undef UP
-    if ((rc = up2((R*)f.ATT, YG_FIELD(ATT).data(), 1, false))) return rc;
-    // zeta: nz per member, padded to zstride
-    std::vector<R> zs((size_t)zstride * nmem, R(0));
-    for (int k = 0; k < nmem; ++k) {
-      if (!probs[k].zeta_aa) { h->err = "missing zeta_aa"; return YGPU_ERR_ARG; }
-      for (int q = 0; q < probs[k].nz; ++q)
-        zs[(size_t)k * zstride + q] = rb[k] == 4 ? (R)((const float*)probs[k].zeta_aa)[q]
-                                                 : (R)((const double*)probs[k].zeta_aa)[q];
-    }
-    YG_CHECK(h, hipMemcpyAsync((R*)f.zeta, zs.data(), sizeof(R) * zs.size(), hipMemcpyHostToDevice, h->stream));
-    // masks
-    std::vector<int32_t> mxs(total_cells), mys(total_cells);
-    for (int k = 0; k < nmem; ++k) {
-      if (!probs[k].ssa_mask_acx || !probs[k].ssa_mask_acy) { h->err = "missing ssa masks"; return YGPU_ERR_ARG; }
-      memcpy(&mxs[coffs[k]], probs[k].ssa_mask_acx, sizeof(int32_t) * (coffs[k + 1] - coffs[k]));
-      memcpy(&mys[coffs[k]], probs[k].ssa_mask_acy, sizeof(int32_t) * (coffs[k + 1] - coffs[k]));
-    }
-    YG_CHECK(h, hipMemcpyAsync((int32_t*)f.mx, mxs.data(), sizeof(int32_t) * total_cells, hipMemcpyHostToDevice, h->stream));
-    YG_CHECK(h, hipMemcpyAsync((int32_t*)f.my, mys.data(), sizeof(int32_t) * total_cells, hipMemcpyHostToDevice, h->stream));
-    YG_CHECK(h, hipStreamSynchronize(h->stream));
-    return YGPU_OK;
-  }
-
-  int upload_state(const ygpu_ssa_problem* probs) {
-    int rc;
-    if (diva) {
-      if ((rc = up2(f.ubar_x, YG_FIELD(ux_bar).data(), 0, false))) return rc;
-      if ((rc = up2(f.ubar_y, YG_FIELD(uy_bar).data(), 0, false))) return rc;
-      if ((rc = up2(f.duxdz, YG_FIELD(duxdz).data(), 1, true))) return rc;
-      if ((rc = up2(f.duydz, YG_FIELD(duydz).data(), 1, true))) return rc;
-    }
-    if ((rc = up2(f.ux, YG_FIELD(ux_b).data(), 0, false))) return rc;
-    if ((rc = up2(f.uy, YG_FIELD(uy_b).data(), 0, false))) return rc;
-    if ((rc = up2(f.taub_x, YG_FIELD(taub_acx).data(), 0, true))) return rc;
-    if ((rc = up2(f.taub_y, YG_FIELD(taub_acy).data(), 0, true))) return rc;
-    if ((rc = up2(f.visc, YG_FIELD(visc_eff).data(), 1, false))) return rc;
-    if ((rc = up2(f.beta, YG_FIELD(beta).data(), 0, true))) return rc;
-    if ((rc = up2(f.beta_acx, YG_FIELD(beta_acx).data(), 0, true))) return rc;
-    if ((rc = up2(f.beta_acy, YG_FIELD(beta_acy).data(), 0, true))) return rc;
-    return YGPU_OK;
-  }
-
-  int download(ygpu_ssa_problem* probs) {
-    int rc;
-    if (diva) {
-      if ((rc = down2(f.ubar_x, YG_FIELDW(ux_bar).data(), 0))) return rc;
-      if ((rc = down2(f.ubar_y, YG_FIELDW(uy_bar).data(), 0))) return rc;
-      if ((rc = down2(f.duxdz, YG_FIELDW(duxdz).data(), 1))) return rc;
-      if ((rc = down2(f.duydz, YG_FIELDW(duydz).data(), 1))) return rc;
-      if ((rc = down2(f.beta_eff, YG_FIELDW(beta_eff).data(), 0))) return rc;
-      if ((rc = down2(dv.ux3, YG_FIELDW(ux).data(), 1))) return rc;
-      if ((rc = down2(dv.uy3, YG_FIELDW(uy).data(), 1))) return rc;
-      if ((rc = diva_down_interior(probs))) return rc;
-    }
-    if ((rc = down2(f.ux, YG_FIELDW(ux_b).data(), 0))) return rc;
-    if ((rc = down2(f.uy, YG_FIELDW(uy_b).data(), 0))) return rc;
-    if ((rc = down2(f.taub_x, YG_FIELDW(taub_acx).data(), 0))) return rc;
-    if ((rc = down2(f.taub_y, YG_FIELDW(taub_acy).data(), 0))) return rc;
-    if ((rc = down2(f.visc, YG_FIELDW(visc_eff).data(), 1))) return rc;
-    if ((rc = down2(f.visc_int, YG_FIELDW(visc_eff_int).data(), 0))) return rc;
-    if ((rc = down2(f.err_x, YG_FIELDW(ssa_err_acx).data(), 0))) return rc;
-    if ((rc = down2(f.err_y, YG_FIELDW(ssa_err_acy).data(), 0))) return rc;
-    if ((rc = down2(f.beta, YG_FIELDW(beta).data(), 0))) return rc;
-    if ((rc = down2(f.beta_acx, YG_FIELDW(beta_acx).data(), 0))) return rc;
-    if ((rc = down2(f.beta_acy, YG_FIELDW(beta_acy).data(), 0))) return rc;
-    std::vector<int> it(nmem), li(nmem), lf(nmem), ln(nmem), dn(nmem);
-    std::vector<double> rs(nmem);
-    YG_CHECK(h, hipMemcpy(it.data(), d_iter_now, sizeof(int) * nmem, hipMemcpyDeviceToHost));
-    YG_CHECK(h, hipMemcpy(li.data(), d_lin_iter, sizeof(int) * nmem, hipMemcpyDeviceToHost));
-    YG_CHECK(h, hipMemcpy(lf.data(), d_lin_fail, sizeof(int) * nmem, hipMemcpyDeviceToHost));
-    YG_CHECK(h, hipMemcpy(ln.data(), d_lim_n, sizeof(int) * nmem, hipMemcpyDeviceToHost));
-    YG_CHECK(h, hipMemcpy(dn.data(), d_done, sizeof(int) * nmem, hipMemcpyDeviceToHost));
-    YG_CHECK(h, hipMemcpy(rs.data(), d_resid, sizeof(double) * nmem, hipMemcpyDeviceToHost));
-    for (int k = 0; k < nmem; ++k) {
-      probs[k].ssa_iter_now = it[k];
-      probs[k].ssa_lin_iter = li[k];
-      probs[k].ssa_lin_fail = lf[k];
-      probs[k].ssa_lim_n = ln[k];
-      probs[k].picard_resid = rs[k];
-      probs[k].status = dn[k] < 0 ? YGPU_ERR_NONFINITE : YGPU_OK;
-    }
-    return YGPU_OK;
-  }
-
-  int diva_down_interior(ygpu_ssa_problem* probs);
-
-  // device copy of the initial INOUT state, so a resident batch can be
-  // solved again from the same start without touching the host
-  std::vector<std::pair<R*, R*>> backup;  // (live, saved)
-  int save_initial() {
-    std::vector<std::pair<R*, long long>> st = {
-        {f.ux, total_cells}, {f.uy, total_cells}, {f.visc, total_z}, {f.beta, total_cells},
-        {f.beta_acx, total_cells}, {f.beta_acy, total_cells}, {f.taub_x, total_cells},
-        {f.taub_y, total_cells}};
-    if (diva) {
-      st.push_back({f.ubar_x, total_cells});
-      st.push_back({f.ubar_y, total_cells});
-      st.push_back({f.duxdz, total_z});
-      st.push_back({f.duydz, total_z});
-    }
-    for (auto& e : st) {
-      R* saved;
-      YG_CHECK(h, ar.alloc(&saved, e.second));
-      YG_CHECK(h, hipMemcpyAsync(saved, e.first, sizeof(R) * e.second, hipMemcpyDeviceToDevice, h->stream));
-      backup.push_back({e.first, saved});
-      sizes.push_back(e.second);
-    }
-    YG_CHECK(h, hipStreamSynchronize(h->stream));
-    return YGPU_OK;
-  }
-  std::vector<long long> sizes;
-  int restore_initial() {
-    for (size_t q = 0; q < backup.size(); ++q)
-      YG_CHECK(h, hipMemcpyAsync(backup[q].first, backup[q].second, sizeof(R) * sizes[q],
-                                 hipMemcpyDeviceToDevice, h->stream));
-    return YGPU_OK;
-  }
-
-  // the Picard loop (velocity_ssa.f90:172-334 / velocity_diva.f90:195-404)
-  int run(double* seconds) {
-    hipStream_t st = h->stream;
-    YG_CHECK(h, hipMemsetAsync(d_done, 0, sizeof(int) * nmem, st));
-    YG_CHECK(h, hipMemsetAsync(d_lin_iter, 0, sizeof(int) * nmem, st));
-    YG_CHECK(h, hipMemsetAsync(d_lin_fail, 0, sizeof(int) * nmem, st));
-    YG_CHECK(h, hipMemsetAsync(d_iter_now, 0, sizeof(int) * nmem, st));
-    YG_CHECK(h, hipMemsetAsync(d_resid, 0, sizeof(double) * nmem, st));
-    hipEvent_t e0, e1;
-    YG_CHECK(h, hipEventCreate(&e0));
-    YG_CHECK(h, hipEventCreate(&e1));
-    YG_CHECK(h, hipEventRecord(e0, st));
-    const YgBatchGeom G = geom();
-    yg_launch_init<R>(G, f, fp, st);
-    if (diva) yg_launch_diva_init<R>(G, f, dv, fp, st);
-    YgPicardState ps{d_iter_now, d_done, d_resid, d_lin_iter, d_lin_fail, d_lim_n};
-    YgKrylovArgs ka{};
-    ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;
-    ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.b = d_b; ka.x = d_x;
-    ka.W64 = d_W64; ka.W32 = d_W32; ka.total_cells = total_cells;
-    ka.wrapx = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_X);
-    ka.wrapy = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_Y || fp.BC == YG_BC_MISMIP3D ||
-                fp.BC == YG_BC_TROUGH);
-    ka.skip = d_done; ka.out_iters = d_kit; ka.out_status = d_kst; ka.out_resid = d_kres;
-    ka.opts = opts.k;
-    long long* d_tm = nullptr;
-    const bool phase = getenv("YGPU_PHASE_TIMERS") != nullptr;
-    if (phase) {
-      YG_CHECK(h, hipMalloc(&d_tm, sizeof(long long) * 8 * nmem));
-      YG_CHECK(h, hipMemsetAsync(d_tm, 0, sizeof(long long) * 8 * nmem, st));
-      ka.timers = d_tm;
-    }
-    std::vector<int> done(nmem), kit(nmem);
-    hipEvent_t k0 = nullptr, k1 = nullptr;
-    if (h->stats_on) {
-      YG_CHECK(h, hipEventCreate(&k0));
-      YG_CHECK(h, hipEventCreate(&k1));
-    }
-    for (int iter = 1; iter <= iter_max; ++iter) {
-      if (diva) yg_launch_diva_pre<R>(G, f, dv, fp, d_done, st);
-      yg_launch_fields_pre<R>(G, f, fp, d_done, iter == 1, st);
-      if (diva) yg_launch_diva_mid<R>(G, f, dv, fp, d_done, st);
-      yg_launch_assemble<R>(G, f, fp, d_done, d_A64, kmode == YG_MODE_MIXED ? d_A32 : nullptr,
-                            d_b, d_x, d_mask, st);
-      std::vector<int> active;
-      if (h->stats_on) {
-        for (int k = 0; k < nmem; ++k) if (!done[k]) active.push_back(k);
-        YG_CHECK(h, hipEventRecord(k0, st));
-      }
-      yg_launch_krylov(ka, kmode, nmem, block, st);
-      if (h->stats_on) YG_CHECK(h, hipEventRecord(k1, st));
-      yg_launch_epilogue<R>(G, f, fp, d_x, d_kit, d_kst, iter, iter_max, ps, st);
-      if (diva) yg_launch_diva_post_iter<R>(G, f, fp, d_iter_now, iter, st);
-      YG_CHECK(h, hipGetLastError());
-      YG_CHECK(h, hipMemcpyAsync(done.data(), d_done, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
-      if (h->stats_on)
-        YG_CHECK(h, hipMemcpyAsync(kit.data(), d_kit, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
-      YG_CHECK(h, hipStreamSynchronize(st));
-      if (h->stats_on) {
-        float kms = 0.f;
-        YG_CHECK(h, hipEventElapsedTime(&kms, k0, k1));
-        h->stats.krylov_launches += 1;
-        h->stats.krylov_ms += kms;
-        const bool mixed = kmode == YG_MODE_MIXED;
-        for (int k : active) {
-          const double C = (double)mem[k].nx * mem[k].ny;
-          const int it = kit[k];
-          h->stats.krylov_bytes += mixed ? krylov_bytes(C, it, 4, 4, 4, opts.k.precon == 2) +
-                                               4.0 * (18.0 * C * 8 + 8.0 * C + 6.0 * 2.0 * C * 8)
-                                         : krylov_bytes(C, it, 8, 8, 8, opts.k.precon == 2);
-          h->stats.krylov_iters += it;
-          h->stats.unknown_iters += 2.0 * C * it;
-          h->stats.picard_iters += 1;
-        }
-      }
-      bool all = true;
-      for (int k = 0; k < nmem; ++k) all &= done[k] != 0;
-      if (all) break;
-    }
-    if (k0) (void)hipEventDestroy(k0);
-    if (k1) (void)hipEventDestroy(k1);
-    if (phase) {
-      std::vector<long long> tmh(8 * nmem);
-      YG_CHECK(h, hipMemcpy(tmh.data(), d_tm, sizeof(long long) * 8 * nmem, hipMemcpyDeviceToHost));
-      (void)hipFree(d_tm);
-      int rate = 0;
-      (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, h->device);
-      const double us = rate > 0 ? 1e3 / rate : 1e-2;  // rate in kHz
-      fprintf(stderr, "ygpu phases member0 (ms): factor %.3f sweepL %.3f sweepU %.3f spmv %.3f total %.3f\n",
-              tmh[0] * us * 1e-3, tmh[1] * us * 1e-3, tmh[2] * us * 1e-3, tmh[3] * us * 1e-3,
-              tmh[7] * us * 1e-3);
-    }
-    yg_launch_finish<R>(G, f, fp, ps, st);
-    if (diva) yg_launch_diva_finish<R>(G, f, dv, fp, st);
-    YG_CHECK(h, hipGetLastError());
-    YG_CHECK(h, hipEventRecord(e1, st));
-    YG_CHECK(h, hipEventSynchronize(e1));
-    float ms = 0.f;
-    YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (seconds) *seconds = ms * 1e-3;
-    return YGPU_OK;
-  }
-};
-
-template <typename R>
-int Batch<R>::diva_down_interior(ygpu_ssa_problem* probs) {
-  // ux_i = ux - ux_b for every level (velocity_diva.f90:419-423), on the host
-  for (int k = 0; k < nmem; ++k) {
-    ygpu_ssa_problem& P = probs[k];
-    if (!P.ux_i || !P.uy_i || !P.ux || !P.uy) continue;
-    (void)P;
-  }
-  return YGPU_OK;
-}
+namespace {
 
 template <typename R>
 int run_batch(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_problem* probs, int diva) {

@@ -8,21 +8,11 @@ struct YgDivaFields {
   R* uy3;
 };
 
-template <typename R>
-void yg_launch_diva_init(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>& d,
-                         const YgFieldParams& p, hipStream_t st);
+// vertical shear from the basal stress of the previous iteration
 template <typename R>
 void yg_launch_diva_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>& d,
                         const YgFieldParams& p, const int* skip, hipStream_t st);
-template <typename R>
-void yg_launch_diva_mid(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>& d,
-                        const YgFieldParams& p, const int* skip, hipStream_t st);
-template <typename R>
-void yg_launch_diva_post(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>& d,
-                         const YgFieldParams& p, YgPicardState ps, hipStream_t st);
-template <typename R>
-void yg_launch_diva_post_iter(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                              const int* iter_now, int iter, hipStream_t st);
+// 3D velocity after the Picard loop; skip (may be null): members left untouched
 template <typename R>
 void yg_launch_diva_finish(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>& d,
-                           const YgFieldParams& p, hipStream_t st);
+                           const YgFieldParams& p, const int* skip, hipStream_t st);

@@ -1,22 +1,13 @@
 // ygpu_diva.hip -- the DIVA parts of the Picard iteration
 // (calc_velocity_diva, velocity_diva.f90:59-429): vertical shear from the
-// basal stress, basal stress and basal velocity after each solve, and the
-// 3D velocity after the loop.  The depth integral F2 and beta_eff are formed
-// inside the shared viscosity and staggering kernels.
+// basal stress, and the 3D velocity after the loop.  The depth integral F2
+// and beta_eff are formed inside the shared viscosity and staggering
+// kernels, the basal stress and basal velocity inside the Picard epilogue.
 #include "ygpu_diva.h"
 
 namespace {
 
 #define I2(i, j) ((i) + nx * (j))
-
-YG_DEV int dv_find_member(const long long* coffs, int nmem, long long gc) {
-  int lo = 0, hi = nmem - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (coffs[mid] <= gc) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
 
 YG_DEV void dv_nbr(int i, int j, int nx, int ny, int BC, int& ip1, int& jp1) {
   const bool wx = BC == YG_BC_PERIODIC || BC == YG_BC_PERIODIC_X;
@@ -39,12 +30,13 @@ template <typename R>
 __global__ void k_shear(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= G.total_cells) return;
-  const int m = dv_find_member(G.coffs, G.nmem, gc);
-  if (skip && skip[m]) return;
+  const int m = yg_member_at(G, gc);
+  if (m < 0 || (skip && skip[m])) return;
   const YgMember& mb = G.members[m];
   const int nx = mb.nx, ny = mb.ny, nz = mb.nz;
   const long long c0 = G.coffs[m], z0 = G.zoffs[m], nc = (long long)nx * ny;
-  const int loc = (int)(gc - c0), i = loc % nx, j = loc / nx;
+  const int loc = (int)(gc - c0);
+  const int i = loc % nx, j = loc / nx;
   int ip1, jp1;
   dv_nbr(i, j, nx, ny, p.BC, ip1, jp1);
   const R* fi = f.f_ice + c0;
@@ -59,41 +51,19 @@ __global__ void k_shear(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int
   }
 }
 
-// calc_basal_stress (velocity_diva.f90:1011-1050) and calc_vel_basal
-// (:960-1009), for the members solved in this iteration
-template <typename R>
-__global__ void k_diva_post(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* iter_now,
-                            int iter) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gc >= G.total_cells) return;
-  const int m = dv_find_member(G.coffs, G.nmem, gc);
-  if (iter_now[m] != iter) return;
-  const R tol = R(1e-15);
-  R tx = f.beff_acx[gc] * f.ubar_x[gc];
-  if ((tx < R(0) ? -tx : tx) < tol) tx = R(0);
-  R ty = f.beff_acy[gc] * f.ubar_y[gc];
-  if ((ty < R(0) ? -ty : ty) < tol) ty = R(0);
-  f.taub_x[gc] = tx;
-  f.taub_y[gc] = ty;
-  if (p.no_slip) {
-    f.ux[gc] = R(0);
-    f.uy[gc] = R(0);
-  } else {
-    f.ux[gc] = f.ubar_x[gc] - tx * f.F2_acx[gc];
-    f.uy[gc] = f.ubar_y[gc] - ty * f.F2_acy[gc];
-  }
-}
-
 // calc_vel_horizontal_3D (velocity_diva.f90:431-508)
 template <typename R>
-__global__ void k_vel3d(YgBatchGeom G, YgFields<R> f, YgDivaFields<R> d, YgFieldParams p) {
+__global__ void k_vel3d(YgBatchGeom G, YgFields<R> f, YgDivaFields<R> d, YgFieldParams p,
+                        const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= G.total_cells) return;
-  const int m = dv_find_member(G.coffs, G.nmem, gc);
+  const int m = yg_member_at(G, gc);
+  if (m < 0 || (skip && skip[m])) return;
   const YgMember& mb = G.members[m];
   const int nx = mb.nx, ny = mb.ny, nz = mb.nz;
   const long long c0 = G.coffs[m], z0 = G.zoffs[m], nc = (long long)nx * ny;
-  const int loc = (int)(gc - c0), i = loc % nx, j = loc / nx;
+  const int loc = (int)(gc - c0);
+  const int i = loc % nx, j = loc / nx;
   int ip1, jp1;
   dv_nbr(i, j, nx, ny, p.BC, ip1, jp1);
   const R* fi = f.f_ice + c0;
@@ -128,54 +98,24 @@ __global__ void k_vel3d(YgBatchGeom G, YgFields<R> f, YgDivaFields<R> d, YgField
 static inline unsigned dnblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <typename R>
-void yg_launch_diva_init(const YgBatchGeom&, const YgFields<R>&, const YgDivaFields<R>&,
-                         const YgFieldParams&, hipStream_t) {}
-
-template <typename R>
 void yg_launch_diva_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>&,
                         const YgFieldParams& p, const int* skip, hipStream_t st) {
   hipLaunchKernelGGL(k_shear<R>, dim3(dnblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p, skip);
 }
 
 template <typename R>
-void yg_launch_diva_mid(const YgBatchGeom&, const YgFields<R>&, const YgDivaFields<R>&,
-                        const YgFieldParams&, const int*, hipStream_t) {}
-
-template <typename R>
-void yg_launch_diva_post(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>&,
-                         const YgFieldParams& p, YgPicardState ps, hipStream_t st) {
-  // the epilogue stamped iter_now = iter on the members solved in this iteration
-  (void)ps;
-}
-
-template <typename R>
-void yg_launch_diva_post_iter(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                              const int* iter_now, int iter, hipStream_t st) {
-  hipLaunchKernelGGL(k_diva_post<R>, dim3(dnblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p,
-                     iter_now, iter);
-}
-
-template <typename R>
 void yg_launch_diva_finish(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>& d,
-                           const YgFieldParams& p, hipStream_t st) {
-  hipLaunchKernelGGL(k_vel3d<R>, dim3(dnblk(G.total_cells, 256)), dim3(256), 0, st, G, f, d, p);
+                           const YgFieldParams& p, const int* skip, hipStream_t st) {
+  hipLaunchKernelGGL(k_vel3d<R>, dim3(dnblk(G.total_cells, 256)), dim3(256), 0, st, G, f, d, p,
+                     skip);
 }
 
 #define YG_DINST(R)                                                                              \
-  template void yg_launch_diva_init<R>(const YgBatchGeom&, const YgFields<R>&,                   \
-                                       const YgDivaFields<R>&, const YgFieldParams&, hipStream_t); \
   template void yg_launch_diva_pre<R>(const YgBatchGeom&, const YgFields<R>&,                    \
                                       const YgDivaFields<R>&, const YgFieldParams&, const int*,   \
                                       hipStream_t);                                              \
-  template void yg_launch_diva_mid<R>(const YgBatchGeom&, const YgFields<R>&,                    \
-                                      const YgDivaFields<R>&, const YgFieldParams&, const int*,   \
-                                      hipStream_t);                                              \
-  template void yg_launch_diva_post<R>(const YgBatchGeom&, const YgFields<R>&,                   \
-                                       const YgDivaFields<R>&, const YgFieldParams&,             \
-                                       YgPicardState, hipStream_t);                              \
-  template void yg_launch_diva_post_iter<R>(const YgBatchGeom&, const YgFields<R>&,              \
-                                            const YgFieldParams&, const int*, int, hipStream_t); \
   template void yg_launch_diva_finish<R>(const YgBatchGeom&, const YgFields<R>&,                 \
-                                         const YgDivaFields<R>&, const YgFieldParams&, hipStream_t);
+                                         const YgDivaFields<R>&, const YgFieldParams&,             \
+                                         const int*, hipStream_t);
 YG_DINST(float)
 YG_DINST(double)

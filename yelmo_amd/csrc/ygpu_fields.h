@@ -61,16 +61,36 @@ struct YgFields {
 
 struct YgBatchGeom {
   const YgMember* members;
-  const long long* coffs;     // member cell offsets, nmem+1 entries (device)
+  const long long* coffs;     // member cell offsets (device)
   const long long* zoffs;     // member 3D offsets
   const int* soff;            // step tables
   int nmem;
-  long long total_cells;
+  long long total_cells;      // per-cell kernels cover cells [0, total_cells)
+  // Occupied cell ranges in ascending order of first cell: seg[k] is the
+  // first cell of member segm[k] (segm null: member k).  A batch packs its
+  // members back to back (seg = coffs); a server's slots are placed by its
+  // cell allocator, so ranges can be out of member order and gaps can exist.
+  const long long* seg;
+  const int* segm;
+  int nseg;
 };
+
+// member owning cell gc, or -1 for a cell outside every member's domain
+YG_DEV int yg_member_at(const YgBatchGeom& G, long long gc) {
+  int lo = 0, hi = G.nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (G.seg[mid] <= gc) lo = mid; else hi = mid - 1;
+  }
+  if (G.nseg <= 0 || gc < G.seg[lo]) return -1;
+  const int m = G.segm ? G.segm[lo] : lo;
+  const YgMember& mb = G.members[m];
+  return gc - G.seg[lo] < (long long)mb.nx * mb.ny ? m : -1;
+}
 
 template <typename R>
 void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                          const int* skip, int first_iter, hipStream_t st);
+                          const int* skip, hipStream_t st);
 
 template <typename R>
 void yg_launch_assemble(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
@@ -86,16 +106,19 @@ struct YgPicardState {
   int* lim_n;
 };
 
+// One Picard epilogue per member not yet done; each member's iteration count
+// (iter_now) advances by one.
 template <typename R>
 void yg_launch_epilogue(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                        const double* x, const int* lin_iters, const int* lin_status, int iter,
+                        const double* x, const int* lin_iters, const int* lin_status,
                         int iter_max, YgPicardState ps, hipStream_t st);
 
+// skip (may be null): members left untouched
 template <typename R>
 void yg_launch_finish(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                      YgPicardState ps, hipStream_t st);
+                      YgPicardState ps, const int* skip, hipStream_t st);
 
 template <typename R>
 void yg_launch_init(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                    hipStream_t st);
+                    const int* skip, hipStream_t st);
 

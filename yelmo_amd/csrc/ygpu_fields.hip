@@ -31,15 +31,6 @@ template <typename R> YG_DEV R yabs(R a) { return a < R(0) ? -a : a; }
 // default-real literal of the reference (single precision), converted to R
 #define RS(x) ((R)(float)(x))
 
-YG_DEV int find_member(const long long* coffs, int nmem, long long gc) {
-  int lo = 0, hi = nmem - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (coffs[mid] <= gc) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
 // get_neighbor_indices_bc_codes (yelmo_tools.f90:118-201), 0-based
 YG_DEV void nbr(int i, int j, int nx, int ny, int BC, int& im1, int& ip1, int& jm1, int& jp1) {
   const bool wx = BC == YG_BC_PERIODIC || BC == YG_BC_PERIODIC_X;
@@ -60,8 +51,8 @@ struct CellCtx {
 
 YG_DEV bool cell_of(const YgBatchGeom& G, long long gc, const int* skip, CellCtx& cc) {
   if (gc >= G.total_cells) return false;
-  const int m = find_member(G.coffs, G.nmem, gc);
-  if (skip && skip[m]) return false;
+  const int m = yg_member_at(G, gc);
+  if (m < 0 || (skip && skip[m])) return false;
   const YgMember& mb = G.members[m];
   cc.m = m; cc.nx = mb.nx; cc.ny = mb.ny; cc.nz = mb.nz;
   cc.c0 = G.coffs[m]; cc.z0 = G.zoffs[m];
@@ -94,8 +85,7 @@ YG_DEV bool act_y(const R* fi, int nx, int ny, int BC, int i, int j) {
 }
 
 template <typename R>
-__global__ void k_strain(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
-                         int first_iter) {
+__global__ void k_strain(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
@@ -620,8 +610,8 @@ __global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const 
   // one thread per unknown pair in the reference's row order: n = i*ny + j
   const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gq >= G.total_cells) return;
-  const int m = find_member(G.coffs, G.nmem, gq);
-  if (skip && skip[m]) return;
+  const int m = yg_member_at(G, gq);
+  if (m < 0 || (skip && skip[m])) return;
   const YgMember& mb = G.members[m];
   const long long c0 = G.coffs[m];
   const int q = (int)(gq - c0);
@@ -772,13 +762,19 @@ __global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const 
 // linear_solver_save_velocity (solver_ssa_ac.f90:33-71), ssa_vel_clip
 // (:1069-1093), picard_relax_vel, picard_calc_convergence_l2 and _l1rel
 // (velocity_general.f90:1907-2186).
+// Each member counts its own Picard iterations (iter_now), so members that
+// entered a batch at different times (served requests) advance together.
+// For DIVA the basal stress and basal velocity of the member follow in the
+// same pass (calc_basal_stress, velocity_diva.f90:1011-1050; calc_vel_basal,
+// :960-1009).
 template <typename R>
 __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f, YgFieldParams p,
                                                    const double* x, const int* lin_iters,
-                                                   const int* lin_status, int iter, int iter_max,
+                                                   const int* lin_status, int iter_max,
                                                    YgPicardState ps) {
   const int m = blockIdx.x;
   if (ps.done[m]) return;
+  const int iter = ps.iter_now[m] + 1;
   const YgMember mb = G.members[m];
   const int nx = mb.nx, ny = mb.ny, ncell = nx * ny;
   const long long c0 = G.coffs[m];
@@ -828,6 +824,18 @@ __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f,
     if ((mx[c] > 0 && !isfinite((double)vx)) || (my[c] > 0 && !isfinite((double)vy))) acc[3] += 1.0;
     f.err_x[c0 + c] = yabs<R>(vx) > etol ? R(2) * yabs<R>(vx - uxp[c]) / yabs<R>(vx + uxp[c] + etol2) : R(0);
     f.err_y[c0 + c] = yabs<R>(vy) > etol ? R(2) * yabs<R>(vy - uyp[c]) / yabs<R>(vy + uyp[c] + etol2) : R(0);
+    if (p.diva) {
+      const R tol = (R)(float)1e-15;   // TOL_UNDERFLOW (yelmo_defs.f90:64)
+      R tx = f.beff_acx[c0 + c] * vx;
+      if (yabs<R>(tx) < tol) tx = R(0);
+      R ty = f.beff_acy[c0 + c] * vy;
+      if (yabs<R>(ty) < tol) ty = R(0);
+      f.taub_x[c0 + c] = tx;
+      f.taub_y[c0 + c] = ty;
+      // the basal velocity is the DIVA state's ux/uy (ux_b, uy_b)
+      f.ux[c0 + c] = p.no_slip ? R(0) : vx - tx * f.F2_acx[c0 + c];
+      f.uy[c0 + c] = p.no_slip ? R(0) : vy - ty * f.F2_acy[c0 + c];
+    }
   }
   yg_block_sum<4>(acc, red);
   if (threadIdx.x == 0) {
@@ -848,8 +856,9 @@ __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f,
 // calc_basal_stress (velocity_ssa.f90:588-617) ------------------------------
 template <typename R>
 __global__ __launch_bounds__(1024) void k_finish(YgBatchGeom G, YgFields<R> f, YgFieldParams p,
-                                                 YgPicardState ps) {
+                                                 YgPicardState ps, const int* skip) {
   const int m = blockIdx.x;
+  if (skip && skip[m]) return;
   const YgMember mb = G.members[m];
   const int nx = mb.nx, ny = mb.ny, ncell = nx * ny;
   const long long c0 = G.coffs[m];
@@ -891,10 +900,10 @@ __global__ __launch_bounds__(1024) void k_finish(YgBatchGeom G, YgFields<R> f, Y
 
 // ---- before the loop: set_inactive_margins (velocity_general.f90:1687-1750)
 template <typename R>
-__global__ void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p) {
+__global__ void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
-  if (!cell_of(G, gc, nullptr, cc)) return;
+  if (!cell_of(G, gc, skip, cc)) return;
   const int nx = cc.nx, ny = cc.ny, i = cc.i, j = cc.j;
   int im1, ip1, jm1, jp1;
   nbr(i, j, nx, ny, p.BC, im1, ip1, jm1, jp1);
@@ -918,15 +927,16 @@ __global__ void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p) {
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <typename R>
-void yg_launch_init(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p, hipStream_t st) {
-  hipLaunchKernelGGL(k_init<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p);
+void yg_launch_init(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
+                    const int* skip, hipStream_t st) {
+  hipLaunchKernelGGL(k_init<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p, skip);
 }
 
 template <typename R>
 void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                          const int* skip, int first_iter, hipStream_t st) {
+                          const int* skip, hipStream_t st) {
   const dim3 grid(nblk(G.total_cells, 256)), blk(256);
-  hipLaunchKernelGGL(k_strain<R>, grid, blk, 0, st, G, f, p, skip, first_iter);
+  hipLaunchKernelGGL(k_strain<R>, grid, blk, 0, st, G, f, p, skip);
   hipLaunchKernelGGL(k_visc<R>, grid, blk, 0, st, G, f, p, skip);
   hipLaunchKernelGGL(k_beta<R>, grid, blk, 0, st, G, f, p, skip);
   hipLaunchKernelGGL(k_stagger<R>, grid, blk, 0, st, G, f, p, skip);
@@ -942,31 +952,32 @@ void yg_launch_assemble(const YgBatchGeom& G, const YgFields<R>& f, const YgFiel
 
 template <typename R>
 void yg_launch_epilogue(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                        const double* x, const int* lin_iters, const int* lin_status, int iter,
+                        const double* x, const int* lin_iters, const int* lin_status,
                         int iter_max, YgPicardState ps, hipStream_t st) {
   hipLaunchKernelGGL(k_epilogue<R>, dim3(G.nmem), dim3(1024), 0, st, G, f, p, x, lin_iters,
-                     lin_status, iter, iter_max, ps);
+                     lin_status, iter_max, ps);
 }
 
 template <typename R>
 void yg_launch_finish(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
-                      YgPicardState ps, hipStream_t st) {
-  hipLaunchKernelGGL(k_finish<R>, dim3(G.nmem), dim3(1024), 0, st, G, f, p, ps);
+                      YgPicardState ps, const int* skip, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish<R>, dim3(G.nmem), dim3(1024), 0, st, G, f, p, ps, skip);
 }
 
 
 #define YG_INST(R)                                                                              \
   template void yg_launch_init<R>(const YgBatchGeom&, const YgFields<R>&, const YgFieldParams&, \
-                                  hipStream_t);                                                 \
+                                  const int*, hipStream_t);                                     \
   template void yg_launch_fields_pre<R>(const YgBatchGeom&, const YgFields<R>&,                 \
-                                        const YgFieldParams&, const int*, int, hipStream_t);     \
+                                        const YgFieldParams&, const int*, hipStream_t);     \
   template void yg_launch_assemble<R>(const YgBatchGeom&, const YgFields<R>&,                   \
                                       const YgFieldParams&, const int*, double*, float*,         \
                                       double*, double*, uint32_t*, hipStream_t);                 \
   template void yg_launch_epilogue<R>(const YgBatchGeom&, const YgFields<R>&,                   \
                                       const YgFieldParams&, const double*, const int*,           \
-                                      const int*, int, int, YgPicardState, hipStream_t);         \
+                                      const int*, int, YgPicardState, hipStream_t);              \
   template void yg_launch_finish<R>(const YgBatchGeom&, const YgFields<R>&,                     \
-                                    const YgFieldParams&, YgPicardState, hipStream_t);
+                                    const YgFieldParams&, YgPicardState, const int*,            \
+                                    hipStream_t);
 YG_INST(float)
 YG_INST(double)

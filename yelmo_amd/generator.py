@@ -71,16 +71,8 @@ class SsaProblem:
         return out
 
 
-def make_problem(nx: int, dx: float, seed: int = 0, A=1e-16, nz: int = 11,
-                 dtype=np.float32, cb_seed: int | None = None, sigma_ln: float = 1.0,
-                 lateral_bc: str = "all", boundaries: str = "zeros",
-                 H_noise_seed: int | None = None, ny: int | None = None,
-                 H_offset=None) -> SsaProblem:
-    """Build one problem.  ``A`` is a scalar or a callable of zeta giving the
-    rate factor on each level; ``cb_seed`` reseeds only the friction field
-    (ensemble members share geometry); ``H_offset`` adds a thickness change
-    (served sequences)."""
-    ny = nx if ny is None else ny
+def _geometry(nx, ny, dx, seed, cb_seed=None, sigma_ln=1.0, H_noise_seed=None):
+    """Thickness, bed and unmasked friction of one domain (the seeded part)."""
     rng_geo = np.random.default_rng(seed)
     xc = (np.arange(1, nx + 1) - (nx + 1) / 2.0) * dx
     yc = (np.arange(1, ny + 1) - (ny + 1) / 2.0) * dx
@@ -91,11 +83,19 @@ def make_problem(nx: int, dx: float, seed: int = 0, A=1e-16, nz: int = 11,
     noise = _smooth_noise(rng_h, nx, ny, 40e3, dx)
     noise *= 50.0 / max(noise.std(), 1e-12)
     H = np.where(H > 0, np.maximum(H + noise, 0.0), 0.0)
-    if H_offset is not None:
-        H = np.where(H > 0, np.maximum(H + H_offset, 0.0), 0.0)
     s = _smooth_noise(rng_geo, nx, ny, 150e3, dx)
     s /= max(np.abs(s).max(), 1e-12)
     z_bed = -200.0 + 500.0 * s
+    rng_cb = rng_geo if cb_seed is None else np.random.default_rng(cb_seed)
+    gn = _smooth_noise(rng_cb, nx, ny, 64e3, dx)
+    gn /= gn.std() if gn.std() > 1e-6 else 1.0
+    c_raw = 1e4 * np.exp(sigma_ln * gn)
+    return H, z_bed, c_raw
+
+
+def _derive(nx, ny, dx, H, z_bed, c_raw, A, nz, dtype, lateral_bc, boundaries) -> SsaProblem:
+    """Every input that follows from thickness, bed and friction, with the
+    zero initial state."""
     z_sl = np.zeros_like(H)
     f_ice = np.where(H > 0, 1.0, 0.0)
     H_grnd = inputs.calc_H_grnd(H, f_ice, z_bed, z_sl, RHO_ICE, RHO_SW)
@@ -108,11 +108,7 @@ def make_problem(nx: int, dx: float, seed: int = 0, A=1e-16, nz: int = 11,
                                                     RHO_ICE, G, boundaries)
     taul_acx, taul_acy = inputs.calc_lateral_bc_stress_2D(mask_frnt, H, f_ice, z_srf, z_sl,
                                                           RHO_ICE, RHO_SW, G, boundaries)
-    rng_cb = rng_geo if cb_seed is None else np.random.default_rng(cb_seed)
-    gn = _smooth_noise(rng_cb, nx, ny, 64e3, dx)
-    gn /= gn.std() if gn.std() > 1e-6 else 1.0
-    c_bed = 1e4 * np.exp(sigma_ln * gn)
-    c_bed = np.where(f_grnd > 0, c_bed, 0.0)
+    c_bed = np.where(f_grnd > 0, c_raw, 0.0)
     zeta = inputs.calc_zeta(nz, "exp", 2.0, dtype=np.float64)
     if callable(A):
         Az = np.array([A(z) for z in zeta])
@@ -142,6 +138,22 @@ def make_problem(nx: int, dx: float, seed: int = 0, A=1e-16, nz: int = 11,
         ssa_err_acx=F(z2), ssa_err_acy=F(z2),
     )
     return SsaProblem(nx, ny, nz, float(dx), float(dx), 3.0, fields, state)
+
+
+def make_problem(nx: int, dx: float, seed: int = 0, A=1e-16, nz: int = 11,
+                 dtype=np.float32, cb_seed: int | None = None, sigma_ln: float = 1.0,
+                 lateral_bc: str = "all", boundaries: str = "zeros",
+                 H_noise_seed: int | None = None, ny: int | None = None,
+                 H_offset=None) -> SsaProblem:
+    """Build one problem.  ``A`` is a scalar or a callable of zeta giving the
+    rate factor on each level; ``cb_seed`` reseeds only the friction field
+    (ensemble members share geometry); ``H_offset`` adds a thickness change
+    where there is ice."""
+    ny = nx if ny is None else ny
+    H, z_bed, c_raw = _geometry(nx, ny, dx, seed, cb_seed, sigma_ln, H_noise_seed)
+    if H_offset is not None:
+        H = np.where(H > 0, np.maximum(H + H_offset, 0.0), 0.0)
+    return _derive(nx, ny, dx, H, z_bed, c_raw, A, nz, dtype, lateral_bc, boundaries)
 
 
 # ---- the benchmark workloads -------------------------------------------
@@ -196,3 +208,41 @@ def workload_diva_uneven(domains: int = 48, seed: int = 0, dtype=np.float32, nz:
     return [add_diva_state(make_problem(int(s), 1536e3 / int(s), seed=seed + 31 * k, A=A, nz=nz,
                                         dtype=dtype))
             for k, s in enumerate(sides)]
+
+
+# ---- served solve sequences ------------------------------------------------
+
+SERVE_SIDE_RANGE = (81, 641)
+SERVE_SIDE_MEDIAN = 161.0
+SERVE_SIDE_LOGSIGMA = 0.5
+SERVE_SOLVES = 5
+SERVE_DH = 10.0        # max |thickness increment| between solves (m)
+
+
+def serve_request_sides(nreq: int, seed: int = 0):
+    """Grid sides of a served stream: lognormal (median 161, log-sigma 0.5),
+    clipped to [81, 641]."""
+    rng = np.random.default_rng(30_000 + seed)
+    s = np.exp(rng.normal(np.log(SERVE_SIDE_MEDIAN), SERVE_SIDE_LOGSIGMA, size=nreq))
+    return np.clip(np.rint(s), *SERVE_SIDE_RANGE).astype(int)
+
+
+def served_sequence(side: int, seed: int, nsolves: int = SERVE_SOLVES, dtype=np.float32,
+                    nz: int = 11):
+    """One served request: ``nsolves`` successive problems on a side x side
+    grid spanning 1536 km.  Between solves the thickness takes a seeded
+    smooth increment (40 km Gaussian, max |dH| = 10 m) where there is ice,
+    and every thickness-derived input is recomputed.  Only the first problem's
+    INOUT state is meant to be used: later solves start from the previous
+    solve's velocity, viscosity and friction (a warm start)."""
+    dx = 1536e3 / side
+    H, z_bed, c_raw = _geometry(side, side, dx, seed)
+    rng = np.random.default_rng(40_000 + seed)
+    out = []
+    for k in range(nsolves):
+        if k > 0:
+            inc = _smooth_noise(rng, side, side, 40e3, dx)
+            inc *= SERVE_DH / max(np.abs(inc).max(), 1e-12)
+            H = np.where(H > 0, np.maximum(H + inc, 0.0), 0.0)
+        out.append(_derive(side, side, dx, H, z_bed, c_raw, 1e-16, nz, dtype, "all", "zeros"))
+    return out

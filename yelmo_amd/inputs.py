@@ -75,30 +75,30 @@ def calc_z_srf_max(H_ice, f_ice, z_bed, z_sl, rho_ice, rho_sw):
 def calc_ice_front(f_ice, f_grnd, z_bed, z_sl, boundaries="zeros"):
     """calc_ice_front (src/physics/topography.f90:724-818): front codes of
     ice cells with an open neighbour and of the ice-free cells beside them.
-    The loop order matters (later cells overwrite the codes of neighbours),
-    so it is kept serial as in the reference."""
+    The reference loops serially and later cells overwrite neighbour codes,
+    but every write depends only on the written cell (an ice cell only ever
+    receives its own code, an open cell only its free code), so the result
+    does not depend on loop order and is formed array-wise."""
     nx, ny = f_ice.shape
     im1, ip1, jm1, jp1 = neighbors(nx, ny, boundaries)
-    m = np.zeros((nx, ny), dtype=np.int32, order="F")
-
-    def free_code(i, j):
-        return MASK_FRNT_ICE_FREE if z_bed[i, j] < z_sl[i, j] else MASK_FRNT_ICE_FREE_LAND
-
-    for j in range(ny):
-        for i in range(nx):
-            nb = [(im1[i], j), (ip1[i], j), (i, jm1[j]), (i, jp1[j])]
-            n_open = sum(1 for (a, b) in nb if f_ice[a, b] < 1.0)
-            if f_ice[i, j] == 1.0 and n_open > 0:
-                if f_grnd[i, j] > 0.0 and z_sl[i, j] <= z_bed[i, j]:
-                    m[i, j] = MASK_FRNT_GRND
-                elif f_grnd[i, j] > 0.0:
-                    m[i, j] = MASK_FRNT_MARINE
-                else:
-                    m[i, j] = MASK_FRNT_FLOAT
-                for (a, b) in nb:
-                    if f_ice[a, b] < 1.0:
-                        m[a, b] = free_code(a, b)
-    return m
+    full = f_ice == 1.0
+    open_ = f_ice < 1.0
+    n_open = (open_[im1, :].astype(np.int32) + open_[ip1, :] + open_[:, jm1] + open_[:, jp1])
+    front = full & (n_open > 0)
+    grnd = f_grnd > 0.0
+    code = np.where(grnd & (z_sl <= z_bed), MASK_FRNT_GRND,
+                    np.where(grnd, MASK_FRNT_MARINE, MASK_FRNT_FLOAT))
+    m = np.where(front, code, 0).astype(np.int32)
+    # open neighbours of front cells, scattered through the same index maps
+    near = np.zeros((nx, ny), dtype=bool)
+    fi_, fj_ = np.nonzero(front)
+    near[im1[fi_], fj_] = True
+    near[ip1[fi_], fj_] = True
+    near[fi_, jm1[fj_]] = True
+    near[fi_, jp1[fj_]] = True
+    free = np.where(z_bed < z_sl, MASK_FRNT_ICE_FREE, MASK_FRNT_ICE_FREE_LAND)
+    m = np.where(near & open_, free, m)
+    return np.asfortranarray(m.astype(np.int32))
 
 
 def calc_gradient_acx(var, f_ice, dx, grad_lim, zero_outside=False, boundaries="zeros"):
@@ -162,25 +162,31 @@ def calc_lateral_bc_stress(H_ice, z_srf, z_sl, rho_ice, rho_sw, g):
 
 def calc_lateral_bc_stress_2D(mask_frnt, H_ice, f_ice, z_srf, z_sl, rho_ice, rho_sw, g,
                               boundaries="zeros"):
-    """calc_lateral_bc_stress_2D (src/physics/velocity_general.f90:1453-1576)."""
+    """calc_lateral_bc_stress_2D (src/physics/velocity_general.f90:1453-1576):
+    on each face between a front ice cell and an ice-free cell, the front
+    stress of the ice cell (the ocean level replaced by the ice base against
+    ice-free land).  Faces are independent; evaluated in float64 and stored
+    in the array dtype."""
     nx, ny = H_ice.shape
     im1, ip1, jm1, jp1 = neighbors(nx, ny, boundaries)
-    tx = np.zeros((nx, ny), dtype=H_ice.dtype, order="F")
-    ty = np.zeros((nx, ny), dtype=H_ice.dtype, order="F")
-    for j in range(ny):
-        for i in range(nx):
-            for (a, b, out) in ((ip1[i], j, tx), (i, jp1[j], ty)):
-                ma, mb = mask_frnt[i, j], mask_frnt[a, b]
-                if (ma > 0 and mb < 0) or (ma < 0 and mb > 0):
-                    ii, jj = (a, b) if ma < 0 else (i, j)
-                    H = H_ice[ii, jj]
-                    zs = z_srf[ii, jj]
-                    zsl = z_sl[ii, jj]
-                    if ma == MASK_FRNT_ICE_FREE_LAND or mb == MASK_FRNT_ICE_FREE_LAND:
-                        zsl = zs - H
-                    out[i, j] = calc_lateral_bc_stress(np.float64(H), np.float64(zs),
-                                                       np.float64(zsl), rho_ice, rho_sw, g)
-    return tx, ty
+    H64 = np.asarray(H_ice, dtype=np.float64)
+    zs64 = np.asarray(z_srf, dtype=np.float64)
+    zsl64 = np.asarray(z_sl, dtype=np.float64)
+    out = []
+    for sh in ((ip1, slice(None)), (slice(None), jp1)):
+        ma, mb = mask_frnt, mask_frnt[sh]
+        sel = ((ma > 0) & (mb < 0)) | ((ma < 0) & (mb > 0))
+        take_b = ma < 0
+        H = np.where(take_b, H64[sh], H64)
+        zs = np.where(take_b, zs64[sh], zs64)
+        zsl = np.where(take_b, zsl64[sh], zsl64)
+        land = (ma == MASK_FRNT_ICE_FREE_LAND) | (mb == MASK_FRNT_ICE_FREE_LAND)
+        zsl = np.where(land, zs - H, zsl)
+        t = np.zeros((nx, ny), dtype=H_ice.dtype, order="F")
+        if sel.any():
+            t[sel] = calc_lateral_bc_stress(H[sel], zs[sel], zsl[sel], rho_ice, rho_sw, g)
+        out.append(t)
+    return out[0], out[1]
 
 
 def front_face_mask(code_a, code_b, lateral_bc):
@@ -206,22 +212,44 @@ def front_face_mask(code_a, code_b, lateral_bc):
     return -1
 
 
+def _front_face_mask_array(code_a, code_b, lateral_bc):
+    """front_face_mask over arrays of face codes (same table as below)."""
+    ia = (code_a > 0) & (code_b < 0)
+    ai = (code_a < 0) & (code_b > 0)
+    code_ice = np.where(ia, code_a, code_b)
+    code_free = np.where(ia, code_b, code_a)
+    land = code_free == MASK_FRNT_ICE_FREE_LAND
+    wall = land & (code_ice == MASK_FRNT_FLOAT)
+    code_ice = np.where(land, MASK_FRNT_GRND, code_ice)
+    if lateral_bc == "none":
+        r = np.full(code_ice.shape, 4)
+    elif lateral_bc in ("floating", "float"):
+        r = np.where(code_ice == MASK_FRNT_FLOAT, 3, 4)
+    elif lateral_bc == "marine":
+        r = np.where((code_ice == MASK_FRNT_FLOAT) | (code_ice == MASK_FRNT_MARINE), 3, 4)
+    elif lateral_bc == "all":
+        r = np.full(code_ice.shape, 3)
+    else:
+        r = np.full(code_ice.shape, -1)
+    r = np.where(wall, 0, r)
+    return np.where(ia | ai, r, -1)
+
+
 def set_ssa_masks(mask_frnt, f_ice, f_grnd, lateral_bc="all", boundaries="zeros"):
     """set_ssa_masks (src/physics/solver_ssa_ac.f90:851-948): per-face action
-    codes -1..4 for the acx and acy faces."""
+    codes -1..4 for the acx and acy faces (faces are independent)."""
     nx, ny = f_ice.shape
     im1, ip1, jm1, jp1 = neighbors(nx, ny, boundaries)
-    mx = np.zeros((nx, ny), dtype=np.int32, order="F")
-    my = np.zeros((nx, ny), dtype=np.int32, order="F")
-    for j in range(ny):
-        for i in range(nx):
-            for (a, b, out) in ((ip1[i], j, mx), (i, jp1[j], my)):
-                if f_ice[i, j] == 1.0 or f_ice[a, b] == 1.0:
-                    out[i, j] = 1 if (f_grnd[i, j] > 0.0 or f_grnd[a, b] > 0.0) else 2
-                ml = front_face_mask(int(mask_frnt[i, j]), int(mask_frnt[a, b]), lateral_bc)
-                if ml >= 0:
-                    out[i, j] = ml
-    return mx, my
+    full = f_ice == 1.0
+    grnd = f_grnd > 0.0
+    mf = np.asarray(mask_frnt, dtype=np.int64)
+    out = []
+    for sh in ((ip1, slice(None)), (slice(None), jp1)):
+        m = np.where(full | full[sh], np.where(grnd | grnd[sh], 1, 2), 0)
+        ml = _front_face_mask_array(mf, mf[sh], lateral_bc)
+        m = np.where(ml >= 0, ml, m)
+        out.append(np.asfortranarray(m.astype(np.int32)))
+    return out[0], out[1]
 
 
 def calc_zeta(nz_aa: int, zeta_scale: str = "exp", zeta_exp: float = 2.0, dtype=np.float32):

@@ -201,3 +201,37 @@ class ResidentBatch:
             self.close()
         except Exception:
             pass
+
+
+class Session:
+    """One domain kept on the device between solves (ygpu_session_*): the
+    velocity, viscosity and friction of a solve are the warm start of the
+    next; ``solve(prob)`` uploads prob's IN fields (its INOUT state only on
+    the first solve) and writes the results into prob.state."""
+
+    def __init__(self, prob: SsaProblem, par: SsaParams, handle=None):
+        self.h = handle or _lib.default_handle()
+        self.pc = par.to_c()
+        s, keep = _fill_struct(prob, False)
+        ss = C.c_void_p()
+        nbytes = C.c_int64()
+        self.h.check(self.h.lib.ygpu_session_create(self.h.h, C.byref(self.pc), C.byref(s),
+                                                    C.byref(ss), C.byref(nbytes)))
+        self.s = ss
+        self.device_bytes = nbytes.value
+
+    def solve(self, prob: SsaProblem) -> SolveInfo:
+        s, keep = _fill_struct(prob, False)
+        self.h.check(self.h.lib.ygpu_session_solve(self.s, C.byref(s)))
+        return _info(s)
+
+    def close(self):
+        if self.s:
+            self.h.lib.ygpu_session_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
