@@ -191,6 +191,143 @@ def run_diva(args):
     print(json.dumps(_finite(line)), flush=True)
 
 
+def _serve_run(srv, seqs, arrivals, record):
+    """Drives a server over a stream of requests.  Request k (the solve
+    sequence seqs[k]) arrives at arrivals[k] seconds after the start; each
+    solve after the first is submitted when its predecessor finishes.
+    Returns per-request latency (arrival -> last solve done), per-solve
+    latency (submit -> done), the makespan and the peak number of solves
+    running at once.  record(k, j, prob) sees every finished solve."""
+    n = len(seqs)
+    order = np.argsort(arrivals, kind="stable")
+    owner, t_submit, req_lat, solve_lat = {}, {}, np.zeros(n), []
+    nxt, done, peak, steps = 0, 0, 0, 0
+    t0 = time.perf_counter()
+    t_note = t0
+    while done < n:
+        now = time.perf_counter() - t0
+        if time.perf_counter() - t_note > 20.0:
+            t_note = time.perf_counter()
+            print(f"#   {now:.0f}s: {done}/{n} requests done, {len(owner)} in flight",
+                  file=sys.stderr, flush=True)
+        while nxt < n and arrivals[order[nxt]] <= now:
+            k = int(order[nxt])
+            t = srv.submit(seqs[k][0], k, last=len(seqs[k]) == 1)
+            owner[t] = (k, 0)
+            t_submit[t] = time.perf_counter() - t0
+            nxt += 1
+        if not owner:
+            time.sleep(max(0.0, min(arrivals[order[nxt]] - now, 0.01)))
+            continue
+        nr, _ = srv.step()
+        steps += 1
+        peak = max(peak, nr)
+        for t in srv.poll():
+            k, j = owner.pop(t)
+            tn = time.perf_counter() - t0
+            solve_lat.append(tn - t_submit.pop(t))
+            srv.fetch(t, seqs[k][j])
+            record(k, j, seqs[k][j])
+            if j + 1 < len(seqs[k]):
+                t2 = srv.submit(seqs[k][j + 1], k, cont=True, last=j + 2 == len(seqs[k]))
+                owner[t2] = (k, j + 1)
+                t_submit[t2] = time.perf_counter() - t0
+            else:
+                req_lat[k] = tn - arrivals[k]
+                done += 1
+    return req_lat, np.array(solve_lat), time.perf_counter() - t0, peak, steps
+
+
+def run_serve(args):
+    """Served stream of SSA solve sequences (continuous batching).  Requests
+    of lognormal grid side (clipped to 81..641, 1536 km across) each ask for
+    5 successive solves with seeded smooth +-10 m thickness increments
+    between them, warm-started from the previous solve's device-resident
+    state.  Per precision: a closed-loop run (every request queued at the
+    start) gives the capacity in solves/s; then requests arrive Poisson at
+    70% of that capacity and p50/p99 latency is measured.  The mixed-precision
+    velocities of every solve are compared with the FP64 server's."""
+    from yelmo_amd import generator
+    from yelmo_amd.params import SsaParams
+    from yelmo_amd.serve import Server
+    nreq = args.requests
+    sides = generator.serve_request_sides(nreq, seed=0)
+    t_gen = time.perf_counter()
+    seqs = [generator.served_sequence(int(s), 1000 + k, generator.SERVE_SOLVES)
+            for k, s in enumerate(sides)]
+    print(f"# generated {nreq} requests x {generator.SERVE_SOLVES} solves in "
+          f"{time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    nsolves = sum(len(s) for s in seqs)
+    budget = int(args.budget_gb * (1 << 30))
+    vel = {}
+    out = {}
+    for prec in ("fp64", "mixed"):
+        par = SsaParams(precision=prec)
+        srv = Server(par, max_slots=args.slots, max_side=generator.SERVE_SIDE_RANGE[1],
+                     nz_max=seqs[0][0].nz, hbm_budget=budget)
+        # warm-up: one small sequence through the same server
+        warm = generator.served_sequence(81, 99, 2)
+        _serve_run(srv, [warm], np.zeros(1), lambda k, j, p: None)
+        errs = []
+
+        def record(k, j, p, prec=prec, errs=errs):
+            v = np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]).astype(np.float64)
+            if prec == "fp64":
+                vel[(k, j)] = v
+            else:
+                errs.append(rel_l2(v, vel[(k, j)]))
+
+        lat_c, slat_c, span_c, peak_c, steps_c = _serve_run(srv, seqs, np.zeros(nreq), record)
+        capacity = nsolves / span_c
+        rate = args.load * capacity / generator.SERVE_SOLVES
+        rng = np.random.default_rng(50_000)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+        errs_closed = list(errs)
+        errs.clear()
+        lat, slat, span, peak, steps = _serve_run(srv, seqs, arrivals, record)
+        srv.close()
+        out[prec] = {
+            "capacity_solves_per_s": capacity,
+            "closed_loop": {"makespan_s": span_c, "peak_running": peak_c, "steps": steps_c,
+                            "request_latency_s": {"p50": float(np.percentile(lat_c, 50)),
+                                                  "p99": float(np.percentile(lat_c, 99))}},
+            "poisson": {"arrival_rate_req_per_s": rate, "solves_per_s": nsolves / span,
+                        "makespan_s": span, "peak_running": peak, "steps": steps,
+                        "request_latency_s": {"p50": float(np.percentile(lat, 50)),
+                                              "p99": float(np.percentile(lat, 99))},
+                        "solve_latency_s": {"p50": float(np.percentile(slat, 50)),
+                                            "p99": float(np.percentile(slat, 99))}},
+            "server_device_bytes": srv.device_bytes,
+        }
+        if prec == "mixed":
+            out[prec]["max_rel_l2_vs_fp64"] = max(errs_closed + errs)
+        print(f"# {prec}: capacity {capacity:.1f} solves/s, p99 {out[prec]['poisson']['request_latency_s']['p99']:.2f}s",
+              file=sys.stderr, flush=True)
+    mixed = out["mixed"]
+    line = {
+        "metric": "served SSA solves/s and p50/p99 request latency, continuous batching",
+        "value": mixed["poisson"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": nsolves, "warmup": 1,
+        "ms_per_step": 1e3 * mixed["poisson"]["makespan_s"] / nsolves,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{nreq} requests x {generator.SERVE_SOLVES} warm-started SSA "
+                               f"solves, side lognormal (median 161) in [81, 641], 1536 km "
+                               f"across, Poisson arrivals at {args.load:.0%} of the measured "
+                               f"capacity, mixed FP32/FP64",
+                   "requests": nreq, "sides": [int(s) for s in sides], "slots": args.slots,
+                   "hbm_budget_gb": args.budget_gb},
+        "p50_latency_s": mixed["poisson"]["request_latency_s"]["p50"],
+        "p99_latency_s": mixed["poisson"]["request_latency_s"]["p99"],
+        "max_rel_l2_vs_fp64": mixed["max_rel_l2_vs_fp64"],
+        "error_ok": mixed["max_rel_l2_vs_fp64"] <= 1e-6,
+        "mixed": mixed, "fp64": out["fp64"],
+        "speedup_capacity_vs_fp64": mixed["capacity_solves_per_s"] / out["fp64"]["capacity_solves_per_s"],
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(_finite(line)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,9 +343,15 @@ def main():
     ap.add_argument("--mixed-only", action="store_true",
                     help="time the mixed batched solve only (for counter runs): no FP64 "
                          "reference, no CPU baseline, error fields null")
-    ap.add_argument("--workload", choices=("ensemble", "diva"), default="ensemble",
+    ap.add_argument("--workload", choices=("ensemble", "diva", "serve"), default="ensemble",
                     help="ensemble: the 64-member 161^2 SSA ensemble (the headline line); "
-                         "diva: 48 DIVA domains of uneven size (96..384)^2, nz = 21")
+                         "diva: 48 DIVA domains of uneven size (96..384)^2, nz = 21; "
+                         "serve: a served stream of warm-started solve sequences")
+    ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
+    ap.add_argument("--slots", type=int, default=128, help="serve: server member slots")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="serve: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run a plain device copy of MIB MiB "
                          "five times (the reference kernel for scaling HBM byte counters)")
@@ -236,6 +379,8 @@ def main():
 
     if args.workload == "diva":
         return run_diva(args)
+    if args.workload == "serve":
+        return run_serve(args)
 
     t_gen = time.perf_counter()
     base = generator.workload_ensemble(args.members, args.side, seed=0, dtype=np.float32)

@@ -197,8 +197,9 @@ int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out);  /* returns and resets     
 /* ---- served solve sequences with continuous batching (no counterpart in
  * the reference, which solves one domain per call; these are the submit /
  * poll entry points of a request loop).  A server holds max_slots member
- * slots and one device arena sized by hbm_budget_bytes (domains up to
- * max_side x max_side, up to nz_max levels).  ygpu_submit copies one solve's
+ * slots and one device arena of at most hbm_budget_bytes (domains up to
+ * max_side x max_side, up to nz_max levels; the arena is never larger than
+ * max_slots domains of the largest size).  ygpu_submit copies one solve's
  * arrays and queues it under a caller-chosen sequence id: the first solve of
  * a sequence carries its INOUT state; solves flagged YGPU_SEQ_CONTINUE read
  * only the IN fields (and masks) and start from the previous solve's

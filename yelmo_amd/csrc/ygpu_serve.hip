@@ -232,6 +232,8 @@ struct Server : YgServerBase {
     S = nslots; nzmax = nz_max; maxside = max_side;
     const double bpc = bytes_per_cell(B, nz_max);
     long long cap = (long long)((double)budget / bpc) / kGranule * kGranule;
+    // no more cells than the slots can occupy at once
+    cap = std::min(cap, (long long)S * granules((long long)max_side * max_side));
     if (cap < granules((long long)max_side * max_side)) {
       h->err = "server: the HBM budget does not hold one domain of the largest side";
       return YGPU_ERR_ARG;
