@@ -50,7 +50,7 @@ BND = {"zeros": 0, "infinite": 1, "mask": 2, "MISMIP3D": 3, "TROUGH": 4, "period
 
 def params_c(par) -> OracleParams:
     c = OracleParams()
-    c.ssa_solver = 0
+    c.ssa_solver = {"residual": 0, "energy": 1}[par.ssa_solver]
     c.boundaries = BND[par.boundaries]
     c.visc_method = par.visc_method
     c.beta_method = par.beta_method
@@ -237,6 +237,47 @@ def assemble(prob, beta_acx, beta_acy, N_aa, boundaries="zeros", lim=None, dp=Tr
                               *[_p(a) for a in rest], W(prob.dx), W(prob.dy), BND[boundaries],
                               *[(_p(a) if a is not None else None) for a in limp], _p(Nab), nx, ny)
     return ap, ai[:k], av[:k], bv, xv, Nab
+
+
+def assemble_energy(prob, beta_acx, beta_acy, N_aa, boundaries="zeros", lim=None, dp=True):
+    """The oracle's energy-formulation assembly (solver_ssa_ac_energy.f90)
+    on given fields; returns 1-based CSR (a_ptr, a_index, a_value, b, x0)
+    and copy_from (0-based root of each tied unknown, -1 elsewhere)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = prob.nx, prob.ny
+    nc = nx * ny
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=dt))
+
+    class Lgs(C.Structure):
+        _fields_ = [("nmax", C.c_int), ("a_ptr", C.c_void_p), ("a_index", C.c_void_p),
+                    ("a_value", C.c_void_p), ("b_value", C.c_void_p), ("x_value", C.c_void_p)]
+
+    cap = 48 * 2 * nc
+    ap = np.zeros(2 * nc + 1, dtype=np.int32)
+    ai = np.zeros(cap, dtype=np.int32)
+    av = np.zeros(cap)
+    bv = np.zeros(2 * nc)
+    xv = np.zeros(2 * nc)
+    cf = np.zeros(2 * nc, dtype=np.int32)
+    lgs = Lgs(2 * nc, ap.ctypes.data, ai.ctypes.data, av.ctypes.data, bv.ctypes.data,
+              xv.ctypes.data)
+    st, fl = prob.state, prob.fields
+    args = [F(st["ux_b"]), F(st["uy_b"]), F(beta_acx), F(beta_acy), F(N_aa)]
+    masks = [np.asfortranarray(st["ssa_mask_acx"], dtype=np.int32),
+             np.asfortranarray(st["ssa_mask_acy"], dtype=np.int32)]
+    rest = [F(fl["H_ice"]), F(fl["f_ice"]), F(fl["z_bed"]), F(fl["z_sl"]), F(fl["taud_acx"]),
+            F(fl["taud_acy"]), F(fl["taul_int_acx"]), F(fl["taul_int_acy"])]
+    W = C.c_double if dp else C.c_float
+    limp = [None] * 4 if lim is None else [F(a) for a in lim]
+    L.oracle_assemble_ssa_energy.restype = C.c_int
+    k = L.oracle_assemble_ssa_energy(C.byref(lgs), cap, _p(cf), *[_p(a) for a in args],
+                                     *[_p(a) for a in masks], *[_p(a) for a in rest],
+                                     W(prob.dx), W(prob.dy), BND[boundaries],
+                                     *[(_p(a) if a is not None else None) for a in limp], nx, ny)
+    if k < 0:
+        raise RuntimeError(f"oracle_assemble_ssa_energy failed ({k})")
+    return ap, ai[:k], av[:k], bv, xv, cf
 
 
 def set_ssa_masks(mask_frnt, f_ice, f_grnd, lateral_bc="all", boundaries="zeros", dp=False):

@@ -106,7 +106,7 @@ static void nbr(int i, int j, int nx, int ny, int BC,
 /* Parameters: mirrors ssa_param_class (velocity_ssa.f90:22-52).  Layout
  * is shared with the Python test driver (oracle/oracle.py).               */
 typedef struct {
-    int32_t ssa_solver;      /* 0 residual, 1 energy (not restated here)   */
+    int32_t ssa_solver;      /* 0 residual, 1 energy                       */
     int32_t boundaries;      /* OB_*                                       */
     int32_t visc_method;     /* 0, 1, 2                                    */
     int32_t beta_method;     /* -1 .. 5                                    */
@@ -917,6 +917,55 @@ int oracle_assemble_ssa(oracle_lgs *lgs, const wp *ux, const wp *uy, const wp *b
     return k;
 }
 
+/* the energy formulation (solver_ssa_ac_energy.f90) */
+#include "energy_oracle.inc"
+
+/* The assembler par->ssa_solver selects (velocity_ssa.f90:270-283,
+ * velocity_diva.f90:323-336); copy_from is -1 everywhere for the residual
+ * form.  Returns the entry count or a negative error.                      */
+static int assemble_selected(const oracle_ssa_params *par, oracle_lgs *L, int cap,
+                             int32_t *copy_from, const wp *ux, const wp *uy, const wp *bx,
+                             const wp *by, const wp *N_aa, const int32_t *mx, const int32_t *my,
+                             const wp *H_ice, const wp *f_ice, const wp *z_bed, const wp *z_sl,
+                             const wp *taud_acx, const wp *taud_acy, const wp *taul_acx,
+                             const wp *taul_acy, wp dx, wp dy, const wp *lkx, const wp *lky,
+                             const wp *lrx, const wp *lry, int nx, int ny)
+{
+    if (par->ssa_solver == 1)
+        return oracle_assemble_ssa_energy(L, cap, copy_from, ux, uy, bx, by, N_aa, mx, my, H_ice,
+                                          f_ice, z_bed, z_sl, taud_acx, taud_acy, taul_acx,
+                                          taul_acy, dx, dy, par->boundaries, lkx, lky, lrx, lry,
+                                          nx, ny);
+    for (int q = 0; q < 2 * nx * ny; q++) copy_from[q] = -1;
+    return oracle_assemble_ssa(L, ux, uy, bx, by, N_aa, mx, my, H_ice, f_ice, z_bed, z_sl,
+                               taud_acx, taud_acy, taul_acx, taul_acy, dx, dy, par->boundaries,
+                               lkx, lky, lrx, lry, NULL, nx, ny);
+}
+
+/* linear_solver_save_velocity (solver_ssa_ac.f90:33-71) without the
+ * underflow clip: rows tied to another row take that row's solution        */
+static void save_velocity(wp *ux, wp *uy, const oracle_lgs *L, const int32_t *copy_from, int nx,
+                          int ny)
+{
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            const int n = i * ny + j;
+            int r = 2 * n;
+            if (copy_from[r] >= 0) r = copy_from[r];
+            ux[IX(i, j)] = (wp)L->x_value[r];
+            r = 2 * n + 1;
+            if (copy_from[r] >= 0) r = copy_from[r];
+            uy[IX(i, j)] = (wp)L->x_value[r];
+        }
+}
+
+/* CSR capacity of an assembly: 18 per cell (solver_linear.F90:85), or the
+ * energy form's row limit                                                   */
+static int csr_capacity(const oracle_ssa_params *par, int nc)
+{
+    return par->ssa_solver == 1 ? EN_NNZ_ROW_MAX * 2 * nc : 18 * nc;
+}
+
 /* ------------------------------------------------------------------ */
 /* FP64 Krylov solve behind linear_solver_matrix_solve
  * (solver_linear.F90:151-400).  LIS is not vendored: BiCGSTAB / CG with
@@ -1408,10 +1457,12 @@ int oracle_calc_velocity_ssa(
     oracle_lgs L;
     L.nmax = 2 * nc;
     L.a_ptr = calloc(L.nmax + 1, sizeof(int32_t));
-    L.a_index = calloc((size_t)18 * nc, sizeof(int32_t));
-    L.a_value = calloc((size_t)18 * nc, sizeof(double));
+    const int cap = csr_capacity(par, nc);
+    L.a_index = calloc((size_t)cap, sizeof(int32_t));
+    L.a_value = calloc((size_t)cap, sizeof(double));
     L.b_value = calloc(L.nmax, sizeof(double));
     L.x_value = calloc(L.nmax, sizeof(double));
+    int32_t *cfrom = malloc(sizeof(int32_t) * L.nmax);
     int rc = 0;
     memcpy(bsx, beta_acx, sizeof(wp) * nc);
     memcpy(bsy, beta_acy, sizeof(wp) * nc);
@@ -1451,10 +1502,10 @@ int oracle_calc_velocity_ssa(
             oracle_vel_lim_drag(lkx, lky, lrx, lry, ux_b, uy_b, ssa_mask_acx, ssa_mask_acy,
                                 (wp)par->ssa_vel_max, (wp)par->ssa_vel_lim_tau, nx, ny,
                                 par->boundaries);
-        oracle_assemble_ssa(&L, ux_b, uy_b, bsx, bsy, visc_eff_int, ssa_mask_acx, ssa_mask_acy,
-                            H_ice, f_ice, z_bed, z_sl, taud_acx, taud_acy, taul_int_acx,
-                            taul_int_acy, dx, dy, par->boundaries, lkx, lky, lrx, lry, NULL, nx,
-                            ny);
+        if (assemble_selected(par, &L, cap, cfrom, ux_b, uy_b, bsx, bsy, visc_eff_int,
+                              ssa_mask_acx, ssa_mask_acy, H_ice, f_ice, z_bed, z_sl, taud_acx,
+                              taud_acy, taul_int_acx, taul_int_acy, dx, dy, lkx, lky, lrx, lry,
+                              nx, ny) < 0) { rc = -7; goto out; }
         int32_t it = 0, st = 0;
         double res = 0, secs = 0;
 #ifdef ORACLE_DEBUG_DUMP
@@ -1475,12 +1526,7 @@ int oracle_calc_velocity_ssa(
         if (st != 0) lin_fail++;
         if (lin_hist) lin_hist[iter - 1] = it;
         /* linear_solver_save_velocity (solver_ssa_ac.f90:33-71) */
-        for (int i = 0; i < nx; i++)
-            for (int j = 0; j < ny; j++) {
-                int n = i * ny + j;
-                ux_b[IX(i, j)] = (wp)L.x_value[2 * n];
-                uy_b[IX(i, j)] = (wp)L.x_value[2 * n + 1];
-            }
+        save_velocity(ux_b, uy_b, &L, cfrom, nx, ny);
         for (int c = 0; c < nc; c++) {
             if (WABS(ux_b[c]) < R(1e-15)) ux_b[c] = R(0);
             if (WABS(uy_b[c]) < R(1e-15)) uy_b[c] = R(0);
@@ -1529,7 +1575,7 @@ out:
     out_i[2] = lin_fail;
     if (solve_seconds) *solve_seconds = tsolve;
     free(visc_nm1); free(ux_nm1); free(uy_nm1); free(bsx); free(bsy);
-    free(lkx); free(lky); free(lrx); free(lry);
+    free(lkx); free(lky); free(lrx); free(lry); free(cfrom);
     free(L.a_ptr); free(L.a_index); free(L.a_value); free(L.b_value); free(L.x_value);
     return rc;
 }
