@@ -61,7 +61,8 @@ typedef struct ygpu_handle ygpu_handle;
 /* Mirrors ssa_param_class (velocity_ssa.f90:22-52) and diva_param_class
  * (velocity_diva.f90:22-51); strings become the enums above.             */
 typedef struct ygpu_ssa_params {
-  int32_t ssa_solver;     /* 0 "residual" (1 "energy": not available)           */
+  int32_t ssa_solver;     /* 0 "residual" (solver_ssa_ac.f90), 1 "energy"
+                             (solver_ssa_ac_energy.f90, the reference default)   */
   int32_t boundaries;     /* YGPU_BND_*                                          */
   int32_t visc_method;    /* 0, 1 (quadrature), 2 (aa-nodes)                     */
   int32_t beta_method;    /* -1 .. 5                                             */
@@ -75,8 +76,8 @@ typedef struct ygpu_ssa_params {
       ssa_vel_lim_tau, ssa_vel_max, ssa_iter_rel, ssa_iter_conv, rho_ice, rho_sw, g;
   int32_t no_slip;        /* DIVA: "diva-noslip"                                 */
   int32_t block;          /* threads per member block (0: pick automatically)   */
-  char ssa_lis_opt[256];  /* LIS option string, e.g.
-                             "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false";
+  char ssa_lis_opt[256];  /* LIS option string, e.g. the energy default
+                             "-i cg -p jacobi -maxiter 200 -tol 1.0e-2 -initx_zeros false";
                              extensions: -ir_inner_tol T, -ir_max N              */
 } ygpu_ssa_params;
 

@@ -33,6 +33,9 @@ def test_default_params_mirror_yelmo_defaults():
     assert p.ssa_iter_max == 20 and abs(p.ssa_iter_rel - 0.7) < 1e-12
     assert p.beta_min == 100.0 and p.eps_0 == 1e-6 and p.ssa_vel_max == 1e4
     assert p.rho_ice == 910.0 and p.rho_sw == 1028.0 and p.g == 9.81
+    # ssa_solver = "energy" with ssa_lis_opt_energy (input/yelmo_defaults.nml:159-161)
+    assert p.ssa_solver == 1
+    assert p.ssa_lis_opt.decode() == "-i cg -p jacobi -maxiter 200 -tol 1.0e-2 -initx_zeros false"
     assert ctypes.sizeof(_lib.SsaParamsC) == 4 * 10 + 8 * 15 + 4 * 2 + 256
 
 

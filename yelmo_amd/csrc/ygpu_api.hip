@@ -174,7 +174,7 @@ const char* ygpu_last_error(const ygpu_handle* h) { return h ? h->err.c_str() : 
 void ygpu_default_params(ygpu_ssa_params* p) {
   if (!p) return;
   memset(p, 0, sizeof(*p));
-  p->ssa_solver = 0;
+  p->ssa_solver = 1;   // "energy" (input/yelmo_defaults.nml:159)
   p->boundaries = YGPU_BND_ZEROS;
   p->visc_method = 1;
   p->beta_method = 1;
@@ -190,7 +190,8 @@ void ygpu_default_params(ygpu_ssa_params* p) {
   p->rho_ice = 910.0; p->rho_sw = 1028.0; p->g = 9.81;
   p->no_slip = 0;
   p->block = 0;
-  strncpy(p->ssa_lis_opt, "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false",
+  // ssa_lis_opt_energy (input/yelmo_defaults.nml:161)
+  strncpy(p->ssa_lis_opt, "-i cg -p jacobi -maxiter 200 -tol 1.0e-2 -initx_zeros false",
           sizeof(p->ssa_lis_opt) - 1);
 }
 
@@ -542,6 +543,7 @@ int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny,
   if (par->precision == YGPU_PREC_MIXED) b += C * (YG_NA * 4 + YG_NF * 4 + YG_NW32 * 2 * 4);
   else b += C * YG_NF * 8;
   b += rb * (C * 34 + Z * 3 + 2 * Z);
+  if (par->ssa_solver == 1) b += C * 2 * 4;   // copy_from of the energy form
   return b;
 }
 

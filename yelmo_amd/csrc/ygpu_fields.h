@@ -30,6 +30,7 @@ struct YgFieldParams {
   int visc_method, beta_method, beta_gl_scale, beta_gl_stag, vel_lim_method;
   int diva;            // 1: DIVA viscosity / beta_eff
   int no_slip;         // DIVA no-slip variant
+  int energy;          // 1: energy-formulation assembly (ssa_solver "energy")
   double visc_const, beta_const, beta_q, beta_u0, beta_gl_f, H_grnd_lim, beta_min, eps_0,
       vel_lim_tau, vel_max, iter_rel, iter_conv, n_glen;
 };
@@ -57,6 +58,11 @@ struct YgFields {
   R *F2, *F2_acx, *F2_acy;
   R *beff_acx, *beff_acy, *beta_eff;
   R *ubar_x, *ubar_y;         // DIVA depth-averaged velocity (the solved field)
+  // energy formulation: per unknown (2 per cell, global row order) the
+  // member-local unknown whose solution a tied unknown takes, or -1; and a
+  // per-member flag set when a row falls outside the stencil
+  int32_t* cfrom;
+  int* asm_err;
 };
 
 struct YgBatchGeom {

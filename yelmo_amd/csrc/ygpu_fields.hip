@@ -758,6 +758,335 @@ __global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const 
   maskv[gq] = pm[0] | (pm[1] << 16);
 }
 
+// ---- energy-formulation assembly (solver_ssa_ac_energy.f90:70-708) --------
+// The matrix is the Hessian of the discrete SSA energy (cell membrane,
+// corner shear and face drag terms, each a small local Hessian) with
+// Dirichlet unknowns lifted to the right-hand side and free-slip edge
+// unknowns folded into their inner root.  Every free row has the 9-slot
+// pattern of the residual assembly (inner rows equal -A_residual dx dy), so
+// it goes into the same stencil records.  One thread per cell gathers both
+// rows from the elements around the unknown and around the edge unknowns
+// tied to it, in the reference's order of terms.
+namespace en {
+
+enum { FREE = 0, DIR = 1, TIED = 2 };
+
+struct Geo {
+  int nx, ny;
+  bool per[4], fre[4];  // sides 0 right, 1 upper, 2 left, 3 lower
+  bool per_x, per_y;
+};
+
+template <typename R>
+struct In {
+  const int32_t *mx, *my;
+  const R *ux, *uy;
+};
+
+// classify_ux / classify_uy / edge (:318-386)
+template <typename R>
+YG_DEV int classify(const Geo& g, const In<R>& in, int i, int j, int comp, int& pi, int& pj,
+                    double& val) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = I2(i, j);
+  const int mask = comp ? in.my[c] : in.mx[c];
+  val = 0.0;
+  if (mask == 0) return DIR;
+  if (mask == -1) { val = (double)(comp ? in.uy[c] : in.ux[c]); return DIR; }
+  int side = -1;
+  const bool xl = i == 0 && !g.per[2], xr = i == nx - 1 && !g.per[0];
+  const bool yl = j == 0 && !g.per[3], yu = j == ny - 1 && !g.per[1];
+  if (comp == 0) {
+    if (xl) { side = 2; pi = 1; pj = j; }
+    else if (xr) { side = 0; pi = nx - 2; pj = j; }
+    else if (yl) { side = 3; pi = i; pj = 1; }
+    else if (yu) { side = 1; pi = i; pj = ny - 2; }
+  } else {
+    if (yl) { side = 3; pi = i; pj = 1; }
+    else if (yu) { side = 1; pi = i; pj = ny - 2; }
+    else if (xl) { side = 2; pi = 1; pj = j; }
+    else if (xr) { side = 0; pi = nx - 2; pj = j; }
+  }
+  if (side < 0) return FREE;
+  return g.fre[side] ? TIED : DIR;
+}
+
+// Kind after following a tied chain (:212-235): FREE (root = itself), DIR
+// (value of the chain's end), TIED (root ri, rj), -1 for a chain that does
+// not end within 8 steps.
+template <typename R>
+YG_DEV int resolve(const Geo& g, const In<R>& in, int i, int j, int comp, int& ri, int& rj,
+                   double& val) {
+  int pi = i, pj = j;
+  const int k = classify<R>(g, in, i, j, comp, pi, pj, val);
+  ri = i; rj = j;
+  if (k != TIED) return k;
+  for (int depth = 0; depth < 8; ++depth) {
+    int qi = pi, qj = pj;
+    double v;
+    const int k2 = classify<R>(g, in, pi, pj, comp, qi, qj, v);
+    if (k2 == FREE) { ri = pi; rj = pj; return TIED; }
+    if (k2 == DIR) { val = v; return DIR; }
+    pi = qi; pj = qj;
+  }
+  return -1;
+}
+
+// wrap_index (:420-443)
+YG_DEV bool wrap(int& k, int nk, bool periodic, bool free_lo, bool free_hi) {
+  if (k < 0) {
+    if (periodic) k += nk;
+    else if (free_lo) k = 0;
+    else return false;
+  } else if (k >= nk) {
+    if (periodic) k -= nk;
+    else if (free_hi) k = nk - 1;
+    else return false;
+  }
+  return true;
+}
+
+// dof_ux / dof_uy (:395-418): false for a zero ghost
+YG_DEV bool dof(const Geo& g, int& i, int& j) {
+  return wrap(i, g.nx, g.per_x, g.fre[2], g.fre[0]) && wrap(j, g.ny, g.per_y, g.fre[3], g.fre[1]);
+}
+
+// element_index (:445-457)
+YG_DEV bool elem(int& k, int nk, bool periodic) {
+  if (k < 0 || k >= nk) {
+    if (!periodic) return false;
+    k = ((k % nk) + nk) % nk;
+  }
+  return true;
+}
+
+// stencil slot of unknown (ci, cj, ccomp) in the row of (i, j, comp)
+YG_DEV int slot(const Geo& g, int comp, int i, int j, int ci, int cj, int ccomp) {
+  int di = ci - i, dj = cj - j;
+  if (g.per_x) { if (di > 1) di -= g.nx; else if (di < -1) di += g.nx; }
+  if (g.per_y) { if (dj > 1) dj -= g.ny; else if (dj < -1) dj += g.ny; }
+#pragma unroll
+  for (int s = 0; s < YG_NSLOT; ++s) {
+    const int8_t* t = comp ? kSlotUy[s] : kSlotUx[s];
+    if (t[0] == di && t[1] == dj && t[2] == ccomp) return s;
+  }
+  return -1;
+}
+
+struct Row {
+  double a[YG_NSLOT];
+  double b;
+  bool bad;
+};
+
+YG_DEV void add_slot(Row& r, int s, double v) {
+#pragma unroll
+  for (int t = 0; t < YG_NSLOT; ++t)
+    if (t == s) r.a[t] += v;
+}
+
+// add_element's inner loop for one column (:626-651)
+template <typename R>
+YG_DEV void add_col(Row& r, const Geo& g, const In<R>& in, int comp, int i, int j, int ci, int cj,
+                    int ccomp, double h) {
+  int ri, rj;
+  double v;
+  const int k = resolve<R>(g, in, ci, cj, ccomp, ri, rj, v);
+  if (k == DIR) { r.b -= h * v; return; }
+  const int s = k < 0 ? -1 : slot(g, comp, i, j, ri, rj, ccomp);
+  if (s < 0) { r.bad = true; return; }
+  add_slot(r, s, h);
+}
+
+// add_element (:626-651) for an element with dofs (dc, di, dj, valid)
+template <typename R>
+YG_DEV void add_element(Row& r, const Geo& g, const In<R>& in, int comp, int i, int j, int mc,
+                        int mi, int mj, const int dc[4], const int di[4], const int dj[4],
+                        const bool dv[4], const double H[4][4]) {
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    if (!dv[l] || dc[l] != mc || di[l] != mi || dj[l] != mj) continue;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (dv[k]) add_col<R>(r, g, in, comp, i, j, di[k], dj[k], dc[k], H[l][k]);
+  }
+}
+
+}  // namespace en
+
+// add_member (:488-564): every term of unknown (mc, mi, mj), the row's root
+// or an unknown tied to it, into the row of (comp, i, j)
+template <typename R>
+YG_DEV void en_member(en::Row& r, const en::Geo& g, const en::In<R>& in, const YgFields<R>& f,
+                      long long c0, const R* N, const R* fi, const R* zb, const R* zs, int BC,
+                      double dx, double dy, int comp, int i, int j, int mi, int mj) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = I2(mi, mj);
+  const int ip1 = mi + 1 == nx ? 0 : mi + 1, jp1 = mj + 1 == ny ? 0 : mj + 1;
+  const double dxdy = dx * dy, dyodx = dy / dx, dxody = dx / dy;
+  const int mask = comp ? in.my[c] : in.mx[c];
+  const double w = (mask == 3 || mask == 4) ? 0.5 : 1.0;
+  R beta_now = comp ? f.bss_y[c0 + c] : f.bss_x[c0 + c];
+  beta_now = beta_now + (comp ? f.lky[c0 + c] : f.lkx[c0 + c]);
+  r.b += w * (double)(comp ? f.lry[c0 + c] : f.lrx[c0 + c]) * dxdy;
+  r.a[0] += w * (double)beta_now * dxdy;
+  if (mask == 3) {
+    const bool out = comp ? (fi[c] == R(1) && fi[I2(mi, jp1)] < R(1))
+                          : (fi[c] == R(1) && fi[I2(ip1, mj)] < R(1));
+    const double tl = (double)(comp ? f.taul_acy[c0 + c] : f.taul_acx[c0 + c]) * (comp ? dx : dy);
+    r.b += out ? tl : -tl;
+  } else {
+    r.b -= (double)(comp ? f.taud_acy[c0 + c] : f.taud_acx[c0 + c]) * dxdy;
+  }
+  // elements holding the member: two cells, two corners
+  int ce[4][2];
+  if (comp == 0) {
+    ce[0][0] = mi; ce[0][1] = mj; ce[1][0] = mi + 1; ce[1][1] = mj;
+    ce[2][0] = mi; ce[2][1] = mj; ce[3][0] = mi; ce[3][1] = mj - 1;
+  } else {
+    ce[0][0] = mi; ce[0][1] = mj; ce[1][0] = mi; ce[1][1] = mj + 1;
+    ce[2][0] = mi; ce[2][1] = mj; ce[3][0] = mi - 1; ce[3][1] = mj;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    int ci = ce[e][0], cj = ce[e][1];
+    if (!en::elem(ci, nx, g.per_x) || !en::elem(cj, ny, g.per_y)) continue;
+    int dc[4], di[4], dj[4];
+    bool dv[4];
+    double H[4][4];
+    if (e < 2) {
+      // cell_dofs (:566-596): membrane term of cell (ci, cj)
+      const int oi[4] = {ci - 1, ci, ci, ci}, oj[4] = {cj, cj, cj - 1, cj};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        dc[k] = k < 2 ? 0 : 1; di[k] = oi[k]; dj[k] = oj[k];
+        dv[k] = en::dof(g, di[k], dj[k]);
+      }
+      const double Nc = (double)N[I2(ci, cj)];
+      const double hx = 4.0 * Nc * dyodx, hy = 4.0 * Nc * dxody, hc = 2.0 * Nc;
+      const double h[4][4] = {{hx, -hx, hc, -hc}, {-hx, hx, -hc, hc},
+                              {hc, -hc, hy, -hy}, {-hc, hc, -hy, hy}};
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) H[a][b] = h[a][b];
+    } else {
+      // corner_dofs (:598-624): shear term of corner (ci, cj)
+      const int oi[4] = {ci, ci, ci, ci + 1}, oj[4] = {cj, cj + 1, cj, cj};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        dc[k] = k < 2 ? 0 : 1; di[k] = oi[k]; dj[k] = oj[k];
+        dv[k] = en::dof(g, di[k], dj[k]);
+      }
+      const double Nc = (double)visc_ab<R>(N, fi, zb, zs, nx, ny, BC, ci, cj);
+      const double hx = Nc * dxody, hy = Nc * dyodx;
+      const double h[4][4] = {{hx, -hx, Nc, -Nc}, {-hx, hx, -Nc, Nc},
+                              {Nc, -Nc, hy, -hy}, {-Nc, Nc, -hy, hy}};
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) H[a][b] = h[a][b];
+    }
+    en::add_element<R>(r, g, in, comp, i, j, comp, mi, mj, dc, di, dj, dv, H);
+  }
+}
+
+template <typename R>
+__global__ void k_assemble_energy(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
+                                  double* A64, float* A32, double* bvec, double* x0,
+                                  uint32_t* maskv) {
+  const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gq >= G.total_cells) return;
+  const int m = yg_member_at(G, gq);
+  if (m < 0 || (skip && skip[m])) return;
+  const YgMember& mb = G.members[m];
+  const long long c0 = G.coffs[m];
+  const int q = (int)(gq - c0);
+  const int nx = mb.nx, ny = mb.ny;
+  const int i = q / ny, j = q - i * ny;
+  en::Geo g;
+  g.nx = nx; g.ny = ny;
+  // border types (:144-167)
+  int b[4];  // 0 no-slip, 1 free-slip, 2 periodic
+  switch (p.bnd) {
+    case YG_B_MISMIP3D:
+    case YG_B_TROUGH: b[0] = 1; b[1] = 2; b[2] = 0; b[3] = 2; break;
+    case YG_B_PERIODIC: b[0] = b[1] = b[2] = b[3] = 2; break;
+    case YG_B_PERIODIC_X: b[0] = 2; b[1] = 1; b[2] = 2; b[3] = 1; break;
+    case YG_B_PERIODIC_Y: b[0] = 1; b[1] = 2; b[2] = 1; b[3] = 2; break;
+    case YG_B_INFINITE:
+    case YG_B_MASK: b[0] = b[1] = b[2] = b[3] = 1; break;
+    default: b[0] = b[1] = b[2] = b[3] = 0; break;
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) { g.per[s] = b[s] == 2; g.fre[s] = b[s] == 1; }
+  g.per_x = g.per[0] && g.per[2];
+  g.per_y = g.per[1] && g.per[3];
+  en::In<R> in;
+  in.mx = f.mx + c0; in.my = f.my + c0;
+  in.ux = (p.diva ? f.ubar_x : f.ux) + c0;
+  in.uy = (p.diva ? f.ubar_y : f.uy) + c0;
+  const R* N = f.visc_int + c0;
+  const R* fi = f.f_ice + c0;
+  const R* zb = f.z_bed + c0;
+  const R* zs = f.z_sl + c0;
+  const double dx = (double)(R)mb.dx, dy = (double)(R)mb.dy;
+  double a[2][YG_NSLOT], bb[2], xx[2];
+  uint32_t pm[2] = {0u, 0u};
+  bool bad = false;
+#pragma unroll
+  for (int comp = 0; comp < 2; ++comp) {
+    int ri, rj;
+    double v;
+    const int k = en::resolve<R>(g, in, i, j, comp, ri, rj, v);
+    en::Row r;
+#pragma unroll
+    for (int s = 0; s < YG_NSLOT; ++s) r.a[s] = 0.0;
+    r.b = 0.0;
+    r.bad = false;
+    int cf = -1;
+    if (k == en::FREE) {
+      en_member<R>(r, g, in, f, c0, N, fi, zb, zs, p.BC, dx, dy, comp, i, j, i, j);
+      // unknowns of this component tied to (i, j): edge neighbours, visited
+      // in increasing unknown order
+      for (int ii = i - 1; ii <= i + 1; ++ii)
+        for (int jj = j - 1; jj <= j + 1; ++jj) {
+          if ((ii == i && jj == j) || ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+          int ti, tj;
+          double tv;
+          if (en::resolve<R>(g, in, ii, jj, comp, ti, tj, tv) == en::TIED && ti == i && tj == j)
+            en_member<R>(r, g, in, f, c0, N, fi, zb, zs, p.BC, dx, dy, comp, i, j, ii, jj);
+        }
+      xx[comp] = (double)(comp ? in.uy[I2(i, j)] : in.ux[I2(i, j)]);
+      bb[comp] = r.b;
+    } else {
+      r.a[0] = 1.0;
+      if (k == en::DIR) { bb[comp] = v; xx[comp] = v; }
+      else if (k == en::TIED) { bb[comp] = 0.0; xx[comp] = 0.0; cf = 2 * (ri * ny + rj) + comp; }
+      else { bb[comp] = 0.0; xx[comp] = 0.0; r.bad = true; }
+    }
+    bad |= r.bad;
+#pragma unroll
+    for (int s = 0; s < YG_NSLOT; ++s) {
+      a[comp][s] = r.a[s];
+      if (r.a[s] != 0.0) pm[comp] |= 1u << s;
+    }
+    f.cfrom[2 * gq + comp] = cf;
+  }
+  if (bad) f.asm_err[m] = 1;
+  const size_t base = (size_t)YG_NA * gq;
+#pragma unroll
+  for (int s = 0; s < YG_NSLOT; ++s) {
+    if (A64) { A64[base + 2 * s] = a[0][s]; A64[base + 2 * s + 1] = a[1][s]; }
+    if (A32) { A32[base + 2 * s] = (float)a[0][s]; A32[base + 2 * s + 1] = (float)a[1][s]; }
+  }
+  if (A64) { A64[base + 18] = 0.0; A64[base + 19] = 0.0; }
+  if (A32) { A32[base + 18] = 0.0f; A32[base + 19] = 0.0f; }
+  bvec[2 * gq] = bb[0]; bvec[2 * gq + 1] = bb[1];
+  x0[2 * gq] = xx[0]; x0[2 * gq + 1] = xx[1];
+  maskv[gq] = pm[0] | (pm[1] << 16);
+}
+
 // ---- Picard epilogue (one block per member) --------------------------------
 // linear_solver_save_velocity (solver_ssa_ac.f90:33-71), ssa_vel_clip
 // (:1069-1093), picard_relax_vel, picard_calc_convergence_l2 and _l1rel
@@ -792,7 +1121,14 @@ __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f,
   for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
     const int i = c % nx, j = c / nx;
     const long long n = c0 + (long long)i * ny + j;
-    R vx = (R)x[2 * n], vy = (R)x[2 * n + 1];
+    long long sx = 2 * n, sy = 2 * n + 1;
+    if (f.cfrom) {
+      // rows tied to another row take its solution (solver_ssa_ac.f90:51-59)
+      const int qx = f.cfrom[sx], qy = f.cfrom[sy];
+      if (qx >= 0) sx = 2 * c0 + qx;
+      if (qy >= 0) sy = 2 * c0 + qy;
+    }
+    R vx = (R)x[sx], vy = (R)x[sy];
     if (yabs<R>(vx) < tol_uf) vx = R(0);
     if (yabs<R>(vy) < tol_uf) vy = R(0);
     if (vx > ulim) vx = ulim;
@@ -946,8 +1282,12 @@ template <typename R>
 void yg_launch_assemble(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
                         const int* skip, double* A64, float* A32, double* b, double* x0,
                         uint32_t* mask, hipStream_t st) {
-  hipLaunchKernelGGL(k_assemble<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p,
-                     skip, A64, A32, b, x0, mask);
+  if (p.energy)
+    hipLaunchKernelGGL(k_assemble_energy<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G,
+                       f, p, skip, A64, A32, b, x0, mask);
+  else
+    hipLaunchKernelGGL(k_assemble<R>, dim3(nblk(G.total_cells, 256)), dim3(256), 0, st, G, f, p,
+                       skip, A64, A32, b, x0, mask);
 }
 
 template <typename R>

@@ -110,7 +110,7 @@ struct Batch {
     if (rc) { h->err = e; return rc; }
     prec = par->precision;
     kmode = prec == YGPU_PREC_MIXED ? YG_MODE_MIXED : YG_MODE_FP64;
-    if (par->ssa_solver != 0) { h->err = "ssa_solver \"energy\" is not available"; return YGPU_ERR_UNSUPPORTED; }
+    if (par->ssa_solver != 0 && par->ssa_solver != 1) { h->err = "unknown ssa_solver"; return YGPU_ERR_ARG; }
     const int BC = bc_code(par->boundaries);
     if (BC < 0) { h->err = "unknown boundaries code"; return YGPU_ERR_ARG; }
     if (opts.k.precon == 2 && (BC == YG_BC_PERIODIC || BC == YG_BC_PERIODIC_X ||
@@ -122,6 +122,7 @@ struct Batch {
     fp.visc_method = par->visc_method; fp.beta_method = par->beta_method;
     fp.beta_gl_scale = par->beta_gl_scale; fp.beta_gl_stag = par->beta_gl_stag;
     fp.vel_lim_method = par->ssa_vel_lim_method; fp.diva = is_diva; fp.no_slip = par->no_slip;
+    fp.energy = par->ssa_solver == 1;
     fp.visc_const = par->visc_const; fp.beta_const = par->beta_const; fp.beta_q = par->beta_q;
     fp.beta_u0 = par->beta_u0; fp.beta_gl_f = par->beta_gl_f; fp.H_grnd_lim = par->H_grnd_lim;
     fp.beta_min = par->beta_min; fp.eps_0 = par->eps_0; fp.vel_lim_tau = par->ssa_vel_lim_tau;
@@ -244,6 +245,11 @@ struct Batch {
     f.zstride = zstride;
     YG_CHECK(h, ar.alloc((int32_t**)&f.mx, C));
     YG_CHECK(h, ar.alloc((int32_t**)&f.my, C));
+    if (fp.energy) {
+      YG_CHECK(h, ar.alloc(&f.cfrom, 2 * C));
+      YG_CHECK(h, ar.alloc(&f.asm_err, nmem));
+      YG_CHECK(h, hipMemsetAsync(f.asm_err, 0, sizeof(int) * nmem, h->stream));
+    }
     if (diva) {
       R** d2[] = {&f.F2, &f.F2_acx, &f.F2_acy, &f.beff_acx, &f.beff_acy, &f.beta_eff, &f.ubar_x,
                   &f.ubar_y};
@@ -445,6 +451,7 @@ struct Batch {
     YG_CHECK(h, hipMemsetAsync(d_lin_fail, 0, sizeof(int) * nmem, st));
     YG_CHECK(h, hipMemsetAsync(d_iter_now, 0, sizeof(int) * nmem, st));
     YG_CHECK(h, hipMemsetAsync(d_resid, 0, sizeof(double) * nmem, st));
+    if (f.asm_err) YG_CHECK(h, hipMemsetAsync(f.asm_err, 0, sizeof(int) * nmem, st));
     hipEvent_t e0, e1;
     YG_CHECK(h, hipEventCreate(&e0));
     YG_CHECK(h, hipEventCreate(&e1));
@@ -525,6 +532,17 @@ struct Batch {
       fprintf(stderr, "ygpu phases member0 (ms): factor %.3f sweepL %.3f sweepU %.3f spmv %.3f total %.3f\n",
               tmh[0] * us * 1e-3, tmh[1] * us * 1e-3, tmh[2] * us * 1e-3, tmh[3] * us * 1e-3,
               tmh[7] * us * 1e-3);
+    }
+    if (f.asm_err) {
+      std::vector<int> bad(nmem);
+      YG_CHECK(h, hipMemcpyAsync(bad.data(), f.asm_err, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
+      YG_CHECK(h, hipStreamSynchronize(st));
+      for (int k = 0; k < nmem; ++k)
+        if (bad[k]) {
+          h->err = "energy assembly: domain " + std::to_string(k) +
+                   " has a row outside the 9-point stencil";
+          return YGPU_ERR_UNSUPPORTED;
+        }
     }
     yg_launch_finish<R>(G, f, fp, ps, nullptr, st);
     if (diva) yg_launch_diva_finish<R>(G, f, dv, fp, nullptr, st);

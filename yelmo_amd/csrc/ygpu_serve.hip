@@ -151,6 +151,7 @@ struct Req {
   // results
   std::vector<R> out2d, out3d;
   int iter_now = 0, lin_iter = 0, lin_fail = 0, lim_n = 0, status = YGPU_OK;
+  int asm_bad = 0;             // energy assembly met a row outside the stencil
   double resid = 0.0, seconds = 0.0;
 };
 
@@ -216,6 +217,7 @@ struct Server : YgServerBase {
     double c = 4.0 + YG_NA * 8.0 + 2 * 8.0 + 2 * 8.0 + YG_NW64 * 2 * 8.0 + 8.0;
     if (b.kmode == YG_MODE_MIXED) c += YG_NA * 4.0 + YG_NF * 4.0 + YG_NW32 * 2 * 4.0;
     else c += YG_NF * 8.0;
+    if (b.fp.energy) c += 2 * 4.0;
     c += sizeof(R) * (34.0 + 3.0 * nz + (b.diva ? 8.0 + 4.0 * nz : 0.0));
     return c;
   }
@@ -414,6 +416,8 @@ struct Server : YgServerBase {
     YG_CHECK(h, hipMemcpyAsync(B.d_lin_iter + s, &h_zero_i[s], sizeof(int), hipMemcpyHostToDevice, st));
     YG_CHECK(h, hipMemcpyAsync(B.d_lin_fail + s, &h_zero_i[s], sizeof(int), hipMemcpyHostToDevice, st));
     YG_CHECK(h, hipMemcpyAsync(B.d_resid + s, &h_zero_d[s], sizeof(double), hipMemcpyHostToDevice, st));
+    if (B.f.asm_err)
+      YG_CHECK(h, hipMemcpyAsync(B.f.asm_err + s, &h_zero_i[s], sizeof(int), hipMemcpyHostToDevice, st));
     return YGPU_OK;
   }
 
@@ -499,6 +503,9 @@ struct Server : YgServerBase {
     YG_CHECK(h, hipMemcpyAsync(&r.lin_fail, B.d_lin_fail + s, sizeof(int), hipMemcpyDeviceToHost, st));
     YG_CHECK(h, hipMemcpyAsync(&r.lim_n, B.d_lim_n + s, sizeof(int), hipMemcpyDeviceToHost, st));
     YG_CHECK(h, hipMemcpyAsync(&r.resid, B.d_resid + s, sizeof(double), hipMemcpyDeviceToHost, st));
+    r.asm_bad = 0;
+    if (B.f.asm_err)
+      YG_CHECK(h, hipMemcpyAsync(&r.asm_bad, B.f.asm_err + s, sizeof(int), hipMemcpyDeviceToHost, st));
     return YGPU_OK;
   }
 
@@ -549,7 +556,8 @@ struct Server : YgServerBase {
           auto node = running.find(sl.ticket);
           std::unique_ptr<Req<R>> r = std::move(node->second);
           running.erase(node);
-          r->status = h_done[s] < 0 ? YGPU_ERR_NONFINITE : YGPU_OK;
+          r->status = r->asm_bad ? YGPU_ERR_UNSUPPORTED
+                                 : h_done[s] < 0 ? YGPU_ERR_NONFINITE : YGPU_OK;
           r->seconds = std::chrono::duration<double>(now - r->t_admit).count();
           if (r->flags & YGPU_SEQ_LAST) {
             cells.give(sl.coff, sl.ncell);

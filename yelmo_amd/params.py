@@ -13,8 +13,15 @@ from . import _lib
 DEFAULT_LIS_OPT = "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false"
 
 
+ENERGY_LIS_OPT = "-i cg -p jacobi -maxiter 200 -tol 1.0e-2 -initx_zeros false"
+
+
 @dataclass
 class SsaParams:
+    """Defaults are the benchmark workloads' settings (residual assembly,
+    BiCGSTAB + ILU(0) at rtol 1e-8, Picard tolerance 1e-4, power-law
+    friction q = 0.2); ``SsaParams.yelmo_defaults()`` gives the &ydyn
+    defaults of the reference."""
     ssa_solver: str = "residual"
     ssa_lis_opt: str = DEFAULT_LIS_OPT
     boundaries: str = "zeros"
@@ -47,6 +54,16 @@ class SsaParams:
 
     def with_(self, **kw) -> "SsaParams":
         return replace(self, **kw)
+
+    @classmethod
+    def yelmo_defaults(cls, **kw) -> "SsaParams":
+        """The &ydyn defaults (input/yelmo_defaults.nml:140-173): energy
+        assembly with CG + Jacobi at rtol 1e-2, linear friction
+        (beta_method 1, q 1, u0 100), Picard tolerance 1e-2 within 20
+        iterations."""
+        base = cls(ssa_solver="energy", ssa_lis_opt=ENERGY_LIS_OPT, beta_method=1, beta_q=1.0,
+                   beta_u0=100.0, ssa_iter_max=20, ssa_iter_conv=1e-2)
+        return replace(base, **kw)
 
     def to_c(self) -> _lib.SsaParamsC:
         c = _lib.SsaParamsC()
