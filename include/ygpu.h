@@ -197,28 +197,30 @@ int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out);  /* returns and resets     
 
 /* ---- served solve sequences with continuous batching (no counterpart in
  * the reference, which solves one domain per call; these are the submit /
- * poll entry points of a request loop).  A server holds max_slots member
- * slots and one device arena of at most hbm_budget_bytes (domains up to
- * max_side x max_side, up to nz_max levels; the arena is never larger than
- * max_slots domains of the largest size).  ygpu_submit copies one solve's
- * arrays and queues it under a caller-chosen sequence id: the first solve of
- * a sequence carries its INOUT state; solves flagged YGPU_SEQ_CONTINUE read
- * only the IN fields (and masks) and start from the previous solve's
- * device-resident velocity, viscosity and friction; YGPU_SEQ_LAST releases
- * the sequence's slot once that solve finishes.  ygpu_server_step admits
- * queued solves as slots and arena cells free up (by device footprint) and
- * runs one Picard iteration of every admitted solve in shared launches;
- * finished solves are downloaded at once.  ygpu_poll returns 1 and the next
- * finished ticket (0 when none); ygpu_fetch copies that solve's results into
- * out (same grid; solve_seconds = time from admission to completion).  The
- * host arrays passed to ygpu_submit may be reused as soon as it returns.
- * One server per handle stream; calls on one server are not thread-safe.   */
+ * poll entry points of a request loop).  A server holds nlanes size classes:
+ * lane l takes domains up to lane_max_side[l] (increasing) in lane_slots[l]
+ * member slots, with its own HIP stream and its share of one device arena of
+ * at most hbm_budget_bytes (never more than its slots can fill; up to nz_max
+ * levels).  ygpu_submit copies one solve's arrays and queues it under a
+ * caller-chosen sequence id: the first solve of a sequence carries its INOUT
+ * state; solves flagged YGPU_SEQ_CONTINUE read only the IN fields (and masks)
+ * and start from the previous solve's device-resident velocity, viscosity
+ * and friction; YGPU_SEQ_LAST releases the sequence's slot once that solve
+ * finishes.  A sequence holds its slot until then.  ygpu_server_step admits
+ * queued solves as slots and arena cells free up (the smallest lane that
+ * takes the grid, else a larger one with room), starts one Picard iteration
+ * of every admitted solve on each idle lane, and returns once some lane has
+ * finished an iteration; finished solves are downloaded at once.  ygpu_poll
+ * returns 1 and the next finished ticket (0 when none); ygpu_fetch copies that
+ * solve's results into out (same grid; solve_seconds = time from admission to
+ * completion).  Host arrays passed to ygpu_submit may be reused as soon as it
+ * returns.  Calls on one server are not thread-safe.                        */
 #define YGPU_SEQ_CONTINUE 1
 #define YGPU_SEQ_LAST 2
 typedef struct ygpu_server ygpu_server;
-int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva, int32_t max_slots,
-                       int32_t max_side, int32_t nz_max, int64_t hbm_budget_bytes,
-                       ygpu_server** out, int64_t* device_bytes);
+int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva, int32_t nlanes,
+                       const int32_t* lane_max_side, const int32_t* lane_slots, int32_t nz_max,
+                       int64_t hbm_budget_bytes, ygpu_server** out, int64_t* device_bytes);
 int ygpu_submit(ygpu_server* s, const ygpu_ssa_problem* prob, int64_t seq, int32_t flags,
                 int64_t* ticket);
 int ygpu_server_step(ygpu_server* s, int32_t* n_running, int32_t* n_queued);

@@ -43,12 +43,12 @@ def _alone(seq, par):
     return out
 
 
-def _served(seqs, par, *, max_slots, budget, arrive_every=1):
+def _served(seqs, par, *, max_slots=4, budget, arrive_every=1, lanes=None):
     """All sequences through one server; sequence k arrives at step
     k * arrive_every and submits its next solve as soon as the previous one
     is fetched.  Returns results[seq][solve] = (state, info)."""
     srv = Server(par, max_slots=max_slots, max_side=max(s[0].nx for s in seqs),
-                 nz_max=seqs[0][0].nz, hbm_budget=budget)
+                 nz_max=seqs[0][0].nz, hbm_budget=budget, lanes=lanes)
     res = [[None] * len(s) for s in seqs]
     owner = {}
     arrived = 0
@@ -62,7 +62,7 @@ def _served(seqs, par, *, max_slots, budget, arrive_every=1):
         nr, nq = srv.step()
         max_running = max(max_running, nr)
         steps += 1
-        assert steps < 2000, "server made no progress"
+        assert steps < 20000, "server made no progress"
         for t in srv.poll():
             k, j = owner.pop(t)
             q = seqs[k][j].copy()
@@ -97,6 +97,30 @@ def test_served_alone_equals_in_mix(prec):
             assert inf_m.status == inf_a.status == 0
             assert (inf_m.ssa_iter_now, inf_m.ssa_lin_iter) == (inf_a.ssa_iter_now,
                                                                inf_a.ssa_lin_iter)
+            for n in COMPARE:
+                assert np.array_equal(st_m[n], st_a[n]), (k, j, n)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_served_lanes_alone_equals_in_mix(prec):
+    """Two size-class lanes stepping on their own streams (sides up to 45
+    with two slots, up to 65 with two slots); small sequences overflow into
+    the large lane when theirs is full.  Every solve equals its sequence
+    alone, bit for bit."""
+    print(f"case serve lanes {prec}", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision=prec)
+    seqs = _sequences((33, 65, 41, 57, 45, 37), 3, np.float64, seed=21)
+    alone = [_alone(s, par) for s in seqs]
+    mixed, max_running = _served(seqs, par, budget=1 << 30, arrive_every=0,
+                                 lanes=[(45, 2), (65, 2)])
+    assert max_running >= 3
+    for k in range(len(seqs)):
+        for j in range(len(seqs[k])):
+            st_a, inf_a = alone[k][j]
+            st_m, inf_m = mixed[k][j]
+            assert inf_m.status == inf_a.status == 0
+            assert (inf_m.ssa_iter_now, inf_m.ssa_lin_iter) == (inf_a.ssa_iter_now,
+                                                               inf_a.ssa_lin_iter), (k, j)
             for n in COMPARE:
                 assert np.array_equal(st_m[n], st_a[n]), (k, j, n)
 

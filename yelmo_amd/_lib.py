@@ -123,7 +123,7 @@ def load() -> C.CDLL:
     lib.ygpu_stats_enable.restype = C.c_int
     lib.ygpu_stats_get.argtypes = [PP, C.POINTER(StatsC)]
     lib.ygpu_stats_get.restype = C.c_int
-    lib.ygpu_server_create.argtypes = [PP, C.POINTER(SsaParamsC), I, I, I, I, C.c_int64,
+    lib.ygpu_server_create.argtypes = [PP, C.POINTER(SsaParamsC), I, I, PP, PP, I, C.c_int64,
                                        C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
     lib.ygpu_server_create.restype = C.c_int
     lib.ygpu_submit.argtypes = [PP, C.POINTER(SsaProblemC), C.c_int64, I, C.POINTER(C.c_int64)]

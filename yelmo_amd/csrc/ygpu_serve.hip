@@ -1,24 +1,30 @@
 // ygpu_serve.hip -- continuous batching of served solve sequences.
 //
-// A server owns a fixed table of member slots and one device arena of cells
-// whose size is set by an HBM budget.  A request is one solve of a
-// sequence; it is admitted into a slot as soon as a slot and a large enough
-// cell range are free, and from then on advances one Picard iteration per
-// step in the same launches as every other admitted solve, so requests that
-// arrive while others run overlap on the device.  A finished solve's
-// outputs are downloaded at once; its slot keeps the velocity, viscosity and
-// friction resident as the warm start of the sequence's next solve, until
-// the solve submitted with YGPU_SEQ_LAST finishes.
+// A server owns member slots and one device arena of cells whose size is
+// set by an HBM budget, split into lanes by domain size.  Each lane has its
+// own slots, its own part of the arena and its own HIP stream.  A request is
+// one solve of a sequence; it is admitted into a slot of the smallest lane
+// that takes its grid (or a larger one with room) as soon as a slot and a
+// large enough cell range are free.  From then on it advances one Picard
+// iteration per lane step, in the same launches as every other admitted
+// solve of its lane, so requests that arrive while others run overlap on the
+// device.  Lanes step independently: a lane of small domains runs many
+// Picard iterations while a lane of large domains runs one, and their
+// kernels share the CUs.  A finished solve's outputs are downloaded at
+// once; its slot keeps the velocity, viscosity and friction resident as the
+// warm start of the sequence's next solve, until the solve submitted with
+// YGPU_SEQ_LAST finishes.
 //
 // Per-member state is what makes this work with the batch kernels: every
 // kernel skips members whose done flag is set, each member counts its own
 // Picard iterations, and per-cell kernels find their member through the
-// sorted list of occupied cell ranges, so slots can sit anywhere in the
-// arena.  A member's arithmetic depends only on its own data and the block
-// size, which the server fixes, so a solve's result is the same whatever
-// else shares the device.
+// sorted list of occupied cell ranges, so slots can sit anywhere in a lane's
+// arena.  A member's arithmetic depends only on its own data and the lane's
+// block size, so a solve's result is the same whatever else shares the
+// device.
 #include <deque>
 #include <map>
+#include <thread>
 #include <memory>
 #include <unordered_map>
 
@@ -31,8 +37,9 @@ namespace {
 // Cell ranges are handed out in whole granules: 64 cells keep the work
 // vector offset (2 values per cell) a multiple of 4 and the records aligned.
 constexpr long long kGranule = 64;
-// A new sequence that cannot be placed may be overtaken by smaller ones for
-// this many steps; after that, admission of new sequences waits for it.
+// A new sequence that cannot be placed may be overtaken by later ones for
+// this many lane steps; after that, no later sequence takes a lane it could
+// use until it is placed.
 constexpr int kMaxBypassSteps = 8;
 
 // first-fit allocator of cell ranges; released ranges merge with neighbours
@@ -159,6 +166,7 @@ enum SlotState { kFree = 0, kRunning = 1, kResident = 2 };
 
 struct Slot {
   int state = kFree;
+  int lane = 0;
   int64_t seq = 0;
   int64_t ticket = 0;
   long long coff = 0, ncell = 0;  // cell range of the arena
@@ -167,6 +175,21 @@ struct Slot {
   YgMember M{};
   long long zoff = 0;
   std::vector<int> st;
+};
+
+// One size class: slots [s0, s0 + ns), cells [c0, c0 + cap), its stream.
+struct Lane {
+  int s0 = 0, ns = 0, side = 0, block = 512;
+  long long c0 = 0, cap = 0;
+  CellRanges cells;          // offsets relative to c0
+  hipStream_t stream = nullptr;
+  hipEvent_t done_ev = nullptr;
+  int* h_done = nullptr;     // pinned: done flags of the lane's slots
+  bool inflight = false;
+  int nseg = 0;
+  long long hi_cell = 0;     // one past the last occupied cell
+  std::vector<long long> h_seg;
+  std::vector<int> h_segm, h_sel;
 };
 
 }  // namespace
@@ -189,7 +212,7 @@ struct Server : YgServerBase {
   int S = 0, nzmax = 0, maxside = 0;
   double n_glen = 0.0;
   bool have_glen = false;
-  CellRanges cells;
+  std::vector<Lane> lanes;
   std::vector<Slot> slots;
   std::deque<std::unique_ptr<Req<R>>> pending;
   std::unordered_map<int64_t, std::unique_ptr<Req<R>>> running;  // ticket -> solve in a slot
@@ -199,18 +222,23 @@ struct Server : YgServerBase {
   struct Seq { int slot = -1; int nx = 0, ny = 0, nz = 0; bool ended = false; };
   std::unordered_map<int64_t, Seq> seqs;
   int64_t next_ticket = 1;
-  // device per-slot selection flags and occupied-range list; host mirrors
+  // device per-slot selection flags and occupied-range lists (per lane)
   int* d_sel = nullptr;
   long long* d_seg = nullptr;
   int* d_segm = nullptr;
-  std::vector<int> h_done, h_sel, h_zero_i, h_idle;
-  std::vector<long long> h_seg;
-  std::vector<int> h_segm;
+  std::vector<int> h_zero_i, h_idle;
   std::vector<double> h_zero_d;
-  int nseg = 0;
-  long long hi_cell = 0;
 
   int64_t device_bytes() const override { return (int64_t)B.ar.bytes; }
+
+  ~Server() override {
+    for (Lane& L : lanes) {
+      if (L.stream) (void)hipStreamSynchronize(L.stream);
+      if (L.done_ev) (void)hipEventDestroy(L.done_ev);
+      if (L.stream) (void)hipStreamDestroy(L.stream);
+      if (L.h_done) (void)hipHostFree(L.h_done);
+    }
+  }
 
   // device bytes per arena cell at this precision and nz
   static double bytes_per_cell(const Batch<R>& b, int nz) {
@@ -222,42 +250,74 @@ struct Server : YgServerBase {
     return c;
   }
 
-  int create(ygpu_handle* hh, const ygpu_ssa_params* par, int diva, int nslots, int max_side,
-             int nz_max, int64_t budget) {
+  int create(ygpu_handle* hh, const ygpu_ssa_params* par, int diva, int nlanes,
+             const int32_t* lane_side, const int32_t* lane_slots, int nz_max, int64_t budget) {
     h = hh;
     int rc = B.setup_params(hh, par, diva);
     if (rc) return rc;
-    if (nslots < 1 || max_side < 3 || max_side > YG_MAX_LINES || nz_max < 2 || budget <= 0) {
-      h->err = "server: bad slot count, side, nz or budget";
+    if (nlanes < 1 || nlanes > 8 || !lane_side || !lane_slots || nz_max < 2 || budget <= 0) {
+      h->err = "server: bad lane list, nz or budget";
       return YGPU_ERR_ARG;
     }
-    S = nslots; nzmax = nz_max; maxside = max_side;
+    nzmax = nz_max;
+    S = 0;
+    double want = 0.0;  // cells the lanes need to fill every slot at full size
+    for (int l = 0; l < nlanes; ++l) {
+      if (lane_slots[l] < 1 || lane_side[l] < 3 || lane_side[l] > YG_MAX_LINES ||
+          (l > 0 && lane_side[l] <= lane_side[l - 1])) {
+        h->err = "server: lane sides must increase, each lane needs a slot";
+        return YGPU_ERR_ARG;
+      }
+      S += lane_slots[l];
+      want += (double)lane_slots[l] * granules((long long)lane_side[l] * lane_side[l]);
+    }
+    maxside = lane_side[nlanes - 1];
     const double bpc = bytes_per_cell(B, nz_max);
-    long long cap = (long long)((double)budget / bpc) / kGranule * kGranule;
-    // no more cells than the slots can occupy at once
-    cap = std::min(cap, (long long)S * granules((long long)max_side * max_side));
-    if (cap < granules((long long)max_side * max_side)) {
-      h->err = "server: the HBM budget does not hold one domain of the largest side";
-      return YGPU_ERR_ARG;
+    const double have = (double)budget / bpc;   // cells the budget holds
+    lanes.assign(nlanes, Lane{});
+    long long c0 = 0;
+    int s0 = 0;
+    for (int l = 0; l < nlanes; ++l) {
+      Lane& L = lanes[l];
+      const long long full = (long long)lane_slots[l] * granules((long long)lane_side[l] * lane_side[l]);
+      long long cap = want <= have ? full : (long long)(have * (double)full / want);
+      cap = cap / kGranule * kGranule;
+      if (cap < granules((long long)lane_side[l] * lane_side[l])) {
+        h->err = "server: the HBM budget does not hold one domain of lane " + std::to_string(l);
+        return YGPU_ERR_ARG;
+      }
+      L.s0 = s0; L.ns = lane_slots[l]; L.side = lane_side[l];
+      L.c0 = c0; L.cap = cap;
+      L.cells.init(cap);
+      L.block = Batch<R>::pick_block(par, lane_side[l]);
+      L.h_seg.assign(L.ns, 0);
+      L.h_segm.assign(L.ns, 0);
+      L.h_sel.assign(L.ns, 1);
+      s0 += L.ns;
+      c0 += cap;
     }
-    cells.init(cap);
     B.nmem = S;
     B.mem.assign(S, YgMember{});
     B.coffs.assign(S + 1, 0);
     B.zoffs.assign(S + 1, 0);
     B.soff.assign((size_t)S * (YG_MAX_STEPS + 1), 0);
-    B.total_cells = cap;
-    B.total_z = cap * nz_max;
-    B.total_v = 2 * cap;
+    B.total_cells = c0;
+    B.total_z = c0 * nz_max;
+    B.total_v = 2 * c0;
     B.zstride = nz_max;
-    B.block = Batch<R>::pick_block(par, max_side);
+    B.block = lanes.back().block;
     if ((rc = B.allocate())) return rc;
     YG_CHECK(h, B.ar.alloc(&d_sel, S));
     YG_CHECK(h, B.ar.alloc(&d_seg, S));
     YG_CHECK(h, B.ar.alloc(&d_segm, S));
+    for (Lane& L : lanes) {
+      YG_CHECK(h, hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+      YG_CHECK(h, hipEventCreateWithFlags(&L.done_ev, hipEventDisableTiming));
+      YG_CHECK(h, hipHostMalloc((void**)&L.h_done, sizeof(int) * L.ns, hipHostMallocDefault));
+    }
     slots.assign(S, Slot{});
-    h_done.assign(S, 3);
-    h_sel.assign(S, 1);
+    for (Lane& L : lanes)
+      for (int k = 0; k < L.ns; ++k) slots[L.s0 + k].lane = (int)(&L - lanes.data());
     h_idle.assign(S, 3);
     h_zero_i.assign(S, 0);
     h_zero_d.assign(S, 0.0);
@@ -280,7 +340,7 @@ struct Server : YgServerBase {
       return YGPU_ERR_ARG;
     }
     if (p->nx > maxside || p->ny > maxside || p->nz > nzmax) {
-      h->err = "submit: domain larger than the server's maximum side or nz";
+      h->err = "submit: domain larger than the server's largest lane or nz";
       return YGPU_ERR_ARG;
     }
     if (cont && (p->nx != it->second.nx || p->ny != it->second.ny || p->nz != it->second.nz)) {
@@ -344,39 +404,55 @@ struct Server : YgServerBase {
     return YGPU_OK;
   }
 
-  // sorted list of occupied cell ranges -> device
-  int publish_ranges() {
+  // sorted list of a lane's occupied cell ranges -> device
+  int publish_ranges(Lane& L) {
     std::vector<std::pair<long long, int>> occ;
-    hi_cell = 0;
-    for (int s = 0; s < S; ++s)
-      if (slots[s].state != kFree) {
-        occ.push_back({slots[s].coff, s});
-        hi_cell = std::max(hi_cell, slots[s].coff + slots[s].ncell);
-      }
+    L.hi_cell = L.c0;
+    for (int k = 0; k < L.ns; ++k) {
+      const Slot& sl = slots[L.s0 + k];
+      if (sl.state == kFree) continue;
+      occ.push_back({sl.coff, k});
+      L.hi_cell = std::max(L.hi_cell, sl.coff + sl.ncell);
+    }
     std::sort(occ.begin(), occ.end());
-    nseg = (int)occ.size();
-    h_seg.resize(std::max(nseg, 1));
-    h_segm.resize(std::max(nseg, 1));
-    for (int k = 0; k < nseg; ++k) { h_seg[k] = occ[k].first; h_segm[k] = occ[k].second; }
-    if (nseg) {
-      YG_CHECK(h, hipMemcpyAsync(d_seg, h_seg.data(), sizeof(long long) * nseg, hipMemcpyHostToDevice, h->stream));
-      YG_CHECK(h, hipMemcpyAsync(d_segm, h_segm.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, h->stream));
+    L.nseg = (int)occ.size();
+    for (int k = 0; k < L.nseg; ++k) { L.h_seg[k] = occ[k].first; L.h_segm[k] = occ[k].second; }
+    if (L.nseg) {
+      YG_CHECK(h, hipMemcpyAsync(d_seg + L.s0, L.h_seg.data(), sizeof(long long) * L.nseg,
+                                 hipMemcpyHostToDevice, L.stream));
+      YG_CHECK(h, hipMemcpyAsync(d_segm + L.s0, L.h_segm.data(), sizeof(int) * L.nseg,
+                                 hipMemcpyHostToDevice, L.stream));
     }
     return YGPU_OK;
   }
 
-  YgBatchGeom geom() const {
+  // the lane's slots as a batch: member arrays offset by the lane's first slot
+  YgBatchGeom geom(const Lane& L) const {
     YgBatchGeom G = B.geom();
-    G.total_cells = hi_cell;
-    G.seg = d_seg; G.segm = d_segm; G.nseg = nseg;
+    G.members = B.d_mem + L.s0;
+    G.coffs = B.d_coffs + L.s0;
+    G.zoffs = B.d_zoffs + L.s0;
+    G.nmem = L.ns;
+    G.total_cells = L.hi_cell;
+    G.seg = d_seg + L.s0; G.segm = d_segm + L.s0; G.nseg = L.nseg;
     return G;
   }
+  YgFields<R> fields(const Lane& L) const {
+    YgFields<R> f = B.f;
+    f.zeta += (size_t)L.s0 * B.zstride;
+    if (f.asm_err) f.asm_err += L.s0;
+    return f;
+  }
+  YgPicardState pstate(const Lane& L) const {
+    return YgPicardState{B.d_iter_now + L.s0, B.d_done + L.s0, B.d_resid + L.s0,
+                         B.d_lin_iter + L.s0, B.d_lin_fail + L.s0, B.d_lim_n + L.s0};
+  }
 
-  // copies one admitted solve into its slot (asynchronous on the stream)
+  // copies one admitted solve into its slot (asynchronous on the lane's stream)
   int load(int s, Req<R>& r) {
     Slot& sl = slots[s];
     const long long c0 = sl.coff, nc = r.nc;
-    hipStream_t st = h->stream;
+    hipStream_t st = lanes[sl.lane].stream;
     R* in_dev[kNIn2D] = {(R*)B.f.c_bed, (R*)B.f.f_slide, (R*)B.f.taud_acx,
                          (R*)B.f.taud_acy, (R*)B.f.taul_acx, (R*)B.f.taul_acy, (R*)B.f.H_ice,
                          (R*)B.f.f_ice, (R*)B.f.H_grnd, (R*)B.f.f_grnd, (R*)B.f.z_bed,
@@ -421,43 +497,62 @@ struct Server : YgServerBase {
     return YGPU_OK;
   }
 
-  // admits queued solves into slots; returns the number admitted
-  int admit(int* n_admitted) {
-    *n_admitted = 0;
-    bool blocked_new = false;
-    std::vector<int> admitted;
+  // lanes a grid may use, smallest first
+  int first_lane(int nx, int ny) const {
+    const int side = std::max(nx, ny);
+    for (size_t l = 0; l < lanes.size(); ++l)
+      if (lanes[l].side >= side) return (int)l;
+    return -1;
+  }
+
+  // admits queued solves into slots of idle lanes; count_wait: a lane step
+  // ended since the last call (the unit of a request's waiting time)
+  int admit(bool count_wait) {
+    int block_from = (int)lanes.size();   // lanes held for a long-waiting request
+    std::vector<std::vector<int>> admitted(lanes.size());
     for (auto it = pending.begin(); it != pending.end();) {
       Req<R>& r = **it;
       Seq& sq = seqs[r.seq];
       int s = -1;
       if (!r.first) {
-        // continuation: its sequence's slot must hold the previous result
-        if (sq.slot < 0 || slots[sq.slot].state != kResident) { ++it; continue; }
+        // continuation: its sequence's slot must hold the previous result,
+        // and its lane must be between steps
+        if (sq.slot < 0 || slots[sq.slot].state != kResident ||
+            lanes[slots[sq.slot].lane].inflight) { ++it; continue; }
         s = sq.slot;
       } else {
-        if (blocked_new || sq.slot >= 0) { ++it; continue; }
-        int fs = -1;
-        for (int k = 0; k < S; ++k) if (slots[k].state == kFree) { fs = k; break; }
+        if (sq.slot >= 0) { ++it; continue; }
         const long long need = granules(r.nc);
-        const long long c0 = fs >= 0 ? cells.take(need) : -1;
-        if (fs < 0 || c0 < 0) {
-          if (++r.waited > kMaxBypassSteps) blocked_new = true;
+        const int l0 = first_lane(r.nx, r.ny);
+        for (int l = l0; l >= 0 && l < block_from && s < 0; ++l) {
+          Lane& L = lanes[l];
+          if (L.inflight) continue;
+          int fs = -1;
+          for (int k = 0; k < L.ns; ++k)
+            if (slots[L.s0 + k].state == kFree) { fs = L.s0 + k; break; }
+          if (fs < 0) continue;
+          const long long off = L.cells.take(need);
+          if (off < 0) continue;
+          s = fs;
+          Slot& sl = slots[s];
+          sl.coff = L.c0 + off; sl.ncell = need; sl.seq = r.seq;
+          sl.nx = r.nx; sl.ny = r.ny; sl.nz = r.nz;
+          sl.zoff = sl.coff * nzmax;
+          ygpu_ssa_problem P{};
+          P.nx = r.nx; P.ny = r.ny; P.nz = r.nz; P.real_bytes = 8; P.dx = r.dx; P.dy = r.dy;
+          int rc = B.describe(P, s, sl.M, sl.st);
+          if (rc) return rc;
+          sl.M.coff = sl.coff;
+          sl.M.toff = (long long)s * (YG_MAX_STEPS + 1);
+          sl.M.woff = 2 * sl.coff;
+          sq.slot = s;
+        }
+        if (s < 0) {
+          if (count_wait) ++r.waited;
+          if (r.waited > kMaxBypassSteps) block_from = std::min(block_from, l0);
           ++it;
           continue;
         }
-        s = fs;
-        Slot& sl = slots[s];
-        sl.coff = c0; sl.ncell = need; sl.seq = r.seq;
-        sl.nx = r.nx; sl.ny = r.ny; sl.nz = r.nz;
-        sl.zoff = c0 * nzmax;
-        ygpu_ssa_problem P{};
-        P.nx = r.nx; P.ny = r.ny; P.nz = r.nz; P.real_bytes = 8; P.dx = r.dx; P.dy = r.dy;
-        int rc = B.describe(P, s, sl.M, sl.st);
-        if (rc) return rc;
-        sl.M.coff = c0;
-        sl.M.toff = (long long)s * (YG_MAX_STEPS + 1);
-        sl.M.woff = 2 * c0;
-        sq.slot = s;
       }
       Slot& sl = slots[s];
       sl.state = kRunning;
@@ -465,28 +560,63 @@ struct Server : YgServerBase {
       int rc = load(s, r);
       if (rc) return rc;
       r.t_admit = std::chrono::steady_clock::now();
-      admitted.push_back(s);
+      admitted[sl.lane].push_back(s);
       running[r.ticket] = std::move(*it);
       it = pending.erase(it);
     }
-    *n_admitted = (int)admitted.size();
-    if (admitted.empty()) return YGPU_OK;
-    int rc = publish_ranges();
-    if (rc) return rc;
-    // set_inactive_margins and the error fields of the admitted slots only
-    std::fill(h_sel.begin(), h_sel.end(), 1);
-    for (int s : admitted) h_sel[s] = 0;
-    YG_CHECK(h, hipMemcpyAsync(d_sel, h_sel.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
-    yg_launch_init<R>(geom(), B.f, B.fp, d_sel, h->stream);
+    for (size_t l = 0; l < lanes.size(); ++l) {
+      if (admitted[l].empty()) continue;
+      Lane& L = lanes[l];
+      int rc = publish_ranges(L);
+      if (rc) return rc;
+      // set_inactive_margins and the error fields of the admitted slots only
+      std::fill(L.h_sel.begin(), L.h_sel.end(), 1);
+      for (int s : admitted[l]) L.h_sel[s - L.s0] = 0;
+      YG_CHECK(h, hipMemcpyAsync(d_sel + L.s0, L.h_sel.data(), sizeof(int) * L.ns,
+                                 hipMemcpyHostToDevice, L.stream));
+      yg_launch_init<R>(geom(L), fields(L), B.fp, d_sel + L.s0, L.stream);
+      YG_CHECK(h, hipGetLastError());
+      // the lane waits for its host copies before the next admission reuses them
+      YG_CHECK(h, hipStreamSynchronize(L.stream));
+    }
+    return YGPU_OK;
+  }
+
+  // one Picard iteration of every running solve of the lane, then an
+  // asynchronous read of its done flags
+  int launch(Lane& L) {
+    hipStream_t st = L.stream;
+    const YgBatchGeom G = geom(L);
+    const YgFields<R> f = fields(L);
+    const YgPicardState ps = pstate(L);
+    YgKrylovArgs ka{};
+    ka.members = B.d_mem + L.s0; ka.soff = B.d_soff; ka.mask = B.d_mask;
+    ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32;
+    ka.b = B.d_b; ka.x = B.d_x; ka.W64 = B.d_W64; ka.W32 = B.d_W32;
+    ka.total_cells = L.hi_cell;
+    ka.wrapx = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_X);
+    ka.wrapy = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_Y ||
+                B.fp.BC == YG_BC_MISMIP3D || B.fp.BC == YG_BC_TROUGH);
+    ka.skip = ps.done; ka.out_iters = B.d_kit + L.s0; ka.out_status = B.d_kst + L.s0;
+    ka.out_resid = B.d_kres + L.s0;
+    ka.opts = B.opts.k;
+    if (B.diva) yg_launch_diva_pre<R>(G, f, B.dv, B.fp, ps.done, st);
+    yg_launch_fields_pre<R>(G, f, B.fp, ps.done, st);
+    yg_launch_assemble<R>(G, f, B.fp, ps.done, B.d_A64,
+                          B.kmode == YG_MODE_MIXED ? B.d_A32 : nullptr, B.d_b, B.d_x, B.d_mask, st);
+    yg_launch_krylov(ka, B.kmode, L.ns, L.block, st);
+    yg_launch_epilogue<R>(G, f, B.fp, B.d_x, B.d_kit + L.s0, B.d_kst + L.s0, B.iter_max, ps, st);
     YG_CHECK(h, hipGetLastError());
+    YG_CHECK(h, hipMemcpyAsync(L.h_done, ps.done, sizeof(int) * L.ns, hipMemcpyDeviceToHost, st));
+    YG_CHECK(h, hipEventRecord(L.done_ev, st));
+    L.inflight = true;
     return YGPU_OK;
   }
 
   // downloads the results of a finished slot into its request
-  int collect(int s, Req<R>& r) {
+  int collect(int s, Req<R>& r, hipStream_t st) {
     Slot& sl = slots[s];
     const long long c0 = sl.coff, nc = r.nc, nzc = nc * r.nz;
-    hipStream_t st = h->stream;
     std::vector<const R*> o2 = {B.f.ux, B.f.uy, B.f.taub_x, B.f.taub_y, B.f.visc_int, B.f.err_x,
                                 B.f.err_y, B.f.beta, B.f.beta_acx, B.f.beta_acy};
     if (B.diva) { o2.push_back(B.f.ubar_x); o2.push_back(B.f.ubar_y); o2.push_back(B.f.beta_eff); }
@@ -509,73 +639,88 @@ struct Server : YgServerBase {
     return YGPU_OK;
   }
 
-  int step(int32_t* n_running, int32_t* n_queued) override {
-    int rc, nadm = 0;
-    if ((rc = admit(&nadm))) return rc;
-    int nrun = 0;
-    for (int s = 0; s < S; ++s) nrun += slots[s].state == kRunning;
-    if (nrun > 0) {
-      hipStream_t st = h->stream;
-      const YgBatchGeom G = geom();
-      YgPicardState ps{B.d_iter_now, B.d_done, B.d_resid, B.d_lin_iter, B.d_lin_fail, B.d_lim_n};
-      YgKrylovArgs ka{};
-      ka.members = B.d_mem; ka.soff = B.d_soff; ka.mask = B.d_mask;
-      ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32;
-      ka.b = B.d_b; ka.x = B.d_x; ka.W64 = B.d_W64; ka.W32 = B.d_W32;
-      ka.total_cells = hi_cell;
-      ka.wrapx = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_X);
-      ka.wrapy = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_Y ||
-                  B.fp.BC == YG_BC_MISMIP3D || B.fp.BC == YG_BC_TROUGH);
-      ka.skip = B.d_done; ka.out_iters = B.d_kit; ka.out_status = B.d_kst; ka.out_resid = B.d_kres;
-      ka.opts = B.opts.k;
-      if (B.diva) yg_launch_diva_pre<R>(G, B.f, B.dv, B.fp, B.d_done, st);
-      yg_launch_fields_pre<R>(G, B.f, B.fp, B.d_done, st);
-      yg_launch_assemble<R>(G, B.f, B.fp, B.d_done, B.d_A64,
-                            B.kmode == YG_MODE_MIXED ? B.d_A32 : nullptr, B.d_b, B.d_x, B.d_mask, st);
-      yg_launch_krylov(ka, B.kmode, S, B.block, st);
-      yg_launch_epilogue<R>(G, B.f, B.fp, B.d_x, B.d_kit, B.d_kst, B.iter_max, ps, st);
-      YG_CHECK(h, hipGetLastError());
-      YG_CHECK(h, hipMemcpyAsync(h_done.data(), B.d_done, sizeof(int) * S, hipMemcpyDeviceToHost, st));
-      YG_CHECK(h, hipStreamSynchronize(st));
-      std::vector<int> fin;
-      for (int s = 0; s < S; ++s)
-        if (slots[s].state == kRunning && h_done[s] != 0) fin.push_back(s);
-      if (!fin.empty()) {
-        std::fill(h_sel.begin(), h_sel.end(), 1);
-        for (int s : fin) h_sel[s] = 0;
-        YG_CHECK(h, hipMemcpyAsync(d_sel, h_sel.data(), sizeof(int) * S, hipMemcpyHostToDevice, st));
-        yg_launch_finish<R>(G, B.f, B.fp, ps, d_sel, st);
-        if (B.diva) yg_launch_diva_finish<R>(G, B.f, B.dv, B.fp, d_sel, st);
-        YG_CHECK(h, hipGetLastError());
-        for (int s : fin)
-          if ((rc = collect(s, *running[slots[s].ticket]))) return rc;
-        YG_CHECK(h, hipStreamSynchronize(st));
-        const auto now = std::chrono::steady_clock::now();
-        for (int s : fin) {
-          Slot& sl = slots[s];
-          auto node = running.find(sl.ticket);
-          std::unique_ptr<Req<R>> r = std::move(node->second);
-          running.erase(node);
-          r->status = r->asm_bad ? YGPU_ERR_UNSUPPORTED
-                                 : h_done[s] < 0 ? YGPU_ERR_NONFINITE : YGPU_OK;
-          r->seconds = std::chrono::duration<double>(now - r->t_admit).count();
-          if (r->flags & YGPU_SEQ_LAST) {
-            cells.give(sl.coff, sl.ncell);
-            sl = Slot{};
-            seqs.erase(r->seq);
-          } else {
-            sl.state = kResident;
-          }
-          // finished members stay skipped (their done flag is set)
-          completed.push_back(r->ticket);
-          results[r->ticket] = std::move(r);
-        }
-        if ((rc = publish_ranges())) return rc;
+  // a lane's step has ended: finish and download the solves that completed
+  int complete(Lane& L) {
+    L.inflight = false;
+    std::vector<int> fin;
+    for (int k = 0; k < L.ns; ++k)
+      if (slots[L.s0 + k].state == kRunning && L.h_done[k] != 0) fin.push_back(L.s0 + k);
+    if (fin.empty()) return YGPU_OK;
+    hipStream_t st = L.stream;
+    std::fill(L.h_sel.begin(), L.h_sel.end(), 1);
+    for (int s : fin) L.h_sel[s - L.s0] = 0;
+    YG_CHECK(h, hipMemcpyAsync(d_sel + L.s0, L.h_sel.data(), sizeof(int) * L.ns, hipMemcpyHostToDevice, st));
+    const YgBatchGeom G = geom(L);
+    const YgFields<R> f = fields(L);
+    yg_launch_finish<R>(G, f, B.fp, pstate(L), d_sel + L.s0, st);
+    if (B.diva) yg_launch_diva_finish<R>(G, f, B.dv, B.fp, d_sel + L.s0, st);
+    YG_CHECK(h, hipGetLastError());
+    int rc;
+    for (int s : fin)
+      if ((rc = collect(s, *running[slots[s].ticket], st))) return rc;
+    YG_CHECK(h, hipStreamSynchronize(st));
+    const auto now = std::chrono::steady_clock::now();
+    bool freed = false;
+    for (int s : fin) {
+      Slot& sl = slots[s];
+      auto node = running.find(sl.ticket);
+      std::unique_ptr<Req<R>> r = std::move(node->second);
+      running.erase(node);
+      r->status = r->asm_bad ? YGPU_ERR_UNSUPPORTED
+                             : L.h_done[s - L.s0] < 0 ? YGPU_ERR_NONFINITE : YGPU_OK;
+      r->seconds = std::chrono::duration<double>(now - r->t_admit).count();
+      if (r->flags & YGPU_SEQ_LAST) {
+        L.cells.give(sl.coff - L.c0, sl.ncell);
+        const int lane = sl.lane;
+        sl = Slot{};
+        sl.lane = lane;
+        seqs.erase(r->seq);
+        freed = true;
+      } else {
+        sl.state = kResident;
       }
+      // finished members stay skipped (their done flag is set)
+      completed.push_back(r->ticket);
+      results[r->ticket] = std::move(r);
+    }
+    if (freed && (rc = publish_ranges(L))) return rc;
+    return YGPU_OK;
+  }
+
+  bool lane_has_work(const Lane& L) const {
+    for (int k = 0; k < L.ns; ++k)
+      if (slots[L.s0 + k].state == kRunning) return true;
+    return false;
+  }
+
+  // Advances every lane by at least nothing and returns once one lane has
+  // finished a Picard iteration (or when nothing runs).
+  int step(int32_t* n_running, int32_t* n_queued) override {
+    int rc;
+    bool progressed = false;
+    for (int spin = 0;; ++spin) {
+      bool ended = spin == 0;
+      for (Lane& L : lanes) {
+        if (!L.inflight) continue;
+        const hipError_t q = hipEventQuery(L.done_ev);
+        if (q == hipErrorNotReady) continue;
+        YG_CHECK(h, q);
+        if ((rc = complete(L))) return rc;
+        progressed = ended = true;
+      }
+      if ((rc = admit(ended))) return rc;
+      bool any_inflight = false;
+      for (Lane& L : lanes) {
+        if (!L.inflight && lane_has_work(L) && (rc = launch(L))) return rc;
+        any_inflight |= L.inflight;
+      }
+      if (progressed || !any_inflight) break;
+      // a short back-off: the lanes' kernels run for milliseconds
+      std::this_thread::sleep_for(std::chrono::microseconds(spin < 100 ? 20 : 200));
     }
     if (n_running) {
       int k = 0;
-      for (int s = 0; s < S; ++s) k += slots[s].state == kRunning;
+      for (const Slot& sl : slots) k += sl.state == kRunning;
       *n_running = k;
     }
     if (n_queued) *n_queued = (int)pending.size();
@@ -640,9 +785,9 @@ struct ygpu_server {
 
 extern "C" {
 
-int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva, int32_t max_slots,
-                       int32_t max_side, int32_t nz_max, int64_t hbm_budget_bytes,
-                       ygpu_server** out, int64_t* device_bytes) {
+int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva, int32_t nlanes,
+                       const int32_t* lane_max_side, const int32_t* lane_slots, int32_t nz_max,
+                       int64_t hbm_budget_bytes, ygpu_server** out, int64_t* device_bytes) {
   if (!h || !par || !out) return YGPU_ERR_ARG;
   *out = nullptr;
   h->err.clear();
@@ -652,11 +797,11 @@ int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva,
   if (par->precision != YGPU_PREC_REF) {
     auto* t = new Server<double>();
     sb = t;
-    rc = t->create(h, par, diva ? 1 : 0, max_slots, max_side, nz_max, hbm_budget_bytes);
+    rc = t->create(h, par, diva ? 1 : 0, nlanes, lane_max_side, lane_slots, nz_max, hbm_budget_bytes);
   } else {
     auto* t = new Server<float>();
     sb = t;
-    rc = t->create(h, par, diva ? 1 : 0, max_slots, max_side, nz_max, hbm_budget_bytes);
+    rc = t->create(h, par, diva ? 1 : 0, nlanes, lane_max_side, lane_slots, nz_max, hbm_budget_bytes);
   }
   if (rc) { delete sb; return rc; }
   if (device_bytes) *device_bytes = sb->device_bytes();

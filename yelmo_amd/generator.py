@@ -8,7 +8,7 @@ are fixed here so every workload is reproducible:
 
 * thickness  H = 3000 m * sqrt(max(0, 1 - (r/550 km)^2)) plus white noise
   smoothed with a 40 km Gaussian and scaled to 50 m standard deviation, where
-  H > 0, clipped at 0;
+  H > 0, clipped at 0, without one-cell-wide ice tongues;
 * bed        z_bed = -200 m + 500 m * s, s white noise smoothed with a 150 km
   Gaussian, normalised to max|s| = 1;
 * friction   c_bed = 1e4 Pa * exp(sigma_ln * g), g white noise smoothed with
@@ -93,9 +93,26 @@ def _geometry(nx, ny, dx, seed, cb_seed=None, sigma_ln=1.0, H_noise_seed=None):
     return H, z_bed, c_raw
 
 
+def _prune_tongues(H, boundaries):
+    """Removes ice cells that have no ice on both sides along x or along y
+    (one-cell-wide tongues), repeatedly.  Both faces of such a cell are
+    calving fronts whose lateral-stress rows (solver_ssa_ac.f90:402-471)
+    are the same equation, so the SSA system would be singular."""
+    nx, ny = H.shape
+    im1, ip1, jm1, jp1 = inputs.neighbors(nx, ny, boundaries)
+    H = H.copy()
+    while True:
+        ice = H > 0
+        strip = ice & ((~ice[im1, :] & ~ice[ip1, :]) | (~ice[:, jm1] & ~ice[:, jp1]))
+        if not strip.any():
+            return H
+        H[strip] = 0.0
+
+
 def _derive(nx, ny, dx, H, z_bed, c_raw, A, nz, dtype, lateral_bc, boundaries) -> SsaProblem:
     """Every input that follows from thickness, bed and friction, with the
     zero initial state."""
+    H = _prune_tongues(H, boundaries)
     z_sl = np.zeros_like(H)
     f_ice = np.where(H > 0, 1.0, 0.0)
     H_grnd = inputs.calc_H_grnd(H, f_ice, z_bed, z_sl, RHO_ICE, RHO_SW)

@@ -16,6 +16,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from . import _lib
 from .generator import SsaProblem
 from .params import SsaParams
@@ -23,15 +25,24 @@ from .velocity import SolveInfo, _fill_struct, _info
 
 
 class Server:
+    """``lanes``: (max side, slots) per size class, sides increasing; each
+    lane steps on its own stream.  A single (max_side, max_slots) lane is
+    the default."""
+
     def __init__(self, par: SsaParams, *, diva: bool = False, max_slots: int = 64,
-                 max_side: int = 641, nz_max: int = 11, hbm_budget: int = 8 << 30, handle=None):
+                 max_side: int = 641, nz_max: int = 11, hbm_budget: int = 8 << 30,
+                 lanes=None, handle=None):
         self.h = handle or _lib.default_handle()
         self.diva = diva
         self.pc = par.to_c()
+        lanes = list(lanes) if lanes else [(max_side, max_slots)]
+        side = np.array([int(a) for a, _ in lanes], dtype=np.int32)
+        slots = np.array([int(b) for _, b in lanes], dtype=np.int32)
         srv = C.c_void_p()
         nbytes = C.c_int64()
         self.h.check(self.h.lib.ygpu_server_create(self.h.h, C.byref(self.pc), int(diva),
-                                                   max_slots, max_side, nz_max, int(hbm_budget),
+                                                   len(lanes), side.ctypes.data,
+                                                   slots.ctypes.data, nz_max, int(hbm_budget),
                                                    C.byref(srv), C.byref(nbytes)))
         self.s = srv
         self.device_bytes = nbytes.value
