@@ -277,9 +277,10 @@ def run_serve(args):
             if info.status != 0 or not np.isfinite(v).all():
                 failed.append((k, j, info.status))
             if prec == "fp64":
-                vel[(k, j)] = v
+                vel[(k, j)] = (v, info.ssa_iter_now)
             else:
-                errs.append(rel_l2(v, vel[(k, j)]))
+                v64, it64 = vel[(k, j)]
+                errs.append((rel_l2(v, v64), k, j, info.ssa_iter_now, it64))
 
         lat_c, slat_c, span_c, peak_c, steps_c, t90_c = _serve_run(srv, seqs, np.zeros(nreq),
                                                                    record)
@@ -309,8 +310,19 @@ def run_serve(args):
             "failed_solves": len(failed),
         }
         if prec == "mixed":
-            e = np.array(errs_closed + errs, dtype=np.float64)
+            allerr = errs_closed + errs
+            e = np.array([a[0] for a in allerr], dtype=np.float64)
             out[prec]["max_rel_l2_vs_fp64"] = float(np.max(e)) if np.isfinite(e).all() else float("nan")
+            same = [a[0] for a in allerr if a[3] == a[4]]
+            # solves whose Picard loops stopped at the same iteration in both
+            # precisions; a solve that stops one iteration apart differs by
+            # about the Picard tolerance
+            out[prec]["max_rel_l2_vs_fp64_same_picard"] = float(np.max(same)) if same else None
+            out[prec]["solves_picard_count_differs"] = len(allerr) - len(same)
+            w = max(allerr, key=lambda a: a[0])
+            out[prec]["worst_solve"] = {"request": w[1], "solve": w[2], "side": int(sides[w[1]]),
+                                        "picard_mixed": w[3], "picard_fp64": w[4],
+                                        "rel_l2": w[0]}
         print(f"# {prec}: capacity {capacity:.1f} solves/s, p99 {out[prec]['poisson']['request_latency_s']['p99']:.2f}s",
               file=sys.stderr, flush=True)
     mixed = out["mixed"]
@@ -333,6 +345,7 @@ def run_serve(args):
         # own distance from converged linear solves is 0.3-3e-6 on these
         # grids (the mixed path is within 3e-8 of converged)
         "max_rel_l2_vs_fp64": mixed["max_rel_l2_vs_fp64"],
+        "max_rel_l2_vs_fp64_same_picard": mixed["max_rel_l2_vs_fp64_same_picard"],
         "failed_solves": mixed["failed_solves"] + out["fp64"]["failed_solves"],
         "mixed": mixed, "fp64": out["fp64"],
         "speedup_capacity_vs_fp64": mixed["capacity_solves_per_s"] / out["fp64"]["capacity_solves_per_s"],
