@@ -143,6 +143,14 @@ def run_diva(args):
     rb.close()
 
     par = SsaParams(precision="mixed")
+    sched = {}
+    if not args.no_tune:
+        from yelmo_amd.tune import autotune_sizes
+        t_tune = time.perf_counter()
+        sched = autotune_sizes(base, par, diva=True, handle=h)
+        print(f"# autotuned {len(sched)} sizes in {time.perf_counter() - t_tune:.1f}s: " +
+              ", ".join(f"{k[0]}:{v.ilu_blocks}x{v.block}" for k, v in sorted(sched.items())),
+              file=sys.stderr, flush=True)
     probs = [p.copy() for p in base]
     rb = ResidentBatch(probs, par, diva=True, handle=h)
     for _ in range(args.warmup):
@@ -163,6 +171,7 @@ def run_diva(args):
     rb.close()
     errs = [rel_l2(vel(p), ref[k]) for k, p in enumerate(probs)]
     unknowns = sum(2 * p.nx * p.ny for p in base)
+    kblock = max([512] + [v.block for v in sched.values()])
     launches = max(st["krylov_launches"], 1)
     avg_ms = st["krylov_ms"] / launches
     achieved = (st["krylov_bytes"] / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -174,7 +183,9 @@ def run_diva(args):
         "data": "synthetic",
         "config": {"workload": f"DIVA, 48 domains, sides {sorted(set(sides))} spanning 1536 km, "
                                f"nz 21, A(zeta) = 1e-16 exp(-3(1-zeta)), mixed FP32/FP64",
-                   "unknowns_total": unknowns},
+                   "unknowns_total": unknowns,
+                   "schedules": {f"{k[0]}x{k[1]}": [v.ilu_blocks, v.block]
+                                 for k, v in sorted(sched.items())}},
         "max_rel_l2_vs_fp64": max(errs), "error_ok": max(errs) <= 1e-6,
         "fp64_batched_solves_per_s": 48 / t_fp64,
         "speedup_vs_fp64_batched": (48 * args.steps / elapsed) / (48 / t_fp64),
@@ -183,7 +194,7 @@ def run_diva(args):
         "picard_iters_mean": float(np.mean([i.ssa_iter_now for i in info])),
         "krylov_iters_per_solve": st["krylov_iters"] / (48 * args.steps),
         "unknown_iters_per_s": st["unknown_iters"] / elapsed,
-        "roofline": {"bound": "hbm", "kernel": "yg_krylov_kernel<1, 512>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": f"yg_krylov_kernel<1, {kblock}>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms},
         "cpu_baseline": None,
@@ -366,6 +377,9 @@ def main():
     ap.add_argument("--single-sample", type=int, default=4,
                     help="members solved one at a time on the FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the schedule autotuner (use the size rule: ceil(nx/64) line "
+                         "blocks)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time the mixed batched solve only (for counter runs): no FP64 "
                          "reference, no CPU baseline, error fields null")
@@ -427,6 +441,14 @@ def main():
         ref, plain, err64, t_fp64_batch, t_fp64_single = fp64_paths(args, base, par_fp64, h, vel)
 
     # ---- mixed precision, batched: the measured value ---------------------------
+    sched = {}
+    if not args.no_tune:
+        from yelmo_amd.tune import autotune_sizes
+        sched = autotune_sizes(base[:1], par_mixed, handle=h)
+        print("# autotuned schedule " + ", ".join(f"{k[0]}:{v.ilu_blocks} blocks x {v.block} "
+                                                   f"threads ({v.solve_ms:.2f} ms)"
+                                                   for k, v in sched.items()),
+              file=sys.stderr, flush=True)
     probs = [p.copy() for p in base]
     rb = ResidentBatch(probs, par_mixed, handle=h)
     for _ in range(args.warmup):
@@ -461,6 +483,7 @@ def main():
     launches = max(st["krylov_launches"], 1)
     avg_ms = st["krylov_ms"] / launches
     bytes_per_launch = st["krylov_bytes"] / launches
+    kblock = max([512] + [v.block for v in sched.values()])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
     # ---- CPU baseline: the oracle (FP32 fields + FP64 BiCGSTAB/ILU(0), the
@@ -497,6 +520,7 @@ def main():
                         f"{2 * args.side * args.side} unknowns each), mixed FP32/FP64, batched",
             "members": args.members, "side": args.side,
             "solver": par_mixed.ssa_lis_opt, "picard_tol": par_mixed.ssa_iter_conv,
+            "schedule": {f"{k[0]}x{k[1]}": [v.ilu_blocks, v.block] for k, v in sched.items()},
         },
         # per-member velocity error of the mixed path against the FP64 path
         # with converged linear solves (rtol 1e-12), and against the FP64
@@ -516,7 +540,7 @@ def main():
         "picard_iters_mean": float(np.mean([i.ssa_iter_now for i in info])),
         "krylov_iters_per_solve": st["krylov_iters"] / max(args.members * args.steps, 1),
         "unknown_iters_per_s": st["unknown_iters"] / elapsed,
-        "roofline": {"bound": "hbm", "kernel": "yg_krylov_kernel<1, 512>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": f"yg_krylov_kernel<1, {kblock}>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,
                      "bytes_per_launch": bytes_per_launch, "launches": launches},

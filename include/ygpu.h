@@ -228,6 +228,22 @@ int ygpu_poll(ygpu_server* s, int64_t* ticket);
 int ygpu_fetch(ygpu_server* s, int64_t ticket, ygpu_ssa_problem* out);
 int ygpu_server_destroy(ygpu_server* s);
 
+/* ---- schedule autotuner of the mixed-precision path (no counterpart in the
+ * reference).  The persistent Krylov kernel's schedule for a domain is its
+ * number of ILU(0) line blocks (one wave per block; at least ceil(nx/64))
+ * and the threads per member block (512 or 1024).  ygpu_autotune times the
+ * first Picard iteration's linear solve of prob (best of reps) under each
+ * candidate schedule and records the fastest converged one in the handle
+ * for prob's grid size (nx, ny); batches, sessions and servers created on
+ * the handle afterwards use it for every member of that size when the
+ * option string leaves the block count open (-p ilu, no -ygpu_ilu_blocks).
+ * ygpu_set_schedule records a schedule directly (ilu_blocks <= 0 removes
+ * it; block 0 keeps the size rule).                                         */
+int ygpu_autotune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob,
+                  int32_t diva, int32_t reps, int32_t* ilu_blocks, int32_t* block,
+                  double* solve_ms);
+int ygpu_set_schedule(ygpu_handle* h, int32_t nx, int32_t ny, int32_t ilu_blocks, int32_t block);
+
 /* device bytes a domain of this size needs (admission control)               */
 int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny, int32_t nz);
 

@@ -65,7 +65,7 @@ EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_default_param
            "ygpu_batch_download", "ygpu_batch_destroy", "ygpu_session_create", "ygpu_session_solve",
            "ygpu_session_destroy", "ygpu_stats_enable", "ygpu_stats_get",
            "ygpu_server_create", "ygpu_submit", "ygpu_server_step", "ygpu_poll", "ygpu_fetch",
-           "ygpu_server_destroy", "ygpu_footprint_bytes")
+           "ygpu_server_destroy", "ygpu_autotune", "ygpu_set_schedule", "ygpu_footprint_bytes")
 
 YGPU_SEQ_CONTINUE = 1
 YGPU_SEQ_LAST = 2
@@ -136,6 +136,11 @@ def load() -> C.CDLL:
     lib.ygpu_fetch.restype = C.c_int
     lib.ygpu_server_destroy.argtypes = [PP]
     lib.ygpu_server_destroy.restype = C.c_int
+    lib.ygpu_autotune.argtypes = [PP, C.POINTER(SsaParamsC), C.POINTER(SsaProblemC), I, I,
+                                  C.POINTER(I), C.POINTER(I), C.POINTER(D)]
+    lib.ygpu_autotune.restype = C.c_int
+    lib.ygpu_set_schedule.argtypes = [PP, I, I, I, I]
+    lib.ygpu_set_schedule.restype = C.c_int
     lib.ygpu_footprint_bytes.argtypes = [C.POINTER(SsaParamsC), I, I, I]
     lib.ygpu_footprint_bytes.restype = C.c_int64
     _lib = lib

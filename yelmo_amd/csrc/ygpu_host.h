@@ -10,7 +10,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ygpu.h"
@@ -24,6 +26,9 @@ struct ygpu_handle {
   std::string err;
   bool stats_on = false;
   ygpu_stats stats{};
+  // Krylov schedules of the mixed path per grid (nx, ny): ILU(0) line blocks
+  // and threads per member block, from ygpu_autotune or ygpu_set_schedule
+  std::map<std::pair<int, int>, std::pair<int, int>> sched;
 };
 
 #define YG_CHECK(h, call)                                                          \
@@ -160,8 +165,16 @@ struct Batch {
     M.dx = P.dx; M.dy = P.dy;
     M.vlen = (int)(((2LL * P.nx * P.ny + 3) / 4) * 4);
     M.coff = M.toff = M.woff = 0;
+    M.nb = 0;
+    tuned_block = 0;
+    if (kmode == YG_MODE_MIXED && opts.k.precon == 2 && opts.k.ilu_blocks < 0) {
+      auto it = h->sched.find({P.nx, P.ny});
+      if (it != h->sched.end()) { M.nb = it->second.first; tuned_block = it->second.second; }
+    }
     return YGPU_OK;
   }
+
+  int tuned_block = 0;   // threads the last described member's schedule asks for
 
   // threads per member block: 512 up to 512 grid lines, else 1024
   static int pick_block(const ygpu_ssa_params* par, int maxnx) {
@@ -179,12 +192,13 @@ struct Batch {
     coffs.assign(np + 1, 0);
     zoffs.assign(np + 1, 0);
     soff.clear();
-    int maxnx = 0;
+    int maxnx = 0, want_block = 0;
     for (int k = 0; k < np; ++k) {
       const ygpu_ssa_problem& P = probs[k];
       YgMember& M = mem[k];
       std::vector<int> st;
       if ((rc = describe(P, k, M, st))) return rc;
+      want_block = std::max(want_block, tuned_block);
       if (k > 0 && P.n_glen != probs[0].n_glen) {
         h->err = "domains of one call must share n_glen";
         return YGPU_ERR_ARG;
@@ -203,6 +217,7 @@ struct Batch {
     total_cells = coffs[np];
     total_z = zoffs[np];
     block = pick_block(par, maxnx);
+    if (par->block <= 0 && want_block > block) block = std::min(want_block, 1024);
     return allocate();
   }
 

@@ -1210,7 +1210,8 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     // line-to-wave mapping of the recurrences (exact or block ILU(0))
     const int g = threadIdx.x, lane = g & 63, w = g >> 6;
     const int wmax = (int)(blockDim.x >> 6);
-    int nb = a.opts.ilu_blocks;
+    // a member's own schedule (autotuned) before the options' rule
+    int nb = mb.nb != 0 ? mb.nb : a.opts.ilu_blocks;
     if (nb < 0) nb = MODE == YG_MODE_MIXED ? (mb.nx + 63) / 64 : 0;
     if (nb == 0) {
       c.coupled = mb.nx > 64;

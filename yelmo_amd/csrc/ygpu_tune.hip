@@ -1,0 +1,142 @@
+// ygpu_tune.hip -- the autotuner of the mixed-precision Krylov schedule.
+//
+// For one domain, the persistent Krylov kernel's schedule is the number of
+// ILU(0) line blocks (one wave sweeps each block; more, shorter blocks mean
+// shorter dependency chains per wave but a weaker block-Jacobi
+// preconditioner and so more Krylov iterations) and the threads per member
+// block (512 or 1024, which bounds the block count).  ygpu_autotune times
+// the first Picard iteration's linear solve of the caller's own domain under
+// each candidate schedule on the device and records the fastest in the
+// handle for that grid size; batches, sessions and servers created on the
+// handle afterwards give every member of that size its tuned schedule.
+#include "ygpu_host.h"
+
+using namespace yg;
+
+namespace {
+
+struct Candidate {
+  int nb, block;
+  double ms = 0.0;
+  int iters = 0, status = -1;
+};
+
+std::vector<Candidate> candidates(int nx) {
+  std::vector<Candidate> out;
+  const int nb0 = (nx + 63) / 64;
+  for (int block : {512, 1024}) {
+    if (block < nb0 * 64) continue;
+    const int nbmax = std::min(block / 64, nx);
+    std::vector<int> nbs;
+    for (int f : {2, 3, 4, 6, 8})   // nb0 * f / 2: 1x, 1.5x, 2x, 3x, 4x
+      nbs.push_back(std::min(nbmax, std::max(nb0, nb0 * f / 2)));
+    std::sort(nbs.begin(), nbs.end());
+    nbs.erase(std::unique(nbs.begin(), nbs.end()), nbs.end());
+    for (int nb : nbs) out.push_back(Candidate{nb, block});
+  }
+  return out;
+}
+
+int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob, int diva,
+         int reps, int32_t* nb_out, int32_t* block_out, double* ms_out) {
+  if (par->precision != YGPU_PREC_MIXED) {
+    h->err = "autotune: the schedule belongs to the mixed-precision path";
+    return YGPU_ERR_ARG;
+  }
+  ygpu_ssa_params p = *par;
+  p.block = 0;
+  Batch<double> B;
+  int rc = B.setup(h, &p, 1, prob, diva);
+  if (rc) return rc;
+  if (B.opts.k.precon != 2) {
+    h->err = "autotune: the schedule is that of the ILU(0) preconditioner (-p ilu)";
+    return YGPU_ERR_ARG;
+  }
+  if ((rc = B.upload_inputs(prob))) return rc;
+  if ((rc = B.upload_state(prob))) return rc;
+  hipStream_t st = h->stream;
+  const YgBatchGeom G = B.geom();
+  YG_CHECK(h, hipMemsetAsync(B.d_done, 0, sizeof(int), st));
+  // the fields of the first Picard iteration (velocity_ssa.f90:172-262)
+  yg_launch_init<double>(G, B.f, B.fp, nullptr, st);
+  if (B.diva) yg_launch_diva_pre<double>(G, B.f, B.dv, B.fp, B.d_done, st);
+  yg_launch_fields_pre<double>(G, B.f, B.fp, B.d_done, st);
+  YG_CHECK(h, hipGetLastError());
+  YgKrylovArgs ka{};
+  ka.members = B.d_mem; ka.soff = B.d_soff; ka.mask = B.d_mask;
+  ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32;
+  ka.b = B.d_b; ka.x = B.d_x; ka.W64 = B.d_W64; ka.W32 = B.d_W32;
+  ka.total_cells = B.total_cells;
+  ka.wrapx = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_X);
+  ka.wrapy = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_Y ||
+              B.fp.BC == YG_BC_MISMIP3D || B.fp.BC == YG_BC_TROUGH);
+  ka.skip = B.d_done; ka.out_iters = B.d_kit; ka.out_status = B.d_kst; ka.out_resid = B.d_kres;
+  ka.opts = B.opts.k;
+  hipEvent_t e0, e1;
+  YG_CHECK(h, hipEventCreate(&e0));
+  YG_CHECK(h, hipEventCreate(&e1));
+  std::vector<Candidate> cand = candidates(prob->nx);
+  bool warm = false;
+  for (Candidate& c : cand) {
+    B.mem[0].nb = c.nb;
+    YG_CHECK(h, hipMemcpyAsync(B.d_mem, B.mem.data(), sizeof(YgMember), hipMemcpyHostToDevice, st));
+    double best = 1e300;
+    for (int r = 0; r < reps + (warm ? 0 : 1); ++r) {
+      // the same system every time: assembly rewrites the matrix, b and x0
+      yg_launch_assemble<double>(G, B.f, B.fp, B.d_done, B.d_A64, B.d_A32, B.d_b, B.d_x,
+                                 B.d_mask, st);
+      YG_CHECK(h, hipEventRecord(e0, st));
+      yg_launch_krylov(ka, YG_MODE_MIXED, 1, c.block, st);
+      YG_CHECK(h, hipEventRecord(e1, st));
+      YG_CHECK(h, hipGetLastError());
+      YG_CHECK(h, hipEventSynchronize(e1));
+      float ms = 0.f;
+      YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
+      if (warm) best = std::min(best, (double)ms);
+      warm = true;
+    }
+    YG_CHECK(h, hipMemcpy(&c.iters, B.d_kit, sizeof(int), hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(&c.status, B.d_kst, sizeof(int), hipMemcpyDeviceToHost));
+    c.ms = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const Candidate* win = nullptr;
+  for (const Candidate& c : cand)
+    if (c.status == 0 && (!win || c.ms < win->ms)) win = &c;
+  if (!win) {
+    h->err = "autotune: no schedule converged on this domain";
+    return YGPU_ERR_UNSUPPORTED;
+  }
+  h->sched[{prob->nx, prob->ny}] = {win->nb, win->block};
+  if (nb_out) *nb_out = win->nb;
+  if (block_out) *block_out = win->block;
+  if (ms_out) *ms_out = win->ms;
+  return YGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ygpu_autotune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob,
+                  int32_t diva, int32_t reps, int32_t* ilu_blocks, int32_t* block,
+                  double* solve_ms) {
+  if (!h || !par || !prob) return YGPU_ERR_ARG;
+  h->err.clear();
+  if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  return tune(h, par, prob, diva ? 1 : 0, reps < 1 ? 1 : reps, ilu_blocks, block, solve_ms);
+}
+
+int ygpu_set_schedule(ygpu_handle* h, int32_t nx, int32_t ny, int32_t ilu_blocks, int32_t block) {
+  if (!h || nx < 3 || ny < 3) return YGPU_ERR_ARG;
+  if (ilu_blocks <= 0) {
+    h->sched.erase({nx, ny});
+    return YGPU_OK;
+  }
+  if (block != 0 && block != 512 && block != 1024) return YGPU_ERR_ARG;
+  h->sched[{nx, ny}] = {ilu_blocks, block};
+  return YGPU_OK;
+}
+
+}  // extern "C"
