@@ -143,6 +143,7 @@ def run_diva(args):
     rb.close()
 
     par = SsaParams(precision="mixed")
+    par = par.with_(ssa_lis_opt=par.ssa_lis_opt + f" -ygpu_factor {args.factor}")
     sched = {}
     if not args.no_tune:
         from yelmo_amd.tune import autotune_sizes
@@ -182,7 +183,9 @@ def run_diva(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"DIVA, 48 domains, sides {sorted(set(sides))} spanning 1536 km, "
-                               f"nz 21, A(zeta) = 1e-16 exp(-3(1-zeta)), mixed FP32/FP64",
+                               f"nz 21, A(zeta) = 1e-16 exp(-3(1-zeta)), mixed FP32/FP64, "
+                               f"ILU factors {args.factor}",
+                   "solver": par.ssa_lis_opt,
                    "unknowns_total": unknowns,
                    "schedules": {f"{k[0]}x{k[1]}": [v.ilu_blocks, v.block]
                                  for k, v in sorted(sched.items())}},
@@ -377,6 +380,9 @@ def main():
     ap.add_argument("--single-sample", type=int, default=4,
                     help="members solved one at a time on the FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--factor", choices=("fp32", "bf16"), default="bf16",
+                    help="storage of the mixed path's ILU(0) factors: bf16 (with the U rows "
+                         "scaled by their FP32 inverse diagonals) or fp32")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the schedule autotuner (use the size rule: ceil(nx/64) line "
                          "blocks)")
@@ -428,6 +434,8 @@ def main():
     print(f"# generated {args.members} x {args.side}^2 in {time.perf_counter() - t_gen:.1f}s",
           file=sys.stderr, flush=True)
     par_mixed = SsaParams(precision="mixed")
+    par_mixed = par_mixed.with_(ssa_lis_opt=par_mixed.ssa_lis_opt +
+                                f" -ygpu_factor {args.factor}")
     par_fp64 = SsaParams(precision="fp64")
     h = _lib.default_handle()
 
@@ -517,7 +525,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"ensemble {args.members} x {args.side}x{args.side} SSA (dx 8 km, "
-                        f"{2 * args.side * args.side} unknowns each), mixed FP32/FP64, batched",
+                        f"{2 * args.side * args.side} unknowns each), mixed FP32/FP64, batched, "
+                        f"ILU factors {args.factor}",
             "members": args.members, "side": args.side,
             "solver": par_mixed.ssa_lis_opt, "picard_tol": par_mixed.ssa_iter_conv,
             "schedule": {f"{k[0]}x{k[1]}": [v.ilu_blocks, v.block] for k, v in sched.items()},

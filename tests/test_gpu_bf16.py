@@ -1,0 +1,74 @@
+"""BF16 ILU(0) factors with per-row scaling (-ygpu_factor bf16) in the
+mixed-precision path, through the C-ABI.  The preconditioner changes (its
+factors carry 8-bit mantissas), so iteration counts differ from the FP32
+factors; the FP64 refinement still meets the same residual criterion, and
+the velocity stays within the mixed bar of the converged FP64 oracle:
+1e-6 rel L2."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests._helpers import rel_l2, vel_rel_l2
+from yelmo_amd import generator, velocity
+from yelmo_amd.params import SsaParams
+
+pytestmark = pytest.mark.gpu
+
+BF16 = "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false -ygpu_factor bf16"
+
+
+def _converged(par):
+    return par.with_(precision="fp64",
+                     ssa_lis_opt="-i bicgstab -p ilu -maxiter 3000 -tol 1.0e-12 -initx_zeros false")
+
+
+@pytest.mark.parametrize("n,dx", [(81, 16e3), (161, 8e3)])
+def test_bf16_factors_ssa_within_mixed_bar(n, dx):
+    print(f"case bf16 ssa n={n}", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed", ssa_lis_opt=BF16)
+    p = generator.make_problem(n, dx, seed=6, dtype=np.float32)
+    ref_st, ref = oracle.calc_velocity_ssa(p.astype(np.float64), _converged(par), dp=True)
+    g = p.copy()
+    info = velocity.calc_velocity_ssa(g, par)
+    f32 = p.copy()
+    info32 = velocity.calc_velocity_ssa(f32, par.with_(ssa_lis_opt=BF16.replace("bf16", "fp32")))
+    err = vel_rel_l2(g.state, ref_st)
+    print("picard", info.ssa_iter_now, ref["ssa_iter_now"], "krylov bf16", info.ssa_lin_iter,
+          "fp32", info32.ssa_lin_iter, "rel L2", err, flush=True)
+    assert info.status == 0 and info.ssa_lin_fail == 0
+    assert err <= 1e-6
+
+
+def test_bf16_factors_diva_within_mixed_bar():
+    print("case bf16 diva", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed", ssa_lis_opt=BF16)
+    p = generator.make_problem(96, 16e3, seed=7, dtype=np.float64, nz=21)
+    ref_st, _ = oracle.calc_velocity_diva(p, _converged(par), dp=True)
+    g = oracle.diva_state(p.copy())
+    info = velocity.calc_velocity_diva(g, par)
+    err = vel_rel_l2(g.state, ref_st, "ux_bar", "uy_bar")
+    print("picard", info.ssa_iter_now, "krylov", info.ssa_lin_iter, "rel L2", err, flush=True)
+    assert info.status == 0 and err <= 1e-6
+
+
+def test_bf16_factors_csr_solve_meets_tolerance():
+    """Inner boundary: the CSR of a mid-Picard state solved with BF16
+    factors meets the FP64 residual criterion and agrees with the oracle's
+    converged solve."""
+    print("case bf16 csr", flush=True)
+    par = SsaParams(visc_method=2, beta_method=4, ssa_iter_max=4)
+    p = generator.make_problem(81, 16e3, seed=0, dtype=np.float64)
+    st, _ = oracle.calc_velocity_ssa(p, par, dp=True)
+    q = p.copy()
+    q.state.update({k: st[k] for k in ("ux_b", "uy_b")})
+    ap, ai, av, b, x0, _ = oracle.assemble(q, np.asfortranarray(st["beta_acx"]),
+                                           np.asfortranarray(st["beta_acy"]),
+                                           np.asfortranarray(st["visc_eff_int"]))
+    exact = oracle.csr_solve(ap, ai, av, b, x0,
+                             "-i bicgstab -p ilu -maxiter 3000 -tol 1.0e-12 -initx_zeros false")
+    lgs = velocity.Lgs(ap, ai, av, b, x0.copy())
+    velocity.linear_solver_matrix_solve(lgs, BF16 + " -ygpu_precision mixed")
+    err = rel_l2(lgs.x_value, exact["x"])
+    print("iters", lgs.lin_iter, "status", lgs.status, "resid", lgs.L2_rel_norm, "err", err)
+    assert lgs.status == 0 and lgs.L2_rel_norm <= 1e-8
+    assert err <= 1e-6

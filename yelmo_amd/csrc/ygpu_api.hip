@@ -7,6 +7,7 @@ namespace yg {
 int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
   o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1; o.k.ir_target = 0.0;
+  o.k.factor16 = 0;
   if (!s) return 0;
   std::vector<std::string> tok;
   std::string cur;
@@ -36,6 +37,11 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k == "-ygpu_ir_tol") o.k.ir_target = atof(v.c_str());
     else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
     else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
+    else if (k == "-ygpu_factor") {
+      if (v == "bf16") o.k.factor16 = 1;
+      else if (v == "fp32") o.k.factor16 = 0;
+      else { err = "unknown factor storage: " + v; return YGPU_ERR_ARG; }
+    }
     else if (k == "-ygpu_ny") o.ny_hint = atoi(v.c_str());
     else if (k == "-ygpu_precision") {
       if (v == "fp64") o.prec = 0;
@@ -350,6 +356,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
     for (size_t q = 0; q < A.size(); ++q) A32[q] = (float)A[q];
     YG_CHECK(h, ar.alloc(&d_A32, A32.size())); YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * cells));
     YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * M.vlen));
+    if (o.k.factor16) YG_CHECK(h, ar.alloc(&ka.F16, (size_t)24 * cells));
     YG_CHECK(h, hipMemcpyAsync(d_A32, A32.data(), sizeof(float) * A32.size(), hipMemcpyHostToDevice, st));
   }
   ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;

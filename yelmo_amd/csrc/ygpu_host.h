@@ -92,6 +92,7 @@ struct Batch {
   uint32_t* d_mask = nullptr;
   double *d_A64 = nullptr, *d_b = nullptr, *d_x = nullptr, *d_F64 = nullptr, *d_W64 = nullptr;
   float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
+  uint16_t* d_F16 = nullptr;   // BF16 line-block factors (-ygpu_factor bf16)
   int *d_done = nullptr, *d_iter_now = nullptr, *d_lin_iter = nullptr, *d_lin_fail = nullptr,
       *d_lim_n = nullptr, *d_kit = nullptr, *d_kst = nullptr;
   double *d_resid = nullptr, *d_kres = nullptr;
@@ -236,6 +237,7 @@ struct Batch {
       YG_CHECK(h, ar.alloc(&d_A32, (size_t)YG_NA * C));
       YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * C));
       YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * total_v));
+      if (opts.k.factor16) YG_CHECK(h, ar.alloc(&d_F16, (size_t)24 * C));
     } else {
       YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * C));
     }
@@ -476,7 +478,8 @@ struct Batch {
     YgPicardState ps{d_iter_now, d_done, d_resid, d_lin_iter, d_lin_fail, d_lim_n};
     YgKrylovArgs ka{};
     ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;
-    ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.b = d_b; ka.x = d_x;
+    ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.F16 = d_F16;
+    ka.b = d_b; ka.x = d_x;
     ka.W64 = d_W64; ka.W32 = d_W32; ka.total_cells = total_cells;
     ka.wrapx = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_X);
     ka.wrapy = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_Y || fp.BC == YG_BC_MISMIP3D ||

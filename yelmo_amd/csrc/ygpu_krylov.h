@@ -31,6 +31,8 @@ struct YgKrylovOpts {
   double tol;
   double inner_tol; // mixed mode: relative tolerance of each FP32 inner solve
   double ir_target; // mixed mode: residual the refinement stops at (0: tol / 10)
+  int factor16;     // mixed mode: 1 = ILU(0) factors stored in BF16 (U rows scaled
+                    // by their FP32 inverse diagonals)
 };
 
 struct YgKrylovArgs {
@@ -41,6 +43,7 @@ struct YgKrylovArgs {
   const float* A32;
   double* F64;              // ILU factors (skewed wavefront order) / jacobi 1/diag
   float* F32;
+  uint16_t* F16;            // mixed mode, BF16 factors: 48 bytes per cell (or null)
   const double* b;          // 2 per cell, the reference's row order 2n, 2n+1
   double* x;                // 2 per cell, x0 in, solution out
   double* W64;              // YG_NW64 * 2 per cell

@@ -651,6 +651,39 @@ YG_DEV void load_vals(__amdgpu_buffer_rsrc_t r, unsigned off, double (&v)[N]) {
   }
 }
 
+// BF16 storage of the line-block factors (mixed mode, option factor16).
+// Per cell in the same block-skewed order as the FP32 factors:
+//   L part: 8 BF16 values (F0..F7), one 16-byte chunk, at FB + 16 q bytes;
+//   U part: 8 BF16 values of the off-diagonal U entries already multiplied by
+//     their row's inverse diagonal (ux row: slots 5,3,7,1,6; uy row: 6,1,3),
+//     then the two inverse diagonals in FP32 and 8 pad bytes: 32 bytes at
+//     FB + 16 ncell + 32 q bytes.  The inverse diagonal is the row scale, so
+//     a U row is x = inv_d y - sum (inv_d U) x_j.
+YG_DEV uint16_t to_bf16(float v) {
+  const uint32_t b = __builtin_bit_cast(uint32_t, v);
+  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);   // nearest, ties to even
+}
+YG_DEV float from_bf16(uint32_t h16) { return __builtin_bit_cast(float, h16 << 16); }
+
+YG_DEV void load_bf16x8(__amdgpu_buffer_rsrc_t r, unsigned off, float (&v)[8]) {
+  const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v[2 * e] = from_bf16(w[e] & 0xFFFFu);
+    v[2 * e + 1] = from_bf16(w[e] >> 16);
+  }
+}
+YG_DEV void load_bf16_urow(__amdgpu_buffer_rsrc_t r, unsigned off, float (&v)[10]) {
+  float g[8];
+  load_bf16x8(r, off, g);
+  const float2 d = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0));
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = g[e];
+  v[8] = d.x;
+  v[9] = d.y;
+}
+
 // Geometry of one wave's line block in the streamed sweeps, all scalars.
 // Per step t the lanes with cells are glo(t) <= lane <= min(L-1, t/2); a
 // lane's slot in the block-skewed order is blk_soff(t) - glo(t) + lane.
@@ -813,11 +846,148 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   }
 }
 
+// The streamed sweeps over BF16 factors (FP32 arithmetic): the same
+// recurrences as sweep_L_str / sweep_U_str<float>, with a factor cell of 16
+// (L) or 24 (U) bytes instead of 32 and 48.
+YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
+                               const float* __restrict__ rhs, float* __restrict__ ys) {
+  PhaseTimer ptm(c, 1);
+  constexpr int D = 12;
+  constexpr unsigned SF = 16, SV = 8;
+  using T = float;
+  using V2 = typename Vec2<T>::type;
+  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
+  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  const int nc = uni(c.ncell);
+  if (w >= nwl) return;
+  const int i0 = w * lpw, L = min(lpw, nx - i0);
+  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB, (size_t)8 * nc);
+  const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
+  const int lane_q = i0 * ny + lane;
+  const int lane_n = (i0 + lane) * ny - 2 * lane;
+  T f[D][8];
+  V2 r[D];
+  StepQ qc = step_q(G, 0), qn = step_q(G, 0);
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    load_bf16x8(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+    r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
+    step_next(G, qn);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+  for (int t0 = 0; t0 < G.Tb; t0 += D) {
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      const bool h = lane >= qc.lo && lane <= qc.hi;
+      const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
+      T yx = r[s].x - f[s][0] * n2x - f[s][1] * o1x - f[s][2] * o1y;
+      T yy = r[s].y - f[s][3] * n2x - f[s][4] * n2y - f[s][5] * n1x - f[s][6] * o1y -
+             f[s][7] * yx;
+      yx = h ? yx : (T)0;
+      yy = h ? yy : (T)0;
+      V2 o;
+      o.x = yx;
+      o.y = yy;
+      Pair16<T>::store(o, rY, cell_off(qc, lane, lane_q, SV));
+      o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
+      step_next(G, qc);
+      load_bf16x8(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+      r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
+      step_next(G, qn);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
+                               const float* __restrict__ ys, float* __restrict__ z) {
+  PhaseTimer ptm(c, 2);
+  constexpr int D = 12;
+  constexpr unsigned SF = 32, SV = 8;
+  using T = float;
+  using V2 = typename Vec2<T>::type;
+  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
+  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  const int nc = uni(c.ncell);
+  if (w >= nwl) return;
+  const int i0 = w * lpw, L = min(lpw, nx - i0);
+  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + (size_t)8 * nc, (size_t)16 * nc);
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
+  const int lane_q = i0 * ny + lane;
+  const int lane_n = (i0 + lane) * ny - 2 * lane;
+  T f[D][10];
+  V2 yv[D];
+  StepQ qc = step_q(G, G.Tb - 1), qn = step_q(G, G.Tb - 1);
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    load_bf16_urow(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+    yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
+    step_prev(G, qn);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+  for (int u0 = 0; u0 < G.Tb; u0 += D) {
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      const bool h = lane >= qc.lo && lane <= qc.hi;
+      const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
+      // f: 0..4 scaled ux-row U entries, 5..7 scaled uy-row U entries,
+      // 8 ux-row inverse diagonal, 9 uy-row inverse diagonal
+      T zy = f[s][9] * yv[s].y - f[s][5] * o1x - f[s][6] * o1y - f[s][7] * n2y;
+      T zx = f[s][8] * yv[s].x - f[s][0] * zy - f[s][1] * o1x - f[s][2] * n1y -
+             f[s][3] * n2x - f[s][4] * n2y;
+      zx = h ? zx : (T)0;
+      zy = h ? zy : (T)0;
+      V2 o;
+      o.x = zx;
+      o.y = zy;
+      Pair16<T>::store(o, rZ, nat_off(qc, lane, lane_n, SV));
+      o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
+      step_prev(G, qc);
+      load_bf16_urow(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+      yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
+      step_prev(G, qn);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// FP32 line-block factors -> BF16 storage (layout above)
+YG_DEV void factors_to_bf16(const Ctx& c, const float* __restrict__ F, uint16_t* __restrict__ FB) {
+  const int nc = c.ncell;
+  const float* FU = F + (size_t)8 * nc;
+  uint16_t* BU = FB + (size_t)8 * nc;
+  for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+    uint32_t lw[4], uw[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      lw[e] = (uint32_t)to_bf16(F[8 * q + 2 * e]) | ((uint32_t)to_bf16(F[8 * q + 2 * e + 1]) << 16);
+    const float* u = FU + 12 * q;
+    const float g[8] = {u[0] * u[5], u[1] * u[5], u[2] * u[5], u[3] * u[5], u[4] * u[5],
+                        u[6] * u[9], u[7] * u[9], u[8] * u[9]};
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      uw[e] = (uint32_t)to_bf16(g[2 * e]) | ((uint32_t)to_bf16(g[2 * e + 1]) << 16);
+    uint4* dl = reinterpret_cast<uint4*>(FB + 8 * q);
+    *dl = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    uint4* du = reinterpret_cast<uint4*>(BU + 16 * q);
+    du[0] = make_uint4(uw[0], uw[1], uw[2], uw[3]);
+    du[1] = make_uint4(__builtin_bit_cast(uint32_t, u[5]), __builtin_bit_cast(uint32_t, u[9]), 0u, 0u);
+  }
+  __syncthreads();
+}
+
 // ---- preconditioner --------------------------------------------------------
 template <typename T>
 struct Precon {
   int kind;        // 0 none, 1 jacobi, 2 ilu
   const T* F;      // ILU factors, or 1/diag pairs for jacobi
+  const uint16_t* FB = nullptr;   // BF16 copy of line-block factors (mixed mode)
 };
 
 // out = M^-1 in; tmp is scratch of the same size
@@ -826,6 +996,13 @@ YG_DEV void precon_apply(const Ctx& c, const Precon<T>& P, const T* in, T* tmp, 
   if (P.kind == 2 && c.coupled) {
     sweep_L<T>(c, P.F, in, tmp);
     sweep_U<T>(c, P.F, tmp, out);
+  } else if (P.kind == 2 && P.FB) {
+    if constexpr (sizeof(T) == 4) {
+      sweep_L_str16(c, P.FB, in, tmp);
+      __syncthreads();
+      sweep_U_str16(c, P.FB, tmp, out);
+      __syncthreads();
+    }
   } else if (P.kind == 2) {
     sweep_L_str<T>(c, P.F, in, tmp);
     __syncthreads();
@@ -1281,6 +1458,11 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     }
     Precon<float> P;
     build_precon<float>(c, A32, F, P, o.precon, &s_bad);
+    if (o.factor16 && P.kind == 2 && !c.coupled && a.F16) {
+      uint16_t* FB = a.F16 + (size_t)24 * mb.coff;   // 48 bytes per cell
+      factors_to_bf16(c, F, FB);
+      P.FB = FB;
+    }
     YgKrylovOpts oi = o;
     oi.tol = o.inner_tol;
     oi.conv_cond = 0;

@@ -246,6 +246,7 @@ struct Server : YgServerBase {
     if (b.kmode == YG_MODE_MIXED) c += YG_NA * 4.0 + YG_NF * 4.0 + YG_NW32 * 2 * 4.0;
     else c += YG_NF * 8.0;
     if (b.fp.energy) c += 2 * 4.0;
+    if (b.kmode == YG_MODE_MIXED && b.opts.k.factor16) c += 48.0;
     c += sizeof(R) * (34.0 + 3.0 * nz + (b.diva ? 8.0 + 4.0 * nz : 0.0));
     return c;
   }
@@ -591,7 +592,7 @@ struct Server : YgServerBase {
     const YgPicardState ps = pstate(L);
     YgKrylovArgs ka{};
     ka.members = B.d_mem + L.s0; ka.soff = B.d_soff; ka.mask = B.d_mask;
-    ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32;
+    ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32; ka.F16 = B.d_F16;
     ka.b = B.d_b; ka.x = B.d_x; ka.W64 = B.d_W64; ka.W32 = B.d_W32;
     ka.total_cells = L.hi_cell;
     ka.wrapx = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_X);

@@ -64,7 +64,7 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
   YG_CHECK(h, hipGetLastError());
   YgKrylovArgs ka{};
   ka.members = B.d_mem; ka.soff = B.d_soff; ka.mask = B.d_mask;
-  ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32;
+  ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32; ka.F16 = B.d_F16;
   ka.b = B.d_b; ka.x = B.d_x; ka.W64 = B.d_W64; ka.W32 = B.d_W32;
   ka.total_cells = B.total_cells;
   ka.wrapx = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_X);
