@@ -7,7 +7,8 @@ rate factor drawn log-uniformly in [1e-17, 1e-16]), each member solved from
 rest through the full Picard loop (BiCGSTAB + ILU(0) at rtol 1e-8, Picard
 tolerance 1e-4, at most 50 iterations) in one batched call of the
 mixed-precision path (FP32 stencil SpMV and ILU(0) inside FP64 iterative
-refinement).  Inputs are resident in HBM before the timed region; each step
+refinement; the ILU(0) factors stored in BF16 with the U rows scaled by their
+FP32 inverse diagonals, --factor fp32 keeps them in FP32).  Inputs are resident in HBM before the timed region; each step
 restores the initial state on the device and solves all members again.
 
 The line also reports the plain FP64 GPU path on the same ensemble (batched,

@@ -61,7 +61,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
 // stencil SpMVs (18 slot values + 8 B of mask/coordinates per cell), two
 // ILU(0) applications (18 factor values per cell), 27 vector passes of two
 // values per cell (the fused updates and dot products).
-double krylov_bytes(double C, int iters, int pA, int pF, int pv, bool ilu) {
+double krylov_bytes(double C, int iters, int pA, double pF, int pv, bool ilu) {
   const double setup = 18.0 * C * pA + 8.0 * C + 4.0 * 2.0 * C * pv;
   const double fact = ilu ? (18.0 * C * pA + 4.0 * C + 18.0 * C * pF) : 0.0;
   const double per_it = 36.0 * C * pA + 16.0 * C + (ilu ? 36.0 * C * pF : 4.0 * C * pF) +

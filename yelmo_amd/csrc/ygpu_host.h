@@ -51,7 +51,8 @@ struct Opts {
 
 int parse_opts(const char* s, Opts& o, std::string& err);
 // Algorithmic HBM bytes of one member's Krylov launch (defined with parse_opts)
-double krylov_bytes(double C, int iters, int pA, int pF, int pv, bool ilu);
+// pF: bytes per factor value (BF16 line-block factors: 40 bytes per cell / 18)
+double krylov_bytes(double C, int iters, int pA, double pF, int pv, bool ilu);
 // skewed step table of an nx x ny member: soff[t] = cells before step t
 void step_table(int nx, int ny, std::vector<int>& soff, int& T);
 // boundaries enum of the C-ABI -> neighbour-index code of the kernels
@@ -526,7 +527,8 @@ struct Batch {
         for (int k : active) {
           const double C = (double)mem[k].nx * mem[k].ny;
           const int it = kit[k];
-          h->stats.krylov_bytes += mixed ? krylov_bytes(C, it, 4, 4, 4, opts.k.precon == 2) +
+          const double pF = opts.k.factor16 ? 40.0 / 18.0 : 4.0;
+          h->stats.krylov_bytes += mixed ? krylov_bytes(C, it, 4, pF, 4, opts.k.precon == 2) +
                                                4.0 * (18.0 * C * 8 + 8.0 * C + 6.0 * 2.0 * C * 8)
                                          : krylov_bytes(C, it, 8, 8, 8, opts.k.precon == 2);
           h->stats.krylov_iters += it;
