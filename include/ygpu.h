@@ -122,6 +122,17 @@ void ygpu_default_params(ygpu_ssa_params* par);
  * (solver_linear.F90:134-143), non-periodic.  Option grammar: LIS's -i, -p,
  * -maxiter, -tol, -initx_zeros, -conv_cond; extensions -ygpu_ny NY (grid hint),
  * -ygpu_precision fp64|mixed.                                                */
+/* Extensions of the option string read by the outer entry points too:
+ *   -ygpu_factor bf16|fp32   storage of the mixed path's ILU(0) factors
+ *                            (BF16 with U rows scaled by their FP32 inverse
+ *                            diagonals; default fp32)
+ *   -ygpu_ilu_blocks N       ILU(0) over N line blocks (mixed path default:
+ *                            one per 64 lines, or the autotuned schedule)
+ *   -ygpu_ir_tol T           residual at which refinement stops (tol/10)
+ *   -ir_inner_tol T          relative tolerance of each FP32 inner solve
+ *   -ygpu_graph N            Picard iterations per captured hipGraph (4; 0 =
+ *                            launch each iteration and read the flags after
+ *                            it; runs with kernel statistics on are eager)   */
 int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
                    const int32_t* a_idx1, const double* a_val, const double* b, double* x_inout,
                    const char* opts, int32_t* iters, int32_t* status, double* rel_resid,

@@ -7,7 +7,7 @@ namespace yg {
 int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
   o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1; o.k.ir_target = 0.0;
-  o.k.factor16 = 0;
+  o.k.factor16 = 0; o.k.graph_chunk = 4;
   if (!s) return 0;
   std::vector<std::string> tok;
   std::string cur;
@@ -37,6 +37,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k == "-ygpu_ir_tol") o.k.ir_target = atof(v.c_str());
     else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
     else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
+    else if (k == "-ygpu_graph") o.k.graph_chunk = std::max(0, atoi(v.c_str()));
     else if (k == "-ygpu_factor") {
       if (v == "bf16") o.k.factor16 = 1;
       else if (v == "fp32") o.k.factor16 = 0;
@@ -324,7 +325,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   const int kmode = o.prec == 2 ? YG_MODE_MIXED : YG_MODE_FP64;
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   Arena ar;
-  YgMember M;
+  YgMember M{};
   M.nx = nx; M.ny = ny; M.nz = 1; M.T = T; M.coff = 0; M.toff = 0; M.dx = M.dy = 1.0;
   M.woff = 0;
   M.vlen = (int)(((2LL * cells + 3) / 4) * 4);

@@ -461,6 +461,65 @@ struct Batch {
     return YGPU_OK;
   }
 
+  // ---- one Picard iteration, and a graph of several ---------------------
+  // Every kernel skips members whose done flag is set and each member counts
+  // its own iterations, so iterations can run back to back without looking
+  // at the flags in between: a graph of `chunk` iterations replayed until all
+  // members are done gives the same result as the loop that checks after
+  // each iteration.  ev0/ev1 (may be null) bracket the Krylov launch.
+  void enqueue_iteration(const YgBatchGeom& G, const YgPicardState& ps, const YgKrylovArgs& ka,
+                         hipEvent_t ev0, hipEvent_t ev1, hipStream_t st) {
+    if (diva) yg_launch_diva_pre<R>(G, f, dv, fp, d_done, st);
+    yg_launch_fields_pre<R>(G, f, fp, d_done, st);
+    yg_launch_assemble<R>(G, f, fp, d_done, d_A64, kmode == YG_MODE_MIXED ? d_A32 : nullptr, d_b,
+                          d_x, d_mask, st);
+    if (ev0) (void)hipEventRecord(ev0, st);
+    yg_launch_krylov(ka, kmode, nmem, block, st);
+    if (ev1) (void)hipEventRecord(ev1, st);
+    yg_launch_epilogue<R>(G, f, fp, d_x, d_kit, d_kst, iter_max, ps, st);
+  }
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gchunk = 0;
+  bool graph_failed = false;
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+  ~Batch() { drop_graph(); }
+
+  // Captures `chunk` iterations on the handle's stream, once per batch and
+  // chunk size; false if the runtime refused (the eager loop is used then).
+  // Kernel statistics time each Krylov launch with HIP events, which a
+  // captured graph does not support, so a run with statistics stays eager.
+  bool ensure_graph(const YgBatchGeom& G, const YgPicardState& ps, const YgKrylovArgs& ka,
+                    int chunk) {
+    if (graph_failed) return false;
+    if (gexec && gchunk == chunk) return true;
+    drop_graph();
+    yg_krylov_prepare();
+    hipStream_t st = h->stream;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      graph_failed = true;
+      return false;
+    }
+    for (int k = 0; k < chunk; ++k) enqueue_iteration(G, ps, ka, nullptr, nullptr, st);
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    const hipError_t el = hipGetLastError();
+    if (ec != hipSuccess || el != hipSuccess || !graph ||
+        hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      graph_failed = true;
+      drop_graph();
+      return false;
+    }
+    gchunk = chunk;
+    return true;
+  }
+
   // the Picard loop (velocity_ssa.f90:172-334 / velocity_diva.f90:195-404)
   int run(double* seconds) {
     hipStream_t st = h->stream;
@@ -500,19 +559,24 @@ struct Batch {
       YG_CHECK(h, hipEventCreate(&k0));
       YG_CHECK(h, hipEventCreate(&k1));
     }
-    for (int iter = 1; iter <= iter_max; ++iter) {
-      if (diva) yg_launch_diva_pre<R>(G, f, dv, fp, d_done, st);
-      yg_launch_fields_pre<R>(G, f, fp, d_done, st);
-      yg_launch_assemble<R>(G, f, fp, d_done, d_A64, kmode == YG_MODE_MIXED ? d_A32 : nullptr,
-                            d_b, d_x, d_mask, st);
-      std::vector<int> active;
-      if (h->stats_on) {
-        for (int k = 0; k < nmem; ++k) if (!done[k]) active.push_back(k);
-        YG_CHECK(h, hipEventRecord(k0, st));
+    const int chunk = (phase || h->stats_on) ? 0 : opts.k.graph_chunk;
+    const bool graphed = chunk > 0 && ensure_graph(G, ps, ka, chunk);
+    if (graphed) {
+      // replays of the captured chunk, one flag read per replay
+      for (int base = 0; base < iter_max; base += chunk) {
+        YG_CHECK(h, hipGraphLaunch(gexec, st));
+        YG_CHECK(h, hipMemcpyAsync(done.data(), d_done, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
+        YG_CHECK(h, hipStreamSynchronize(st));
+        bool all = true;
+        for (int k = 0; k < nmem; ++k) all &= done[k] != 0;
+        if (all) break;
       }
-      yg_launch_krylov(ka, kmode, nmem, block, st);
-      if (h->stats_on) YG_CHECK(h, hipEventRecord(k1, st));
-      yg_launch_epilogue<R>(G, f, fp, d_x, d_kit, d_kst, iter_max, ps, st);
+    }
+    for (int iter = 1; !graphed && iter <= iter_max; ++iter) {
+      std::vector<int> active;
+      if (h->stats_on)
+        for (int k = 0; k < nmem; ++k) if (!done[k]) active.push_back(k);
+      enqueue_iteration(G, ps, ka, k0, k1, st);
       YG_CHECK(h, hipGetLastError());
       YG_CHECK(h, hipMemcpyAsync(done.data(), d_done, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
       if (h->stats_on)

@@ -33,6 +33,7 @@ struct YgKrylovOpts {
   double ir_target; // mixed mode: residual the refinement stops at (0: tol / 10)
   int factor16;     // mixed mode: 1 = ILU(0) factors stored in BF16 (U rows scaled
                     // by their FP32 inverse diagonals)
+  int graph_chunk;  // host side: Picard iterations per captured graph (0: no graph)
 };
 
 struct YgKrylovArgs {
@@ -59,3 +60,5 @@ struct YgKrylovArgs {
 };
 
 void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, hipStream_t st);
+// sets the kernels' launch attributes; call before capturing a launch in a graph
+void yg_krylov_prepare();

@@ -6,3 +6,5 @@ void yg_launch_krylov_1024(const YgKrylovArgs& a, int mode, int nmembers, int bl
                            hipStream_t st) {
   yg_launch_krylov_nt<1024>(a, mode, nmembers, block, st);
 }
+
+void yg_krylov_attrs_1024() { yg_krylov_attrs_nt<1024>(); }

@@ -1511,26 +1511,28 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   }
 }
 
-// Launch of this translation unit's block size (set attributes once).
+// Dynamic-LDS limits of both modes, once per process (before any launch, so
+// that no attribute call happens while a stream is being captured).
+template <int NT>
+void yg_krylov_attrs_nt() {
+  static bool done = false;
+  if (done) return;
+  (void)hipFuncSetAttribute((const void*)yg_krylov_kernel<YG_MODE_FP64, NT>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
+  (void)hipFuncSetAttribute((const void*)yg_krylov_kernel<YG_MODE_MIXED, NT>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
+  done = true;
+}
+
+// Launch of this translation unit's block size.
 template <int NT>
 void yg_launch_krylov_nt(const YgKrylovArgs& a, int mode, int nmembers, int block,
                          hipStream_t st) {
-  static bool attr_set[2] = {false, false};
-  if (mode == YG_MODE_FP64) {
-    if (!attr_set[0]) {
-      (void)hipFuncSetAttribute((const void*)yg_krylov_kernel<YG_MODE_FP64, NT>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
-      attr_set[0] = true;
-    }
+  yg_krylov_attrs_nt<NT>();
+  if (mode == YG_MODE_FP64)
     hipLaunchKernelGGL((yg_krylov_kernel<YG_MODE_FP64, NT>), dim3(nmembers), dim3(block),
                        YG_DYN_LDS, st, a);
-  } else {
-    if (!attr_set[1]) {
-      (void)hipFuncSetAttribute((const void*)yg_krylov_kernel<YG_MODE_MIXED, NT>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
-      attr_set[1] = true;
-    }
+  else
     hipLaunchKernelGGL((yg_krylov_kernel<YG_MODE_MIXED, NT>), dim3(nmembers), dim3(block),
                        YG_DYN_LDS, st, a);
-  }
 }

@@ -352,7 +352,7 @@ struct Server : YgServerBase {
       h->err = "submit: every request of a server shares n_glen";
       return YGPU_ERR_ARG;
     }
-    YgMember M;
+    YgMember M{};
     std::vector<int> st;
     int rc = B.describe(*p, 0, M, st);
     if (rc) return rc;
