@@ -485,6 +485,16 @@ def main():
         rb.run()
     _sync()
     graphed_sps = args.members * ng / (time.perf_counter() - tg)
+    # the stencil SpMV over all members in one full-device launch, on the
+    # systems of the last Picard iteration: the SpMV roofline of the metric
+    spmv = {}
+    for name, fp32 in (("fp64", False), ("fp32", True)):
+        _, sms, sbytes = rb.spmv(fp32=fp32, reps=50)
+        gbs = sbytes / (sms * 1e-3) / 1e9 if sms > 0 else 0.0
+        spmv[name] = {"kernel": f"k_spmv_records<{'float' if fp32 else 'double'}>",
+                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": gbs / HBM_PEAK_GBS, "avg_launch_ms": sms,
+                      "bytes_per_launch": sbytes}
     errs, errs_plain, maxabs = [float("nan")], [float("nan")], [float("nan")]
     if ref is not None:
         errs, errs_plain, maxabs = [], [], []
@@ -565,6 +575,10 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,
                      "bytes_per_launch": bytes_per_launch, "launches": launches},
+        # y = A x of all 64 members in one launch (HBM-bound; the FP64 records
+        # and vectors, 292 MB, exceed the 256 MB Infinity Cache, the FP32 ones
+        # do not)
+        "spmv": spmv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(_finite(line)), flush=True)

@@ -138,6 +138,12 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
                    const char* opts, int32_t* iters, int32_t* status, double* rel_resid,
                    double* seconds);
 
+/* y = A x for a CSR matrix of the same pattern (LIS's lis_matvec at the
+ * boundary of solver_linear.F90; 1-based arrays as in lgs), on the device.  */
+int ygpu_csr_spmv(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
+                  const int32_t* a_idx1, const double* a_val, const double* x, double* y,
+                  const char* opts);
+
 /* ---- outer boundary: calc_velocity_ssa (velocity_ssa.f90:60-63), argument
  * order as in the reference, wp = sp arrays.                                  */
 int ygpu_velocity_ssa(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int32_t ny,
@@ -178,6 +184,15 @@ int ygpu_batch_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
 int ygpu_batch_run(ygpu_batch* b, int32_t reset, double* seconds);
 int ygpu_batch_download(ygpu_batch* b, ygpu_ssa_problem* probs);
 int ygpu_batch_destroy(ygpu_batch* b);
+/* y = A x over every domain of a resident batch with the stencil records of
+ * the systems assembled in its last Picard iteration and x its last
+ * solution, in one full-device launch (fp32 != 0: the mixed mode's FP32
+ * records and vectors); reps timed launches after one untimed.  y (may be
+ * NULL) receives the 2-per-cell result of every domain in batch order;
+ * ms_per_launch the average launch time; bytes_per_launch the algorithmic
+ * bytes (18 matrix values, one velocity pair read and written per cell).  */
+int ygpu_batch_spmv(ygpu_batch* b, int32_t fp32, int32_t reps, double* y, double* ms_per_launch,
+                    double* bytes_per_launch);
 
 /* ---- resident sessions (warm starts for served solve sequences).
  * A session keeps one domain's fields, velocity, viscosity and workspace on

@@ -62,7 +62,7 @@ class SsaProblemC(C.Structure):
 EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_default_params",
            "ygpu_csr_solve", "ygpu_velocity_ssa", "ygpu_velocity_ssa_batch",
            "ygpu_velocity_diva_batch", "ygpu_batch_create", "ygpu_batch_run",
-           "ygpu_batch_download", "ygpu_batch_destroy", "ygpu_session_create", "ygpu_session_solve",
+           "ygpu_batch_download", "ygpu_batch_destroy", "ygpu_batch_spmv", "ygpu_csr_spmv", "ygpu_session_create", "ygpu_session_solve",
            "ygpu_session_destroy", "ygpu_stats_enable", "ygpu_stats_get",
            "ygpu_server_create", "ygpu_submit", "ygpu_server_step", "ygpu_poll", "ygpu_fetch",
            "ygpu_server_destroy", "ygpu_autotune", "ygpu_set_schedule", "ygpu_footprint_bytes")
@@ -117,6 +117,10 @@ def load() -> C.CDLL:
     lib.ygpu_batch_run.restype = C.c_int
     lib.ygpu_batch_download.argtypes = [PP, C.POINTER(SsaProblemC)]
     lib.ygpu_batch_download.restype = C.c_int
+    lib.ygpu_batch_spmv.argtypes = [PP, I, I, PP, C.POINTER(D), C.POINTER(D)]
+    lib.ygpu_batch_spmv.restype = C.c_int
+    lib.ygpu_csr_spmv.argtypes = [PP, I, I, PP, PP, PP, PP, PP, C.c_char_p]
+    lib.ygpu_csr_spmv.restype = C.c_int
     lib.ygpu_batch_destroy.argtypes = [PP]
     lib.ygpu_batch_destroy.restype = C.c_int
     lib.ygpu_stats_enable.argtypes = [PP, I]

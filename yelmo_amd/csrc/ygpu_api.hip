@@ -134,6 +134,53 @@ int run_batch(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_probl
   return YGPU_OK;
 }
 
+// CSR (1-based, linear_solver_class) -> stencil records in the row order.
+// The grid's ny comes from the widest coupling (uy(i+1,j) is 2 ny + 1
+// columns right of ux(i,j)) unless ny_hint is given.  need_diag: every row
+// must hold its diagonal (a preconditioner divides by it).
+int csr_records(ygpu_handle* h, int n, const int32_t* a_ptr1, const int32_t* a_idx1,
+                const double* a_val, int ny_hint, bool need_diag, int& nx, int& ny,
+                std::vector<double>& A, std::vector<uint32_t>& mask) {
+  const int cells = n / 2;
+  ny = ny_hint;
+  if (ny <= 0) {
+    long long maxd = 0;
+    for (int r = 0; r < n; ++r)
+      for (int k = a_ptr1[r] - 1; k < a_ptr1[r + 1] - 1; ++k) maxd = std::max<long long>(maxd, (long long)(a_idx1[k] - 1) - r);
+    ny = maxd >= 2 ? (int)(maxd / 2) : cells;
+  }
+  if (ny <= 0 || cells % ny) { h->err = "cannot infer the grid of this CSR pattern (give -ygpu_ny)"; return YGPU_ERR_UNSUPPORTED; }
+  nx = cells / ny;
+  if (nx > YG_MAX_LINES || ny > 65535) { h->err = "grid too large"; return YGPU_ERR_UNSUPPORTED; }
+  A.assign((size_t)YG_NA * cells, 0.0);
+  mask.assign(cells, 0u);
+  static const int SU[2][9][3] = {
+      {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {1, 0, 1}, {1, -1, 1}, {0, -1, 1}},
+      {{0, 0, 1}, {0, 1, 1}, {0, -1, 1}, {1, 0, 1}, {-1, 0, 1}, {0, 0, 0}, {0, 1, 0}, {-1, 1, 0}, {-1, 0, 0}}};
+  for (int r = 0; r < n; ++r) {
+    const int cell = r / 2, comp = r & 1;
+    const int i = cell / ny, j = cell % ny;
+    const int q = cell;   // vectors and the stencil matrix keep the row order
+    for (int k = a_ptr1[r] - 1; k < a_ptr1[r + 1] - 1; ++k) {
+      const int col = a_idx1[k] - 1;
+      if (col < 0 || col >= n) { h->err = "column index out of range"; return YGPU_ERR_ARG; }
+      const int c2 = col / 2, comp2 = col & 1;
+      const int di = c2 / ny - i, dj = c2 % ny - j;
+      int s = -1;
+      for (int t = 0; t < 9; ++t)
+        if (SU[comp][t][0] == di && SU[comp][t][1] == dj && SU[comp][t][2] == comp2) { s = t; break; }
+      if (s < 0) { h->err = "entry outside the C-grid SSA stencil"; return YGPU_ERR_UNSUPPORTED; }
+      A[(size_t)YG_NA * q + 2 * s + comp] += a_val[k];
+      mask[q] |= 1u << (16 * comp + s);
+    }
+    if (need_diag && !((mask[q] >> (16 * comp)) & 1u)) {
+      h->err = "row without a diagonal entry";
+      return YGPU_ERR_UNSUPPORTED;
+    }
+  }
+  return YGPU_OK;
+}
+
 int run_any(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_problem* probs, int diva) {
   if (!h || !par || np < 1 || !probs) return YGPU_ERR_ARG;
   h->err.clear();
@@ -277,51 +324,16 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
     return YGPU_OK;
   }
   const int cells = n / 2;
-  // grid: ny from the widest coupling (uy(i+1,j) is 2 ny + 1 columns right of ux(i,j))
-  int ny = o.ny_hint;
-  if (ny <= 0) {
-    long long maxd = 0;
-    for (int r = 0; r < n; ++r)
-      for (int k = a_ptr1[r] - 1; k < a_ptr1[r + 1] - 1; ++k) maxd = std::max<long long>(maxd, (long long)(a_idx1[k] - 1) - r);
-    ny = maxd >= 2 ? (int)(maxd / 2) : cells;
-  }
-  if (ny <= 0 || cells % ny) { h->err = "cannot infer the grid of this CSR pattern (give -ygpu_ny)"; return YGPU_ERR_UNSUPPORTED; }
-  const int nx = cells / ny;
-  if (nx > YG_MAX_LINES || ny > 65535) { h->err = "grid too large"; return YGPU_ERR_UNSUPPORTED; }
+  int nx = 0, ny = 0;
+  std::vector<double> A;
+  std::vector<uint32_t> mask;
+  if ((rc = csr_records(h, n, a_ptr1, a_idx1, a_val, o.ny_hint, o.k.precon != 0, nx, ny, A, mask)))
+    return rc;
   std::vector<int> soff;
   int T;
   step_table(nx, ny, soff, T);
   if (T > YG_MAX_STEPS) { h->err = "grid too large for the step table"; return YGPU_ERR_UNSUPPORTED; }
-  // CSR -> stencil matrix records (one per cell, in the row order)
-  std::vector<double> A((size_t)YG_NA * cells, 0.0), bs(n), xs(n);
-  std::vector<uint32_t> mask(cells, 0u);
-  static const int SU[2][9][3] = {
-      {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {1, 0, 1}, {1, -1, 1}, {0, -1, 1}},
-      {{0, 0, 1}, {0, 1, 1}, {0, -1, 1}, {1, 0, 1}, {-1, 0, 1}, {0, 0, 0}, {0, 1, 0}, {-1, 1, 0}, {-1, 0, 0}}};
-  for (int r = 0; r < n; ++r) {
-    const int cell = r / 2, comp = r & 1;
-    const int i = cell / ny, j = cell % ny;
-    const int q = cell;   // vectors and the stencil matrix keep the row order
-    (void)i; (void)j;
-    bs[r] = b[r];
-    xs[r] = x_inout[r];
-    for (int k = a_ptr1[r] - 1; k < a_ptr1[r + 1] - 1; ++k) {
-      const int col = a_idx1[k] - 1;
-      if (col < 0 || col >= n) { h->err = "column index out of range"; return YGPU_ERR_ARG; }
-      const int c2 = col / 2, comp2 = col & 1;
-      const int di = c2 / ny - i, dj = c2 % ny - j;
-      int s = -1;
-      for (int t = 0; t < 9; ++t)
-        if (SU[comp][t][0] == di && SU[comp][t][1] == dj && SU[comp][t][2] == comp2) { s = t; break; }
-      if (s < 0) { h->err = "entry outside the C-grid SSA stencil"; return YGPU_ERR_UNSUPPORTED; }
-      A[(size_t)YG_NA * q + 2 * s + comp] += a_val[k];
-      mask[q] |= 1u << (16 * comp + s);
-    }
-    if (!((mask[q] >> (16 * comp)) & 1u) && o.k.precon != 0) {
-      h->err = "row without a diagonal entry";
-      return YGPU_ERR_UNSUPPORTED;
-    }
-  }
+  std::vector<double> bs(b, b + n), xs(x_inout, x_inout + n);
   const int kmode = o.prec == 2 ? YG_MODE_MIXED : YG_MODE_FP64;
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   Arena ar;
@@ -408,6 +420,7 @@ struct YgBatchBase {
   virtual ~YgBatchBase() {}
   virtual int run(int reset, double* secs) = 0;
   virtual int download(ygpu_ssa_problem* probs) = 0;
+  virtual int spmv(int fp32, int reps, double* y, double* ms, double* bytes) = 0;
   int64_t bytes = 0;
 };
 
@@ -420,6 +433,9 @@ struct YgBatchT : YgBatchBase {
     return B.run(secs);
   }
   int download(ygpu_ssa_problem* probs) override { return B.download(probs); }
+  int spmv(int fp32, int reps, double* y, double* ms, double* bytes) override {
+    return B.spmv(fp32, reps, y, ms, bytes);
+  }
 };
 
 struct ygpu_batch {
@@ -527,6 +543,62 @@ int ygpu_batch_run(ygpu_batch* b, int32_t reset, double* seconds) {
 int ygpu_batch_download(ygpu_batch* b, ygpu_ssa_problem* probs) {
   if (!b || !probs) return YGPU_ERR_ARG;
   return b->b->download(probs);
+}
+
+int ygpu_batch_spmv(ygpu_batch* b, int32_t fp32, int32_t reps, double* y, double* ms_per_launch,
+                    double* bytes_per_launch) {
+  if (!b || reps < 0) return YGPU_ERR_ARG;
+  b->h->err.clear();
+  if (hipSetDevice(b->h->device) != hipSuccess) { b->h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  return b->b->spmv(fp32 ? 1 : 0, reps, y, ms_per_launch, bytes_per_launch);
+}
+
+int ygpu_csr_spmv(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
+                  const int32_t* a_idx1, const double* a_val, const double* x, double* y,
+                  const char* opts) {
+  if (!h) return YGPU_ERR_ARG;
+  h->err.clear();
+  if (n <= 0 || (n & 1) || !a_ptr1 || !a_idx1 || !a_val || !x || !y) {
+    h->err = "invalid arguments (n must be even: two unknowns per cell)";
+    return YGPU_ERR_ARG;
+  }
+  if (a_ptr1[0] != 1 || a_ptr1[n] - 1 != nnz || nnz < 0) {
+    h->err = "a_ptr must be 1-based with a_ptr(n+1)-1 == nnz";
+    return YGPU_ERR_ARG;
+  }
+  Opts o;
+  std::string e;
+  int rc = parse_opts(opts, o, e);
+  if (rc) { h->err = e; return rc; }
+  int nx = 0, ny = 0;
+  std::vector<double> A;
+  std::vector<uint32_t> mask;
+  if ((rc = csr_records(h, n, a_ptr1, a_idx1, a_val, o.ny_hint, false, nx, ny, A, mask))) return rc;
+  if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
+  Arena ar;
+  YgMember M{};
+  M.nx = nx; M.ny = ny; M.nz = 1; M.dx = M.dy = 1.0;
+  YgMember* d_mem; long long* d_coff; double *d_A, *d_x, *d_y;
+  YG_CHECK(h, ar.alloc(&d_mem, 1));
+  YG_CHECK(h, ar.alloc(&d_coff, 1));
+  YG_CHECK(h, ar.alloc(&d_A, A.size()));
+  YG_CHECK(h, ar.alloc(&d_x, n));
+  YG_CHECK(h, ar.alloc(&d_y, n));
+  hipStream_t st = h->stream;
+  const long long zero = 0;
+  YG_CHECK(h, hipMemcpyAsync(d_mem, &M, sizeof(M), hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_coff, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_A, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, st));
+  YG_CHECK(h, hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  YgBatchGeom G{};
+  G.members = d_mem; G.coffs = d_coff; G.zoffs = d_coff; G.soff = nullptr; G.nmem = 1;
+  G.total_cells = n / 2; G.seg = d_coff; G.segm = nullptr; G.nseg = 1;
+  // the CSR boundary is non-periodic: a wrapped coupling is outside the stencil above
+  yg_launch_spmv<double>(G, d_A, d_x, d_y, 0, 0, st);
+  YG_CHECK(h, hipGetLastError());
+  YG_CHECK(h, hipMemcpyAsync(y, d_y, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  YG_CHECK(h, hipStreamSynchronize(st));
+  return YGPU_OK;
 }
 
 int ygpu_batch_destroy(ygpu_batch* b) {

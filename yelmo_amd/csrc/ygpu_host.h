@@ -19,6 +19,7 @@
 #include "ygpu_diva.h"
 #include "ygpu_fields.h"
 #include "ygpu_krylov.h"
+#include "ygpu_spmv.h"
 
 struct ygpu_handle {
   int device = 0;
@@ -518,6 +519,58 @@ struct Batch {
     }
     gchunk = chunk;
     return true;
+  }
+
+  // y = A x0 over every member with the stencil records of the last
+  // assembled systems (x0: the last solution), reps timed launches
+  int spmv(int fp32, int reps, double* y_host, double* ms, double* bytes) {
+    if (fp32 && !d_A32) { h->err = "spmv: FP32 records exist in the mixed mode only"; return YGPU_ERR_ARG; }
+    hipStream_t st = h->stream;
+    const YgBatchGeom G = geom();
+    const int wrapx = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_X);
+    const int wrapy = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_Y ||
+                       fp.BC == YG_BC_MISMIP3D || fp.BC == YG_BC_TROUGH);
+    const long long n = 2 * total_cells;
+    Arena tmp;
+    double* y64 = nullptr;
+    float *x32 = nullptr, *y32 = nullptr;
+    if (fp32) {
+      YG_CHECK(h, tmp.alloc(&x32, n));
+      YG_CHECK(h, tmp.alloc(&y32, n));
+      yg_launch_to_float(d_x, x32, n, st);
+    } else {
+      YG_CHECK(h, tmp.alloc(&y64, n));
+    }
+    auto launch = [&]() {
+      if (fp32) yg_launch_spmv<float>(G, d_A32, x32, y32, wrapx, wrapy, st);
+      else yg_launch_spmv<double>(G, d_A64, d_x, y64, wrapx, wrapy, st);
+    };
+    launch();   // untimed first launch
+    hipEvent_t e0, e1;
+    YG_CHECK(h, hipEventCreate(&e0));
+    YG_CHECK(h, hipEventCreate(&e1));
+    YG_CHECK(h, hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) launch();
+    YG_CHECK(h, hipEventRecord(e1, st));
+    YG_CHECK(h, hipGetLastError());
+    YG_CHECK(h, hipEventSynchronize(e1));
+    float t = 0.f;
+    YG_CHECK(h, hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = reps > 0 ? t / reps : 0.0;
+    // 18 matrix values, one velocity pair read and one written per cell
+    if (bytes) *bytes = (double)total_cells * (fp32 ? 22.0 * 4 : 22.0 * 8);
+    if (y_host) {
+      if (fp32) {
+        std::vector<float> yf(n);
+        YG_CHECK(h, hipMemcpy(yf.data(), y32, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (long long k = 0; k < n; ++k) y_host[k] = yf[k];
+      } else {
+        YG_CHECK(h, hipMemcpy(y_host, y64, sizeof(double) * n, hipMemcpyDeviceToHost));
+      }
+    }
+    return YGPU_OK;
   }
 
   // the Picard loop (velocity_ssa.f90:172-334 / velocity_diva.f90:195-404)

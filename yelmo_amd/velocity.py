@@ -135,6 +135,21 @@ class Lgs:
         return len(self.b_value)
 
 
+def csr_spmv(a_ptr, a_index, a_value, x, opts: str = "", handle=None) -> np.ndarray:
+    """y = A x on the device for 1-based CSR arrays of the C-grid stencil
+    (lis_matvec at the linear-solver boundary)."""
+    h = handle or _lib.default_handle()
+    ap = np.ascontiguousarray(a_ptr, dtype=np.int32)
+    ai = np.ascontiguousarray(a_index, dtype=np.int32)
+    av = np.ascontiguousarray(a_value, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(xx)
+    n = len(ap) - 1
+    h.check(h.lib.ygpu_csr_spmv(h.h, n, int(ap[-1] - 1), ap.ctypes.data, ai.ctypes.data,
+                                av.ctypes.data, xx.ctypes.data, y.ctypes.data, opts.encode()))
+    return y
+
+
 def linear_solver_matrix_solve(lgs: Lgs, lis_settings: str, handle=None) -> Lgs:
     """linear_solver_matrix_solve (solver_linear.F90:151): solves in place on
     the GPU, LIS option grammar, status 0/2/4."""
@@ -190,6 +205,18 @@ class ResidentBatch:
     def download(self) -> list[SolveInfo]:
         self.h.check(self.h.lib.ygpu_batch_download(self.b, self.arr))
         return [_info(self.arr[k]) for k in range(len(self.problems))]
+
+    def spmv(self, fp32: bool = False, reps: int = 10, want_y: bool = False):
+        """y = A x of every domain's last assembled system (x: its last
+        solution) in one launch; returns (y or None, ms per launch,
+        algorithmic bytes per launch)."""
+        n = 2 * sum(p.nx * p.ny for p in self.problems)
+        y = np.zeros(n) if want_y else None
+        ms, nbytes = C.c_double(), C.c_double()
+        self.h.check(self.h.lib.ygpu_batch_spmv(self.b, int(fp32), int(reps),
+                                                 None if y is None else y.ctypes.data,
+                                                 C.byref(ms), C.byref(nbytes)))
+        return y, ms.value, nbytes.value
 
     def close(self):
         if self.b:
