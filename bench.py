@@ -173,7 +173,7 @@ def run_diva(args):
     rb.close()
     errs = [rel_l2(vel(p), ref[k]) for k, p in enumerate(probs)]
     unknowns = sum(2 * p.nx * p.ny for p in base)
-    kblock = max([512] + [v.block for v in sched.values()])
+    kname = _krylov_kernel_name(st)
     launches = max(st["krylov_launches"], 1)
     avg_ms = st["krylov_ms"] / launches
     achieved = (st["krylov_bytes"] / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -189,7 +189,8 @@ def run_diva(args):
                    "solver": par.ssa_lis_opt,
                    "unknowns_total": unknowns,
                    "schedules": {f"{k[0]}x{k[1]}": [v.ilu_blocks, v.block]
-                                 for k, v in sorted(sched.items())}},
+                                 for k, v in sorted(sched.items())},
+                   "workgroups_per_member": st["krylov_group"]},
         "max_rel_l2_vs_fp64": max(errs), "error_ok": max(errs) <= 1e-6,
         "fp64_batched_solves_per_s": 48 / t_fp64,
         "speedup_vs_fp64_batched": (48 * args.steps / elapsed) / (48 / t_fp64),
@@ -198,12 +199,20 @@ def run_diva(args):
         "picard_iters_mean": float(np.mean([i.ssa_iter_now for i in info])),
         "krylov_iters_per_solve": st["krylov_iters"] / (48 * args.steps),
         "unknown_iters_per_s": st["unknown_iters"] / elapsed,
-        "roofline": {"bound": "hbm", "kernel": f"yg_krylov_kernel<1, {kblock}>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms},
         "cpu_baseline": None,
     }
     print(json.dumps(_finite(line)), flush=True)
+
+
+def _krylov_kernel_name(st):
+    """The Krylov kernel the statistics' last launch ran: the member-group
+    kernel (several workgroups per member) or the one-workgroup kernel."""
+    if st["krylov_group"] > 1:
+        return f"yg_krylov_group_kernel<{st['krylov_block']}>"
+    return f"yg_krylov_kernel<1, {st['krylov_block']}>"
 
 
 def _serve_run(srv, seqs, arrivals, record):
@@ -512,7 +521,7 @@ def main():
     launches = max(st["krylov_launches"], 1)
     avg_ms = st["krylov_ms"] / launches
     bytes_per_launch = st["krylov_bytes"] / launches
-    kblock = max([512] + [v.block for v in sched.values()])
+    kname = _krylov_kernel_name(st)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
     # ---- CPU baseline: the oracle (FP32 fields + FP64 BiCGSTAB/ILU(0), the
@@ -551,6 +560,7 @@ def main():
             "members": args.members, "side": args.side,
             "solver": par_mixed.ssa_lis_opt, "picard_tol": par_mixed.ssa_iter_conv,
             "schedule": {f"{k[0]}x{k[1]}": [v.ilu_blocks, v.block] for k, v in sched.items()},
+            "workgroups_per_member": st["krylov_group"],
         },
         # per-member velocity error of the mixed path against the FP64 path
         # with converged linear solves (rtol 1e-12), and against the FP64
@@ -571,7 +581,7 @@ def main():
         "picard_iters_mean": float(np.mean([i.ssa_iter_now for i in info])),
         "krylov_iters_per_solve": st["krylov_iters"] / max(args.members * args.steps, 1),
         "unknown_iters_per_s": st["unknown_iters"] / elapsed,
-        "roofline": {"bound": "hbm", "kernel": f"yg_krylov_kernel<1, {kblock}>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,
                      "bytes_per_launch": bytes_per_launch, "launches": launches},

@@ -217,6 +217,8 @@ typedef struct ygpu_stats {
   int64_t krylov_iters;         /* Krylov iterations summed over members        */
   double unknown_iters;         /* sum over members of N * iterations           */
   int64_t picard_iters;         /* Picard iterations summed over members        */
+  int32_t krylov_group;         /* workgroups per member of the last launch     */
+  int32_t krylov_block;         /* threads per workgroup of the last launch     */
 } ygpu_stats;
 int ygpu_stats_enable(ygpu_handle* h, int32_t on);
 int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out);  /* returns and resets     */

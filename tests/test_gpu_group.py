@@ -1,0 +1,108 @@
+"""Member groups (-ygpu_group G): G workgroups, one per CU, share each
+member's mixed-precision solve (block-Jacobi ILU(0) line blocks split over
+the workgroups, group sums and group barriers between them).  The
+preconditioner's line blocks change with G (their count is rounded up to a
+multiple of G) and sums are taken in another order, so iteration counts
+differ from one workgroup per member; the refinement meets the same FP64
+residual criterion and every member stays within the mixed bar of the
+converged FP64 oracle:
+
+  velocity vs converged FP64 oracle   1e-6 rel L2 (the mixed-precision bar)
+  graph replays vs eager loop         bitwise (group sums have a fixed order)
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests._helpers import vel_rel_l2
+from yelmo_amd import _lib, generator
+from yelmo_amd.params import SsaParams
+from yelmo_amd.velocity import ResidentBatch
+
+pytestmark = pytest.mark.gpu
+
+OPT = "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false"
+STATE = ("ux_b", "uy_b", "visc_eff", "beta_acx", "taub_acx", "ssa_err_acx")
+
+
+def _converged(par):
+    return par.with_(precision="fp64",
+                     ssa_lis_opt="-i bicgstab -p ilu -maxiter 3000 -tol 1.0e-12 -initx_zeros false")
+
+
+def _batch(probs, par, opt, runs=1, diva=False):
+    h = _lib.default_handle()
+    rb = ResidentBatch(probs, par.with_(ssa_lis_opt=opt), diva=diva, handle=h)
+    out = []
+    for _ in range(runs):
+        rb.run(reset=True)
+        info = rb.download()
+        out.append(([{n: p.state[n].copy() for n in STATE} for p in probs],
+                    [(i.ssa_iter_now, i.ssa_lin_iter, i.status) for i in info]))
+    rb.close()
+    return out
+
+
+@pytest.mark.parametrize("G", [2, 4, 8])
+def test_group_members_within_mixed_bar(G):
+    """Three members of different sizes (the largest sets the groups' line
+    counts), factors in BF16."""
+    print(f"case group G={G}", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    sides = ((161, 8e3, 5), (129, 10e3, 6), (97, 14e3, 7))
+    probs = [generator.make_problem(n, dx, seed=s, dtype=np.float64) for n, dx, s in sides]
+    (states, infos), = _batch([p.copy() for p in probs], par,
+                              OPT + f" -ygpu_factor bf16 -ygpu_group {G}")
+    for k, p in enumerate(probs):
+        ref_st, ref = oracle.calc_velocity_ssa(p.copy(), _converged(par), dp=True)
+        err = vel_rel_l2(states[k], ref_st)
+        print(k, "picard/krylov/status", infos[k], "oracle picard", ref["ssa_iter_now"],
+              "rel L2", err, flush=True)
+        assert infos[k][2] == 0
+        assert err <= 1e-6
+
+
+def test_group_members_in_turn_and_empty_ranks():
+    """40 members of 41 x 41 with G = 8: 32 groups take the first 32
+    members and then the rest; 41 lines make 7 line blocks, so one
+    workgroup of every group owns no lines.  FP32 factors."""
+    print("case group many", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    probs = [generator.make_problem(41, 32e3, seed=100 + k, dtype=np.float64) for k in range(40)]
+    (states, infos), = _batch([p.copy() for p in probs], par,
+                              OPT + " -ygpu_factor fp32 -ygpu_group 8")
+    worst = 0.0
+    for k, p in enumerate(probs):
+        ref_st, _ = oracle.calc_velocity_ssa(p.copy(), _converged(par), dp=True)
+        err = vel_rel_l2(states[k], ref_st)
+        worst = max(worst, err)
+        assert infos[k][2] == 0, (k, infos[k])
+        assert err <= 1e-6, (k, err)
+    print("worst rel L2", worst, flush=True)
+
+
+def test_group_graph_replays_equal_eager_loop():
+    print("case group graph", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    probs = [generator.make_problem(n, 10e3, seed=20 + n, dtype=np.float64) for n in (97, 113, 81)]
+    opt = OPT + " -ygpu_factor bf16 -ygpu_group 4"
+    (eager,) = _batch([p.copy() for p in probs], par, opt + " -ygpu_graph 0")
+    runs = _batch([p.copy() for p in probs], par, opt + " -ygpu_graph 3", runs=3)
+    for r, (states, infos) in enumerate(runs):
+        print(r, infos, flush=True)
+        assert infos == eager[1]
+        for k, st in enumerate(states):
+            for n in STATE:
+                assert np.array_equal(st[n], eager[0][k][n]), (r, k, n)
+
+
+def test_group_diva_within_mixed_bar():
+    print("case group diva", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    p = generator.make_problem(96, 16e3, seed=7, dtype=np.float64, nz=21)
+    ref_st, _ = oracle.calc_velocity_diva(p.copy(), _converged(par), dp=True)
+    g = oracle.diva_state(p.copy())
+    (states, infos), = _batch([g], par, OPT + " -ygpu_factor bf16 -ygpu_group 2", diva=True)
+    err = vel_rel_l2(g.state, ref_st, "ux_bar", "uy_bar")
+    print("picard/krylov/status", infos[0], "rel L2", err, flush=True)
+    assert infos[0][2] == 0 and err <= 1e-6

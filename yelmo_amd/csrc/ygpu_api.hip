@@ -38,6 +38,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
     else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
     else if (k == "-ygpu_graph") o.k.graph_chunk = std::max(0, atoi(v.c_str()));
+    else if (k == "-ygpu_group") o.group = atoi(v.c_str());
     else if (k == "-ygpu_factor") {
       if (v == "bf16") o.k.factor16 = 1;
       else if (v == "fp32") o.k.factor16 = 0;

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <string>
 #include <utility>
 #include <vector>
@@ -27,9 +28,10 @@ struct ygpu_handle {
   std::string err;
   bool stats_on = false;
   ygpu_stats stats{};
-  // Krylov schedules of the mixed path per grid (nx, ny): ILU(0) line blocks
-  // and threads per member block, from ygpu_autotune or ygpu_set_schedule
-  std::map<std::pair<int, int>, std::pair<int, int>> sched;
+  // Krylov schedules of the mixed path per grid and workgroups per member
+  // (nx, ny, G): ILU(0) line blocks and threads per member block, from
+  // ygpu_autotune or ygpu_set_schedule
+  std::map<std::tuple<int, int, int>, std::pair<int, int>> sched;
 };
 
 #define YG_CHECK(h, call)                                                          \
@@ -48,6 +50,7 @@ struct Opts {
   YgKrylovOpts k;
   int ny_hint = 0;
   int prec = -1;  // -1 unset, 0 fp64, 2 mixed
+  int group = 0;  // workgroups per member: 0 automatic, 1 one (no groups), 2, 4 or 8
 };
 
 int parse_opts(const char* s, Opts& o, std::string& err);
@@ -79,6 +82,9 @@ struct Batch {
   ygpu_handle* h = nullptr;
   Arena ar;
   int nmem = 0, diva = 0, prec = 0, kmode = 0, block = 1024;
+  // member groups: workgroups per member (1: the one-block kernel) and groups
+  int group = 1, ngroups = 0;
+  int *d_gsync = nullptr, *d_gfail = nullptr;
   long long total_cells = 0, total_z = 0, total_v = 0;
   int zstride = 0;
   std::vector<YgMember> mem;
@@ -171,7 +177,7 @@ struct Batch {
     M.nb = 0;
     tuned_block = 0;
     if (kmode == YG_MODE_MIXED && opts.k.precon == 2 && opts.k.ilu_blocks < 0) {
-      auto it = h->sched.find({P.nx, P.ny});
+      auto it = h->sched.find({P.nx, P.ny, group});
       if (it != h->sched.end()) { M.nb = it->second.first; tuned_block = it->second.second; }
     }
     return YGPU_OK;
@@ -196,6 +202,12 @@ struct Batch {
     zoffs.assign(np + 1, 0);
     soff.clear();
     int maxnx = 0, want_block = 0;
+    long long maxcells = 0;
+    for (int k = 0; k < np; ++k) {
+      maxnx = std::max(maxnx, probs[k].nx);
+      maxcells = std::max(maxcells, (long long)probs[k].nx * probs[k].ny);
+    }
+    if ((rc = pick_group(maxnx, maxcells))) return rc;   // before the schedules are looked up
     for (int k = 0; k < np; ++k) {
       const ygpu_ssa_problem& P = probs[k];
       YgMember& M = mem[k];
@@ -214,7 +226,6 @@ struct Batch {
       coffs[k + 1] = coffs[k] + (long long)P.nx * P.ny;
       zoffs[k + 1] = zoffs[k] + (long long)P.nx * P.ny * P.nz;
       zstride = std::max(zstride, P.nz);
-      maxnx = std::max(maxnx, P.nx);
       fp.n_glen = P.n_glen;
     }
     total_cells = coffs[np];
@@ -222,6 +233,46 @@ struct Batch {
     block = pick_block(par, maxnx);
     if (par->block <= 0 && want_block > block) block = std::min(want_block, 1024);
     return allocate();
+  }
+
+  // Workgroups per member: the member-group kernel covers the mixed mode
+  // with BiCGSTAB and block-Jacobi ILU(0).  Automatic: the largest G of 8,
+  // 4, 2 whose groups (a multiple of 8, one per member) fit the CUs and
+  // leave every workgroup at least 4096 cells of the largest member.
+  int pick_group(int maxnx, long long maxcells) {
+    group = 1;
+    ngroups = 0;
+    const int want = opts.group;
+    if (want == 1) return YGPU_OK;
+    const bool fits = kmode == YG_MODE_MIXED && opts.k.solver == 4 && opts.k.precon == 2 &&
+                      opts.k.ilu_blocks != 0;
+    if (want != 0 && want != 2 && want != 4 && want != 8) {
+      h->err = "-ygpu_group takes 0, 1, 2, 4 or 8";
+      return YGPU_ERR_ARG;
+    }
+    if (!fits) {
+      if (want == 0) return YGPU_OK;
+      h->err = "-ygpu_group: member groups run the mixed precision, BiCGSTAB, block ILU(0) solve";
+      return YGPU_ERR_UNSUPPORTED;
+    }
+    int ncu = 0;
+    YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    const int groups8 = ((nmem + 7) / 8) * 8;
+    int G = want;
+    if (G == 0) {
+      G = 1;
+      for (int g : {8, 4, 2})
+        if (groups8 * g <= ncu && maxcells / g >= 4096) { G = g; break; }
+      if (G == 1) return YGPU_OK;
+    }
+    if (8 * G > ncu) { h->err = "-ygpu_group: more workgroups than CUs"; return YGPU_ERR_ARG; }
+    if (maxnx > 64 * G * (YG_GROUP_THREADS / 64)) {
+      h->err = "-ygpu_group: a member has more lines than the group's waves can sweep";
+      return YGPU_ERR_ARG;
+    }
+    group = G;
+    ngroups = std::min(groups8, (ncu / G) / 8 * 8);
+    return YGPU_OK;
   }
 
   int allocate() {
@@ -249,6 +300,10 @@ struct Batch {
     d_lim_n = ints[4]; d_kit = ints[5]; d_kst = ints[6];
     YG_CHECK(h, ar.alloc(&d_resid, nmem));
     YG_CHECK(h, ar.alloc(&d_kres, nmem));
+    if (group > 1) {
+      YG_CHECK(h, ar.alloc(&d_gsync, (size_t)ngroups * (YG_GSYNC_BYTES / 4)));
+      YG_CHECK(h, ar.alloc(&d_gfail, 1));
+    }
     // fields
     R** r2d[] = {(R**)&f.c_bed, (R**)&f.f_slide, (R**)&f.taud_acx, (R**)&f.taud_acy,
                  (R**)&f.taul_acx, (R**)&f.taul_acy, (R**)&f.H_ice, (R**)&f.f_ice,
@@ -475,7 +530,8 @@ struct Batch {
     yg_launch_assemble<R>(G, f, fp, d_done, d_A64, kmode == YG_MODE_MIXED ? d_A32 : nullptr, d_b,
                           d_x, d_mask, st);
     if (ev0) (void)hipEventRecord(ev0, st);
-    yg_launch_krylov(ka, kmode, nmem, block, st);
+    if (group > 1) yg_launch_krylov_group(ka, ngroups, st);
+    else yg_launch_krylov(ka, kmode, nmem, block, st);
     if (ev1) (void)hipEventRecord(ev1, st);
     yg_launch_epilogue<R>(G, f, fp, d_x, d_kit, d_kst, iter_max, ps, st);
   }
@@ -599,6 +655,8 @@ struct Batch {
                 fp.BC == YG_BC_TROUGH);
     ka.skip = d_done; ka.out_iters = d_kit; ka.out_status = d_kst; ka.out_resid = d_kres;
     ka.opts = opts.k;
+    ka.group = group; ka.nmembers = nmem; ka.gsync = d_gsync; ka.group_fail = d_gfail;
+    if (d_gfail) YG_CHECK(h, hipMemsetAsync(d_gfail, 0, sizeof(int), st));
     long long* d_tm = nullptr;
     const bool phase = getenv("YGPU_PHASE_TIMERS") != nullptr;
     if (phase) {
@@ -639,6 +697,8 @@ struct Batch {
         float kms = 0.f;
         YG_CHECK(h, hipEventElapsedTime(&kms, k0, k1));
         h->stats.krylov_launches += 1;
+        h->stats.krylov_group = group;
+        h->stats.krylov_block = group > 1 ? YG_GROUP_THREADS : block;
         h->stats.krylov_ms += kms;
         const bool mixed = kmode == YG_MODE_MIXED;
         for (int k : active) {
@@ -669,6 +729,16 @@ struct Batch {
       fprintf(stderr, "ygpu phases member0 (ms): factor %.3f sweepL %.3f sweepU %.3f spmv %.3f total %.3f\n",
               tmh[0] * us * 1e-3, tmh[1] * us * 1e-3, tmh[2] * us * 1e-3, tmh[3] * us * 1e-3,
               tmh[7] * us * 1e-3);
+    }
+    if (d_gfail) {
+      int gf = 0;
+      YG_CHECK(h, hipMemcpyAsync(&gf, d_gfail, sizeof(int), hipMemcpyDeviceToHost, st));
+      YG_CHECK(h, hipStreamSynchronize(st));
+      if (gf) {
+        h->err = "member group: a workgroup waited more than 2 s for its group "
+                 "(the group's workgroups were not resident together)";
+        return YGPU_ERR_DEVICE;
+      }
     }
     if (f.asm_err) {
       std::vector<int> bad(nmem);

@@ -57,8 +57,22 @@ struct YgKrylovArgs {
   double* out_resid;
   long long* timers;        // optional: per member 8 phase totals (wall clock ticks)
   YgKrylovOpts opts;
+  // member groups (yg_launch_krylov_group): workgroups per member, members,
+  // per-group synchronization words (YG_GSYNC_BYTES each, cleared by the
+  // launcher) and a flag set when a group wait timed out (cleared by the host)
+  int group;
+  int nmembers;
+  int* gsync;
+  int* group_fail;
 };
 
 void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, hipStream_t st);
+// The member-group kernel: ngroups groups of a.group workgroups (512 threads
+// each; ngroups a multiple of 8, ngroups * a.group at most the CU count) take
+// the a.nmembers members in turn.  Mixed mode, BiCGSTAB, block-Jacobi ILU(0)
+// only.  Clears the groups' synchronization words first.
+void yg_launch_krylov_group(const YgKrylovArgs& a, int ngroups, hipStream_t st);
+#define YG_GSYNC_BYTES 1024
+#define YG_GROUP_THREADS 512
 // sets the kernels' launch attributes; call before capturing a launch in a graph
 void yg_krylov_prepare();

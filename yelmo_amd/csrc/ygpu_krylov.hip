@@ -5,6 +5,7 @@
 void yg_launch_krylov_1024(const YgKrylovArgs& a, int mode, int nmembers, int block,
                            hipStream_t st);
 void yg_krylov_attrs_1024();
+void yg_krylov_group_attrs();
 
 void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, hipStream_t st) {
   if (block <= 512) yg_launch_krylov_nt<512>(a, mode, nmembers, block, st);
@@ -14,4 +15,5 @@ void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, 
 void yg_krylov_prepare() {
   yg_krylov_attrs_nt<512>();
   yg_krylov_attrs_1024();
+  yg_krylov_group_attrs();
 }

@@ -38,6 +38,7 @@ namespace {
 //            of lpw <= 64 lines, one per wave; couplings between blocks are
 //            dropped from the factorization, so waves never wait for each
 //            other inside a sweep.
+struct Grp;
 struct Ctx {
   const YgMember* mb;
   int nx, ny, T;
@@ -51,6 +52,19 @@ struct Ctx {
   int nwl;                    // waves that own lines
   int lpw;                    // lines per wave (the line block of a wave)
   long long* tm;              // phase timers (thread 0), may be null
+  // Offset (values) from a factor pointer to the U part of its line-block
+  // factors: 8 ncell for a whole member (FP32/FP64 and BF16 alike); a part of
+  // a member (the lines of one block of a member group) addresses the
+  // member's U part from its own L part.
+  long long fu, fu16;
+  // The cells [cb, ce) of the member whose rows this block computes in the
+  // SpMV and vector passes (the whole member, or this block's share in a
+  // member group), and the group (null for one block per member).
+  int cb, ce;
+  const Grp* grp;
+  // the context of the preconditioner's part of the member (itself for one
+  // block per member)
+  const Ctx* pc;
 };
 
 // optional phase timing: 0 factor, 1 sweep L, 2 sweep U, 3 spmv, 4 vectors
@@ -85,6 +99,85 @@ YG_DEV T& lds(int k) { return reinterpret_cast<T*>(yg_dyn_lds)[k]; }
 __shared__ int yg_soff[YG_MAX_STEPS + 1];
 __shared__ double yg_ring[16][3][10];
 __shared__ double yg_red[16 * 4];   // block reductions
+
+// ---- member groups ------------------------------------------------------
+// G workgroups (one per CU) can share one member's solve.  Workgroup b runs
+// on XCD b % 8 (the dispatcher deals workgroups round-robin over the 8
+// XCDs), so a group is made of workgroups with equal b % 8 and shares one
+// L2: stores reach that L2 when they complete (the L1 writes through), and a
+// load that skips the L1 (device scope) sees them.  A group barrier is then
+//   every wave waits for its own stores, block barrier, one lane adds 1 to
+//   the group's counter and waits for it to reach G x (barriers so far);
+// and data written by another block of the group (the halo lines of x in the
+// SpMV, the partial sums of the reductions) is read only with device-scope
+// loads.  No cache is written back or invalidated.  Every wait is bounded in
+// time: a block that waits longer than YG_GRP_SPIN sets the group's error
+// flag, after which no block of the group waits again (the solve runs on to
+// its iteration limit on meaningless data and is reported failed).
+#define YG_GRP_SPIN 200000000ll   // wall-clock ticks (100 MHz): 2 s
+struct Grp {
+  int* cnt;       // arrivals at group barriers (monotonic over the launch)
+  int* err;       // set when a wait timed out
+  double* part;   // partial sums: [2 (barrier parity)][G][4]
+  int G, rank;
+};
+__shared__ int yg_gbar;    // group barriers this block has passed
+__shared__ int yg_gfail;   // a wait of this group timed out
+
+YG_DEV int grp_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+YG_DEV double grp_loadd(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier k of the group (k = yg_gbar + 1 on every block).
+YG_DEV void grp_barrier(const Grp& g) {
+  __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have reached the L2
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int k = yg_gbar + 1;
+    if (!yg_gfail) {
+      __hip_atomic_fetch_add(g.cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const long long t0 = wall_clock64();
+      while (grp_load(g.cnt) < g.G * k) {
+        if (grp_load(g.err) || wall_clock64() - t0 > YG_GRP_SPIN) {
+          yg_gfail = 1;
+          __hip_atomic_store(g.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    yg_gbar = k;
+  }
+  __syncthreads();
+}
+
+// Sums of K values over the block (one block per member) or over the group:
+// every block publishes its block sums, passes a group barrier and adds the
+// G partial sums in rank order, so every block of the group gets the same
+// bits.  Partial sums alternate between two buffers by barrier parity: a
+// block can reach the next sum only after every block has passed this
+// barrier, and the sum after that only after every block has read this one.
+template <int K>
+YG_DEV void csum(const Ctx& c, double (&v)[K]) {
+  yg_block_sum<K>(v, yg_red);
+  if (!c.grp) return;
+  const Grp& g = *c.grp;
+  double* slot = g.part + (size_t)((yg_gbar + 1) & 1) * g.G * 4;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int e = 0; e < K; ++e) slot[g.rank * 4 + e] = v[e];
+  }
+  grp_barrier(g);
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    double t = 0.0;
+    for (int r = 0; r < g.G; ++r) t += grp_loadd(slot + r * 4 + e);
+    v[e] = t;
+  }
+}
 
 // Global-memory views of pointers handed to the (non-inlined) wavefront
 // phases: the compiler then emits global (not flat) accesses, whose
@@ -333,7 +426,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
           YG_GLOBAL TA* fl = Fg + 8 * qb;
           fl[0] = (TA)a[2]; fl[1] = (TA)a[4]; fl[2] = (TA)a[8]; fl[3] = (TA)b[8];
           fl[4] = (TA)b[4]; fl[5] = (TA)b[7]; fl[6] = (TA)b[2]; fl[7] = (TA)b[5];
-          YG_GLOBAL TA* fu = Fg + 8 * nc + 12 * qb;
+          YG_GLOBAL TA* fu = Fg + c.fu + 12 * qb;
           fu[0] = (TA)X.x5; fu[1] = (TA)X.x3; fu[2] = (TA)X.x7; fu[3] = (TA)X.x1;
           fu[4] = (TA)X.x6; fu[5] = (TA)X.xd; fu[6] = (TA)X.y6; fu[7] = (TA)X.y1;
           fu[8] = (TA)X.y3; fu[9] = (TA)X.yd; fu[10] = (TA)0; fu[11] = (TA)0;
@@ -616,17 +709,21 @@ YG_DEV __amdgpu_buffer_rsrc_t yg_rsrc(const T* base, size_t count) {
 }
 
 template <typename T> struct Pair16;   // T pair (8 or 16 bytes) through buffer loads
+// load<true>: a device-scope load (cache policy SC1), which skips the L1:
+// data another workgroup of a member group wrote.
 template <> struct Pair16<float> {
+  template <bool COH = false>
   static YG_DEV float2 load(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, COH ? 16 : 0));
   }
   static YG_DEV void store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, off, 0, 0);
   }
 };
 template <> struct Pair16<double> {
+  template <bool COH = false>
   static YG_DEV double2 load(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, COH ? 16 : 0));
   }
   static YG_DEV void store(double2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, 0, 0);
@@ -665,24 +762,17 @@ YG_DEV uint16_t to_bf16(float v) {
 }
 YG_DEV float from_bf16(uint32_t h16) { return __builtin_bit_cast(float, h16 << 16); }
 
-YG_DEV void load_bf16x8(__amdgpu_buffer_rsrc_t r, unsigned off, float (&v)[8]) {
-  const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    v[2 * e] = from_bf16(w[e] & 0xFFFFu);
-    v[2 * e + 1] = from_bf16(w[e] >> 16);
-  }
+// The sweeps keep prefetched BF16 words packed in registers and widen them
+// only at their step.  Widened at load time, each value would be an
+// arithmetic result of a load still in flight; the compiler then moves the
+// widening of every prefetched step to the top of the loop (an operation on
+// all incoming values of a loop-carried value becomes one operation after
+// it), where it waits for all of them, and the prefetch depth collapses.
+YG_DEV uint4 load_raw16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
-YG_DEV void load_bf16_urow(__amdgpu_buffer_rsrc_t r, unsigned off, float (&v)[10]) {
-  float g[8];
-  load_bf16x8(r, off, g);
-  const float2 d = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0));
-#pragma unroll
-  for (int e = 0; e < 8; ++e) v[e] = g[e];
-  v[8] = d.x;
-  v[9] = d.y;
-}
+YG_DEV float bf_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+YG_DEV float bf_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); }
 
 // Geometry of one wave's line block in the streamed sweeps, all scalars.
 // Per step t the lanes with cells are glo(t) <= lane <= min(L-1, t/2); a
@@ -805,7 +895,7 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   if (w >= nwl) return;
   const int i0 = w * lpw, L = min(lpw, nx - i0);
   const BlockGeom G{L, ny + 2 * (L - 1), ny};
-  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + (size_t)8 * nc, (size_t)12 * nc);
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + c.fu, (size_t)12 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
   const int lane_q = i0 * ny + lane;
@@ -852,7 +942,7 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
 YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ rhs, float* __restrict__ ys) {
   PhaseTimer ptm(c, 1);
-  constexpr int D = 12;
+  constexpr int D = 24;   // steps in flight: 6 registers per step (4 packed factor words, 2 rhs)
   constexpr unsigned SF = 16, SV = 8;
   using T = float;
   using V2 = typename Vec2<T>::type;
@@ -867,12 +957,12 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
   const int lane_q = i0 * ny + lane;
   const int lane_n = (i0 + lane) * ny - 2 * lane;
-  T f[D][8];
+  uint4 fw[D];
   V2 r[D];
   StepQ qc = step_q(G, 0), qn = step_q(G, 0);
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    load_bf16x8(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+    fw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
     r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
     step_next(G, qn);
     __builtin_amdgcn_sched_barrier(0);
@@ -882,10 +972,11 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       const bool h = lane >= qc.lo && lane <= qc.hi;
+      const T f0 = bf_lo(fw[s].x), f1 = bf_hi(fw[s].x), f2 = bf_lo(fw[s].y), f3 = bf_hi(fw[s].y);
+      const T f4 = bf_lo(fw[s].z), f5 = bf_hi(fw[s].z), f6 = bf_lo(fw[s].w), f7 = bf_hi(fw[s].w);
       const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
-      T yx = r[s].x - f[s][0] * n2x - f[s][1] * o1x - f[s][2] * o1y;
-      T yy = r[s].y - f[s][3] * n2x - f[s][4] * n2y - f[s][5] * n1x - f[s][6] * o1y -
-             f[s][7] * yx;
+      T yx = r[s].x - f0 * n2x - f1 * o1x - f2 * o1y;
+      T yy = r[s].y - f3 * n2x - f4 * n2y - f5 * n1x - f6 * o1y - f7 * yx;
       yx = h ? yx : (T)0;
       yy = h ? yy : (T)0;
       V2 o;
@@ -894,7 +985,7 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
       Pair16<T>::store(o, rY, cell_off(qc, lane, lane_q, SV));
       o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
       step_next(G, qc);
-      load_bf16x8(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+      fw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
       r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
       step_next(G, qn);
       __builtin_amdgcn_sched_barrier(0);
@@ -905,7 +996,7 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
 YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ ys, float* __restrict__ z) {
   PhaseTimer ptm(c, 2);
-  constexpr int D = 12;
+  constexpr int D = 20;   // 8 registers per step: 4 packed words, the 2 inverse diagonals, y
   constexpr unsigned SF = 32, SV = 8;
   using T = float;
   using V2 = typename Vec2<T>::type;
@@ -915,17 +1006,20 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   if (w >= nwl) return;
   const int i0 = w * lpw, L = min(lpw, nx - i0);
   const BlockGeom G{L, ny + 2 * (L - 1), ny};
-  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + (size_t)8 * nc, (size_t)16 * nc);
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + c.fu16, (size_t)16 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
   const int lane_q = i0 * ny + lane;
   const int lane_n = (i0 + lane) * ny - 2 * lane;
-  T f[D][10];
+  uint4 gw[D];   // the 8 scaled off-diagonal U entries
+  V2 dg[D];      // ux-row and uy-row inverse diagonals
   V2 yv[D];
   StepQ qc = step_q(G, G.Tb - 1), qn = step_q(G, G.Tb - 1);
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    load_bf16_urow(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+    const unsigned off = cell_off(qn, lane, lane_q, SF);
+    gw[s] = load_raw16(rF, off);
+    dg[s] = Pair16<T>::load(rF, off + 16);
     yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
     step_prev(G, qn);
     __builtin_amdgcn_sched_barrier(0);
@@ -935,12 +1029,13 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       const bool h = lane >= qc.lo && lane <= qc.hi;
+      // 0..4 scaled ux-row U entries (slots 5,3,7,1,6), 5..7 scaled uy-row
+      // entries (slots 6,1,3)
+      const T g0 = bf_lo(gw[s].x), g1 = bf_hi(gw[s].x), g2 = bf_lo(gw[s].y), g3 = bf_hi(gw[s].y);
+      const T g4 = bf_lo(gw[s].z), g5 = bf_hi(gw[s].z), g6 = bf_lo(gw[s].w), g7 = bf_hi(gw[s].w);
       const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
-      // f: 0..4 scaled ux-row U entries, 5..7 scaled uy-row U entries,
-      // 8 ux-row inverse diagonal, 9 uy-row inverse diagonal
-      T zy = f[s][9] * yv[s].y - f[s][5] * o1x - f[s][6] * o1y - f[s][7] * n2y;
-      T zx = f[s][8] * yv[s].x - f[s][0] * zy - f[s][1] * o1x - f[s][2] * n1y -
-             f[s][3] * n2x - f[s][4] * n2y;
+      T zy = dg[s].y * yv[s].y - g5 * o1x - g6 * o1y - g7 * n2y;
+      T zx = dg[s].x * yv[s].x - g0 * zy - g1 * o1x - g2 * n1y - g3 * n2x - g4 * n2y;
       zx = h ? zx : (T)0;
       zy = h ? zy : (T)0;
       V2 o;
@@ -949,7 +1044,9 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
       Pair16<T>::store(o, rZ, nat_off(qc, lane, lane_n, SV));
       o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
       step_prev(G, qc);
-      load_bf16_urow(rF, cell_off(qn, lane, lane_q, SF), f[s]);
+      const unsigned off = cell_off(qn, lane, lane_q, SF);
+      gw[s] = load_raw16(rF, off);
+      dg[s] = Pair16<T>::load(rF, off + 16);
       yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
       step_prev(G, qn);
       __builtin_amdgcn_sched_barrier(0);
@@ -960,8 +1057,8 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
 // FP32 line-block factors -> BF16 storage (layout above)
 YG_DEV void factors_to_bf16(const Ctx& c, const float* __restrict__ F, uint16_t* __restrict__ FB) {
   const int nc = c.ncell;
-  const float* FU = F + (size_t)8 * nc;
-  uint16_t* BU = FB + (size_t)8 * nc;
+  const float* FU = F + c.fu;
+  uint16_t* BU = FB + c.fu16;
   for (int q = threadIdx.x; q < nc; q += blockDim.x) {
     uint32_t lw[4], uw[4];
 #pragma unroll
@@ -990,26 +1087,31 @@ struct Precon {
   const uint16_t* FB = nullptr;   // BF16 copy of line-block factors (mixed mode)
 };
 
-// out = M^-1 in; tmp is scratch of the same size
+// out = M^-1 in; tmp is scratch of the same size.  The sweeps run on the
+// part of the member this block owns (c.pc: the whole member, or the lines
+// of this block of a member group, whose line blocks do not couple to other
+// blocks' lines); in, tmp, out are member vectors.
 template <typename T>
 YG_DEV void precon_apply(const Ctx& c, const Precon<T>& P, const T* in, T* tmp, T* out) {
-  if (P.kind == 2 && c.coupled) {
-    sweep_L<T>(c, P.F, in, tmp);
-    sweep_U<T>(c, P.F, tmp, out);
+  const Ctx& pc = *c.pc;
+  const size_t o = 2 * (size_t)c.cb;
+  if (P.kind == 2 && pc.coupled) {
+    sweep_L<T>(pc, P.F, in, tmp);
+    sweep_U<T>(pc, P.F, tmp, out);
   } else if (P.kind == 2 && P.FB) {
     if constexpr (sizeof(T) == 4) {
-      sweep_L_str16(c, P.FB, in, tmp);
+      if (pc.ncell > 0) sweep_L_str16(pc, P.FB, in + o, tmp + o);
       __syncthreads();
-      sweep_U_str16(c, P.FB, tmp, out);
+      if (pc.ncell > 0) sweep_U_str16(pc, P.FB, tmp + o, out + o);
       __syncthreads();
     }
   } else if (P.kind == 2) {
-    sweep_L_str<T>(c, P.F, in, tmp);
+    if (pc.ncell > 0) sweep_L_str<T>(pc, P.F, in + o, tmp + o);
     __syncthreads();
-    sweep_U_str<T>(c, P.F, tmp, out);
+    if (pc.ncell > 0) sweep_U_str<T>(pc, P.F, tmp + o, out + o);
     __syncthreads();
   } else {
-    for (int q = threadIdx.x; q < 2 * c.ncell; q += blockDim.x)
+    for (int q = 2 * c.cb + threadIdx.x; q < 2 * c.ce; q += blockDim.x)
       out[q] = P.kind == 1 ? in[q] * P.F[q] : in[q];
     __syncthreads();
   }
@@ -1030,7 +1132,7 @@ struct SolveOut {
 // non-periodic border into another line meets a zero coefficient (absent
 // slots hold exact zeros).  Periodic dimensions wrap explicitly.
 // wrap: bit 0 periodic in x, bit 1 periodic in y (wave-uniform)
-template <typename TA, typename TV>
+template <typename TA, typename TV, bool COH>
 YG_DEV void spmv_cell(int nx, int ny, int wrap, __amdgpu_buffer_rsrc_t rA,
                       __amdgpu_buffer_rsrc_t rX, int n, double& yx, double& yy) {
   using V2 = typename Vec2<TV>::type;
@@ -1045,7 +1147,7 @@ YG_DEV void spmv_cell(int nx, int ny, int wrap, __amdgpu_buffer_rsrc_t rA,
     nip = ip * ny + j; nim = im * ny + j; njp = i * ny + jp; njm = i * ny + jm;
     nipjm = ip * ny + jm; nimjp = im * ny + jp;
   }
-  auto ldx = [&](int k) { return Pair16<TV>::load(rX, (unsigned)k * SV); };
+  auto ldx = [&](int k) { return Pair16<TV>::template load<COH>(rX, (unsigned)k * SV); };
   const V2 x0 = ldx(n), xip = ldx(nip), xim = ldx(nim), xjp = ldx(njp), xjm = ldx(njm),
            xipjm = ldx(nipjm), ximjp = ldx(nimjp);
   TA a[sizeof(TA) == 4 ? 20 : 18];   // the record (FP64: its 18 values, not the pad)
@@ -1077,16 +1179,18 @@ struct Sums2 {
   double s0, s1;
 };
 
-// One SpMV over the member, four cells per thread per trip (the loads of all
-// four in flight together), with the fused dots.  A device function of its
-// own (not inlined) so that its addresses and resources get their own
-// register allocation.
-template <int MODE, typename TA, typename TV>
+// One SpMV over the rows of cells [c.cb, c.ce) of the member, four cells per
+// thread per trip (the loads of all four in flight together), with the fused
+// dots (summed over the member: over the group when the member has one).
+// COH: x is read with device-scope loads (a member group: the halo lines
+// belong to other workgroups).  A device function of its own (not inlined)
+// so that its addresses and resources get their own register allocation.
+template <int MODE, typename TA, typename TV, bool COH = false>
 YG_NOINLINE Sums2 spmv_dots(const Ctx& c, const TA* A, const TV* x, TV* y, const TV* u, TV* w1,
                             TV* w2) {
   PhaseTimer ptm(c, 3);
   using V2 = typename Vec2<TV>::type;
-  const int nc = uni(c.ncell), B = blockDim.x;
+  const int nc = uni(c.ncell), B = blockDim.x, cb = uni(c.cb), ce = uni(c.ce);
   const int nx = uni(c.nx), ny = uni(c.ny), wrap = uni((c.wrapx ? 1 : 0) | (c.wrapy ? 2 : 0));
   const __amdgpu_buffer_rsrc_t rA = yg_rsrc(A, (size_t)YG_NA * nc);
   const __amdgpu_buffer_rsrc_t rX = yg_rsrc(x, (size_t)2 * nc);
@@ -1100,14 +1204,16 @@ YG_NOINLINE Sums2 spmv_dots(const Ctx& c, const TA* A, const TV* x, TV* y, const
   // operations: a cell index past the member reads zeros (matrix record and
   // u out of range) and its stores are dropped, so every load of the trip
   // is issued before the first is waited for.
-  for (int q0 = threadIdx.x; q0 < nc; q0 += 4 * B) {
+  for (int q0 = cb + threadIdx.x; q0 < ce; q0 += 4 * B) {
     double ax[4], ay[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) spmv_cell<TA, TV>(nx, ny, wrap, rA, rX, q0 + k * B, ax[k], ay[k]);
+    for (int k = 0; k < 4; ++k)
+      // a cell past the range reads as the zero record past the member
+      spmv_cell<TA, TV, COH>(nx, ny, wrap, rA, rX, q0 + k * B < ce ? q0 + k * B : nc, ax[k], ay[k]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int q = q0 + k * B;
-      const unsigned off = q < nc ? (unsigned)q * SV : YG_OOB;
+      const unsigned off = q < ce ? (unsigned)q * SV : YG_OOB;
       V2 o;
       o.x = (TV)ax[k];
       o.y = (TV)ay[k];
@@ -1131,7 +1237,7 @@ YG_NOINLINE Sums2 spmv_dots(const Ctx& c, const TA* A, const TV* x, TV* y, const
     }
   }
   double sums[2] = {s0, s1};
-  yg_block_sum<2>(sums, yg_red);
+  csum<2>(c, sums);
   return Sums2{sums[0], sums[1]};
 }
 
@@ -1141,23 +1247,24 @@ struct alignas(16) C16 {
   T v[16 / sizeof(T)];
 };
 
-// Element-wise pass over n values: out[o][k] = f(in[0][k], ..., in[NI-1][k]).
-// The vectors are 16-byte aligned; thread g takes chunks g, g + B, ...,
-// four per trip (two for passes of four or more inputs) with every load of
-// a trip issued before its arithmetic;
-// the tail (fewer values than one chunk) goes value by value.  Arithmetic
-// in double: f(const double (&xi)[NI], double (&xo)[NO]); dots accumulate
-// in f's captures.
+// Element-wise pass over the values [lo, hi): out[o][k] = f(in[0][k], ...,
+// in[NI-1][k]).  The vectors are 16-byte aligned; thread g takes the aligned
+// chunks g, g + B, ... inside the range, four per trip (two for passes of
+// four or more inputs) with every load of a trip issued before its
+// arithmetic; the values before the first and after the last whole chunk go
+// one by one.  Arithmetic in double: f(const double (&xi)[NI], double
+// (&xo)[NO]); dots accumulate in f's captures.
 template <typename T, int NI, int NO, typename F>
-YG_DEV void vmap(int n, const T* const (&in)[NI], T* const (&out)[NO], F&& f) {
+YG_DEV void vmap(int lo, int hi, const T* const (&in)[NI], T* const (&out)[NO], F&& f) {
   constexpr int E = 16 / sizeof(T), U = NI >= 4 ? 2 : 4;
-  const int nch = n / E, B = blockDim.x;
-  for (int k0 = threadIdx.x; k0 < nch; k0 += U * B) {
+  const int B = blockDim.x;
+  const int c0 = (lo + E - 1) / E, c1 = max(c0, hi / E);   // whole chunks [c0, c1)
+  for (int k0 = c0 + threadIdx.x; k0 < c1; k0 += U * B) {
     C16<T> ci[U][NI];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 + u * B;
-      if (k < nch) {
+      if (k < c1) {
 #pragma unroll
         for (int i = 0; i < NI; ++i) ci[u][i] = reinterpret_cast<const C16<T>*>(in[i])[k];
       }
@@ -1165,7 +1272,7 @@ YG_DEV void vmap(int n, const T* const (&in)[NI], T* const (&out)[NO], F&& f) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 + u * B;
-      if (k < nch) {
+      if (k < c1) {
         C16<T> co[NO > 0 ? NO : 1];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -1181,7 +1288,11 @@ YG_DEV void vmap(int n, const T* const (&in)[NI], T* const (&out)[NO], F&& f) {
       }
     }
   }
-  for (int q = nch * E + threadIdx.x; q < n; q += B) {
+  // the values outside the whole chunks: [lo, c0 E) and [c1 E, hi)
+  const int h0 = min(hi, c0 * E), t0 = max(h0, c1 * E);
+  const int nh = h0 - lo, nt = max(0, hi - t0);
+  for (int g = threadIdx.x; g < nh + nt; g += B) {
+    const int q = g < nh ? lo + g : t0 + (g - nh);
     double xi[NI], xo[NO > 0 ? NO : 1];
 #pragma unroll
     for (int i = 0; i < NI; ++i) xi[i] = (double)in[i][q];
@@ -1191,10 +1302,14 @@ YG_DEV void vmap(int n, const T* const (&in)[NI], T* const (&out)[NO], F&& f) {
   }
 }
 
-template <typename TA, typename TV>
+// BiCGSTAB over the rows [c.cb, c.ce) of the member.  GRP: a member group
+// (the rows are this block's share; every sum is a group sum and a group
+// barrier precedes each SpMV whose x other blocks wrote).  GRP requires
+// x = 0 on entry (the refinement's inner solves), where r0 = b needs no SpMV.
+template <typename TA, typename TV, bool GRP = false>
 YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x,
                          TV* W, const YgKrylovOpts& o) {
-  const int n2 = 2 * c.ncell;
+  const int lo = 2 * c.cb, hi = 2 * c.ce;
   const size_t vl = c.vlen;   // vector stride in W (16-byte aligned)
   TV* __restrict__ r = W;
   TV* __restrict__ rt = W + 1 * vl;
@@ -1206,9 +1321,17 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
   TV* __restrict__ tt = W + 7 * vl;
   TV* __restrict__ ytmp = W + 8 * vl;
   SolveOut out{0, 4, 0.0};
-  double red[3];
+  double red[2];
   // r = b - A x, rt = r, (r,r), (b,b)
-  {
+  if (GRP) {
+    red[0] = 0.0;
+    vmap<TV, 1, 2>(lo, hi, {b}, {r, rt}, [&](const double (&xi)[1], double (&xo)[2]) {
+      xo[0] = xo[1] = xi[0];
+      red[0] += xi[0] * xi[0];
+    });
+    csum<1>(c, *reinterpret_cast<double(*)[1]>(red));
+    red[1] = red[0];
+  } else {
     const Sums2 z = spmv_dots<YG_SPMV_RES, TA, TV>(c, A, x, tt, b, r, rt);
     red[0] = z.s0;
     red[1] = z.s1;
@@ -1223,40 +1346,47 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     if (rho == 0.0) { out.status = 2; out.iters = it; return out; }
     // p = r + beta (p - omega v); phat = M^-1 p
     if (it == 1) {
-      vmap<TV, 1, 1>(n2, {r}, {p}, [&](const double (&xi)[1], double (&xo)[1]) { xo[0] = xi[0]; });
+      vmap<TV, 1, 1>(lo, hi, {r}, {p}, [&](const double (&xi)[1], double (&xo)[1]) { xo[0] = xi[0]; });
     } else {
       const double beta = (rho / rho_old) * (alpha / omega);
-      vmap<TV, 3, 1>(n2, {r, p, v}, {p}, [&](const double (&xi)[3], double (&xo)[1]) {
+      vmap<TV, 3, 1>(lo, hi, {r, p, v}, {p}, [&](const double (&xi)[3], double (&xo)[1]) {
         xo[0] = xi[0] + beta * (xi[1] - omega * xi[2]);
       });
     }
     __syncthreads();
     precon_apply<TV>(c, P, p, ytmp, ph);
+    if (GRP) grp_barrier(*c.grp);   // the halo lines of phat
     // v = A phat, (rt, v)
-    red[0] = spmv_dots<YG_SPMV_DOT, TA, TV>(c, A, ph, v, rt, nullptr, nullptr).s0;
+    red[0] = spmv_dots<YG_SPMV_DOT, TA, TV, GRP>(c, A, ph, v, rt, nullptr, nullptr).s0;
     if (red[0] == 0.0) { out.status = 2; out.iters = it; return out; }
     alpha = rho / red[0];
     // s = r - alpha v and ||s||: converged on s ends the iteration early
     red[0] = 0.0;
-    vmap<TV, 2, 1>(n2, {r, v}, {s}, [&](const double (&xi)[2], double (&xo)[1]) {
+    vmap<TV, 2, 1>(lo, hi, {r, v}, {s}, [&](const double (&xi)[2], double (&xo)[1]) {
       const double sq = (double)(TV)(xi[0] - alpha * xi[1]);
       xo[0] = sq;
       red[0] += sq * sq;
     });
-    yg_block_sum<1>(*reinterpret_cast<double(*)[1]>(red), yg_red);
+    if (GRP) {
+      // shat first: the sum of ||s|| then doubles as the barrier before the
+      // SpMV on shat (on the last iteration shat goes unused)
+      __syncthreads();
+      precon_apply<TV>(c, P, s, ytmp, sh);
+    }
+    csum<1>(c, *reinterpret_cast<double(*)[1]>(red));
     const double sn = sqrt(red[0]) * bnrm;
     if (sn <= o.tol) {
-      vmap<TV, 2, 1>(n2, {x, ph}, {x}, [&](const double (&xi)[2], double (&xo)[1]) {
+      vmap<TV, 2, 1>(lo, hi, {x, ph}, {x}, [&](const double (&xi)[2], double (&xo)[1]) {
         xo[0] = xi[0] + alpha * xi[1];
       });
       __syncthreads();
       out.resid = sn; out.iters = it; out.status = 0;
       return out;
     }
-    precon_apply<TV>(c, P, s, ytmp, sh);
+    if (!GRP) precon_apply<TV>(c, P, s, ytmp, sh);
     // t = A shat, (t,t), (t,s)
     {
-      const Sums2 z = spmv_dots<YG_SPMV_DOT2, TA, TV>(c, A, sh, tt, s, nullptr, nullptr);
+      const Sums2 z = spmv_dots<YG_SPMV_DOT2, TA, TV, GRP>(c, A, sh, tt, s, nullptr, nullptr);
       red[0] = z.s0;
       red[1] = z.s1;
     }
@@ -1265,7 +1395,7 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     if (omega == 0.0) { out.status = 2; out.iters = it; return out; }
     // x += alpha phat + omega shat; r = s - omega t; (r,r), (rt,r)
     red[0] = red[1] = 0.0;
-    vmap<TV, 6, 2>(n2, {x, ph, sh, s, tt, rt}, {x, r},
+    vmap<TV, 6, 2>(lo, hi, {x, ph, sh, s, tt, rt}, {x, r},
                    [&](const double (&xi)[6], double (&xo)[2]) {
       xo[0] = xi[0] + alpha * xi[1] + omega * xi[2];
       const double rq = (double)(TV)(xi[3] - omega * xi[4]);
@@ -1273,7 +1403,7 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
       red[0] += rq * rq;
       red[1] += xi[5] * rq;
     });
-    yg_block_sum<2>(*reinterpret_cast<double(*)[2]>(red), yg_red);
+    csum<2>(c, red);
     out.resid = sqrt(red[0]) * bnrm;
     out.iters = it;
     if (out.resid <= o.tol) { out.status = 0; return out; }
@@ -1383,6 +1513,11 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   c.wrapx = a.wrapx; c.wrapy = a.wrapy;
   c.mask = a.mask + mb.coff;
   c.maxcnt = mb.maxcnt;
+  c.fu = c.fu16 = 8ll * ncell;
+  c.cb = 0;
+  c.ce = ncell;
+  c.grp = nullptr;
+  c.pc = &c;
   {
     // line-to-wave mapping of the recurrences (exact or block ILU(0))
     const int g = threadIdx.x, lane = g & 63, w = g >> 6;
@@ -1535,4 +1670,195 @@ void yg_launch_krylov_nt(const YgKrylovArgs& a, int mode, int nmembers, int bloc
   else
     hipLaunchKernelGGL((yg_krylov_kernel<YG_MODE_MIXED, NT>), dim3(nmembers), dim3(block),
                        YG_DYN_LDS, st, a);
+}
+
+// ---------------------------------------------------------------------------
+// The member-group kernel (mixed mode, block-Jacobi ILU(0), BiCGSTAB): G
+// workgroups share each member.  Workgroup b is rank (b/8) % G of group
+// (b/8/G)*8 + b%8, so the workgroups of a group run on one XCD (the groups
+// of a launch spread evenly over the 8); group gi takes the members gi,
+// gi + ngroups, ...  Rank r owns the line blocks [r nbr, (r+1) nbr) of its
+// member (nbr = ceil(line blocks / G)): it factorizes them and runs their
+// sweeps, and its SpMV rows and vector passes are the cells of their lines.
+// Line blocks do not couple in block-Jacobi ILU(0), so the preconditioner
+// needs no exchange; the SpMV reads the neighbouring ranks' edge lines after
+// a group barrier.  The iteration is the one-block kernel's mixed mode
+// (FP64 refinement around FP32 BiCGSTAB) with every sum a group sum.
+template <int NT>
+__global__ __launch_bounds__(NT) void yg_krylov_group_kernel(YgKrylovArgs a) {
+  const int G = a.group;
+  const int blk = blockIdx.x, slot = blk >> 3;
+  const int gi = (slot / G) * 8 + (blk & 7), rank = slot % G;
+  const int ngroups = gridDim.x / G;
+  unsigned char* gs = reinterpret_cast<unsigned char*>(a.gsync) + (size_t)gi * YG_GSYNC_BYTES;
+  Grp grp;
+  grp.cnt = reinterpret_cast<int*>(gs);
+  grp.err = reinterpret_cast<int*>(gs) + 1;
+  grp.part = reinterpret_cast<double*>(gs + 64);   // 2 x G x 4 doubles
+  grp.G = G;
+  grp.rank = rank;
+  __shared__ int s_bad;
+  __shared__ long long s_tm[8];
+  if (threadIdx.x == 0) { yg_gbar = 0; yg_gfail = 0; }
+  __syncthreads();
+  for (int m = gi; m < a.nmembers; m += ngroups) {
+    if (a.skip && a.skip[m]) continue;
+    const YgMember mb = a.members[m];
+    const int nx = mb.nx, ny = mb.ny, ncell = nx * ny;
+    // the member's line blocks (at most 64 lines each, at most one per wave
+    // of the group) and this rank's lines [i0, i1)
+    int nb = mb.nb != 0 ? mb.nb : a.opts.ilu_blocks;
+    if (nb <= 0) nb = (nx + 63) / 64;
+    nb = ((max(nb, 1) + G - 1) / G) * G;   // the same number of blocks on every rank
+    nb = min(max(nb, (nx + 63) / 64), min(G * (NT / 64), nx));
+    const int lpw = (nx + nb - 1) / nb;
+    const int nwl = (nx + lpw - 1) / lpw;
+    const int nbr = (nwl + G - 1) / G;
+    const int i0 = min(nx, rank * nbr * lpw), i1 = min(nx, (rank + 1) * nbr * lpw);
+    const int Lg = i1 - i0;
+    const int T = Lg > 0 ? ny + 2 * (Lg - 1) : 0;
+    __syncthreads();   // the previous member's step table is no longer read
+    for (int k = threadIdx.x; k <= T; k += blockDim.x) yg_soff[k] = blk_soff(k, Lg, ny);
+    if (threadIdx.x == 0) s_bad = 0;
+    if (threadIdx.x < 8) s_tm[threadIdx.x] = 0;
+    __syncthreads();
+    const long long t_kernel0 = wall_clock64();
+    Ctx cm;   // the member: SpMV rows, vector passes and sums of this rank
+    cm.mb = &a.members[m];
+    cm.nx = nx; cm.ny = ny; cm.T = mb.T; cm.ncell = ncell; cm.vlen = mb.vlen;
+    cm.wrapx = a.wrapx; cm.wrapy = a.wrapy;
+    cm.mask = a.mask + mb.coff;
+    cm.maxcnt = mb.maxcnt;
+    cm.line = -1; cm.coupled = 0; cm.nwl = 0; cm.lpw = lpw;
+    cm.tm = a.timers ? s_tm : nullptr;
+    cm.fu = cm.fu16 = 8ll * ncell;
+    cm.cb = i0 * ny; cm.ce = i1 * ny;
+    cm.grp = &grp;
+    Ctx cr = cm;   // this rank's lines: factorization and sweeps
+    cr.nx = Lg; cr.T = T; cr.ncell = Lg * ny;
+    cr.mask = cm.mask + cm.cb;
+    cr.maxcnt = min(Lg, (ny + 1) / 2);
+    cr.nwl = (Lg + lpw - 1) / lpw;
+    {
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = w * lpw + lane;
+      cr.line = (w < cr.nwl && lane < lpw && i < Lg) ? i : -1;
+    }
+    cr.fu = 8ll * ncell + 4ll * cm.cb;     // the member's U part from this part's L part
+    cr.fu16 = 8ll * ncell + 8ll * cm.cb;
+    cr.cb = 0; cr.ce = cr.ncell;
+    cr.grp = nullptr;
+    cr.pc = &cr;
+    cm.pc = &cr;
+    const size_t c2 = 2 * (size_t)mb.coff;
+    const double* b64 = a.b + c2;
+    double* x64 = a.x + c2;
+    const int lo = 2 * cm.cb, hi = 2 * cm.ce;
+    // b == 0 shortcut (solver_linear.F90:162-174)
+    {
+      double bm[1] = {0.0};
+      for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) bm[0] += b64[q] != 0.0 ? 1.0 : 0.0;
+      csum<1>(cm, bm);
+      if (bm[0] == 0.0) {
+        for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) x64[q] = 0.0;
+        if (rank == 0 && threadIdx.x == 0) {
+          a.out_iters[m] = 0; a.out_status[m] = 0; a.out_resid[m] = 0.0;
+        }
+        continue;
+      }
+    }
+    const YgKrylovOpts o = a.opts;
+    const double* A64 = a.A64 + (size_t)YG_NA * mb.coff;
+    const float* A32 = a.A32 + (size_t)YG_NA * mb.coff;
+    float* F = a.F32 + (size_t)YG_NF * mb.coff;
+    float* W = a.W32 + (size_t)YG_NW32 * mb.woff;   // 9 inner Krylov vectors ...
+    float* d = W + (size_t)9 * mb.vlen;              // ... the correction ...
+    float* rin = W + (size_t)10 * mb.vlen;           // ... and the scaled residual
+    double* r64 = a.W64 + (size_t)YG_NW64 * mb.woff; // FP64 residual ...
+    double* y64 = r64 + mb.vlen;                      // ... and A x
+    if (o.initx_zeros) {
+      for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) x64[q] = 0.0;
+    }
+    Precon<float> P;
+    P.kind = 2;
+    P.F = F + (size_t)8 * cm.cb;   // this part's L factors (block-skewed order)
+    if (Lg > 0) ilu0_factor<float>(cr, A32 + (size_t)YG_NA * cm.cb, F + (size_t)8 * cm.cb, &s_bad);
+    if (o.factor16 && a.F16) {
+      uint16_t* FB = a.F16 + (size_t)24 * mb.coff + (size_t)8 * cm.cb;
+      if (Lg > 0) factors_to_bf16(cr, P.F, FB);
+      P.FB = FB;
+    }
+    double bad[1] = {(double)s_bad};
+    csum<1>(cm, bad);   // a zero pivot on any rank
+    YgKrylovOpts oi = o;
+    oi.tol = o.inner_tol;
+    oi.conv_cond = 0;
+    oi.initx_zeros = 1;
+    double nrm0 = -1.0, res = 1.0;
+    int iters = 0, status = 4;
+    const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
+    for (int ir = 0; ir < o.ir_max; ++ir) {
+      grp_barrier(grp);   // x's edge lines are current on every rank
+      double red[2];
+      {
+        const Sums2 z = spmv_dots<YG_SPMV_RES, double, double, true>(cm, A64, x64, y64, b64, r64, nullptr);
+        red[0] = z.s0;
+        red[1] = z.s1;
+      }
+      const double rn = sqrt(red[0]);
+      if (nrm0 < 0.0) nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : rn;
+      if (nrm0 == 0.0) { res = 0.0; status = 0; break; }
+      res = rn / nrm0;
+      status = res <= o.tol ? 0 : 4;
+      if (res <= target) break;
+      if (iters >= o.maxiter) break;
+      const double sc = rn > 0.0 ? 1.0 / rn : 1.0;
+      for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) {
+        rin[q] = (float)(r64[q] * sc);
+        d[q] = 0.0f;
+      }
+      __syncthreads();
+      oi.maxiter = o.maxiter - iters;
+      SolveOut so = bicgstab<float, float, true>(cm, A32, P, rin, d, W, oi);
+      iters += so.iters;
+      for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) x64[q] += rn * (double)d[q];
+      __syncthreads();
+      if (so.status == 2 && so.iters <= 1) { status = 2; break; }
+    }
+    if (bad[0] != 0.0 && status == 0) status = 2;
+    if (yg_gfail) {
+      status = 2;
+      if (threadIdx.x == 0) __hip_atomic_store(a.group_fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (rank == 0 && threadIdx.x == 0) {
+      a.out_iters[m] = iters; a.out_status[m] = status; a.out_resid[m] = res;
+      if (a.timers) {
+        s_tm[7] = wall_clock64() - t_kernel0;
+        for (int k = 0; k < 8; ++k) a.timers[8 * m + k] += s_tm[k];
+      }
+    }
+  }
+}
+
+namespace {
+__global__ void k_clear_gsync(int* g, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) g[k] = 0;
+}
+}  // namespace
+
+template <int NT>
+void yg_krylov_group_attrs_nt() {
+  static bool done = false;
+  if (done) return;
+  (void)hipFuncSetAttribute((const void*)yg_krylov_group_kernel<NT>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
+  done = true;
+}
+
+template <int NT>
+void yg_launch_krylov_group_nt(const YgKrylovArgs& a, int ngroups, hipStream_t st) {
+  yg_krylov_group_attrs_nt<NT>();
+  const int nwords = ngroups * (YG_GSYNC_BYTES / 4);
+  hipLaunchKernelGGL(k_clear_gsync, dim3((nwords + 255) / 256), dim3(256), 0, st, a.gsync, nwords);
+  hipLaunchKernelGGL(yg_krylov_group_kernel<NT>, dim3(ngroups * a.group), dim3(NT), YG_DYN_LDS, st, a);
 }

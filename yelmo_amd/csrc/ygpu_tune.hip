@@ -4,11 +4,15 @@
 // ILU(0) line blocks (one wave sweeps each block; more, shorter blocks mean
 // shorter dependency chains per wave but a weaker block-Jacobi
 // preconditioner and so more Krylov iterations) and the threads per member
-// block (512 or 1024, which bounds the block count).  ygpu_autotune times
-// the first Picard iteration's linear solve of the caller's own domain under
-// each candidate schedule on the device and records the fastest in the
-// handle for that grid size; batches, sessions and servers created on the
-// handle afterwards give every member of that size its tuned schedule.
+// block (512 or 1024, which bounds the block count).  With G workgroups per
+// member (member groups), the blocks are split over the G workgroups (a
+// multiple of G of them, 512 threads each).  ygpu_autotune times the first
+// Picard iteration's linear solve of the caller's own domain under each
+// candidate schedule on the device, for one workgroup per member and for
+// each group size a batch of such domains may choose, and records the
+// fastest per (grid size, G) in the handle; batches, sessions and servers
+// created on the handle afterwards give every member of that size the
+// schedule of the group size they run with.
 #include "ygpu_host.h"
 
 using namespace yg;
@@ -21,9 +25,17 @@ struct Candidate {
   int iters = 0, status = -1;
 };
 
-std::vector<Candidate> candidates(int nx) {
+std::vector<Candidate> candidates(int nx, int G) {
   std::vector<Candidate> out;
   const int nb0 = (nx + 63) / 64;
+  if (G > 1) {
+    // multiples of G, at least one block per 64 lines, at most one per wave
+    const int nbmax = std::min(G * (YG_GROUP_THREADS / 64), nx);
+    const int first = ((nb0 + G - 1) / G) * G;
+    for (int nb = first, k = 0; nb <= nbmax && k < 4; nb += G, ++k)
+      out.push_back(Candidate{nb, YG_GROUP_THREADS});
+    return out;
+  }
   for (int block : {512, 1024}) {
     if (block < nb0 * 64) continue;
     const int nbmax = std::min(block / 64, nx);
@@ -55,12 +67,12 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
   if ((rc = B.upload_inputs(prob))) return rc;
   if ((rc = B.upload_state(prob))) return rc;
   hipStream_t st = h->stream;
-  const YgBatchGeom G = B.geom();
+  const YgBatchGeom Gm = B.geom();
   YG_CHECK(h, hipMemsetAsync(B.d_done, 0, sizeof(int), st));
   // the fields of the first Picard iteration (velocity_ssa.f90:172-262)
-  yg_launch_init<double>(G, B.f, B.fp, nullptr, st);
-  if (B.diva) yg_launch_diva_pre<double>(G, B.f, B.dv, B.fp, B.d_done, st);
-  yg_launch_fields_pre<double>(G, B.f, B.fp, B.d_done, st);
+  yg_launch_init<double>(Gm, B.f, B.fp, nullptr, st);
+  if (B.diva) yg_launch_diva_pre<double>(Gm, B.f, B.dv, B.fp, B.d_done, st);
+  yg_launch_fields_pre<double>(Gm, B.f, B.fp, B.d_done, st);
   YG_CHECK(h, hipGetLastError());
   YgKrylovArgs ka{};
   ka.members = B.d_mem; ka.soff = B.d_soff; ka.mask = B.d_mask;
@@ -72,46 +84,77 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
               B.fp.BC == YG_BC_MISMIP3D || B.fp.BC == YG_BC_TROUGH);
   ka.skip = B.d_done; ka.out_iters = B.d_kit; ka.out_status = B.d_kst; ka.out_resid = B.d_kres;
   ka.opts = B.opts.k;
+  // synchronization words of 8 groups (one member: the first group works,
+  // the group count must be a multiple of 8)
+  Arena tmp;
+  int ncu = 0;
+  YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+  YG_CHECK(h, tmp.alloc(&ka.gsync, (size_t)8 * (YG_GSYNC_BYTES / 4)));
+  YG_CHECK(h, tmp.alloc(&ka.group_fail, 1));
+  YG_CHECK(h, hipMemsetAsync(ka.group_fail, 0, sizeof(int), st));
+  ka.nmembers = 1;
   hipEvent_t e0, e1;
   YG_CHECK(h, hipEventCreate(&e0));
   YG_CHECK(h, hipEventCreate(&e1));
-  std::vector<Candidate> cand = candidates(prob->nx);
-  bool warm = false;
-  for (Candidate& c : cand) {
-    B.mem[0].nb = c.nb;
-    YG_CHECK(h, hipMemcpyAsync(B.d_mem, B.mem.data(), sizeof(YgMember), hipMemcpyHostToDevice, st));
-    double best = 1e300;
-    for (int r = 0; r < reps + (warm ? 0 : 1); ++r) {
-      // the same system every time: assembly rewrites the matrix, b and x0
-      yg_launch_assemble<double>(G, B.f, B.fp, B.d_done, B.d_A64, B.d_A32, B.d_b, B.d_x,
-                                 B.d_mask, st);
-      YG_CHECK(h, hipEventRecord(e0, st));
-      yg_launch_krylov(ka, YG_MODE_MIXED, 1, c.block, st);
-      YG_CHECK(h, hipEventRecord(e1, st));
-      YG_CHECK(h, hipGetLastError());
-      YG_CHECK(h, hipEventSynchronize(e1));
-      float ms = 0.f;
-      YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
-      if (warm) best = std::min(best, (double)ms);
-      warm = true;
+  const long long cells = (long long)prob->nx * prob->ny;
+  const bool group_ok = B.opts.k.solver == 4 && B.opts.k.ilu_blocks != 0;
+  const Candidate* win1 = nullptr;
+  std::vector<Candidate> all;
+  for (int G : {1, 2, 4, 8}) {
+    // group sizes a batch may choose for this domain (pick_group's rule)
+    if (G > 1 && (!group_ok || cells / G < 4096 || 8 * G > ncu ||
+                  prob->nx > 64 * G * (YG_GROUP_THREADS / 64)))
+      continue;
+    std::vector<Candidate> cand = candidates(prob->nx, G);
+    ka.group = G;
+    bool warm = false;
+    for (Candidate& c : cand) {
+      B.mem[0].nb = c.nb;
+      YG_CHECK(h, hipMemcpyAsync(B.d_mem, B.mem.data(), sizeof(YgMember), hipMemcpyHostToDevice, st));
+      double best = 1e300;
+      for (int r = 0; r < reps + (warm ? 0 : 1); ++r) {
+        // the same system every time: assembly rewrites the matrix, b and x0
+        yg_launch_assemble<double>(Gm, B.f, B.fp, B.d_done, B.d_A64, B.d_A32,
+                                   B.d_b, B.d_x, B.d_mask, st);
+        YG_CHECK(h, hipEventRecord(e0, st));
+        if (G > 1) yg_launch_krylov_group(ka, 8, st);
+        else yg_launch_krylov(ka, YG_MODE_MIXED, 1, c.block, st);
+        YG_CHECK(h, hipEventRecord(e1, st));
+        YG_CHECK(h, hipGetLastError());
+        YG_CHECK(h, hipEventSynchronize(e1));
+        float ms = 0.f;
+        YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
+        if (warm) best = std::min(best, (double)ms);
+        warm = true;
+      }
+      YG_CHECK(h, hipMemcpy(&c.iters, B.d_kit, sizeof(int), hipMemcpyDeviceToHost));
+      YG_CHECK(h, hipMemcpy(&c.status, B.d_kst, sizeof(int), hipMemcpyDeviceToHost));
+      c.ms = best;
     }
-    YG_CHECK(h, hipMemcpy(&c.iters, B.d_kit, sizeof(int), hipMemcpyDeviceToHost));
-    YG_CHECK(h, hipMemcpy(&c.status, B.d_kst, sizeof(int), hipMemcpyDeviceToHost));
-    c.ms = best;
+    int gf = 0;
+    YG_CHECK(h, hipMemcpy(&gf, ka.group_fail, sizeof(int), hipMemcpyDeviceToHost));
+    if (gf) {
+      h->err = "autotune: a member group's workgroups were not resident together";
+      return YGPU_ERR_DEVICE;
+    }
+    const Candidate* win = nullptr;
+    for (const Candidate& c : cand)
+      if (c.status == 0 && (!win || c.ms < win->ms)) win = &c;
+    if (win) {
+      h->sched[{prob->nx, prob->ny, G}] = {win->nb, win->block};
+      if (G == 1) all.push_back(*win);
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  const Candidate* win = nullptr;
-  for (const Candidate& c : cand)
-    if (c.status == 0 && (!win || c.ms < win->ms)) win = &c;
-  if (!win) {
+  if (!all.empty()) win1 = &all[0];
+  if (!win1) {
     h->err = "autotune: no schedule converged on this domain";
     return YGPU_ERR_UNSUPPORTED;
   }
-  h->sched[{prob->nx, prob->ny}] = {win->nb, win->block};
-  if (nb_out) *nb_out = win->nb;
-  if (block_out) *block_out = win->block;
-  if (ms_out) *ms_out = win->ms;
+  if (nb_out) *nb_out = win1->nb;
+  if (block_out) *block_out = win1->block;
+  if (ms_out) *ms_out = win1->ms;
   return YGPU_OK;
 }
 
@@ -128,14 +171,15 @@ int ygpu_autotune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_pro
   return tune(h, par, prob, diva ? 1 : 0, reps < 1 ? 1 : reps, ilu_blocks, block, solve_ms);
 }
 
+// One schedule for every group size (the member-group kernel rounds the
+// block count up to a multiple of its group size).
 int ygpu_set_schedule(ygpu_handle* h, int32_t nx, int32_t ny, int32_t ilu_blocks, int32_t block) {
   if (!h || nx < 3 || ny < 3) return YGPU_ERR_ARG;
-  if (ilu_blocks <= 0) {
-    h->sched.erase({nx, ny});
-    return YGPU_OK;
-  }
   if (block != 0 && block != 512 && block != 1024) return YGPU_ERR_ARG;
-  h->sched[{nx, ny}] = {ilu_blocks, block};
+  for (int G : {1, 2, 4, 8}) {
+    if (ilu_blocks <= 0) h->sched.erase({nx, ny, G});
+    else h->sched[{nx, ny, G}] = {ilu_blocks, G > 1 ? YG_GROUP_THREADS : block};
+  }
   return YGPU_OK;
 }
 
