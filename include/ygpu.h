@@ -78,6 +78,8 @@ typedef struct ygpu_ssa_params {
   int32_t block;          /* threads per member block (0: pick automatically)   */
   char ssa_lis_opt[256];  /* LIS option string, e.g. the energy default
                              "-i cg -p jacobi -maxiter 200 -tol 1.0e-2 -initx_zeros false";
+                             -i cg | bicgstab; minres and bicgsafe are accepted
+                             and solved with bicgstab to the same tolerance;
                              extensions: -ir_inner_tol T, -ir_max N              */
 } ygpu_ssa_params;
 

@@ -149,6 +149,10 @@ static int parse_lis_opts(const char *s, lis_opts *o)
         if (!strcmp(tok, "-i")) {
             if (!strcmp(val, "cg") || !strcmp(val, "1")) o->solver = 1;
             else if (!strcmp(val, "bicgstab") || !strcmp(val, "4")) o->solver = 4;
+            /* LIS's MINRES and BiCGSafe (named by the reference's namelists,
+             * input/yelmo_defaults.nml:160) solve the same system here with
+             * BiCGSTAB, as the library does */
+            else if (!strcmp(val, "minres") || !strcmp(val, "bicgsafe")) o->solver = 4;
             else return -2;
         } else if (!strcmp(tok, "-p")) {
             if (!strcmp(val, "none") || !strcmp(val, "0")) o->precon = 0;

@@ -23,6 +23,13 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     if (k == "-i") {
       if (v == "cg" || v == "1") o.k.solver = 1;
       else if (v == "bicgstab" || v == "4") o.k.solver = 4;
+      // The reference's namelists also name LIS's MINRES and BiCGSafe
+      // (residual-formulation defaults, input/yelmo_defaults.nml:160,
+      // par/*.nml).  The residual matrix is nonsymmetric with a diagonal of
+      // both signs, where MINRES's SPD-preconditioner recurrences do not
+      // apply; both solve the same system to the same tolerance with
+      // BiCGSTAB here.
+      else if (v == "minres" || v == "bicgsafe") o.k.solver = 4;
       else { err = "unsupported Krylov solver: " + v; return YGPU_ERR_UNSUPPORTED; }
     } else if (k == "-p") {
       if (v == "none" || v == "0") o.k.precon = 0;

@@ -264,22 +264,23 @@ template <> YG_DEV uint32_t stage_to_mask<float>(float v) { return __float_as_ui
 template <> YG_DEV uint32_t stage_to_mask<double>(double v) { return (uint32_t)v; }
 
 // Neighbour lanes exchange values by DPP wave shifts (a VALU operation).
+// bound_ctrl set: the lane without a source lane reads 0.
 YG_DEV float dpp_shr1(float v) {   // lane i <- lane i-1, lane 0 <- 0
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 YG_DEV float dpp_shl1(float v) {   // lane i <- lane i+1, lane 63 <- 0
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 YG_DEV double dpp_shr1(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 YG_DEV double dpp_shl1(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x130, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -777,6 +778,10 @@ YG_DEV float bf_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000
 // Geometry of one wave's line block in the streamed sweeps, all scalars.
 // Per step t the lanes with cells are glo(t) <= lane <= min(L-1, t/2); a
 // lane's slot in the block-skewed order is blk_soff(t) - glo(t) + lane.
+// A lane without a cell at a step loads from an out-of-range offset, which
+// reads zero factors and a zero right-hand side, so its result is an exact
+// zero (the value its neighbours must see for a cell that does not exist)
+// without a select; its store goes out of range and is dropped.
 struct BlockGeom {
   int L, Tb, ny;
 };
@@ -858,13 +863,10 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   for (int t0 = 0; t0 < G.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      const bool h = lane >= qc.lo && lane <= qc.hi;
       const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
       T yx = r[s].x - f[s][0] * n2x - f[s][1] * o1x - f[s][2] * o1y;
       T yy = r[s].y - f[s][3] * n2x - f[s][4] * n2y - f[s][5] * n1x - f[s][6] * o1y -
              f[s][7] * yx;
-      yx = h ? yx : (T)0;
-      yy = h ? yy : (T)0;
       V2 o;
       o.x = yx;
       o.y = yy;
@@ -914,14 +916,11 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   for (int u0 = 0; u0 < G.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      const bool h = lane >= qc.lo && lane <= qc.hi;
       // line i+1: cell j-1 (one step back) and cell j (two steps back)
       const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
       T zy = (yv[s].y - f[s][6] * o1x - f[s][7] * o1y - f[s][8] * n2y) * f[s][9];
       T zx = (yv[s].x - f[s][0] * zy - f[s][1] * o1x - f[s][2] * n1y - f[s][3] * n2x -
               f[s][4] * n2y) * f[s][5];
-      zx = h ? zx : (T)0;
-      zy = h ? zy : (T)0;
       V2 o;
       o.x = zx;
       o.y = zy;
@@ -971,14 +970,11 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   for (int t0 = 0; t0 < G.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      const bool h = lane >= qc.lo && lane <= qc.hi;
       const T f0 = bf_lo(fw[s].x), f1 = bf_hi(fw[s].x), f2 = bf_lo(fw[s].y), f3 = bf_hi(fw[s].y);
       const T f4 = bf_lo(fw[s].z), f5 = bf_hi(fw[s].z), f6 = bf_lo(fw[s].w), f7 = bf_hi(fw[s].w);
       const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
       T yx = r[s].x - f0 * n2x - f1 * o1x - f2 * o1y;
       T yy = r[s].y - f3 * n2x - f4 * n2y - f5 * n1x - f6 * o1y - f7 * yx;
-      yx = h ? yx : (T)0;
-      yy = h ? yy : (T)0;
       V2 o;
       o.x = yx;
       o.y = yy;
@@ -1028,7 +1024,6 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   for (int u0 = 0; u0 < G.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      const bool h = lane >= qc.lo && lane <= qc.hi;
       // 0..4 scaled ux-row U entries (slots 5,3,7,1,6), 5..7 scaled uy-row
       // entries (slots 6,1,3)
       const T g0 = bf_lo(gw[s].x), g1 = bf_hi(gw[s].x), g2 = bf_lo(gw[s].y), g3 = bf_hi(gw[s].y);
@@ -1036,8 +1031,6 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
       const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
       T zy = dg[s].y * yv[s].y - g5 * o1x - g6 * o1y - g7 * n2y;
       T zx = dg[s].x * yv[s].x - g0 * zy - g1 * o1x - g2 * n1y - g3 * n2x - g4 * n2y;
-      zx = h ? zx : (T)0;
-      zy = h ? zy : (T)0;
       V2 o;
       o.x = zx;
       o.y = zy;
