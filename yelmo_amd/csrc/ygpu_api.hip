@@ -377,7 +377,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
     for (size_t q = 0; q < A.size(); ++q) A32[q] = (float)A[q];
     YG_CHECK(h, ar.alloc(&d_A32, A32.size())); YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * cells));
     YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * M.vlen));
-    if (o.k.factor16) YG_CHECK(h, ar.alloc(&ka.F16, (size_t)24 * cells));
+    if (o.k.factor16) YG_CHECK(h, ar.alloc(&ka.F16, (size_t)YG_NF16 * cells));
     YG_CHECK(h, hipMemcpyAsync(d_A32, A32.data(), sizeof(float) * A32.size(), hipMemcpyHostToDevice, st));
   }
   ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;

@@ -290,7 +290,7 @@ struct Batch {
       YG_CHECK(h, ar.alloc(&d_A32, (size_t)YG_NA * C));
       YG_CHECK(h, ar.alloc(&d_F32, (size_t)YG_NF * C));
       YG_CHECK(h, ar.alloc(&d_W32, (size_t)YG_NW32 * total_v));
-      if (opts.k.factor16) YG_CHECK(h, ar.alloc(&d_F16, (size_t)24 * C));
+      if (opts.k.factor16) YG_CHECK(h, ar.alloc(&d_F16, (size_t)YG_NF16 * C));
     } else {
       YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * C));
     }

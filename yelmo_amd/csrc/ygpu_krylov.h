@@ -14,6 +14,11 @@
 // kernel's dynamic LDS.
 #define YG_MAX_STEP_CELLS 400
 
+// BF16 line-block factors, 16-bit values allocated per cell: 20 are used (L
+// 8, scaled U 8, the two FP32 inverse diagonals 4); 24 keep every member's
+// factors 16-byte aligned
+#define YG_NF16 24
+
 // Krylov workspace vectors per member (2 values per cell each)
 #define YG_NW64 9
 #define YG_NW32 11
@@ -44,7 +49,7 @@ struct YgKrylovArgs {
   const float* A32;
   double* F64;              // ILU factors (skewed wavefront order) / jacobi 1/diag
   float* F32;
-  uint16_t* F16;            // mixed mode, BF16 factors: 48 bytes per cell (or null)
+  uint16_t* F16;            // mixed mode, BF16 factors: YG_NF16 values per cell (or null)
   const double* b;          // 2 per cell, the reference's row order 2n, 2n+1
   double* x;                // 2 per cell, x0 in, solution out
   double* W64;              // YG_NW64 * 2 per cell

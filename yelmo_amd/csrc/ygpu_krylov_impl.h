@@ -52,11 +52,12 @@ struct Ctx {
   int nwl;                    // waves that own lines
   int lpw;                    // lines per wave (the line block of a wave)
   long long* tm;              // phase timers (thread 0), may be null
-  // Offset (values) from a factor pointer to the U part of its line-block
-  // factors: 8 ncell for a whole member (FP32/FP64 and BF16 alike); a part of
-  // a member (the lines of one block of a member group) addresses the
-  // member's U part from its own L part.
-  long long fu, fu16;
+  // Offsets (values) from a factor pointer to the U part of its line-block
+  // factors: 8 ncell for a whole member (FP32/FP64 and BF16 alike), and, for
+  // BF16 factors, to the inverse diagonals (16 ncell); a part of a member
+  // (the lines of one block of a member group) addresses the member's U
+  // part and diagonals from its own L part.
+  long long fu, fu16, fd16;
   // The cells [cb, ce) of the member whose rows this block computes in the
   // SpMV and vector passes (the whole member, or this block's share in a
   // member group), and the group (null for one block per member).
@@ -750,13 +751,15 @@ YG_DEV void load_vals(__amdgpu_buffer_rsrc_t r, unsigned off, double (&v)[N]) {
 }
 
 // BF16 storage of the line-block factors (mixed mode, option factor16).
-// Per cell in the same block-skewed order as the FP32 factors:
-//   L part: 8 BF16 values (F0..F7), one 16-byte chunk, at FB + 16 q bytes;
+// Three arrays over the cells, in the same block-skewed order as the FP32
+// factors, so that each sweep load is one contiguous range across the lanes:
+//   L part: 8 BF16 values (F0..F7), 16 bytes at FB + 16 q bytes;
 //   U part: 8 BF16 values of the off-diagonal U entries already multiplied by
 //     their row's inverse diagonal (ux row: slots 5,3,7,1,6; uy row: 6,1,3),
-//     then the two inverse diagonals in FP32 and 8 pad bytes: 32 bytes at
-//     FB + 16 ncell + 32 q bytes.  The inverse diagonal is the row scale, so
-//     a U row is x = inv_d y - sum (inv_d U) x_j.
+//     16 bytes at FB + 16 ncell + 16 q bytes;
+//   the two inverse diagonals in FP32, 8 bytes at FB + 32 ncell + 8 q bytes.
+// The inverse diagonal is the row scale, so a U row is
+// x = inv_d y - sum (inv_d U) x_j.
 YG_DEV uint16_t to_bf16(float v) {
   const uint32_t b = __builtin_bit_cast(uint32_t, v);
   return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);   // nearest, ties to even
@@ -993,7 +996,7 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ ys, float* __restrict__ z) {
   PhaseTimer ptm(c, 2);
   constexpr int D = 20;   // 8 registers per step: 4 packed words, the 2 inverse diagonals, y
-  constexpr unsigned SF = 32, SV = 8;
+  constexpr unsigned SF = 16, SD = 8, SV = 8;
   using T = float;
   using V2 = typename Vec2<T>::type;
   const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
@@ -1002,7 +1005,8 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   if (w >= nwl) return;
   const int i0 = w * lpw, L = min(lpw, nx - i0);
   const BlockGeom G{L, ny + 2 * (L - 1), ny};
-  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + c.fu16, (size_t)16 * nc);
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + c.fu16, (size_t)8 * nc);
+  const __amdgpu_buffer_rsrc_t rD = yg_rsrc(FB + c.fd16, (size_t)4 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
   const int lane_q = i0 * ny + lane;
@@ -1013,9 +1017,8 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   StepQ qc = step_q(G, G.Tb - 1), qn = step_q(G, G.Tb - 1);
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    const unsigned off = cell_off(qn, lane, lane_q, SF);
-    gw[s] = load_raw16(rF, off);
-    dg[s] = Pair16<T>::load(rF, off + 16);
+    gw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
+    dg[s] = Pair16<T>::load(rD, cell_off(qn, lane, lane_q, SD));
     yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
     step_prev(G, qn);
     __builtin_amdgcn_sched_barrier(0);
@@ -1037,9 +1040,8 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
       Pair16<T>::store(o, rZ, nat_off(qc, lane, lane_n, SV));
       o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
       step_prev(G, qc);
-      const unsigned off = cell_off(qn, lane, lane_q, SF);
-      gw[s] = load_raw16(rF, off);
-      dg[s] = Pair16<T>::load(rF, off + 16);
+      gw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
+      dg[s] = Pair16<T>::load(rD, cell_off(qn, lane, lane_q, SD));
       yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
       step_prev(G, qn);
       __builtin_amdgcn_sched_barrier(0);
@@ -1052,6 +1054,7 @@ YG_DEV void factors_to_bf16(const Ctx& c, const float* __restrict__ F, uint16_t*
   const int nc = c.ncell;
   const float* FU = F + c.fu;
   uint16_t* BU = FB + c.fu16;
+  uint16_t* BD = FB + c.fd16;
   for (int q = threadIdx.x; q < nc; q += blockDim.x) {
     uint32_t lw[4], uw[4];
 #pragma unroll
@@ -1065,9 +1068,8 @@ YG_DEV void factors_to_bf16(const Ctx& c, const float* __restrict__ F, uint16_t*
       uw[e] = (uint32_t)to_bf16(g[2 * e]) | ((uint32_t)to_bf16(g[2 * e + 1]) << 16);
     uint4* dl = reinterpret_cast<uint4*>(FB + 8 * q);
     *dl = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-    uint4* du = reinterpret_cast<uint4*>(BU + 16 * q);
-    du[0] = make_uint4(uw[0], uw[1], uw[2], uw[3]);
-    du[1] = make_uint4(__builtin_bit_cast(uint32_t, u[5]), __builtin_bit_cast(uint32_t, u[9]), 0u, 0u);
+    *reinterpret_cast<uint4*>(BU + 8 * q) = make_uint4(uw[0], uw[1], uw[2], uw[3]);
+    *reinterpret_cast<float2*>(BD + 4 * q) = make_float2(u[5], u[9]);
   }
   __syncthreads();
 }
@@ -1507,6 +1509,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   c.mask = a.mask + mb.coff;
   c.maxcnt = mb.maxcnt;
   c.fu = c.fu16 = 8ll * ncell;
+  c.fd16 = 16ll * ncell;
   c.cb = 0;
   c.ce = ncell;
   c.grp = nullptr;
@@ -1587,7 +1590,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     Precon<float> P;
     build_precon<float>(c, A32, F, P, o.precon, &s_bad);
     if (o.factor16 && P.kind == 2 && !c.coupled && a.F16) {
-      uint16_t* FB = a.F16 + (size_t)24 * mb.coff;   // 48 bytes per cell
+      uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff;
       factors_to_bf16(c, F, FB);
       P.FB = FB;
     }
@@ -1725,6 +1728,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_group_kernel(YgKrylovArgs a) {
     cm.line = -1; cm.coupled = 0; cm.nwl = 0; cm.lpw = lpw;
     cm.tm = a.timers ? s_tm : nullptr;
     cm.fu = cm.fu16 = 8ll * ncell;
+    cm.fd16 = 16ll * ncell;
     cm.cb = i0 * ny; cm.ce = i1 * ny;
     cm.grp = &grp;
     Ctx cr = cm;   // this rank's lines: factorization and sweeps
@@ -1737,7 +1741,8 @@ __global__ __launch_bounds__(NT) void yg_krylov_group_kernel(YgKrylovArgs a) {
       cr.line = (w < cr.nwl && lane < lpw && i < Lg) ? i : -1;
     }
     cr.fu = 8ll * ncell + 4ll * cm.cb;     // the member's U part from this part's L part
-    cr.fu16 = 8ll * ncell + 8ll * cm.cb;
+    cr.fu16 = 8ll * ncell;
+    cr.fd16 = 16ll * ncell - 4ll * cm.cb;
     cr.cb = 0; cr.ce = cr.ncell;
     cr.grp = nullptr;
     cr.pc = &cr;
@@ -1776,7 +1781,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_group_kernel(YgKrylovArgs a) {
     P.F = F + (size_t)8 * cm.cb;   // this part's L factors (block-skewed order)
     if (Lg > 0) ilu0_factor<float>(cr, A32 + (size_t)YG_NA * cm.cb, F + (size_t)8 * cm.cb, &s_bad);
     if (o.factor16 && a.F16) {
-      uint16_t* FB = a.F16 + (size_t)24 * mb.coff + (size_t)8 * cm.cb;
+      uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff + (size_t)8 * cm.cb;
       if (Lg > 0) factors_to_bf16(cr, P.F, FB);
       P.FB = FB;
     }

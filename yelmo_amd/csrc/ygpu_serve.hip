@@ -246,7 +246,7 @@ struct Server : YgServerBase {
     if (b.kmode == YG_MODE_MIXED) c += YG_NA * 4.0 + YG_NF * 4.0 + YG_NW32 * 2 * 4.0;
     else c += YG_NF * 8.0;
     if (b.fp.energy) c += 2 * 4.0;
-    if (b.kmode == YG_MODE_MIXED && b.opts.k.factor16) c += 48.0;
+    if (b.kmode == YG_MODE_MIXED && b.opts.k.factor16) c += 2.0 * YG_NF16;
     c += sizeof(R) * (34.0 + 3.0 * nz + (b.diva ? 8.0 + 4.0 * nz : 0.0));
     return c;
   }
