@@ -162,3 +162,31 @@ def test_served_sequence_matches_oracle_chain():
         assert abs(info.ssa_iter_now - ref["ssa_iter_now"]) <= 1
         assert err <= 1e-8
         prev = ref_st
+
+
+def test_served_group_lane_alone_equals_in_mix():
+    """The largest lane runs its members with member groups (here -ygpu_group
+    4: four workgroups per member); sequences of that lane give the result of
+    the same sequence solved alone in a session with the same option, bit for
+    bit, while a small lane of one workgroup per member runs beside it."""
+    print("case serve group lane", flush=True)
+    base = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    par = base.with_(ssa_lis_opt=base.ssa_lis_opt + " -ygpu_group 4")
+    seqs = _sequences((97, 129, 81, 113), 3, np.float64, seed=31)
+    alone = [_alone(s, par) for s in seqs]
+    small = _sequences((41,), 3, np.float64, seed=77)
+    small_alone = _alone(small[0], base.with_(ssa_lis_opt=base.ssa_lis_opt + " -ygpu_group 1"))
+    mixed, max_running = _served(seqs + small, par, budget=1 << 30, arrive_every=0,
+                                 lanes=[(65, 1), (129, 3)])
+    assert max_running >= 3
+    for k in range(len(seqs)):
+        for j in range(len(seqs[k])):
+            st_a, inf_a = alone[k][j]
+            st_m, inf_m = mixed[k][j]
+            assert inf_m.status == inf_a.status == 0
+            assert (inf_m.ssa_iter_now, inf_m.ssa_lin_iter) == (inf_a.ssa_iter_now,
+                                                               inf_a.ssa_lin_iter), (k, j)
+            for n in COMPARE:
+                assert np.array_equal(st_m[n], st_a[n]), (k, j, n)
+    for j in range(len(small[0])):
+        assert np.array_equal(mixed[len(seqs)][j][0]["ux_b"], small_alone[j][0]["ux_b"]), j

@@ -115,6 +115,11 @@ __shared__ double yg_red[16 * 4];   // block reductions
 // time: a block that waits longer than YG_GRP_SPIN sets the group's error
 // flag, after which no block of the group waits again (the solve runs on to
 // its iteration limit on meaningless data and is reported failed).
+// The placement is checked, not assumed: every block reads its XCD from the
+// XCC_ID hardware register and the group compares them behind a first
+// barrier with device-scope fences; a group that finds itself spread over
+// several XCDs keeps those fences on every barrier (L2 write-back before the
+// arrival, invalidation after the wait: correct across L2s, slower).
 #define YG_GRP_SPIN 200000000ll   // wall-clock ticks (100 MHz): 2 s
 struct Grp {
   int* cnt;       // arrivals at group barriers (monotonic over the launch)
@@ -124,6 +129,7 @@ struct Grp {
 };
 __shared__ int yg_gbar;    // group barriers this block has passed
 __shared__ int yg_gfail;   // a wait of this group timed out
+__shared__ int yg_gfence;  // the group spans several XCDs: barriers fence
 
 YG_DEV int grp_load(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -138,6 +144,7 @@ YG_DEV void grp_barrier(const Grp& g) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const int k = yg_gbar + 1;
+    if (yg_gfence) __threadfence();   // release at device scope
     if (!yg_gfail) {
       __hip_atomic_fetch_add(g.cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const long long t0 = wall_clock64();
@@ -150,7 +157,32 @@ YG_DEV void grp_barrier(const Grp& g) {
         __builtin_amdgcn_s_sleep(1);
       }
     }
+    if (yg_gfence) __threadfence();   // acquire at device scope
     yg_gbar = k;
+  }
+  __syncthreads();
+}
+
+// XCD of the calling workgroup (hardware register XCC_ID, bits 3:0)
+YG_DEV int xcc_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 0xf; }
+
+// The group's first barrier, with device-scope fences, also compares the
+// XCDs of its workgroups (published in the synchronization words); from
+// then on the barriers fence only if they differ.
+YG_DEV void grp_start(const Grp& g, int* xcc_words) {
+  if (threadIdx.x == 0) {
+    yg_gbar = 0;
+    yg_gfail = 0;
+    yg_gfence = 1;
+    xcc_words[g.rank] = xcc_id();
+  }
+  __syncthreads();
+  grp_barrier(g);
+  if (threadIdx.x == 0) {
+    const int x0 = grp_load(xcc_words);
+    int spread = 0;
+    for (int r = 1; r < g.G; ++r) spread |= grp_load(xcc_words + r) != x0;
+    yg_gfence = spread;
   }
   __syncthreads();
 }
@@ -1690,13 +1722,12 @@ __global__ __launch_bounds__(NT) void yg_krylov_group_kernel(YgKrylovArgs a) {
   Grp grp;
   grp.cnt = reinterpret_cast<int*>(gs);
   grp.err = reinterpret_cast<int*>(gs) + 1;
-  grp.part = reinterpret_cast<double*>(gs + 64);   // 2 x G x 4 doubles
+  grp.part = reinterpret_cast<double*>(gs + 64);   // 2 x G x 4 doubles (ints 8..15: XCDs)
   grp.G = G;
   grp.rank = rank;
   __shared__ int s_bad;
   __shared__ long long s_tm[8];
-  if (threadIdx.x == 0) { yg_gbar = 0; yg_gfail = 0; }
-  __syncthreads();
+  grp_start(grp, reinterpret_cast<int*>(gs) + 8);
   for (int m = gi; m < a.nmembers; m += ngroups) {
     if (a.skip && a.skip[m]) continue;
     const YgMember mb = a.members[m];

@@ -190,6 +190,10 @@ struct Lane {
   long long hi_cell = 0;     // one past the last occupied cell
   std::vector<long long> h_seg;
   std::vector<int> h_segm, h_sel;
+  // member groups (the largest lane only): workgroups per member, groups,
+  // their synchronization words and the wait-timeout flag (device, pinned)
+  int group = 1, ngroups = 0;
+  int *d_gsync = nullptr, *d_gfail = nullptr, *h_gfail = nullptr;
 };
 
 }  // namespace
@@ -237,6 +241,7 @@ struct Server : YgServerBase {
       if (L.done_ev) (void)hipEventDestroy(L.done_ev);
       if (L.stream) (void)hipStreamDestroy(L.stream);
       if (L.h_done) (void)hipHostFree(L.h_done);
+      if (L.h_gfail) (void)hipHostFree(L.h_gfail);
     }
   }
 
@@ -316,6 +321,7 @@ struct Server : YgServerBase {
       YG_CHECK(h, hipEventCreateWithFlags(&L.done_ev, hipEventDisableTiming));
       YG_CHECK(h, hipHostMalloc((void**)&L.h_done, sizeof(int) * L.ns, hipHostMallocDefault));
     }
+    if ((rc = setup_groups())) return rc;
     slots.assign(S, Slot{});
     for (Lane& L : lanes)
       for (int k = 0; k < L.ns; ++k) slots[L.s0 + k].lane = (int)(&L - lanes.data());
@@ -541,7 +547,9 @@ struct Server : YgServerBase {
           sl.zoff = sl.coff * nzmax;
           ygpu_ssa_problem P{};
           P.nx = r.nx; P.ny = r.ny; P.nz = r.nz; P.real_bytes = 8; P.dx = r.dx; P.dy = r.dy;
+          B.group = L.group;   // the schedule of the lane's group size
           int rc = B.describe(P, s, sl.M, sl.st);
+          B.group = 1;
           if (rc) return rc;
           sl.M.coff = sl.coff;
           sl.M.toff = (long long)s * (YG_MAX_STEPS + 1);
@@ -583,6 +591,46 @@ struct Server : YgServerBase {
     return YGPU_OK;
   }
 
+  // Member groups in the largest lane (mixed precision, BiCGSTAB, line-block
+  // ILU(0)): G workgroups per member, G from the batch rule for one domain of
+  // the lane's largest size, or -ygpu_group G; every member of the lane runs
+  // with that G.  Only one lane uses groups: its launches follow each other
+  // on its stream, and every kernel of the other lanes ends without waiting
+  // for anything, so the CUs a group still needs become free while its
+  // resident workgroups wait.
+  int setup_groups() {
+    const int want = B.opts.group;
+    const bool fits = B.kmode == YG_MODE_MIXED && B.opts.k.solver == 4 && B.opts.k.precon == 2 &&
+                      B.opts.k.ilu_blocks != 0;
+    if (want == 1 || !fits) return YGPU_OK;
+    if (want != 0 && want != 2 && want != 4 && want != 8) {
+      h->err = "-ygpu_group takes 0, 1, 2, 4 or 8";
+      return YGPU_ERR_ARG;
+    }
+    int ncu = 0;
+    YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    Lane& L = lanes.back();
+    const long long cells = (long long)L.side * L.side;
+    int G = want;
+    if (G == 0) {
+      G = 1;
+      for (int g : {8, 4, 2})
+        if (8 * g <= ncu && cells / g >= 4096) { G = g; break; }
+    }
+    const int groups8 = ((L.ns + 7) / 8) * 8;
+    while (G > 1 && groups8 * G > ncu) G /= 2;
+    if (G < 2 || L.side > 64 * G * (YG_GROUP_THREADS / 64)) return YGPU_OK;
+    L.group = G;
+    L.ngroups = groups8;
+    YG_CHECK(h, B.ar.alloc(&L.d_gsync, (size_t)L.ngroups * (YG_GSYNC_BYTES / 4)));
+    YG_CHECK(h, B.ar.alloc(&L.d_gfail, 1));
+    YG_CHECK(h, hipMemsetAsync(L.d_gfail, 0, sizeof(int), h->stream));
+    YG_CHECK(h, hipHostMalloc((void**)&L.h_gfail, sizeof(int), hipHostMallocDefault));
+    *L.h_gfail = 0;
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    return YGPU_OK;
+  }
+
   // one Picard iteration of every running solve of the lane, then an
   // asynchronous read of its done flags
   int launch(Lane& L) {
@@ -605,10 +653,17 @@ struct Server : YgServerBase {
     yg_launch_fields_pre<R>(G, f, B.fp, ps.done, st);
     yg_launch_assemble<R>(G, f, B.fp, ps.done, B.d_A64,
                           B.kmode == YG_MODE_MIXED ? B.d_A32 : nullptr, B.d_b, B.d_x, B.d_mask, st);
-    yg_launch_krylov(ka, B.kmode, L.ns, L.block, st);
+    if (L.group > 1) {
+      ka.group = L.group; ka.nmembers = L.ns; ka.gsync = L.d_gsync; ka.group_fail = L.d_gfail;
+      yg_launch_krylov_group(ka, L.ngroups, st);
+    } else {
+      yg_launch_krylov(ka, B.kmode, L.ns, L.block, st);
+    }
     yg_launch_epilogue<R>(G, f, B.fp, B.d_x, B.d_kit + L.s0, B.d_kst + L.s0, B.iter_max, ps, st);
     YG_CHECK(h, hipGetLastError());
     YG_CHECK(h, hipMemcpyAsync(L.h_done, ps.done, sizeof(int) * L.ns, hipMemcpyDeviceToHost, st));
+    if (L.h_gfail)
+      YG_CHECK(h, hipMemcpyAsync(L.h_gfail, L.d_gfail, sizeof(int), hipMemcpyDeviceToHost, st));
     YG_CHECK(h, hipEventRecord(L.done_ev, st));
     L.inflight = true;
     return YGPU_OK;
@@ -643,6 +698,10 @@ struct Server : YgServerBase {
   // a lane's step has ended: finish and download the solves that completed
   int complete(Lane& L) {
     L.inflight = false;
+    if (L.h_gfail && *L.h_gfail) {
+      h->err = "member group: a workgroup waited more than 2 s for its group";
+      return YGPU_ERR_DEVICE;
+    }
     std::vector<int> fin;
     for (int k = 0; k < L.ns; ++k)
       if (slots[L.s0 + k].state == kRunning && L.h_done[k] != 0) fin.push_back(L.s0 + k);
