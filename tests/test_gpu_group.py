@@ -43,16 +43,16 @@ def _batch(probs, par, opt, runs=1, diva=False):
     return out
 
 
-@pytest.mark.parametrize("G", [2, 4, 8])
-def test_group_members_within_mixed_bar(G):
+@pytest.mark.parametrize("G,threads", [(2, 512), (4, 512), (8, 512), (8, 256)])
+def test_group_members_within_mixed_bar(G, threads):
     """Three members of different sizes (the largest sets the groups' line
-    counts), factors in BF16."""
-    print(f"case group G={G}", flush=True)
+    counts), factors in BF16; 256 threads: two workgroups per CU."""
+    print(f"case group G={G} threads={threads}", flush=True)
     par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
     sides = ((161, 8e3, 5), (129, 10e3, 6), (97, 14e3, 7))
     probs = [generator.make_problem(n, dx, seed=s, dtype=np.float64) for n, dx, s in sides]
     (states, infos), = _batch([p.copy() for p in probs], par,
-                              OPT + f" -ygpu_factor bf16 -ygpu_group {G}")
+                              OPT + f" -ygpu_factor bf16 -ygpu_group {G} -ygpu_group_threads {threads}")
     for k, p in enumerate(probs):
         ref_st, ref = oracle.calc_velocity_ssa(p.copy(), _converged(par), dp=True)
         err = vel_rel_l2(states[k], ref_st)

@@ -46,6 +46,13 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
     else if (k == "-ygpu_graph") o.k.graph_chunk = std::max(0, atoi(v.c_str()));
     else if (k == "-ygpu_group") o.group = atoi(v.c_str());
+    else if (k == "-ygpu_group_threads") {
+      o.group_threads = atoi(v.c_str());
+      if (o.group_threads != 256 && o.group_threads != 512) {
+        err = "-ygpu_group_threads takes 256 or 512";
+        return YGPU_ERR_ARG;
+      }
+    }
     else if (k == "-ygpu_factor") {
       if (v == "bf16") o.k.factor16 = 1;
       else if (v == "fp32") o.k.factor16 = 0;

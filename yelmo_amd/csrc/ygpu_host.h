@@ -51,6 +51,7 @@ struct Opts {
   int ny_hint = 0;
   int prec = -1;  // -1 unset, 0 fp64, 2 mixed
   int group = 0;  // workgroups per member: 0 automatic, 1 one (no groups), 2, 4 or 8
+  int group_threads = 512;   // threads per group workgroup: 512 (one per CU) or 256 (two)
 };
 
 int parse_opts(const char* s, Opts& o, std::string& err);
@@ -83,7 +84,7 @@ struct Batch {
   Arena ar;
   int nmem = 0, diva = 0, prec = 0, kmode = 0, block = 1024;
   // member groups: workgroups per member (1: the one-block kernel) and groups
-  int group = 1, ngroups = 0;
+  int group = 1, ngroups = 0, gthreads = 512;
   int *d_gsync = nullptr, *d_gfail = nullptr;
   long long total_cells = 0, total_z = 0, total_v = 0;
   int zstride = 0;
@@ -257,6 +258,8 @@ struct Batch {
     }
     int ncu = 0;
     YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    gthreads = opts.group_threads;
+    ncu *= 512 / gthreads;   // workgroups the CUs hold at once
     const int groups8 = ((nmem + 7) / 8) * 8;
     int G = want;
     if (G == 0) {
@@ -266,7 +269,7 @@ struct Batch {
       if (G == 1) return YGPU_OK;
     }
     if (8 * G > ncu) { h->err = "-ygpu_group: more workgroups than CUs"; return YGPU_ERR_ARG; }
-    if (maxnx > 64 * G * (YG_GROUP_THREADS / 64)) {
+    if (maxnx > 64 * G * (gthreads / 64)) {
       h->err = "-ygpu_group: a member has more lines than the group's waves can sweep";
       return YGPU_ERR_ARG;
     }
@@ -530,7 +533,7 @@ struct Batch {
     yg_launch_assemble<R>(G, f, fp, d_done, d_A64, kmode == YG_MODE_MIXED ? d_A32 : nullptr, d_b,
                           d_x, d_mask, st);
     if (ev0) (void)hipEventRecord(ev0, st);
-    if (group > 1) yg_launch_krylov_group(ka, ngroups, st);
+    if (group > 1) yg_launch_krylov_group(ka, ngroups, gthreads, st);
     else yg_launch_krylov(ka, kmode, nmem, block, st);
     if (ev1) (void)hipEventRecord(ev1, st);
     yg_launch_epilogue<R>(G, f, fp, d_x, d_kit, d_kst, iter_max, ps, st);

@@ -72,11 +72,12 @@ struct YgKrylovArgs {
 };
 
 void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, hipStream_t st);
-// The member-group kernel: ngroups groups of a.group workgroups (512 threads
-// each; ngroups a multiple of 8, ngroups * a.group at most the CU count) take
-// the a.nmembers members in turn.  Mixed mode, BiCGSTAB, block-Jacobi ILU(0)
-// only.  Clears the groups' synchronization words first.
-void yg_launch_krylov_group(const YgKrylovArgs& a, int ngroups, hipStream_t st);
+// The member-group kernel: ngroups groups of a.group workgroups (threads 512,
+// one workgroup per CU, or 256, two per CU; ngroups a multiple of 8, the
+// workgroups at most what the CUs hold at once) take the a.nmembers members
+// in turn.  Mixed mode, BiCGSTAB, block-Jacobi ILU(0) only.  Clears the
+// groups' synchronization words first.
+void yg_launch_krylov_group(const YgKrylovArgs& a, int ngroups, int threads, hipStream_t st);
 #define YG_GSYNC_BYTES 1024
 #define YG_GROUP_THREADS 512
 // sets the kernels' launch attributes; call before capturing a launch in a graph

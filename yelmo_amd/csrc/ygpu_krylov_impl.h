@@ -66,6 +66,7 @@ struct Ctx {
   // the context of the preconditioner's part of the member (itself for one
   // block per member)
   const Ctx* pc;
+  int lds;   // bytes of the kernel's dynamic LDS (the factorization's stage)
 };
 
 // optional phase timing: 0 factor, 1 sweep L, 2 sweep U, 3 spmv, 4 vectors
@@ -347,7 +348,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
   const bool ring_out = c.coupled && w + 1 < c.nwl && lane == 63;
   const size_t nc = c.ncell;
   constexpr int NV = 19;  // 18 slot values + the presence mask
-  const int capc = min(2 * (int)blockDim.x, (int)(YG_DYN_LDS / sizeof(TA)) / (2 * NV));
+  const int capc = min(2 * (int)blockDim.x, (int)(c.lds / sizeof(TA)) / (2 * NV));
   const int C = max(1, capc / c.maxcnt);
   const int stride = C * c.maxcnt;
   auto stage = [&](int sb, int t0) {   // the chunk starting at t0 -> stage at offset sb
@@ -1546,6 +1547,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   c.ce = ncell;
   c.grp = nullptr;
   c.pc = &c;
+  c.lds = YG_DYN_LDS;
   {
     // line-to-wave mapping of the recurrences (exact or block ILU(0))
     const int g = threadIdx.x, lane = g & 63, w = g >> 6;
@@ -1712,8 +1714,13 @@ void yg_launch_krylov_nt(const YgKrylovArgs& a, int mode, int nmembers, int bloc
 // needs no exchange; the SpMV reads the neighbouring ranks' edge lines after
 // a group barrier.  The iteration is the one-block kernel's mixed mode
 // (FP64 refinement around FP32 BiCGSTAB) with every sum a group sum.
+// Dynamic LDS of the group kernel: 120 KB with 512 threads (one workgroup
+// per CU); 56 KB with 256 threads, so that two workgroups (of different
+// members) share a CU, each with at most 256 VGPRs per lane.
+template <int NT> constexpr int yg_group_lds() { return NT == 512 ? YG_DYN_LDS : 56 * 1024; }
+
 template <int NT>
-__global__ __launch_bounds__(NT) void yg_krylov_group_kernel(YgKrylovArgs a) {
+__global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) {
   const int G = a.group;
   const int blk = blockIdx.x, slot = blk >> 3;
   const int gi = (slot / G) * 8 + (blk & 7), rank = slot % G;
@@ -1762,6 +1769,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_group_kernel(YgKrylovArgs a) {
     cm.fd16 = 16ll * ncell;
     cm.cb = i0 * ny; cm.ce = i1 * ny;
     cm.grp = &grp;
+    cm.lds = yg_group_lds<NT>();
     Ctx cr = cm;   // this rank's lines: factorization and sweeps
     cr.nx = Lg; cr.T = T; cr.ncell = Lg * ny;
     cr.mask = cm.mask + cm.cb;
@@ -1880,7 +1888,7 @@ void yg_krylov_group_attrs_nt() {
   static bool done = false;
   if (done) return;
   (void)hipFuncSetAttribute((const void*)yg_krylov_group_kernel<NT>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, YG_DYN_LDS);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, yg_group_lds<NT>());
   done = true;
 }
 
@@ -1889,5 +1897,6 @@ void yg_launch_krylov_group_nt(const YgKrylovArgs& a, int ngroups, hipStream_t s
   yg_krylov_group_attrs_nt<NT>();
   const int nwords = ngroups * (YG_GSYNC_BYTES / 4);
   hipLaunchKernelGGL(k_clear_gsync, dim3((nwords + 255) / 256), dim3(256), 0, st, a.gsync, nwords);
-  hipLaunchKernelGGL(yg_krylov_group_kernel<NT>, dim3(ngroups * a.group), dim3(NT), YG_DYN_LDS, st, a);
+  hipLaunchKernelGGL(yg_krylov_group_kernel<NT>, dim3(ngroups * a.group), dim3(NT),
+                     yg_group_lds<NT>(), st, a);
 }

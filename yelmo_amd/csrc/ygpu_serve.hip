@@ -655,7 +655,7 @@ struct Server : YgServerBase {
                           B.kmode == YG_MODE_MIXED ? B.d_A32 : nullptr, B.d_b, B.d_x, B.d_mask, st);
     if (L.group > 1) {
       ka.group = L.group; ka.nmembers = L.ns; ka.gsync = L.d_gsync; ka.group_fail = L.d_gfail;
-      yg_launch_krylov_group(ka, L.ngroups, st);
+      yg_launch_krylov_group(ka, L.ngroups, YG_GROUP_THREADS, st);
     } else {
       yg_launch_krylov(ka, B.kmode, L.ns, L.block, st);
     }

@@ -117,7 +117,7 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
         yg_launch_assemble<double>(Gm, B.f, B.fp, B.d_done, B.d_A64, B.d_A32,
                                    B.d_b, B.d_x, B.d_mask, st);
         YG_CHECK(h, hipEventRecord(e0, st));
-        if (G > 1) yg_launch_krylov_group(ka, 8, st);
+        if (G > 1) yg_launch_krylov_group(ka, 8, YG_GROUP_THREADS, st);
         else yg_launch_krylov(ka, YG_MODE_MIXED, 1, c.block, st);
         YG_CHECK(h, hipEventRecord(e1, st));
         YG_CHECK(h, hipGetLastError());
