@@ -12,7 +12,11 @@
 // each group size a batch of such domains may choose, and records the
 // fastest per (grid size, G) in the handle; batches, sessions and servers
 // created on the handle afterwards give every member of that size the
-// schedule of the group size they run with.
+// schedule of the group size they run with.  The timed launch holds copies
+// of the domain that fill the CUs (up to 64, within about 4 GB), so that
+// the member shares the device's memory system as in a full batch: a
+// schedule timed alone on an idle device favours more, shorter line blocks
+// than a full device does.
 #include "ygpu_host.h"
 
 using namespace yg;
@@ -49,26 +53,33 @@ std::vector<Candidate> candidates(int nx, int G) {
   return out;
 }
 
-int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob, int diva,
-         int reps, int32_t* nb_out, int32_t* block_out, double* ms_out) {
-  if (par->precision != YGPU_PREC_MIXED) {
-    h->err = "autotune: the schedule belongs to the mixed-precision path";
-    return YGPU_ERR_ARG;
-  }
+// Times the candidates of one group size G on a batch of `copies` copies of
+// the domain (so that the member shares the device as in a full batch) and
+// records the fastest; returns it in *win (status -1: none converged).
+int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob, int diva,
+               int reps, int G, int copies, Candidate* win) {
   ygpu_ssa_params p = *par;
   p.block = 0;
+  const std::string opt = std::string(par->ssa_lis_opt) + " -ygpu_group " + std::to_string(G);
+  if (opt.size() >= sizeof(p.ssa_lis_opt)) {
+    h->err = "autotune: option string too long";
+    return YGPU_ERR_ARG;
+  }
+  std::snprintf(p.ssa_lis_opt, sizeof(p.ssa_lis_opt), "%s", opt.c_str());
+  std::vector<ygpu_ssa_problem> probs(copies, *prob);
   Batch<double> B;
-  int rc = B.setup(h, &p, 1, prob, diva);
+  int rc = B.setup(h, &p, copies, probs.data(), diva);
   if (rc) return rc;
   if (B.opts.k.precon != 2) {
     h->err = "autotune: the schedule is that of the ILU(0) preconditioner (-p ilu)";
     return YGPU_ERR_ARG;
   }
-  if ((rc = B.upload_inputs(prob))) return rc;
-  if ((rc = B.upload_state(prob))) return rc;
+  if (B.group != G) { win->status = -1; return YGPU_OK; }   // G does not apply to this domain
+  if ((rc = B.upload_inputs(probs.data()))) return rc;
+  if ((rc = B.upload_state(probs.data()))) return rc;
   hipStream_t st = h->stream;
   const YgBatchGeom Gm = B.geom();
-  YG_CHECK(h, hipMemsetAsync(B.d_done, 0, sizeof(int), st));
+  YG_CHECK(h, hipMemsetAsync(B.d_done, 0, sizeof(int) * copies, st));
   // the fields of the first Picard iteration (velocity_ssa.f90:172-262)
   yg_launch_init<double>(Gm, B.f, B.fp, nullptr, st);
   if (B.diva) yg_launch_diva_pre<double>(Gm, B.f, B.dv, B.fp, B.d_done, st);
@@ -84,77 +95,88 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
               B.fp.BC == YG_BC_MISMIP3D || B.fp.BC == YG_BC_TROUGH);
   ka.skip = B.d_done; ka.out_iters = B.d_kit; ka.out_status = B.d_kst; ka.out_resid = B.d_kres;
   ka.opts = B.opts.k;
-  // synchronization words of 8 groups (one member: the first group works,
-  // the group count must be a multiple of 8)
-  Arena tmp;
-  int ncu = 0;
-  YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
-  YG_CHECK(h, tmp.alloc(&ka.gsync, (size_t)8 * (YG_GSYNC_BYTES / 4)));
-  YG_CHECK(h, tmp.alloc(&ka.group_fail, 1));
-  YG_CHECK(h, hipMemsetAsync(ka.group_fail, 0, sizeof(int), st));
-  ka.nmembers = 1;
+  ka.group = B.group; ka.nmembers = copies; ka.gsync = B.d_gsync; ka.group_fail = B.d_gfail;
+  if (B.d_gfail) YG_CHECK(h, hipMemsetAsync(B.d_gfail, 0, sizeof(int), st));
   hipEvent_t e0, e1;
   YG_CHECK(h, hipEventCreate(&e0));
   YG_CHECK(h, hipEventCreate(&e1));
-  const long long cells = (long long)prob->nx * prob->ny;
-  const bool group_ok = B.opts.k.solver == 4 && B.opts.k.ilu_blocks != 0;
-  const Candidate* win1 = nullptr;
-  std::vector<Candidate> all;
-  for (int G : {1, 2, 4, 8}) {
-    // group sizes a batch may choose for this domain (pick_group's rule)
-    if (G > 1 && (!group_ok || cells / G < 4096 || 8 * G > ncu ||
-                  prob->nx > 64 * G * (YG_GROUP_THREADS / 64)))
-      continue;
-    std::vector<Candidate> cand = candidates(prob->nx, G);
-    ka.group = G;
-    bool warm = false;
-    for (Candidate& c : cand) {
-      B.mem[0].nb = c.nb;
-      YG_CHECK(h, hipMemcpyAsync(B.d_mem, B.mem.data(), sizeof(YgMember), hipMemcpyHostToDevice, st));
-      double best = 1e300;
-      for (int r = 0; r < reps + (warm ? 0 : 1); ++r) {
-        // the same system every time: assembly rewrites the matrix, b and x0
-        yg_launch_assemble<double>(Gm, B.f, B.fp, B.d_done, B.d_A64, B.d_A32,
-                                   B.d_b, B.d_x, B.d_mask, st);
-        YG_CHECK(h, hipEventRecord(e0, st));
-        if (G > 1) yg_launch_krylov_group(ka, 8, YG_GROUP_THREADS, st);
-        else yg_launch_krylov(ka, YG_MODE_MIXED, 1, c.block, st);
-        YG_CHECK(h, hipEventRecord(e1, st));
-        YG_CHECK(h, hipGetLastError());
-        YG_CHECK(h, hipEventSynchronize(e1));
-        float ms = 0.f;
-        YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
-        if (warm) best = std::min(best, (double)ms);
-        warm = true;
-      }
-      YG_CHECK(h, hipMemcpy(&c.iters, B.d_kit, sizeof(int), hipMemcpyDeviceToHost));
-      YG_CHECK(h, hipMemcpy(&c.status, B.d_kst, sizeof(int), hipMemcpyDeviceToHost));
-      c.ms = best;
+  std::vector<Candidate> cand = candidates(prob->nx, G);
+  bool warm = false;
+  for (Candidate& c : cand) {
+    for (YgMember& M : B.mem) M.nb = c.nb;
+    YG_CHECK(h, hipMemcpyAsync(B.d_mem, B.mem.data(), sizeof(YgMember) * copies,
+                               hipMemcpyHostToDevice, st));
+    double best = 1e300;
+    for (int r = 0; r < reps + (warm ? 0 : 1); ++r) {
+      // the same systems every time: assembly rewrites the matrix, b and x0
+      yg_launch_assemble<double>(Gm, B.f, B.fp, B.d_done, B.d_A64, B.d_A32, B.d_b, B.d_x,
+                                 B.d_mask, st);
+      YG_CHECK(h, hipEventRecord(e0, st));
+      if (G > 1) yg_launch_krylov_group(ka, B.ngroups, B.gthreads, st);
+      else yg_launch_krylov(ka, YG_MODE_MIXED, copies, c.block, st);
+      YG_CHECK(h, hipEventRecord(e1, st));
+      YG_CHECK(h, hipGetLastError());
+      YG_CHECK(h, hipEventSynchronize(e1));
+      float ms = 0.f;
+      YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
+      if (warm) best = std::min(best, (double)ms);
+      warm = true;
     }
+    YG_CHECK(h, hipMemcpy(&c.iters, B.d_kit, sizeof(int), hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(&c.status, B.d_kst, sizeof(int), hipMemcpyDeviceToHost));
+    c.ms = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (B.d_gfail) {
     int gf = 0;
-    YG_CHECK(h, hipMemcpy(&gf, ka.group_fail, sizeof(int), hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(&gf, B.d_gfail, sizeof(int), hipMemcpyDeviceToHost));
     if (gf) {
       h->err = "autotune: a member group's workgroups were not resident together";
       return YGPU_ERR_DEVICE;
     }
-    const Candidate* win = nullptr;
-    for (const Candidate& c : cand)
-      if (c.status == 0 && (!win || c.ms < win->ms)) win = &c;
-    if (win) {
-      h->sched[{prob->nx, prob->ny, G}] = {win->nb, win->block};
-      if (G == 1) all.push_back(*win);
-    }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (!all.empty()) win1 = &all[0];
-  if (!win1) {
+  win->status = -1;
+  for (const Candidate& c : cand)
+    if (c.status == 0 && (win->status != 0 || c.ms < win->ms)) *win = c;
+  if (win->status == 0) h->sched[{prob->nx, prob->ny, G}] = {win->nb, win->block};
+  return YGPU_OK;
+}
+
+int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob, int diva,
+         int reps, int32_t* nb_out, int32_t* block_out, double* ms_out) {
+  if (par->precision != YGPU_PREC_MIXED) {
+    h->err = "autotune: the schedule belongs to the mixed-precision path";
+    return YGPU_ERR_ARG;
+  }
+  int ncu = 0;
+  YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+  const long long cells = (long long)prob->nx * prob->ny;
+  // copies that fill the device's CUs, within about 4 GB of device memory
+  const long long by_mem = std::max(1LL, (4LL << 30) / (cells * 1400));
+  Opts o;
+  std::string e;
+  int rc0 = parse_opts(par->ssa_lis_opt, o, e);
+  if (rc0) { h->err = e; return rc0; }
+  const bool group_ok = o.k.solver == 4 && o.k.precon == 2 && o.k.ilu_blocks != 0;
+  Candidate win1{};
+  win1.status = -1;
+  for (int G : {1, 2, 4, 8}) {
+    // the group sizes pick_group may choose for this domain
+    if (G > 1 && (!group_ok || cells / G < 4096 || 8 * G > ncu)) continue;
+    const int copies = (int)std::min<long long>({(long long)ncu / G, by_mem, 64LL});
+    Candidate w{};
+    const int rc = tune_group(h, par, prob, diva, reps, G, std::max(1, copies), &w);
+    if (rc) return rc;
+    if (G == 1) win1 = w;
+  }
+  if (win1.status != 0) {
     h->err = "autotune: no schedule converged on this domain";
     return YGPU_ERR_UNSUPPORTED;
   }
-  if (nb_out) *nb_out = win1->nb;
-  if (block_out) *block_out = win1->block;
-  if (ms_out) *ms_out = win1->ms;
+  if (nb_out) *nb_out = win1.nb;
+  if (block_out) *block_out = win1.block;
+  if (ms_out) *ms_out = win1.ms;
   return YGPU_OK;
 }
 
