@@ -7,9 +7,10 @@ rate factor drawn log-uniformly in [1e-17, 1e-16]), each member solved from
 rest through the full Picard loop (BiCGSTAB + ILU(0) at rtol 1e-8, Picard
 tolerance 1e-4, at most 50 iterations) in one batched call of the
 mixed-precision path (FP32 stencil SpMV and ILU(0) inside FP64 iterative
-refinement; the ILU(0) factors stored in BF16 with the U rows scaled by their
-FP32 inverse diagonals, --factor fp32 keeps them in FP32).  Inputs are resident in HBM before the timed region; each step
-restores the initial state on the device and solves all members again.
+refinement, several workgroups per member; --factor bf16 stores the ILU(0)
+factors in BF16 with the U rows scaled by their FP32 inverse diagonals).
+Inputs are resident in HBM before the timed region; each step restores the
+initial state on the device and solves all members again.
 
 The line also reports the plain FP64 GPU path on the same ensemble (batched,
 and one member at a time), the per-member relative L2 and max velocity error
@@ -390,7 +391,7 @@ def main():
     ap.add_argument("--single-sample", type=int, default=4,
                     help="members solved one at a time on the FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--factor", choices=("fp32", "bf16"), default="bf16",
+    ap.add_argument("--factor", choices=("fp32", "bf16"), default="fp32",
                     help="storage of the mixed path's ILU(0) factors: bf16 (with the U rows "
                          "scaled by their FP32 inverse diagonals) or fp32")
     ap.add_argument("--no-tune", action="store_true",

@@ -7,8 +7,8 @@
 // block (512 or 1024, which bounds the block count).  With G workgroups per
 // member (member groups), the blocks are split over the G workgroups (a
 // multiple of G of them, 512 threads each).  ygpu_autotune times the first
-// Picard iteration's linear solve of the caller's own domain under each
-// candidate schedule on the device, for one workgroup per member and for
+// Picard solve of the caller's own domain under each candidate schedule on
+// the device (all its linear solves), for one workgroup per member and for
 // each group size a batch of such domains may choose, and records the
 // fastest per (grid size, G) in the handle; batches, sessions and servers
 // created on the handle afterwards give every member of that size the
@@ -54,8 +54,9 @@ std::vector<Candidate> candidates(int nx, int G) {
 }
 
 // Times the candidates of one group size G on a batch of `copies` copies of
-// the domain (so that the member shares the device as in a full batch) and
-// records the fastest; returns it in *win (status -1: none converged).
+// the domain (so that the member shares the device as in a full batch): the
+// whole Picard solve of the batch from the same start, per candidate; records
+// the fastest; returns it in *win (status -1: none converged).
 int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob, int diva,
                int reps, int G, int copies, Candidate* win) {
   ygpu_ssa_params p = *par;
@@ -77,64 +78,33 @@ int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_proble
   if (B.group != G) { win->status = -1; return YGPU_OK; }   // G does not apply to this domain
   if ((rc = B.upload_inputs(probs.data()))) return rc;
   if ((rc = B.upload_state(probs.data()))) return rc;
-  hipStream_t st = h->stream;
-  const YgBatchGeom Gm = B.geom();
-  YG_CHECK(h, hipMemsetAsync(B.d_done, 0, sizeof(int) * copies, st));
-  // the fields of the first Picard iteration (velocity_ssa.f90:172-262)
-  yg_launch_init<double>(Gm, B.f, B.fp, nullptr, st);
-  if (B.diva) yg_launch_diva_pre<double>(Gm, B.f, B.dv, B.fp, B.d_done, st);
-  yg_launch_fields_pre<double>(Gm, B.f, B.fp, B.d_done, st);
-  YG_CHECK(h, hipGetLastError());
-  YgKrylovArgs ka{};
-  ka.members = B.d_mem; ka.soff = B.d_soff; ka.mask = B.d_mask;
-  ka.A64 = B.d_A64; ka.A32 = B.d_A32; ka.F64 = B.d_F64; ka.F32 = B.d_F32; ka.F16 = B.d_F16;
-  ka.b = B.d_b; ka.x = B.d_x; ka.W64 = B.d_W64; ka.W32 = B.d_W32;
-  ka.total_cells = B.total_cells;
-  ka.wrapx = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_X);
-  ka.wrapy = (B.fp.BC == YG_BC_PERIODIC || B.fp.BC == YG_BC_PERIODIC_Y ||
-              B.fp.BC == YG_BC_MISMIP3D || B.fp.BC == YG_BC_TROUGH);
-  ka.skip = B.d_done; ka.out_iters = B.d_kit; ka.out_status = B.d_kst; ka.out_resid = B.d_kres;
-  ka.opts = B.opts.k;
-  ka.group = B.group; ka.nmembers = copies; ka.gsync = B.d_gsync; ka.group_fail = B.d_gfail;
-  if (B.d_gfail) YG_CHECK(h, hipMemsetAsync(B.d_gfail, 0, sizeof(int), st));
-  hipEvent_t e0, e1;
-  YG_CHECK(h, hipEventCreate(&e0));
-  YG_CHECK(h, hipEventCreate(&e1));
+  if ((rc = B.save_initial())) return rc;
   std::vector<Candidate> cand = candidates(prob->nx, G);
   bool warm = false;
   for (Candidate& c : cand) {
     for (YgMember& M : B.mem) M.nb = c.nb;
     YG_CHECK(h, hipMemcpyAsync(B.d_mem, B.mem.data(), sizeof(YgMember) * copies,
-                               hipMemcpyHostToDevice, st));
+                               hipMemcpyHostToDevice, h->stream));
+    if (G == 1 && B.block != c.block) {   // a captured graph holds the block size
+      B.block = c.block;
+      B.drop_graph();
+    }
     double best = 1e300;
     for (int r = 0; r < reps + (warm ? 0 : 1); ++r) {
-      // the same systems every time: assembly rewrites the matrix, b and x0
-      yg_launch_assemble<double>(Gm, B.f, B.fp, B.d_done, B.d_A64, B.d_A32, B.d_b, B.d_x,
-                                 B.d_mask, st);
-      YG_CHECK(h, hipEventRecord(e0, st));
-      if (G > 1) yg_launch_krylov_group(ka, B.ngroups, B.gthreads, st);
-      else yg_launch_krylov(ka, YG_MODE_MIXED, copies, c.block, st);
-      YG_CHECK(h, hipEventRecord(e1, st));
-      YG_CHECK(h, hipGetLastError());
-      YG_CHECK(h, hipEventSynchronize(e1));
-      float ms = 0.f;
-      YG_CHECK(h, hipEventElapsedTime(&ms, e0, e1));
-      if (warm) best = std::min(best, (double)ms);
+      // the whole Picard solve from the same start (its later, warm-started
+      // linear solves favour other schedules than the first one alone)
+      if ((rc = B.restore_initial())) return rc;
+      double secs = 0.0;
+      if ((rc = B.run(&secs))) return rc;
+      if (warm) best = std::min(best, 1e3 * secs);
       warm = true;
     }
-    YG_CHECK(h, hipMemcpy(&c.iters, B.d_kit, sizeof(int), hipMemcpyDeviceToHost));
-    YG_CHECK(h, hipMemcpy(&c.status, B.d_kst, sizeof(int), hipMemcpyDeviceToHost));
+    int lin_it = 0, lin_fail = 0;
+    YG_CHECK(h, hipMemcpy(&lin_it, B.d_lin_iter, sizeof(int), hipMemcpyDeviceToHost));
+    YG_CHECK(h, hipMemcpy(&lin_fail, B.d_lin_fail, sizeof(int), hipMemcpyDeviceToHost));
+    c.iters = lin_it;
+    c.status = lin_fail == 0 ? 0 : 4;
     c.ms = best;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (B.d_gfail) {
-    int gf = 0;
-    YG_CHECK(h, hipMemcpy(&gf, B.d_gfail, sizeof(int), hipMemcpyDeviceToHost));
-    if (gf) {
-      h->err = "autotune: a member group's workgroups were not resident together";
-      return YGPU_ERR_DEVICE;
-    }
   }
   win->status = -1;
   for (const Candidate& c : cand)
