@@ -150,7 +150,7 @@ def run_diva(args):
     if not args.no_tune:
         from yelmo_amd.tune import autotune_sizes
         t_tune = time.perf_counter()
-        sched = autotune_sizes(base, par, diva=True, handle=h)
+        sched = autotune_sizes(base, par, diva=True, handle=h, progress=sys.stderr)
         print(f"# autotuned {len(sched)} sizes in {time.perf_counter() - t_tune:.1f}s: " +
               ", ".join(f"{k[0]}:{v.ilu_blocks}x{v.block}" for k, v in sorted(sched.items())),
               file=sys.stderr, flush=True)
@@ -463,7 +463,7 @@ def main():
     sched = {}
     if not args.no_tune:
         from yelmo_amd.tune import autotune_sizes
-        sched = autotune_sizes(base[:1], par_mixed, handle=h)
+        sched = autotune_sizes(base[:1], par_mixed, handle=h, progress=sys.stderr)
         print("# autotuned schedule " + ", ".join(f"{k[0]}:{v.ilu_blocks} blocks x {v.block} "
                                                    f"threads ({v.solve_ms:.2f} ms)"
                                                    for k, v in sched.items()),

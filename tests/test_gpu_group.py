@@ -106,3 +106,18 @@ def test_group_diva_within_mixed_bar():
     err = vel_rel_l2(g.state, ref_st, "ux_bar", "uy_bar")
     print("picard/krylov/status", infos[0], "rel L2", err, flush=True)
     assert infos[0][2] == 0 and err <= 1e-6
+
+
+def test_group_energy_formulation_within_mixed_bar():
+    """The energy formulation's systems (element Hessians, free-slip
+    folding) under member groups: mixed BiCGSTAB + line-block ILU(0) with
+    G = 4 against the converged FP64 oracle of the same formulation."""
+    print("case group energy", flush=True)
+    par = SsaParams(ssa_solver="energy", boundaries="zeros", visc_method=2, beta_method=4,
+                    precision="mixed")
+    p = generator.make_problem(97, 14e3, seed=12, dtype=np.float64)
+    ref_st, _ = oracle.calc_velocity_ssa(p.copy(), _converged(par), dp=True)
+    (states, infos), = _batch([p.copy()], par, OPT + " -ygpu_factor fp32 -ygpu_group 4")
+    err = vel_rel_l2(states[0], ref_st)
+    print("picard/krylov/status", infos[0], "rel L2", err, flush=True)
+    assert infos[0][2] == 0 and err <= 1e-6

@@ -7,13 +7,14 @@
 // block (512 or 1024, which bounds the block count).  With G workgroups per
 // member (member groups), the blocks are split over the G workgroups (a
 // multiple of G of them, 512 threads each).  ygpu_autotune times the first
-// Picard solve of the caller's own domain under each candidate schedule on
-// the device (all its linear solves), for one workgroup per member and for
+// Picard solve of the caller's own domain (its first ten iterations: the
+// solve from rest and warm-started ones) under each candidate schedule on
+// the device, for one workgroup per member and for
 // each group size a batch of such domains may choose, and records the
 // fastest per (grid size, G) in the handle; batches, sessions and servers
 // created on the handle afterwards give every member of that size the
-// schedule of the group size they run with.  The timed launch holds copies
-// of the domain that fill the CUs (up to 64, within about 4 GB), so that
+// schedule of the group size they run with.  The timed batch holds copies
+// of the domain that fill the CUs (up to 64, within about 2 GB), so that
 // the member shares the device's memory system as in a full batch: a
 // schedule timed alone on an idle device favours more, shorter line blocks
 // than a full device does.
@@ -53,14 +54,19 @@ std::vector<Candidate> candidates(int nx, int G) {
   return out;
 }
 
+constexpr int kTunePicard = 10;   // Picard iterations of a timed solve
+
 // Times the candidates of one group size G on a batch of `copies` copies of
 // the domain (so that the member shares the device as in a full batch): the
-// whole Picard solve of the batch from the same start, per candidate; records
-// the fastest; returns it in *win (status -1: none converged).
+// batch's Picard solve (at most kTunePicard iterations) from the same start,
+// per candidate; records the fastest; returns it in *win (status -1: none
+// converged).
 int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob, int diva,
                int reps, int G, int copies, Candidate* win) {
   ygpu_ssa_params p = *par;
   p.block = 0;
+  // the first Picard iterations: the solve from rest and warm-started ones
+  p.ssa_iter_max = std::min(p.ssa_iter_max, kTunePicard);
   const std::string opt = std::string(par->ssa_lis_opt) + " -ygpu_group " + std::to_string(G);
   if (opt.size() >= sizeof(p.ssa_lis_opt)) {
     h->err = "autotune: option string too long";
@@ -122,8 +128,8 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
   int ncu = 0;
   YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
   const long long cells = (long long)prob->nx * prob->ny;
-  // copies that fill the device's CUs, within about 4 GB of device memory
-  const long long by_mem = std::max(1LL, (4LL << 30) / (cells * 1400));
+  // copies that fill the device's CUs, within about 2 GB of device memory
+  const long long by_mem = std::max(1LL, (2LL << 30) / (cells * 1400));
   Opts o;
   std::string e;
   int rc0 = parse_opts(par->ssa_lis_opt, o, e);

@@ -1,14 +1,16 @@
 """Autotuner of the mixed-precision Krylov schedule (ygpu_autotune).
 
-``autotune(prob, par)`` times the first Picard iteration's linear solve of
-``prob`` on the device for each candidate schedule (ILU(0) line blocks,
-threads per member block) and records the fastest in the handle for that
-grid size; every batch, session and server created on the handle later uses
-it for members of that size.
+``autotune(prob, par)`` times the Picard solve (its first ten iterations) of
+a device-filling batch of copies of ``prob`` for each candidate schedule
+(ILU(0) line blocks, threads per member block; for one workgroup per member
+and for each group size a batch may pick) and records the fastest in the
+handle per grid size and group size; every batch, session and server created
+on the handle later uses it for members of that size.
 """
 from __future__ import annotations
 
 import ctypes as C
+import time
 from dataclasses import dataclass
 
 from . import _lib
@@ -36,12 +38,17 @@ def autotune(prob: SsaProblem, par: SsaParams, *, diva: bool = False, reps: int 
 
 
 def autotune_sizes(problems, par: SsaParams, *, diva: bool = False, reps: int = 2,
-                   handle=None) -> dict:
-    """Tunes once per distinct grid size among ``problems``."""
+                   handle=None, progress=None) -> dict:
+    """Tunes once per distinct grid size among ``problems``; ``progress``
+    (a file, e.g. sys.stderr) gets one line per tuned size."""
     out = {}
     for p in problems:
         if (p.nx, p.ny) not in out:
+            t0 = time.perf_counter()
             out[(p.nx, p.ny)] = autotune(p, par, diva=diva, reps=reps, handle=handle)
+            if progress is not None:
+                print(f"# tuned {p.nx}x{p.ny} in {time.perf_counter() - t0:.1f}s", file=progress,
+                      flush=True)
     return out
 
 
