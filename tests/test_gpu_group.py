@@ -121,3 +121,24 @@ def test_group_energy_formulation_within_mixed_bar():
     err = vel_rel_l2(states[0], ref_st)
     print("picard/krylov/status", infos[0], "rel L2", err, flush=True)
     assert infos[0][2] == 0 and err <= 1e-6
+
+
+@pytest.mark.parametrize("G,cols", [(1, 2), (4, 2), (2, 3)])
+def test_column_tiles_within_mixed_bar(G, cols):
+    """Block-Jacobi ILU(0) over tiles: every line block cut into `cols`
+    column blocks (-ygpu_ilu_cols), with one workgroup per member and with
+    member groups; the refinement meets the same criterion and the velocity
+    stays within the mixed bar of the converged FP64 oracle."""
+    print(f"case tiles G={G} cols={cols}", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    sides = ((161, 8e3, 5), (113, 12e3, 8))
+    probs = [generator.make_problem(n, dx, seed=s, dtype=np.float64) for n, dx, s in sides]
+    for fac in ("fp32", "bf16"):
+        (states, infos), = _batch([p.copy() for p in probs], par,
+                                  OPT + f" -ygpu_factor {fac} -ygpu_group {G} -ygpu_ilu_cols {cols}")
+        for k, p in enumerate(probs):
+            ref_st, _ = oracle.calc_velocity_ssa(p.copy(), _converged(par), dp=True)
+            err = vel_rel_l2(states[k], ref_st)
+            print(fac, k, "picard/krylov/status", infos[k], "rel L2", err, flush=True)
+            assert infos[k][2] == 0
+            assert err <= 1e-6

@@ -7,6 +7,7 @@ namespace yg {
 int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
   o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1; o.k.ir_target = 0.0;
+  o.k.ilu_cols = 1;
   o.k.factor16 = 0; o.k.graph_chunk = 4;
   if (!s) return 0;
   std::vector<std::string> tok;
@@ -44,6 +45,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k == "-ygpu_ir_tol") o.k.ir_target = atof(v.c_str());
     else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
     else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
+    else if (k == "-ygpu_ilu_cols") o.k.ilu_cols = std::max(1, atoi(v.c_str()));
     else if (k == "-ygpu_graph") o.k.graph_chunk = std::max(0, atoi(v.c_str()));
     else if (k == "-ygpu_group") o.group = atoi(v.c_str());
     else if (k == "-ygpu_group_threads") {

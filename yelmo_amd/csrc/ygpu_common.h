@@ -65,6 +65,7 @@ struct YgMember {
   long long woff;    // first value of this member's work vectors (multiple of 4)
   int vlen;          // length of one work vector: 2*ncell rounded up to a multiple of 4
   int nb;            // ILU(0) line blocks of this member (0: the solve options decide)
+  int ncol;          // column blocks per line block (tiles; 0: the solve options decide)
   double dx, dy;
 };
 

@@ -31,7 +31,10 @@ struct ygpu_handle {
   // Krylov schedules of the mixed path per grid and workgroups per member
   // (nx, ny, G): ILU(0) line blocks and threads per member block, from
   // ygpu_autotune or ygpu_set_schedule
-  std::map<std::tuple<int, int, int>, std::pair<int, int>> sched;
+  struct Sched {
+    int nb, block, ncol;   // line blocks, threads per block, column blocks (0: options)
+  };
+  std::map<std::tuple<int, int, int>, Sched> sched;
 };
 
 #define YG_CHECK(h, call)                                                          \
@@ -176,10 +179,15 @@ struct Batch {
     M.vlen = (int)(((2LL * P.nx * P.ny + 3) / 4) * 4);
     M.coff = M.toff = M.woff = 0;
     M.nb = 0;
+    M.ncol = 0;
     tuned_block = 0;
     if (kmode == YG_MODE_MIXED && opts.k.precon == 2 && opts.k.ilu_blocks < 0) {
       auto it = h->sched.find({P.nx, P.ny, group});
-      if (it != h->sched.end()) { M.nb = it->second.first; tuned_block = it->second.second; }
+      if (it != h->sched.end()) {
+        M.nb = it->second.nb;
+        M.ncol = it->second.ncol;
+        tuned_block = it->second.block;
+      }
     }
     return YGPU_OK;
   }

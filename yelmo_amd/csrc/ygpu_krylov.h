@@ -30,6 +30,8 @@ struct YgKrylovOpts {
   int initx_zeros;
   int conv_cond;    // 0 ||r||/||r0||, 1 ||r||/||b||
   int ir_max;       // mixed mode: refinement sweeps
+  int ilu_cols;     // block-Jacobi ILU(0): column blocks per line block (tiles of
+                    // lines x columns; 1: whole lines)
   int ilu_blocks;   // 0 exact ILU(0); n > 0 block-Jacobi ILU(0) over n line blocks
                     // (at least one per 64 lines); -1 automatic: exact in the
                     // FP64 mode, one block per 64 lines in the mixed mode

@@ -37,7 +37,9 @@ namespace {
 //   blocked  (block-Jacobi ILU(0)): the lines are split into nb line blocks
 //            of lpw <= 64 lines, one per wave; couplings between blocks are
 //            dropped from the factorization, so waves never wait for each
-//            other inside a sweep.
+//            other inside a sweep.  A line block may further be cut into
+//            tiles of consecutive columns (one per wave in the sweeps, which
+//            then need mcol + 2(lpw-1) steps instead of ny + 2(lpw-1)).
 struct Grp;
 struct Ctx {
   const YgMember* mb;
@@ -51,6 +53,11 @@ struct Ctx {
   int coupled;                // waves exchange boundary values every step
   int nwl;                    // waves that own lines
   int lpw;                    // lines per wave (the line block of a wave)
+  // blocked scheme: each line block is split into ncol tiles of mcol
+  // consecutive columns (the last one narrower); couplings between tiles
+  // are dropped like those between line blocks, and each wave of a sweep
+  // runs one tile (ncol = 1: whole lines)
+  int ncol, mcol;
   long long* tm;              // phase timers (thread 0), may be null
   // Offsets (values) from a factor pointer to the U part of its line-block
   // factors: 8 ncell for a whole member (FP32/FP64 and BF16 alike), and, for
@@ -101,6 +108,14 @@ YG_DEV T& lds(int k) { return reinterpret_cast<T*>(yg_dyn_lds)[k]; }
 __shared__ int yg_soff[YG_MAX_STEPS + 1];
 __shared__ double yg_ring[16][3][10];
 __shared__ double yg_red[16 * 4];   // block reductions
+
+// Column tiles of the blocked scheme: want tiles per line block, at most
+// `fit` (the waves per line block), at least 8 columns each.
+YG_DEV void yg_tiles(int want, int fit, int ny, Ctx& c) {
+  int n = max(1, min(want, min(fit, ny / 8)));
+  c.mcol = (ny + n - 1) / n;
+  c.ncol = (ny + c.mcol - 1) / c.mcol;
+}
 
 // ---- member groups ------------------------------------------------------
 // G workgroups (one per CU) can share one member's solve.  Workgroup b runs
@@ -399,7 +414,22 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
           a[s] = (double)lds<TA>(sb + (2 * s) * stride + l);
           b[s] = (double)lds<TA>(sb + (2 * s + 1) * stride + l);
         }
-        const uint32_t mx = m & 0x1ffu, my = (m >> 16) & 0x1ffu;
+        uint32_t mx = m & 0x1ffu, my = (m >> 16) & 0x1ffu;
+        if (!c.coupled && c.ncol > 1) {
+          // tile edges: couplings to the column before the tile's first
+          // (ux row: ux(i,j-1), uy(i+1,j-1), uy(i,j-1); uy row: uy(i,j-1))
+          // and after its last (ux row: ux(i,j+1); uy row: uy(i,j+1),
+          // ux(i,j+1), ux(i-1,j+1)) are dropped
+          const int j = t - 2 * c.line, jt = j % c.mcol;
+          if (jt == 0 && j > 0) {
+            a[4] = a[7] = a[8] = 0.0; b[2] = 0.0;
+            mx &= ~((1u << 4) | (1u << 7) | (1u << 8)); my &= ~(1u << 2);
+          }
+          if (jt == c.mcol - 1 && j + 1 < c.ny) {
+            a[3] = 0.0; b[1] = b[6] = b[7] = 0.0;
+            mx &= ~(1u << 3); my &= ~((1u << 1) | (1u << 6) | (1u << 7));
+          }
+        }
 #define PX(s) ((mx >> (s)) & 1u)
 #define PY(s) ((my >> (s)) & 1u)
         const URow& O = o1;
@@ -454,10 +484,13 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
           Fg[14 * nc + q] = (TA)X.y6; Fg[15 * nc + q] = (TA)X.y1; Fg[16 * nc + q] = (TA)X.y3;
           Fg[17 * nc + q] = (TA)X.yd;
         } else {
-          // block-skewed AoS: block of this wave, its local step and slot
+          // tile-skewed AoS: the tiles of a line block follow each other;
+          // within a tile the cells go by its wavefront steps
           const int i0 = w * c.lpw, k = c.line - i0, Lb = min(c.lpw, c.nx - i0);
-          const int j = t - 2 * c.line, tb = j + 2 * k;
-          const size_t qb = (size_t)i0 * c.ny + blk_soff(tb, Lb, c.ny) + k - glo_of(tb, c.ny);
+          const int j = t - 2 * c.line, j0 = j - j % c.mcol, M = min(c.mcol, c.ny - j0);
+          const int tb = j - j0 + 2 * k;
+          const size_t qb = (size_t)i0 * c.ny + (size_t)Lb * j0 + blk_soff(tb, Lb, M) + k -
+                            glo_of(tb, M);
           YG_GLOBAL TA* fl = Fg + 8 * qb;
           fl[0] = (TA)a[2]; fl[1] = (TA)a[4]; fl[2] = (TA)a[8]; fl[3] = (TA)b[8];
           fl[4] = (TA)b[4]; fl[5] = (TA)b[7]; fl[6] = (TA)b[2]; fl[7] = (TA)b[5];
@@ -877,14 +910,17 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
   const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
   const int nc = uni(c.ncell);
-  if (w >= nwl) return;
-  const int i0 = w * lpw, L = min(lpw, nx - i0);
-  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
+  if (w >= nwl * ncol) return;
+  // the wave's tile: line block wb, columns [j0, j0 + M)
+  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
+  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
+  const BlockGeom G{L, M + 2 * (L - 1), M};
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F, (size_t)8 * nc);
   const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
-  const int lane_q = i0 * ny + lane;               // + off: block-skewed slot
-  const int lane_n = (i0 + lane) * ny - 2 * lane;  // + t: natural cell
+  const int lane_q = i0 * ny + L * j0 + lane;           // + off: tile-skewed slot
+  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;  // + t: natural cell
   T f[D][8];
   V2 r[D];
   StepQ qc = step_q(G, 0), qn = step_q(G, 0);   // the step computed, the step loaded
@@ -930,14 +966,17 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
   const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
   const int nc = uni(c.ncell);
-  if (w >= nwl) return;
-  const int i0 = w * lpw, L = min(lpw, nx - i0);
-  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
+  if (w >= nwl * ncol) return;
+  // the wave's tile: line block wb, columns [j0, j0 + M)
+  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
+  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
+  const BlockGeom G{L, M + 2 * (L - 1), M};
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + c.fu, (size_t)12 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
-  const int lane_q = i0 * ny + lane;
-  const int lane_n = (i0 + lane) * ny - 2 * lane;
+  const int lane_q = i0 * ny + L * j0 + lane;
+  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;
   T f[D][12];
   V2 yv[D];
   StepQ qc = step_q(G, G.Tb - 1), qn = step_q(G, G.Tb - 1);
@@ -984,14 +1023,17 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
   const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
   const int nc = uni(c.ncell);
-  if (w >= nwl) return;
-  const int i0 = w * lpw, L = min(lpw, nx - i0);
-  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
+  if (w >= nwl * ncol) return;
+  // the wave's tile: line block wb, columns [j0, j0 + M)
+  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
+  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
+  const BlockGeom G{L, M + 2 * (L - 1), M};
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB, (size_t)8 * nc);
   const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
-  const int lane_q = i0 * ny + lane;
-  const int lane_n = (i0 + lane) * ny - 2 * lane;
+  const int lane_q = i0 * ny + L * j0 + lane;
+  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;
   uint4 fw[D];
   V2 r[D];
   StepQ qc = step_q(G, 0), qn = step_q(G, 0);
@@ -1035,15 +1077,18 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
   const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
   const int nc = uni(c.ncell);
-  if (w >= nwl) return;
-  const int i0 = w * lpw, L = min(lpw, nx - i0);
-  const BlockGeom G{L, ny + 2 * (L - 1), ny};
+  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
+  if (w >= nwl * ncol) return;
+  // the wave's tile: line block wb, columns [j0, j0 + M)
+  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
+  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
+  const BlockGeom G{L, M + 2 * (L - 1), M};
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + c.fu16, (size_t)8 * nc);
   const __amdgpu_buffer_rsrc_t rD = yg_rsrc(FB + c.fd16, (size_t)4 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
-  const int lane_q = i0 * ny + lane;
-  const int lane_n = (i0 + lane) * ny - 2 * lane;
+  const int lane_q = i0 * ny + L * j0 + lane;
+  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;
   uint4 gw[D];   // the 8 scaled off-diagonal U entries
   V2 dg[D];      // ux-row and uy-row inverse diagonals
   V2 yv[D];
@@ -1555,6 +1600,8 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     // a member's own schedule (autotuned) before the options' rule
     int nb = mb.nb != 0 ? mb.nb : a.opts.ilu_blocks;
     if (nb < 0) nb = MODE == YG_MODE_MIXED ? (mb.nx + 63) / 64 : 0;
+    c.ncol = 1;
+    c.mcol = mb.ny;
     if (nb == 0) {
       c.coupled = mb.nx > 64;
       c.nwl = (mb.nx + 63) / 64;
@@ -1569,6 +1616,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
       c.nwl = (mb.nx + lpw - 1) / lpw;
       const int i = w * lpw + lane;
       c.line = (w < c.nwl && lane < lpw && i < mb.nx) ? i : -1;
+      yg_tiles(mb.ncol != 0 ? mb.ncol : a.opts.ilu_cols, wmax / c.nwl, mb.ny, c);
     }
   }
   __shared__ long long s_tm[8];
@@ -1764,6 +1812,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     cm.mask = a.mask + mb.coff;
     cm.maxcnt = mb.maxcnt;
     cm.line = -1; cm.coupled = 0; cm.nwl = 0; cm.lpw = lpw;
+    cm.ncol = 1; cm.mcol = ny;
     cm.tm = a.timers ? s_tm : nullptr;
     cm.fu = cm.fu16 = 8ll * ncell;
     cm.fd16 = 16ll * ncell;
@@ -1779,6 +1828,8 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
       const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = w * lpw + lane;
       cr.line = (w < cr.nwl && lane < lpw && i < Lg) ? i : -1;
     }
+    // the same tiles on every rank: as many as the fullest rank's waves allow
+    yg_tiles(mb.ncol != 0 ? mb.ncol : a.opts.ilu_cols, (NT / 64) / nbr, ny, cr);
     cr.fu = 8ll * ncell + 4ll * cm.cb;     // the member's U part from this part's L part
     cr.fu16 = 8ll * ncell;
     cr.fd16 = 16ll * ncell - 4ll * cm.cb;

@@ -25,7 +25,7 @@ using namespace yg;
 namespace {
 
 struct Candidate {
-  int nb, block;
+  int nb, block, ncol = 1;
   double ms = 0.0;
   int iters = 0, status = -1;
 };
@@ -88,7 +88,7 @@ int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_proble
   std::vector<Candidate> cand = candidates(prob->nx, G);
   bool warm = false;
   for (Candidate& c : cand) {
-    for (YgMember& M : B.mem) M.nb = c.nb;
+    for (YgMember& M : B.mem) { M.nb = c.nb; M.ncol = c.ncol; }
     YG_CHECK(h, hipMemcpyAsync(B.d_mem, B.mem.data(), sizeof(YgMember) * copies,
                                hipMemcpyHostToDevice, h->stream));
     if (G == 1 && B.block != c.block) {   // a captured graph holds the block size
@@ -115,7 +115,7 @@ int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_proble
   win->status = -1;
   for (const Candidate& c : cand)
     if (c.status == 0 && (win->status != 0 || c.ms < win->ms)) *win = c;
-  if (win->status == 0) h->sched[{prob->nx, prob->ny, G}] = {win->nb, win->block};
+  if (win->status == 0) h->sched[{prob->nx, prob->ny, G}] = {win->nb, win->block, win->ncol};
   return YGPU_OK;
 }
 
@@ -176,7 +176,7 @@ int ygpu_set_schedule(ygpu_handle* h, int32_t nx, int32_t ny, int32_t ilu_blocks
   if (block != 0 && block != 512 && block != 1024) return YGPU_ERR_ARG;
   for (int G : {1, 2, 4, 8}) {
     if (ilu_blocks <= 0) h->sched.erase({nx, ny, G});
-    else h->sched[{nx, ny, G}] = {ilu_blocks, G > 1 ? YG_GROUP_THREADS : block};
+    else h->sched[{nx, ny, G}] = {ilu_blocks, G > 1 ? YG_GROUP_THREADS : block, 0};
   }
   return YGPU_OK;
 }
