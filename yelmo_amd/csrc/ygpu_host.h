@@ -608,9 +608,13 @@ struct Batch {
     } else {
       YG_CHECK(h, tmp.alloc(&y64, n));
     }
+    const std::vector<int> bm = yg_spmv_block_members(coffs.data(), nmem, total_cells);
+    int* d_bm = nullptr;
+    YG_CHECK(h, tmp.alloc(&d_bm, bm.size()));
+    YG_CHECK(h, hipMemcpyAsync(d_bm, bm.data(), sizeof(int) * bm.size(), hipMemcpyHostToDevice, st));
     auto launch = [&]() {
-      if (fp32) yg_launch_spmv<float>(G, d_A32, x32, y32, wrapx, wrapy, st);
-      else yg_launch_spmv<double>(G, d_A64, d_x, y64, wrapx, wrapy, st);
+      if (fp32) yg_launch_spmv<float>(G, d_bm, d_A32, x32, y32, wrapx, wrapy, st);
+      else yg_launch_spmv<double>(G, d_bm, d_A64, d_x, y64, wrapx, wrapy, st);
     };
     launch();   // untimed first launch
     hipEvent_t e0, e1;

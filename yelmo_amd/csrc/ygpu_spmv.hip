@@ -4,7 +4,9 @@
 // the CSR boundary).  One thread per cell computes both rows of its cell
 // from the cell's 20-value record and the 7 neighbour velocity pairs.  It is
 // HBM-bound: the record (80 B FP32 / 160 B FP64) is read once, through LDS,
-// the neighbour pairs mostly come from cache, and y is written once.
+// the neighbour pairs mostly come from cache, and y is written once.  The
+// members of a batch are contiguous; each block finds its members from a
+// host table (the member of every block's first cell).
 #include "ygpu_spmv.h"
 
 namespace {
@@ -15,50 +17,46 @@ template <>
 struct Pair<double> { using type = double2; };
 
 constexpr int kSpmvBlock = 256;
-constexpr int kSpmvSegs = 1024;   // member ranges searched in LDS (more: in global memory)
 
 template <typename T>
-__global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, const T* __restrict__ A,
+__global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, const int* __restrict__ blk_m,
+                                                             const T* __restrict__ A,
                                                              const T* __restrict__ x,
                                                              T* __restrict__ y, int wrapx,
                                                              int wrapy) {
   using P2 = typename Pair<T>::type;
   // The block's records are contiguous: stage them through LDS with 16-byte
   // loads of consecutive addresses across the lanes (a per-cell load would
-  // put the lanes of one instruction 80 or 160 bytes apart)
+  // put the lanes of one instruction 80 or 160 bytes apart), non-temporal
+  // (each record is read once; x's neighbour pairs are what the caches keep).
   __shared__ uint4 rec[kSpmvBlock * YG_NA * sizeof(T) / 16];
-  // the occupied-range list too, so that each thread's member search is a
-  // search in LDS instead of a chain of dependent global loads
-  __shared__ long long s_seg[kSpmvSegs];
   const long long g0 = (long long)blockIdx.x * kSpmvBlock;
+  // the members of the block's cells run from m0 (its first cell's) to at
+  // most m1 (the next block's first cell's), from a host table; the first
+  // member's geometry is loaded before the records are waited for
+  const int m0 = blk_m[blockIdx.x], m1 = blk_m[blockIdx.x + 1];
+  const YgMember M0 = G.members[m0];
+  const long long c00 = G.coffs[m0];
   const long long nblk = min((long long)kSpmvBlock, G.total_cells - g0);
   const int nchunk = (int)(nblk * YG_NA * sizeof(T) / 16);
-  const uint4* src = reinterpret_cast<const uint4*>(A + (size_t)YG_NA * g0);
-  for (int k = threadIdx.x; k < nchunk; k += kSpmvBlock) rec[k] = src[k];
-  const bool lds_seg = G.nseg <= kSpmvSegs && !G.segm;
-  if (lds_seg)
-    for (int k = threadIdx.x; k < G.nseg; k += kSpmvBlock) s_seg[k] = G.seg[k];
+  typedef unsigned V4u __attribute__((ext_vector_type(4)));
+  const V4u* src = reinterpret_cast<const V4u*>(A + (size_t)YG_NA * g0);
+  for (int k = threadIdx.x; k < nchunk; k += kSpmvBlock) {
+    const V4u v = __builtin_nontemporal_load(src + k);
+    rec[k] = make_uint4(v.x, v.y, v.z, v.w);
+  }
   __syncthreads();
   const long long gq = g0 + threadIdx.x;
   if (gq >= G.total_cells) return;
-  int m;
-  if (lds_seg) {
-    int lo = 0, hi = G.nseg - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_seg[mid] <= gq) lo = mid; else hi = mid - 1;
-    }
-    m = lo;
-    const YgMember& mm = G.members[m];
-    if (gq < s_seg[lo] || gq - s_seg[lo] >= (long long)mm.nx * mm.ny) return;
-  } else {
-    m = yg_member_at(G, gq);
-    if (m < 0) return;
+  int m = m0, nx = M0.nx, ny = M0.ny;
+  long long c0 = c00;
+  if (m1 != m0) {   // the block spans several members
+    while (m < m1 && gq >= G.coffs[m + 1]) ++m;
+    const YgMember& mb = G.members[m];
+    nx = mb.nx; ny = mb.ny; c0 = G.coffs[m];
   }
-  const YgMember& mb = G.members[m];
-  const long long c0 = G.coffs[m];
-  const int nx = mb.nx, ny = mb.ny;
   const int q = (int)(gq - c0);
+  if (q >= nx * ny) return;   // between members (none in a contiguous batch)
   const int i = q / ny, j = q - i * ny;
   // neighbour cells in the row order n = i*ny + j; an index outside a
   // non-periodic domain only meets a zero record value, so it is clamped
@@ -92,17 +90,30 @@ __global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, cons
 }  // namespace
 
 template <typename T>
-void yg_launch_spmv(const YgBatchGeom& G, const T* A, const T* x, T* y, int wrapx, int wrapy,
-                    hipStream_t st) {
+void yg_launch_spmv(const YgBatchGeom& G, const int* blk_m, const T* A, const T* x, T* y,
+                    int wrapx, int wrapy, hipStream_t st) {
   const unsigned nb = (unsigned)((G.total_cells + kSpmvBlock - 1) / kSpmvBlock);
-  hipLaunchKernelGGL(k_spmv_records<T>, dim3(nb), dim3(kSpmvBlock), 0, st, G, A, x, y, wrapx,
-                     wrapy);
+  hipLaunchKernelGGL(k_spmv_records<T>, dim3(nb), dim3(kSpmvBlock), 0, st, G, blk_m, A, x, y,
+                     wrapx, wrapy);
 }
 
-template void yg_launch_spmv<float>(const YgBatchGeom&, const float*, const float*, float*, int,
-                                    int, hipStream_t);
-template void yg_launch_spmv<double>(const YgBatchGeom&, const double*, const double*, double*,
-                                     int, int, hipStream_t);
+std::vector<int> yg_spmv_block_members(const long long* coffs, int nmem, long long total) {
+  const long long nb = (total + kSpmvBlock - 1) / kSpmvBlock;
+  std::vector<int> t(nb + 1);
+  int m = 0;
+  for (long long b = 0; b < nb; ++b) {
+    const long long g = b * kSpmvBlock;
+    while (m + 1 < nmem && g >= coffs[m + 1]) ++m;
+    t[b] = m;
+  }
+  t[nb] = nmem - 1;
+  return t;
+}
+
+template void yg_launch_spmv<float>(const YgBatchGeom&, const int*, const float*, const float*,
+                                    float*, int, int, hipStream_t);
+template void yg_launch_spmv<double>(const YgBatchGeom&, const int*, const double*, const double*,
+                                     double*, int, int, hipStream_t);
 
 namespace {
 __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, long long n) {
