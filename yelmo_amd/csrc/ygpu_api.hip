@@ -611,12 +611,13 @@ int ygpu_csr_spmv(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
   G.members = d_mem; G.coffs = d_coff; G.zoffs = d_coff; G.soff = nullptr; G.nmem = 1;
   G.total_cells = n / 2; G.seg = d_coff; G.segm = nullptr; G.nseg = 1;
   const long long coff0 = 0;
-  const std::vector<int> bm = yg_spmv_block_members(&coff0, 1, n / 2);
-  int* d_bm = nullptr;
-  YG_CHECK(h, ar.alloc(&d_bm, bm.size()));
-  YG_CHECK(h, hipMemcpyAsync(d_bm, bm.data(), sizeof(int) * bm.size(), hipMemcpyHostToDevice, st));
+  const std::vector<int> ch = yg_chunk_members(&coff0, 1, n / 2);
+  int* d_ch = nullptr;
+  YG_CHECK(h, ar.alloc(&d_ch, ch.size()));
+  YG_CHECK(h, hipMemcpyAsync(d_ch, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice, st));
+  G.chunk_seg = d_ch;
   // the CSR boundary is non-periodic: a wrapped coupling is outside the stencil above
-  yg_launch_spmv<double>(G, d_bm, d_A, d_x, d_y, 0, 0, st);
+  yg_launch_spmv<double>(G, d_A, d_x, d_y, 0, 0, st);
   YG_CHECK(h, hipGetLastError());
   YG_CHECK(h, hipMemcpyAsync(y, d_y, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   YG_CHECK(h, hipStreamSynchronize(st));

@@ -79,14 +79,25 @@ struct YgBatchGeom {
   const long long* seg;
   const int* segm;
   int nseg;
+  // For a batch (seg = coffs): the member of the first cell of every chunk
+  // of YG_CHUNK cells, and nmem - 1 after the last chunk (null: search).
+  const int* chunk_seg;
 };
+#define YG_CHUNK 256
 
 // member owning cell gc, or -1 for a cell outside every member's domain
 YG_DEV int yg_member_at(const YgBatchGeom& G, long long gc) {
-  int lo = 0, hi = G.nseg - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (G.seg[mid] <= gc) lo = mid; else hi = mid - 1;
+  int lo = 0;
+  if (G.chunk_seg) {
+    // from the chunk's first member forward (a chunk rarely holds a member start)
+    lo = G.chunk_seg[gc / YG_CHUNK];
+    while (lo + 1 < G.nseg && G.seg[lo + 1] <= gc) ++lo;
+  } else {
+    int hi = G.nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (G.seg[mid] <= gc) lo = mid; else hi = mid - 1;
+    }
   }
   if (G.nseg <= 0 || gc < G.seg[lo]) return -1;
   const int m = G.segm ? G.segm[lo] : lo;

@@ -262,10 +262,17 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
                       RS(0.25) * (s * s);
         if (!p.diva) eps_sq_n[q] = eps_sq_n[q] + eps_0_sq;
       }
+      // SSA: the strain-rate term of a node is the same on every level, so
+      // its power is taken once per node (the same value the level loop
+      // would compute nz times)
+      R epow[4];
+      if (!p.diva) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) epow[q] = ypow<R>(eps_sq_n[q], p1);
+      }
       for (int k = 0; k < nz; ++k) {
         R ATTn[4];
         gq_nodes<R>(0, ATTn, ATT + k * nc, all, nx, i, j, im1, ip1, jm1, jp1);
-        R e[4];
         if (p.diva) {
           R dzx[4], dzy[4];
           // DIVA: vertical shear at the nodes from the acx/acy shear fields
@@ -273,14 +280,12 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
           gq_nodes<R>(2, dzy, f.duydz + cc.z0 + k * nc, ay, nx, i, j, im1, ip1, jm1, jp1);
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            e[q] = eps_sq_n[q] + R(0.25) * (dzx[q] * dzx[q]) + R(0.25) * (dzy[q] * dzy[q]) + eps_0_sq;
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) e[q] = eps_sq_n[q];
+            epow[q] = ypow<R>(eps_sq_n[q] + R(0.25) * (dzx[q] * dzx[q]) +
+                                  R(0.25) * (dzy[q] * dzy[q]) + eps_0_sq, p1);
         }
         R sum = R(0);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sum += (RS(0.5) * ypow<R>(e[q], p1) * ypow<R>(ATTn[q], p2)) * R(1);
+        for (int q = 0; q < 4; ++q) sum += (RS(0.5) * epow[q] * ypow<R>(ATTn[q], p2)) * R(1);
         visc[c + k * nc] = sum / R(4);
       }
     } else {

@@ -100,6 +100,7 @@ struct Batch {
   // device
   YgMember* d_mem = nullptr;
   long long *d_coffs = nullptr, *d_zoffs = nullptr;
+  int* d_chunk = nullptr;   // member of every YG_CHUNK-cell chunk's first cell
   int* d_soff = nullptr;
   uint32_t* d_mask = nullptr;
   double *d_A64 = nullptr, *d_b = nullptr, *d_x = nullptr, *d_F64 = nullptr, *d_W64 = nullptr;
@@ -116,6 +117,7 @@ struct Batch {
     G.members = d_mem; G.coffs = d_coffs; G.zoffs = d_zoffs; G.soff = d_soff;
     G.nmem = nmem; G.total_cells = total_cells;
     G.seg = d_coffs; G.segm = nullptr; G.nseg = nmem;
+    G.chunk_seg = d_chunk;
     return G;
   }
 
@@ -350,6 +352,9 @@ struct Batch {
     YG_CHECK(h, hipMemcpyAsync(d_coffs, mc.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
     YG_CHECK(h, hipMemcpyAsync(d_zoffs, mz.data(), sizeof(long long) * (nmem + 1), hipMemcpyHostToDevice, h->stream));
     YG_CHECK(h, hipMemcpyAsync(d_soff, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, h->stream));
+    const std::vector<int> ch = yg_chunk_members(coffs.data(), nmem, total_cells);
+    YG_CHECK(h, ar.alloc(&d_chunk, ch.size()));
+    YG_CHECK(h, hipMemcpyAsync(d_chunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice, h->stream));
     YG_CHECK(h, hipStreamSynchronize(h->stream));
     return YGPU_OK;
   }
@@ -608,13 +613,9 @@ struct Batch {
     } else {
       YG_CHECK(h, tmp.alloc(&y64, n));
     }
-    const std::vector<int> bm = yg_spmv_block_members(coffs.data(), nmem, total_cells);
-    int* d_bm = nullptr;
-    YG_CHECK(h, tmp.alloc(&d_bm, bm.size()));
-    YG_CHECK(h, hipMemcpyAsync(d_bm, bm.data(), sizeof(int) * bm.size(), hipMemcpyHostToDevice, st));
     auto launch = [&]() {
-      if (fp32) yg_launch_spmv<float>(G, d_bm, d_A32, x32, y32, wrapx, wrapy, st);
-      else yg_launch_spmv<double>(G, d_bm, d_A64, d_x, y64, wrapx, wrapy, st);
+      if (fp32) yg_launch_spmv<float>(G, d_A32, x32, y32, wrapx, wrapy, st);
+      else yg_launch_spmv<double>(G, d_A64, d_x, y64, wrapx, wrapy, st);
     };
     launch();   // untimed first launch
     hipEvent_t e0, e1;

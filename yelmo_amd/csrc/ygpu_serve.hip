@@ -442,6 +442,7 @@ struct Server : YgServerBase {
     G.nmem = L.ns;
     G.total_cells = L.hi_cell;
     G.seg = d_seg + L.s0; G.segm = d_segm + L.s0; G.nseg = L.nseg;
+    G.chunk_seg = nullptr;   // ranges change with admissions: searched
     return G;
   }
   YgFields<R> fields(const Lane& L) const {

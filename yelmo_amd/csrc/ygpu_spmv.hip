@@ -5,8 +5,8 @@
 // from the cell's 20-value record and the 7 neighbour velocity pairs.  It is
 // HBM-bound: the record (80 B FP32 / 160 B FP64) is read once, through LDS,
 // the neighbour pairs mostly come from cache, and y is written once.  The
-// members of a batch are contiguous; each block finds its members from a
-// host table (the member of every block's first cell).
+// members of a batch are contiguous; each block finds its members from the
+// batch's chunk table (the member of every chunk's first cell).
 #include "ygpu_spmv.h"
 
 namespace {
@@ -16,11 +16,10 @@ struct Pair { using type = float2; };
 template <>
 struct Pair<double> { using type = double2; };
 
-constexpr int kSpmvBlock = 256;
+constexpr int kSpmvBlock = YG_CHUNK;   // one block per chunk of the member table
 
 template <typename T>
-__global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, const int* __restrict__ blk_m,
-                                                             const T* __restrict__ A,
+__global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, const T* __restrict__ A,
                                                              const T* __restrict__ x,
                                                              T* __restrict__ y, int wrapx,
                                                              int wrapy) {
@@ -32,9 +31,9 @@ __global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, cons
   __shared__ uint4 rec[kSpmvBlock * YG_NA * sizeof(T) / 16];
   const long long g0 = (long long)blockIdx.x * kSpmvBlock;
   // the members of the block's cells run from m0 (its first cell's) to at
-  // most m1 (the next block's first cell's), from a host table; the first
-  // member's geometry is loaded before the records are waited for
-  const int m0 = blk_m[blockIdx.x], m1 = blk_m[blockIdx.x + 1];
+  // most m1 (the next block's first cell's), from the batch's chunk table;
+  // the first member's geometry is loaded before the records are waited for
+  const int m0 = G.chunk_seg[blockIdx.x], m1 = G.chunk_seg[blockIdx.x + 1];
   const YgMember M0 = G.members[m0];
   const long long c00 = G.coffs[m0];
   const long long nblk = min((long long)kSpmvBlock, G.total_cells - g0);
@@ -90,19 +89,19 @@ __global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, cons
 }  // namespace
 
 template <typename T>
-void yg_launch_spmv(const YgBatchGeom& G, const int* blk_m, const T* A, const T* x, T* y,
-                    int wrapx, int wrapy, hipStream_t st) {
+void yg_launch_spmv(const YgBatchGeom& G, const T* A, const T* x, T* y, int wrapx, int wrapy,
+                    hipStream_t st) {
   const unsigned nb = (unsigned)((G.total_cells + kSpmvBlock - 1) / kSpmvBlock);
-  hipLaunchKernelGGL(k_spmv_records<T>, dim3(nb), dim3(kSpmvBlock), 0, st, G, blk_m, A, x, y,
-                     wrapx, wrapy);
+  hipLaunchKernelGGL(k_spmv_records<T>, dim3(nb), dim3(kSpmvBlock), 0, st, G, A, x, y, wrapx,
+                     wrapy);
 }
 
-std::vector<int> yg_spmv_block_members(const long long* coffs, int nmem, long long total) {
-  const long long nb = (total + kSpmvBlock - 1) / kSpmvBlock;
+std::vector<int> yg_chunk_members(const long long* coffs, int nmem, long long total) {
+  const long long nb = (total + YG_CHUNK - 1) / YG_CHUNK;
   std::vector<int> t(nb + 1);
   int m = 0;
   for (long long b = 0; b < nb; ++b) {
-    const long long g = b * kSpmvBlock;
+    const long long g = b * YG_CHUNK;
     while (m + 1 < nmem && g >= coffs[m + 1]) ++m;
     t[b] = m;
   }
@@ -110,10 +109,10 @@ std::vector<int> yg_spmv_block_members(const long long* coffs, int nmem, long lo
   return t;
 }
 
-template void yg_launch_spmv<float>(const YgBatchGeom&, const int*, const float*, const float*,
-                                    float*, int, int, hipStream_t);
-template void yg_launch_spmv<double>(const YgBatchGeom&, const int*, const double*, const double*,
-                                     double*, int, int, hipStream_t);
+template void yg_launch_spmv<float>(const YgBatchGeom&, const float*, const float*, float*, int,
+                                    int, hipStream_t);
+template void yg_launch_spmv<double>(const YgBatchGeom&, const double*, const double*, double*,
+                                     int, int, hipStream_t);
 
 namespace {
 __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, long long n) {
