@@ -54,7 +54,7 @@ struct Opts {
   int ny_hint = 0;
   int prec = -1;  // -1 unset, 0 fp64, 2 mixed
   int group = 0;  // workgroups per member: 0 automatic, 1 one (no groups), 2, 4 or 8
-  int group_threads = 512;   // threads per group workgroup: 512 (one per CU) or 256 (two)
+  int group_threads = 0;   // threads per group workgroup: 512 (one per CU), 256 (two), 0 automatic
 };
 
 int parse_opts(const char* s, Opts& o, std::string& err);
@@ -248,8 +248,10 @@ struct Batch {
 
   // Workgroups per member: the member-group kernel covers the mixed mode
   // with BiCGSTAB and block-Jacobi ILU(0).  Automatic: the largest G of 8,
-  // 4, 2 whose groups (a multiple of 8, one per member) fit the CUs and
-  // leave every workgroup at least 4096 cells of the largest member.
+  // 4, 2 that leaves every workgroup at least 4096 cells of the largest
+  // member and whose groups (a multiple of 8, one per member) fit the CUs
+  // with 512-thread workgroups (one per CU) or else with 256-thread ones
+  // (two per CU).
   int pick_group(int maxnx, long long maxcells) {
     group = 1;
     ngroups = 0;
@@ -266,18 +268,25 @@ struct Batch {
       h->err = "-ygpu_group: member groups run the mixed precision, BiCGSTAB, block ILU(0) solve";
       return YGPU_ERR_UNSUPPORTED;
     }
-    int ncu = 0;
-    YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
-    gthreads = opts.group_threads;
-    ncu *= 512 / gthreads;   // workgroups the CUs hold at once
+    int ncu0 = 0;
+    YG_CHECK(h, hipDeviceGetAttribute(&ncu0, hipDeviceAttributeMultiprocessorCount, h->device));
     const int groups8 = ((nmem + 7) / 8) * 8;
-    int G = want;
+    // do all groups of G workgroups of `thr` threads fit the CUs at once
+    auto all_fit = [&](int g, int thr) { return groups8 * g <= ncu0 * (512 / thr); };
+    int G = want, thr = opts.group_threads;
     if (G == 0) {
       G = 1;
-      for (int g : {8, 4, 2})
-        if (groups8 * g <= ncu && maxcells / g >= 4096) { G = g; break; }
+      for (int g : {8, 4, 2}) {
+        if (maxcells / g < 4096) continue;
+        if (thr != 256 && all_fit(g, 512)) { G = g; thr = 512; break; }
+        if (thr != 512 && all_fit(g, 256)) { G = g; thr = 256; break; }
+      }
       if (G == 1) return YGPU_OK;
+    } else if (thr == 0) {
+      thr = all_fit(G, 512) ? 512 : 256;
     }
+    gthreads = thr;
+    const int ncu = ncu0 * (512 / gthreads);   // workgroups the CUs hold at once
     if (8 * G > ncu) { h->err = "-ygpu_group: more workgroups than CUs"; return YGPU_ERR_ARG; }
     if (maxnx > 64 * G * (gthreads / 64)) {
       h->err = "-ygpu_group: a member has more lines than the group's waves can sweep";
@@ -714,7 +723,7 @@ struct Batch {
         YG_CHECK(h, hipEventElapsedTime(&kms, k0, k1));
         h->stats.krylov_launches += 1;
         h->stats.krylov_group = group;
-        h->stats.krylov_block = group > 1 ? YG_GROUP_THREADS : block;
+        h->stats.krylov_block = group > 1 ? gthreads : block;
         h->stats.krylov_ms += kms;
         const bool mixed = kmode == YG_MODE_MIXED;
         for (int k : active) {
