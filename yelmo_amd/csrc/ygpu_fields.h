@@ -42,6 +42,11 @@ struct YgFields {
   const R *c_bed, *f_slide, *taud_acx, *taud_acy, *taul_acx, *taul_acy, *H_ice, *f_ice, *H_grnd,
       *f_grnd, *z_bed, *z_sl;
   const R* ATT;      // 3D, per member at zoff
+  // ATT^(-1/n) at the 4 quadrature nodes of every cell and level (node q,
+  // level k at (q nz + k) ncell + cell, per member at 4 zoff), filled once
+  // per solve by k_init for visc_method 1 (ATT is an input of the solve);
+  // null: computed in every Picard iteration
+  R* attp;
   const R* zeta;     // nz per member at member index * zmax
   int zstride;
   const int32_t *mx, *my;

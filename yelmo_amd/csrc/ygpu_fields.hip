@@ -270,9 +270,19 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
 #pragma unroll
         for (int q = 0; q < 4; ++q) epow[q] = ypow<R>(eps_sq_n[q], p1);
       }
+      const R* attp = f.attp ? f.attp + 4 * cc.z0 + c : nullptr;
       for (int k = 0; k < nz; ++k) {
-        R ATTn[4];
-        gq_nodes<R>(0, ATTn, ATT + k * nc, all, nx, i, j, im1, ip1, jm1, jp1);
+        // ATT^p2 at the nodes: filled once per solve, or computed here
+        R ATTp[4];
+        if (attp) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) ATTp[q] = attp[(q * nz + k) * nc];
+        } else {
+          R ATTn[4];
+          gq_nodes<R>(0, ATTn, ATT + k * nc, all, nx, i, j, im1, ip1, jm1, jp1);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) ATTp[q] = ypow<R>(ATTn[q], p2);
+        }
         if (p.diva) {
           R dzx[4], dzy[4];
           // DIVA: vertical shear at the nodes from the acx/acy shear fields
@@ -285,7 +295,7 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
         }
         R sum = R(0);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sum += (RS(0.5) * epow[q] * ypow<R>(ATTn[q], p2)) * R(1);
+        for (int q = 0; q < 4; ++q) sum += (RS(0.5) * epow[q] * ATTp[q]) * R(1);
         visc[c + k * nc] = sum / R(4);
       }
     } else {
@@ -1261,6 +1271,23 @@ __global__ void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
   }
   f.err_x[g] = R(1);
   f.err_y[g] = R(1);
+  // ATT^(-1/n) at the quadrature nodes for visc_method 1 (the same values
+  // k_visc computes from ATT; ATT does not change during a solve)
+  if (f.attp && p.visc_method == 1) {
+    const int nz = cc.nz;
+    const long long nc = (long long)nx * ny;
+    const int c = I2(i, j);
+    const R p2 = -RS(1.0) / (R)p.n_glen;
+    const R* ATT = f.ATT + cc.z0;
+    R* attp = f.attp + 4 * cc.z0 + c;
+    auto all = [&](int) { return true; };
+    for (int k = 0; k < nz; ++k) {
+      R ATTn[4];
+      gq_nodes<R>(0, ATTn, ATT + k * nc, all, nx, i, j, im1, ip1, jm1, jp1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) attp[(q * nz + k) * nc] = ypow<R>(ATTn[q], p2);
+    }
+  }
 }
 
 }  // namespace

@@ -337,6 +337,7 @@ struct Batch {
     YG_CHECK(h, ar.alloc((R**)&f.ATT, Z));
     YG_CHECK(h, ar.alloc(&f.visc, Z));
     YG_CHECK(h, ar.alloc(&f.visc_nm1, Z));
+    if (fp.visc_method == 1) YG_CHECK(h, ar.alloc(&f.attp, 4 * Z));
     YG_CHECK(h, ar.alloc((R**)&f.zeta, (size_t)zstride * nmem));
     f.zstride = zstride;
     YG_CHECK(h, ar.alloc((int32_t**)&f.mx, C));
