@@ -370,9 +370,26 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
     for (int r = 0; r < 2; ++r) {
       int q, nat, l;
       if (chunk_cell(c, t0, min(c.T, t0 + C), g + r * (int)blockDim.x, q, nat, l)) {
+        // the record's 18 slot values in 16-byte loads (records are 16-byte aligned)
+        const YG_GLOBAL TA* rp = Ag + (size_t)nat * YG_NA;
+        if constexpr (sizeof(TA) == 4) {
+          typedef float F4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-        for (int v = 0; v < 18; ++v)
-          lds<TA>(sb + v * stride + l) = Ag[(size_t)nat * YG_NA + v];
+          for (int e = 0; e < 5; ++e) {
+            const F4 v4 = reinterpret_cast<const YG_GLOBAL F4*>(rp)[e];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (4 * e + u < 18) lds<TA>(sb + (4 * e + u) * stride + l) = v4[u];
+          }
+        } else {
+          typedef double D2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+          for (int e = 0; e < 9; ++e) {
+            const D2 v2 = reinterpret_cast<const YG_GLOBAL D2*>(rp)[e];
+            lds<TA>(sb + (2 * e) * stride + l) = v2[0];
+            lds<TA>(sb + (2 * e + 1) * stride + l) = v2[1];
+          }
+        }
         lds<TA>(sb + 18 * stride + l) = mask_to_stage<TA>(gc(c.mask)[nat]);
       }
     }
@@ -491,13 +508,29 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
           const int tb = j - j0 + 2 * k;
           const size_t qb = (size_t)i0 * c.ny + (size_t)Lb * j0 + blk_soff(tb, Lb, M) + k -
                             glo_of(tb, M);
-          YG_GLOBAL TA* fl = Fg + 8 * qb;
-          fl[0] = (TA)a[2]; fl[1] = (TA)a[4]; fl[2] = (TA)a[8]; fl[3] = (TA)b[8];
-          fl[4] = (TA)b[4]; fl[5] = (TA)b[7]; fl[6] = (TA)b[2]; fl[7] = (TA)b[5];
-          YG_GLOBAL TA* fu = Fg + c.fu + 12 * qb;
-          fu[0] = (TA)X.x5; fu[1] = (TA)X.x3; fu[2] = (TA)X.x7; fu[3] = (TA)X.x1;
-          fu[4] = (TA)X.x6; fu[5] = (TA)X.xd; fu[6] = (TA)X.y6; fu[7] = (TA)X.y1;
-          fu[8] = (TA)X.y3; fu[9] = (TA)X.yd; fu[10] = (TA)0; fu[11] = (TA)0;
+          // 16-byte stores (the L and U records are 16-byte aligned)
+          const TA lv8[8] = {(TA)a[2], (TA)a[4], (TA)a[8], (TA)b[8],
+                             (TA)b[4], (TA)b[7], (TA)b[2], (TA)b[5]};
+          const TA uv12[12] = {(TA)X.x5, (TA)X.x3, (TA)X.x7, (TA)X.x1, (TA)X.x6, (TA)X.xd,
+                               (TA)X.y6, (TA)X.y1, (TA)X.y3, (TA)X.yd, (TA)0, (TA)0};
+          constexpr int E = 16 / sizeof(TA);
+          typedef TA V __attribute__((ext_vector_type(E)));
+          YG_GLOBAL V* fl = reinterpret_cast<YG_GLOBAL V*>(Fg + 8 * qb);
+          YG_GLOBAL V* fu = reinterpret_cast<YG_GLOBAL V*>(Fg + c.fu + 12 * qb);
+#pragma unroll
+          for (int e = 0; e < 8 / E; ++e) {
+            V v;
+#pragma unroll
+            for (int u = 0; u < E; ++u) v[u] = lv8[E * e + u];
+            fl[e] = v;
+          }
+#pragma unroll
+          for (int e = 0; e < 12 / E; ++e) {
+            V v;
+#pragma unroll
+            for (int u = 0; u < E; ++u) v[u] = uv12[E * e + u];
+            fu[e] = v;
+          }
         }
       }
       o2 = o1;
