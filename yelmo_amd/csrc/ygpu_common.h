@@ -49,10 +49,12 @@ __constant__ static const int8_t kSlotUy[9][3] = {
 // Two storage layouts, YG_NF values per cell:
 //   coupled (exact ILU(0) over more than 64 lines): SoA, F[v*ncell + q] over
 //     the member's skewed cell index q;
-//   streamed (line blocks of at most 64 lines, one per wave): AoS in
-//     block-skewed order, L part [q][8] (F0..F7) then U part [q][12]
-//     (F8..F17 and two pad values), so that one lane's cell is 2 (3)
-//     16-byte chunks and one step of a block is a contiguous range.
+//   streamed (line blocks of at most 64 lines, one per wave): AoS in the
+//     rotated tile order of the sweeps (cell (k, j) of a tile of L lines and
+//     M columns at slot ((j + 2k) mod M) L + k), L part [q][8] (F0..F7) then
+//     U part [q][12] (F8..F17 and two pad values), so that one lane's cell is
+//     2 (3) 16-byte chunks and one wavefront step of a tile is a contiguous
+//     range.
 #define YG_NF 20
 
 // One member (one independent system) of a batch.

@@ -501,13 +501,13 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
           Fg[14 * nc + q] = (TA)X.y6; Fg[15 * nc + q] = (TA)X.y1; Fg[16 * nc + q] = (TA)X.y3;
           Fg[17 * nc + q] = (TA)X.yd;
         } else {
-          // tile-skewed AoS: the tiles of a line block follow each other;
-          // within a tile the cells go by its wavefront steps
+          // rotated tile order (the streamed sweeps): the tiles of a line
+          // block follow each other; cell (k, j) of a tile at slot
+          // ((j + 2k) mod M) L + k
           const int i0 = w * c.lpw, k = c.line - i0, Lb = min(c.lpw, c.nx - i0);
           const int j = t - 2 * c.line, j0 = j - j % c.mcol, M = min(c.mcol, c.ny - j0);
           const int tb = j - j0 + 2 * k;
-          const size_t qb = (size_t)i0 * c.ny + (size_t)Lb * j0 + blk_soff(tb, Lb, M) + k -
-                            glo_of(tb, M);
+          const size_t qb = (size_t)i0 * c.ny + (size_t)Lb * j0 + (size_t)(tb % M) * Lb + k;
           // 16-byte stores (the L and U records are 16-byte aligned)
           const TA lv8[8] = {(TA)a[2], (TA)a[4], (TA)a[8], (TA)b[8],
                              (TA)b[4], (TA)b[7], (TA)b[2], (TA)b[5]};
@@ -779,12 +779,12 @@ YG_NOINLINE void sweep_U(const Ctx& c, const T* __restrict__ F, const T* __restr
 // Each wave sweeps its own line block over the block's own wavefront
 // (T_b = ny + 2(L-1) steps) and never waits for another wave.  At each
 // step a lane loads its cell's factor values as 16-byte chunks of the
-// block-skewed AoS layout (one step of a block is one contiguous range, so
+// rotated AoS layout (below; one step of a tile is one contiguous range, so
 // the loads of a wave are coalesced) into a ring of D register slots filled
 // D steps ahead of use, so the memory latency overlaps D steps of the
-// recurrence.  The forward sweep writes y in block-skewed order (read back
-// coalesced by the backward sweep); the right-hand side and the result are
-// read and written in the natural row order of the Krylov vectors.
+// recurrence.  The forward sweep writes y in the same rotated order (read
+// back coalesced by the backward sweep); the right-hand side and the result
+// are read and written in the natural row order of the Krylov vectors.
 
 // ---- bounds-checked buffer access ------------------------------------------
 // The streamed sweeps read and write through buffer resources (base, byte
@@ -850,7 +850,7 @@ YG_DEV void load_vals(__amdgpu_buffer_rsrc_t r, unsigned off, double (&v)[N]) {
 }
 
 // BF16 storage of the line-block factors (mixed mode, option factor16).
-// Three arrays over the cells, in the same block-skewed order as the FP32
+// Three arrays over the cells, in the same rotated tile order as the FP32
 // factors, so that each sweep load is one contiguous range across the lanes:
 //   L part: 8 BF16 values (F0..F7), 16 bytes at FB + 16 q bytes;
 //   U part: 8 BF16 values of the off-diagonal U entries already multiplied by
@@ -877,62 +877,67 @@ YG_DEV uint4 load_raw16(__amdgpu_buffer_rsrc_t r, unsigned off) {
 YG_DEV float bf_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
 YG_DEV float bf_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); }
 
-// Geometry of one wave's line block in the streamed sweeps, all scalars.
-// Per step t the lanes with cells are glo(t) <= lane <= min(L-1, t/2); a
-// lane's slot in the block-skewed order is blk_soff(t) - glo(t) + lane.
-// A lane without a cell at a step loads from an out-of-range offset, which
-// reads zero factors and a zero right-hand side, so its result is an exact
-// zero (the value its neighbours must see for a cell that does not exist)
-// without a select; its store goes out of range and is dropped.
-struct BlockGeom {
-  int L, Tb, ny;
-};
-struct StepQ {
-  int t, lo, hi, off;   // lanes lo..hi have cells at step t; slot of a lane = off + lane
-};
-YG_DEV StepQ step_q(const BlockGeom& G, int t) {
-  StepQ r;
-  r.t = t;
-  r.lo = glo_of(t, G.ny);
-  r.hi = (t >= 0 && t < G.Tb) ? min(G.L - 1, t >> 1) : -1;
-  r.off = blk_soff(t, G.L, G.ny) - r.lo;
-  return r;
-}
-// The next step (t + 1) from this one, with a few scalar operations:
-// the slot offset advances by the cells of step t less the change of lo.
-// Exact while t + 1 lies inside [0, Tb); outside it no lane has a cell.
-YG_DEV void step_next(const BlockGeom& G, StepQ& q) {
-  const int t1 = q.t + 1;
-  const int lo1 = max(0, (t1 - G.ny + 2) >> 1);
-  q.off += q.hi + 1 - lo1;
-  q.hi = t1 < G.Tb ? min(G.L - 1, t1 >> 1) : -1;
-  q.lo = lo1;
-  q.t = t1;
-}
-// The previous step (t - 1), for the backward sweep.  Exact while t - 1 lies
-// inside [0, Tb).
-YG_DEV void step_prev(const BlockGeom& G, StepQ& q) {
-  const int t1 = q.t - 1;
-  const int lo1 = max(0, (t1 - G.ny + 2) >> 1);
-  const int hi1 = (t1 >= 0 && t1 < G.Tb) ? min(G.L - 1, t1 >> 1) : -1;
-  q.off -= hi1 + 1 - q.lo;
-  q.hi = hi1;
-  q.lo = lo1;
-  q.t = t1;
-}
-// byte offset of a lane's cell at a step, or out of range if it has none
-YG_DEV unsigned cell_off(const StepQ& q, int lane, int lane_base, unsigned bytes) {
-  const unsigned o = (unsigned)(lane_base + q.off) * bytes;
-  return (lane >= q.lo && lane <= q.hi) ? o : YG_OOB;
-}
-// natural-order byte offset (lane_n + t) of a lane's cell at the step of q
-YG_DEV unsigned nat_off(const StepQ& q, int lane, int lane_n, unsigned bytes) {
-  const unsigned o = (unsigned)(lane_n + q.t) * bytes;
-  return (lane >= q.lo && lane <= q.hi) ? o : YG_OOB;
-}
-
+// Rotated tile order of the line-block factors.  A tile has L lines (one
+// lane each) and M columns and is swept in Tb = M + 2(L-1) wavefront steps:
+// line k handles column j = t - 2k at step t.  Cell (k, j) is stored at slot
+// ((j + 2k) mod M) L + k of the tile's range.  For each line, j -> (j + 2k)
+// mod M is a rotation, so the tile's M x L slots hold its L x M cells once
+// each, and the cells of step t are the contiguous slots (t mod M) L + [0, L):
+// the loads of a wave are coalesced, and the next step's slots are one add and
+// a wrap away.  Tiles follow each other by line block, then by column.
+//
+// A lane without a cell at step t (j outside [0, M), or a lane past the
+// tile's lines) passes an out-of-range offset, which loads zero factors and a
+// zero right-hand side, so its result is an exact zero (the value its
+// neighbours must see for a cell that does not exist) without a select, and
+// its store is dropped.
 YG_DEV int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+struct SweepTile {
+  int L, M, Tb, LM;   // lines, columns, steps, cells (wave-uniform)
+  int base;           // the tile's first slot in the member's factor order
+  int tl;             // -2 lane, or far out of range for a lane past the lines
+  int lane_n;         // the natural cell of this lane's line at step t is lane_n + t
+};
+// the wave's tile: line block wb, columns [j0, j0 + M); false for a wave without one
+YG_DEV bool sweep_tile(const Ctx& c, SweepTile& g) {
+  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
+  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
+  if (w >= nwl * ncol) return false;
+  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol, i0 = wb * lpw;
+  g.L = min(lpw, nx - i0);
+  g.M = min(mcol, ny - j0);
+  g.Tb = g.M + 2 * (g.L - 1);
+  g.LM = g.L * g.M;
+  g.base = i0 * ny + g.L * j0;
+  g.tl = lane < g.L ? -2 * lane : -(1 << 30);
+  g.lane_n = (i0 + lane) * ny + j0 - 2 * lane;
+  return true;
+}
+// does this lane have a cell at step t
+YG_DEV bool on_step(const SweepTile& g, int t) { return (unsigned)(t + g.tl) < (unsigned)g.M; }
+// byte offset of this lane's cell at step t in the tile's rotated order
+// (sq = (t mod M) L), or out of range.  A slot index is below 2^24, so a
+// cell size that is not a power of two takes a full-rate 24-bit multiply.
+YG_DEV unsigned rot_off(const SweepTile& g, int t, int sq, unsigned bytes) {
+  const unsigned q = (unsigned)(sq + (int)(threadIdx.x & 63));
+  const unsigned o = (bytes & (bytes - 1)) ? __umul24(q, bytes) : q * bytes;
+  return on_step(g, t) ? o : YG_OOB;
+}
+// natural-order byte offset of this lane's cell at step t, or out of range
+YG_DEV unsigned nat_off(const SweepTile& g, int t, unsigned bytes) {
+  return on_step(g, t) ? (unsigned)(g.lane_n + t) * bytes : YG_OOB;
+}
+// (t mod M) L of the next and of the previous step
+YG_DEV int rot_next(const SweepTile& g, int sq) { sq += g.L; return sq == g.LM ? 0 : sq; }
+YG_DEV int rot_prev(const SweepTile& g, int sq) { return (sq == 0 ? g.LM : sq) - g.L; }
+
+// Every sweep keeps D steps in flight in a ring of register slots: slot s is
+// refilled right after its step is computed, and a sched_barrier after each
+// step keeps the steps in program order, so the wait before a step covers
+// exactly that step's loads.  The store offset of a step is computed with its
+// loads and kept in the slot.
 template <typename T>
 YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
                              T* __restrict__ ys) {
@@ -940,32 +945,26 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   constexpr int D = sizeof(T) == 4 ? 12 : 6;   // register slots (steps in flight)
   constexpr unsigned SF = 8 * sizeof(T), SV = 2 * sizeof(T);   // bytes per cell
   using V2 = typename Vec2<T>::type;
-  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
-  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  SweepTile g;
+  if (!sweep_tile(c, g)) return;
   const int nc = uni(c.ncell);
-  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
-  if (w >= nwl * ncol) return;
-  // the wave's tile: line block wb, columns [j0, j0 + M)
-  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
-  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
-  const BlockGeom G{L, M + 2 * (L - 1), M};
-  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F, (size_t)8 * nc);
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + (size_t)8 * g.base, (size_t)8 * g.LM);
   const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
-  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
-  const int lane_q = i0 * ny + L * j0 + lane;           // + off: tile-skewed slot
-  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;  // + t: natural cell
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
   T f[D][8];
   V2 r[D];
-  StepQ qc = step_q(G, 0), qn = step_q(G, 0);   // the step computed, the step loaded
+  unsigned so[D];
+  int sq = 0;   // (t mod M) L of the step loaded next
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    load_vals<8>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
-    r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
-    step_next(G, qn);
-    __builtin_amdgcn_sched_barrier(0);   // slots issued in order (see the loop)
+    load_vals<8>(rF, rot_off(g, s, sq, SF), f[s]);
+    r[s] = Pair16<T>::load(rR, nat_off(g, s, SV));
+    so[s] = rot_off(g, s, sq, SV);
+    sq = rot_next(g, sq);
+    __builtin_amdgcn_sched_barrier(0);
   }
   T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
-  for (int t0 = 0; t0 < G.Tb; t0 += D) {
+  for (int t0 = 0; t0 < g.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
@@ -975,15 +974,13 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
       V2 o;
       o.x = yx;
       o.y = yy;
-      Pair16<T>::store(o, rY, cell_off(qc, lane, lane_q, SV));
+      Pair16<T>::store(o, rY, so[s]);
       o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
-      step_next(G, qc);
-      load_vals<8>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
-      r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
-      step_next(G, qn);
-      // keep the steps in program order: slot s is refilled only after its
-      // use, and no later step's loads move ahead of it, so the wait before
-      // each step covers exactly that step's loads
+      const int t = t0 + s + D;   // the step loaded into slot s
+      load_vals<8>(rF, rot_off(g, t, sq, SF), f[s]);
+      r[s] = Pair16<T>::load(rR, nat_off(g, t, SV));
+      so[s] = rot_off(g, t, sq, SV);
+      sq = rot_next(g, sq);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -996,32 +993,27 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   constexpr int D = sizeof(T) == 4 ? 10 : 6;
   constexpr unsigned SF = 12 * sizeof(T), SV = 2 * sizeof(T);
   using V2 = typename Vec2<T>::type;
-  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
-  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  SweepTile g;
+  if (!sweep_tile(c, g)) return;
   const int nc = uni(c.ncell);
-  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
-  if (w >= nwl * ncol) return;
-  // the wave's tile: line block wb, columns [j0, j0 + M)
-  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
-  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
-  const BlockGeom G{L, M + 2 * (L - 1), M};
-  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + c.fu, (size_t)12 * nc);
-  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + c.fu + (size_t)12 * g.base, (size_t)12 * g.LM);
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
-  const int lane_q = i0 * ny + L * j0 + lane;
-  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;
   T f[D][12];
   V2 yv[D];
-  StepQ qc = step_q(G, G.Tb - 1), qn = step_q(G, G.Tb - 1);
+  unsigned so[D];
+  const int tz = g.Tb - 1;   // the last step; the sweep runs t = tz - u
+  int sq = (tz % g.M) * g.L;
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    load_vals<12>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
-    yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
-    step_prev(G, qn);
+    load_vals<12>(rF, rot_off(g, tz - s, sq, SF), f[s]);
+    yv[s] = Pair16<T>::load(rY, rot_off(g, tz - s, sq, SV));
+    so[s] = nat_off(g, tz - s, SV);
+    sq = rot_prev(g, sq);
     __builtin_amdgcn_sched_barrier(0);
   }
   T o1x = 0, o1y = 0, o2x = 0, o2y = 0;  // own cells j+1, j+2
-  for (int u0 = 0; u0 < G.Tb; u0 += D) {
+  for (int u0 = 0; u0 < g.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       // line i+1: cell j-1 (one step back) and cell j (two steps back)
@@ -1032,12 +1024,13 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
       V2 o;
       o.x = zx;
       o.y = zy;
-      Pair16<T>::store(o, rZ, nat_off(qc, lane, lane_n, SV));
+      Pair16<T>::store(o, rZ, so[s]);
       o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
-      step_prev(G, qc);
-      load_vals<12>(rF, cell_off(qn, lane, lane_q, SF), f[s]);
-      yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
-      step_prev(G, qn);
+      const int t = tz - (u0 + s + D);
+      load_vals<12>(rF, rot_off(g, t, sq, SF), f[s]);
+      yv[s] = Pair16<T>::load(rY, rot_off(g, t, sq, SV));
+      so[s] = nat_off(g, t, SV);
+      sq = rot_prev(g, sq);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1049,36 +1042,30 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
 YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ rhs, float* __restrict__ ys) {
   PhaseTimer ptm(c, 1);
-  constexpr int D = 24;   // steps in flight: 6 registers per step (4 packed factor words, 2 rhs)
+  constexpr int D = 20;   // steps in flight: 7 registers per step (4 packed factor words, 2 rhs, 1 offset)
   constexpr unsigned SF = 16, SV = 8;
   using T = float;
   using V2 = typename Vec2<T>::type;
-  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
-  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  SweepTile g;
+  if (!sweep_tile(c, g)) return;
   const int nc = uni(c.ncell);
-  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
-  if (w >= nwl * ncol) return;
-  // the wave's tile: line block wb, columns [j0, j0 + M)
-  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
-  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
-  const BlockGeom G{L, M + 2 * (L - 1), M};
-  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB, (size_t)8 * nc);
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + (size_t)8 * g.base, (size_t)8 * g.LM);
   const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
-  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
-  const int lane_q = i0 * ny + L * j0 + lane;
-  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
   uint4 fw[D];
   V2 r[D];
-  StepQ qc = step_q(G, 0), qn = step_q(G, 0);
+  unsigned so[D];
+  int sq = 0;
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    fw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
-    r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
-    step_next(G, qn);
+    fw[s] = load_raw16(rF, rot_off(g, s, sq, SF));
+    r[s] = Pair16<T>::load(rR, nat_off(g, s, SV));
+    so[s] = rot_off(g, s, sq, SV);
+    sq = rot_next(g, sq);
     __builtin_amdgcn_sched_barrier(0);
   }
   T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
-  for (int t0 = 0; t0 < G.Tb; t0 += D) {
+  for (int t0 = 0; t0 < g.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       const T f0 = bf_lo(fw[s].x), f1 = bf_hi(fw[s].x), f2 = bf_lo(fw[s].y), f3 = bf_hi(fw[s].y);
@@ -1089,12 +1076,13 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
       V2 o;
       o.x = yx;
       o.y = yy;
-      Pair16<T>::store(o, rY, cell_off(qc, lane, lane_q, SV));
+      Pair16<T>::store(o, rY, so[s]);
       o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
-      step_next(G, qc);
-      fw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
-      r[s] = Pair16<T>::load(rR, nat_off(qn, lane, lane_n, SV));
-      step_next(G, qn);
+      const int t = t0 + s + D;
+      fw[s] = load_raw16(rF, rot_off(g, t, sq, SF));
+      r[s] = Pair16<T>::load(rR, nat_off(g, t, SV));
+      so[s] = rot_off(g, t, sq, SV);
+      sq = rot_next(g, sq);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1103,39 +1091,34 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
 YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ ys, float* __restrict__ z) {
   PhaseTimer ptm(c, 2);
-  constexpr int D = 20;   // 8 registers per step: 4 packed words, the 2 inverse diagonals, y
+  constexpr int D = 18;   // 9 registers per step: 4 packed words, the 2 inverse diagonals, y, offset
   constexpr unsigned SF = 16, SD = 8, SV = 8;
   using T = float;
   using V2 = typename Vec2<T>::type;
-  const int lane = threadIdx.x & 63, w = uni(threadIdx.x >> 6);
-  const int nwl = uni(c.nwl), ny = uni(c.ny), nx = uni(c.nx), lpw = uni(c.lpw);
+  SweepTile g;
+  if (!sweep_tile(c, g)) return;
   const int nc = uni(c.ncell);
-  const int ncol = uni(c.ncol), mcol = uni(c.mcol);
-  if (w >= nwl * ncol) return;
-  // the wave's tile: line block wb, columns [j0, j0 + M)
-  const int wb = w / ncol, j0 = (w - wb * ncol) * mcol;
-  const int i0 = wb * lpw, L = min(lpw, nx - i0), M = min(mcol, ny - j0);
-  const BlockGeom G{L, M + 2 * (L - 1), M};
-  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + c.fu16, (size_t)8 * nc);
-  const __amdgpu_buffer_rsrc_t rD = yg_rsrc(FB + c.fd16, (size_t)4 * nc);
-  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + c.fu16 + (size_t)8 * g.base, (size_t)8 * g.LM);
+  const __amdgpu_buffer_rsrc_t rD = yg_rsrc(FB + c.fd16 + (size_t)4 * g.base, (size_t)4 * g.LM);
+  const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
-  const int lane_q = i0 * ny + L * j0 + lane;
-  const int lane_n = (i0 + lane) * ny + j0 - 2 * lane;
   uint4 gw[D];   // the 8 scaled off-diagonal U entries
   V2 dg[D];      // ux-row and uy-row inverse diagonals
   V2 yv[D];
-  StepQ qc = step_q(G, G.Tb - 1), qn = step_q(G, G.Tb - 1);
+  unsigned so[D];
+  const int tz = g.Tb - 1;   // the last step
+  int sq = (tz % g.M) * g.L;
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    gw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
-    dg[s] = Pair16<T>::load(rD, cell_off(qn, lane, lane_q, SD));
-    yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
-    step_prev(G, qn);
+    gw[s] = load_raw16(rF, rot_off(g, tz - s, sq, SF));
+    dg[s] = Pair16<T>::load(rD, rot_off(g, tz - s, sq, SD));
+    yv[s] = Pair16<T>::load(rY, rot_off(g, tz - s, sq, SV));
+    so[s] = nat_off(g, tz - s, SV);
+    sq = rot_prev(g, sq);
     __builtin_amdgcn_sched_barrier(0);
   }
   T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
-  for (int u0 = 0; u0 < G.Tb; u0 += D) {
+  for (int u0 = 0; u0 < g.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       // 0..4 scaled ux-row U entries (slots 5,3,7,1,6), 5..7 scaled uy-row
@@ -1148,13 +1131,14 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
       V2 o;
       o.x = zx;
       o.y = zy;
-      Pair16<T>::store(o, rZ, nat_off(qc, lane, lane_n, SV));
+      Pair16<T>::store(o, rZ, so[s]);
       o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
-      step_prev(G, qc);
-      gw[s] = load_raw16(rF, cell_off(qn, lane, lane_q, SF));
-      dg[s] = Pair16<T>::load(rD, cell_off(qn, lane, lane_q, SD));
-      yv[s] = Pair16<T>::load(rY, cell_off(qn, lane, lane_q, SV));
-      step_prev(G, qn);
+      const int t = tz - (u0 + s + D);
+      gw[s] = load_raw16(rF, rot_off(g, t, sq, SF));
+      dg[s] = Pair16<T>::load(rD, rot_off(g, t, sq, SD));
+      yv[s] = Pair16<T>::load(rY, rot_off(g, t, sq, SV));
+      so[s] = nat_off(g, t, SV);
+      sq = rot_prev(g, sq);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1901,7 +1885,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     }
     Precon<float> P;
     P.kind = 2;
-    P.F = F + (size_t)8 * cm.cb;   // this part's L factors (block-skewed order)
+    P.F = F + (size_t)8 * cm.cb;   // this part's L factors (rotated tile order)
     if (Lg > 0) ilu0_factor<float>(cr, A32 + (size_t)YG_NA * cm.cb, F + (size_t)8 * cm.cb, &s_bad);
     if (o.factor16 && a.F16) {
       uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff + (size_t)8 * cm.cb;
