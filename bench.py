@@ -391,9 +391,10 @@ def main():
     ap.add_argument("--single-sample", type=int, default=4,
                     help="members solved one at a time on the FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--factor", choices=("fp32", "bf16"), default="fp32",
+    ap.add_argument("--factor", choices=("fp32", "bf16", "bf16u"), default="fp32",
                     help="storage of the mixed path's ILU(0) factors: bf16 (with the U rows "
-                         "scaled by their FP32 inverse diagonals) or fp32")
+                         "scaled by their FP32 inverse diagonals), bf16u (the U part in BF16, "
+                         "the L part in FP32) or fp32")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the schedule autotuner (use the size rule: ceil(nx/64) line "
                          "blocks)")

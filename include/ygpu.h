@@ -125,9 +125,10 @@ void ygpu_default_params(ygpu_ssa_params* par);
  * -maxiter, -tol, -initx_zeros, -conv_cond; extensions -ygpu_ny NY (grid hint),
  * -ygpu_precision fp64|mixed.                                                */
 /* Extensions of the option string read by the outer entry points too:
- *   -ygpu_factor bf16|fp32   storage of the mixed path's ILU(0) factors
- *                            (BF16 with U rows scaled by their FP32 inverse
- *                            diagonals; default fp32)
+ *   -ygpu_factor bf16|bf16u|fp32  storage of the mixed path's ILU(0)
+ *                            factors: BF16 with U rows scaled by their FP32
+ *                            inverse diagonals; bf16u: the U part in BF16
+ *                            and the L part in FP32; default fp32
  *   -ygpu_ilu_blocks N       ILU(0) over N line blocks (mixed path default:
  *                            one per 64 lines, or the autotuned schedule)
  *   -ygpu_ir_tol T           residual at which refinement stops (tol/10)

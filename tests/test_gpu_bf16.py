@@ -1,4 +1,5 @@
-"""BF16 ILU(0) factors with per-row scaling (-ygpu_factor bf16) in the
+"""BF16 ILU(0) factors with per-row scaling (-ygpu_factor bf16), and BF16 U
+factors with FP32 L factors (-ygpu_factor bf16u), in the
 mixed-precision path, through the C-ABI.  The preconditioner changes (its
 factors carry 8-bit mantissas), so iteration counts differ from the FP32
 factors; the FP64 refinement still meets the same residual criterion, and
@@ -14,7 +15,8 @@ from yelmo_amd.params import SsaParams
 
 pytestmark = pytest.mark.gpu
 
-BF16 = "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false -ygpu_factor bf16"
+BF16 = "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false -ygpu_factor {}"
+MODES = ("bf16", "bf16u")
 
 
 def _converged(par):
@@ -22,26 +24,28 @@ def _converged(par):
                      ssa_lis_opt="-i bicgstab -p ilu -maxiter 3000 -tol 1.0e-12 -initx_zeros false")
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("n,dx", [(81, 16e3), (161, 8e3)])
-def test_bf16_factors_ssa_within_mixed_bar(n, dx):
-    print(f"case bf16 ssa n={n}", flush=True)
-    par = SsaParams(visc_method=1, beta_method=2, precision="mixed", ssa_lis_opt=BF16)
+def test_bf16_factors_ssa_within_mixed_bar(n, dx, mode):
+    print(f"case {mode} ssa n={n}", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed", ssa_lis_opt=BF16.format(mode))
     p = generator.make_problem(n, dx, seed=6, dtype=np.float32)
     ref_st, ref = oracle.calc_velocity_ssa(p.astype(np.float64), _converged(par), dp=True)
     g = p.copy()
     info = velocity.calc_velocity_ssa(g, par)
     f32 = p.copy()
-    info32 = velocity.calc_velocity_ssa(f32, par.with_(ssa_lis_opt=BF16.replace("bf16", "fp32")))
+    info32 = velocity.calc_velocity_ssa(f32, par.with_(ssa_lis_opt=BF16.format("fp32")))
     err = vel_rel_l2(g.state, ref_st)
-    print("picard", info.ssa_iter_now, ref["ssa_iter_now"], "krylov bf16", info.ssa_lin_iter,
+    print("picard", info.ssa_iter_now, ref["ssa_iter_now"], "krylov", mode, info.ssa_lin_iter,
           "fp32", info32.ssa_lin_iter, "rel L2", err, flush=True)
     assert info.status == 0 and info.ssa_lin_fail == 0
     assert err <= 1e-6
 
 
-def test_bf16_factors_diva_within_mixed_bar():
-    print("case bf16 diva", flush=True)
-    par = SsaParams(visc_method=1, beta_method=2, precision="mixed", ssa_lis_opt=BF16)
+@pytest.mark.parametrize("mode", MODES)
+def test_bf16_factors_diva_within_mixed_bar(mode):
+    print(f"case {mode} diva", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed", ssa_lis_opt=BF16.format(mode))
     p = generator.make_problem(96, 16e3, seed=7, dtype=np.float64, nz=21)
     ref_st, _ = oracle.calc_velocity_diva(p, _converged(par), dp=True)
     g = oracle.diva_state(p.copy())
@@ -51,11 +55,12 @@ def test_bf16_factors_diva_within_mixed_bar():
     assert info.status == 0 and err <= 1e-6
 
 
-def test_bf16_factors_csr_solve_meets_tolerance():
+@pytest.mark.parametrize("mode", MODES)
+def test_bf16_factors_csr_solve_meets_tolerance(mode):
     """Inner boundary: the CSR of a mid-Picard state solved with BF16
     factors meets the FP64 residual criterion and agrees with the oracle's
     converged solve."""
-    print("case bf16 csr", flush=True)
+    print(f"case {mode} csr", flush=True)
     par = SsaParams(visc_method=2, beta_method=4, ssa_iter_max=4)
     p = generator.make_problem(81, 16e3, seed=0, dtype=np.float64)
     st, _ = oracle.calc_velocity_ssa(p, par, dp=True)
@@ -67,7 +72,7 @@ def test_bf16_factors_csr_solve_meets_tolerance():
     exact = oracle.csr_solve(ap, ai, av, b, x0,
                              "-i bicgstab -p ilu -maxiter 3000 -tol 1.0e-12 -initx_zeros false")
     lgs = velocity.Lgs(ap, ai, av, b, x0.copy())
-    velocity.linear_solver_matrix_solve(lgs, BF16 + " -ygpu_precision mixed")
+    velocity.linear_solver_matrix_solve(lgs, BF16.format(mode) + " -ygpu_precision mixed")
     err = rel_l2(lgs.x_value, exact["x"])
     print("iters", lgs.lin_iter, "status", lgs.status, "resid", lgs.L2_rel_norm, "err", err)
     assert lgs.status == 0 and lgs.L2_rel_norm <= 1e-8

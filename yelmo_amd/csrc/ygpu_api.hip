@@ -57,6 +57,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     }
     else if (k == "-ygpu_factor") {
       if (v == "bf16") o.k.factor16 = 1;
+      else if (v == "bf16u") o.k.factor16 = 2;
       else if (v == "fp32") o.k.factor16 = 0;
       else { err = "unknown factor storage: " + v; return YGPU_ERR_ARG; }
     }

@@ -730,7 +730,8 @@ struct Batch {
         for (int k : active) {
           const double C = (double)mem[k].nx * mem[k].ny;
           const int it = kit[k];
-          const double pF = opts.k.factor16 ? 40.0 / 18.0 : 4.0;
+          // bytes per factor value: BF16 L + U cells 16 + 24 B, FP32 L + BF16 U 32 + 24 B
+          const double pF = opts.k.factor16 == 1 ? 40.0 / 18.0 : opts.k.factor16 == 2 ? 56.0 / 18.0 : 4.0;
           h->stats.krylov_bytes += mixed ? krylov_bytes(C, it, 4, pF, 4, opts.k.precon == 2) +
                                                4.0 * (18.0 * C * 8 + 8.0 * C + 6.0 * 2.0 * C * 8)
                                          : krylov_bytes(C, it, 8, 8, 8, opts.k.precon == 2);

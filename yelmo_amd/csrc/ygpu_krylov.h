@@ -39,7 +39,8 @@ struct YgKrylovOpts {
   double inner_tol; // mixed mode: relative tolerance of each FP32 inner solve
   double ir_target; // mixed mode: residual the refinement stops at (0: tol / 10)
   int factor16;     // mixed mode: 1 = ILU(0) factors stored in BF16 (U rows scaled
-                    // by their FP32 inverse diagonals)
+                    // by their FP32 inverse diagonals); 2 = the U part in BF16 and
+                    // the L part in FP32
   int graph_chunk;  // host side: Picard iterations per captured graph (0: no graph)
 };
 

@@ -505,9 +505,12 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
           // block follow each other; cell (k, j) of a tile at slot
           // ((j + 2k) mod M) L + k
           const int i0 = w * c.lpw, k = c.line - i0, Lb = min(c.lpw, c.nx - i0);
-          const int j = t - 2 * c.line, j0 = j - j % c.mcol, M = min(c.mcol, c.ny - j0);
-          const int tb = j - j0 + 2 * k;
-          const size_t qb = (size_t)i0 * c.ny + (size_t)Lb * j0 + (size_t)(tb % M) * Lb + k;
+          const int j = t - 2 * c.line, jt = c.ncol > 1 ? j % c.mcol : j;
+          const int j0 = j - jt, M = min(c.mcol, c.ny - j0);
+          // tb mod M with tb = jt + 2k < M + 2(Lb - 1): a few subtractions
+          int rq = jt + 2 * k;
+          while (rq >= M) rq -= M;
+          const size_t qb = (size_t)i0 * c.ny + (size_t)Lb * j0 + (size_t)rq * Lb + k;
           // 16-byte stores (the L and U records are 16-byte aligned)
           const TA lv8[8] = {(TA)a[2], (TA)a[4], (TA)a[8], (TA)b[8],
                              (TA)b[4], (TA)b[7], (TA)b[2], (TA)b[5]};
@@ -1175,6 +1178,7 @@ struct Precon {
   int kind;        // 0 none, 1 jacobi, 2 ilu
   const T* F;      // ILU factors, or 1/diag pairs for jacobi
   const uint16_t* FB = nullptr;   // BF16 copy of line-block factors (mixed mode)
+  bool fb_l = true;                // the forward sweep also reads BF16 (else F's FP32 L part)
 };
 
 // out = M^-1 in; tmp is scratch of the same size.  The sweeps run on the
@@ -1190,7 +1194,10 @@ YG_DEV void precon_apply(const Ctx& c, const Precon<T>& P, const T* in, T* tmp, 
     sweep_U<T>(pc, P.F, tmp, out);
   } else if (P.kind == 2 && P.FB) {
     if constexpr (sizeof(T) == 4) {
-      if (pc.ncell > 0) sweep_L_str16(pc, P.FB, in + o, tmp + o);
+      if (pc.ncell > 0) {
+        if (P.fb_l) sweep_L_str16(pc, P.FB, in + o, tmp + o);
+        else sweep_L_str<T>(pc, P.F, in + o, tmp + o);
+      }
       __syncthreads();
       if (pc.ncell > 0) sweep_U_str16(pc, P.FB, tmp + o, out + o);
       __syncthreads();
@@ -1692,6 +1699,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
       uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff;
       factors_to_bf16(c, F, FB);
       P.FB = FB;
+      P.fb_l = o.factor16 == 1;
     }
     YgKrylovOpts oi = o;
     oi.tol = o.inner_tol;
@@ -1891,6 +1899,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
       uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff + (size_t)8 * cm.cb;
       if (Lg > 0) factors_to_bf16(cr, P.F, FB);
       P.FB = FB;
+      P.fb_l = o.factor16 == 1;
     }
     double bad[1] = {(double)s_bad};
     csum<1>(cm, bad);   // a zero pivot on any rank
