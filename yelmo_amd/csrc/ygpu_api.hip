@@ -646,7 +646,9 @@ int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny,
   int64_t b = C * (4 + 4 + YG_NA * 8 + 2 * 8 + 2 * 8 + YG_NW64 * 2 * 8 + 8);
   if (par->precision == YGPU_PREC_MIXED) b += C * (YG_NA * 4 + YG_NF * 4 + YG_NW32 * 2 * 4);
   else b += C * YG_NF * 8;
-  b += rb * (C * 34 + Z * 3 + 2 * Z);
+  // 2D fields, ATT and the viscosity (3D), ATT^(-1/n) at the quadrature nodes
+  // (visc_method 1), and 2 3D fields of margin
+  b += rb * (C * 34 + Z * 2 + (par->visc_method == 1 ? 4 * Z : 0) + 2 * Z);
   if (par->ssa_solver == 1) b += C * 2 * 4;   // copy_from of the energy form
   return b;
 }

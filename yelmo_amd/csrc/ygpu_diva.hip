@@ -27,7 +27,7 @@ YG_DEV R stag_margin(R v0, R v1, R f0, R f1) {
 
 // calc_vertical_shear_3D (velocity_diva.f90:510-571)
 template <typename R>
-__global__ void k_shear(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+__global__ __launch_bounds__(256) void k_shear(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= G.total_cells) return;
   const int m = yg_member_at(G, gc);
@@ -53,7 +53,7 @@ __global__ void k_shear(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int
 
 // calc_vel_horizontal_3D (velocity_diva.f90:431-508)
 template <typename R>
-__global__ void k_vel3d(YgBatchGeom G, YgFields<R> f, YgDivaFields<R> d, YgFieldParams p,
+__global__ __launch_bounds__(256) void k_vel3d(YgBatchGeom G, YgFields<R> f, YgDivaFields<R> d, YgFieldParams p,
                         const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= G.total_cells) return;

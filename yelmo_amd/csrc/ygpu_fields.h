@@ -52,7 +52,7 @@ struct YgFields {
   const int32_t *mx, *my;
   // state
   R *ux, *uy, *ux_nm1, *uy_nm1;
-  R *visc, *visc_nm1;   // 3D
+  R *visc;   // 3D
   R *visc_int;
   R *beta, *beta_acx, *beta_acy, *bss_x, *bss_y;  // bss: friction used by the matrix
   R *lkx, *lky, *lrx, *lry;                       // limit drag

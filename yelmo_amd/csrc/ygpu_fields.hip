@@ -85,7 +85,7 @@ YG_DEV bool act_y(const R* fi, int nx, int ny, int BC, int i, int j) {
 }
 
 template <typename R>
-__global__ void k_strain(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+__global__ __launch_bounds__(256) void k_strain(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
@@ -212,7 +212,10 @@ YG_DEV void gq_nodes(int kind, R n[4], const R* v, ACT act, int nx, int i, int j
 // velocity_diva.f90:573-822 add 1/4 (du/dz)^2 + 1/4 (dv/dz)^2),
 // picard_relax_visc (velocity_general.f90:2188-2213), calc_visc_eff_int
 // (velocity_general.f90:2215-2261).
-template <typename R>
+// SSA1: the SSA form of visc_method 1 with ATT^(-1/n) at the nodes filled once
+// per solve (the common case), compiled without the other forms' code so that
+// it needs few registers.
+template <typename R, bool SSA1>
 __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
@@ -223,146 +226,143 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
   const R* fi = f.f_ice + cc.c0;
   const R visc_min = R(1e5);
   R* visc = f.visc + cc.z0;
-  R* vnm1 = f.visc_nm1 + cc.z0;
   const R* ATT = f.ATT + cc.z0;
   const R* zeta = f.zeta + (long long)cc.m * f.zstride;
   const R rel = (R)p.iter_rel;
   const R n_glen = (R)p.n_glen;
-  // keep the previous iterate
-  for (int k = 0; k < nz; ++k) vnm1[c + k * nc] = visc[c + k * nc];
-  if (p.visc_method == 0) {
-    for (int k = 0; k < nz; ++k) visc[c + k * nc] = (R)p.visc_const;
-  } else if (fi[c] != R(1)) {
-    for (int k = 0; k < nz; ++k) visc[c + k * nc] = visc_min;
+  const bool ice = fi[c] == R(1);
+  const int vm = SSA1 ? 1 : p.visc_method;
+  const bool diva = SSA1 ? false : (bool)p.diva;
+  const R eps_0 = (R)p.eps_0, eps_0_sq = eps_0 * eps_0;
+  int im1, ip1, jm1, jp1;
+  nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
+  const R dx = (R)cc.dx, dy = (R)cc.dy;
+  const R p1 = (RS(1.0) - n_glen) / (RS(2.0) * n_glen);
+  const R p2 = -RS(1.0) / n_glen;
+  // per-cell terms of the level values
+  R eps_sq_n[4], epow[4];        // visc_method 1: strain rates at the quadrature nodes
+  R eps_sq_h = R(0), eps_sq_aa = R(0);   // visc_method 2: the cell-centre strain rate
+  const R* attp = nullptr;
+  if (vm == 1 && ice) {
+    const R* dudx = f.dudx + cc.c0;
+    const R* dudy = f.dudy + cc.c0;
+    const R* dvdx = f.dvdx + cc.c0;
+    const R* dvdy = f.dvdy + cc.c0;
+    auto ax = [&](int qq) { return act_x<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
+    auto ay = [&](int qq) { return act_y<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
+    R dudxn[4], dudyn[4], dvdxn[4], dvdyn[4];
+    gq_nodes<R>(1, dudxn, dudx, ax, nx, i, j, im1, ip1, jm1, jp1);
+    gq_nodes<R>(1, dudyn, dudy, ax, nx, i, j, im1, ip1, jm1, jp1);
+    gq_nodes<R>(2, dvdxn, dvdx, ay, nx, i, j, im1, ip1, jm1, jp1);
+    gq_nodes<R>(2, dvdyn, dvdy, ay, nx, i, j, im1, ip1, jm1, jp1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const R s = dudyn[q] + dvdxn[q];
+      eps_sq_n[q] = dudxn[q] * dudxn[q] + dvdyn[q] * dvdyn[q] + dudxn[q] * dvdyn[q] +
+                    RS(0.25) * (s * s);
+      if (!diva) eps_sq_n[q] = eps_sq_n[q] + eps_0_sq;
+    }
+    // SSA: the strain-rate term of a node is the same on every level, so
+    // its power is taken once per node (the same value the level loop
+    // would compute nz times)
+    if (!diva) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) epow[q] = ypow<R>(eps_sq_n[q], p1);
+    }
+    attp = (SSA1 || f.attp) ? f.attp + 4 * cc.z0 + c : nullptr;
+  } else if (vm == 2 && ice) {
+    const R* ux = (diva ? f.ubar_x : f.ux) + cc.c0;
+    const R* uy = (diva ? f.ubar_y : f.uy) + cc.c0;
+    const R dudx_aa = (ux[c] - ux[I2(im1, j)]) / dx;
+    const R dvdy_aa = (uy[c] - uy[I2(i, jm1)]) / dy;
+    const R dudy_aa_1 = (ux[I2(i, jp1)] - ux[I2(i, jm1)]) / (R(2) * dy);
+    const R dudy_aa_2 = (ux[I2(im1, jp1)] - ux[I2(im1, jm1)]) / (R(2) * dy);
+    const R dudy_aa = R(0.5) * (dudy_aa_1 + dudy_aa_2);
+    const R dvdx_aa_1 = (uy[I2(ip1, j)] - uy[I2(im1, j)]) / (R(2) * dx);
+    const R dvdx_aa_2 = (uy[I2(ip1, jm1)] - uy[I2(im1, jm1)]) / (R(2) * dx);
+    const R dvdx_aa = R(0.5) * (dvdx_aa_1 + dvdx_aa_2);
+    const R s = dudy_aa + dvdx_aa;
+    eps_sq_h = dudx_aa * dudx_aa + dvdy_aa * dvdy_aa + dudx_aa * dvdy_aa + R(0.25) * (s * s);
+    eps_sq_aa = eps_sq_h + eps_0_sq;
   }
-  if (p.visc_method != 0 && fi[c] == R(1)) {
-    const R eps_0 = (R)p.eps_0, eps_0_sq = eps_0 * eps_0;
-    int im1, ip1, jm1, jp1;
-    nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
-    const R dx = (R)cc.dx, dy = (R)cc.dy;
-    if (p.visc_method == 1) {
-      const R p1 = (RS(1.0) - n_glen) / (RS(2.0) * n_glen);
-      const R p2 = -RS(1.0) / n_glen;
-      const R* dudx = f.dudx + cc.c0;
-      const R* dudy = f.dudy + cc.c0;
-      const R* dvdx = f.dvdx + cc.c0;
-      const R* dvdy = f.dvdy + cc.c0;
+  // The viscosity of level k before relaxation
+  auto level_value = [&](int k) -> R {
+    if (vm == 0) return (R)p.visc_const;
+    if (!ice) return visc_min;
+    if (vm == 1) {
       auto ax = [&](int qq) { return act_x<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
       auto ay = [&](int qq) { return act_y<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
       auto all = [&](int) { return true; };
-      R dudxn[4], dudyn[4], dvdxn[4], dvdyn[4], eps_sq_n[4];
-      gq_nodes<R>(1, dudxn, dudx, ax, nx, i, j, im1, ip1, jm1, jp1);
-      gq_nodes<R>(1, dudyn, dudy, ax, nx, i, j, im1, ip1, jm1, jp1);
-      gq_nodes<R>(2, dvdxn, dvdx, ay, nx, i, j, im1, ip1, jm1, jp1);
-      gq_nodes<R>(2, dvdyn, dvdy, ay, nx, i, j, im1, ip1, jm1, jp1);
+      // ATT^p2 at the nodes: filled once per solve, or computed here
+      R ATTp[4];
+      if (attp) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const R s = dudyn[q] + dvdxn[q];
-        eps_sq_n[q] = dudxn[q] * dudxn[q] + dvdyn[q] * dvdyn[q] + dudxn[q] * dvdyn[q] +
-                      RS(0.25) * (s * s);
-        if (!p.diva) eps_sq_n[q] = eps_sq_n[q] + eps_0_sq;
+        for (int q = 0; q < 4; ++q) ATTp[q] = attp[(q * nz + k) * nc];
+      } else {
+        R ATTn[4];
+        gq_nodes<R>(0, ATTn, ATT + k * nc, all, nx, i, j, im1, ip1, jm1, jp1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ATTp[q] = ypow<R>(ATTn[q], p2);
       }
-      // SSA: the strain-rate term of a node is the same on every level, so
-      // its power is taken once per node (the same value the level loop
-      // would compute nz times)
-      R epow[4];
-      if (!p.diva) {
+      R ep[4];
+      if (diva) {
+        R dzx[4], dzy[4];
+        // DIVA: vertical shear at the nodes from the acx/acy shear fields
+        gq_nodes<R>(1, dzx, f.duxdz + cc.z0 + k * nc, ax, nx, i, j, im1, ip1, jm1, jp1);
+        gq_nodes<R>(2, dzy, f.duydz + cc.z0 + k * nc, ay, nx, i, j, im1, ip1, jm1, jp1);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) epow[q] = ypow<R>(eps_sq_n[q], p1);
+        for (int q = 0; q < 4; ++q)
+          ep[q] = ypow<R>(eps_sq_n[q] + R(0.25) * (dzx[q] * dzx[q]) +
+                              R(0.25) * (dzy[q] * dzy[q]) + eps_0_sq, p1);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ep[q] = epow[q];
       }
-      const R* attp = f.attp ? f.attp + 4 * cc.z0 + c : nullptr;
-      for (int k = 0; k < nz; ++k) {
-        // ATT^p2 at the nodes: filled once per solve, or computed here
-        R ATTp[4];
-        if (attp) {
+      R sum = R(0);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) ATTp[q] = attp[(q * nz + k) * nc];
-        } else {
-          R ATTn[4];
-          gq_nodes<R>(0, ATTn, ATT + k * nc, all, nx, i, j, im1, ip1, jm1, jp1);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) ATTp[q] = ypow<R>(ATTn[q], p2);
-        }
-        if (p.diva) {
-          R dzx[4], dzy[4];
-          // DIVA: vertical shear at the nodes from the acx/acy shear fields
-          gq_nodes<R>(1, dzx, f.duxdz + cc.z0 + k * nc, ax, nx, i, j, im1, ip1, jm1, jp1);
-          gq_nodes<R>(2, dzy, f.duydz + cc.z0 + k * nc, ay, nx, i, j, im1, ip1, jm1, jp1);
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            epow[q] = ypow<R>(eps_sq_n[q] + R(0.25) * (dzx[q] * dzx[q]) +
-                                  R(0.25) * (dzy[q] * dzy[q]) + eps_0_sq, p1);
-        }
-        R sum = R(0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sum += (RS(0.5) * epow[q] * ATTp[q]) * R(1);
-        visc[c + k * nc] = sum / R(4);
+      for (int q = 0; q < 4; ++q) sum += (RS(0.5) * ep[q] * ATTp[q]) * R(1);
+      return sum / R(4);
+    }
+    R e = eps_sq_aa;
+    if (diva) {
+      // DIVA aa form: shear of the cell's own faces, averaged to the centre
+      const R* dzx = f.duxdz + cc.z0 + k * nc;
+      const R* dzy = f.duydz + cc.z0 + k * nc;
+      const R ddx = R(0.5) * (dzx[c] + dzx[I2(im1, j)]);
+      const R ddy = R(0.5) * (dzy[c] + dzy[I2(i, jm1)]);
+      e = eps_sq_h + R(0.25) * (ddx * ddx) + R(0.25) * (ddy * ddy) + eps_0_sq;
+    }
+    return R(0.5) * ypow<R>(e, p1) * ypow<R>(ATT[c + k * nc], p2);
+  };
+  // One pass over the levels: the previous iterate, the new value, its
+  // relaxation against the previous iterate, the depth integral (trapezoid
+  // from the base, yelmo_tools.f90:1092-1122) and, for DIVA, calc_F_integral
+  // with n = 2 (velocity_diva.f90:824-869), each level's relaxed value held in
+  // registers for the next level's trapezoid.
+  const bool relax = vm != 0 || diva;
+  const R H = f.H_ice[cc.c0 + c];
+  R s = R(0), F2 = R(0), vprev = R(0), fprev = R(0);
+  for (int k = 0; k < nz; ++k) {
+    const long long q = c + k * nc;
+    const R old = visc[q];
+    R v = level_value(k);
+    if (relax) v = yexp<R>((RS(1.0) - rel) * ylog<R>(old) + rel * ylog<R>(v));
+    visc[q] = v;
+    if (ice) {
+      const R fk = diva ? (H / v) * ((R(1) - zeta[k]) * (R(1) - zeta[k])) : R(0);
+      if (k > 0) {
+        const R mid = R(0.5) * (v + vprev);
+        s = s + mid * (zeta[k] - zeta[k - 1]);
+        if (diva) F2 = F2 + R(0.5) * (fk + fprev) * (zeta[k] - zeta[k - 1]);
       }
-    } else {
-      const R p1 = (R(1) - n_glen) / (R(2) * n_glen);
-      const R p2 = -R(1) / n_glen;
-      const R* ux = (p.diva ? f.ubar_x : f.ux) + cc.c0;
-      const R* uy = (p.diva ? f.ubar_y : f.uy) + cc.c0;
-      const R dudx_aa = (ux[c] - ux[I2(im1, j)]) / dx;
-      const R dvdy_aa = (uy[c] - uy[I2(i, jm1)]) / dy;
-      const R dudy_aa_1 = (ux[I2(i, jp1)] - ux[I2(i, jm1)]) / (R(2) * dy);
-      const R dudy_aa_2 = (ux[I2(im1, jp1)] - ux[I2(im1, jm1)]) / (R(2) * dy);
-      const R dudy_aa = R(0.5) * (dudy_aa_1 + dudy_aa_2);
-      const R dvdx_aa_1 = (uy[I2(ip1, j)] - uy[I2(im1, j)]) / (R(2) * dx);
-      const R dvdx_aa_2 = (uy[I2(ip1, jm1)] - uy[I2(im1, jm1)]) / (R(2) * dx);
-      const R dvdx_aa = R(0.5) * (dvdx_aa_1 + dvdx_aa_2);
-      const R s = dudy_aa + dvdx_aa;
-      const R eps_sq_h = dudx_aa * dudx_aa + dvdy_aa * dvdy_aa + dudx_aa * dvdy_aa +
-                         R(0.25) * (s * s);
-      const R eps_sq_aa = eps_sq_h + eps_0_sq;
-      for (int k = 0; k < nz; ++k) {
-        R e = eps_sq_aa;
-        if (p.diva) {
-          // DIVA aa form: shear of the cell's own faces, averaged to the centre
-          const R* dzx = f.duxdz + cc.z0 + k * nc;
-          const R* dzy = f.duydz + cc.z0 + k * nc;
-          const R ddx = R(0.5) * (dzx[c] + dzx[I2(im1, j)]);
-          const R ddy = R(0.5) * (dzy[c] + dzy[I2(i, jm1)]);
-          e = eps_sq_h + R(0.25) * (ddx * ddx) + R(0.25) * (ddy * ddy) + eps_0_sq;
-        }
-        visc[c + k * nc] = R(0.5) * ypow<R>(e, p1) * ypow<R>(ATT[c + k * nc], p2);
-      }
+      vprev = v;
+      fprev = fk;
     }
   }
-  if (p.visc_method != 0 || p.diva) {
-    for (int k = 0; k < nz; ++k) {
-      const long long q = c + k * nc;
-      visc[q] = yexp<R>((RS(1.0) - rel) * ylog<R>(vnm1[q]) + rel * ylog<R>(visc[q]));
-    }
-  }
-  // depth integral (trapezoid from the base, yelmo_tools.f90:1092-1122)
-  R vi;
-  if (fi[c] == R(1)) {
-    R s = R(0);
-    for (int k = 1; k < nz; ++k) {
-      const R mid = R(0.5) * (visc[c + k * nc] + visc[c + (k - 1) * nc]);
-      s = s + mid * (zeta[k] - zeta[k - 1]);
-    }
-    vi = s * f.H_ice[cc.c0 + c];
-  } else {
-    vi = visc_min;
-  }
+  R vi = ice ? s * H : visc_min;
   if (vi < visc_min) vi = visc_min;
   f.visc_int[cc.c0 + c] = vi;
-  if (p.diva) {
-    // calc_F_integral with n = 2 (velocity_diva.f90:824-869)
-    R F2 = R(0);
-    if (fi[c] == R(1)) {
-      const R H = f.H_ice[cc.c0 + c];
-      R prev = (H / visc[c]) * ((R(1) - zeta[0]) * (R(1) - zeta[0]));
-      for (int k = 1; k < nz; ++k) {
-        const R cur = (H / visc[c + k * nc]) * ((R(1) - zeta[k]) * (R(1) - zeta[k]));
-        F2 = F2 + R(0.5) * (cur + prev) * (zeta[k] - zeta[k - 1]);
-        prev = cur;
-      }
-    }
-    f.F2[cc.c0 + c] = F2;
-  }
+  if (diva) f.F2[cc.c0 + c] = F2;
 }
 
 // ---- basal friction on aa-nodes (basal_dragging.f90:384-579, 949-1292) ---
@@ -440,7 +440,7 @@ YG_DEV R beta_point(const YgFields<R>& f, const YgFieldParams& p, const CellCtx&
 }
 
 template <typename R>
-__global__ void k_beta(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+__global__ __launch_bounds__(256) void k_beta(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
@@ -514,7 +514,7 @@ YG_DEV void lim_drag_face(R& k, R& r, R u, R u_cross, int mask, R s0, R inv_du, 
 }
 
 template <typename R>
-__global__ void k_stagger(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+__global__ __launch_bounds__(256) void k_stagger(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
@@ -641,7 +641,7 @@ YG_DEV void store_record(double* A64, float* A32, size_t base, const double (&a)
 }
 
 template <typename R>
-__global__ void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
+__global__ __launch_bounds__(256) void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
                            double* A64, float* A32, double* bvec, double* x0, uint32_t* maskv) {
   // one thread per unknown pair in the reference's row order: n = i*ny + j
   const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1022,7 +1022,7 @@ YG_DEV void en_member(en::Row& r, const en::Geo& g, const en::In<R>& in, const Y
 }
 
 template <typename R>
-__global__ void k_assemble_energy(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
+__global__ __launch_bounds__(256) void k_assemble_energy(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
                                   double* A64, float* A32, double* bvec, double* x0,
                                   uint32_t* maskv) {
   const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1260,7 +1260,7 @@ __global__ __launch_bounds__(1024) void k_finish(YgBatchGeom G, YgFields<R> f, Y
 
 // ---- before the loop: set_inactive_margins (velocity_general.f90:1687-1750)
 template <typename R>
-__global__ void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+__global__ __launch_bounds__(256) void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
@@ -1314,7 +1314,10 @@ void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFi
                           const int* skip, hipStream_t st) {
   const dim3 grid(nblk(G.total_cells, 256)), blk(256);
   hipLaunchKernelGGL(k_strain<R>, grid, blk, 0, st, G, f, p, skip);
-  hipLaunchKernelGGL(k_visc<R>, grid, blk, 0, st, G, f, p, skip);
+  if (p.visc_method == 1 && !p.diva && f.attp)
+    hipLaunchKernelGGL((k_visc<R, true>), grid, blk, 0, st, G, f, p, skip);
+  else
+    hipLaunchKernelGGL((k_visc<R, false>), grid, blk, 0, st, G, f, p, skip);
   hipLaunchKernelGGL(k_beta<R>, grid, blk, 0, st, G, f, p, skip);
   hipLaunchKernelGGL(k_stagger<R>, grid, blk, 0, st, G, f, p, skip);
 }

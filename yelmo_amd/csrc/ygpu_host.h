@@ -336,7 +336,6 @@ struct Batch {
     for (R** p : r2d) YG_CHECK(h, ar.alloc(p, C));
     YG_CHECK(h, ar.alloc((R**)&f.ATT, Z));
     YG_CHECK(h, ar.alloc(&f.visc, Z));
-    YG_CHECK(h, ar.alloc(&f.visc_nm1, Z));
     if (fp.visc_method == 1) YG_CHECK(h, ar.alloc(&f.attp, 4 * Z));
     YG_CHECK(h, ar.alloc((R**)&f.zeta, (size_t)zstride * nmem));
     f.zstride = zstride;

@@ -252,7 +252,7 @@ struct Server : YgServerBase {
     else c += YG_NF * 8.0;
     if (b.fp.energy) c += 2 * 4.0;
     if (b.kmode == YG_MODE_MIXED && b.opts.k.factor16) c += 2.0 * YG_NF16;
-    c += sizeof(R) * (34.0 + 3.0 * nz + (b.diva ? 8.0 + 4.0 * nz : 0.0) +
+    c += sizeof(R) * (34.0 + 2.0 * nz + (b.diva ? 8.0 + 4.0 * nz : 0.0) +
                       (b.fp.visc_method == 1 ? 4.0 * nz : 0.0));
     return c;
   }
