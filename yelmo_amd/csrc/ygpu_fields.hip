@@ -27,6 +27,10 @@ template <typename R> YG_DEV R ylog(R a);
 template <> YG_DEV float ylog<float>(float a) { return logf(a); }
 template <> YG_DEV double ylog<double>(double a) { return log(a); }
 template <typename R> YG_DEV R yabs(R a) { return a < R(0) ? -a : a; }
+// a^b as exp(b log a) for a > 0: a few units in the last place of
+// |b log a| against pow's correctly rounded result, at a fraction of its cost
+// (the DIVA viscosity takes it at every node of every level)
+template <typename R> YG_DEV R ypow_el(R a, R b) { return yexp<R>(b * ylog<R>(a)); }
 
 // default-real literal of the reference (single precision), converted to R
 #define RS(x) ((R)(float)(x))
@@ -164,6 +168,57 @@ YG_DEV R mean_act(const R (&v)[K], const bool (&a)[K]) {
   return cnt > 0 ? s / (R)cnt : R(0);
 }
 
+// The two points of each corner (SW, SE, NE, NW) that the corner value of an
+// acx (kind 1) or acy (kind 2) field averages
+YG_DEV void gq_pts(int kind, int (&cs)[4][2], int nx, int i, int j, int im1, int ip1, int jm1,
+                   int jp1) {
+  if (kind == 1) {
+    cs[0][0] = I2(im1, jm1); cs[0][1] = I2(im1, j);
+    cs[1][0] = I2(i, jm1);   cs[1][1] = I2(i, j);
+    cs[2][0] = I2(i, j);     cs[2][1] = I2(i, jp1);
+    cs[3][0] = I2(im1, j);   cs[3][1] = I2(im1, jp1);
+  } else {
+    cs[0][0] = I2(im1, jm1); cs[0][1] = I2(i, jm1);
+    cs[1][0] = I2(i, jm1);   cs[1][1] = I2(ip1, jm1);
+    cs[2][0] = I2(i, j);     cs[2][1] = I2(ip1, j);
+    cs[3][0] = I2(im1, j);   cs[3][1] = I2(i, j);
+  }
+}
+
+// The activity flags of those 8 points (bit 2 corner + point), for a field
+// that is interpolated on many levels of one cell
+template <typename ACT>
+YG_DEV uint32_t gq_mask(int kind, ACT act, int nx, int i, int j, int im1, int ip1, int jm1,
+                        int jp1) {
+  int cs[4][2];
+  gq_pts(kind, cs, nx, i, j, im1, ip1, jm1, jp1);
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (act(cs[q][k])) m |= 1u << (2 * q + k);
+  return m;
+}
+
+// gq_nodes of an acx or acy field with the activity flags of gq_mask
+template <typename R>
+YG_DEV void gq_nodes_m(int kind, R n[4], const R* v, uint32_t m, int nx, int i, int j, int im1,
+                       int ip1, int jm1, int jp1) {
+  int cs[4][2];
+  gq_pts(kind, cs, nx, i, j, im1, ip1, jm1, jp1);
+  R c[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    R vv[2];
+    bool aa[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) { vv[k] = v[cs[q][k]]; aa[k] = (m >> (2 * q + k)) & 1u; }
+    c[q] = mean_act<R, 2>(vv, aa);
+  }
+  corners_to_nodes<R>(c, n);
+}
+
 // kind 0 aa (act: flags or all), 1 acx, 2 acy
 template <typename R, typename ACT>
 YG_DEV void gq_nodes(int kind, R n[4], const R* v, ACT act, int nx, int i, int j, int im1,
@@ -184,17 +239,7 @@ YG_DEV void gq_nodes(int kind, R n[4], const R* v, ACT act, int nx, int i, int j
     }
   } else {
     int cs[4][2];
-    if (kind == 1) {
-      cs[0][0] = I2(im1, jm1); cs[0][1] = I2(im1, j);
-      cs[1][0] = I2(i, jm1);   cs[1][1] = I2(i, j);
-      cs[2][0] = I2(i, j);     cs[2][1] = I2(i, jp1);
-      cs[3][0] = I2(im1, j);   cs[3][1] = I2(im1, jp1);
-    } else {
-      cs[0][0] = I2(im1, jm1); cs[0][1] = I2(i, jm1);
-      cs[1][0] = I2(i, jm1);   cs[1][1] = I2(ip1, jm1);
-      cs[2][0] = I2(i, j);     cs[2][1] = I2(ip1, j);
-      cs[3][0] = I2(im1, j);   cs[3][1] = I2(i, j);
-    }
+    gq_pts(kind, cs, nx, i, j, im1, ip1, jm1, jp1);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       R vv[2];
@@ -212,11 +257,13 @@ YG_DEV void gq_nodes(int kind, R n[4], const R* v, ACT act, int nx, int i, int j
 // velocity_diva.f90:573-822 add 1/4 (du/dz)^2 + 1/4 (dv/dz)^2),
 // picard_relax_visc (velocity_general.f90:2188-2213), calc_visc_eff_int
 // (velocity_general.f90:2215-2261).
-// SSA1: the SSA form of visc_method 1 with ATT^(-1/n) at the nodes filled once
-// per solve (the common case), compiled without the other forms' code so that
-// it needs few registers.
-template <typename R, bool SSA1>
-__global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
+// FORM 1: the SSA form of visc_method 1, FORM 2: the DIVA form of visc_method
+// 1, both with ATT^(-1/n) at the nodes filled once per solve (the common
+// cases), each compiled without the other forms' code so that it needs few
+// registers (the DIVA form without spills needs the 256-thread bound it is
+// launched with); FORM 0: every form.
+template <typename R, int FORM>
+__global__ __launch_bounds__(FORM == 2 ? 256 : 1024) void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
   const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
@@ -231,8 +278,8 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
   const R rel = (R)p.iter_rel;
   const R n_glen = (R)p.n_glen;
   const bool ice = fi[c] == R(1);
-  const int vm = SSA1 ? 1 : p.visc_method;
-  const bool diva = SSA1 ? false : (bool)p.diva;
+  const int vm = FORM ? 1 : p.visc_method;
+  const bool diva = FORM ? FORM == 2 : (bool)p.diva;
   const R eps_0 = (R)p.eps_0, eps_0_sq = eps_0 * eps_0;
   int im1, ip1, jm1, jp1;
   nbr(i, j, nx, ny, BC, im1, ip1, jm1, jp1);
@@ -243,6 +290,7 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
   R eps_sq_n[4], epow[4];        // visc_method 1: strain rates at the quadrature nodes
   R eps_sq_h = R(0), eps_sq_aa = R(0);   // visc_method 2: the cell-centre strain rate
   const R* attp = nullptr;
+  uint32_t mdzx = 0, mdzy = 0;   // DIVA, visc_method 1: activity of the shear points
   if (vm == 1 && ice) {
     const R* dudx = f.dudx + cc.c0;
     const R* dudy = f.dudy + cc.c0;
@@ -268,8 +316,12 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
     if (!diva) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) epow[q] = ypow<R>(eps_sq_n[q], p1);
+    } else {
+      // the shear fields' activity flags are the same on every level
+      mdzx = gq_mask(1, ax, nx, i, j, im1, ip1, jm1, jp1);
+      mdzy = gq_mask(2, ay, nx, i, j, im1, ip1, jm1, jp1);
     }
-    attp = (SSA1 || f.attp) ? f.attp + 4 * cc.z0 + c : nullptr;
+    attp = (FORM || f.attp) ? f.attp + 4 * cc.z0 + c : nullptr;
   } else if (vm == 2 && ice) {
     const R* ux = (diva ? f.ubar_x : f.ux) + cc.c0;
     const R* uy = (diva ? f.ubar_y : f.uy) + cc.c0;
@@ -290,8 +342,6 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
     if (vm == 0) return (R)p.visc_const;
     if (!ice) return visc_min;
     if (vm == 1) {
-      auto ax = [&](int qq) { return act_x<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
-      auto ay = [&](int qq) { return act_y<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
       auto all = [&](int) { return true; };
       // ATT^p2 at the nodes: filled once per solve, or computed here
       R ATTp[4];
@@ -308,12 +358,12 @@ __global__ void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int*
       if (diva) {
         R dzx[4], dzy[4];
         // DIVA: vertical shear at the nodes from the acx/acy shear fields
-        gq_nodes<R>(1, dzx, f.duxdz + cc.z0 + k * nc, ax, nx, i, j, im1, ip1, jm1, jp1);
-        gq_nodes<R>(2, dzy, f.duydz + cc.z0 + k * nc, ay, nx, i, j, im1, ip1, jm1, jp1);
+        gq_nodes_m<R>(1, dzx, f.duxdz + cc.z0 + k * nc, mdzx, nx, i, j, im1, ip1, jm1, jp1);
+        gq_nodes_m<R>(2, dzy, f.duydz + cc.z0 + k * nc, mdzy, nx, i, j, im1, ip1, jm1, jp1);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          ep[q] = ypow<R>(eps_sq_n[q] + R(0.25) * (dzx[q] * dzx[q]) +
-                              R(0.25) * (dzy[q] * dzy[q]) + eps_0_sq, p1);
+          ep[q] = ypow_el<R>(eps_sq_n[q] + R(0.25) * (dzx[q] * dzx[q]) +
+                                 R(0.25) * (dzy[q] * dzy[q]) + eps_0_sq, p1);
       } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q) ep[q] = epow[q];
@@ -1314,10 +1364,12 @@ void yg_launch_fields_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgFi
                           const int* skip, hipStream_t st) {
   const dim3 grid(nblk(G.total_cells, 256)), blk(256);
   hipLaunchKernelGGL(k_strain<R>, grid, blk, 0, st, G, f, p, skip);
-  if (p.visc_method == 1 && !p.diva && f.attp)
-    hipLaunchKernelGGL((k_visc<R, true>), grid, blk, 0, st, G, f, p, skip);
+  if (p.visc_method == 1 && f.attp && !p.diva)
+    hipLaunchKernelGGL((k_visc<R, 1>), grid, blk, 0, st, G, f, p, skip);
+  else if (p.visc_method == 1 && f.attp)
+    hipLaunchKernelGGL((k_visc<R, 2>), grid, blk, 0, st, G, f, p, skip);
   else
-    hipLaunchKernelGGL((k_visc<R, false>), grid, blk, 0, st, G, f, p, skip);
+    hipLaunchKernelGGL((k_visc<R, 0>), grid, blk, 0, st, G, f, p, skip);
   hipLaunchKernelGGL(k_beta<R>, grid, blk, 0, st, G, f, p, skip);
   hipLaunchKernelGGL(k_stagger<R>, grid, blk, 0, st, G, f, p, skip);
 }
