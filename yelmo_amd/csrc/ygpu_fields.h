@@ -126,7 +126,14 @@ struct YgPicardState {
   int* lin_iter;      // per member sums
   int* lin_fail;
   int* lim_n;
+  // the epilogue's blocks per member (eparts > 1 needs both arrays): partial
+  // sums [member][part][4] and per-member arrival counters, zero between launches
+  double* epart = nullptr;
+  int* eticket = nullptr;
+  int eparts = 1;
 };
+
+#define YG_EPI_BLOCKS 8   // epilogue blocks per member
 
 // One Picard epilogue per member not yet done; each member's iteration count
 // (iter_now) advances by one.

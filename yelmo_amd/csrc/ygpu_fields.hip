@@ -1161,7 +1161,7 @@ __global__ __launch_bounds__(256) void k_assemble_energy(YgBatchGeom G, YgFields
   maskv[gq] = pm[0] | (pm[1] << 16);
 }
 
-// ---- Picard epilogue (one block per member) --------------------------------
+// ---- Picard epilogue (ps.eparts blocks per member) -------------------------
 // linear_solver_save_velocity (solver_ssa_ac.f90:33-71), ssa_vel_clip
 // (:1069-1093), picard_relax_vel, picard_calc_convergence_l2 and _l1rel
 // (velocity_general.f90:1907-2186).
@@ -1170,12 +1170,17 @@ __global__ __launch_bounds__(256) void k_assemble_energy(YgBatchGeom G, YgFields
 // For DIVA the basal stress and basal velocity of the member follow in the
 // same pass (calc_basal_stress, velocity_diva.f90:1011-1050; calc_vel_basal,
 // :960-1009).
+// A member's cells are split over ps.eparts blocks.  Each block publishes its
+// block-reduced sums; the block that arrives last (a per-member counter, reset
+// to zero by that block) adds the parts in part order, so the result does not
+// depend on which block finishes when, and takes the convergence decision.
 template <typename R>
 __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f, YgFieldParams p,
                                                    const double* x, const int* lin_iters,
                                                    const int* lin_status, int iter_max,
                                                    YgPicardState ps) {
-  const int m = blockIdx.x;
+  const int EP = ps.eparts > 1 && ps.epart && ps.eticket ? ps.eparts : 1;
+  const int m = blockIdx.x / EP, part = blockIdx.x - m * EP;
   if (ps.done[m]) return;
   const int iter = ps.iter_now[m] + 1;
   const YgMember mb = G.members[m];
@@ -1192,7 +1197,8 @@ __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f,
   const R etol = RS(1e-2), etol2 = RS(1e-5);
   __shared__ double red[16 * 4];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};  // res1, res2, nchk, nonfinite
-  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+  const int chunk = (ncell + EP - 1) / EP, c_lo = part * chunk, c_hi = min(ncell, c_lo + chunk);
+  for (int c = c_lo + threadIdx.x; c < c_hi; c += blockDim.x) {
     const int i = c % nx, j = c / nx;
     const long long n = c0 + (long long)i * ny + j;
     long long sx = 2 * n, sy = 2 * n + 1;
@@ -1248,6 +1254,18 @@ __global__ __launch_bounds__(1024) void k_epilogue(YgBatchGeom G, YgFields<R> f,
     }
   }
   yg_block_sum<4>(acc, red);
+  if (threadIdx.x == 0 && EP > 1) {
+    double* mine = ps.epart + ((size_t)m * EP + part) * 4;
+    for (int k = 0; k < 4; ++k) mine[k] = acc[k];
+    __threadfence();   // the part is visible before the arrival is counted
+    if (atomicAdd(&ps.eticket[m], 1) != EP - 1) return;
+    __threadfence();   // every other part is visible to the last block
+    const double* parts = ps.epart + (size_t)m * EP * 4;
+    for (int k = 0; k < 4; ++k) acc[k] = 0.0;
+    for (int q = 0; q < EP; ++q)
+      for (int k = 0; k < 4; ++k) acc[k] += parts[q * 4 + k];
+    ps.eticket[m] = 0;
+  }
   if (threadIdx.x == 0) {
     const R resid = acc[2] > 0.0 ? (R)(sqrt(acc[0]) / (sqrt(acc[1]) + 1e-10)) : R(0);
     int d = 0;
@@ -1390,7 +1408,8 @@ template <typename R>
 void yg_launch_epilogue(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
                         const double* x, const int* lin_iters, const int* lin_status,
                         int iter_max, YgPicardState ps, hipStream_t st) {
-  hipLaunchKernelGGL(k_epilogue<R>, dim3(G.nmem), dim3(1024), 0, st, G, f, p, x, lin_iters,
+  const int ep = ps.eparts > 1 && ps.epart && ps.eticket ? ps.eparts : 1;
+  hipLaunchKernelGGL(k_epilogue<R>, dim3(G.nmem * ep), dim3(1024), 0, st, G, f, p, x, lin_iters,
                      lin_status, iter_max, ps);
 }
 

@@ -109,6 +109,8 @@ struct Batch {
   int *d_done = nullptr, *d_iter_now = nullptr, *d_lin_iter = nullptr, *d_lin_fail = nullptr,
       *d_lim_n = nullptr, *d_kit = nullptr, *d_kst = nullptr;
   double *d_resid = nullptr, *d_kres = nullptr;
+  double* d_epart = nullptr;   // epilogue partial sums, YG_EPI_BLOCKS x 4 per member
+  int* d_eticket = nullptr;    // epilogue arrival counters
   YgFields<R> f{};
   YgDivaFields<R> dv{};
 
@@ -322,6 +324,9 @@ struct Batch {
     d_lim_n = ints[4]; d_kit = ints[5]; d_kst = ints[6];
     YG_CHECK(h, ar.alloc(&d_resid, nmem));
     YG_CHECK(h, ar.alloc(&d_kres, nmem));
+    YG_CHECK(h, ar.alloc(&d_epart, (size_t)nmem * YG_EPI_BLOCKS * 4));
+    YG_CHECK(h, ar.alloc(&d_eticket, nmem));
+    YG_CHECK(h, hipMemsetAsync(d_eticket, 0, sizeof(int) * nmem, h->stream));
     if (group > 1) {
       YG_CHECK(h, ar.alloc(&d_gsync, (size_t)ngroups * (YG_GSYNC_BYTES / 4)));
       YG_CHECK(h, ar.alloc(&d_gfail, 1));
@@ -669,7 +674,8 @@ struct Batch {
     YG_CHECK(h, hipEventRecord(e0, st));
     const YgBatchGeom G = geom();
     yg_launch_init<R>(G, f, fp, nullptr, st);
-    YgPicardState ps{d_iter_now, d_done, d_resid, d_lin_iter, d_lin_fail, d_lim_n};
+    YgPicardState ps{d_iter_now, d_done, d_resid, d_lin_iter, d_lin_fail, d_lim_n,
+                     d_epart, d_eticket, YG_EPI_BLOCKS};
     YgKrylovArgs ka{};
     ka.members = d_mem; ka.soff = d_soff; ka.mask = d_mask;
     ka.A64 = d_A64; ka.A32 = d_A32; ka.F64 = d_F64; ka.F32 = d_F32; ka.F16 = d_F16;

@@ -454,7 +454,9 @@ struct Server : YgServerBase {
   }
   YgPicardState pstate(const Lane& L) const {
     return YgPicardState{B.d_iter_now + L.s0, B.d_done + L.s0, B.d_resid + L.s0,
-                         B.d_lin_iter + L.s0, B.d_lin_fail + L.s0, B.d_lim_n + L.s0};
+                         B.d_lin_iter + L.s0, B.d_lin_fail + L.s0, B.d_lim_n + L.s0,
+                         B.d_epart + (size_t)L.s0 * YG_EPI_BLOCKS * 4, B.d_eticket + L.s0,
+                         YG_EPI_BLOCKS};
   }
 
   // copies one admitted solve into its slot (asynchronous on the lane's stream)
