@@ -28,7 +28,7 @@ YG_DEV R stag_margin(R v0, R v1, R f0, R f1) {
 // calc_vertical_shear_3D (velocity_diva.f90:510-571)
 template <typename R>
 __global__ __launch_bounds__(256) void k_shear(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gc = yg_xcd_cell();
   if (gc >= G.total_cells) return;
   const int m = yg_member_at(G, gc);
   if (m < 0 || (skip && skip[m])) return;
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_shear(YgBatchGeom G, YgFields<R> f, YgF
 template <typename R>
 __global__ __launch_bounds__(256) void k_vel3d(YgBatchGeom G, YgFields<R> f, YgDivaFields<R> d, YgFieldParams p,
                         const int* skip) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gc = yg_xcd_cell();
   if (gc >= G.total_cells) return;
   const int m = yg_member_at(G, gc);
   if (m < 0 || (skip && skip[m])) return;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_vel3d(YgBatchGeom G, YgFields<R> f, YgD
 
 }  // namespace
 
-static inline unsigned dnblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline unsigned dnblk(long long n, int b) { return yg_xcd_grid(n, b); }
 
 template <typename R>
 void yg_launch_diva_pre(const YgBatchGeom& G, const YgFields<R>& f, const YgDivaFields<R>&,

@@ -135,6 +135,23 @@ struct YgPicardState {
 
 #define YG_EPI_BLOCKS 8   // epilogue blocks per member
 
+// Cells of the per-cell field kernels in XCD-local ranges.  The dispatcher
+// deals workgroups round-robin over the 8 XCDs (workgroup b on XCD b % 8), so
+// workgroup b takes logical block (b % 8) * (gridDim / 8) + b / 8: each XCD
+// walks one contiguous eighth of the cells, and the neighbours of a cell one
+// grid line away are mostly read through that XCD's L2.  A grid that is not a
+// multiple of 8 blocks keeps the plain order.
+YG_DEV long long yg_xcd_cell() {
+  unsigned lb = blockIdx.x;
+  if ((gridDim.x & 7u) == 0) lb = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  return (long long)lb * blockDim.x + threadIdx.x;
+}
+// blocks of b threads for n cells, rounded up to a multiple of 8
+static inline unsigned yg_xcd_grid(long long n, int b) {
+  const unsigned nb = (unsigned)((n + b - 1) / b);
+  return (nb + 7u) & ~7u;
+}
+
 // One Picard epilogue per member not yet done; each member's iteration count
 // (iter_now) advances by one.
 template <typename R>

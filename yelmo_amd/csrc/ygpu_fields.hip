@@ -90,7 +90,7 @@ YG_DEV bool act_y(const R* fi, int nx, int ny, int BC, int i, int j) {
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_strain(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gc = yg_xcd_cell();
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
   const int nx = cc.nx, ny = cc.ny, i = cc.i, j = cc.j, BC = p.BC;
@@ -264,7 +264,7 @@ YG_DEV void gq_nodes(int kind, R n[4], const R* v, ACT act, int nx, int i, int j
 // launched with); FORM 0: every form.
 template <typename R, int FORM>
 __global__ __launch_bounds__(FORM == 2 ? 256 : 1024) void k_visc(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gc = yg_xcd_cell();
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
   const int nx = cc.nx, ny = cc.ny, nz = cc.nz, i = cc.i, j = cc.j, BC = p.BC;
@@ -491,7 +491,7 @@ YG_DEV R beta_point(const YgFields<R>& f, const YgFieldParams& p, const CellCtx&
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_beta(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gc = yg_xcd_cell();
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
   if (p.beta_method == -1) return;
@@ -565,7 +565,7 @@ YG_DEV void lim_drag_face(R& k, R& r, R u, R u_cross, int mask, R s0, R inv_du, 
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_stagger(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gc = yg_xcd_cell();
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
   const int nx = cc.nx, ny = cc.ny, i = cc.i, j = cc.j;
@@ -693,16 +693,19 @@ YG_DEV void store_record(double* A64, float* A32, size_t base, const double (&a)
 template <typename R>
 __global__ __launch_bounds__(256) void k_assemble(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
                            double* A64, float* A32, double* bvec, double* x0, uint32_t* maskv) {
-  // one thread per unknown pair in the reference's row order: n = i*ny + j
-  const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gq >= G.total_cells) return;
-  const int m = yg_member_at(G, gq);
+  // one thread per cell, in the fields' order (i fastest, so the field reads
+  // of a wave are coalesced); the cell's unknown pair, record, b and x0 sit
+  // at its index in the reference's row order n = i*ny + j
+  const long long gt = yg_xcd_cell();
+  if (gt >= G.total_cells) return;
+  const int m = yg_member_at(G, gt);
   if (m < 0 || (skip && skip[m])) return;
   const YgMember& mb = G.members[m];
   const long long c0 = G.coffs[m];
-  const int q = (int)(gq - c0);
+  const int q = (int)(gt - c0);
   const int nx = mb.nx, ny = mb.ny;
-  const int i = q / ny, j = q - i * ny;
+  const int i = q % nx, j = q / nx;
+  const long long gq = c0 + (long long)i * ny + j;
   const int ncell = nx * ny;
   // border types (solver_ssa_ac.f90:148-196): 0 no-slip, 1 free-slip, 2 periodic
   int bc[4];  // 0 right, 1 upper, 2 left, 3 lower
@@ -1075,15 +1078,18 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_assemble_energy(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip,
                                   double* A64, float* A32, double* bvec, double* x0,
                                   uint32_t* maskv) {
-  const long long gq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gq >= G.total_cells) return;
-  const int m = yg_member_at(G, gq);
+  // one thread per cell in the fields' order; outputs at the cell's index
+  // in the reference's row order n = i*ny + j
+  const long long gt = yg_xcd_cell();
+  if (gt >= G.total_cells) return;
+  const int m = yg_member_at(G, gt);
   if (m < 0 || (skip && skip[m])) return;
   const YgMember& mb = G.members[m];
   const long long c0 = G.coffs[m];
-  const int q = (int)(gq - c0);
+  const int q = (int)(gt - c0);
   const int nx = mb.nx, ny = mb.ny;
-  const int i = q / ny, j = q - i * ny;
+  const int i = q % nx, j = q / nx;
+  const long long gq = c0 + (long long)i * ny + j;
   en::Geo g;
   g.nx = nx; g.ny = ny;
   // border types (:144-167)
@@ -1329,7 +1335,7 @@ __global__ __launch_bounds__(1024) void k_finish(YgBatchGeom G, YgFields<R> f, Y
 // ---- before the loop: set_inactive_margins (velocity_general.f90:1687-1750)
 template <typename R>
 __global__ __launch_bounds__(256) void k_init(YgBatchGeom G, YgFields<R> f, YgFieldParams p, const int* skip) {
-  const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gc = yg_xcd_cell();
   CellCtx cc;
   if (!cell_of(G, gc, skip, cc)) return;
   const int nx = cc.nx, ny = cc.ny, i = cc.i, j = cc.j;
@@ -1369,7 +1375,8 @@ __global__ __launch_bounds__(256) void k_init(YgBatchGeom G, YgFields<R> f, YgFi
 
 }  // namespace
 
-static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+// grids of the per-cell kernels: a multiple of 8 blocks (yg_xcd_cell)
+static inline unsigned nblk(long long n, int b) { return yg_xcd_grid(n, b); }
 
 template <typename R>
 void yg_launch_init(const YgBatchGeom& G, const YgFields<R>& f, const YgFieldParams& p,
