@@ -1304,9 +1304,16 @@ YG_NOINLINE Sums2 spmv_dots(const Ctx& c, const TA* A, const TV* x, TV* y, const
   for (int q0 = cb + threadIdx.x; q0 < ce; q0 += 4 * B) {
     double ax[4], ay[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 4; ++k) {
       // a cell past the range reads as the zero record past the member
-      spmv_cell<TA, TV, COH>(nx, ny, wrap, rA, rX, q0 + k * B < ce ? q0 + k * B : nc, ax[k], ay[k]);
+      const int q = q0 + k * B < ce ? q0 + k * B : nc;
+      // only the first and last line of the range read lines that other
+      // workgroups wrote; the other cells' x loads may go through the L1
+      if (COH && (q - cb < ny || ce - q <= ny))
+        spmv_cell<TA, TV, true>(nx, ny, wrap, rA, rX, q, ax[k], ay[k]);
+      else
+        spmv_cell<TA, TV, false>(nx, ny, wrap, rA, rX, q, ax[k], ay[k]);
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int q = q0 + k * B;
