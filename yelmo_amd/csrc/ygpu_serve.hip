@@ -595,13 +595,15 @@ struct Server : YgServerBase {
     return YGPU_OK;
   }
 
-  // Member groups in the largest lane (mixed precision, BiCGSTAB, line-block
-  // ILU(0)): G workgroups per member, G from the batch rule for one domain of
-  // the lane's largest size, or -ygpu_group G; every member of the lane runs
-  // with that G.  Only one lane uses groups: its launches follow each other
-  // on its stream, and every kernel of the other lanes ends without waiting
-  // for anything, so the CUs a group still needs become free while its
-  // resident workgroups wait.
+  // Member groups (mixed precision, BiCGSTAB, line-block ILU(0)): G
+  // workgroups per member, every member of a lane with its lane's G.
+  // -ygpu_group G sets G for the largest lane.  By default each lane takes G
+  // from the batch rule for one domain of its largest size, largest lane
+  // first, as long as the Krylov launches of all lanes (slots rounded up to 8,
+  // times G workgroups, each holding a CU) fit the CUs at once.  Then every
+  // workgroup of every group can be resident together, so no group waits for
+  // a CU that another group holds, and every other kernel ends without
+  // waiting for anything.
   int setup_groups() {
     const int want = B.opts.group;
     const bool fits = B.kmode == YG_MODE_MIXED && B.opts.k.solver == 4 && B.opts.k.precon == 2 &&
@@ -613,24 +615,42 @@ struct Server : YgServerBase {
     }
     int ncu = 0;
     YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
-    Lane& L = lanes.back();
-    const long long cells = (long long)L.side * L.side;
-    int G = want;
-    if (G == 0) {
-      G = 1;
-      for (int g : {8, 4, 2})
-        if (8 * g <= ncu && cells / g >= 4096) { G = g; break; }
+    auto groups8 = [](const Lane& L) { return ((L.ns + 7) / 8) * 8; };
+    auto enable = [&](Lane& L, int G) -> int {
+      L.group = G;
+      L.ngroups = groups8(L);
+      YG_CHECK(h, B.ar.alloc(&L.d_gsync, (size_t)L.ngroups * (YG_GSYNC_BYTES / 4)));
+      YG_CHECK(h, B.ar.alloc(&L.d_gfail, 1));
+      YG_CHECK(h, hipMemsetAsync(L.d_gfail, 0, sizeof(int), h->stream));
+      YG_CHECK(h, hipHostMalloc((void**)&L.h_gfail, sizeof(int), hipHostMallocDefault));
+      *L.h_gfail = 0;
+      return YGPU_OK;
+    };
+    auto fits_lines = [&](const Lane& L, int G) { return L.side <= 64 * G * (YG_GROUP_THREADS / 64); };
+    if (want != 0) {
+      Lane& L = lanes.back();
+      int G = want;
+      while (G > 1 && groups8(L) * G > ncu) G /= 2;
+      if (G >= 2 && fits_lines(L, G)) {
+        int rc = enable(L, G);
+        if (rc) return rc;
+      }
+    } else {
+      int used = 0;   // Krylov workgroups of all lanes per launch round
+      for (const Lane& L : lanes) used += groups8(L);
+      for (auto it = lanes.rbegin(); it != lanes.rend(); ++it) {
+        Lane& L = *it;
+        const long long cells = (long long)L.side * L.side;
+        int G = 1;
+        for (int g : {8, 4, 2})
+          if (8 * g <= ncu && cells / g >= 4096) { G = g; break; }
+        while (G > 1 && used + groups8(L) * (G - 1) > ncu) G /= 2;
+        if (G < 2 || !fits_lines(L, G)) continue;
+        used += groups8(L) * (G - 1);
+        int rc = enable(L, G);
+        if (rc) return rc;
+      }
     }
-    const int groups8 = ((L.ns + 7) / 8) * 8;
-    while (G > 1 && groups8 * G > ncu) G /= 2;
-    if (G < 2 || L.side > 64 * G * (YG_GROUP_THREADS / 64)) return YGPU_OK;
-    L.group = G;
-    L.ngroups = groups8;
-    YG_CHECK(h, B.ar.alloc(&L.d_gsync, (size_t)L.ngroups * (YG_GSYNC_BYTES / 4)));
-    YG_CHECK(h, B.ar.alloc(&L.d_gfail, 1));
-    YG_CHECK(h, hipMemsetAsync(L.d_gfail, 0, sizeof(int), h->stream));
-    YG_CHECK(h, hipHostMalloc((void**)&L.h_gfail, sizeof(int), hipHostMallocDefault));
-    *L.h_gfail = 0;
     YG_CHECK(h, hipStreamSynchronize(h->stream));
     return YGPU_OK;
   }
