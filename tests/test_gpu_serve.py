@@ -190,3 +190,29 @@ def test_served_group_lane_alone_equals_in_mix():
                 assert np.array_equal(st_m[n], st_a[n]), (k, j, n)
     for j in range(len(small[0])):
         assert np.array_equal(mixed[len(seqs)][j][0]["ux_b"], small_alone[j][0]["ux_b"]), j
+
+
+def test_served_group_lanes_alone_equals_in_mix():
+    """By default every lane whose domains are large enough takes member
+    groups (here 97^2: 2 workgroups per member, 129^2: 4), both group kernels
+    in flight at once; each sequence gives the result of the same sequence
+    solved alone in a session, whose batch rule picks the same group size,
+    bit for bit."""
+    print("case serve group lanes", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    seqs = _sequences((91, 129, 97, 129), 3, np.float64, seed=41)
+    alone = [_alone(s, par) for s in seqs]
+    mixed, max_running = _served(seqs, par, budget=1 << 30, arrive_every=0,
+                                 lanes=[(97, 2), (129, 2)])
+    assert max_running >= 3
+    for k in range(len(seqs)):
+        for j in range(len(seqs[k])):
+            st_a, inf_a = alone[k][j]
+            st_m, inf_m = mixed[k][j]
+            print(k, j, "iters", inf_a.ssa_iter_now, inf_m.ssa_iter_now, "lin",
+                  inf_a.ssa_lin_iter, inf_m.ssa_lin_iter, flush=True)
+            assert inf_m.status == inf_a.status == 0
+            assert (inf_m.ssa_iter_now, inf_m.ssa_lin_iter) == (inf_a.ssa_iter_now,
+                                                               inf_a.ssa_lin_iter), (k, j)
+            for n in COMPARE:
+                assert np.array_equal(st_m[n], st_a[n]), (k, j, n)
