@@ -338,20 +338,24 @@ YG_DEV double dpp_shl1(double v) {
 // of the two most recent cells of each lane stay in registers; a lane reads
 // the line before it through wave shifts, and across waves (coupled scheme)
 // through an LDS ring.
+template <typename CT>
 struct URow {   // U part of the two rows of one cell
-  double x5, x3, x7, x1, x6, xd;  // ux row: slots 5,3,7,1,6, 1/diag
-  double y6, y1, y3, yd;          // uy row: slots 6,1,3, 1/diag
+  CT x5, x3, x7, x1, x6, xd;  // ux row: slots 5,3,7,1,6, 1/diag
+  CT y6, y1, y3, yd;          // uy row: slots 6,1,3, 1/diag
 };
 
-YG_DEV void urow_zero(URow& u) {
-  u.x5 = u.x3 = u.x7 = u.x1 = u.x6 = u.xd = 0.0;
-  u.y6 = u.y1 = u.y3 = u.yd = 0.0;
+template <typename CT>
+YG_DEV void urow_zero(URow<CT>& u) {
+  u.x5 = u.x3 = u.x7 = u.x1 = u.x6 = u.xd = CT(0);
+  u.y6 = u.y1 = u.y3 = u.yd = CT(0);
 }
 
-YG_DEV double shup(double v) { return dpp_shr1(v); }
+template <typename CT>
+YG_DEV CT shup(CT v) { return dpp_shr1(v); }
 
-// TA is both the matrix and the factor type (FP64 path: double; the FP32
-// inner solve of the mixed path: float)
+// TA is the matrix and the factor type, and the arithmetic type of the
+// elimination (FP64 path: double; the FP32 inner solve of the mixed path:
+// float, whose factors are FP32 in any case)
 template <typename TA>
 YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restrict__ F,
                         int* bad) {
@@ -394,7 +398,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
       }
     }
   };
-  URow o1, o2;  // own cells at steps t-1 and t-2
+  URow<TA> o1, o2;  // own cells at steps t-1 and t-2
   urow_zero(o1);
   urow_zero(o2);
   const int nch = (c.T + C - 1) / C;
@@ -405,12 +409,12 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
     const int sb = (ch & 1) * NV * stride;
     for (int t = t0; t < t1; ++t) {
       // P = line i-1 cell j (its step t-2), Q = line i-1 cell j+1 (its step t-1)
-      URow P, Q;
+      URow<TA> P, Q;
       P.x5 = shup(o2.x5); P.x3 = shup(o2.x3); P.x7 = shup(o2.x7); P.x1 = shup(o2.x1);
       P.x6 = shup(o2.x6); P.xd = shup(o2.xd); P.y6 = shup(o2.y6); P.y3 = shup(o2.y3);
-      P.yd = shup(o2.yd); P.y1 = 0.0;
+      P.yd = shup(o2.yd); P.y1 = TA(0);
       Q.x7 = shup(o1.x7); Q.x1 = shup(o1.x1); Q.x6 = shup(o1.x6); Q.xd = shup(o1.xd);
-      Q.x5 = Q.x3 = Q.y6 = Q.y1 = Q.y3 = Q.yd = 0.0;
+      Q.x5 = Q.x3 = Q.y6 = Q.y1 = Q.y3 = Q.yd = TA(0);
       if (ring_in) {
         const double* rp = yg_ring[w - 1][(t + 1) % 3];  // step t-2
         const double* rq = yg_ring[w - 1][(t + 2) % 3];  // step t-1
@@ -420,16 +424,16 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
         Q.y6 = rq[6]; Q.y1 = rq[7]; Q.y3 = rq[8]; Q.yd = rq[9];
       }
       const int l = step_slot(c, t, t0);
-      URow X;
+      URow<TA> X;
       urow_zero(X);
       if (l >= 0) {
         const int q = yg_soff[t] + c.line - glo_of(t, c.ny);
         const uint32_t m = stage_to_mask<TA>(lds<TA>(sb + 18 * stride + l));
-        double a[9], b[9];
+        TA a[9], b[9];
 #pragma unroll
         for (int s = 0; s < 9; ++s) {
-          a[s] = (double)lds<TA>(sb + (2 * s) * stride + l);
-          b[s] = (double)lds<TA>(sb + (2 * s + 1) * stride + l);
+          a[s] = lds<TA>(sb + (2 * s) * stride + l);
+          b[s] = lds<TA>(sb + (2 * s + 1) * stride + l);
         }
         uint32_t mx = m & 0x1ffu, my = (m >> 16) & 0x1ffu;
         if (!c.coupled && c.ncol > 1) {
@@ -449,8 +453,8 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
         }
 #define PX(s) ((mx >> (s)) & 1u)
 #define PY(s) ((my >> (s)) & 1u)
-        const URow& O = o1;
-        double lv;
+        const URow<TA>& O = o1;
+        TA lv;
         // ux row, lower entries in column order: ux(i-1,j), ux(i,j-1), uy(i,j-1)
         lv = a[2] * P.xd; a[2] = lv;
         if (PX(8)) a[8] -= lv * P.x7;
@@ -464,7 +468,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
         if (PX(0)) a[0] -= lv * O.y6;
         if (PX(5)) a[5] -= lv * O.y1;
         if (PX(7)) a[7] -= lv * O.y3;
-        X.xd = 1.0 / a[0];
+        X.xd = TA(1) / a[0];
         X.x5 = a[5]; X.x3 = a[3]; X.x7 = a[7]; X.x1 = a[1]; X.x6 = a[6];
         // uy row, lower entries: ux(i-1,j), uy(i-1,j), ux(i-1,j+1), uy(i,j-1), ux(i,j)
         lv = b[8] * P.xd; b[8] = lv;
@@ -487,7 +491,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
         if (PY(0)) b[0] -= lv * X.x5;
         if (PY(6)) b[6] -= lv * X.x3;
         if (PY(3)) b[3] -= lv * X.x6;
-        X.yd = 1.0 / b[0];
+        X.yd = TA(1) / b[0];
         X.y6 = b[6]; X.y1 = b[1]; X.y3 = b[3];
 #undef PX
 #undef PY
