@@ -185,23 +185,33 @@ YG_DEV void gq_pts(int kind, int (&cs)[4][2], int nx, int i, int j, int im1, int
   }
 }
 
-// The activity flags of those 8 points (bit 2 corner + point), for a field
-// that is interpolated on many levels of one cell
+// The activity flags of those 8 points (bit 2 corner + point) from a test on
+// their grid coordinates, for fields that share them (several fields, or one
+// field on many levels of a cell)
 template <typename ACT>
-YG_DEV uint32_t gq_mask(int kind, ACT act, int nx, int i, int j, int im1, int ip1, int jm1,
-                        int jp1) {
-  int cs[4][2];
-  gq_pts(kind, cs, nx, i, j, im1, ip1, jm1, jp1);
+YG_DEV uint32_t gq_mask_ij(int kind, ACT act, int i, int j, int im1, int ip1, int jm1, int jp1) {
+  int ci[4][2], cj[4][2];
+  if (kind == 1) {
+    ci[0][0] = im1; cj[0][0] = jm1; ci[0][1] = im1; cj[0][1] = j;
+    ci[1][0] = i;   cj[1][0] = jm1; ci[1][1] = i;   cj[1][1] = j;
+    ci[2][0] = i;   cj[2][0] = j;   ci[2][1] = i;   cj[2][1] = jp1;
+    ci[3][0] = im1; cj[3][0] = j;   ci[3][1] = im1; cj[3][1] = jp1;
+  } else {
+    ci[0][0] = im1; cj[0][0] = jm1; ci[0][1] = i;   cj[0][1] = jm1;
+    ci[1][0] = i;   cj[1][0] = jm1; ci[1][1] = ip1; cj[1][1] = jm1;
+    ci[2][0] = i;   cj[2][0] = j;   ci[2][1] = ip1; cj[2][1] = j;
+    ci[3][0] = im1; cj[3][0] = j;   ci[3][1] = i;   cj[3][1] = j;
+  }
   uint32_t m = 0;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
     for (int k = 0; k < 2; ++k)
-      if (act(cs[q][k])) m |= 1u << (2 * q + k);
+      if (act(ci[q][k], cj[q][k])) m |= 1u << (2 * q + k);
   return m;
 }
 
-// gq_nodes of an acx or acy field with the activity flags of gq_mask
+// gq_nodes of an acx or acy field with the activity flags of gq_mask_ij
 template <typename R>
 YG_DEV void gq_nodes_m(int kind, R n[4], const R* v, uint32_t m, int nx, int i, int j, int im1,
                        int ip1, int jm1, int jp1) {
@@ -296,13 +306,17 @@ __global__ __launch_bounds__(FORM == 2 ? 256 : 1024) void k_visc(YgBatchGeom G, 
     const R* dudy = f.dudy + cc.c0;
     const R* dvdx = f.dvdx + cc.c0;
     const R* dvdy = f.dvdy + cc.c0;
-    auto ax = [&](int qq) { return act_x<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
-    auto ay = [&](int qq) { return act_y<R>(fi, nx, ny, BC, qq % nx, qq / nx); };
+    // activity of the acx and acy points, shared by the strain-rate fields
+    // and (DIVA) the shear fields of every level
+    const uint32_t max_ = gq_mask_ij(1, [&](int qi, int qj) { return act_x<R>(fi, nx, ny, BC, qi, qj); },
+                                     i, j, im1, ip1, jm1, jp1);
+    const uint32_t may_ = gq_mask_ij(2, [&](int qi, int qj) { return act_y<R>(fi, nx, ny, BC, qi, qj); },
+                                     i, j, im1, ip1, jm1, jp1);
     R dudxn[4], dudyn[4], dvdxn[4], dvdyn[4];
-    gq_nodes<R>(1, dudxn, dudx, ax, nx, i, j, im1, ip1, jm1, jp1);
-    gq_nodes<R>(1, dudyn, dudy, ax, nx, i, j, im1, ip1, jm1, jp1);
-    gq_nodes<R>(2, dvdxn, dvdx, ay, nx, i, j, im1, ip1, jm1, jp1);
-    gq_nodes<R>(2, dvdyn, dvdy, ay, nx, i, j, im1, ip1, jm1, jp1);
+    gq_nodes_m<R>(1, dudxn, dudx, max_, nx, i, j, im1, ip1, jm1, jp1);
+    gq_nodes_m<R>(1, dudyn, dudy, max_, nx, i, j, im1, ip1, jm1, jp1);
+    gq_nodes_m<R>(2, dvdxn, dvdx, may_, nx, i, j, im1, ip1, jm1, jp1);
+    gq_nodes_m<R>(2, dvdyn, dvdy, may_, nx, i, j, im1, ip1, jm1, jp1);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const R s = dudyn[q] + dvdxn[q];
@@ -318,8 +332,8 @@ __global__ __launch_bounds__(FORM == 2 ? 256 : 1024) void k_visc(YgBatchGeom G, 
       for (int q = 0; q < 4; ++q) epow[q] = ypow<R>(eps_sq_n[q], p1);
     } else {
       // the shear fields' activity flags are the same on every level
-      mdzx = gq_mask(1, ax, nx, i, j, im1, ip1, jm1, jp1);
-      mdzy = gq_mask(2, ay, nx, i, j, im1, ip1, jm1, jp1);
+      mdzx = max_;
+      mdzy = may_;
     }
     attp = (FORM || f.attp) ? f.attp + 4 * cc.z0 + c : nullptr;
   } else if (vm == 2 && ice) {
@@ -448,11 +462,11 @@ YG_DEV R beta_point(const YgFields<R>& f, const YgFieldParams& p, const CellCtx&
         }
       } else {
         auto aa = [&](int qq_) { return fi[qq_] == R(1) && fg[qq_] > R(0); };
-        auto ax = [&](int qq_) { const int qi = qq_ % nx, qj = qq_ / nx; int a1, a2, a3, a4; nbr(qi, qj, nx, ny, BC, a1, a2, a3, a4); return fi[qq_] == R(1) || fi[I2(a2, qj)] == R(1); };
-        auto ay = [&](int qq_) { const int qi = qq_ % nx, qj = qq_ / nx; int a1, a2, a3, a4; nbr(qi, qj, nx, ny, BC, a1, a2, a3, a4); return fi[qq_] == R(1) || fi[I2(qi, a4)] == R(1); };
+        auto ax = [&](int qi, int qj) { int a1, a2, a3, a4; nbr(qi, qj, nx, ny, BC, a1, a2, a3, a4); return fi[I2(qi, qj)] == R(1) || fi[I2(a2, qj)] == R(1); };
+        auto ay = [&](int qi, int qj) { int a1, a2, a3, a4; nbr(qi, qj, nx, ny, BC, a1, a2, a3, a4); return fi[I2(qi, qj)] == R(1) || fi[I2(qi, a4)] == R(1); };
         gq_nodes<R>(0, cbn, cb, aa, nx, i, j, im1, ip1, jm1, jp1);
-        gq_nodes<R>(1, uxn, ux, ax, nx, i, j, im1, ip1, jm1, jp1);
-        gq_nodes<R>(2, uyn, uy, ay, nx, i, j, im1, ip1, jm1, jp1);
+        gq_nodes_m<R>(1, uxn, ux, gq_mask_ij(1, ax, i, j, im1, ip1, jm1, jp1), nx, i, j, im1, ip1, jm1, jp1);
+        gq_nodes_m<R>(2, uyn, uy, gq_mask_ij(2, ay, i, j, im1, ip1, jm1, jp1), nx, i, j, im1, ip1, jm1, jp1);
       }
       R sum = R(0);
 #pragma unroll
