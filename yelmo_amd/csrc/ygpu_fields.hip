@@ -29,7 +29,7 @@ template <> YG_DEV double ylog<double>(double a) { return log(a); }
 template <typename R> YG_DEV R yabs(R a) { return a < R(0) ? -a : a; }
 // a^b as exp(b log a) for a > 0: a few units in the last place of
 // |b log a| against pow's correctly rounded result, at a fraction of its cost
-// (the DIVA viscosity takes it at every node of every level)
+// (the viscosity and the friction law take it at every quadrature node)
 template <typename R> YG_DEV R ypow_el(R a, R b) { return yexp<R>(b * ylog<R>(a)); }
 
 // default-real literal of the reference (single precision), converted to R
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(FORM == 2 ? 256 : 1024) void k_visc(YgBatchGeom G, 
     // would compute nz times)
     if (!diva) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) epow[q] = ypow<R>(eps_sq_n[q], p1);
+      for (int q = 0; q < 4; ++q) epow[q] = ypow_el<R>(eps_sq_n[q], p1);
     } else {
       // the shear fields' activity flags are the same on every level
       mdzx = max_;
@@ -473,13 +473,13 @@ YG_DEV R beta_point(const YgFields<R>& f, const YgFieldParams& p, const CellCtx&
       for (int k = 0; k < 4; ++k) {
         const R uxyn = ysqrt<R>(uxn[k] * uxn[k] + uyn[k] * uyn[k] + ub_sq_min);
         const R ratio = coulomb ? uxyn / (uxyn + u0) : uxyn / u0;
-        sum += (cbn[k] * ypow<R>(ratio, qq) * (R(1) / uxyn)) * R(1);
+        sum += (cbn[k] * ypow_el<R>(ratio, qq) * (R(1) / uxyn)) * R(1);
       }
       beta = sum / R(4);
     } else {
       const R uxy_b = ub_min;
       const R ratio = coulomb ? uxy_b / (uxy_b + u0) : uxy_b / u0;
-      beta = cb[c] * ypow<R>(ratio, qq) * (R(1) / uxy_b);
+      beta = cb[c] * ypow_el<R>(ratio, qq) * (R(1) / uxy_b);
     }
   }
   const R q_frz = (bm == 0 || bm == 1) ? R(1) : q;
