@@ -58,6 +58,14 @@ extern "C" {
 
 typedef struct ygpu_handle ygpu_handle;
 
+/* The optional neff_hook of calc_velocity_diva (velocity_diva.f90:99-107):
+ * called on the host after every Picard iteration, once the iteration's
+ * basal velocity (ux_b, uy_b: (nx,ny) arrays of real_bytes each, the
+ * caller's own arrays) is known; it updates c_bed in place (the same real
+ * type).  ctx is the hook_ctx the caller registered.                        */
+typedef void (*ygpu_neff_hook)(void* ctx, int32_t nx, int32_t ny, int32_t real_bytes, void* c_bed,
+                               const void* ux_b, const void* uy_b);
+
 /* Mirrors ssa_param_class (velocity_ssa.f90:22-52) and diva_param_class
  * (velocity_diva.f90:22-51); strings become the enums above.             */
 typedef struct ygpu_ssa_params {
@@ -79,7 +87,8 @@ typedef struct ygpu_ssa_params {
   char ssa_lis_opt[256];  /* LIS option string, e.g. the energy default
                              "-i cg -p jacobi -maxiter 200 -tol 1.0e-2 -initx_zeros false";
                              -i cg | bicgstab; minres and bicgsafe are accepted
-                             and solved with bicgstab to the same tolerance;
+                             and solved with bicgstab to the same tolerance,
+                             and ygpu_last_note says so;
                              extensions: -ir_inner_tol T, -ir_max N              */
 } ygpu_ssa_params;
 
@@ -105,12 +114,24 @@ typedef struct ygpu_ssa_problem {
   int32_t pad;
   double picard_resid;    /* last Picard residual                                 */
   double solve_seconds;   /* device time of the whole batch call                   */
+  /* DIVA only (NULL otherwise): neff_hook and its context.  With a hook,
+   * c_bed is INOUT (the hook's last update is left in it), Picard
+   * iterations run one launch sequence at a time, and an iteration counts
+   * as converged only if the hook changed c_bed by at most ssa_iter_conv in
+   * relative L1 (velocity_diva.f90:396-400).                                  */
+  ygpu_neff_hook neff_hook;
+  void* hook_ctx;
 } ygpu_ssa_problem;
 
 /* ---- handle ------------------------------------------------------------ */
 int ygpu_create(int32_t device, ygpu_handle** out);
 int ygpu_destroy(ygpu_handle* h);
 const char* ygpu_last_error(const ygpu_handle* h);
+/* What the last call that read an option string did differently from what
+ * it was asked, or "" (e.g. "-i minres solved with BiCGSTAB").  Calls that
+ * take options (one-shot solves, batch/session/server creation, the CSR
+ * entry points, the autotuner) reset it.                                     */
+const char* ygpu_last_note(const ygpu_handle* h);
 /* fills params with Yelmo's &ydyn defaults (input/yelmo_defaults.nml:140-173)
  * and input/yelmo_phys_const.nml:6-10 (no handle needed)                     */
 void ygpu_default_params(ygpu_ssa_params* par);
@@ -169,6 +190,26 @@ int ygpu_velocity_ssa(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, in
  * the device failed.                                                          */
 int ygpu_velocity_ssa_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
                             ygpu_ssa_problem* probs);
+
+/* ---- outer boundary: calc_velocity_diva (velocity_diva.f90:59-64),
+ * argument order as in the reference, wp = sp arrays.  c_bed is INOUT as in
+ * the reference (the hook updates it); neff_hook may be NULL (the optional
+ * argument absent).                                                         */
+int ygpu_velocity_diva(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int32_t ny,
+                       int32_t nz, float* ux, float* uy, float* ux_bar, float* uy_bar, float* ux_b,
+                       float* uy_b, float* ux_i, float* uy_i, float* taub_acx, float* taub_acy,
+                       float* beta, float* beta_acx, float* beta_acy, float* beta_eff,
+                       float* visc_eff, float* visc_eff_int, float* duxdz, float* duydz,
+                       int32_t* ssa_mask_acx, int32_t* ssa_mask_acy, float* ssa_err_acx,
+                       float* ssa_err_acy, int32_t* ssa_iter_now, int32_t* ssa_lin_iter,
+                       int32_t* ssa_lin_fail, int32_t* ssa_lim_n, float* c_bed,
+                       const float* f_slide, const float* taud_acx, const float* taud_acy,
+                       const float* taul_int_acx, const float* taul_int_acy, const float* H_ice,
+                       const float* f_ice, const float* H_grnd, const float* f_grnd,
+                       const float* f_grnd_acx, const float* f_grnd_acy, const float* ATT,
+                       const float* zeta_aa, const float* z_sl, const float* z_bed,
+                       const float* z_srf, float dx, float dy, float n_glen,
+                       ygpu_neff_hook neff_hook, void* hook_ctx);
 
 /* ---- calc_velocity_diva (velocity_diva.f90:59-429), batched form.         */
 int ygpu_velocity_diva_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,

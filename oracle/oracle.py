@@ -103,8 +103,15 @@ def calc_velocity_ssa(prob, par, dp: bool):
 
 
 
-def calc_velocity_diva(prob, par, dp: bool):
+# neff_hook of calc_velocity_diva: (ctx, nx, ny, real_bytes, c_bed, ux_b, uy_b)
+NEFF_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                        C.c_void_p, C.c_void_p)
+
+
+def calc_velocity_diva(prob, par, dp: bool, neff_hook=None):
     """The oracle's DIVA Picard loop (calc_velocity_diva) on a copy of prob.
+    neff_hook (a NEFF_HOOK) updates c_bed from u_b after every iteration;
+    the updated c_bed is returned in the state dict as "c_bed".
     Returns (state dict, info dict)."""
     L = lib(dp)
     dt = np.float64 if dp else np.float32
@@ -129,7 +136,9 @@ def calc_velocity_diva(prob, par, dp: bool):
     pc = params_c(par)
     rc = L.oracle_calc_velocity_diva(
         *[_p(a) for a in args], W(P.dx), W(P.dy), W(P.n_glen), C.byref(pc), int(par.no_slip),
-        par.ssa_lis_opt.encode(), P.nx, P.ny, P.nz, _p(rh), _p(lh), C.byref(secs))
+        par.ssa_lis_opt.encode(), P.nx, P.ny, P.nz, _p(rh), _p(lh), C.byref(secs),
+        neff_hook if neff_hook is not None else NEFF_HOOK(), None)
+    st["c_bed"] = fl["c_bed"]
     info = dict(rc=rc, ssa_iter_now=int(out_i[0]), ssa_lin_iter=int(out_i[1]),
                 ssa_lin_fail=int(out_i[2]), ssa_lim_n=int(out_i[3]),
                 resid_hist=rh[: max(out_i[0], 0)].copy(), lin_hist=lh[: max(out_i[0], 0)].copy(),

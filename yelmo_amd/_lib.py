@@ -56,11 +56,18 @@ class SsaProblemC(C.Structure):
     _fields_ += [(n, C.c_int32) for n in ("ssa_iter_now", "ssa_lin_iter", "ssa_lin_fail",
                                           "ssa_lim_n", "status", "pad")]
     _fields_ += [("picard_resid", C.c_double), ("solve_seconds", C.c_double)]
+    _fields_ += [("neff_hook", C.c_void_p), ("hook_ctx", C.c_void_p)]
+
+
+# neff_hook of calc_velocity_diva: (ctx, nx, ny, real_bytes, c_bed, ux_b, uy_b)
+NEFF_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                        C.c_void_p, C.c_void_p)
 
 
 # every symbol include/ygpu.h declares
-EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_default_params",
-           "ygpu_csr_solve", "ygpu_velocity_ssa", "ygpu_velocity_ssa_batch",
+EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_last_note",
+           "ygpu_default_params", "ygpu_csr_solve", "ygpu_velocity_ssa", "ygpu_velocity_diva",
+           "ygpu_velocity_ssa_batch",
            "ygpu_velocity_diva_batch", "ygpu_batch_create", "ygpu_batch_run",
            "ygpu_batch_download", "ygpu_batch_destroy", "ygpu_batch_spmv", "ygpu_csr_spmv", "ygpu_session_create", "ygpu_session_solve",
            "ygpu_session_destroy", "ygpu_stats_enable", "ygpu_stats_get",
@@ -95,6 +102,16 @@ def load() -> C.CDLL:
     lib.ygpu_destroy.restype = C.c_int
     lib.ygpu_last_error.argtypes = [PP]
     lib.ygpu_last_error.restype = C.c_char_p
+    lib.ygpu_last_note.argtypes = [PP]
+    lib.ygpu_last_note.restype = C.c_char_p
+    # the outer drop-ins take calc_velocity_ssa's / calc_velocity_diva's
+    # arguments one by one: pointers for the arrays, then dx, dy, n_glen
+    lib.ygpu_velocity_ssa.argtypes = [PP, C.POINTER(SsaParamsC), I, I, I] + [PP] * 34 + \
+        [C.c_float] * 3
+    lib.ygpu_velocity_ssa.restype = C.c_int
+    lib.ygpu_velocity_diva.argtypes = [PP, C.POINTER(SsaParamsC), I, I, I] + [PP] * 43 + \
+        [C.c_float] * 3 + [NEFF_HOOK, PP]
+    lib.ygpu_velocity_diva.restype = C.c_int
     lib.ygpu_default_params.argtypes = [C.POINTER(SsaParamsC)]
     lib.ygpu_default_params.restype = None
     lib.ygpu_csr_solve.argtypes = [PP, I, I, PP, PP, PP, PP, PP, C.c_char_p, C.POINTER(I),
@@ -172,6 +189,10 @@ class Handle:
     def check(self, rc):
         if rc != YGPU_OK:
             raise YgpuError(rc, self.lib.ygpu_last_error(self.h).decode())
+
+    def note(self) -> str:
+        """What the last call that read options did differently ("" if nothing)."""
+        return self.lib.ygpu_last_note(self.h).decode()
 
     def stats_enable(self, on: bool = True):
         self.check(self.lib.ygpu_stats_enable(self.h, int(on)))

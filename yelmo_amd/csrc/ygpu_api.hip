@@ -4,71 +4,86 @@
 
 namespace yg {
 
+// Option strings are read in place, one (key, value) pair of blank-separated
+// tokens at a time, with no heap allocation on the success path (the
+// no-allocation entry point parses with this too).
+namespace {
+struct Tok {
+  const char* p;
+  int n;
+  bool is(const char* w) const { return (int)strlen(w) == n && strncmp(p, w, n) == 0; }
+  int to_int() const { char b[32]; const int m = std::min(n, 31); memcpy(b, p, m); b[m] = 0; return atoi(b); }
+  double to_double() const { char b[64]; const int m = std::min(n, 63); memcpy(b, p, m); b[m] = 0; return atof(b); }
+};
+bool next_tok(const char*& s, Tok& t) {
+  while (*s == ' ' || *s == '\t' || *s == '\n') ++s;
+  if (!*s) return false;
+  t.p = s;
+  while (*s && *s != ' ' && *s != '\t' && *s != '\n') ++s;
+  t.n = (int)(s - t.p);
+  return true;
+}
+}  // namespace
+
 int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.solver = 4; o.k.precon = 0; o.k.maxiter = 1000; o.k.initx_zeros = 1; o.k.conv_cond = 0;
   o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1; o.k.ir_target = 0.0;
   o.k.ilu_cols = 1;
   o.k.factor16 = 0; o.k.graph_chunk = 4;
+  o.substituted = nullptr;
   if (!s) return 0;
-  std::vector<std::string> tok;
-  std::string cur;
-  for (const char* p = s; *p; ++p) {
-    if (*p == ' ' || *p == '\t' || *p == '\n') { if (!cur.empty()) { tok.push_back(cur); cur.clear(); } }
-    else cur += *p;
-  }
-  if (!cur.empty()) tok.push_back(cur);
-  if (tok.size() % 2) { err = "option string: missing value"; return YGPU_ERR_ARG; }
-  for (size_t q = 0; q < tok.size(); q += 2) {
-    const std::string& k = tok[q];
-    const std::string& v = tok[q + 1];
-    if (k == "-i") {
-      if (v == "cg" || v == "1") o.k.solver = 1;
-      else if (v == "bicgstab" || v == "4") o.k.solver = 4;
+  Tok k, v;
+  while (next_tok(s, k)) {
+    if (!next_tok(s, v)) { err = "option string: missing value"; return YGPU_ERR_ARG; }
+    if (k.is("-i")) {
+      if (v.is("cg") || v.is("1")) o.k.solver = 1;
+      else if (v.is("bicgstab") || v.is("4")) o.k.solver = 4;
       // The reference's namelists also name LIS's MINRES and BiCGSafe
       // (residual-formulation defaults, input/yelmo_defaults.nml:160,
       // par/*.nml).  The residual matrix is nonsymmetric with a diagonal of
       // both signs, where MINRES's SPD-preconditioner recurrences do not
       // apply; both solve the same system to the same tolerance with
-      // BiCGSTAB here.
-      else if (v == "minres" || v == "bicgsafe") o.k.solver = 4;
-      else { err = "unsupported Krylov solver: " + v; return YGPU_ERR_UNSUPPORTED; }
-    } else if (k == "-p") {
-      if (v == "none" || v == "0") o.k.precon = 0;
-      else if (v == "jacobi" || v == "1") o.k.precon = 1;
-      else if (v == "ilu" || v == "2") o.k.precon = 2;
-      else { err = "unsupported preconditioner: " + v; return YGPU_ERR_UNSUPPORTED; }
-    } else if (k == "-maxiter") o.k.maxiter = atoi(v.c_str());
-    else if (k == "-tol") o.k.tol = atof(v.c_str());
-    else if (k == "-initx_zeros") o.k.initx_zeros = (v == "true" || v == "1");
-    else if (k == "-conv_cond") o.k.conv_cond = atoi(v.c_str());
-    else if (k == "-ir_inner_tol") o.k.inner_tol = atof(v.c_str());
-    else if (k == "-ygpu_ir_tol") o.k.ir_target = atof(v.c_str());
-    else if (k == "-ir_max") o.k.ir_max = atoi(v.c_str());
-    else if (k == "-ygpu_ilu_blocks") o.k.ilu_blocks = atoi(v.c_str());
-    else if (k == "-ygpu_ilu_cols") o.k.ilu_cols = std::max(1, atoi(v.c_str()));
-    else if (k == "-ygpu_graph") o.k.graph_chunk = std::max(0, atoi(v.c_str()));
-    else if (k == "-ygpu_group") o.group = atoi(v.c_str());
-    else if (k == "-ygpu_group_threads") {
-      o.group_threads = atoi(v.c_str());
+      // BiCGSTAB here, and the call leaves a note (ygpu_last_note) saying so.
+      else if (v.is("minres")) { o.k.solver = 4; o.substituted = "-i minres solved with BiCGSTAB"; }
+      else if (v.is("bicgsafe")) { o.k.solver = 4; o.substituted = "-i bicgsafe solved with BiCGSTAB"; }
+      else { err = "unsupported Krylov solver: " + std::string(v.p, v.n); return YGPU_ERR_UNSUPPORTED; }
+    } else if (k.is("-p")) {
+      if (v.is("none") || v.is("0")) o.k.precon = 0;
+      else if (v.is("jacobi") || v.is("1")) o.k.precon = 1;
+      else if (v.is("ilu") || v.is("2")) o.k.precon = 2;
+      else { err = "unsupported preconditioner: " + std::string(v.p, v.n); return YGPU_ERR_UNSUPPORTED; }
+    } else if (k.is("-maxiter")) o.k.maxiter = v.to_int();
+    else if (k.is("-tol")) o.k.tol = v.to_double();
+    else if (k.is("-initx_zeros")) o.k.initx_zeros = (v.is("true") || v.is("1"));
+    else if (k.is("-conv_cond")) o.k.conv_cond = v.to_int();
+    else if (k.is("-ir_inner_tol")) o.k.inner_tol = v.to_double();
+    else if (k.is("-ygpu_ir_tol")) o.k.ir_target = v.to_double();
+    else if (k.is("-ir_max")) o.k.ir_max = v.to_int();
+    else if (k.is("-ygpu_ilu_blocks")) o.k.ilu_blocks = v.to_int();
+    else if (k.is("-ygpu_ilu_cols")) o.k.ilu_cols = std::max(1, v.to_int());
+    else if (k.is("-ygpu_graph")) o.k.graph_chunk = std::max(0, v.to_int());
+    else if (k.is("-ygpu_group")) o.group = v.to_int();
+    else if (k.is("-ygpu_group_threads")) {
+      o.group_threads = v.to_int();
       if (o.group_threads != 256 && o.group_threads != 512) {
         err = "-ygpu_group_threads takes 256 or 512";
         return YGPU_ERR_ARG;
       }
     }
-    else if (k == "-ygpu_factor") {
-      if (v == "bf16") o.k.factor16 = 1;
-      else if (v == "bf16u") o.k.factor16 = 2;
-      else if (v == "fp32") o.k.factor16 = 0;
-      else { err = "unknown factor storage: " + v; return YGPU_ERR_ARG; }
+    else if (k.is("-ygpu_factor")) {
+      if (v.is("bf16")) o.k.factor16 = 1;
+      else if (v.is("bf16u")) o.k.factor16 = 2;
+      else if (v.is("fp32")) o.k.factor16 = 0;
+      else { err = "unknown factor storage: " + std::string(v.p, v.n); return YGPU_ERR_ARG; }
     }
-    else if (k == "-ygpu_ny") o.ny_hint = atoi(v.c_str());
-    else if (k == "-ygpu_precision") {
-      if (v == "fp64") o.prec = 0;
-      else if (v == "mixed") o.prec = 2;
-      else { err = "unknown precision: " + v; return YGPU_ERR_ARG; }
-    } else if (k == "-omp_num_threads" || k == "-print" || k == "-scale" || k == "-f") {
+    else if (k.is("-ygpu_ny")) o.ny_hint = v.to_int();
+    else if (k.is("-ygpu_precision")) {
+      if (v.is("fp64")) o.prec = 0;
+      else if (v.is("mixed")) o.prec = 2;
+      else { err = "unknown precision: " + std::string(v.p, v.n); return YGPU_ERR_ARG; }
+    } else if (k.is("-omp_num_threads") || k.is("-print") || k.is("-scale") || k.is("-f")) {
       // accepted; no effect here
-    } else { err = "unknown option: " + k; return YGPU_ERR_ARG; }
+    } else { err = "unknown option: " + std::string(k.p, k.n); return YGPU_ERR_ARG; }
   }
   if (o.k.maxiter < 0) { err = "negative -maxiter"; return YGPU_ERR_ARG; }
   return 0;
@@ -128,6 +143,7 @@ int run_batch(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_probl
   if ((rc = B.upload_inputs(probs))) return rc;
   if ((rc = B.upload_state(probs))) return rc;
   double secs = 0.0;
+  B.hook_probs = probs;
   if ((rc = B.run(&secs))) return rc;
   if ((rc = B.download(probs))) return rc;
   if (diva) {
@@ -201,7 +217,12 @@ int csr_records(ygpu_handle* h, int n, const int32_t* a_ptr1, const int32_t* a_i
 
 int run_any(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_problem* probs, int diva) {
   if (!h || !par || np < 1 || !probs) return YGPU_ERR_ARG;
-  h->err.clear();
+  h->err.clear(); h->note.clear();
+  for (int k = 0; k < np; ++k)
+    if (probs[k].neff_hook && !diva) {
+      h->err = "neff_hook is an argument of calc_velocity_diva only";
+      return YGPU_ERR_ARG;
+    }
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   // fields are FP64 except in the reference mode (the reference's wp = sp)
   if (par->precision == YGPU_PREC_FP64 || par->precision == YGPU_PREC_MIXED)
@@ -242,6 +263,8 @@ int ygpu_destroy(ygpu_handle* h) {
 }
 
 const char* ygpu_last_error(const ygpu_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+const char* ygpu_last_note(const ygpu_handle* h) { return h ? h->note.c_str() : ""; }
 
 void ygpu_default_params(ygpu_ssa_params* p) {
   if (!p) return;
@@ -312,13 +335,52 @@ int ygpu_velocity_ssa(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, in
   return P.status;
 }
 
+int ygpu_velocity_diva(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int32_t ny,
+                       int32_t nz, float* ux, float* uy, float* ux_bar, float* uy_bar, float* ux_b,
+                       float* uy_b, float* ux_i, float* uy_i, float* taub_acx, float* taub_acy,
+                       float* beta, float* beta_acx, float* beta_acy, float* beta_eff,
+                       float* visc_eff, float* visc_eff_int, float* duxdz, float* duydz,
+                       int32_t* ssa_mask_acx, int32_t* ssa_mask_acy, float* ssa_err_acx,
+                       float* ssa_err_acy, int32_t* ssa_iter_now, int32_t* ssa_lin_iter,
+                       int32_t* ssa_lin_fail, int32_t* ssa_lim_n, float* c_bed,
+                       const float* f_slide, const float* taud_acx, const float* taud_acy,
+                       const float* taul_int_acx, const float* taul_int_acy, const float* H_ice,
+                       const float* f_ice, const float* H_grnd, const float* f_grnd,
+                       const float* f_grnd_acx, const float* f_grnd_acy, const float* ATT,
+                       const float* zeta_aa, const float* z_sl, const float* z_bed,
+                       const float* z_srf, float dx, float dy, float n_glen,
+                       ygpu_neff_hook neff_hook, void* hook_ctx) {
+  ygpu_ssa_problem P;
+  memset(&P, 0, sizeof(P));
+  P.nx = nx; P.ny = ny; P.nz = nz; P.real_bytes = 4;
+  P.dx = dx; P.dy = dy; P.n_glen = n_glen;
+  P.ux = ux; P.uy = uy; P.ux_bar = ux_bar; P.uy_bar = uy_bar; P.ux_b = ux_b; P.uy_b = uy_b;
+  P.ux_i = ux_i; P.uy_i = uy_i; P.taub_acx = taub_acx; P.taub_acy = taub_acy;
+  P.beta = beta; P.beta_acx = beta_acx; P.beta_acy = beta_acy; P.beta_eff = beta_eff;
+  P.visc_eff = visc_eff; P.visc_eff_int = visc_eff_int; P.duxdz = duxdz; P.duydz = duydz;
+  P.ssa_mask_acx = ssa_mask_acx; P.ssa_mask_acy = ssa_mask_acy;
+  P.ssa_err_acx = ssa_err_acx; P.ssa_err_acy = ssa_err_acy;
+  P.c_bed = c_bed; P.f_slide = f_slide; P.taud_acx = taud_acx; P.taud_acy = taud_acy;
+  P.taul_int_acx = taul_int_acx; P.taul_int_acy = taul_int_acy; P.H_ice = H_ice; P.f_ice = f_ice;
+  P.H_grnd = H_grnd; P.f_grnd = f_grnd; P.f_grnd_acx = f_grnd_acx; P.f_grnd_acy = f_grnd_acy;
+  P.ATT = ATT; P.zeta_aa = zeta_aa; P.z_sl = z_sl; P.z_bed = z_bed; P.z_srf = z_srf;
+  P.neff_hook = neff_hook; P.hook_ctx = hook_ctx;
+  int rc = run_any(h, par, 1, &P, 1);
+  if (rc) return rc;
+  if (ssa_iter_now) *ssa_iter_now = P.ssa_iter_now;
+  if (ssa_lin_iter) *ssa_lin_iter = P.ssa_lin_iter;
+  if (ssa_lin_fail) *ssa_lin_fail = P.ssa_lin_fail;
+  if (ssa_lim_n) *ssa_lim_n = P.ssa_lim_n;
+  return P.status;
+}
+
 // ---- inner boundary -------------------------------------------------------
 int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
                    const int32_t* a_idx1, const double* a_val, const double* b, double* x_inout,
                    const char* opts, int32_t* iters, int32_t* status, double* rel_resid,
                    double* seconds) {
   if (!h) return YGPU_ERR_ARG;
-  h->err.clear();
+  h->err.clear(); h->note.clear();
   if (n <= 0 || (n & 1) || !a_ptr1 || !a_idx1 || !a_val || !b || !x_inout || !iters || !status ||
       !rel_resid) {
     h->err = "invalid arguments (n must be even: two unknowns per cell)";
@@ -332,6 +394,7 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
   std::string e;
   int rc = parse_opts(opts, o, e);
   if (rc) { h->err = e; return rc; }
+  if (o.substituted) h->note = o.substituted;
   if (seconds) *seconds = 0.0;
   // b == 0 shortcut (solver_linear.F90:162-174)
   bool allzero = true;
@@ -497,7 +560,8 @@ int ygpu_session_create(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_s
                         ygpu_session** out, int64_t* device_bytes) {
   if (!h || !par || !prob || !out) return YGPU_ERR_ARG;
   *out = nullptr;
-  h->err.clear();
+  h->err.clear(); h->note.clear();
+  if (prob->neff_hook) { h->err = "neff_hook: one-shot DIVA entry points only"; return YGPU_ERR_ARG; }
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   YgSessionBase* sb = nullptr;
   int rc;
@@ -530,7 +594,9 @@ int ygpu_batch_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
                       int64_t* device_bytes) {
   if (!h || !par || nprob < 1 || !probs || !out) return YGPU_ERR_ARG;
   *out = nullptr;
-  h->err.clear();
+  h->err.clear(); h->note.clear();
+  for (int k = 0; k < nprob; ++k)
+    if (probs[k].neff_hook) { h->err = "neff_hook: one-shot DIVA entry points only"; return YGPU_ERR_ARG; }
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   YgBatchBase* bb = nullptr;
   int rc;
@@ -575,7 +641,7 @@ int ygpu_csr_spmv(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
                   const int32_t* a_idx1, const double* a_val, const double* x, double* y,
                   const char* opts) {
   if (!h) return YGPU_ERR_ARG;
-  h->err.clear();
+  h->err.clear(); h->note.clear();
   if (n <= 0 || (n & 1) || !a_ptr1 || !a_idx1 || !a_val || !x || !y) {
     h->err = "invalid arguments (n must be even: two unknowns per cell)";
     return YGPU_ERR_ARG;

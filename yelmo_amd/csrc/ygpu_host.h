@@ -26,6 +26,7 @@ struct ygpu_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  std::string note;   // what the last call did differently from what it was asked
   bool stats_on = false;
   ygpu_stats stats{};
   // Krylov schedules of the mixed path per grid and workgroups per member
@@ -55,6 +56,7 @@ struct Opts {
   int prec = -1;  // -1 unset, 0 fp64, 2 mixed
   int group = 0;  // workgroups per member: 0 automatic, 1 one (no groups), 2, 4 or 8
   int group_threads = 0;   // threads per group workgroup: 512 (one per CU), 256 (two), 0 automatic
+  const char* substituted = nullptr;   // solver named in -i that another one stands in for
 };
 
 int parse_opts(const char* s, Opts& o, std::string& err);
@@ -130,6 +132,7 @@ struct Batch {
     std::string e;
     int rc = parse_opts(par->ssa_lis_opt, opts, e);
     if (rc) { h->err = e; return rc; }
+    if (opts.substituted) h->note = opts.substituted;
     prec = par->precision;
     kmode = prec == YGPU_PREC_MIXED ? YG_MODE_MIXED : YG_MODE_FP64;
     if (par->ssa_solver != 0 && par->ssa_solver != 1) { h->err = "unknown ssa_solver"; return YGPU_ERR_ARG; }
@@ -420,6 +423,8 @@ struct Batch {
 
   std::vector<int> rb;
 
+#define YG_FIELD_OF(arr, member) [&] { std::vector<const void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = (arr)[k].member; return v; }()
+#define YG_FIELDW_OF(arr, member) [&] { std::vector<void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = (arr)[k].member; return v; }()
 #define YG_FIELD(member) [&] { std::vector<const void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = probs[k].member; return v; }()
 #define YG_FIELDW(member) [&] { std::vector<void*> v(nmem); for (int k = 0; k < nmem; ++k) v[k] = probs[k].member; return v; }()
 
@@ -659,9 +664,61 @@ struct Batch {
     return YGPU_OK;
   }
 
+  // ---- neff_hook (velocity_diva.f90:99-107, 396-400) ----------------------
+  // The caller's problems, when their hooks are to be called during run().
+  const ygpu_ssa_problem* hook_probs = nullptr;
+  bool has_hooks() const {
+    if (!hook_probs || !diva) return false;
+    for (int k = 0; k < nmem; ++k) if (hook_probs[k].neff_hook) return true;
+    return false;
+  }
+
+  template <typename T>
+  static bool c_bed_changed(T* c, const T* c_nm1, long long n, double conv) {
+    // sum(abs(c_bed - c_bed_nm1)) > ssa_iter_conv * max(sum(abs(c_bed)), TOL_UNDERFLOW),
+    // summed in the caller's real type in array order
+    T d = T(0), a = T(0);
+    for (long long q = 0; q < n; ++q) {
+      d += std::fabs(c[q] - c_nm1[q]);
+      a += std::fabs(c[q]);
+    }
+    return d > (T)conv * std::max(a, (T)1e-15);
+  }
+
+  // After an iteration of the members in `ran`: the basal velocity goes to
+  // the callers' ux_b/uy_b, each hooked member's hook updates its c_bed, the
+  // new c_bed goes to the device, and a member whose c_bed moved by more than
+  // the Picard tolerance is not done unless it reached ssa_iter_max.
+  int apply_hooks(const std::vector<int>& ran, std::vector<int>& done) {
+    int rc;
+    if ((rc = down2(f.ux, YG_FIELDW_OF(hook_probs, ux_b).data(), 0))) return rc;
+    if ((rc = down2(f.uy, YG_FIELDW_OF(hook_probs, uy_b).data(), 0))) return rc;
+    std::vector<int> it(nmem);
+    YG_CHECK(h, hipMemcpy(it.data(), d_iter_now, sizeof(int) * nmem, hipMemcpyDeviceToHost));
+    bool flags = false;
+    for (int k : ran) {
+      const ygpu_ssa_problem& P = hook_probs[k];
+      if (!P.neff_hook) continue;
+      const long long n = (long long)P.nx * P.ny;
+      void* cb = const_cast<void*>(P.c_bed);
+      std::vector<unsigned char> nm1((size_t)n * P.real_bytes);
+      memcpy(nm1.data(), cb, nm1.size());
+      P.neff_hook(P.hook_ctx, P.nx, P.ny, P.real_bytes, cb, P.ux_b, P.uy_b);
+      const bool moved = P.real_bytes == 4
+                             ? c_bed_changed((float*)cb, (const float*)nm1.data(), n, fp.iter_conv)
+                             : c_bed_changed((double*)cb, (const double*)nm1.data(), n, fp.iter_conv);
+      if (moved && done[k] == 1 && it[k] < iter_max) { done[k] = 0; flags = true; }
+    }
+    if ((rc = up2((R*)f.c_bed, YG_FIELD_OF(hook_probs, c_bed).data(), 0, false))) return rc;
+    if (flags)
+      YG_CHECK(h, hipMemcpy(d_done, done.data(), sizeof(int) * nmem, hipMemcpyHostToDevice));
+    return YGPU_OK;
+  }
+
   // the Picard loop (velocity_ssa.f90:172-334 / velocity_diva.f90:195-404)
   int run(double* seconds) {
     hipStream_t st = h->stream;
+    int rc;
     YG_CHECK(h, hipMemsetAsync(d_done, 0, sizeof(int) * nmem, st));
     YG_CHECK(h, hipMemsetAsync(d_lin_iter, 0, sizeof(int) * nmem, st));
     YG_CHECK(h, hipMemsetAsync(d_lin_fail, 0, sizeof(int) * nmem, st));
@@ -701,7 +758,8 @@ struct Batch {
       YG_CHECK(h, hipEventCreate(&k0));
       YG_CHECK(h, hipEventCreate(&k1));
     }
-    const int chunk = (phase || h->stats_on) ? 0 : opts.k.graph_chunk;
+    const bool hooked = has_hooks();
+    const int chunk = (phase || h->stats_on || hooked) ? 0 : opts.k.graph_chunk;
     const bool graphed = chunk > 0 && ensure_graph(G, ps, ka, chunk);
     if (graphed) {
       // replays of the captured chunk, one flag read per replay
@@ -716,7 +774,7 @@ struct Batch {
     }
     for (int iter = 1; !graphed && iter <= iter_max; ++iter) {
       std::vector<int> active;
-      if (h->stats_on)
+      if (h->stats_on || hooked)
         for (int k = 0; k < nmem; ++k) if (!done[k]) active.push_back(k);
       enqueue_iteration(G, ps, ka, k0, k1, st);
       YG_CHECK(h, hipGetLastError());
@@ -745,6 +803,7 @@ struct Batch {
           h->stats.picard_iters += 1;
         }
       }
+      if (hooked && (rc = apply_hooks(active, done))) return rc;
       bool all = true;
       for (int k = 0; k < nmem; ++k) all &= done[k] != 0;
       if (all) break;

@@ -155,8 +155,14 @@ YG_DEV double grp_loadd(const double* p) {
 }
 
 // Barrier k of the group (k = yg_gbar + 1 on every block).
+// The wait for the wave's stores is bracketed by compiler fences: the
+// s_waitcnt intrinsic touches no memory as far as the compiler knows, so
+// without them a store issued before the barrier could be scheduled after the
+// wait and reach the L2 after this block's arrival is counted.
 YG_DEV void grp_barrier(const Grp& g) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have reached the L2
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __syncthreads();
   if (threadIdx.x == 0) {
     const int k = yg_gbar + 1;

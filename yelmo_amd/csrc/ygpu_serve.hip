@@ -874,7 +874,7 @@ int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva,
                        int64_t hbm_budget_bytes, ygpu_server** out, int64_t* device_bytes) {
   if (!h || !par || !out) return YGPU_ERR_ARG;
   *out = nullptr;
-  h->err.clear();
+  h->err.clear(); h->note.clear();
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   YgServerBase* sb = nullptr;
   int rc;
@@ -897,6 +897,7 @@ int ygpu_submit(ygpu_server* s, const ygpu_ssa_problem* prob, int64_t seq, int32
                 int64_t* ticket) {
   if (!s) return YGPU_ERR_ARG;
   s->h->err.clear();
+  if (prob && prob->neff_hook) { s->h->err = "neff_hook: one-shot DIVA entry points only"; return YGPU_ERR_ARG; }
   return s->s->submit(prob, seq, flags, ticket);
 }
 

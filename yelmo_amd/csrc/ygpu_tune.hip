@@ -164,7 +164,7 @@ int ygpu_autotune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_pro
                   int32_t diva, int32_t reps, int32_t* ilu_blocks, int32_t* block,
                   double* solve_ms) {
   if (!h || !par || !prob) return YGPU_ERR_ARG;
-  h->err.clear();
+  h->err.clear(); h->note.clear();
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   return tune(h, par, prob, diva ? 1 : 0, reps < 1 ? 1 : reps, ilu_blocks, block, solve_ms);
 }

@@ -88,7 +88,10 @@ def _geometry(nx, ny, dx, seed, cb_seed=None, sigma_ln=1.0, H_noise_seed=None):
     z_bed = -200.0 + 500.0 * s
     rng_cb = rng_geo if cb_seed is None else np.random.default_rng(cb_seed)
     gn = _smooth_noise(rng_cb, nx, ny, 64e3, dx)
-    gn /= gn.std() if gn.std() > 1e-6 else 1.0
+    # a field smoothed over far more than the grid is nearly constant: its
+    # spread is then no noise to normalise, and the friction stays uniform
+    sd = gn.std()
+    gn = gn / sd if sd > 1e-3 * np.abs(gn).max() else np.zeros_like(gn)
     c_raw = 1e4 * np.exp(sigma_ln * gn)
     return H, z_bed, c_raw
 

@@ -81,12 +81,19 @@ def _info(s: _lib.SsaProblemC) -> SolveInfo:
                      s.picard_resid, s.solve_seconds)
 
 
-def _run_batch(problems, par: SsaParams, diva: bool, handle=None):
+def _run_batch(problems, par: SsaParams, diva: bool, handle=None, hooks=None):
     h = handle or _lib.default_handle()
     arr = (_lib.SsaProblemC * len(problems))()
     keep = []
     for k, p in enumerate(problems):
+        hook = hooks[k] if hooks is not None else None
+        if hook is not None:
+            # c_bed is INOUT under a hook: the caller's array receives its updates
+            p.fields["c_bed"] = np.asfortranarray(p.fields["c_bed"], dtype=_real_dtype(p))
         s, kp = _fill_struct(p, diva)
+        if hook is not None:
+            s.neff_hook = C.cast(hook, C.c_void_p)
+            kp.append(hook)
         arr[k] = s
         keep.append(kp)
     pc = par.to_c()
@@ -106,9 +113,78 @@ def calc_velocity_ssa_batch(problems, par: SsaParams, handle=None) -> list[Solve
     return _run_batch(problems, par, False, handle)
 
 
-def calc_velocity_diva(prob: SsaProblem, par: SsaParams, handle=None) -> SolveInfo:
-    """One DIVA solve (calc_velocity_diva); updates prob.state in place."""
-    return _run_batch([prob], par, True, handle)[0]
+def calc_velocity_diva(prob: SsaProblem, par: SsaParams, handle=None,
+                       neff_hook=None) -> SolveInfo:
+    """One DIVA solve (calc_velocity_diva); updates prob.state in place.
+    neff_hook (an _lib.NEFF_HOOK) is the reference's optional hook: it
+    updates c_bed (prob.fields["c_bed"], in place) from u_b after every
+    Picard iteration."""
+    return _run_batch([prob], par, True, handle, hooks=[neff_hook])[0]
+
+
+# ---- the outer drop-ins, argument by argument ---------------------------------
+
+_SSA_ARGS = ("ux_b", "uy_b", "taub_acx", "taub_acy", "visc_eff", "visc_eff_int", "ssa_mask_acx",
+             "ssa_mask_acy", "ssa_err_acx", "ssa_err_acy", "#ssa_iter_now", "#ssa_lin_iter",
+             "#ssa_lin_fail", "#ssa_lim_n", "beta", "beta_acx", "beta_acy", "c_bed", "f_slide",
+             "taud_acx", "taud_acy", "taul_int_acx", "taul_int_acy", "H_ice", "f_ice", "H_grnd",
+             "f_grnd", "f_grnd_acx", "f_grnd_acy", "ATT", "zeta_aa", "z_sl", "z_bed", "z_srf")
+_DIVA_ARGS = ("ux", "uy", "ux_bar", "uy_bar", "ux_b", "uy_b", "ux_i", "uy_i", "taub_acx",
+              "taub_acy", "beta", "beta_acx", "beta_acy", "beta_eff", "visc_eff", "visc_eff_int",
+              "duxdz", "duydz", "ssa_mask_acx", "ssa_mask_acy", "ssa_err_acx", "ssa_err_acy",
+              "#ssa_iter_now", "#ssa_lin_iter", "#ssa_lin_fail", "#ssa_lim_n", "c_bed", "f_slide",
+              "taud_acx", "taud_acy", "taul_int_acx", "taul_int_acy", "H_ice", "f_ice", "H_grnd",
+              "f_grnd", "f_grnd_acx", "f_grnd_acy", "ATT", "zeta_aa", "z_sl", "z_bed", "z_srf")
+
+
+def _drop_in_args(prob: SsaProblem, names):
+    """float32 / int32 Fortran arrays of prob in the order of `names`
+    (the prob's own arrays are replaced by them, so results land there);
+    '#name' is an integer scalar output."""
+    st, fl = prob.state, prob.fields
+    outs, ptrs = {}, []
+    for n in names:
+        if n.startswith("#"):
+            outs[n[1:]] = C.c_int32()
+            ptrs.append(C.cast(C.byref(outs[n[1:]]), C.c_void_p))
+            continue
+        d = st if n in st else fl
+        dt = np.int32 if n.startswith("ssa_mask") else np.float32
+        d[n] = np.asfortranarray(d[n], dtype=dt)
+        ptrs.append(C.c_void_p(d[n].ctypes.data))
+    return outs, ptrs
+
+
+def ygpu_velocity_ssa(prob: SsaProblem, par: SsaParams, handle=None) -> SolveInfo:
+    """calc_velocity_ssa through the argument-order drop-in ygpu_velocity_ssa
+    (wp = sp arrays: prob's arrays become float32)."""
+    h = handle or _lib.default_handle()
+    outs, ptrs = _drop_in_args(prob, _SSA_ARGS)
+    pc = par.to_c()
+    rc = h.lib.ygpu_velocity_ssa(h.h, C.byref(pc), prob.nx, prob.ny, prob.nz, *ptrs,
+                                 prob.dx, prob.dy, prob.n_glen)
+    if rc not in (_lib.YGPU_OK, _lib.YGPU_ERR_NONFINITE):
+        h.check(rc)
+    return SolveInfo(outs["ssa_iter_now"].value, outs["ssa_lin_iter"].value,
+                     outs["ssa_lin_fail"].value, outs["ssa_lim_n"].value, rc, float("nan"), 0.0)
+
+
+def ygpu_velocity_diva(prob: SsaProblem, par: SsaParams, handle=None,
+                       neff_hook=None) -> SolveInfo:
+    """calc_velocity_diva through the argument-order drop-in
+    ygpu_velocity_diva (wp = sp arrays); neff_hook as in calc_velocity_diva."""
+    from .generator import add_diva_state
+    h = handle or _lib.default_handle()
+    add_diva_state(prob)
+    outs, ptrs = _drop_in_args(prob, _DIVA_ARGS)
+    pc = par.to_c()
+    hook = neff_hook if neff_hook is not None else _lib.NEFF_HOOK()
+    rc = h.lib.ygpu_velocity_diva(h.h, C.byref(pc), prob.nx, prob.ny, prob.nz, *ptrs,
+                                  prob.dx, prob.dy, prob.n_glen, hook, None)
+    if rc not in (_lib.YGPU_OK, _lib.YGPU_ERR_NONFINITE):
+        h.check(rc)
+    return SolveInfo(outs["ssa_iter_now"].value, outs["ssa_lin_iter"].value,
+                     outs["ssa_lin_fail"].value, outs["ssa_lim_n"].value, rc, float("nan"), 0.0)
 
 
 def calc_velocity_diva_batch(problems, par: SsaParams, handle=None) -> list[SolveInfo]:
