@@ -376,3 +376,169 @@ def calc_beta(prob, par, ux_b, uy_b, dp=True):
                                nx, ny)
     assert rc == 0, rc
     return beta, bx, by
+
+
+# ---- planted mistakes and the small Picard helpers, for the pinning tests ----
+
+PLANTS = {
+    "gq_node_pos": 1, "gq_acx_corner": 2, "gq_act_ignored": 3, "glen_shear": 4, "lim_w": 5,
+    "lim_r": 6, "lim_cross": 7, "conv_vel_tol": 8, "conv_mask": 9, "conv_du_reg": 10,
+    "conv_norm2": 11, "l1rel_tol": 12, "relax_vel": 13, "relax_visc": 14, "clip": 15,
+    "gl_upstream": 16, "beta_min_mask": 17, "diva_shear": 18, "beta_q": 19, "gq_att_act": 20,
+}
+
+
+class planted:
+    """Context manager: switches on one planted mistake in both oracle builds
+    (oracle_plant in ssa_oracle.c), so a test can show its check fails."""
+
+    def __init__(self, name):
+        self.k = PLANTS[name]
+
+    def __enter__(self):
+        for dp in (False, True):
+            C.c_int.in_dll(lib(dp), "oracle_plant").value = self.k
+        return self
+
+    def __exit__(self, *exc):
+        for dp in (False, True):
+            C.c_int.in_dll(lib(dp), "oracle_plant").value = 0
+        return False
+
+
+def _W(dp):
+    return C.c_double if dp else C.c_float
+
+
+def _F(a, dt):
+    return np.asfortranarray(np.asarray(a, dtype=dt))
+
+
+def gq2D_nodes(v, kind, i, j, act=None, boundaries="zeros", dp=True):
+    """The 4 quadrature-node values (SW, SE, NE, NW) of cell (i, j) of field v
+    on the aa ("aa"), acx or acy grid, with active flags act (None: all)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = v.shape
+    vv = _F(v, dt)
+    aa = None if act is None else np.asfortranarray(act, dtype=np.uint8)
+    n = np.zeros(4, dtype=dt)
+    L.oracle_gq2D_nodes(_p(n), _p(vv), None if aa is None else _p(aa),
+                        {"aa": 0, "acx": 1, "acy": 2}[kind], nx, ny, i, j, BND[boundaries])
+    return n
+
+
+def visc_eff_3D_nodes(ux, uy, ATT, f_ice, dx, dy, n_glen, eps_0, boundaries="zeros", dp=True):
+    """calc_visc_eff_3D_nodes (visc_method 1) on given fields."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = _W(dp)
+    nx, ny, nz = ATT.shape
+    visc = np.zeros((nx, ny, nz), dtype=dt, order="F")
+    keep = [_F(ux, dt), _F(uy, dt), _F(ATT, dt), _F(np.zeros((nx, ny)), dt), _F(f_ice, dt),
+            np.zeros(nz, dtype=dt)]
+    L.oracle_visc_eff_3D_nodes(_p(visc), *[_p(a) for a in keep], W(dx), W(dy), W(n_glen),
+                               W(eps_0), nx, ny, nz, BND[boundaries])
+    return visc
+
+
+def visc_eff_3D_nodes_diva(ux, uy, dudz, dvdz, ATT, f_ice, dx, dy, n_glen, eps_0,
+                           boundaries="zeros", dp=True):
+    """The DIVA form of calc_visc_eff_3D_nodes (velocity_diva.f90:573-710)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = _W(dp)
+    nx, ny, nz = ATT.shape
+    visc = np.zeros((nx, ny, nz), dtype=dt, order="F")
+    keep = [_F(ux, dt), _F(uy, dt), _F(dudz, dt), _F(dvdz, dt), _F(ATT, dt), _F(f_ice, dt)]
+    L.oracle_visc_eff_3D_nodes_diva(_p(visc), *[_p(a) for a in keep], W(dx), W(dy), W(n_glen),
+                                    W(eps_0), nx, ny, nz, BND[boundaries])
+    return visc
+
+
+def vel_lim_drag(ux, uy, mask_acx, mask_acy, u_max, tau_c, boundaries="zeros", dp=True):
+    """calc_vel_lim_drag: (k_acx, k_acy, r_acx, r_acy)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = _W(dp)
+    nx, ny = ux.shape
+    outs = [np.zeros((nx, ny), dtype=dt, order="F") for _ in range(4)]
+    keep = [_F(ux, dt), _F(uy, dt), np.asfortranarray(mask_acx, dtype=np.int32),
+            np.asfortranarray(mask_acy, dtype=np.int32)]
+    L.oracle_vel_lim_drag(*[_p(o) for o in outs], *[_p(a) for a in keep], W(u_max), W(tau_c),
+                          nx, ny, BND[boundaries])
+    return outs
+
+
+def picard_conv_l2(ux, uy, ux_prev, uy_prev, mask_acx, mask_acy, tol, iter, iter_max, dp=True):
+    """picard_calc_convergence_l2: (conv, resid, points checked); conv 1
+    converged, 2 iteration limit, 0 not converged, -1 non-finite."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = ux.shape
+    conv = C.c_int()
+    resid = _W(dp)()
+    keep = [_F(ux, dt), _F(uy, dt), _F(ux_prev, dt), _F(uy_prev, dt),
+            np.asfortranarray(mask_acx, dtype=np.int32), np.asfortranarray(mask_acy, dtype=np.int32)]
+    n = L.oracle_picard_conv_l2(C.byref(conv), C.byref(resid), *[_p(a) for a in keep],
+                                _W(dp)(tol), iter, iter_max, nx, ny)
+    return conv.value, float(resid.value), n
+
+
+def picard_l1rel(ux, uy, ux_prev, uy_prev, dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    keep = [_F(ux, dt), _F(uy, dt), _F(ux_prev, dt), _F(uy_prev, dt)]
+    ex, ey = np.zeros_like(keep[0]), np.zeros_like(keep[0])
+    L.oracle_picard_l1rel(_p(ex), _p(ey), *[_p(a) for a in keep], keep[0].size)
+    return ex, ey
+
+
+def picard_relax_vel(ux, uy, ux_prev, uy_prev, rel, dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    a, b = _F(ux, dt).copy(), _F(uy, dt).copy()
+    keep = [_F(ux_prev, dt), _F(uy_prev, dt)]
+    L.oracle_picard_relax_vel(_p(a), _p(b), _p(keep[0]), _p(keep[1]), _W(dp)(rel), a.size)
+    return a, b
+
+
+def picard_relax_visc(visc, visc_prev, rel, dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    v = _F(visc, dt).copy()
+    vp = _F(visc_prev, dt)
+    L.oracle_picard_relax_visc(_p(v), _p(vp), _W(dp)(rel), v.size)
+    return v
+
+
+def ssa_vel_clip(ux, uy, u_max, dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    a, b = _F(ux, dt).copy(), _F(uy, dt).copy()
+    L.oracle_ssa_vel_clip(_p(a), _p(b), _W(dp)(u_max), a.size)
+    return a, b
+
+
+def set_beta_min_grounded(bx, by, mask_acx, mask_acy, beta_min, dp=True):
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    a, b = _F(bx, dt).copy(), _F(by, dt).copy()
+    keep = [np.asfortranarray(mask_acx, dtype=np.int32), np.asfortranarray(mask_acy, dtype=np.int32)]
+    L.oracle_set_beta_min_grounded(_p(a), _p(b), _p(keep[0]), _p(keep[1]), _W(dp)(beta_min),
+                                   a.size)
+    return a, b
+
+
+def stagger_beta(beta, f_ice, f_grnd, par, dp=True):
+    """stagger_beta on given aa-node friction: (beta_acx, beta_acy)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = beta.shape
+    bx = np.zeros((nx, ny), dtype=dt, order="F")
+    by = np.zeros((nx, ny), dtype=dt, order="F")
+    keep = [_F(beta, dt), _F(f_ice, dt), _F(f_grnd, dt)]
+    pc = params_c(par)
+    rc = L.oracle_stagger_beta(_p(bx), _p(by), *[_p(a) for a in keep], C.byref(pc), nx, ny)
+    assert rc == 0, rc
+    return bx, by

@@ -51,6 +51,34 @@ typedef float wp;
 #define R(x) ((wp)(x))            /* x_wp literal                            */
 #define RS(x) ((wp)(float)(x))    /* default-real (single) literal, to wp    */
 
+/* Planted mistakes, for the pinning tests only: oracle_plant = k switches on
+ * mistake k (a plausible slip in the restatement), so a test can show that
+ * its check fails on it.  0 (the default) is the restatement itself.       */
+int oracle_plant = 0;
+#define PLANT(k) (oracle_plant == (k))
+enum {
+    PL_GQ_NODE_POS = 1,      /* Gauss nodes at +-1/2 instead of +-1/sqrt(3)        */
+    PL_GQ_ACX_CORNER = 2,    /* acx corner NE takes face (i, j-1) for (i, j+1)     */
+    PL_GQ_ACT_IGNORED = 3,   /* corner means ignore the active-face flags          */
+    PL_GLEN_SHEAR = 4,       /* 1/4 (du/dy + dv/dx)^2 without the 1/4              */
+    PL_LIM_W = 5,            /* w = u/s instead of (u/s)^2                        */
+    PL_LIM_R = 6,            /* r = k u instead of (k - b) u                      */
+    PL_LIM_CROSS = 7,        /* cross velocity at the acx face from uy(i, j+1)    */
+    PL_CONV_VEL_TOL = 8,     /* faces with |u| <= vel_tol counted in the L2 norm  */
+    PL_CONV_MASK = 9,        /* faces with mask 0 counted in the L2 norm          */
+    PL_CONV_DU_REG = 10,     /* no du_reg in the denominator                      */
+    PL_CONV_NORM2 = 11,      /* norm_method 2 (2 |du| / |u + u_prev|)             */
+    PL_L1REL_TOL = 12,       /* l1rel without the + tol in the denominator        */
+    PL_RELAX_VEL = 13,       /* u_prev + (1 - rel)(u - u_prev)                    */
+    PL_RELAX_VISC = 14,      /* linear instead of log-space viscosity relaxation  */
+    PL_CLIP = 15,            /* clip only positive velocities                     */
+    PL_GL_UPSTREAM = 16,     /* grounding-line face takes the floating side       */
+    PL_BETA_MIN_MASK = 17,   /* beta_min on every free face (mask >= 1)           */
+    PL_DIVA_SHEAR = 18,      /* DIVA viscosity without the 1/4 (dv/dz)^2 term     */
+    PL_BETA_Q = 19,          /* friction law with |u|^q instead of |u|^(q-1)      */
+    PL_GQ_ATT_ACT = 20       /* rate factor corners over grounded cells only      */
+};
+
 /* ------------------------------------------------------------------ */
 /* boundaries strings (yelmo_tools.f90:248-271, solver_ssa_ac.f90:148-196) */
 enum {
@@ -276,7 +304,7 @@ static wp mean_act(const wp *v, const uint8_t *a, int n)
     int cnt = 0;
     wp s = R(0);
     for (int q = 0; q < n; q++)
-        if (!a || a[q]) { s += v[q]; cnt++; }
+        if (!a || a[q] || PLANT(PL_GQ_ACT_IGNORED)) { s += v[q]; cnt++; }
     return cnt > 0 ? s / (wp)cnt : R(0);
 }
 
@@ -284,8 +312,9 @@ static wp mean_act(const wp *v, const uint8_t *a, int n)
 static void gq_corners_to_nodes(const wp c[4], wp n[4])
 {
     static const double xs[4] = {-1, 1, 1, -1}, ys[4] = {-1, -1, 1, 1};
+    const double a = PLANT(PL_GQ_NODE_POS) ? 0.5 : GQ_A;
     for (int q = 0; q < 4; q++) {
-        double xi = xs[q] * GQ_A, eta = ys[q] * GQ_A;
+        double xi = xs[q] * a, eta = ys[q] * a;
         double w0 = 0.25 * (1 - xi) * (1 - eta), w1 = 0.25 * (1 + xi) * (1 - eta);
         double w2 = 0.25 * (1 + xi) * (1 + eta), w3 = 0.25 * (1 - xi) * (1 + eta);
         n[q] = (wp)(w0 * c[0] + w1 * c[1] + w2 * c[2] + w3 * c[3]);
@@ -316,7 +345,7 @@ static void gq_nodes_acx(wp n[4], const wp *v, const uint8_t *act, int nx, int i
     (void)ip1;
     int sw[2] = {IX(im1, jm1), IX(im1, j)};
     int se[2] = {IX(i, jm1), IX(i, j)};
-    int ne[2] = {IX(i, j), IX(i, jp1)};
+    int ne[2] = {IX(i, j), PLANT(PL_GQ_ACX_CORNER) ? IX(i, jm1) : IX(i, jp1)};
     int nw[2] = {IX(im1, j), IX(im1, jp1)};
     int *cs[4] = {sw, se, ne, nw};
     wp c[4];
@@ -346,6 +375,19 @@ static void gq_nodes_acy(wp n[4], const wp *v, const uint8_t *act, int nx, int i
         c[q] = mean_act(vv, aa, 2);
     }
     gq_corners_to_nodes(c, n);
+}
+
+/* The four node values of cell (i, j) (0-based) of field v on the aa
+ * (kind 0), acx (1) or acy (2) grid, with the active flags act (may be
+ * NULL: all active), as gq_nodes_* give them to the kernels above.       */
+void oracle_gq2D_nodes(wp n[4], const wp *v, const uint8_t *act, int kind, int nx, int ny,
+                       int i, int j, int bnd)
+{
+    int im1, ip1, jm1, jp1;
+    nbr(i, j, nx, ny, boundary_code(bnd), &im1, &ip1, &jm1, &jp1);
+    if (kind == 0) gq_nodes_aa(n, v, act, nx, i, j, im1, ip1, jm1, jp1);
+    else if (kind == 1) gq_nodes_acx(n, v, act, nx, i, j, im1, ip1, jm1, jp1);
+    else gq_nodes_acy(n, v, act, nx, i, j, im1, ip1, jm1, jp1);
 }
 
 /* ------------------------------------------------------------------ */
@@ -380,11 +422,13 @@ void oracle_visc_eff_3D_nodes(wp *visc, const wp *ux, const wp *uy, const wp *AT
                 for (int q = 0; q < 4; q++) {
                     wp s = dudyn[q] + dvdxn[q];
                     eps_sq_n[q] = dudxn[q] * dudxn[q] + dvdyn[q] * dvdyn[q] +
-                                  dudxn[q] * dvdyn[q] + RS(0.25) * (s * s) + eps_0_sq;
+                                  dudxn[q] * dvdyn[q] +
+                                  (PLANT(PL_GLEN_SHEAR) ? R(1) : RS(0.25)) * (s * s) + eps_0_sq;
                 }
                 for (int k = 0; k < nz; k++) {
                     wp ATTn[4];
-                    gq_nodes_aa(ATTn, ATT + (size_t)k * nc, NULL, nx, i, j, im1, ip1, jm1, jp1);
+                    gq_nodes_aa(ATTn, ATT + (size_t)k * nc, PLANT(PL_GQ_ATT_ACT) ? act_acx : NULL,
+                                nx, i, j, im1, ip1, jm1, jp1);
                     wp sum = R(0);
                     for (int q = 0; q < 4; q++)
                         sum += (RS(0.5) * WPOW(eps_sq_n[q], p1) * WPOW(ATTn[q], p2)) * R(1);
@@ -435,8 +479,40 @@ void oracle_visc_eff_3D_aa(wp *visc, const wp *ux, const wp *uy, const wp *ATT,
 /* picard_relax_visc (velocity_general.f90:2188-2213) */
 static void picard_relax_visc(wp *visc, const wp *visc_prev, wp rel, size_t n)
 {
+    if (PLANT(PL_RELAX_VISC)) {
+        for (size_t q = 0; q < n; q++) visc[q] = visc_prev[q] + rel * (visc[q] - visc_prev[q]);
+        return;
+    }
     for (size_t q = 0; q < n; q++)
         visc[q] = WEXP((RS(1.0) - rel) * WLOG(visc_prev[q]) + rel * WLOG(visc[q]));
+}
+
+void oracle_picard_relax_visc(wp *visc, const wp *visc_prev, wp rel, int n)
+{
+    picard_relax_visc(visc, visc_prev, rel, (size_t)n);
+}
+
+/* picard_relax_vel (velocity_general.f90:2160-2186) */
+void oracle_picard_relax_vel(wp *ux, wp *uy, const wp *ux_prev, const wp *uy_prev, wp rel, int n)
+{
+    const wp r = PLANT(PL_RELAX_VEL) ? R(1) - rel : rel;
+    for (int c = 0; c < n; c++) {
+        ux[c] = ux_prev[c] + r * (ux[c] - ux_prev[c]);
+        uy[c] = uy_prev[c] + r * (uy[c] - uy_prev[c]);
+    }
+}
+
+/* ssa_vel_clip / limit_vel (solver_ssa_ac.f90:1069-1093,1264-1280): each
+ * component clipped to [-u_max, u_max]; NaN compares false both ways and
+ * passes through                                                           */
+void oracle_ssa_vel_clip(wp *ux, wp *uy, wp u_max, int n)
+{
+    for (int c = 0; c < n; c++) {
+        if (ux[c] > u_max) ux[c] = u_max;
+        if (ux[c] < -u_max && !PLANT(PL_CLIP)) ux[c] = -u_max;
+        if (uy[c] > u_max) uy[c] = u_max;
+        if (uy[c] < -u_max && !PLANT(PL_CLIP)) uy[c] = -u_max;
+    }
 }
 
 /* integrate_trapezoid1D_pt (yelmo_tools.f90:1092-1122) */
@@ -499,7 +575,7 @@ static void beta_aa_law(wp *beta, const wp *ux_b, const wp *uy_b, const wp *c_be
                 for (int p = 0; p < 4; p++) {
                     wp uxyn = WSQRT(uxn[p] * uxn[p] + uyn[p] * uyn[p] + ub_sq_min);
                     wp ratio = coulomb ? uxyn / (uxyn + u_0) : uxyn / u_0;
-                    wp betan = cbn[p] * WPOW(ratio, q) * (R(1) / uxyn);
+                    wp betan = cbn[p] * WPOW(ratio, q) * (PLANT(PL_BETA_Q) ? R(1) : R(1) / uxyn);
                     sum += betan * R(1);
                 }
                 beta[c] = sum / R(4);
@@ -606,17 +682,18 @@ int oracle_stagger_beta(wp *beta_acx, wp *beta_acy, const wp *beta, const wp *f_
                 for (int i = 0; i < nx; i++) {
                     int im1, ip1, jm1, jp1, c = IX(i, j);
                     nbr(i, j, nx, ny, BC, &im1, &ip1, &jm1, &jp1);
+                    const int up = (stag == 1) != PLANT(PL_GL_UPSTREAM);
                     if (f_ice[c] == R(1) && f_ice[IX(ip1, j)] == R(1)) {
                         if (f_grnd[c] > R(0) && f_grnd[IX(ip1, j)] == R(0))
-                            beta_acx[c] = stag == 1 ? beta[c] : beta[IX(ip1, j)];
+                            beta_acx[c] = up ? beta[c] : beta[IX(ip1, j)];
                         else if (f_grnd[c] == R(0) && f_grnd[IX(ip1, j)] > R(0))
-                            beta_acx[c] = stag == 1 ? beta[IX(ip1, j)] : beta[c];
+                            beta_acx[c] = up ? beta[IX(ip1, j)] : beta[c];
                     }
                     if (f_ice[c] == R(1) && f_ice[IX(i, jp1)] == R(1)) {
                         if (f_grnd[c] > R(0) && f_grnd[IX(i, jp1)] == R(0))
-                            beta_acy[c] = stag == 1 ? beta[c] : beta[IX(i, jp1)];
+                            beta_acy[c] = up ? beta[c] : beta[IX(i, jp1)];
                         else if (f_grnd[c] == R(0) && f_grnd[IX(i, jp1)] > R(0))
-                            beta_acy[c] = stag == 1 ? beta[IX(i, jp1)] : beta[c];
+                            beta_acy[c] = up ? beta[IX(i, jp1)] : beta[c];
                     }
                 }
         } else if (stag != 0) {
@@ -658,10 +735,17 @@ int oracle_stagger_beta(wp *beta_acx, wp *beta_acy, const wp *beta, const wp *f_
 static void set_beta_min_grounded(wp *bx, wp *by, const int32_t *mx, const int32_t *my,
                                   wp beta_min, int nc)
 {
+    const int any = PLANT(PL_BETA_MIN_MASK);
     for (int c = 0; c < nc; c++) {
-        if (mx[c] == 1 && bx[c] == R(0)) bx[c] = beta_min;
-        if (my[c] == 1 && by[c] == R(0)) by[c] = beta_min;
+        if ((any ? mx[c] >= 1 : mx[c] == 1) && bx[c] == R(0)) bx[c] = beta_min;
+        if ((any ? my[c] >= 1 : my[c] == 1) && by[c] == R(0)) by[c] = beta_min;
     }
+}
+
+void oracle_set_beta_min_grounded(wp *bx, wp *by, const int32_t *mx, const int32_t *my,
+                                  wp beta_min, int nc)
+{
+    set_beta_min_grounded(bx, by, mx, my, beta_min, nc);
 }
 
 /* calc_vel_lim_drag (solver_ssa_ac.f90:1095-1199) */
@@ -676,9 +760,9 @@ static void lim_drag_face(wp *k, wp *r, wp u, wp u_cross, int mask, wp s0, wp in
     wp tau = tau_c * x * x;
     wp dtauds = R(2) * tau_c * x * inv_du;
     wp b = tau / s;
-    wp w = (u / s) * (u / s);
+    wp w = PLANT(PL_LIM_W) ? u / s : (u / s) * (u / s);
     *k = b * (R(1) - w) + dtauds * w;
-    *r = (*k - b) * u;
+    *r = PLANT(PL_LIM_R) ? *k * u : (*k - b) * u;
 }
 
 void oracle_vel_lim_drag(wp *kx, wp *ky, wp *rx, wp *ry, const wp *ux, const wp *uy,
@@ -692,7 +776,8 @@ void oracle_vel_lim_drag(wp *kx, wp *ky, wp *rx, wp *ry, const wp *ux, const wp 
         for (int i = 0; i < nx; i++) {
             int im1, ip1, jm1, jp1, c = IX(i, j);
             nbr(i, j, nx, ny, BC, &im1, &ip1, &jm1, &jp1);
-            wp u_cross = R(0.25) * (uy[c] + uy[IX(ip1, j)] + uy[IX(i, jm1)] + uy[IX(ip1, jm1)]);
+            const int jx = PLANT(PL_LIM_CROSS) ? jp1 : jm1;
+            wp u_cross = R(0.25) * (uy[c] + uy[IX(ip1, j)] + uy[IX(i, jx)] + uy[IX(ip1, jx)]);
             lim_drag_face(&kx[c], &rx[c], ux[c], u_cross, mx[c], s0, inv_du, tau_c);
             u_cross = R(0.25) * (ux[c] + ux[IX(im1, j)] + ux[IX(i, jp1)] + ux[IX(im1, jp1)]);
             lim_drag_face(&ky[c], &ry[c], uy[c], u_cross, my[c], s0, inv_du, tau_c);
@@ -1379,14 +1464,16 @@ static wp picard_conv_l2(int *conv, const wp *ux, const wp *uy, const wp *uxp, c
     const double du_reg = 1e-10;
     const wp vel_tol = RS(1e-5);
     const wp tol_uf = R(1e-15);
-    double res1 = 0, res2 = 0;
+    double res1 = 0, res2 = 0, res3 = 0;
     int nchk = 0;
-    double *r1 = calloc(ny, 8), *r2 = calloc(ny, 8);
+    double *r1 = calloc(ny, 8), *r2 = calloc(ny, 8), *r3 = calloc(ny, 8);
     for (int j = 0; j < ny; j++) {
-        double a = 0, b = 0;
+        double a = 0, b = 0, e = 0;
         for (int i = 0; i < nx; i++) {
             int c = IX(i, j);
-            if (WABS(ux[c]) > vel_tol && mx[c] > 0) {
+            const wp vt = PLANT(PL_CONV_VEL_TOL) ? R(-1) : vel_tol;
+            const int m0 = PLANT(PL_CONV_MASK) ? -1 : 0;
+            if (WABS(ux[c]) > vt && mx[c] > m0) {
                 nchk++;
                 double t = (double)(ux[c] - uxp[c]);
                 if (fabs(t) < (double)tol_uf) t = 0.0;
@@ -1394,8 +1481,10 @@ static wp picard_conv_l2(int *conv, const wp *ux, const wp *uy, const wp *uxp, c
                 t = (double)uxp[c];
                 if (fabs(t) < (double)tol_uf) t = 0.0;
                 b += t * t;
+                t = (double)(ux[c] + uxp[c]);
+                e += t * t;
             }
-            if (WABS(uy[c]) > vel_tol && my[c] > 0) {
+            if (WABS(uy[c]) > vt && my[c] > m0) {
                 nchk++;
                 double t = (double)(uy[c] - uyp[c]);
                 if (fabs(t) < (double)tol_uf) t = 0.0;
@@ -1403,14 +1492,21 @@ static wp picard_conv_l2(int *conv, const wp *ux, const wp *uy, const wp *uxp, c
                 t = (double)uyp[c];
                 if (fabs(t) < (double)tol_uf) t = 0.0;
                 b += t * t;
+                t = (double)(uy[c] + uyp[c]);
+                e += t * t;
             }
         }
         r1[j] = a;
         r2[j] = b;
+        r3[j] = e;
     }
-    for (int j = 0; j < ny; j++) { res1 += r1[j]; res2 += r2[j]; }
-    free(r1); free(r2);
-    wp resid = nchk > 0 ? (wp)(sqrt(res1) / (sqrt(res2) + du_reg)) : R(0);
+    for (int j = 0; j < ny; j++) { res1 += r1[j]; res2 += r2[j]; res3 += r3[j]; }
+    free(r1); free(r2); free(r3);
+    /* norm_method = 1 (:1976): ||du|| / (||u_prev|| + du_reg) */
+    const double reg = PLANT(PL_CONV_DU_REG) ? 0.0 : du_reg;
+    wp resid = nchk > 0 ? (wp)(PLANT(PL_CONV_NORM2) ? 2.0 * sqrt(res1) / (sqrt(res3) + reg)
+                                                    : sqrt(res1) / (sqrt(res2) + reg))
+                        : R(0);
     *ncheck = nchk;
     if (resid < tol) *conv = 1;
     else if (iter == iter_max) *conv = 2;
@@ -1422,15 +1518,30 @@ static wp picard_conv_l2(int *conv, const wp *ux, const wp *uy, const wp *uxp, c
     return resid;
 }
 
+int oracle_picard_conv_l2(int *conv, wp *resid, const wp *ux, const wp *uy, const wp *uxp,
+                          const wp *uyp, const int32_t *mx, const int32_t *my, wp tol, int iter,
+                          int iter_max, int nx, int ny)
+{
+    int nchk = 0;
+    *resid = picard_conv_l2(conv, ux, uy, uxp, uyp, mx, my, tol, iter, iter_max, nx, ny, &nchk);
+    return nchk;
+}
+
 /* picard_calc_convergence_l1rel_matrix (velocity_general.f90:1907-1948) */
 static void picard_l1rel(wp *ex, wp *ey, const wp *ux, const wp *uy, const wp *uxp,
                          const wp *uyp, int nc)
 {
-    const wp vtol = RS(1e-2), tol = RS(1e-5);
+    const wp vtol = RS(1e-2), tol = PLANT(PL_L1REL_TOL) ? R(0) : RS(1e-5);
     for (int c = 0; c < nc; c++) {
         ex[c] = WABS(ux[c]) > vtol ? R(2) * WABS(ux[c] - uxp[c]) / WABS(ux[c] + uxp[c] + tol) : R(0);
         ey[c] = WABS(uy[c]) > vtol ? R(2) * WABS(uy[c] - uyp[c]) / WABS(uy[c] + uyp[c] + tol) : R(0);
     }
+}
+
+void oracle_picard_l1rel(wp *ex, wp *ey, const wp *ux, const wp *uy, const wp *uxp,
+                         const wp *uyp, int nc)
+{
+    picard_l1rel(ex, ey, ux, uy, uxp, uyp, nc);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1535,19 +1646,8 @@ int oracle_calc_velocity_ssa(
             if (WABS(ux_b[c]) < R(1e-15)) ux_b[c] = R(0);
             if (WABS(uy_b[c]) < R(1e-15)) uy_b[c] = R(0);
         }
-        /* ssa_vel_clip / limit_vel (solver_ssa_ac.f90:1069-1093,1264-1280) */
-        wp ulim = (wp)par->ssa_vel_max;
-        for (int c = 0; c < nc; c++) {
-            if (ux_b[c] > ulim) ux_b[c] = ulim;
-            if (ux_b[c] < -ulim) ux_b[c] = -ulim;
-            if (uy_b[c] > ulim) uy_b[c] = ulim;
-            if (uy_b[c] < -ulim) uy_b[c] = -ulim;
-        }
-        /* picard_relax_vel (velocity_general.f90:2160-2186) */
-        for (int c = 0; c < nc; c++) {
-            ux_b[c] = ux_nm1[c] + rel * (ux_b[c] - ux_nm1[c]);
-            uy_b[c] = uy_nm1[c] + rel * (uy_b[c] - uy_nm1[c]);
-        }
+        oracle_ssa_vel_clip(ux_b, uy_b, (wp)par->ssa_vel_max, nc);
+        oracle_picard_relax_vel(ux_b, uy_b, ux_nm1, uy_nm1, rel, nc);
         int conv = 0, nchk = 0;
         wp resid = picard_conv_l2(&conv, ux_b, uy_b, ux_nm1, uy_nm1, ssa_mask_acx, ssa_mask_acy,
                                   (wp)par->ssa_iter_conv, iter, par->ssa_iter_max, nx, ny, &nchk);

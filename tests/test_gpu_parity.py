@@ -46,6 +46,15 @@ def test_ssa_fp64_matches_fp64_oracle(n, dx, vm, bm):
     assert abs(info.ssa_iter_now - ref["ssa_iter_now"]) <= 1
     assert err <= 1e-8
     assert rel_l2(g.state["visc_eff_int"], ref_st["visc_eff_int"]) <= 1e-7
+    # the diagnostics the velocity does not feed back into
+    for name in ("taub_acx", "taub_acy", "beta_acx", "beta_acy", "beta"):
+        assert rel_l2(g.state[name], ref_st[name]) <= 1e-8, name
+    # l1rel error of the last iteration: relative changes of about the Picard
+    # tolerance, so the solution's 1e-16 agreement shows at about 1e-12
+    for name in ("ssa_err_acx", "ssa_err_acy"):
+        assert rel_l2(g.state[name], ref_st[name]) <= 1e-6, name
+    assert abs(info.ssa_lim_n - ref["ssa_lim_n"]) <= 1
+    assert info.ssa_lin_iter == pytest.approx(ref["ssa_lin_iter"], rel=0.02)
 
 
 @pytest.mark.parametrize("n,dx,vm,bm", CASES[:2])
