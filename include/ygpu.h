@@ -162,6 +162,23 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
                    const char* opts, int32_t* iters, int32_t* status, double* rel_resid,
                    double* seconds);
 
+/* ---- ygpu_csr_solve without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, results and errors as ygpu_csr_solve (seconds: host wall time
+ * of the Krylov launch).  Every device buffer comes from one static device
+ * array of YGPU_NOALLOC_SCRATCH_BYTES per GPU, held from library load until
+ * the process ends; systems above YGPU_NOALLOC_MAX_CELLS cells (n/2), or
+ * with more than 9 entries per row, are rejected with YGPU_ERR_ARG.  All
+ * calls on one GPU share that array (each clears what it needs): calls that
+ * overlap on one GPU, from several threads or handles, race.               */
+#define YGPU_NOALLOC_MAX_CELLS 32768
+#define YGPU_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+int ygpu_csr_solve_noalloc(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,
+                           const int32_t* a_idx1, const double* a_val, const double* b,
+                           double* x_inout, const char* opts, int32_t* iters, int32_t* status,
+                           double* rel_resid, double* seconds);
+
 /* y = A x for a CSR matrix of the same pattern (LIS's lis_matvec at the
  * boundary of solver_linear.F90; 1-based arrays as in lgs), on the device.  */
 int ygpu_csr_spmv(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1,

@@ -66,7 +66,8 @@ NEFF_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_v
 
 # every symbol include/ygpu.h declares
 EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_last_note",
-           "ygpu_default_params", "ygpu_csr_solve", "ygpu_velocity_ssa", "ygpu_velocity_diva",
+           "ygpu_default_params", "ygpu_csr_solve", "ygpu_csr_solve_noalloc",
+           "ygpu_velocity_ssa", "ygpu_velocity_diva",
            "ygpu_velocity_ssa_batch",
            "ygpu_velocity_diva_batch", "ygpu_batch_create", "ygpu_batch_run",
            "ygpu_batch_download", "ygpu_batch_destroy", "ygpu_batch_spmv", "ygpu_csr_spmv", "ygpu_session_create", "ygpu_session_solve",
@@ -117,6 +118,8 @@ def load() -> C.CDLL:
     lib.ygpu_csr_solve.argtypes = [PP, I, I, PP, PP, PP, PP, PP, C.c_char_p, C.POINTER(I),
                                    C.POINTER(I), C.POINTER(D), C.POINTER(D)]
     lib.ygpu_csr_solve.restype = C.c_int
+    lib.ygpu_csr_solve_noalloc.argtypes = lib.ygpu_csr_solve.argtypes
+    lib.ygpu_csr_solve_noalloc.restype = C.c_int
     lib.ygpu_velocity_ssa_batch.argtypes = [PP, C.POINTER(SsaParamsC), I, C.POINTER(SsaProblemC)]
     lib.ygpu_velocity_ssa_batch.restype = C.c_int
     lib.ygpu_velocity_diva_batch.argtypes = [PP, C.POINTER(SsaParamsC), I, C.POINTER(SsaProblemC)]

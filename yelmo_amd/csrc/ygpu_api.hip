@@ -245,6 +245,8 @@ int ygpu_create(int32_t device, ygpu_handle** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return YGPU_ERR_DEVICE;
   if (device < 0 || device >= n) return YGPU_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return YGPU_ERR_DEVICE;
+  // the no-allocation entry point's scratch address on this device
+  if (yg_noalloc_init(device) != 0) return YGPU_ERR_DEVICE;
   ygpu_handle* h = new ygpu_handle();
   h->device = device;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -67,6 +67,10 @@ double krylov_bytes(double C, int iters, int pA, double pF, int pv, bool ilu);
 void step_table(int nx, int ny, std::vector<int>& soff, int& T);
 // boundaries enum of the C-ABI -> neighbour-index code of the kernels
 int bc_code(int bnd);
+}  // namespace yg
+// looks up the device's no-allocation scratch array (ygpu_noalloc.hip); 0 ok
+int yg_noalloc_init(int device);
+namespace yg {
 
 // ---- device allocations of one batch --------------------------------------
 struct Arena {
