@@ -230,8 +230,10 @@ def _serve_run(srv, seqs, arrivals, record):
     sequence seqs[k]) arrives at arrivals[k] seconds after the start; each
     solve after the first is submitted when its predecessor finishes.
     Returns per-request latency (arrival -> last solve done), per-solve
-    latency (submit -> done), the makespan and the peak number of solves
-    running at once.  record(k, j, prob) sees every finished solve."""
+    latency (submit -> done), the makespan, the peak number of solves running
+    at once, the server steps and the time by which 90 % of the solves were
+    done.  record(k, j, prob, info) sees every finished solve (prob holds
+    its results)."""
     n = len(seqs)
     order = np.argsort(arrivals, kind="stable")
     owner, t_submit, req_lat, solve_lat, t_done = {}, {}, np.zeros(n), [], []
@@ -274,18 +276,94 @@ def _serve_run(srv, seqs, arrivals, record):
     return req_lat, np.array(solve_lat), time.perf_counter() - t0, peak, steps, t90
 
 
+def _fresh(seqs):
+    """Copies of the request sequences: every pass starts from the
+    requests' own cold initial state (fetched results land in the copies)."""
+    return [[p.copy() for p in s] for s in seqs]
+
+
+def _vel(p):
+    return np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]).astype(np.float64)
+
+
+def serve_stream(seqs, par, lanes, budget, load=None, timed=True):
+    """Serves the request sequences with continuous batching.  A closed-loop
+    pass (every request queued at the start) gives the capacity in solves/s
+    (completion rate up to 90 % of the solves); with ``load``, a second pass
+    on fresh copies has requests arrive Poisson at that fraction of the
+    capacity and gives the latencies.  Returns (figures, results) with
+    results[(k, j)] = (velocity, Picard iterations, status) of every solve."""
+    from yelmo_amd.serve import Server
+    nreq = len(seqs)
+    nsolves = sum(len(s) for s in seqs)
+    srv = Server(par, nz_max=seqs[0][0].nz, hbm_budget=budget, lanes=lanes)
+    # warm-up: one small sequence through the same server
+    from yelmo_amd import generator
+    warm = generator.served_sequence(81, 99, 2)
+    _serve_run(srv, [warm], np.zeros(1), lambda k, j, p, info: None)
+    res_closed, res_poisson = {}, {}
+
+    def recorder(store):
+        def rec(k, j, p, info):
+            store[(k, j)] = (_vel(p), info.ssa_iter_now, info.status)
+        return rec
+
+    lat_c, slat_c, span_c, peak_c, steps_c, t90_c = _serve_run(srv, _fresh(seqs), np.zeros(nreq),
+                                                               recorder(res_closed))
+    capacity = 0.9 * nsolves / t90_c
+    out = {"capacity_solves_per_s": capacity,
+           "closed_loop": {"makespan_s": span_c, "drain_solves_per_s": nsolves / span_c,
+                           "t90_s": t90_c, "peak_running": peak_c, "steps": steps_c,
+                           "request_latency_s": {"p50": float(np.percentile(lat_c, 50)),
+                                                 "p99": float(np.percentile(lat_c, 99))}},
+           "server_device_bytes": srv.device_bytes,
+           "failed_solves": sum(1 for v in res_closed.values()
+                                if v[2] != 0 or not np.isfinite(v[0]).all())}
+    if load is not None:
+        from yelmo_amd import generator as g
+        rate = load * capacity / g.SERVE_SOLVES
+        rng = np.random.default_rng(50_000)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+        lat, slat, span, peak, steps, _ = _serve_run(srv, _fresh(seqs), arrivals,
+                                                     recorder(res_poisson))
+        out["poisson"] = {"load": load, "arrival_rate_req_per_s": rate,
+                          "solves_per_s": nsolves / span, "makespan_s": span,
+                          "peak_running": peak, "steps": steps,
+                          "request_latency_s": {"p50": float(np.percentile(lat, 50)),
+                                                "p99": float(np.percentile(lat, 99))},
+                          "solve_latency_s": {"p50": float(np.percentile(slat, 50)),
+                                              "p99": float(np.percentile(slat, 99))}}
+        # the same requests served under another arrival pattern: a solve
+        # lands at another place of the device arena, which changes the
+        # chunking of its dot products (rounding), nothing else
+        out["max_rel_l2_poisson_vs_closed_loop"] = max(
+            rel_l2(res_poisson[key][0], res_closed[key][0]) for key in res_closed)
+    srv.close()
+    return out, res_closed
+
+
+def serve_errors(res, ref):
+    """Per-solve relative L2 distance of served velocities from the
+    reference run's, the worst solve and its Picard counts."""
+    errs = [(rel_l2(v, ref[key][0]), key, it, ref[key][1]) for key, (v, it, _) in res.items()]
+    w = max(errs, key=lambda a: a[0])
+    return {"max_rel_l2": w[0], "median_rel_l2": float(np.median([e[0] for e in errs])),
+            "worst_solve": {"request": w[1][0], "solve": w[1][1], "picard": w[2],
+                            "picard_reference": w[3], "rel_l2": w[0]}}
+
+
 def run_serve(args):
     """Served stream of SSA solve sequences (continuous batching).  Requests
     of lognormal grid side (clipped to 81..641, 1536 km across) each ask for
     5 successive solves with seeded smooth +-10 m thickness increments
     between them, warm-started from the previous solve's device-resident
-    state.  Per precision: a closed-loop run (every request queued at the
-    start) gives the capacity in solves/s; then requests arrive Poisson at
-    70% of that capacity and p50/p99 latency is measured.  The mixed-precision
-    velocities of every solve are compared with the FP64 server's."""
+    state.  Per precision (mixed, plain FP64 at the reference's rtol 1e-8):
+    the capacity from a closed-loop pass, then p50/p99 latency with Poisson
+    arrivals at --load of that capacity; every pass serves fresh copies of
+    the requests, so every first solve starts cold.  The error reference is
+    the FP64 server with the linear solves converged to rtol 1e-12."""
     from yelmo_amd import generator
     from yelmo_amd.params import SsaParams
-    from yelmo_amd.serve import Server
     nreq = args.requests
     sides = generator.serve_request_sides(nreq, seed=0)
     t_gen = time.perf_counter()
@@ -296,69 +374,18 @@ def run_serve(args):
     nsolves = sum(len(s) for s in seqs)
     budget = int(args.budget_gb * (1 << 30))
     lanes = [tuple(int(v) for v in part.split(":")) for part in args.lanes.split(",")]
-    vel = {}
-    out = {}
-    for prec in ("fp64", "mixed"):
-        par = SsaParams(precision=prec)
-        srv = Server(par, nz_max=seqs[0][0].nz, hbm_budget=budget, lanes=lanes)
-        # warm-up: one small sequence through the same server
-        warm = generator.served_sequence(81, 99, 2)
-        _serve_run(srv, [warm], np.zeros(1), lambda k, j, p, info: None)
-        errs, failed = [], []
-
-        def record(k, j, p, info, prec=prec, errs=errs, failed=failed):
-            v = np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]).astype(np.float64)
-            if info.status != 0 or not np.isfinite(v).all():
-                failed.append((k, j, info.status))
-            if prec == "fp64":
-                vel[(k, j)] = (v, info.ssa_iter_now)
-            else:
-                v64, it64 = vel[(k, j)]
-                errs.append((rel_l2(v, v64), k, j, info.ssa_iter_now, it64))
-
-        lat_c, slat_c, span_c, peak_c, steps_c, t90_c = _serve_run(srv, seqs, np.zeros(nreq),
-                                                                   record)
-        # capacity: the completion rate while the queue is full, up to 90 %
-        # of all solves (the tail is one large sequence running alone)
-        capacity = 0.9 * nsolves / t90_c
-        rate = args.load * capacity / generator.SERVE_SOLVES
-        rng = np.random.default_rng(50_000)
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-        errs_closed = list(errs)
-        errs.clear()
-        lat, slat, span, peak, steps, _ = _serve_run(srv, seqs, arrivals, record)
-        srv.close()
-        out[prec] = {
-            "capacity_solves_per_s": capacity,
-            "closed_loop": {"makespan_s": span_c, "drain_solves_per_s": nsolves / span_c,
-                            "t90_s": t90_c, "peak_running": peak_c, "steps": steps_c,
-                            "request_latency_s": {"p50": float(np.percentile(lat_c, 50)),
-                                                  "p99": float(np.percentile(lat_c, 99))}},
-            "poisson": {"arrival_rate_req_per_s": rate, "solves_per_s": nsolves / span,
-                        "makespan_s": span, "peak_running": peak, "steps": steps,
-                        "request_latency_s": {"p50": float(np.percentile(lat, 50)),
-                                              "p99": float(np.percentile(lat, 99))},
-                        "solve_latency_s": {"p50": float(np.percentile(slat, 50)),
-                                            "p99": float(np.percentile(slat, 99))}},
-            "server_device_bytes": srv.device_bytes,
-            "failed_solves": len(failed),
-        }
-        if prec == "mixed":
-            allerr = errs_closed + errs
-            e = np.array([a[0] for a in allerr], dtype=np.float64)
-            out[prec]["max_rel_l2_vs_fp64"] = float(np.max(e)) if np.isfinite(e).all() else float("nan")
-            same = [a[0] for a in allerr if a[3] == a[4]]
-            # solves whose Picard loops stopped at the same iteration in both
-            # precisions; a solve that stops one iteration apart differs by
-            # about the Picard tolerance
-            out[prec]["max_rel_l2_vs_fp64_same_picard"] = float(np.max(same)) if same else None
-            out[prec]["solves_picard_count_differs"] = len(allerr) - len(same)
-            w = max(allerr, key=lambda a: a[0])
-            out[prec]["worst_solve"] = {"request": w[1], "solve": w[2], "side": int(sides[w[1]]),
-                                        "picard_mixed": w[3], "picard_fp64": w[4],
-                                        "rel_l2": w[0]}
-        print(f"# {prec}: capacity {capacity:.1f} solves/s, p99 {out[prec]['poisson']['request_latency_s']['p99']:.2f}s",
-              file=sys.stderr, flush=True)
+    out, res = {}, {}
+    for prec in ("mixed", "fp64"):
+        out[prec], res[prec] = serve_stream(seqs, SsaParams(precision=prec), lanes, budget,
+                                            load=args.load)
+        print(f"# {prec}: capacity {out[prec]['capacity_solves_per_s']:.1f} solves/s, p99 "
+              f"{out[prec]['poisson']['request_latency_s']['p99']:.2f}s", file=sys.stderr,
+              flush=True)
+    conv = SsaParams(precision="fp64")
+    conv = conv.with_(ssa_lis_opt=conv.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
+    _, ref = serve_stream(seqs, conv, lanes, budget, load=None)
+    out["mixed"]["vs_fp64_converged"] = serve_errors(res["mixed"], ref)
+    out["fp64"]["vs_fp64_converged"] = serve_errors(res["fp64"], ref)
     mixed = out["mixed"]
     line = {
         "metric": "served SSA solves/s and p50/p99 request latency, continuous batching",
@@ -375,17 +402,146 @@ def run_serve(args):
                    "hbm_budget_gb": args.budget_gb},
         "p50_latency_s": mixed["poisson"]["request_latency_s"]["p50"],
         "p99_latency_s": mixed["poisson"]["request_latency_s"]["p99"],
-        # against the plain FP64 server at the reference's rtol 1e-8, whose
-        # own distance from converged linear solves is 0.3-3e-6 on these
-        # grids (the mixed path is within 3e-8 of converged)
-        "max_rel_l2_vs_fp64": mixed["max_rel_l2_vs_fp64"],
-        "max_rel_l2_vs_fp64_same_picard": mixed["max_rel_l2_vs_fp64_same_picard"],
+        # against the FP64 server with linear solves converged to rtol 1e-12
+        "max_rel_l2_vs_fp64": mixed["vs_fp64_converged"]["max_rel_l2"],
+        "error_ok": mixed["vs_fp64_converged"]["max_rel_l2"] <= 1e-6,
         "failed_solves": mixed["failed_solves"] + out["fp64"]["failed_solves"],
         "mixed": mixed, "fp64": out["fp64"],
         "speedup_capacity_vs_fp64": mixed["capacity_solves_per_s"] / out["fp64"]["capacity_solves_per_s"],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(_finite(line)), flush=True)
+
+
+def served_leg(args):
+    """The bounded served stream of the default line: --serve-requests
+    requests (sides lognormal in [81, 641]) of 5 warm-started solves,
+    mixed precision, capacity and p50/p99 request latency at 70 % load,
+    errors against the converged FP64 server."""
+    from yelmo_amd import generator
+    from yelmo_amd.params import SsaParams
+    nreq = args.serve_requests
+    sides = generator.serve_request_sides(nreq, seed=0)
+    seqs = [generator.served_sequence(int(s), 1000 + k, generator.SERVE_SOLVES)
+            for k, s in enumerate(sides)]
+    budget = int(args.budget_gb * (1 << 30))
+    lanes = [tuple(int(v) for v in part.split(":")) for part in args.lanes.split(",")]
+    mixed, res = serve_stream(seqs, SsaParams(precision="mixed"), lanes, budget, load=args.load)
+    conv = SsaParams(precision="fp64")
+    conv = conv.with_(ssa_lis_opt=conv.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
+    _, ref = serve_stream(seqs, conv, lanes, budget, load=None)
+    e = serve_errors(res, ref)
+    print(f"# served {nreq} requests: capacity {mixed['capacity_solves_per_s']:.1f} solves/s, "
+          f"p99 {mixed['poisson']['request_latency_s']['p99']:.3f}s", file=sys.stderr, flush=True)
+    return {"requests": nreq, "solves": sum(len(q) for q in seqs),
+            "sides": [int(v) for v in sides],
+            "capacity_solves_per_s": mixed["capacity_solves_per_s"],
+            "solves_per_s_at_load": mixed["poisson"]["solves_per_s"], "load": args.load,
+            "p50_request_latency_s": mixed["poisson"]["request_latency_s"]["p50"],
+            "p99_request_latency_s": mixed["poisson"]["request_latency_s"]["p99"],
+            "p50_solve_latency_s": mixed["poisson"]["solve_latency_s"]["p50"],
+            "p99_solve_latency_s": mixed["poisson"]["solve_latency_s"]["p99"],
+            "max_rel_l2_vs_fp64": e["max_rel_l2"], "worst_solve": e["worst_solve"],
+            "max_rel_l2_poisson_vs_closed_loop": mixed["max_rel_l2_poisson_vs_closed_loop"],
+            "failed_solves": mixed["failed_solves"]}
+
+
+def config1_single(args, h):
+    """One 161 x 161 instance (dx 8 km), the mixed path beside the plain
+    FP64 path: solves/s of each (repeated solves of the resident domain),
+    rel-L2 and max velocity error against the FP64 path with converged
+    linear solves, Picard and Krylov counts, the Krylov kernel's HBM rate."""
+    from yelmo_amd import generator
+    from yelmo_amd.params import SsaParams
+    from yelmo_amd.velocity import ResidentBatch
+    p0 = generator.make_problem(161, 8e3, seed=0, dtype=np.float32)
+    par_m = SsaParams(precision="mixed")
+    par_m = par_m.with_(ssa_lis_opt=par_m.ssa_lis_opt + f" -ygpu_factor {args.factor}")
+    par_64 = SsaParams(precision="fp64")
+    conv = par_64.with_(ssa_lis_opt=par_64.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
+    out = {"workload": "one 161 x 161 SSA instance (dx 8 km, 51842 unknowns)"}
+    rb = ResidentBatch([p0.astype(np.float64)], conv, handle=h)
+    rb.run()
+    rb.download()
+    ref = _vel(rb.problems[0])
+    rb.close()
+    for name, par, prob in (("mixed", par_m, p0.copy()), ("fp64", par_64, p0.astype(np.float64))):
+        rb = ResidentBatch([prob], par, handle=h)
+        rb.run()
+        _sync()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rb.run()
+        _sync()
+        dt = (time.perf_counter() - t0) / reps
+        info = rb.download()[0]
+        h.stats_enable(True)
+        rb.run()
+        st = h.stats()
+        h.stats_enable(False)
+        rb.close()
+        v = _vel(prob)
+        avg = st["krylov_ms"] / max(st["krylov_launches"], 1)
+        out[name] = {"solves_per_s": 1.0 / dt, "rel_l2_vs_fp64_converged": rel_l2(v, ref),
+                     "max_abs_vel_err_m_per_yr": float(np.max(np.abs(v - ref))),
+                     "picard_iters": info.ssa_iter_now, "krylov_iters": st["krylov_iters"],
+                     "krylov_kernel": _krylov_kernel_name(st),
+                     "krylov_gb_per_s": (st["krylov_bytes"] / max(st["krylov_launches"], 1)) /
+                                        (avg * 1e-3) / 1e9 if avg > 0 else None}
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    print(f"# single 161^2: mixed {out['mixed']['solves_per_s']:.2f}, fp64 "
+          f"{out['fp64']['solves_per_s']:.2f} solves/s", file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_baseline(args, base):
+    """The oracle (FP32 fields + FP64 BiCGSTAB/ILU(0): the reference's
+    arithmetic) on this host's cores: one thread on --cpu-sample members,
+    then --cpu-threads threads on as many members at once (ctypes releases
+    the GIL around each oracle call)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle
+    from yelmo_amd.params import SsaParams
+    par_ref = SsaParams(precision="ref")
+
+    def solve(k):
+        _, info = oracle.calc_velocity_ssa(base[k % len(base)], par_ref, dp=False)
+        return 2.0 * base[k % len(base)].nx * base[k % len(base)].ny * info["ssa_lin_iter"]
+
+    n1 = min(args.cpu_sample, len(base))
+    t0 = time.perf_counter()
+    ui1 = sum(solve(k) for k in range(n1))
+    t1 = time.perf_counter() - t0
+    nt = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(nt) as ex:
+        uis = list(ex.map(solve, range(nt)))
+    tn = time.perf_counter() - t0
+    return {"value": nt / tn, "unit": "solves/s", "cores": nt, "kind": "port",
+            "unknown_iters_per_s": sum(uis) / tn,
+            "single_thread": {"solves_per_s": n1 / t1, "unknown_iters_per_s": ui1 / t1,
+                              "members": n1},
+            "host_cpus": os.cpu_count(),
+            "sample": f"{nt} members of the {args.members} x {args.side}^2 ensemble solved at "
+                      f"once on {nt} threads ({tn:.1f}s), and {n1} members on 1 thread "
+                      f"({t1:.1f}s); full Picard solve each, C oracle (FP32 fields, FP64 "
+                      f"BiCGSTAB+ILU(0))"}
+
+
+def counted_traffic(kernel):
+    """HBM bytes per launch of `kernel` counted by the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs recorded in profiles/counted_traffic.json (scaled by the
+    plain-copy calibration stated there), or None."""
+    path = os.path.join(ROOT, "profiles", "counted_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        e = d["kernels"][kernel]
+        return e["bytes_per_launch"], e
+    except (OSError, KeyError, ValueError):
+        return None, None
 
 
 def main():
@@ -396,7 +552,14 @@ def main():
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--side", type=int, default=161)
     ap.add_argument("--cpu-sample", type=int, default=2,
-                    help="ensemble members solved by the CPU oracle for the baseline")
+                    help="ensemble members solved by the CPU oracle on one thread")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads (and members) of the all-core CPU baseline (capped at the "
+                         "host's CPUs)")
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests of the bounded served stream in the default line (0: none)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-instance 161^2 leg of the default line")
     ap.add_argument("--single-sample", type=int, default=4,
                     help="members solved one at a time on the FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
@@ -526,6 +689,12 @@ def main():
             errs_plain.append(rel_l2(v, plain[k]))
             maxabs.append(float(np.max(np.abs(v - ref[k]))))
     rb.close()
+    single = None
+    if not args.mixed_only and not args.no_single:
+        single = config1_single(args, h)
+    served = None
+    if not args.mixed_only and args.serve_requests > 0:
+        served = served_leg(args)
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.members * args.steps / elapsed
     fp64_batch_sps = args.members / t_fp64_batch
@@ -537,21 +706,12 @@ def main():
     kname = _krylov_kernel_name(st)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
-    # ---- CPU baseline: the oracle (FP32 fields + FP64 BiCGSTAB/ILU(0), the
-    # reference's arithmetic), one thread, on a bounded sample of members -------
+    # ---- CPU baseline: the oracle (the reference's arithmetic) on this host's
+    # cores, on a bounded sample of members -------------------------------------
     cpu = None
     if not args.no_cpu and not args.mixed_only and args.cpu_sample > 0:
-        from oracle import oracle
-        par_ref = SsaParams(precision="ref")
-        t0 = time.perf_counter()
-        n_cpu = min(args.cpu_sample, args.members)
-        for k in range(n_cpu):
-            oracle.calc_velocity_ssa(base[k], par_ref, dp=False)
-        tc = time.perf_counter() - t0
-        cpu = {"value": n_cpu / tc, "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"{n_cpu} members of the {args.members} x {args.side}^2 ensemble, "
-                         f"full Picard solve each, C oracle (FP32 fields, FP64 BiCGSTAB+ILU(0)), "
-                         f"1 thread, {tc:.1f}s"}
+        cpu = cpu_baseline(args, base)
+    traffic, traffic_src = counted_traffic(kname)
 
     line = {
         "metric": "SSA solves/s at <=1e-6 rel vel err vs plain FP64; SpMV HBM GB/s; p99 latency",
@@ -580,6 +740,9 @@ def main():
         # path at the reference's rtol 1e-8, whose own distance from the
         # converged solution is given beside it
         "max_rel_l2_vs_fp64": max(errs),
+        "error_reference": "plain FP64 GPU path with the linear solves converged to rtol 1e-12",
+        "rel_l2_vs_fp64_per_member": [float(f"{e:.3g}") for e in errs],
+        "max_abs_vel_err_per_member_m_per_yr": [float(f"{e:.3g}") for e in maxabs],
         "max_abs_vel_err_m_per_yr": max(maxabs),
         "max_rel_l2_vs_fp64_rtol1e-8": max(errs_plain),
         "fp64_rtol1e-8_max_rel_l2_vs_converged": max(err64),
@@ -596,8 +759,18 @@ def main():
         "unknown_iters_per_s": st["unknown_iters"] / ns * args.steps / elapsed,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
-                     "bytes_per_launch": bytes_per_launch, "launches": launches},
+                     # counted HBM bytes per launch from the committed counter runs
+                     "traffic": traffic,
+                     "traffic_source": None if traffic_src is None else traffic_src.get("source"),
+                     "avg_launch_ms": avg_ms,
+                     "bytes_per_launch": bytes_per_launch, "launches": launches,
+                     "refine_sweeps_per_launch": st["refine_sweeps"] / launches},
+        # config 1: one 161^2 instance, mixed beside plain FP64
+        "single_161": single,
+        # bounded served stream (continuous batching), p50/p99 request latency
+        "served": served,
+        "p50_latency_s": None if served is None else served["p50_request_latency_s"],
+        "p99_latency_s": None if served is None else served["p99_request_latency_s"],
         # y = A x of all 64 members in one launch (HBM-bound; the FP64 records
         # and vectors, 292 MB, exceed the 256 MB Infinity Cache, the FP32 ones
         # do not)

@@ -54,7 +54,8 @@ extern "C" {
 /* arithmetic of the outer solve */
 #define YGPU_PREC_FP64 0  /* FP64 fields, matrix and Krylov (plain FP64 path)          */
 #define YGPU_PREC_REF 1   /* FP32 fields, FP64 matrix and Krylov (the reference, wp=sp) */
-#define YGPU_PREC_MIXED 2 /* FP64 fields and residual; FP32 SpMV/ILU inside FP64 refinement */
+#define YGPU_PREC_MIXED 2 /* FP32 fields (FP64 with -ygpu_fields fp64), FP64 matrix
+                             records and residual; FP32 SpMV/ILU inside FP64 refinement */
 
 typedef struct ygpu_handle ygpu_handle;
 
@@ -153,6 +154,10 @@ void ygpu_default_params(ygpu_ssa_params* par);
  *   -ygpu_ilu_blocks N       ILU(0) over N line blocks (mixed path default:
  *                            one per 64 lines, or the autotuned schedule)
  *   -ygpu_ir_tol T           residual at which refinement stops (tol/10)
+ *   -ygpu_fields fp32|fp64   arithmetic of the mixed mode's field kernels
+ *                            (viscosity, friction, assembly, Picard update):
+ *                            FP32 by default, as the reference's wp = sp;
+ *                            the Picard norm accumulates in FP64 either way
  *   -ir_inner_tol T          relative tolerance of each FP32 inner solve
  *   -ygpu_graph N            Picard iterations per captured hipGraph (4; 0 =
  *                            launch each iteration and read the flags after
@@ -280,6 +285,8 @@ typedef struct ygpu_stats {
   int64_t picard_iters;         /* Picard iterations summed over members        */
   int32_t krylov_group;         /* workgroups per member of the last launch     */
   int32_t krylov_block;         /* threads per workgroup of the last launch     */
+  int64_t refine_sweeps;        /* mixed mode: FP64 residual passes summed over
+                                   members (their bytes are in krylov_bytes)    */
 } ygpu_stats;
 int ygpu_stats_enable(ygpu_handle* h, int32_t on);
 int ygpu_stats_get(ygpu_handle* h, ygpu_stats* out);  /* returns and resets     */

@@ -83,7 +83,8 @@ class StatsC(C.Structure):
     _fields_ = [("krylov_launches", C.c_int64), ("krylov_ms", C.c_double),
                 ("krylov_bytes", C.c_double), ("krylov_iters", C.c_int64),
                 ("unknown_iters", C.c_double), ("picard_iters", C.c_int64),
-                ("krylov_group", C.c_int32), ("krylov_block", C.c_int32)]
+                ("krylov_group", C.c_int32), ("krylov_block", C.c_int32),
+                ("refine_sweeps", C.c_int64)]
 
 _lib = None
 

@@ -31,6 +31,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.ilu_cols = 1;
   o.k.factor16 = 0; o.k.graph_chunk = 4;
   o.substituted = nullptr;
+  o.fields32 = 1;   // the mixed mode's field kernels compute in FP32 unless -ygpu_fields fp64
   if (!s) return 0;
   Tok k, v;
   while (next_tok(s, k)) {
@@ -77,6 +78,11 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
       else { err = "unknown factor storage: " + std::string(v.p, v.n); return YGPU_ERR_ARG; }
     }
     else if (k.is("-ygpu_ny")) o.ny_hint = v.to_int();
+    else if (k.is("-ygpu_fields")) {
+      if (v.is("fp32")) o.fields32 = 1;
+      else if (v.is("fp64")) o.fields32 = 0;
+      else { err = "unknown field precision: " + std::string(v.p, v.n); return YGPU_ERR_ARG; }
+    }
     else if (k.is("-ygpu_precision")) {
       if (v.is("fp64")) o.prec = 0;
       else if (v.is("mixed")) o.prec = 2;
@@ -103,6 +109,16 @@ double krylov_bytes(double C, int iters, int pA, double pF, int pv, bool ilu) {
   return setup + fact + iters * per_it;
 }
 
+// Per FP64 residual pass: the 18 FP64 slot values + 8 B of mask/coordinates
+// per cell, x and b read, r and A x written (4 vectors of 2 FP64 values per
+// cell).  Per correction: r read and the FP32 scaled residual and zeroed
+// correction written, then the correction read and x read and written.
+double refinement_bytes(double C, int nres) {
+  const double res_pass = 18.0 * C * 8 + 8.0 * C + 4.0 * 2.0 * C * 8;
+  const double corr = 2.0 * C * 8 + 2.0 * 2.0 * C * 4 + 2.0 * C * 4 + 2.0 * 2.0 * C * 8;
+  return nres * res_pass + std::max(0, nres - 1) * corr;
+}
+
 // skewed step table of an nx x ny member: soff[t] = cells before step t
 void step_table(int nx, int ny, std::vector<int>& soff, int& T) {
   T = ny + 2 * (nx - 1);
@@ -115,6 +131,14 @@ void step_table(int nx, int ny, std::vector<int>& soff, int& T) {
   }
 }
 
+
+bool fields32(const ygpu_ssa_params* par) {
+  if (par->precision == YGPU_PREC_REF) return true;
+  if (par->precision != YGPU_PREC_MIXED) return false;
+  Opts o;
+  std::string e;
+  return parse_opts(par->ssa_lis_opt, o, e) == 0 && o.fields32;
+}
 
 int bc_code(int bnd) {
   switch (bnd) {
@@ -224,12 +248,15 @@ int run_any(ygpu_handle* h, const ygpu_ssa_params* par, int np, ygpu_ssa_problem
       return YGPU_ERR_ARG;
     }
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
-  // fields are FP64 except in the reference mode (the reference's wp = sp)
-  if (par->precision == YGPU_PREC_FP64 || par->precision == YGPU_PREC_MIXED)
-    return run_batch<double>(h, par, np, probs, diva);
-  if (par->precision == YGPU_PREC_REF) return run_batch<float>(h, par, np, probs, diva);
-  h->err = "unknown precision";
-  return YGPU_ERR_ARG;
+  if (par->precision != YGPU_PREC_FP64 && par->precision != YGPU_PREC_MIXED &&
+      par->precision != YGPU_PREC_REF) {
+    h->err = "unknown precision";
+    return YGPU_ERR_ARG;
+  }
+  // FP32 fields in the reference mode (the reference's wp = sp) and the
+  // mixed mode with -ygpu_fields fp32; FP64 otherwise
+  if (!fields32(par)) return run_batch<double>(h, par, np, probs, diva);
+  return run_batch<float>(h, par, np, probs, diva);
 }
 
 }  // namespace
@@ -567,7 +594,7 @@ int ygpu_session_create(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_s
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   YgSessionBase* sb = nullptr;
   int rc;
-  if (par->precision != YGPU_PREC_REF) {
+  if (!fields32(par)) {
     auto* t = new YgSessionT<double>();
     rc = t->B.setup(h, par, 1, prob, 0);
     t->bytes = (int64_t)t->B.ar.bytes;
@@ -611,7 +638,7 @@ int ygpu_batch_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
     bb = t;
     return r;
   };
-  if (par->precision != YGPU_PREC_REF) rc = make(new YgBatchT<double>());
+  if (!fields32(par)) rc = make(new YgBatchT<double>());
   else rc = make(new YgBatchT<float>());
   if (rc) { delete bb; return rc; }
   if (device_bytes) *device_bytes = bb->bytes;
@@ -710,7 +737,7 @@ int ygpu_session_destroy(ygpu_session* s) {
 int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny, int32_t nz) {
   if (!par) return -1;
   const int64_t C = (int64_t)nx * ny, Z = C * nz;
-  const int64_t rb = par->precision != YGPU_PREC_REF ? 8 : 4;
+  const int64_t rb = fields32(par) ? 4 : 8;
   int64_t b = C * (4 + 4 + YG_NA * 8 + 2 * 8 + 2 * 8 + YG_NW64 * 2 * 8 + 8);
   if (par->precision == YGPU_PREC_MIXED) b += C * (YG_NA * 4 + YG_NF * 4 + YG_NW32 * 2 * 4);
   else b += C * YG_NF * 8;

@@ -57,16 +57,23 @@ struct Opts {
   int group = 0;  // workgroups per member: 0 automatic, 1 one (no groups), 2, 4 or 8
   int group_threads = 0;   // threads per group workgroup: 512 (one per CU), 256 (two), 0 automatic
   const char* substituted = nullptr;   // solver named in -i that another one stands in for
+  int fields32 = 1;   // mixed mode: field kernels in FP32 (-ygpu_fields fp32, the default)
 };
 
 int parse_opts(const char* s, Opts& o, std::string& err);
 // Algorithmic HBM bytes of one member's Krylov launch (defined with parse_opts)
 // pF: bytes per factor value (BF16 line-block factors: 40 bytes per cell / 18)
 double krylov_bytes(double C, int iters, int pA, double pF, int pv, bool ilu);
+// Algorithmic HBM bytes of the mixed mode's FP64 refinement for one member
+// of C cells that made nres residual passes (and nres - 1 corrections)
+double refinement_bytes(double C, int nres);
 // skewed step table of an nx x ny member: soff[t] = cells before step t
 void step_table(int nx, int ny, std::vector<int>& soff, int& T);
 // boundaries enum of the C-ABI -> neighbour-index code of the kernels
 int bc_code(int bnd);
+// Field kernels in FP32: the reference mode (the reference's wp = sp) and
+// the mixed mode unless -ygpu_fields fp64; FP64 in the FP64 mode.
+bool fields32(const ygpu_ssa_params* par);
 }  // namespace yg
 // looks up the device's no-allocation scratch array (ygpu_noalloc.hip); 0 ok
 int yg_noalloc_init(int device);
@@ -113,7 +120,7 @@ struct Batch {
   float *d_A32 = nullptr, *d_F32 = nullptr, *d_W32 = nullptr;
   uint16_t* d_F16 = nullptr;   // BF16 line-block factors (-ygpu_factor bf16)
   int *d_done = nullptr, *d_iter_now = nullptr, *d_lin_iter = nullptr, *d_lin_fail = nullptr,
-      *d_lim_n = nullptr, *d_kit = nullptr, *d_kst = nullptr;
+      *d_lim_n = nullptr, *d_kit = nullptr, *d_kst = nullptr, *d_ksw = nullptr;
   double *d_resid = nullptr, *d_kres = nullptr;
   double* d_epart = nullptr;   // epilogue partial sums, YG_EPI_BLOCKS x 4 per member
   int* d_eticket = nullptr;    // epilogue arrival counters
@@ -325,10 +332,10 @@ struct Batch {
     } else {
       YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * C));
     }
-    int* ints[7];
-    for (int q = 0; q < 7; ++q) YG_CHECK(h, ar.alloc(&ints[q], nmem));
+    int* ints[8];
+    for (int q = 0; q < 8; ++q) YG_CHECK(h, ar.alloc(&ints[q], nmem));
     d_done = ints[0]; d_iter_now = ints[1]; d_lin_iter = ints[2]; d_lin_fail = ints[3];
-    d_lim_n = ints[4]; d_kit = ints[5]; d_kst = ints[6];
+    d_lim_n = ints[4]; d_kit = ints[5]; d_kst = ints[6]; d_ksw = ints[7];
     YG_CHECK(h, ar.alloc(&d_resid, nmem));
     YG_CHECK(h, ar.alloc(&d_kres, nmem));
     YG_CHECK(h, ar.alloc(&d_epart, (size_t)nmem * YG_EPI_BLOCKS * 4));
@@ -746,6 +753,7 @@ struct Batch {
     ka.wrapy = (fp.BC == YG_BC_PERIODIC || fp.BC == YG_BC_PERIODIC_Y || fp.BC == YG_BC_MISMIP3D ||
                 fp.BC == YG_BC_TROUGH);
     ka.skip = d_done; ka.out_iters = d_kit; ka.out_status = d_kst; ka.out_resid = d_kres;
+    ka.out_sweeps = d_ksw;
     ka.opts = opts.k;
     ka.group = group; ka.nmembers = nmem; ka.gsync = d_gsync; ka.group_fail = d_gfail;
     if (d_gfail) YG_CHECK(h, hipMemsetAsync(d_gfail, 0, sizeof(int), st));
@@ -756,7 +764,7 @@ struct Batch {
       YG_CHECK(h, hipMemsetAsync(d_tm, 0, sizeof(long long) * 8 * nmem, st));
       ka.timers = d_tm;
     }
-    std::vector<int> done(nmem), kit(nmem);
+    std::vector<int> done(nmem), kit(nmem), ksw(nmem);
     hipEvent_t k0 = nullptr, k1 = nullptr;
     if (h->stats_on) {
       YG_CHECK(h, hipEventCreate(&k0));
@@ -783,8 +791,10 @@ struct Batch {
       enqueue_iteration(G, ps, ka, k0, k1, st);
       YG_CHECK(h, hipGetLastError());
       YG_CHECK(h, hipMemcpyAsync(done.data(), d_done, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
-      if (h->stats_on)
+      if (h->stats_on) {
         YG_CHECK(h, hipMemcpyAsync(kit.data(), d_kit, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
+        YG_CHECK(h, hipMemcpyAsync(ksw.data(), d_ksw, sizeof(int) * nmem, hipMemcpyDeviceToHost, st));
+      }
       YG_CHECK(h, hipStreamSynchronize(st));
       if (h->stats_on) {
         float kms = 0.f;
@@ -800,8 +810,9 @@ struct Batch {
           // bytes per factor value: BF16 L + U cells 16 + 24 B, FP32 L + BF16 U 32 + 24 B
           const double pF = opts.k.factor16 == 1 ? 40.0 / 18.0 : opts.k.factor16 == 2 ? 56.0 / 18.0 : 4.0;
           h->stats.krylov_bytes += mixed ? krylov_bytes(C, it, 4, pF, 4, opts.k.precon == 2) +
-                                               4.0 * (18.0 * C * 8 + 8.0 * C + 6.0 * 2.0 * C * 8)
+                                               refinement_bytes(C, ksw[k])
                                          : krylov_bytes(C, it, 8, 8, 8, opts.k.precon == 2);
+          h->stats.refine_sweeps += mixed ? ksw[k] : 0;
           h->stats.krylov_iters += it;
           h->stats.unknown_iters += 2.0 * C * it;
           h->stats.picard_iters += 1;

@@ -63,6 +63,7 @@ struct YgKrylovArgs {
   int* out_iters;
   int* out_status;
   double* out_resid;
+  int* out_sweeps;          // optional (mixed mode): FP64 residual passes of the refinement
   long long* timers;        // optional: per member 8 phase totals (wall clock ticks)
   YgKrylovOpts opts;
   // member groups (yg_launch_krylov_group): workgroups per member, members,

@@ -1730,7 +1730,9 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     // solver-tolerance error of a plain solve into the Picard iterate.  The
     // reported status still refers to the requested tolerance.
     const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
+    int nres = 0;   // FP64 residual passes
     for (int ir = 0; ir < o.ir_max; ++ir) {
+      ++nres;
       double red[2];
       {
         const Sums2 z = spmv_dots<YG_SPMV_RES, double, double>(c, A64, x64, y64, b64, r64, nullptr);
@@ -1758,7 +1760,10 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
       if (so.status == 2 && so.iters <= 1) { status = 2; break; }
     }
     if (s_bad && status == 0) status = 2;
-    if (threadIdx.x == 0) { a.out_iters[m] = iters; a.out_status[m] = status; a.out_resid[m] = res; }
+    if (threadIdx.x == 0) {
+      a.out_iters[m] = iters; a.out_status[m] = status; a.out_resid[m] = res;
+      if (a.out_sweeps) a.out_sweeps[m] = nres;
+    }
     if (a.timers && threadIdx.x == 0) {
       s_tm[7] = wall_clock64() - t_kernel0;
       for (int k = 0; k < 8; ++k) a.timers[8 * m + k] += s_tm[k];
@@ -1927,8 +1932,10 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     double nrm0 = -1.0, res = 1.0;
     int iters = 0, status = 4;
     const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
+    int nres = 0;   // FP64 residual passes
     for (int ir = 0; ir < o.ir_max; ++ir) {
       grp_barrier(grp);   // x's edge lines are current on every rank
+      ++nres;
       double red[2];
       {
         const Sums2 z = spmv_dots<YG_SPMV_RES, double, double, true>(cm, A64, x64, y64, b64, r64, nullptr);
@@ -1962,6 +1969,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     }
     if (rank == 0 && threadIdx.x == 0) {
       a.out_iters[m] = iters; a.out_status[m] = status; a.out_resid[m] = res;
+      if (a.out_sweeps) a.out_sweeps[m] = nres;
       if (a.timers) {
         s_tm[7] = wall_clock64() - t_kernel0;
         for (int k = 0; k < 8; ++k) a.timers[8 * m + k] += s_tm[k];

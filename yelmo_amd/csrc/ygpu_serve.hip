@@ -878,7 +878,7 @@ int ygpu_server_create(ygpu_handle* h, const ygpu_ssa_params* par, int32_t diva,
   if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return YGPU_ERR_DEVICE; }
   YgServerBase* sb = nullptr;
   int rc;
-  if (par->precision != YGPU_PREC_REF) {
+  if (!fields32(par)) {
     auto* t = new Server<double>();
     sb = t;
     rc = t->create(h, par, diva ? 1 : 0, nlanes, lane_max_side, lane_slots, nz_max, hbm_budget_bytes);

@@ -61,6 +61,7 @@ constexpr int kTunePicard = 10;   // Picard iterations of a timed solve
 // batch's Picard solve (at most kTunePicard iterations) from the same start,
 // per candidate; records the fastest; returns it in *win (status -1: none
 // converged).
+template <typename R>
 int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* prob, int diva,
                int reps, int G, int copies, Candidate* win) {
   ygpu_ssa_params p = *par;
@@ -74,7 +75,7 @@ int tune_group(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_proble
   }
   std::snprintf(p.ssa_lis_opt, sizeof(p.ssa_lis_opt), "%s", opt.c_str());
   std::vector<ygpu_ssa_problem> probs(copies, *prob);
-  Batch<double> B;
+  Batch<R> B;
   int rc = B.setup(h, &p, copies, probs.data(), diva);
   if (rc) return rc;
   if (B.opts.k.precon != 2) {
@@ -142,7 +143,9 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
     if (G > 1 && (!group_ok || cells / G < 4096 || 8 * G > ncu)) continue;
     const int copies = (int)std::min<long long>({(long long)ncu / G, by_mem, 64LL});
     Candidate w{};
-    const int rc = tune_group(h, par, prob, diva, reps, G, std::max(1, copies), &w);
+    const int rc = fields32(par)
+                       ? tune_group<float>(h, par, prob, diva, reps, G, std::max(1, copies), &w)
+                       : tune_group<double>(h, par, prob, diva, reps, G, std::max(1, copies), &w);
     if (rc) return rc;
     if (G == 1) win1 = w;
   }
