@@ -385,6 +385,7 @@ PLANTS = {
     "lim_r": 6, "lim_cross": 7, "conv_vel_tol": 8, "conv_mask": 9, "conv_du_reg": 10,
     "conv_norm2": 11, "l1rel_tol": 12, "relax_vel": 13, "relax_visc": 14, "clip": 15,
     "gl_upstream": 16, "beta_min_mask": 17, "diva_shear": 18, "beta_q": 19, "gq_att_act": 20,
+    "adv_inflow": 21, "adv_diag": 22,
 }
 
 
@@ -542,3 +543,25 @@ def stagger_beta(beta, f_ice, f_grnd, par, dp=True):
     rc = L.oracle_stagger_beta(_p(bx), _p(by), *[_p(a) for a in keep], C.byref(pc), nx, ny)
     assert rc == 0, rc
     return bx, by
+
+
+def assemble_advection(H, H_ref, ux, uy, F, mask, dx, dy, dt, boundaries="zeros", dp=False):
+    """linear_solver_matrix_advection_csr_2D (impl-lis advection, with
+    limit_border_inflow applied to the velocities): 1-based CSR
+    (a_ptr, a_idx, a_val), b and x0 in the scalar row order n = i ny + j."""
+    L = lib(dp)
+    dt_ = np.float64 if dp else np.float32
+    W = _W(dp)
+    nx, ny = H.shape
+    n = nx * ny
+    ap = np.zeros(n + 1, dtype=np.int32)
+    ai = np.zeros(5 * n, dtype=np.int32)
+    av = np.zeros(5 * n)
+    b = np.zeros(n)
+    x0 = np.zeros(n)
+    keep = [_F(H, dt_), _F(H_ref, dt_), _F(ux, dt_), _F(uy, dt_), _F(F, dt_),
+            np.asfortranarray(mask, dtype=np.int32)]
+    nnz = L.oracle_assemble_advection(_p(ap), _p(ai), _p(av), _p(b), _p(x0),
+                                      *[_p(a) for a in keep], W(dx), W(dy), W(dt), nx, ny,
+                                      BND[boundaries])
+    return ap, ai[:nnz].copy(), av[:nnz].copy(), b, x0

@@ -76,7 +76,9 @@ enum {
     PL_BETA_MIN_MASK = 17,   /* beta_min on every free face (mask >= 1)           */
     PL_DIVA_SHEAR = 18,      /* DIVA viscosity without the 1/4 (dv/dz)^2 term     */
     PL_BETA_Q = 19,          /* friction law with |u|^q instead of |u|^(q-1)      */
-    PL_GQ_ATT_ACT = 20       /* rate factor corners over grounded cells only      */
+    PL_GQ_ATT_ACT = 20,      /* rate factor corners over grounded cells only      */
+    PL_ADV_INFLOW = 21,      /* advection: the left face takes the right velocity */
+    PL_ADV_DIAG = 22         /* advection: outflow through the top face dropped    */
 };
 
 /* ------------------------------------------------------------------ */
@@ -159,12 +161,14 @@ typedef struct {
     int initx_zeros;
     int conv_cond;   /* 0: ||r||/||r0||, 1: ||r||/||b||                     */
     int ilu_blocks;  /* ILU(0) over this many contiguous row blocks         */
+    int ilu_nowrap_ny; /* > 0: ILU(0) without the couplings that wrap around a
+                          periodic domain (C-grid rows, ny cells per line)     */
 } lis_opts;
 
 static int parse_lis_opts(const char *s, lis_opts *o)
 {
     o->solver = 4; o->precon = 0; o->maxiter = 1000; o->tol = 1e-12;
-    o->initx_zeros = 1; o->conv_cond = 0; o->ilu_blocks = 1;
+    o->initx_zeros = 1; o->conv_cond = 0; o->ilu_blocks = 1; o->ilu_nowrap_ny = 0;
     if (!s) return 0;
     char buf[512];
     strncpy(buf, s, sizeof(buf) - 1);
@@ -195,6 +199,8 @@ static int parse_lis_opts(const char *s, lis_opts *o)
             o->initx_zeros = (!strcmp(val, "true") || !strcmp(val, "1"));
         } else if (!strcmp(tok, "-conv_cond")) {
             o->conv_cond = atoi(val);
+        } else if (!strcmp(tok, "-ilu_nowrap")) {
+            o->ilu_nowrap_ny = atoi(val);
         } else if (!strcmp(tok, "-ilu_blocks")) {
             o->ilu_blocks = atoi(val);
             if (o->ilu_blocks < 1) o->ilu_blocks = 1;
@@ -1101,7 +1107,7 @@ static int cmp_int2(const void *a, const void *b)
     return x[0] < y[0] ? -1 : x[0] > y[0];
 }
 
-static int ilu0_build(ilu0 *F, const csr0 *A, int nb)
+static int ilu0_build(ilu0 *F, const csr0 *A, int nb, int nowrap_ny)
 {
     int n = A->n;
     F->n = n;
@@ -1128,6 +1134,12 @@ static int ilu0_build(ilu0 *F, const csr0 *A, int nb)
         for (int k = lo; k < hi; k++) {
             int c = A->col[k];
             if (c < bs || c >= be) continue;
+            if (nowrap_ny > 0) {
+                /* a coupling more than one line or column away wraps */
+                const int cr = r / 2, cc = c / 2;
+                const int di = cc / nowrap_ny - cr / nowrap_ny, dj = cc % nowrap_ny - cr % nowrap_ny;
+                if (di > 1 || di < -1 || dj > 1 || dj < -1) continue;
+            }
             tmp[2 * cnt] = c;
             tmp[2 * cnt + 1] = k;
             cnt++;
@@ -1339,7 +1351,7 @@ int oracle_csr_solve(int n, const int32_t *a_ptr1, const int32_t *a_idx1, const 
             P.dinv[r] = 1.0 / d;
         }
     } else if (P.kind == 2) {
-        if (ilu0_build(&P.F, &A, o.ilu_blocks) != 0) { rc = -11; goto out; }
+        if (ilu0_build(&P.F, &A, o.ilu_blocks, o.ilu_nowrap_ny) != 0) { rc = -11; goto out; }
     }
     int it = 0;
     double res = 0.0;
@@ -1366,7 +1378,7 @@ int oracle_ilu0_apply(int n, const int32_t *a_ptr1, const int32_t *a_idx1, const
     for (int q = 0; q < nnz; q++) col[q] = a_idx1[q] - 1;
     csr0 A = {n, ptr, col, a_val};
     ilu0 F;
-    int rc = ilu0_build(&F, &A, nb);
+    int rc = ilu0_build(&F, &A, nb, 0);
     if (rc == 0) { ilu0_apply(&F, v, z); ilu0_free(&F); }
     free(ptr); free(col);
     return rc;
@@ -1385,7 +1397,7 @@ int oracle_ilu0_factors(int n, const int32_t *a_ptr1, const int32_t *a_idx1, con
     for (int q = 0; q < nnz; q++) col[q] = a_idx1[q] - 1;
     csr0 A = {n, ptr, col, a_val};
     ilu0 F;
-    int rc = ilu0_build(&F, &A, nb);
+    int rc = ilu0_build(&F, &A, nb, 0);
     if (rc == 0) {
         memcpy(f_ptr, F.ptr, sizeof(int32_t) * (n + 1));
         memcpy(f_col, F.col, sizeof(int32_t) * F.ptr[n]);
@@ -1686,6 +1698,9 @@ out:
 
 /* the DIVA solve (calc_velocity_diva) and its helpers */
 #include "diva_oracle.inc"
+
+/* the implicit thickness advection system (calc_advec2D, "impl-lis") */
+#include "advection_oracle.inc"
 
 /* size of wp, so the Python driver can check which build it loaded */
 int oracle_wp_bytes(void) { return (int)sizeof(wp); }

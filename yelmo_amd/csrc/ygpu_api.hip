@@ -32,6 +32,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.factor16 = 0; o.k.graph_chunk = 4;
   o.substituted = nullptr;
   o.fields32 = 1;   // the mixed mode's field kernels compute in FP32 unless -ygpu_fields fp64
+  o.pattern = 0;
   if (!s) return 0;
   Tok k, v;
   while (next_tok(s, k)) {
@@ -78,6 +79,12 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
       else { err = "unknown factor storage: " + std::string(v.p, v.n); return YGPU_ERR_ARG; }
     }
     else if (k.is("-ygpu_ny")) o.ny_hint = v.to_int();
+    else if (k.is("-ygpu_pattern")) {
+      if (v.is("auto")) o.pattern = 0;
+      else if (v.is("scalar")) o.pattern = 1;
+      else if (v.is("ssa")) o.pattern = 2;
+      else { err = "unknown pattern: " + std::string(v.p, v.n); return YGPU_ERR_ARG; }
+    }
     else if (k.is("-ygpu_fields")) {
       if (v.is("fp32")) o.fields32 = 1;
       else if (v.is("fp64")) o.fields32 = 0;
@@ -410,14 +417,19 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
                    double* seconds) {
   if (!h) return YGPU_ERR_ARG;
   h->err.clear(); h->note.clear();
-  if (n <= 0 || (n & 1) || !a_ptr1 || !a_idx1 || !a_val || !b || !x_inout || !iters || !status ||
+  if (n <= 0 || !a_ptr1 || !a_idx1 || !a_val || !b || !x_inout || !iters || !status ||
       !rel_resid) {
-    h->err = "invalid arguments (n must be even: two unknowns per cell)";
+    h->err = "invalid arguments";
     return YGPU_ERR_ARG;
   }
   if (a_ptr1[0] != 1 || a_ptr1[n] - 1 != nnz || nnz < 0) {
     h->err = "a_ptr must be 1-based with a_ptr(n+1)-1 == nnz";
     return YGPU_ERR_ARG;
+  }
+  int maxrow = 0;
+  for (int r = 0; r < n; ++r) {
+    if (a_ptr1[r + 1] < a_ptr1[r]) { h->err = "a_ptr must be non-decreasing"; return YGPU_ERR_ARG; }
+    maxrow = std::max(maxrow, a_ptr1[r + 1] - a_ptr1[r]);
   }
   Opts o;
   std::string e;
@@ -432,6 +444,17 @@ int ygpu_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_ptr1
     for (int q = 0; q < n; ++q) x_inout[q] = 0.0;
     *iters = 0; *status = 0; *rel_resid = 0.0;
     return YGPU_OK;
+  }
+  // Scalar five-point systems (the implicit advection of
+  // solver_advection.f90, at most five entries per row) go to their own
+  // solver; the C-grid SSA stencil has rows of up to nine entries over two
+  // unknowns per cell.
+  if (o.pattern == 1 || (o.pattern == 0 && maxrow <= 5))
+    return scalar_csr_solve(h, n, nnz, a_ptr1, a_idx1, a_val, b, x_inout, o, iters, status,
+                            rel_resid, seconds);
+  if (n & 1) {
+    h->err = "n must be even for the C-grid SSA stencil (two unknowns per cell)";
+    return YGPU_ERR_ARG;
   }
   const int cells = n / 2;
   int nx = 0, ny = 0;
