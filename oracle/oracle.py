@@ -357,8 +357,10 @@ def visc_eff_int(visc, H_ice, f_ice, zeta, dp=True):
     return out
 
 
-def calc_beta(prob, par, ux_b, uy_b, dp=True):
-    """calc_beta then stagger_beta on the oracle; returns (beta, beta_acx, beta_acy)."""
+def calc_beta(prob, par, ux_b, uy_b, dp=True, u_stag=None):
+    """calc_beta then stagger_beta on the oracle; returns (beta, beta_acx,
+    beta_acy).  u_stag: the (ux, uy) pair stagger_beta reads (ux_b, uy_b by
+    default, as in velocity_ssa.f90:247; the depth average for DIVA)."""
     L = lib(dp)
     dt = np.float64 if dp else np.float32
     nx, ny = prob.nx, prob.ny
@@ -368,11 +370,15 @@ def calc_beta(prob, par, ux_b, uy_b, dp=True):
     bx = np.zeros((nx, ny), dtype=dt, order="F")
     by = np.zeros((nx, ny), dtype=dt, order="F")
     pc = params_c(par)
+    z = np.zeros((nx, ny))
+    g = lambda k: F(fl.get(k, z))   # fields only beta_gl_scale 2 / beta_gl_stag 3, 4 read
     keep = [F(fl["c_bed"]), F(fl["f_slide"]), F(ux_b), F(uy_b), F(fl["H_ice"]), F(fl["f_ice"]),
-            F(fl["H_grnd"]), F(fl["f_grnd"])]
+            F(fl["H_grnd"]), F(fl["f_grnd"]), g("z_bed"), g("z_sl")]
     rc = L.oracle_calc_beta(_p(beta), *[_p(a) for a in keep], C.byref(pc), nx, ny)
     assert rc == 0, rc
-    rc = L.oracle_stagger_beta(_p(bx), _p(by), _p(beta), _p(keep[5]), _p(keep[7]), C.byref(pc),
+    us = (keep[2], keep[3]) if u_stag is None else (F(u_stag[0]), F(u_stag[1]))
+    st = [keep[4], keep[5], us[0], us[1], keep[7], g("f_grnd_acx"), g("f_grnd_acy")]
+    rc = L.oracle_stagger_beta(_p(bx), _p(by), _p(beta), *[_p(a) for a in st], C.byref(pc),
                                nx, ny)
     assert rc == 0, rc
     return beta, bx, by
@@ -385,7 +391,8 @@ PLANTS = {
     "lim_r": 6, "lim_cross": 7, "conv_vel_tol": 8, "conv_mask": 9, "conv_du_reg": 10,
     "conv_norm2": 11, "l1rel_tol": 12, "relax_vel": 13, "relax_visc": 14, "clip": 15,
     "gl_upstream": 16, "beta_min_mask": 17, "diva_shear": 18, "beta_q": 19, "gq_att_act": 20,
-    "adv_inflow": 21, "adv_diag": 22,
+    "adv_inflow": 21, "adv_diag": 22, "zstar_norm": 23, "zstar_rho": 24, "subgrid_wt": 25,
+    "flux_side": 26,
 }
 
 
@@ -531,18 +538,35 @@ def set_beta_min_grounded(bx, by, mask_acx, mask_acy, beta_min, dp=True):
     return a, b
 
 
-def stagger_beta(beta, f_ice, f_grnd, par, dp=True):
-    """stagger_beta on given aa-node friction: (beta_acx, beta_acy)."""
+def stagger_beta(beta, f_ice, f_grnd, par, dp=True, H_ice=None, ux=None, uy=None,
+                 f_grnd_acx=None, f_grnd_acy=None):
+    """stagger_beta on given aa-node friction: (beta_acx, beta_acy).  H_ice,
+    ux, uy and f_grnd_acx/acy are read by beta_gl_stag 3 and 4 only (zeros
+    when not given)."""
     L = lib(dp)
     dt = np.float64 if dp else np.float32
     nx, ny = beta.shape
     bx = np.zeros((nx, ny), dtype=dt, order="F")
     by = np.zeros((nx, ny), dtype=dt, order="F")
-    keep = [_F(beta, dt), _F(f_ice, dt), _F(f_grnd, dt)]
+    z = np.zeros((nx, ny))
+    opt = lambda a: _F(z if a is None else a, dt)
+    keep = [_F(beta, dt), opt(H_ice), _F(f_ice, dt), opt(ux), opt(uy), _F(f_grnd, dt),
+            opt(f_grnd_acx), opt(f_grnd_acy)]
     pc = params_c(par)
     rc = L.oracle_stagger_beta(_p(bx), _p(by), *[_p(a) for a in keep], C.byref(pc), nx, ny)
     assert rc == 0, rc
     return bx, by
+
+
+def beta_gl_flux_weight(beta_a, u_a, u_b, H_a, H_b, f_grnd_ac, dp=True):
+    """calc_beta_gl_flux_weight (basal_dragging.f90:1760-1845): beta_ac for
+    grounded node a, floating node b."""
+    L = lib(dp)
+    W = _W(dp)
+    fn = L.oracle_beta_gl_flux_weight
+    fn.restype = W
+    fn.argtypes = [W] * 6
+    return float(fn(beta_a, u_a, u_b, H_a, H_b, f_grnd_ac))
 
 
 def assemble_advection(H, H_ref, ux, uy, F, mask, dx, dy, dt, boundaries="zeros", dp=False):

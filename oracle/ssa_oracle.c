@@ -78,7 +78,11 @@ enum {
     PL_BETA_Q = 19,          /* friction law with |u|^q instead of |u|^(q-1)      */
     PL_GQ_ATT_ACT = 20,      /* rate factor corners over grounded cells only      */
     PL_ADV_INFLOW = 21,      /* advection: the left face takes the right velocity */
-    PL_ADV_DIAG = 22         /* advection: outflow through the top face dropped    */
+    PL_ADV_DIAG = 22,        /* advection: outflow through the top face dropped    */
+    PL_ZSTAR_NORM = 23,      /* zstar scaling not normalised by H_eff              */
+    PL_ZSTAR_RHO = 24,       /* zstar flotation depth with rho_ice/rho_sw          */
+    PL_SUBGRID_WT = 25,      /* subgrid staggering weight f_grnd_ac, not squared   */
+    PL_FLUX_SIDE = 26        /* flux weight counts segments with lambda < f_grnd   */
 };
 
 /* ------------------------------------------------------------------ */
@@ -140,8 +144,8 @@ typedef struct {
     int32_t boundaries;      /* OB_*                                       */
     int32_t visc_method;     /* 0, 1, 2                                    */
     int32_t beta_method;     /* -1 .. 5                                    */
-    int32_t beta_gl_scale;   /* 0, 1, 3                                    */
-    int32_t beta_gl_stag;    /* -1, 0, 1, 2                                */
+    int32_t beta_gl_scale;   /* 0 .. 3                                     */
+    int32_t beta_gl_stag;    /* -1 .. 4                                    */
     int32_t vel_lim_method;  /* 0 clip, 1 drag                             */
     int32_t ssa_iter_max;
     int32_t ssa_write_log;
@@ -597,9 +601,9 @@ static void beta_aa_law(wp *beta, const wp *ux_b, const wp *uy_b, const wp *c_be
 /* calc_beta (basal_dragging.f90:384-579) */
 int oracle_calc_beta(wp *beta, const wp *c_bed, const wp *f_slide, const wp *ux_b,
                      const wp *uy_b, const wp *H_ice, const wp *f_ice, const wp *H_grnd,
-                     const wp *f_grnd, const oracle_ssa_params *par, int nx, int ny)
+                     const wp *f_grnd, const wp *z_bed, const wp *z_sl,
+                     const oracle_ssa_params *par, int nx, int ny)
 {
-    (void)H_ice;
     int BC = boundary_code(par->boundaries);
     int nc = nx * ny;
     wp q = (wp)par->beta_q, u0 = (wp)par->beta_u0;
@@ -643,6 +647,25 @@ int oracle_calc_beta(wp *beta, const wp *c_bed, const wp *f_slide, const wp *ux_
         }
         break;
     }
+    case 2: { /* scale_beta_gl_zstar with norm (basal_dragging.f90:1296-1352);
+               * calc_H_eff with set_frac_zero (topography.f90:851-874)     */
+        const wp rho_sw_ice = PLANT(PL_ZSTAR_RHO) ? (wp)par->rho_ice / (wp)par->rho_sw
+                                                  : (wp)par->rho_sw / (wp)par->rho_ice;
+        for (int c = 0; c < nc; c++) {
+            if (!(f_ice[c] > R(0))) continue;
+            wp H_eff = H_ice[c] / f_ice[c];
+            if (f_ice[c] < R(1)) H_eff = R(0);
+            wp f_scale;
+            if (z_bed[c] > z_sl[c]) f_scale = H_eff;
+            else {
+                f_scale = H_eff - (z_sl[c] - z_bed[c]) * rho_sw_ice;
+                if (f_scale < R(0)) f_scale = R(0);
+            }
+            if (H_eff > R(0) && !PLANT(PL_ZSTAR_NORM)) f_scale = f_scale / H_eff;
+            beta[c] = beta[c] * f_scale;
+        }
+        break;
+    }
     case 3:
         for (int c = 0; c < nc; c++) beta[c] = f_grnd[c] * beta[c];
         break;
@@ -662,10 +685,35 @@ int oracle_calc_beta(wp *beta, const wp *c_bed, const wp *f_slide, const wp *ux_
     return 0;
 }
 
+/* calc_beta_gl_flux_weight (basal_dragging.f90:1760-1845): node a grounded,
+ * node b floating, 100 segments with thickness and flux linear in between */
+wp oracle_beta_gl_flux_weight(wp beta_a, wp u_a, wp u_b, wp H_a, wp H_b, wp f_grnd_ac)
+{
+    const wp q_a = u_a * H_a, q_b = u_b * H_b;
+    wp uu_grnd = R(0), uu_tot = R(0);
+    for (int k = 1; k <= 100; k++) {
+        const wp lambda = ((wp)k - R(0.5)) / (wp)100;
+        const wp H_now = H_a * lambda + H_b * (R(1) - lambda);
+        const wp q_now = q_a * lambda + q_b * (R(1) - lambda);
+        const wp u_now = H_now > R(0) ? q_now / H_now : R(0);
+        const wp au = u_now < R(0) ? -u_now : u_now;
+        uu_tot += au;
+        const int grounded = PLANT(PL_FLUX_SIDE) ? lambda < f_grnd_ac : lambda > R(1) - f_grnd_ac;
+        if (grounded) uu_grnd += au;
+    }
+    const wp weight = uu_tot > R(0) ? uu_grnd / uu_tot : R(0);
+    return beta_a * weight;
+}
+
 /* stagger_beta (basal_dragging.f90:581-704) with stagger_beta_aa_mean
- * (:1360-1437) and stagger_beta_aa_gl_upstream / _downstream (:1439-1560)  */
-int oracle_stagger_beta(wp *beta_acx, wp *beta_acy, const wp *beta, const wp *f_ice,
-                        const wp *f_grnd, const oracle_ssa_params *par, int nx, int ny)
+ * (:1360-1437), stagger_beta_aa_gl_upstream / _downstream (:1439-1576),
+ * stagger_beta_aa_gl_subgrid (:1578-1658) and _subgrid_flux (:1660-1758).
+ * ux/uy: the velocity the caller staggers with (ux_b for SSA, ux_bar for
+ * DIVA; velocity_ssa.f90:247, velocity_diva.f90:271).                      */
+int oracle_stagger_beta(wp *beta_acx, wp *beta_acy, const wp *beta, const wp *H_ice,
+                        const wp *f_ice, const wp *ux, const wp *uy, const wp *f_grnd,
+                        const wp *f_grnd_acx, const wp *f_grnd_acy,
+                        const oracle_ssa_params *par, int nx, int ny)
 {
     int BC = boundary_code(par->boundaries);
     int stag = par->beta_gl_stag;
@@ -700,6 +748,54 @@ int oracle_stagger_beta(wp *beta_acx, wp *beta_acy, const wp *beta, const wp *f_
                             beta_acy[c] = up ? beta[c] : beta[IX(i, jp1)];
                         else if (f_grnd[c] == R(0) && f_grnd[IX(i, jp1)] > R(0))
                             beta_acy[c] = up ? beta[IX(i, jp1)] : beta[c];
+                    }
+                }
+        } else if (stag == 3) {
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    int im1, ip1, jm1, jp1, c = IX(i, j);
+                    nbr(i, j, nx, ny, BC, &im1, &ip1, &jm1, &jp1);
+                    const int cx = IX(ip1, j), cy = IX(i, jp1);
+                    if (f_ice[c] == R(1) && f_ice[cx] == R(1)) {
+                        const wp wt = PLANT(PL_SUBGRID_WT) ? f_grnd_acx[c] : f_grnd_acx[c] * f_grnd_acx[c];
+                        if (f_grnd[c] > R(0) && f_grnd[cx] == R(0))
+                            beta_acx[c] = wt * beta[c] + (R(1) - wt) * beta[cx];
+                        else if (f_grnd[c] == R(0) && f_grnd[cx] > R(0))
+                            beta_acx[c] = (R(1) - wt) * beta[c] + wt * beta[cx];
+                    }
+                    if (f_ice[c] == R(1) && f_ice[cy] == R(1)) {
+                        const wp wt = PLANT(PL_SUBGRID_WT) ? f_grnd_acy[c] : f_grnd_acy[c] * f_grnd_acy[c];
+                        if (f_grnd[c] > R(0) && f_grnd[cy] == R(0))
+                            beta_acy[c] = wt * beta[c] + (R(1) - wt) * beta[cy];
+                        else if (f_grnd[c] == R(0) && f_grnd[cy] > R(0))
+                            beta_acy[c] = (R(1) - wt) * beta[c] + wt * beta[cy];
+                    }
+                }
+        } else if (stag == 4) {
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    int im1, ip1, jm1, jp1, c = IX(i, j);
+                    nbr(i, j, nx, ny, BC, &im1, &ip1, &jm1, &jp1);
+                    const int cx = IX(ip1, j), cy = IX(i, jp1);
+                    if (f_ice[c] == R(1) && f_ice[cx] == R(1)) {
+                        if (f_grnd[c] > R(0) && f_grnd[cx] == R(0))
+                            beta_acx[c] = oracle_beta_gl_flux_weight(
+                                beta[c], R(0.5) * (ux[IX(im1, j)] + ux[c]), R(0.5) * (ux[cx] + ux[c]),
+                                H_ice[c], H_ice[cx], f_grnd_acx[c]);
+                        else if (f_grnd[c] == R(0) && f_grnd[cx] > R(0))
+                            beta_acx[c] = oracle_beta_gl_flux_weight(
+                                beta[cx], R(0.5) * (ux[cx] + ux[c]), R(0.5) * (ux[IX(im1, j)] + ux[c]),
+                                H_ice[cx], H_ice[c], f_grnd_acx[c]);
+                    }
+                    if (f_ice[c] == R(1) && f_ice[cy] == R(1)) {
+                        if (f_grnd[c] > R(0) && f_grnd[cy] == R(0))
+                            beta_acy[c] = oracle_beta_gl_flux_weight(
+                                beta[c], R(0.5) * (uy[IX(i, jm1)] + uy[c]), R(0.5) * (uy[cy] + uy[c]),
+                                H_ice[c], H_ice[cy], f_grnd_acy[c]);
+                        else if (f_grnd[c] == R(0) && f_grnd[cy] > R(0))
+                            beta_acy[c] = oracle_beta_gl_flux_weight(
+                                beta[cy], R(0.5) * (uy[cy] + uy[c]), R(0.5) * (uy[IX(i, jm1)] + uy[c]),
+                                H_ice[cy], H_ice[c], f_grnd_acy[c]);
                     }
                 }
         } else if (stag != 0) {
@@ -1572,7 +1668,7 @@ int oracle_calc_velocity_ssa(
     const wp *z_srf, wp dx, wp dy, wp n_glen, const oracle_ssa_params *par, const char *lis_opt,
     int nx, int ny, int nz, double *resid_hist, int32_t *lin_hist, double *solve_seconds)
 {
-    (void)f_grnd_acx; (void)f_grnd_acy; (void)z_srf;
+    (void)z_srf;
     int nc = nx * ny;
     int BC = boundary_code(par->boundaries);
     size_t n3 = (size_t)nc * nz;
@@ -1617,9 +1713,10 @@ int oracle_calc_velocity_ssa(
         default: rc = -2; goto out;
         }
         oracle_visc_eff_int(visc_eff_int, visc_eff, H_ice, f_ice, zeta_aa, nx, ny, nz);
-        if (oracle_calc_beta(beta, c_bed, f_slide, ux_b, uy_b, H_ice, f_ice, H_grnd, f_grnd, par,
-                             nx, ny) != 0) { rc = -3; goto out; }
-        if (oracle_stagger_beta(beta_acx, beta_acy, beta, f_ice, f_grnd, par, nx, ny) != 0) {
+        if (oracle_calc_beta(beta, c_bed, f_slide, ux_b, uy_b, H_ice, f_ice, H_grnd, f_grnd, z_bed,
+                             z_sl, par, nx, ny) != 0) { rc = -3; goto out; }
+        if (oracle_stagger_beta(beta_acx, beta_acy, beta, H_ice, f_ice, ux_b, uy_b, f_grnd,
+                                f_grnd_acx, f_grnd_acy, par, nx, ny) != 0) {
             rc = -4; goto out;
         }
         memcpy(bsx, beta_acx, sizeof(wp) * nc);

@@ -404,3 +404,119 @@ def test_diva_viscosity_closed_form_with_vertical_shear():
     assert _diva_visc_case() <= 1e-13
     with oracle.planted("diva_shear"):
         assert _diva_visc_case() > 1e-3
+
+
+# ---- zstar scaling and the subgrid grounding-line staggering ------------------------
+
+def _zstar_beta(H, z_bed, f_ice, z_sl=0.0, beta_const=1000.0):
+    """beta_method 0 (beta = beta_const) scaled with beta_gl_scale 2 on one
+    row of grounded cells."""
+    n = len(H)
+    F = lambda a: np.asfortranarray(np.tile(np.asarray(a, dtype=float)[:, None], (1, 3)))
+    ones = np.ones(n)
+    prob = SimpleNamespace(nx=n, ny=3, fields=dict(
+        c_bed=F(ones), f_slide=F(ones), H_ice=F(H), f_ice=F(f_ice), H_grnd=F(H), f_grnd=F(ones),
+        z_bed=F(z_bed), z_sl=F(np.full(n, z_sl))))
+    par = SsaParams(beta_method=0, beta_const=beta_const, beta_gl_scale=2, beta_min=0.0,
+                    beta_gl_stag=0, boundaries="zeros")
+    beta, _, _ = oracle.calc_beta(prob, par, F(ones), F(ones))
+    return beta[:, 1] / beta_const
+
+
+def test_zstar_scaling_is_the_floating_fraction():
+    """beta_gl_scale 2 (scale_beta_gl_zstar, basal_dragging.f90:1296-1352,
+    Gladstone et al. 2017): beta scales with the thickness above flotation
+    over the thickness, H* / H, H* = H - (z_sl - z_bed) rho_sw / rho_ice for
+    a marine bed.  Fixed points: a cell exactly at flotation -> 0; twice the
+    flotation thickness -> 1/2; below flotation -> 0; land -> 1 whatever H;
+    a partly covered cell (H_eff set to zero) -> 0; an ice-free cell keeps
+    beta."""
+    d = 300.0
+    H_flot = d * 1028.0 / 910.0
+    s = _zstar_beta(H=[H_flot, 2 * H_flot, 0.5 * H_flot, 700.0, 800.0, 0.0, 4 * H_flot],
+                    z_bed=[-d, -d, -d, 50.0, -d, -d, -d],
+                    f_ice=[1, 1, 1, 1, 0.6, 0, 1])
+    assert s[0] == pytest.approx(0.0, abs=1e-12)
+    assert s[1] == pytest.approx(0.5, rel=1e-12)
+    assert s[2] == 0.0
+    assert s[3] == 1.0
+    assert s[4] == 0.0
+    assert s[5] == 1.0
+    assert s[6] == pytest.approx(0.75, rel=1e-12)
+    with oracle.planted("zstar_norm"):
+        assert _zstar_beta([2 * H_flot], [-d], [1])[0] != pytest.approx(0.5, rel=1e-3)
+    with oracle.planted("zstar_rho"):
+        assert _zstar_beta([H_flot], [-d], [1])[0] > 0.1
+
+
+def _gl_case_acx(stag, f_acx, ux=None, H=None):
+    fg = np.array([1, 1, 0, 0, 1, 0, 1, 1], dtype=float)
+    fi = np.array([1, 1, 1, 1, 1, 1, 1, 0], dtype=float)
+    beta = np.array([500.0, 700.0, 900.0, 1100.0, 1300.0, 1500.0, 1700.0, 1900.0])
+    F = lambda a: np.asfortranarray(np.tile(np.asarray(a, dtype=float)[:, None], (1, 4)))
+    ux = np.full(8, 100.0) if ux is None else ux
+    H = np.full(8, 1000.0) if H is None else H
+    par = SsaParams(beta_gl_stag=stag, beta_min=100.0, boundaries="zeros")
+    bx, _ = oracle.stagger_beta(F(beta), F(fi), F(fg), par, H_ice=F(H), ux=F(ux),
+                                uy=F(np.zeros(8)), f_grnd_acx=F(f_acx), f_grnd_acy=F(np.zeros(8)))
+    return bx[:, 1], beta
+
+
+def test_stagger_beta_gl_subgrid_weight():
+    """beta_gl_stag 3 (stagger_beta_aa_gl_subgrid, basal_dragging.f90:
+    1578-1658): at a grounded|floating face the grounded side weighs
+    f_grnd_ac^2: f = 1 gives the grounded cell's beta, f = 0 the floating
+    one's, f = 0.5 a quarter/three-quarter mix, mirrored when the floating
+    cell is on the left; other faces keep the mean staggering."""
+    f = np.array([0.9, 1.0, 0.3, 0.5, 0.0, 0.5, 0.2, 0.2])
+    bx, beta = _gl_case_acx(3, f)
+    mean, _ = _gl_case_acx(0, f)
+    assert bx[0] == mean[0] and bx[2] == mean[2] and bx[6] == mean[6]
+    assert bx[1] == pytest.approx(beta[1], rel=1e-14)                          # f = 1
+    assert bx[3] == pytest.approx(0.75 * beta[3] + 0.25 * beta[4], rel=1e-14)  # mirrored, f = .5
+    assert bx[4] == pytest.approx(beta[5], rel=1e-14)                          # f = 0
+    assert bx[5] == pytest.approx(0.75 * beta[5] + 0.25 * beta[6], rel=1e-14)
+    with oracle.planted("subgrid_wt"):
+        assert _gl_case_acx(3, f)[0][3] != pytest.approx(bx[3], rel=1e-6)
+
+
+def test_flux_weight_closed_forms():
+    """calc_beta_gl_flux_weight (basal_dragging.f90:1760-1845): beta_a times
+    the grounded share of the summed segment speeds of 100 midpoint segments,
+    the grounded part adjoining node a.  With uniform speed and thickness the
+    share is f itself; with the flux linear from U at a to 0 at b (equal
+    thickness) it is the integral of lambda over [1 - f, 1] over that over
+    [0, 1], which the midpoint rule gives exactly: 3/4 at f = 1/2; with
+    constant flux and H from 1 at a to 3 at b it tends to
+    ln(1 + 2 f) / ln 3; reversed flow and a zero velocity give |u| weights and
+    0."""
+    w = oracle.beta_gl_flux_weight
+    assert w(800.0, 50.0, 50.0, 1000.0, 1000.0, 0.37) == pytest.approx(800.0 * 0.37, rel=1e-12)
+    assert w(800.0, 120.0, 0.0, 500.0, 500.0, 0.5) == pytest.approx(600.0, rel=1e-12)
+    assert w(800.0, -120.0, 0.0, 500.0, 500.0, 0.5) == pytest.approx(600.0, rel=1e-12)
+    f = 0.5
+    assert w(1.0, 3.0, 1.0, 1.0, 3.0, f) == pytest.approx(np.log(1 + 2 * f) / np.log(3.0), abs=1e-4)
+    assert w(800.0, 0.0, 0.0, 500.0, 500.0, 0.5) == 0.0
+    with oracle.planted("flux_side"):
+        assert w(800.0, 120.0, 0.0, 500.0, 500.0, 0.5) == pytest.approx(200.0, rel=1e-12)
+
+
+def test_stagger_beta_gl_subgrid_flux_faces():
+    """beta_gl_stag 4 (stagger_beta_aa_gl_subgrid_flux, basal_dragging.f90:
+    1660-1758): under uniform velocity and thickness a grounded|floating face
+    takes beta of the grounded cell times f_grnd_ac, whichever side the
+    grounded cell is on; the floating cell's beta does not enter."""
+    f = np.array([0.9, 0.37, 0.3, 0.61, 0.25, 0.83, 0.2, 0.2])
+    bx, beta = _gl_case_acx(4, f)
+    mean, _ = _gl_case_acx(0, f)
+    assert bx[0] == mean[0] and bx[2] == mean[2] and bx[6] == mean[6]
+    assert bx[1] == pytest.approx(beta[1] * 0.37, rel=1e-12)
+    assert bx[3] == pytest.approx(beta[4] * 0.61, rel=1e-12)
+    assert bx[4] == pytest.approx(beta[4] * 0.25, rel=1e-12)
+    assert bx[5] == pytest.approx(beta[6] * 0.83, rel=1e-12)
+    # the velocities at the two nodes are the means of the faces around them
+    ux = np.array([0.0, 0.0, 40.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    bx2, _ = _gl_case_acx(4, np.full(8, 0.5), ux=ux)
+    # face 1: node a (cell 1) has u = (0 + 0)/2 = 0, node b (cell 2) u = (40 + 0)/2
+    # -> speed linear from 0 at lambda=1 to 20 at lambda=0: grounded share 1/4
+    assert bx2[1] == pytest.approx(beta[1] * 0.25, rel=1e-12)

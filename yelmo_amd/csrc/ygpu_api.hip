@@ -766,7 +766,7 @@ int64_t ygpu_footprint_bytes(const ygpu_ssa_params* par, int32_t nx, int32_t ny,
   else b += C * YG_NF * 8;
   // 2D fields, ATT and the viscosity (3D), ATT^(-1/n) at the quadrature nodes
   // (visc_method 1), and 2 3D fields of margin
-  b += rb * (C * 34 + Z * 2 + (par->visc_method == 1 ? 4 * Z : 0) + 2 * Z);
+  b += rb * (C * 36 + Z * 2 + (par->visc_method == 1 ? 4 * Z : 0) + 2 * Z);
   if (par->ssa_solver == 1) b += C * 2 * 4;   // copy_from of the energy form
   return b;
 }

@@ -32,7 +32,7 @@ struct YgFieldParams {
   int no_slip;         // DIVA no-slip variant
   int energy;          // 1: energy-formulation assembly (ssa_solver "energy")
   double visc_const, beta_const, beta_q, beta_u0, beta_gl_f, H_grnd_lim, beta_min, eps_0,
-      vel_lim_tau, vel_max, iter_rel, iter_conv, n_glen;
+      vel_lim_tau, vel_max, iter_rel, iter_conv, n_glen, rho_ice, rho_sw;
 };
 
 // Device arrays of one batch, natural (Fortran) cell order, type R.
@@ -41,6 +41,7 @@ struct YgFields {
   // inputs (2D unless noted)
   const R *c_bed, *f_slide, *taud_acx, *taud_acy, *taul_acx, *taul_acy, *H_ice, *f_ice, *H_grnd,
       *f_grnd, *z_bed, *z_sl;
+  const R *f_grnd_acx, *f_grnd_acy;   // grounded fraction on the faces (subgrid staggering)
   const R* ATT;      // 3D, per member at zoff
   // ATT^(-1/n) at the 4 quadrature nodes of every cell and level (node q,
   // level k at (q nz + k) ncell + cell, per member at 4 zoff), filled once

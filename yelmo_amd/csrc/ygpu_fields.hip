@@ -496,6 +496,26 @@ YG_DEV R beta_point(const YgFields<R>& f, const YgFieldParams& p, const CellCtx&
     R fs = (hg < lim ? hg : lim) / lim;
     if (fs < R(0)) fs = R(0);
     beta = beta * fs;
+  } else if (p.beta_gl_scale == 2) {
+    // scale_beta_gl_zstar with norm (basal_dragging.f90:1296-1352): the
+    // thickness above flotation over the effective thickness (calc_H_eff
+    // with set_frac_zero, topography.f90:851-874)
+    const R fic = fi[c];
+    if (fic > R(0)) {
+      const R H = f.H_ice[cc.c0 + c];
+      R H_eff = H / fic;
+      if (fic < R(1)) H_eff = R(0);
+      const R zb = f.z_bed[cc.c0 + c], zs = f.z_sl[cc.c0 + c];
+      const R rho_sw_ice = (R)p.rho_sw / (R)p.rho_ice;
+      R fs;
+      if (zb > zs) fs = H_eff;
+      else {
+        fs = H_eff - (zs - zb) * rho_sw_ice;
+        if (fs < R(0)) fs = R(0);
+      }
+      if (H_eff > R(0)) fs = fs / H_eff;
+      beta = beta * fs;
+    }
   } else if (p.beta_gl_scale == 3) {
     beta = fg[c] * beta;
   }
@@ -527,6 +547,25 @@ __global__ __launch_bounds__(256) void k_beta(YgBatchGeom G, YgFields<R> f, YgFi
 }
 
 // ---- staggering, beta_min, matrix friction and limit drag -----------------
+// calc_beta_gl_flux_weight (basal_dragging.f90:1760-1845): node a grounded,
+// node b floating; the grounded share of the summed segment speeds of 100
+// segments, thickness and flux linear between the nodes
+template <typename R>
+YG_DEV R beta_gl_flux_weight(R beta_a, R u_a, R u_b, R H_a, R H_b, R f_grnd_ac) {
+  const R q_a = u_a * H_a, q_b = u_b * H_b;
+  R uu_grnd = R(0), uu_tot = R(0);
+  for (int k = 1; k <= 100; ++k) {
+    const R lambda = ((R)k - R(0.5)) / (R)100;
+    const R H_now = H_a * lambda + H_b * (R(1) - lambda);
+    const R q_now = q_a * lambda + q_b * (R(1) - lambda);
+    const R u_now = H_now > R(0) ? q_now / H_now : R(0);
+    uu_tot = uu_tot + yabs<R>(u_now);
+    if (lambda > R(1) - f_grnd_ac) uu_grnd = uu_grnd + yabs<R>(u_now);
+  }
+  const R weight = uu_tot > R(0) ? uu_grnd / uu_tot : R(0);
+  return beta_a * weight;
+}
+
 // stagger_beta (basal_dragging.f90:581-704), set_beta_min_grounded (:706-780),
 // calc_vel_lim_drag (solver_ssa_ac.f90:1095-1199), DIVA calc_beta_eff
 // (velocity_diva.f90:912-958).
@@ -557,6 +596,43 @@ YG_DEV void stag_point(const YgFields<R>& f, const YgFieldParams& p, const CellC
     if (fi[c] == R(1) && fi[cy] == R(1)) {
       if (fg[c] > R(0) && fg[cy] == R(0)) by = st == 1 ? be[c] : be[cy];
       else if (fg[c] == R(0) && fg[cy] > R(0)) by = st == 1 ? be[cy] : be[c];
+    }
+  } else if (st == 3) {
+    // stagger_beta_aa_gl_subgrid (basal_dragging.f90:1578-1658): weight
+    // f_grnd_ac^2 on the grounded side
+    if (fi[c] == R(1) && fi[cx] == R(1)) {
+      const R g = f.f_grnd_acx[cc.c0 + c], wt = g * g;
+      if (fg[c] > R(0) && fg[cx] == R(0)) bx = wt * be[c] + (R(1) - wt) * be[cx];
+      else if (fg[c] == R(0) && fg[cx] > R(0)) bx = (R(1) - wt) * be[c] + wt * be[cx];
+    }
+    if (fi[c] == R(1) && fi[cy] == R(1)) {
+      const R g = f.f_grnd_acy[cc.c0 + c], wt = g * g;
+      if (fg[c] > R(0) && fg[cy] == R(0)) by = wt * be[c] + (R(1) - wt) * be[cy];
+      else if (fg[c] == R(0) && fg[cy] > R(0)) by = (R(1) - wt) * be[c] + wt * be[cy];
+    }
+  } else if (st == 4) {
+    // stagger_beta_aa_gl_subgrid_flux (basal_dragging.f90:1660-1758) with the
+    // iterate's velocity (the basal one for SSA, the depth average for DIVA)
+    const R* u = (p.diva ? f.ubar_x : f.ux) + cc.c0;
+    const R* v = (p.diva ? f.ubar_y : f.uy) + cc.c0;
+    const R* H = f.H_ice + cc.c0;
+    if (fi[c] == R(1) && fi[cx] == R(1)) {
+      const R g = f.f_grnd_acx[cc.c0 + c];
+      if (fg[c] > R(0) && fg[cx] == R(0))
+        bx = beta_gl_flux_weight<R>(be[c], R(0.5) * (u[I2(im1, j)] + u[c]),
+                                    R(0.5) * (u[I2(ip1, j)] + u[c]), H[c], H[cx], g);
+      else if (fg[c] == R(0) && fg[cx] > R(0))
+        bx = beta_gl_flux_weight<R>(be[cx], R(0.5) * (u[I2(ip1, j)] + u[c]),
+                                    R(0.5) * (u[I2(im1, j)] + u[c]), H[cx], H[c], g);
+    }
+    if (fi[c] == R(1) && fi[cy] == R(1)) {
+      const R g = f.f_grnd_acy[cc.c0 + c];
+      if (fg[c] > R(0) && fg[cy] == R(0))
+        by = beta_gl_flux_weight<R>(be[c], R(0.5) * (v[I2(i, jm1)] + v[c]),
+                                    R(0.5) * (v[I2(i, jp1)] + v[c]), H[c], H[cy], g);
+      else if (fg[c] == R(0) && fg[cy] > R(0))
+        by = beta_gl_flux_weight<R>(be[cy], R(0.5) * (v[I2(i, jp1)] + v[c]),
+                                    R(0.5) * (v[I2(i, jm1)] + v[c]), H[cy], H[c], g);
     }
   }
 }

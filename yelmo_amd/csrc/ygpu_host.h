@@ -172,9 +172,10 @@ struct Batch {
     fp.beta_u0 = par->beta_u0; fp.beta_gl_f = par->beta_gl_f; fp.H_grnd_lim = par->H_grnd_lim;
     fp.beta_min = par->beta_min; fp.eps_0 = par->eps_0; fp.vel_lim_tau = par->ssa_vel_lim_tau;
     fp.vel_max = par->ssa_vel_max; fp.iter_rel = par->ssa_iter_rel; fp.iter_conv = par->ssa_iter_conv;
+    fp.rho_ice = par->rho_ice; fp.rho_sw = par->rho_sw;
     if (par->visc_method < 0 || par->visc_method > 2 || par->beta_method < -1 || par->beta_method > 5 ||
-        !(par->beta_gl_scale == 0 || par->beta_gl_scale == 1 || par->beta_gl_scale == 3) ||
-        par->beta_gl_stag < -1 || par->beta_gl_stag > 2) {
+        par->beta_gl_scale < 0 || par->beta_gl_scale > 3 ||
+        par->beta_gl_stag < -1 || par->beta_gl_stag > 4) {
       h->err = "parameter value not supported";
       return YGPU_ERR_UNSUPPORTED;
     }
@@ -357,7 +358,8 @@ struct Batch {
     // fields
     R** r2d[] = {(R**)&f.c_bed, (R**)&f.f_slide, (R**)&f.taud_acx, (R**)&f.taud_acy,
                  (R**)&f.taul_acx, (R**)&f.taul_acy, (R**)&f.H_ice, (R**)&f.f_ice,
-                 (R**)&f.H_grnd, (R**)&f.f_grnd, (R**)&f.z_bed, (R**)&f.z_sl, &f.ux, &f.uy,
+                 (R**)&f.H_grnd, (R**)&f.f_grnd, (R**)&f.z_bed, (R**)&f.z_sl,
+                 (R**)&f.f_grnd_acx, (R**)&f.f_grnd_acy, &f.ux, &f.uy,
                  &f.ux_nm1, &f.uy_nm1, &f.visc_int, &f.beta, &f.beta_acx, &f.beta_acy, &f.bss_x,
                  &f.bss_y, &f.lkx, &f.lky, &f.lrx, &f.lry, &f.dudx, &f.dudy, &f.dvdx, &f.dvdy,
                  &f.err_x, &f.err_y, &f.taub_x, &f.taub_y};
@@ -457,6 +459,9 @@ struct Batch {
     UP(f.taul_acx, taul_int_acx) UP(f.taul_acy, taul_int_acy) UP(f.H_ice, H_ice) UP(f.f_ice, f_ice)
     UP(f.H_grnd, H_grnd) UP(f.f_grnd, f_grnd) UP(f.z_bed, z_bed) UP(f.z_sl, z_sl)
 #undef UP
+    // the faces' grounded fraction (read by the subgrid staggering only)
+    if ((rc = up2((R*)f.f_grnd_acx, YG_FIELD(f_grnd_acx).data(), 0, true))) return rc;
+    if ((rc = up2((R*)f.f_grnd_acy, YG_FIELD(f_grnd_acy).data(), 0, true))) return rc;
     if ((rc = up2((R*)f.ATT, YG_FIELD(ATT).data(), 1, false))) return rc;
     // zeta: nz per member, padded to zstride
     std::vector<R> zs((size_t)zstride * nmem, R(0));

@@ -119,7 +119,8 @@ const InPtr kIn2D[] = {&ygpu_ssa_problem::c_bed,        &ygpu_ssa_problem::f_sli
                        &ygpu_ssa_problem::taul_int_acx, &ygpu_ssa_problem::taul_int_acy,
                        &ygpu_ssa_problem::H_ice,        &ygpu_ssa_problem::f_ice,
                        &ygpu_ssa_problem::H_grnd,       &ygpu_ssa_problem::f_grnd,
-                       &ygpu_ssa_problem::z_bed,        &ygpu_ssa_problem::z_sl};
+                       &ygpu_ssa_problem::z_bed,        &ygpu_ssa_problem::z_sl,
+                       &ygpu_ssa_problem::f_grnd_acx,   &ygpu_ssa_problem::f_grnd_acy};
 constexpr int kNIn2D = sizeof(kIn2D) / sizeof(kIn2D[0]);
 
 // INOUT state taken from the first solve of a sequence: (field, may be null)
@@ -467,7 +468,7 @@ struct Server : YgServerBase {
     R* in_dev[kNIn2D] = {(R*)B.f.c_bed, (R*)B.f.f_slide, (R*)B.f.taud_acx,
                          (R*)B.f.taud_acy, (R*)B.f.taul_acx, (R*)B.f.taul_acy, (R*)B.f.H_ice,
                          (R*)B.f.f_ice, (R*)B.f.H_grnd, (R*)B.f.f_grnd, (R*)B.f.z_bed,
-                         (R*)B.f.z_sl};
+                         (R*)B.f.z_sl, (R*)B.f.f_grnd_acx, (R*)B.f.f_grnd_acy};
     for (int q = 0; q < kNIn2D; ++q)
       YG_CHECK(h, hipMemcpyAsync(in_dev[q] + c0, &r.in2d[(size_t)q * nc], sizeof(R) * nc,
                                  hipMemcpyHostToDevice, st));
