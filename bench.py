@@ -530,15 +530,18 @@ def cpu_baseline(args, base):
                       f"BiCGSTAB+ILU(0))"}
 
 
-def counted_traffic(kernel):
-    """HBM bytes per launch of `kernel` counted by the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs recorded in profiles/counted_traffic.json (scaled by the
-    plain-copy calibration stated there), or None."""
+def counted_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` counted by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/counted_traffic.json
+    (scaled by the plain-copy calibration stated there), or None when that
+    file holds no count for this kernel on this workload."""
     path = os.path.join(ROOT, "profiles", "counted_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
         e = d["kernels"][kernel]
+        if e.get("workload") != workload:
+            return None, None
         return e["bytes_per_launch"], e
     except (OSError, KeyError, ValueError):
         return None, None
@@ -711,7 +714,11 @@ def main():
     cpu = None
     if not args.no_cpu and not args.mixed_only and args.cpu_sample > 0:
         cpu = cpu_baseline(args, base)
-    traffic, traffic_src = counted_traffic(kname)
+
+    workload = (f"ensemble {args.members} x {args.side}x{args.side} SSA (dx 8 km, "
+                f"{2 * args.side * args.side} unknowns each), mixed FP32/FP64, batched, "
+                f"ILU factors {args.factor}")
+    traffic, traffic_src = counted_traffic(kname, workload)
 
     line = {
         "metric": "SSA solves/s at <=1e-6 rel vel err vs plain FP64; SpMV HBM GB/s; p99 latency",
@@ -727,9 +734,7 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"ensemble {args.members} x {args.side}x{args.side} SSA (dx 8 km, "
-                        f"{2 * args.side * args.side} unknowns each), mixed FP32/FP64, batched, "
-                        f"ILU factors {args.factor}",
+            "workload": workload,
             "members": args.members, "side": args.side,
             "solver": par_mixed.ssa_lis_opt, "picard_tol": par_mixed.ssa_iter_conv,
             "schedule": {f"{k[0]}x{k[1]}": [v.ilu_blocks, v.block] for k, v in sched.items()},
