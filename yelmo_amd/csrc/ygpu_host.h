@@ -846,9 +846,10 @@ struct Batch {
       int rate = 0;
       (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, h->device);
       const double us = rate > 0 ? 1e3 / rate : 1e-2;  // rate in kHz
-      fprintf(stderr, "ygpu phases member0 (ms): factor %.3f sweepL %.3f sweepU %.3f spmv %.3f total %.3f\n",
+      fprintf(stderr, "ygpu phases member0 (ms): factor %.3f sweepL %.3f sweepU %.3f spmv %.3f "
+              "vectors %.3f group-barrier waits %.3f total %.3f\n",
               tmh[0] * us * 1e-3, tmh[1] * us * 1e-3, tmh[2] * us * 1e-3, tmh[3] * us * 1e-3,
-              tmh[7] * us * 1e-3);
+              tmh[4] * us * 1e-3, tmh[5] * us * 1e-3, tmh[7] * us * 1e-3);
     }
     if (d_gfail) {
       int gf = 0;

@@ -146,6 +146,7 @@ struct Grp {
 __shared__ int yg_gbar;    // group barriers this block has passed
 __shared__ int yg_gfail;   // a wait of this group timed out
 __shared__ int yg_gfence;  // the group spans several XCDs: barriers fence
+__shared__ long long yg_gwait;   // wall-clock ticks this block spent waiting at group barriers
 
 YG_DEV int grp_load(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -178,6 +179,7 @@ YG_DEV void grp_barrier(const Grp& g) {
         }
         __builtin_amdgcn_s_sleep(1);
       }
+      yg_gwait += wall_clock64() - t0;
     }
     if (yg_gfence) __threadfence();   // acquire at device scope
     yg_gbar = k;
@@ -196,6 +198,7 @@ YG_DEV void grp_start(const Grp& g, int* xcc_words) {
     yg_gbar = 0;
     yg_gfail = 0;
     yg_gfence = 1;
+    yg_gwait = 0;
     xcc_words[g.rank] = xcc_id();
   }
   __syncthreads();
@@ -951,12 +954,63 @@ YG_DEV int rot_prev(const SweepTile& g, int sq) { return (sq == 0 ? g.LM : sq) -
 // step keeps the steps in program order, so the wait before a step covers
 // exactly that step's loads.  The store offset of a step is computed with its
 // loads and kept in the slot.
+//
+// Addressing: a lane's offset inside a step is fixed (lane x cell bytes in
+// the rotated order, its line's natural cell in the natural order), and the
+// step's own offset is wave-uniform ((t mod M) L or t cells), so it goes in
+// the scalar offset of the buffer instruction; per step a lane only selects
+// its fixed offset or the out-of-range one.  A lane on no cell keeps the
+// out-of-range vector offset whatever the scalar offset adds (below 2^31), so
+// its loads read zeros and its store is dropped.
+//
+// Wave shifts: the line before (forward) or after (backward) is read at the
+// previous step and at the one before.  The shift of a value two steps back
+// is the shift taken one step back, so each step shifts only its two newest
+// values and keeps them for the next step.
+//
+// One wave runs a step's recurrence alone, in order, so the chain of
+// dependent operations from the previous step's results sets the step's
+// time: the terms of values two steps back are summed first, and the
+// previous step's results enter last.
+
+template <int N>
+YG_DEV void load_vals_so(__amdgpu_buffer_rsrc_t r, unsigned off, int soff, float (&v)[N]) {
+#pragma unroll
+  for (int e = 0; e < N / 4; ++e) {
+    const float4 x = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * e, soff, 0));
+    v[4 * e] = x.x; v[4 * e + 1] = x.y; v[4 * e + 2] = x.z; v[4 * e + 3] = x.w;
+  }
+}
+template <int N>
+YG_DEV void load_vals_so(__amdgpu_buffer_rsrc_t r, unsigned off, int soff, double (&v)[N]) {
+#pragma unroll
+  for (int e = 0; e < N / 2; ++e) {
+    const double2 x = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * e, soff, 0));
+    v[2 * e] = x.x; v[2 * e + 1] = x.y;
+  }
+}
+YG_DEV float2 pair_load_so(__amdgpu_buffer_rsrc_t r, unsigned off, int soff, float) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+YG_DEV double2 pair_load_so(__amdgpu_buffer_rsrc_t r, unsigned off, int soff, double) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+}
+YG_DEV void pair_store_so(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, off, soff, 0);
+}
+YG_DEV void pair_store_so(double2 v, __amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, soff, 0);
+}
+YG_DEV uint4 load_raw16_so(__amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+}
+
 template <typename T>
 YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
                              T* __restrict__ ys) {
   PhaseTimer ptm(c, 1);
   constexpr int D = sizeof(T) == 4 ? 12 : 6;   // register slots (steps in flight)
-  constexpr unsigned SF = 8 * sizeof(T), SV = 2 * sizeof(T);   // bytes per cell
+  constexpr int SF = 8 * sizeof(T), SV = 2 * sizeof(T);   // bytes per cell
   using V2 = typename Vec2<T>::type;
   SweepTile g;
   if (!sweep_tile(c, g)) return;
@@ -964,36 +1018,41 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + (size_t)8 * g.base, (size_t)8 * g.LM);
   const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned voF = lane * SF, voY = lane * SV, voN = (unsigned)g.lane_n * SV;
   T f[D][8];
   V2 r[D];
   unsigned so[D];
+  int ss[D];
   int sq = 0;   // (t mod M) L of the step loaded next
+  auto issue = [&](int s, int t) {
+    const bool v = on_step(g, t);
+    load_vals_so<8>(rF, v ? voF : YG_OOB, sq * SF, f[s]);
+    r[s] = pair_load_so(rR, v ? voN : YG_OOB, t * SV, T());
+    so[s] = v ? voY : YG_OOB;
+    ss[s] = sq * SV;
+    sq = rot_next(g, sq);
+  };
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    load_vals<8>(rF, rot_off(g, s, sq, SF), f[s]);
-    r[s] = Pair16<T>::load(rR, nat_off(g, s, SV));
-    so[s] = rot_off(g, s, sq, SV);
-    sq = rot_next(g, sq);
+    issue(s, s);
     __builtin_amdgcn_sched_barrier(0);
   }
-  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+  // o1: this lane's results one step back; p1: the line before, two steps back
+  T o1x = 0, o1y = 0, p1x = 0, p1y = 0;
   for (int t0 = 0; t0 < g.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
-      T yx = r[s].x - f[s][0] * n2x - f[s][1] * o1x - f[s][2] * o1y;
-      T yy = r[s].y - f[s][3] * n2x - f[s][4] * n2y - f[s][5] * n1x - f[s][6] * o1y -
-             f[s][7] * yx;
+      const T n1x = dpp_shr1(o1x), n1y = dpp_shr1(o1y);
+      const T ax = r[s].x - f[s][0] * p1x, ay = r[s].y - f[s][3] * p1x - f[s][4] * p1y;
+      T yx = ax - f[s][1] * o1x - f[s][2] * o1y;
+      T yy = ay - f[s][6] * o1y - f[s][5] * n1x - f[s][7] * yx;
       V2 o;
       o.x = yx;
       o.y = yy;
-      Pair16<T>::store(o, rY, so[s]);
-      o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
-      const int t = t0 + s + D;   // the step loaded into slot s
-      load_vals<8>(rF, rot_off(g, t, sq, SF), f[s]);
-      r[s] = Pair16<T>::load(rR, nat_off(g, t, SV));
-      so[s] = rot_off(g, t, sq, SV);
-      sq = rot_next(g, sq);
+      pair_store_so(o, rY, so[s], ss[s]);
+      p1x = n1x; p1y = n1y; o1x = yx; o1y = yy;
+      issue(s, t0 + s + D);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1004,7 +1063,7 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
                              T* __restrict__ z) {
   PhaseTimer ptm(c, 2);
   constexpr int D = sizeof(T) == 4 ? 10 : 6;
-  constexpr unsigned SF = 12 * sizeof(T), SV = 2 * sizeof(T);
+  constexpr int SF = 12 * sizeof(T), SV = 2 * sizeof(T);
   using V2 = typename Vec2<T>::type;
   SweepTile g;
   if (!sweep_tile(c, g)) return;
@@ -1012,38 +1071,43 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + c.fu + (size_t)12 * g.base, (size_t)12 * g.LM);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned voF = lane * SF, voY = lane * SV, voN = (unsigned)g.lane_n * SV;
   T f[D][12];
   V2 yv[D];
   unsigned so[D];
+  int ss[D];
   const int tz = g.Tb - 1;   // the last step; the sweep runs t = tz - u
   int sq = (tz % g.M) * g.L;
+  auto issue = [&](int s, int t) {
+    const bool v = on_step(g, t);
+    load_vals_so<12>(rF, v ? voF : YG_OOB, sq * SF, f[s]);
+    yv[s] = pair_load_so(rY, v ? voY : YG_OOB, sq * SV, T());
+    so[s] = v ? voN : YG_OOB;
+    ss[s] = t * SV;
+    sq = rot_prev(g, sq);
+  };
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    load_vals<12>(rF, rot_off(g, tz - s, sq, SF), f[s]);
-    yv[s] = Pair16<T>::load(rY, rot_off(g, tz - s, sq, SV));
-    so[s] = nat_off(g, tz - s, SV);
-    sq = rot_prev(g, sq);
+    issue(s, tz - s);
     __builtin_amdgcn_sched_barrier(0);
   }
-  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;  // own cells j+1, j+2
+  // o1: own results one step back; p1: the line after, two steps back
+  T o1x = 0, o1y = 0, p1x = 0, p1y = 0;
   for (int u0 = 0; u0 < g.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       // line i+1: cell j-1 (one step back) and cell j (two steps back)
-      const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
-      T zy = (yv[s].y - f[s][6] * o1x - f[s][7] * o1y - f[s][8] * n2y) * f[s][9];
-      T zx = (yv[s].x - f[s][0] * zy - f[s][1] * o1x - f[s][2] * n1y - f[s][3] * n2x -
-              f[s][4] * n2y) * f[s][5];
+      const T n1y = dpp_shl1(o1y), n1x = dpp_shl1(o1x);
+      const T by = yv[s].y - f[s][8] * p1y, bx = yv[s].x - f[s][3] * p1x - f[s][4] * p1y;
+      T zy = (by - f[s][6] * o1x - f[s][7] * o1y) * f[s][9];
+      T zx = (bx - f[s][1] * o1x - f[s][2] * n1y - f[s][0] * zy) * f[s][5];
       V2 o;
       o.x = zx;
       o.y = zy;
-      Pair16<T>::store(o, rZ, so[s]);
-      o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
-      const int t = tz - (u0 + s + D);
-      load_vals<12>(rF, rot_off(g, t, sq, SF), f[s]);
-      yv[s] = Pair16<T>::load(rY, rot_off(g, t, sq, SV));
-      so[s] = nat_off(g, t, SV);
-      sq = rot_prev(g, sq);
+      pair_store_so(o, rZ, so[s], ss[s]);
+      p1x = n1x; p1y = n1y; o1x = zx; o1y = zy;
+      issue(s, tz - (u0 + s + D));
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1056,7 +1120,7 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ rhs, float* __restrict__ ys) {
   PhaseTimer ptm(c, 1);
   constexpr int D = 20;   // steps in flight: 7 registers per step (4 packed factor words, 2 rhs, 1 offset)
-  constexpr unsigned SF = 16, SV = 8;
+  constexpr int SF = 16, SV = 8;
   using T = float;
   using V2 = typename Vec2<T>::type;
   SweepTile g;
@@ -1065,37 +1129,42 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(FB + (size_t)8 * g.base, (size_t)8 * g.LM);
   const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned voF = lane * SF, voY = lane * SV, voN = (unsigned)g.lane_n * SV;
   uint4 fw[D];
   V2 r[D];
   unsigned so[D];
+  int ss[D];
   int sq = 0;
+  auto issue = [&](int s, int t) {
+    const bool v = on_step(g, t);
+    fw[s] = load_raw16_so(rF, v ? voF : YG_OOB, sq * SF);
+    r[s] = pair_load_so(rR, v ? voN : YG_OOB, t * SV, T());
+    so[s] = v ? voY : YG_OOB;
+    ss[s] = sq * SV;
+    sq = rot_next(g, sq);
+  };
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    fw[s] = load_raw16(rF, rot_off(g, s, sq, SF));
-    r[s] = Pair16<T>::load(rR, nat_off(g, s, SV));
-    so[s] = rot_off(g, s, sq, SV);
-    sq = rot_next(g, sq);
+    issue(s, s);
     __builtin_amdgcn_sched_barrier(0);
   }
-  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+  T o1x = 0, o1y = 0, p1x = 0, p1y = 0;
   for (int t0 = 0; t0 < g.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       const T f0 = bf_lo(fw[s].x), f1 = bf_hi(fw[s].x), f2 = bf_lo(fw[s].y), f3 = bf_hi(fw[s].y);
       const T f4 = bf_lo(fw[s].z), f5 = bf_hi(fw[s].z), f6 = bf_lo(fw[s].w), f7 = bf_hi(fw[s].w);
-      const T n1x = dpp_shr1(o1x), n2x = dpp_shr1(o2x), n2y = dpp_shr1(o2y);
-      T yx = r[s].x - f0 * n2x - f1 * o1x - f2 * o1y;
-      T yy = r[s].y - f3 * n2x - f4 * n2y - f5 * n1x - f6 * o1y - f7 * yx;
+      const T n1x = dpp_shr1(o1x), n1y = dpp_shr1(o1y);
+      const T ax = r[s].x - f0 * p1x, ay = r[s].y - f3 * p1x - f4 * p1y;
+      T yx = ax - f1 * o1x - f2 * o1y;
+      T yy = ay - f6 * o1y - f5 * n1x - f7 * yx;
       V2 o;
       o.x = yx;
       o.y = yy;
-      Pair16<T>::store(o, rY, so[s]);
-      o2x = o1x; o2y = o1y; o1x = yx; o1y = yy;
-      const int t = t0 + s + D;
-      fw[s] = load_raw16(rF, rot_off(g, t, sq, SF));
-      r[s] = Pair16<T>::load(rR, nat_off(g, t, SV));
-      so[s] = rot_off(g, t, sq, SV);
-      sq = rot_next(g, sq);
+      pair_store_so(o, rY, so[s], ss[s]);
+      p1x = n1x; p1y = n1y; o1x = yx; o1y = yy;
+      issue(s, t0 + s + D);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1105,7 +1174,7 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ ys, float* __restrict__ z) {
   PhaseTimer ptm(c, 2);
   constexpr int D = 18;   // 9 registers per step: 4 packed words, the 2 inverse diagonals, y, offset
-  constexpr unsigned SF = 16, SD = 8, SV = 8;
+  constexpr int SF = 16, SD = 8, SV = 8;
   using T = float;
   using V2 = typename Vec2<T>::type;
   SweepTile g;
@@ -1115,22 +1184,30 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   const __amdgpu_buffer_rsrc_t rD = yg_rsrc(FB + c.fd16 + (size_t)4 * g.base, (size_t)4 * g.LM);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
   const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned voF = lane * SF, voD = lane * SD, voY = lane * SV, voN = (unsigned)g.lane_n * SV;
   uint4 gw[D];   // the 8 scaled off-diagonal U entries
   V2 dg[D];      // ux-row and uy-row inverse diagonals
   V2 yv[D];
   unsigned so[D];
+  int ss[D];
   const int tz = g.Tb - 1;   // the last step
   int sq = (tz % g.M) * g.L;
+  auto issue = [&](int s, int t) {
+    const bool v = on_step(g, t);
+    gw[s] = load_raw16_so(rF, v ? voF : YG_OOB, sq * SF);
+    dg[s] = pair_load_so(rD, v ? voD : YG_OOB, sq * SD, T());
+    yv[s] = pair_load_so(rY, v ? voY : YG_OOB, sq * SV, T());
+    so[s] = v ? voN : YG_OOB;
+    ss[s] = t * SV;
+    sq = rot_prev(g, sq);
+  };
 #pragma unroll
   for (int s = 0; s < D; ++s) {
-    gw[s] = load_raw16(rF, rot_off(g, tz - s, sq, SF));
-    dg[s] = Pair16<T>::load(rD, rot_off(g, tz - s, sq, SD));
-    yv[s] = Pair16<T>::load(rY, rot_off(g, tz - s, sq, SV));
-    so[s] = nat_off(g, tz - s, SV);
-    sq = rot_prev(g, sq);
+    issue(s, tz - s);
     __builtin_amdgcn_sched_barrier(0);
   }
-  T o1x = 0, o1y = 0, o2x = 0, o2y = 0;
+  T o1x = 0, o1y = 0, p1x = 0, p1y = 0;
   for (int u0 = 0; u0 < g.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
@@ -1138,20 +1215,16 @@ YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
       // entries (slots 6,1,3)
       const T g0 = bf_lo(gw[s].x), g1 = bf_hi(gw[s].x), g2 = bf_lo(gw[s].y), g3 = bf_hi(gw[s].y);
       const T g4 = bf_lo(gw[s].z), g5 = bf_hi(gw[s].z), g6 = bf_lo(gw[s].w), g7 = bf_hi(gw[s].w);
-      const T n1y = dpp_shl1(o1y), n2x = dpp_shl1(o2x), n2y = dpp_shl1(o2y);
-      T zy = dg[s].y * yv[s].y - g5 * o1x - g6 * o1y - g7 * n2y;
-      T zx = dg[s].x * yv[s].x - g0 * zy - g1 * o1x - g2 * n1y - g3 * n2x - g4 * n2y;
+      const T n1y = dpp_shl1(o1y), n1x = dpp_shl1(o1x);
+      const T by = dg[s].y * yv[s].y - g7 * p1y, bx = dg[s].x * yv[s].x - g3 * p1x - g4 * p1y;
+      T zy = by - g5 * o1x - g6 * o1y;
+      T zx = bx - g1 * o1x - g2 * n1y - g0 * zy;
       V2 o;
       o.x = zx;
       o.y = zy;
-      Pair16<T>::store(o, rZ, so[s]);
-      o2x = o1x; o2y = o1y; o1x = zx; o1y = zy;
-      const int t = tz - (u0 + s + D);
-      gw[s] = load_raw16(rF, rot_off(g, t, sq, SF));
-      dg[s] = Pair16<T>::load(rD, rot_off(g, t, sq, SD));
-      yv[s] = Pair16<T>::load(rY, rot_off(g, t, sq, SV));
-      so[s] = nat_off(g, t, SV);
-      sq = rot_prev(g, sq);
+      pair_store_so(o, rZ, so[s], ss[s]);
+      p1x = n1x; p1y = n1y; o1x = zx; o1y = zy;
+      issue(s, tz - (u0 + s + D));
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1460,8 +1533,10 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     if (rho == 0.0) { out.status = 2; out.iters = it; return out; }
     // p = r + beta (p - omega v); phat = M^-1 p
     if (it == 1) {
+      PhaseTimer ptm(c, 4);
       vmap<TV, 1, 1>(lo, hi, {r}, {p}, [&](const double (&xi)[1], double (&xo)[1]) { xo[0] = xi[0]; });
     } else {
+      PhaseTimer ptm(c, 4);
       const double beta = (rho / rho_old) * (alpha / omega);
       vmap<TV, 3, 1>(lo, hi, {r, p, v}, {p}, [&](const double (&xi)[3], double (&xo)[1]) {
         xo[0] = xi[0] + beta * (xi[1] - omega * xi[2]);
@@ -1476,11 +1551,14 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     alpha = rho / red[0];
     // s = r - alpha v and ||s||: converged on s ends the iteration early
     red[0] = 0.0;
-    vmap<TV, 2, 1>(lo, hi, {r, v}, {s}, [&](const double (&xi)[2], double (&xo)[1]) {
-      const double sq = (double)(TV)(xi[0] - alpha * xi[1]);
-      xo[0] = sq;
-      red[0] += sq * sq;
-    });
+    {
+      PhaseTimer ptm(c, 4);
+      vmap<TV, 2, 1>(lo, hi, {r, v}, {s}, [&](const double (&xi)[2], double (&xo)[1]) {
+        const double sq = (double)(TV)(xi[0] - alpha * xi[1]);
+        xo[0] = sq;
+        red[0] += sq * sq;
+      });
+    }
     if (GRP) {
       // shat first: the sum of ||s|| then doubles as the barrier before the
       // SpMV on shat (on the last iteration shat goes unused)
@@ -1509,14 +1587,17 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     if (omega == 0.0) { out.status = 2; out.iters = it; return out; }
     // x += alpha phat + omega shat; r = s - omega t; (r,r), (rt,r)
     red[0] = red[1] = 0.0;
-    vmap<TV, 6, 2>(lo, hi, {x, ph, sh, s, tt, rt}, {x, r},
-                   [&](const double (&xi)[6], double (&xo)[2]) {
-      xo[0] = xi[0] + alpha * xi[1] + omega * xi[2];
-      const double rq = (double)(TV)(xi[3] - omega * xi[4]);
-      xo[1] = rq;
-      red[0] += rq * rq;
-      red[1] += xi[5] * rq;
-    });
+    {
+      PhaseTimer ptm(c, 4);
+      vmap<TV, 6, 2>(lo, hi, {x, ph, sh, s, tt, rt}, {x, r},
+                     [&](const double (&xi)[6], double (&xo)[2]) {
+        xo[0] = xi[0] + alpha * xi[1] + omega * xi[2];
+        const double rq = (double)(TV)(xi[3] - omega * xi[4]);
+        xo[1] = rq;
+        red[0] += rq * rq;
+        red[1] += xi[5] * rq;
+      });
+    }
     csum<2>(c, red);
     out.resid = sqrt(red[0]) * bnrm;
     out.iters = it;
@@ -1848,7 +1929,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     const int T = Lg > 0 ? ny + 2 * (Lg - 1) : 0;
     __syncthreads();   // the previous member's step table is no longer read
     for (int k = threadIdx.x; k <= T; k += blockDim.x) yg_soff[k] = blk_soff(k, Lg, ny);
-    if (threadIdx.x == 0) s_bad = 0;
+    if (threadIdx.x == 0) { s_bad = 0; yg_gwait = 0; }
     if (threadIdx.x < 8) s_tm[threadIdx.x] = 0;
     __syncthreads();
     const long long t_kernel0 = wall_clock64();
@@ -1971,6 +2052,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
       a.out_iters[m] = iters; a.out_status[m] = status; a.out_resid[m] = res;
       if (a.out_sweeps) a.out_sweeps[m] = nres;
       if (a.timers) {
+        s_tm[5] = yg_gwait;
         s_tm[7] = wall_clock64() - t_kernel0;
         for (int k = 0; k < 8; ++k) a.timers[8 * m + k] += s_tm[k];
       }
