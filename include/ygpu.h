@@ -75,8 +75,8 @@ typedef struct ygpu_ssa_params {
   int32_t boundaries;     /* YGPU_BND_*                                          */
   int32_t visc_method;    /* 0, 1 (quadrature), 2 (aa-nodes)                     */
   int32_t beta_method;    /* -1 .. 5                                             */
-  int32_t beta_gl_scale;  /* 0, 1, 3                                             */
-  int32_t beta_gl_stag;   /* -1, 0, 1, 2                                         */
+  int32_t beta_gl_scale;  /* 0 .. 3 (2: zstar scaling, needs z_bed, z_sl)      */
+  int32_t beta_gl_stag;   /* -1 .. 4 (3, 4: subgrid, need f_grnd_acx/acy)       */
   int32_t ssa_vel_lim_method; /* 0 "clip", 1 "drag"                              */
   int32_t ssa_iter_max;
   int32_t ssa_write_log;
