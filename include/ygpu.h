@@ -233,6 +233,26 @@ int ygpu_velocity_diva(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, i
                        const float* z_srf, float dx, float dy, float n_glen,
                        ygpu_neff_hook neff_hook, void* hook_ctx);
 
+/* ---- per-solve inputs of the velocity solve (what calc_ydyn computes
+ * before calling it, yelmo_dynamics.f90:105-122,464, with the topography
+ * step's surface slopes, yelmo_topography.f90:1373-1374,1472), on the
+ * device, wp = sp arrays (nx, ny):
+ *   mask_frnt        calc_ice_front (topography.f90:724-818)
+ *   dzsdx, dzsdy     calc_gradient_acx/acy of z_srf (yelmo_tools.f90:341-478;
+ *                    zero_outside false, slope_bg_x/y, clipped at grad_lim)
+ *   taud_acx/acy     calc_driving_stress (velocity_general.f90:1118-1205)
+ *   taul_int_acx/acy calc_lateral_bc_stress_2D (velocity_general.f90:1453-1609)
+ *   ssa_mask_acx/acy set_ssa_masks (solver_ssa_ac.f90:851-996)
+ * par supplies boundaries, rho_ice, rho_sw and g; lateral_bc is ssa_lat_bc
+ * (0 "none", 1 "floating", 2 "marine", 3 "all").  Any output may be NULL. */
+int ygpu_dyn_inputs(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int32_t ny,
+                    float dx, float dy, float taud_lim, float grad_lim, float slope_bg_x,
+                    float slope_bg_y, int32_t lateral_bc, const float* H_ice, const float* f_ice,
+                    const float* f_grnd, const float* z_bed, const float* z_sl,
+                    const float* z_srf, int32_t* mask_frnt, float* dzsdx, float* dzsdy,
+                    float* taud_acx, float* taud_acy, float* taul_int_acx, float* taul_int_acy,
+                    int32_t* ssa_mask_acx, int32_t* ssa_mask_acy);
+
 /* ---- calc_velocity_diva (velocity_diva.f90:59-429), batched form.         */
 int ygpu_velocity_diva_batch(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nprob,
                              ygpu_ssa_problem* probs);

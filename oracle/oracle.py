@@ -302,6 +302,89 @@ def set_ssa_masks(mask_frnt, f_ice, f_grnd, lateral_bc="all", boundaries="zeros"
     return mx, my
 
 
+def calc_ice_front(f_ice, f_grnd, z_bed, z_sl, boundaries="zeros", dp=False):
+    """calc_ice_front (topography.f90:724-818): the int32 front codes."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    nx, ny = f_ice.shape
+    m = np.zeros((nx, ny), dtype=np.int32, order="F")
+    keep = [_F(a, dt) for a in (f_ice, f_grnd, z_bed, z_sl)]
+    L.oracle_calc_ice_front(_p(m), *[_p(a) for a in keep], nx, ny, BND[boundaries])
+    return m
+
+
+def calc_gradient(var, f_ice, dxy, grad_lim, direction, zero_outside=False, slope_bg=0.0,
+                  boundaries="zeros", dp=False):
+    """calc_gradient_acx (direction 0) / calc_gradient_acy (1),
+    yelmo_tools.f90:341-478."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = _W(dp)
+    nx, ny = var.shape
+    d = np.zeros((nx, ny), dtype=dt, order="F")
+    keep = [_F(var, dt), _F(f_ice, dt)]
+    L.oracle_calc_gradient(_p(d), _p(keep[0]), _p(keep[1]), W(dxy), W(grad_lim),
+                           int(zero_outside), W(slope_bg), direction, nx, ny, BND[boundaries])
+    return d
+
+
+def calc_driving_stress(H_ice, f_ice, dzsdx, dzsdy, taud_lim, rho_ice, g, boundaries="zeros",
+                        dp=False):
+    """calc_driving_stress (velocity_general.f90:1118-1205)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = _W(dp)
+    nx, ny = H_ice.shape
+    tx = np.zeros((nx, ny), dtype=dt, order="F")
+    ty = np.zeros((nx, ny), dtype=dt, order="F")
+    keep = [_F(a, dt) for a in (H_ice, f_ice, dzsdx, dzsdy)]
+    L.oracle_calc_driving_stress(_p(tx), _p(ty), *[_p(a) for a in keep], W(taud_lim),
+                                 W(rho_ice), W(g), nx, ny, BND[boundaries])
+    return tx, ty
+
+
+def lateral_bc_stress(H, z_srf, z_sl, rho_ice, rho_sw, g, dp=False):
+    """calc_lateral_bc_stress (velocity_general.f90:1578-1609) at one point."""
+    L = lib(dp)
+    W = _W(dp)
+    fn = L.oracle_lateral_bc_stress
+    fn.restype = W
+    fn.argtypes = [W] * 6
+    return float(fn(H, z_srf, z_sl, rho_ice, rho_sw, g))
+
+
+def calc_lateral_bc_stress_2D(mask_frnt, H_ice, f_ice, z_srf, z_sl, rho_ice, rho_sw, g,
+                              boundaries="zeros", dp=False):
+    """calc_lateral_bc_stress_2D (velocity_general.f90:1453-1576)."""
+    L = lib(dp)
+    dt = np.float64 if dp else np.float32
+    W = _W(dp)
+    nx, ny = H_ice.shape
+    tx = np.zeros((nx, ny), dtype=dt, order="F")
+    ty = np.zeros((nx, ny), dtype=dt, order="F")
+    mf = np.asfortranarray(mask_frnt, dtype=np.int32)
+    keep = [_F(a, dt) for a in (H_ice, f_ice, z_srf, z_sl)]
+    L.oracle_calc_lateral_bc_stress_2D(_p(tx), _p(ty), _p(mf), *[_p(a) for a in keep],
+                                       W(rho_ice), W(rho_sw), W(g), nx, ny, BND[boundaries])
+    return tx, ty
+
+
+def dyn_inputs(H_ice, f_ice, f_grnd, z_bed, z_sl, z_srf, dx, dy, taud_lim, grad_lim,
+               rho_ice, rho_sw, g, lateral_bc="all", slope_bg=(0.0, 0.0), boundaries="zeros",
+               dp=False):
+    """The per-solve inputs in calc_ydyn's order: ice front, surface slopes,
+    driving stress, lateral front stress, SSA face masks.  Returns a dict."""
+    mf = calc_ice_front(f_ice, f_grnd, z_bed, z_sl, boundaries, dp)
+    sx = calc_gradient(z_srf, f_ice, dx, grad_lim, 0, False, slope_bg[0], boundaries, dp)
+    sy = calc_gradient(z_srf, f_ice, dy, grad_lim, 1, False, slope_bg[1], boundaries, dp)
+    tx, ty = calc_driving_stress(H_ice, f_ice, sx, sy, taud_lim, rho_ice, g, boundaries, dp)
+    lx, ly = calc_lateral_bc_stress_2D(mf, H_ice, f_ice, z_srf, z_sl, rho_ice, rho_sw, g,
+                                       boundaries, dp)
+    mx, my = set_ssa_masks(mf, f_ice, f_grnd, lateral_bc, boundaries, dp)
+    return dict(mask_frnt=mf, dzsdx=sx, dzsdy=sy, taud_acx=tx, taud_acy=ty, taul_int_acx=lx,
+                taul_int_acy=ly, ssa_mask_acx=mx, ssa_mask_acy=my)
+
+
 def visc_eff_3D_aa(ux, uy, ATT, f_ice, dx, dy, n_glen, eps_0, boundaries="zeros", dp=True):
     L = lib(dp)
     dt = np.float64 if dp else np.float32
@@ -392,7 +475,8 @@ PLANTS = {
     "conv_norm2": 11, "l1rel_tol": 12, "relax_vel": 13, "relax_visc": 14, "clip": 15,
     "gl_upstream": 16, "beta_min_mask": 17, "diva_shear": 18, "beta_q": 19, "gq_att_act": 20,
     "adv_inflow": 21, "adv_diag": 22, "zstar_norm": 23, "zstar_rho": 24, "subgrid_wt": 25,
-    "flux_side": 26,
+    "flux_side": 26, "front_free_bed": 27, "grad_border": 28, "taud_margin": 29,
+    "taul_land": 30, "taul_rho": 31,
 }
 
 

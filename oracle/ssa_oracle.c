@@ -82,7 +82,12 @@ enum {
     PL_ZSTAR_NORM = 23,      /* zstar scaling not normalised by H_eff              */
     PL_ZSTAR_RHO = 24,       /* zstar flotation depth with rho_ice/rho_sw          */
     PL_SUBGRID_WT = 25,      /* subgrid staggering weight f_grnd_ac, not squared   */
-    PL_FLUX_SIDE = 26        /* flux weight counts segments with lambda < f_grnd   */
+    PL_FLUX_SIDE = 26,       /* flux weight counts segments with lambda < f_grnd   */
+    PL_FRONT_FREE_BED = 27,  /* ice-free front code from the front cell's bed      */
+    PL_GRAD_BORDER = 28,     /* no border copy of the gradient                     */
+    PL_TAUD_MARGIN = 29,     /* margin faces take the mean thickness               */
+    PL_TAUL_LAND = 30,       /* sea-level back-pressure also against land          */
+    PL_TAUL_RHO = 31         /* rho_sw in the ice term of the front stress         */
 };
 
 /* ------------------------------------------------------------------ */
@@ -1798,6 +1803,7 @@ out:
 
 /* the implicit thickness advection system (calc_advec2D, "impl-lis") */
 #include "advection_oracle.inc"
+#include "inputs_oracle.inc"
 
 /* size of wp, so the Python driver can check which build it loaded */
 int oracle_wp_bytes(void) { return (int)sizeof(wp); }

@@ -66,7 +66,7 @@ NEFF_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_v
 
 # every symbol include/ygpu.h declares
 EXPORTS = ("ygpu_create", "ygpu_destroy", "ygpu_last_error", "ygpu_last_note",
-           "ygpu_default_params", "ygpu_csr_solve", "ygpu_csr_solve_noalloc",
+           "ygpu_default_params", "ygpu_csr_solve", "ygpu_csr_solve_noalloc", "ygpu_dyn_inputs",
            "ygpu_velocity_ssa", "ygpu_velocity_diva",
            "ygpu_velocity_ssa_batch",
            "ygpu_velocity_diva_batch", "ygpu_batch_create", "ygpu_batch_run",
@@ -121,6 +121,9 @@ def load() -> C.CDLL:
     lib.ygpu_csr_solve.restype = C.c_int
     lib.ygpu_csr_solve_noalloc.argtypes = lib.ygpu_csr_solve.argtypes
     lib.ygpu_csr_solve_noalloc.restype = C.c_int
+    lib.ygpu_dyn_inputs.argtypes = ([C.c_void_p, C.POINTER(SsaParamsC), C.c_int32, C.c_int32]
+                                    + [C.c_float] * 6 + [C.c_int32] + [C.c_void_p] * 15)
+    lib.ygpu_dyn_inputs.restype = C.c_int
     lib.ygpu_velocity_ssa_batch.argtypes = [PP, C.POINTER(SsaParamsC), I, C.POINTER(SsaProblemC)]
     lib.ygpu_velocity_ssa_batch.restype = C.c_int
     lib.ygpu_velocity_diva_batch.argtypes = [PP, C.POINTER(SsaParamsC), I, C.POINTER(SsaProblemC)]

@@ -16,7 +16,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libygpu.so"
 SOURCES = ["ygpu_api.hip", "ygpu_serve.hip", "ygpu_tune.hip", "ygpu_fields.hip",
            "ygpu_krylov.hip", "ygpu_krylov_1024.hip", "ygpu_krylov_group.hip", "ygpu_diva.hip",
-           "ygpu_spmv.hip", "ygpu_noalloc.hip", "ygpu_scalar.hip"]
+           "ygpu_spmv.hip", "ygpu_noalloc.hip", "ygpu_scalar.hip", "ygpu_inputs.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -338,3 +338,31 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+LATERAL_BC = {"none": 0, "floating": 1, "float": 1, "marine": 2, "all": 3}
+
+
+def calc_dyn_inputs(fields: dict, par: SsaParams, dx: float, dy: float, taud_lim: float = 5e5,
+                    grad_lim: float = 0.1, lateral_bc: str = "all", slope_bg=(0.0, 0.0),
+                    handle=None) -> dict:
+    """The per-solve inputs of the velocity solve on the device
+    (``ygpu_dyn_inputs``): what calc_ydyn computes before the call
+    (yelmo_dynamics.f90:105-122,464) from the geometry in ``fields`` (H_ice,
+    f_ice, f_grnd, z_bed, z_sl, z_srf; float32, Fortran order).  Returns
+    mask_frnt, dzsdx, dzsdy, taud_acx/acy, taul_int_acx/acy and
+    ssa_mask_acx/acy."""
+    h = handle or _lib.default_handle()
+    nx, ny = fields["H_ice"].shape
+    ins = [np.asfortranarray(fields[k], dtype=np.float32)
+           for k in ("H_ice", "f_ice", "f_grnd", "z_bed", "z_sl", "z_srf")]
+    f = lambda: np.zeros((nx, ny), dtype=np.float32, order="F")
+    i = lambda: np.zeros((nx, ny), dtype=np.int32, order="F")
+    out = dict(mask_frnt=i(), dzsdx=f(), dzsdy=f(), taud_acx=f(), taud_acy=f(),
+               taul_int_acx=f(), taul_int_acy=f(), ssa_mask_acx=i(), ssa_mask_acy=i())
+    pc = par.to_c()
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    h.check(h.lib.ygpu_dyn_inputs(h.h, C.byref(pc), nx, ny, dx, dy, taud_lim, grad_lim,
+                                  slope_bg[0], slope_bg[1], LATERAL_BC[lateral_bc],
+                                  *[ptr(a) for a in ins], *[ptr(a) for a in out.values()]))
+    return out
