@@ -63,3 +63,90 @@ def test_dyn_inputs_hand_built_front(lateral_bc):
     f = {k: np.asfortranarray(v, dtype=np.float32) for k, v in
          dict(H_ice=H, f_ice=fi, f_grnd=fg, z_bed=zb, z_sl=zsl, z_srf=zs).items()}
     _check(f, 2e3, 3e3, "zeros", lateral_bc, slope_bg=(1e-3, -2e-3), grad_lim=0.05)
+
+
+DERIVED_FIELDS = ("taud_acx", "taud_acy", "taul_int_acx", "taul_int_acy")
+
+
+def _with(p, d):
+    q = p.copy()
+    for k in DERIVED_FIELDS:
+        q.fields[k] = d[k]
+    q.state["ssa_mask_acx"] = d["ssa_mask_acx"]
+    q.state["ssa_mask_acy"] = d["ssa_mask_acy"]
+    return q
+
+
+def _without(p):
+    q = p.copy()
+    for k in DERIVED_FIELDS:
+        q.fields[k] = None
+    q.state["ssa_mask_acx"] = None
+    q.state["ssa_mask_acy"] = None
+    return q
+
+
+def _thicken(p, dH):
+    """The next solve of a sequence: thickness changed where there is ice,
+    with the surface, flotation and grounded fraction that follow."""
+    from yelmo_amd import inputs
+    q = p.copy()
+    f = q.fields
+    H = np.where(f["f_ice"] > 0, np.maximum(f["H_ice"] + dH, 0.0), 0.0).astype(np.float32)
+    f["H_ice"] = np.asfortranarray(H)
+    f["z_srf"] = np.asfortranarray(inputs.calc_z_srf_max(H, f["f_ice"], f["z_bed"], f["z_sl"],
+                                                         RHO_I, RHO_SW).astype(np.float32))
+    f["H_grnd"] = np.asfortranarray(inputs.calc_H_grnd(H, f["f_ice"], f["z_bed"], f["z_sl"],
+                                                       RHO_I, RHO_SW).astype(np.float32))
+    f["f_grnd"] = np.asfortranarray(np.where(f["H_grnd"] > 0, 1.0, 0.0).astype(np.float32))
+    return q
+
+
+def test_session_derives_inputs_on_device():
+    """A session under -ygpu_inputs device, given the geometry only (driving
+    stress, front stress and masks NULL), gives bit for bit the velocity of
+    a session given those inputs from ygpu_dyn_inputs, over two solves of a
+    sequence whose thickness changes between them."""
+    from yelmo_amd.velocity import Session
+    print("case session with device inputs 61^2", flush=True)
+    p0 = generator.make_problem(61, 24e3, seed=5, dtype=np.float32)
+    rng = np.random.default_rng(5)
+    dH = np.asfortranarray(10.0 * rng.standard_normal(p0.fields["H_ice"].shape)).astype(np.float32)
+    lim = "-ygpu_taud_lim 2e5 -ygpu_grad_lim 0.5 -ygpu_lat_bc all"
+    par = SsaParams(precision="mixed")
+    par_dev = par.with_(ssa_lis_opt=par.ssa_lis_opt + " -ygpu_inputs device " + lim)
+    seq = [p0, _thicken(p0, dH)]
+    sh = sd = None
+    for k, p in enumerate(seq):
+        d = calc_dyn_inputs(p.fields, par, p.dx, p.dy, 2e5, 0.5, "all")
+        ph, pd = _with(p, d), _without(p)
+        if k == 0:
+            sh, sd = Session(ph, par), Session(pd, par_dev)
+        ih, idv = sh.solve(ph), sd.solve(pd)
+        assert ih.status == 0 and idv.status == 0
+        assert ih.ssa_iter_now == idv.ssa_iter_now and ih.ssa_lin_iter == idv.ssa_lin_iter
+        for n in ("ux_b", "uy_b", "visc_eff_int", "beta_acx"):
+            assert np.array_equal(ph.state[n], pd.state[n]), (k, n)
+        print("solve", k, "picard", ih.ssa_iter_now, flush=True)
+    sh.close()
+    sd.close()
+
+
+def test_batch_derives_inputs_on_device():
+    from yelmo_amd.velocity import ResidentBatch
+    print("case batch with device inputs", flush=True)
+    probs = [generator.make_problem(n, 1536e3 / n, seed=n, dtype=np.float32) for n in (41, 57)]
+    par = SsaParams(precision="mixed")
+    par_dev = par.with_(ssa_lis_opt=par.ssa_lis_opt +
+                        " -ygpu_inputs device -ygpu_taud_lim 2e5 -ygpu_grad_lim 0.5")
+    ph = [_with(p, calc_dyn_inputs(p.fields, par, p.dx, p.dy, 2e5, 0.5, "all")) for p in probs]
+    pd = [_without(p) for p in probs]
+    for ps, pr in ((ph, par), (pd, par_dev)):
+        rb = ResidentBatch(ps, pr)
+        rb.run()
+        info = rb.download()
+        rb.close()
+        assert all(i.status == 0 for i in info)
+    for a, b in zip(ph, pd):
+        assert np.array_equal(a.state["ux_b"], b.state["ux_b"])
+        assert np.array_equal(a.state["uy_b"], b.state["uy_b"])

@@ -79,6 +79,22 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
       else { err = "unknown factor storage: " + std::string(v.p, v.n); return YGPU_ERR_ARG; }
     }
     else if (k.is("-ygpu_ny")) o.ny_hint = v.to_int();
+    else if (k.is("-ygpu_inputs")) {
+      if (v.is("device")) o.dev_inputs = 1;
+      else if (v.is("host")) o.dev_inputs = 0;
+      else { err = "-ygpu_inputs takes device or host"; return YGPU_ERR_ARG; }
+    }
+    else if (k.is("-ygpu_taud_lim")) o.taud_lim = v.to_double();
+    else if (k.is("-ygpu_grad_lim")) o.grad_lim = v.to_double();
+    else if (k.is("-ygpu_slope_bg_x")) o.slope_x = v.to_double();
+    else if (k.is("-ygpu_slope_bg_y")) o.slope_y = v.to_double();
+    else if (k.is("-ygpu_lat_bc")) {
+      if (v.is("none")) o.lat_bc = 0;
+      else if (v.is("floating") || v.is("float")) o.lat_bc = 1;
+      else if (v.is("marine")) o.lat_bc = 2;
+      else if (v.is("all")) o.lat_bc = 3;
+      else { err = "-ygpu_lat_bc takes none, floating, marine or all"; return YGPU_ERR_ARG; }
+    }
     else if (k.is("-ygpu_pattern")) {
       if (v.is("auto")) o.pattern = 0;
       else if (v.is("scalar")) o.pattern = 1;

@@ -59,6 +59,13 @@ struct Opts {
   const char* substituted = nullptr;   // solver named in -i that another one stands in for
   int fields32 = 1;   // mixed mode: field kernels in FP32 (-ygpu_fields fp32, the default)
   int pattern = 0;    // CSR pattern: 0 automatic, 1 scalar five-point, 2 C-grid SSA stencil
+  // -ygpu_inputs device: the driving stress, lateral front stress and SSA
+  // masks of each solve derived on the device from its geometry (z_srf
+  // required; the caller's arrays for them may be null and receive the
+  // derived values when given), with calc_ydyn's parameters
+  int dev_inputs = 0;
+  double taud_lim = 2e5, grad_lim = 0.5, slope_x = 0.0, slope_y = 0.0;
+  int lat_bc = 3;
 };
 
 int parse_opts(const char* s, Opts& o, std::string& err);
@@ -80,6 +87,20 @@ int scalar_csr_solve(ygpu_handle* h, int32_t n, int32_t nnz, const int32_t* a_pt
 // Field kernels in FP32: the reference mode (the reference's wp = sp) and
 // the mixed mode unless -ygpu_fields fp64; FP64 in the FP64 mode.
 bool fields32(const ygpu_ssa_params* par);
+// The per-solve inputs of one domain on the device (ygpu_inputs.hip): ice
+// front codes, surface slopes, driving stress, lateral front stress and SSA
+// face masks from the geometry.  Outputs sx, sy may be null.
+template <typename R>
+struct YgDynArgs {
+  int nx, ny, bnd, BC, lateral_bc;
+  R taud_lim, grad_lim, slope_x, slope_y, dx, dy, rho_ice, rho_sw, g;
+  const R *H, *fi, *fg, *zb, *zsl, *zs;
+  int* mf;
+  R *sx, *sy, *tdx, *tdy, *tlx, *tly;
+  int *mx, *my;
+};
+template <typename R>
+int launch_dyn_inputs(const YgDynArgs<R>& a, hipStream_t st);
 }  // namespace yg
 // looks up the device's no-allocation scratch array (ygpu_noalloc.hip); 0 ok
 int yg_noalloc_init(int device);
@@ -132,6 +153,10 @@ struct Batch {
   int* d_eticket = nullptr;    // epilogue arrival counters
   YgFields<R> f{};
   YgDivaFields<R> dv{};
+  // -ygpu_inputs device: the surface elevation and the front codes
+  R* d_zsrf = nullptr;
+  int* d_mf = nullptr;
+  double grav = 9.81;
 
   YgBatchGeom geom() const {
     YgBatchGeom G;
@@ -173,6 +198,7 @@ struct Batch {
     fp.beta_min = par->beta_min; fp.eps_0 = par->eps_0; fp.vel_lim_tau = par->ssa_vel_lim_tau;
     fp.vel_max = par->ssa_vel_max; fp.iter_rel = par->ssa_iter_rel; fp.iter_conv = par->ssa_iter_conv;
     fp.rho_ice = par->rho_ice; fp.rho_sw = par->rho_sw;
+    grav = par->g;
     if (par->visc_method < 0 || par->visc_method > 2 || par->beta_method < -1 || par->beta_method > 5 ||
         par->beta_gl_scale < 0 || par->beta_gl_scale > 3 ||
         par->beta_gl_stag < -1 || par->beta_gl_stag > 4) {
@@ -371,6 +397,10 @@ struct Batch {
     f.zstride = zstride;
     YG_CHECK(h, ar.alloc((int32_t**)&f.mx, C));
     YG_CHECK(h, ar.alloc((int32_t**)&f.my, C));
+    if (opts.dev_inputs) {
+      YG_CHECK(h, ar.alloc(&d_zsrf, C));
+      YG_CHECK(h, ar.alloc(&d_mf, C));
+    }
     if (fp.energy) {
       YG_CHECK(h, ar.alloc(&f.cfrom, 2 * C));
       YG_CHECK(h, ar.alloc(&f.asm_err, nmem));
@@ -455,9 +485,14 @@ struct Batch {
     for (int k = 0; k < nmem; ++k) rb[k] = probs[k].real_bytes;
     int rc;
 #define UP(dst, member) if ((rc = up2((R*)(dst), YG_FIELD(member).data(), 0, false))) return rc;
-    UP(f.c_bed, c_bed) UP(f.f_slide, f_slide) UP(f.taud_acx, taud_acx) UP(f.taud_acy, taud_acy)
-    UP(f.taul_acx, taul_int_acx) UP(f.taul_acy, taul_int_acy) UP(f.H_ice, H_ice) UP(f.f_ice, f_ice)
+    UP(f.c_bed, c_bed) UP(f.f_slide, f_slide) UP(f.H_ice, H_ice) UP(f.f_ice, f_ice)
     UP(f.H_grnd, H_grnd) UP(f.f_grnd, f_grnd) UP(f.z_bed, z_bed) UP(f.z_sl, z_sl)
+    if (opts.dev_inputs) {
+      UP(d_zsrf, z_srf)
+    } else {
+      UP(f.taud_acx, taud_acx) UP(f.taud_acy, taud_acy)
+      UP(f.taul_acx, taul_int_acx) UP(f.taul_acy, taul_int_acy)
+    }
 #undef UP
     // the faces' grounded fraction (read by the subgrid staggering only)
     if ((rc = up2((R*)f.f_grnd_acx, YG_FIELD(f_grnd_acx).data(), 0, true))) return rc;
@@ -472,6 +507,7 @@ struct Batch {
                                                  : (R)((const double*)probs[k].zeta_aa)[q];
     }
     YG_CHECK(h, hipMemcpyAsync((R*)f.zeta, zs.data(), sizeof(R) * zs.size(), hipMemcpyHostToDevice, h->stream));
+    if (opts.dev_inputs) return derive_inputs(probs);
     // masks
     std::vector<int32_t> mxs(total_cells), mys(total_cells);
     for (int k = 0; k < nmem; ++k) {
@@ -481,6 +517,31 @@ struct Batch {
     }
     YG_CHECK(h, hipMemcpyAsync((int32_t*)f.mx, mxs.data(), sizeof(int32_t) * total_cells, hipMemcpyHostToDevice, h->stream));
     YG_CHECK(h, hipMemcpyAsync((int32_t*)f.my, mys.data(), sizeof(int32_t) * total_cells, hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    return YGPU_OK;
+  }
+
+  // -ygpu_inputs device: calc_ydyn's inputs of every member (driving stress,
+  // lateral front stress, SSA face masks; yelmo_dynamics.f90:105-122,464)
+  // from the resident geometry, in place of uploaded ones
+  int derive_inputs(const ygpu_ssa_problem* probs) {
+    for (int k = 0; k < nmem; ++k) {
+      const long long c0 = coffs[k];
+      YgDynArgs<R> a{};
+      a.nx = probs[k].nx; a.ny = probs[k].ny; a.bnd = fp.bnd; a.BC = fp.BC;
+      a.lateral_bc = opts.lat_bc;
+      a.taud_lim = (R)opts.taud_lim; a.grad_lim = (R)opts.grad_lim;
+      a.slope_x = (R)opts.slope_x; a.slope_y = (R)opts.slope_y;
+      a.dx = (R)probs[k].dx; a.dy = (R)probs[k].dy;
+      a.rho_ice = (R)fp.rho_ice; a.rho_sw = (R)fp.rho_sw; a.g = (R)grav;
+      a.H = f.H_ice + c0; a.fi = f.f_ice + c0; a.fg = f.f_grnd + c0; a.zb = f.z_bed + c0;
+      a.zsl = f.z_sl + c0; a.zs = d_zsrf + c0;
+      a.mf = d_mf + c0; a.sx = nullptr; a.sy = nullptr;
+      a.tdx = (R*)f.taud_acx + c0; a.tdy = (R*)f.taud_acy + c0;
+      a.tlx = (R*)f.taul_acx + c0; a.tly = (R*)f.taul_acy + c0;
+      a.mx = (int*)f.mx + c0; a.my = (int*)f.my + c0;
+      if (launch_dyn_inputs<R>(a, h->stream)) { h->err = "per-solve input kernels failed"; return YGPU_ERR_DEVICE; }
+    }
     YG_CHECK(h, hipStreamSynchronize(h->stream));
     return YGPU_OK;
   }

@@ -26,6 +26,7 @@
 
 using yg::Arena;
 using yg::bc_code;
+using yg::YgDynArgs;
 
 namespace {
 
@@ -36,14 +37,6 @@ namespace {
 #define YGI_FRNT_MARINE 2
 #define YGI_FRNT_GRND 3
 
-struct InArgs {
-  int nx, ny, bnd, BC, lateral_bc;
-  float taud_lim, grad_lim, slope_x, slope_y, dx, dy, rho_ice, rho_sw, g;
-  const float *H, *fi, *fg, *zb, *zsl, *zs;
-  int* mf;
-  float *sx, *sy, *tdx, *tdy, *tlx, *tly;
-  int *mx, *my;
-};
 
 __device__ __forceinline__ void inbr(int i, int j, int nx, int ny, int BC, int& im1, int& ip1,
                                      int& jm1, int& jp1) {
@@ -58,7 +51,8 @@ __device__ __forceinline__ void inbr(int i, int j, int nx, int ny, int BC, int& 
 
 // the (unclipped) difference quotient of z_srf on the face after cell (i, j)
 // in direction dir, before the border copy
-__device__ __forceinline__ float face_diff(const InArgs& a, int i, int j, int dir) {
+template <typename R>
+__device__ __forceinline__ R face_diff(const yg::YgDynArgs<R>& a, int i, int j, int dir) {
 #pragma clang fp contract(off)
   int im1, ip1, jm1, jp1;
   inbr(i, j, a.nx, a.ny, a.BC, im1, ip1, jm1, jp1);
@@ -69,7 +63,8 @@ __device__ __forceinline__ float face_diff(const InArgs& a, int i, int j, int di
 
 // calc_gradient_acx/acy with zero_outside = .false. (the z_srf slopes of
 // yelmo_topography.f90:1373-1374): border copy, background slope, clip
-__device__ __forceinline__ float slope(const InArgs& a, int i, int j, int dir) {
+template <typename R>
+__device__ __forceinline__ R slope(const yg::YgDynArgs<R>& a, int i, int j, int dir) {
 #pragma clang fp contract(off)
   const int b = a.bnd;
   if (dir == 0) {
@@ -82,21 +77,23 @@ __device__ __forceinline__ float slope(const InArgs& a, int i, int j, int dir) {
     if (copy && j == 0) j = 1;
     else if (copy && j == a.ny - 1) j = a.ny - 2;
   }
-  float v = face_diff(a, i, j, dir) + (dir == 0 ? a.slope_x : a.slope_y);
+  R v = face_diff<R>(a, i, j, dir) + (dir == 0 ? a.slope_x : a.slope_y);
   if (v > a.grad_lim) v = a.grad_lim;
   if (v < -a.grad_lim) v = -a.grad_lim;
   return v;
 }
 
-__device__ __forceinline__ float hmid(float Ha, float Hb, float fa, float fb) {
+template <typename R>
+__device__ __forceinline__ R hmid(R Ha, R Hb, R fa, R fb) {
 #pragma clang fp contract(off)
-  if (fa == 1.0f && fb < 1.0f) return 0.5f * (Ha + 0.0f);
-  if (fa < 1.0f && fb == 1.0f) return 0.5f * (0.0f + Hb);
-  return 0.5f * (Ha + Hb);
+  if (fa == R(1) && fb < R(1)) return R(0.5) * (Ha + R(0));
+  if (fa < R(1) && fb == R(1)) return R(0.5) * (R(0) + Hb);
+  return R(0.5) * (Ha + Hb);
 }
 
 // ice front codes, slopes and driving stress
-__global__ void k_front_slope_taud(InArgs a) {
+template <typename R>
+__global__ void k_front_slope_taud(yg::YgDynArgs<R> a) {
 #pragma clang fp contract(off)
   const long long n = (long long)a.nx * a.ny;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
@@ -107,29 +104,29 @@ __global__ void k_front_slope_taud(InArgs a) {
     const long long nb[4] = {im1 + (long long)a.nx * j, ip1 + (long long)a.nx * j,
                              i + (long long)a.nx * jm1, i + (long long)a.nx * jp1};
     int code = YGI_FRNT_NONE;
-    if (a.fi[c] == 1.0f) {
+    if (a.fi[c] == R(1)) {
       int nopen = 0;
-      for (int k = 0; k < 4; ++k) nopen += a.fi[nb[k]] < 1.0f;
+      for (int k = 0; k < 4; ++k) nopen += a.fi[nb[k]] < R(1);
       if (nopen > 0) {
-        if (a.fg[c] > 0.0f && a.zsl[c] <= a.zb[c]) code = YGI_FRNT_GRND;
-        else if (a.fg[c] > 0.0f) code = YGI_FRNT_MARINE;
+        if (a.fg[c] > R(0) && a.zsl[c] <= a.zb[c]) code = YGI_FRNT_GRND;
+        else if (a.fg[c] > R(0)) code = YGI_FRNT_MARINE;
         else code = YGI_FRNT_FLOAT;
       }
-    } else if (a.fi[c] < 1.0f) {
+    } else if (a.fi[c] < R(1)) {
       bool beside = false;
-      for (int k = 0; k < 4; ++k) beside |= nb[k] != c && a.fi[nb[k]] == 1.0f;
+      for (int k = 0; k < 4; ++k) beside |= nb[k] != c && a.fi[nb[k]] == R(1);
       if (beside) code = a.zb[c] < a.zsl[c] ? YGI_FRNT_ICE_FREE : YGI_FRNT_ICE_FREE_LAND;
     }
     a.mf[c] = code;
-    const float gx = slope(a, i, j, 0), gy = slope(a, i, j, 1);
-    a.sx[c] = gx;
-    a.sy[c] = gy;
-    const float rhog = a.rho_ice * a.g;
-    float t = rhog * hmid(a.H[c], a.H[nb[1]], a.fi[c], a.fi[nb[1]]) * gx;
+    const R gx = slope<R>(a, i, j, 0), gy = slope<R>(a, i, j, 1);
+    if (a.sx) a.sx[c] = gx;
+    if (a.sy) a.sy[c] = gy;
+    const R rhog = a.rho_ice * a.g;
+    R t = rhog * hmid<R>(a.H[c], a.H[nb[1]], a.fi[c], a.fi[nb[1]]) * gx;
     if (t > a.taud_lim) t = a.taud_lim;
     if (t < -a.taud_lim) t = -a.taud_lim;
     a.tdx[c] = t;
-    t = rhog * hmid(a.H[c], a.H[nb[3]], a.fi[c], a.fi[nb[3]]) * gy;
+    t = rhog * hmid<R>(a.H[c], a.H[nb[3]], a.fi[c], a.fi[nb[3]]) * gy;
     if (t > a.taud_lim) t = a.taud_lim;
     if (t < -a.taud_lim) t = -a.taud_lim;
     a.tdy[c] = t;
@@ -138,15 +135,15 @@ __global__ void k_front_slope_taud(InArgs a) {
 
 // calc_lateral_bc_stress with the reference's kinds (wp = sp values, the
 // double literals 1.d0 and 0.5d0)
-__device__ __forceinline__ float lateral_stress(float H, float zs, float zsl, float rho_ice,
-                                                float rho_sw, float g) {
+template <typename R>
+__device__ __forceinline__ R lateral_stress(R H, R zs, R zsl, R rho_ice, R rho_sw, R g) {
 #pragma clang fp contract(off)
   const double q = (double)((zs - zsl) / H);
-  const float f_sub = (float)(1.0 - (q < 1.0 ? q : 1.0));
-  const float H_ocn = H * f_sub;
-  const float H2 = H * H, O2 = H_ocn * H_ocn;
-  return (float)(0.5 * (double)rho_ice * (double)g * (double)H2 -
-                 0.5 * (double)rho_sw * (double)g * (double)O2);
+  const R f_sub = (R)(1.0 - (q < 1.0 ? q : 1.0));
+  const R H_ocn = H * f_sub;
+  const R H2 = H * H, O2 = H_ocn * H_ocn;
+  return (R)(0.5 * (double)rho_ice * (double)g * (double)H2 -
+             0.5 * (double)rho_sw * (double)g * (double)O2);
 }
 
 // front_face_mask (solver_ssa_ac.f90:950-996); lateral_bc 0 none,
@@ -170,7 +167,8 @@ __device__ __forceinline__ int face_code(int a, int b, int lateral_bc) {
 }
 
 // lateral front stress and SSA face masks of the faces after each cell
-__global__ void k_taul_masks(InArgs a) {
+template <typename R>
+__global__ void k_taul_masks(yg::YgDynArgs<R> a) {
 #pragma clang fp contract(off)
   const long long n = (long long)a.nx * a.ny;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
@@ -181,18 +179,18 @@ __global__ void k_taul_masks(InArgs a) {
     for (int dir = 0; dir < 2; ++dir) {
       const long long q = dir == 0 ? ip1 + (long long)a.nx * j : i + (long long)a.nx * jp1;
       const int ma = a.mf[c], mb = a.mf[q];
-      float t = 0.0f;
+      R t = R(0);
       if ((ma > 0 && mb < 0) || (ma < 0 && mb > 0)) {
         const long long k = ma < 0 ? q : c;   // the ice-covered cell
-        const float H = a.H[k], zs = a.zs[k];
-        float zsl = a.zsl[k];
+        const R H = a.H[k], zs = a.zs[k];
+        R zsl = a.zsl[k];
         if (ma == YGI_FRNT_ICE_FREE_LAND || mb == YGI_FRNT_ICE_FREE_LAND) zsl = zs - H;
-        t = lateral_stress(H, zs, zsl, a.rho_ice, a.rho_sw, a.g);
+        t = lateral_stress<R>(H, zs, zsl, a.rho_ice, a.rho_sw, a.g);
       }
       // set_ssa_masks: active faces by ice cover and grounding, then the
       // front faces' codes
       int m = 0;
-      if (a.fi[c] == 1.0f || a.fi[q] == 1.0f) m = (a.fg[c] > 0.0f || a.fg[q] > 0.0f) ? 1 : 2;
+      if (a.fi[c] == R(1) || a.fi[q] == R(1)) m = (a.fg[c] > R(0) || a.fg[q] > R(0)) ? 1 : 2;
       const int ml = face_code(ma, mb, a.lateral_bc);
       if (ml >= 0) m = ml;
       if (dir == 0) { a.tlx[c] = t; a.mx[c] = m; }
@@ -202,6 +200,24 @@ __global__ void k_taul_masks(InArgs a) {
 }
 
 }  // namespace
+
+namespace yg {
+
+template <typename R>
+int launch_dyn_inputs(const YgDynArgs<R>& a, hipStream_t st) {
+  const long long n = (long long)a.nx * a.ny;
+  const int threads = 256;
+  const int blocks = (int)std::min<long long>((n + threads - 1) / threads, 65535);
+  hipLaunchKernelGGL(k_front_slope_taud<R>, dim3(blocks), dim3(threads), 0, st, a);
+  if (hipGetLastError() != hipSuccess) return YGPU_ERR_DEVICE;
+  hipLaunchKernelGGL(k_taul_masks<R>, dim3(blocks), dim3(threads), 0, st, a);
+  if (hipGetLastError() != hipSuccess) return YGPU_ERR_DEVICE;
+  return YGPU_OK;
+}
+template int launch_dyn_inputs<float>(const YgDynArgs<float>&, hipStream_t);
+template int launch_dyn_inputs<double>(const YgDynArgs<double>&, hipStream_t);
+
+}  // namespace yg
 
 extern "C" {
 
@@ -233,7 +249,7 @@ int ygpu_dyn_inputs(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int3
   const float* src[6] = {H_ice, f_ice, f_grnd, z_bed, z_sl, z_srf};
   for (int k = 0; k < 6; ++k)
     YG_CHECK(h, hipMemcpyAsync(d_in + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, st));
-  InArgs a{};
+  yg::YgDynArgs<float> a{};
   a.nx = nx; a.ny = ny; a.bnd = par->boundaries; a.BC = bc_code(par->boundaries);
   a.lateral_bc = lateral_bc;
   a.taud_lim = taud_lim; a.grad_lim = grad_lim; a.slope_x = slope_bg_x; a.slope_y = slope_bg_y;
@@ -244,12 +260,10 @@ int ygpu_dyn_inputs(ygpu_handle* h, const ygpu_ssa_params* par, int32_t nx, int3
   a.sx = d_out; a.sy = d_out + n; a.tdx = d_out + 2 * n; a.tdy = d_out + 3 * n;
   a.tlx = d_out + 4 * n; a.tly = d_out + 5 * n;
   a.mf = d_int; a.mx = d_int + n; a.my = d_int + 2 * n;
-  const int threads = 256;
-  const int blocks = (int)std::min<size_t>((n + threads - 1) / threads, 65535);
-  hipLaunchKernelGGL(k_front_slope_taud, dim3(blocks), dim3(threads), 0, st, a);
-  YG_CHECK(h, hipGetLastError());
-  hipLaunchKernelGGL(k_taul_masks, dim3(blocks), dim3(threads), 0, st, a);
-  YG_CHECK(h, hipGetLastError());
+  if (yg::launch_dyn_inputs<float>(a, st)) {
+    h->err = "ygpu_dyn_inputs: kernel launch failed";
+    return YGPU_ERR_DEVICE;
+  }
   float* fo[6] = {dzsdx, dzsdy, taud_acx, taud_acy, taul_int_acx, taul_int_acy};
   for (int k = 0; k < 6; ++k)
     if (fo[k]) YG_CHECK(h, hipMemcpyAsync(fo[k], d_out + k * n, n * sizeof(float), hipMemcpyDeviceToHost, st));

@@ -263,6 +263,10 @@ struct Server : YgServerBase {
     h = hh;
     int rc = B.setup_params(hh, par, diva);
     if (rc) return rc;
+    if (B.opts.dev_inputs) {
+      h->err = "-ygpu_inputs device: batches, sessions and one-shot calls only";
+      return YGPU_ERR_UNSUPPORTED;
+    }
     if (nlanes < 1 || nlanes > 8 || !lane_side || !lane_slots || nz_max < 2 || budget <= 0) {
       h->err = "server: bad lane list, nz or budget";
       return YGPU_ERR_ARG;

@@ -58,10 +58,18 @@ def _fill_struct(prob: SsaProblem, diva: bool) -> tuple[_lib.SsaProblemC, list]:
     for n in _lib.PROBLEM_PTRS_INOUT + _lib.PROBLEM_PTRS_ERR:
         st[n] = arr(st[n])
         setattr(s, n, st[n].ctypes.data)
+    # fields and masks set to None pass NULL (what the library derives
+    # itself, e.g. under -ygpu_inputs device)
     for n in ("ssa_mask_acx", "ssa_mask_acy"):
+        if st.get(n) is None:
+            setattr(s, n, None)
+            continue
         st[n] = arr(st[n], np.int32)
         setattr(s, n, st[n].ctypes.data)
     for n in _lib.PROBLEM_PTRS_IN:
+        if fl.get(n) is None:
+            setattr(s, n, None)
+            continue
         a = arr(fl[n])
         setattr(s, n, a.ctypes.data)
     if diva:
