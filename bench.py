@@ -60,6 +60,54 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def baseline_legs(args, base, h):
+    """Fairer comparisons for the value: (1) the plain FP64 path with every
+    CU busy (4 copies of the ensemble, one workgroup per member on 256 CUs;
+    the 64-member FP64 batch leaves 192 CUs idle); (2) Yelmo's production
+    defaults (&ydyn: energy assembly, CG + Jacobi at rtol 1e-2, Picard
+    tolerance 1e-2), batched, FP64 and mixed."""
+    from yelmo_amd.params import SsaParams
+    from yelmo_amd.velocity import ResidentBatch
+    out = {}
+    par64 = SsaParams(precision="fp64")
+    probs = [p.astype(np.float64) for p in base for _ in range(4)]
+    rb = ResidentBatch(probs, par64, handle=h)
+    rb.run()
+    _sync()
+    t0 = time.perf_counter()
+    rb.run()
+    _sync()
+    t = time.perf_counter() - t0
+    rb.close()
+    out["fp64_full_device"] = {"members": len(probs), "solves_per_s": len(probs) / t,
+                               "note": "4 copies of the ensemble, one workgroup per member"}
+    print(f"# fp64 full device ({len(probs)} members): {len(probs) / t:.1f} solves/s",
+          file=sys.stderr, flush=True)
+    del probs
+    prod = {}
+    for prec in ("fp64", "mixed"):
+        par = SsaParams.yelmo_defaults(precision=prec)
+        ps = [p.astype(np.float64) if prec == "fp64" else p.copy() for p in base]
+        rb = ResidentBatch(ps, par, handle=h)
+        rb.run()
+        _sync()
+        t0 = time.perf_counter()
+        rb.run()
+        _sync()
+        t = time.perf_counter() - t0
+        info = rb.download()
+        rb.close()
+        prod[prec] = {"solves_per_s": len(ps) / t,
+                      "picard_iters_mean": float(np.mean([i.ssa_iter_now for i in info])),
+                      "krylov_iters_per_solve": float(np.mean([i.ssa_lin_iter for i in info])),
+                      "failed": sum(1 for i in info if i.status != 0)}
+        print(f"# production defaults (energy, CG + Jacobi 1e-2) {prec}: "
+              f"{len(ps) / t:.1f} solves/s", file=sys.stderr, flush=True)
+    out["production_defaults"] = {"solver": SsaParams.yelmo_defaults().ssa_lis_opt,
+                                  "form": "energy", **prod}
+    return out
+
+
 def fp64_paths(args, base, par_fp64, h, vel):
     """The FP64 GPU path on the ensemble: the converged error reference, the
     batched run at the reference's solver settings and the single-member
@@ -569,6 +617,8 @@ def main():
     ap.add_argument("--single-sample", type=int, default=4,
                     help="members solved one at a time on the FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-baselines", action="store_true",
+                    help="skip the full-device FP64 and production-default legs")
     ap.add_argument("--factor", choices=("fp32", "bf16", "bf16u"), default="fp32",
                     help="storage of the mixed path's ILU(0) factors: bf16 (with the U rows "
                          "scaled by their FP32 inverse diagonals), bf16u (the U part in BF16, "
@@ -703,6 +753,9 @@ def main():
     served = None
     if not args.mixed_only and args.serve_requests > 0:
         served = served_leg(args)
+    baselines = None
+    if not args.mixed_only and not args.no_baselines:
+        baselines = baseline_legs(args, base, h)
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.members * args.steps / elapsed
     fp64_batch_sps = args.members / t_fp64_batch
@@ -761,6 +814,11 @@ def main():
         "fp64_single_solves_per_s": fp64_single_sps,
         "speedup_vs_fp64_single": value / fp64_single_sps,
         "speedup_vs_fp64_batched": value / fp64_batch_sps,
+        # the plain FP64 path with all CUs busy, and Yelmo's production
+        # defaults (energy form, CG + Jacobi at rtol 1e-2)
+        "baselines": baselines,
+        "speedup_vs_fp64_full_device": (None if baselines is None else
+                                        value / baselines["fp64_full_device"]["solves_per_s"]),
         "eager_stats_solves_per_s": eager_sps,
         "step_latency_ms": {"p50": 1e3 * float(np.percentile(step_times, 50)),
                             "p99": 1e3 * float(np.percentile(step_times, 99))},

@@ -1489,6 +1489,82 @@ YG_DEV void vmap(int lo, int hi, const T* const (&in)[NI], T* const (&out)[NO], 
   }
 }
 
+// The refinement's passes over value pairs [lo, hi) (lo, hi even; the
+// vectors start 16-byte aligned): eight pairs per thread per trip, every
+// load of a trip issued before the first is used (a plain loop waits for
+// each load in turn, about a memory latency per element of a thread).
+constexpr int YG_RU = 8;
+
+// rin = (float)(r64 sc), d = 0
+YG_DEV void refine_scale(const double* __restrict__ r64, float* __restrict__ rin,
+                         float* __restrict__ d, double sc, int lo, int hi) {
+  const int p0 = lo >> 1, p1 = hi >> 1, B = blockDim.x;
+  for (int k0 = p0 + (int)threadIdx.x; k0 < p1; k0 += YG_RU * B) {
+    double2 v[YG_RU];
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) v[u] = reinterpret_cast<const double2*>(r64)[k];
+    }
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) {
+        float2 o;
+        o.x = (float)(v[u].x * sc);
+        o.y = (float)(v[u].y * sc);
+        reinterpret_cast<float2*>(rin)[k] = o;
+        reinterpret_cast<float2*>(d)[k] = make_float2(0.0f, 0.0f);
+      }
+    }
+  }
+}
+
+// x64 += rn d
+YG_DEV void refine_update(double* __restrict__ x64, const float* __restrict__ d, double rn,
+                          int lo, int hi) {
+  const int p0 = lo >> 1, p1 = hi >> 1, B = blockDim.x;
+  for (int k0 = p0 + (int)threadIdx.x; k0 < p1; k0 += YG_RU * B) {
+    double2 x[YG_RU];
+    float2 dv[YG_RU];
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) {
+        x[u] = reinterpret_cast<const double2*>(x64)[k];
+        dv[u] = reinterpret_cast<const float2*>(d)[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) {
+        double2 o;
+        o.x = x[u].x + rn * (double)dv[u].x;
+        o.y = x[u].y + rn * (double)dv[u].y;
+        reinterpret_cast<double2*>(x64)[k] = o;
+      }
+    }
+  }
+}
+
+// number of nonzero values of b64 in [lo, hi) (this thread's share)
+YG_DEV double count_nonzero(const double* __restrict__ b64, int lo, int hi) {
+  const int p0 = lo >> 1, p1 = hi >> 1, B = blockDim.x;
+  double n = 0.0;
+  for (int k0 = p0 + (int)threadIdx.x; k0 < p1; k0 += YG_RU * B) {
+    double2 v[YG_RU];
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      v[u] = k < p1 ? reinterpret_cast<const double2*>(b64)[k] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) n += (v[u].x != 0.0 ? 1.0 : 0.0) + (v[u].y != 0.0 ? 1.0 : 0.0);
+  }
+  return n;
+}
+
 // BiCGSTAB over the rows [c.cb, c.ce) of the member.  GRP: a member group
 // (the rows are this block's share; every sum is a group sum and a group
 // barrier precedes each SpMV whose x other blocks wrote).  GRP requires
@@ -1751,7 +1827,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
   // b == 0 shortcut (solver_linear.F90:162-174)
   {
     double bm[1] = {0.0};
-    for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) bm[0] += b64[q] != 0.0 ? 1.0 : 0.0;
+    bm[0] = count_nonzero(b64, 0, 2 * ncell);
     yg_block_sum<1>(bm, yg_red);
     if (bm[0] == 0.0) {
       for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] = 0.0;
@@ -1828,15 +1904,12 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
       if (res <= target) break;
       if (iters >= o.maxiter) break;
       const double sc = rn > 0.0 ? 1.0 / rn : 1.0;
-      for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) {
-        rin[q] = (float)(r64[q] * sc);
-        d[q] = 0.0f;
-      }
+      refine_scale(r64, rin, d, sc, 0, 2 * ncell);
       __syncthreads();
       oi.maxiter = o.maxiter - iters;
       SolveOut so = krylov<float>(c, A32, P, rin, d, W, oi);
       iters += so.iters;
-      for (int q = threadIdx.x; q < 2 * ncell; q += blockDim.x) x64[q] += rn * (double)d[q];
+      refine_update(x64, d, rn, 0, 2 * ncell);
       __syncthreads();
       if (so.status == 2 && so.iters <= 1) { status = 2; break; }
     }
@@ -1972,7 +2045,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     // b == 0 shortcut (solver_linear.F90:162-174)
     {
       double bm[1] = {0.0};
-      for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) bm[0] += b64[q] != 0.0 ? 1.0 : 0.0;
+      bm[0] = count_nonzero(b64, lo, hi);
       csum<1>(cm, bm);
       if (bm[0] == 0.0) {
         for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) x64[q] = 0.0;
@@ -2031,15 +2104,12 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
       if (res <= target) break;
       if (iters >= o.maxiter) break;
       const double sc = rn > 0.0 ? 1.0 / rn : 1.0;
-      for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) {
-        rin[q] = (float)(r64[q] * sc);
-        d[q] = 0.0f;
-      }
+      refine_scale(r64, rin, d, sc, lo, hi);
       __syncthreads();
       oi.maxiter = o.maxiter - iters;
       SolveOut so = bicgstab<float, float, true>(cm, A32, P, rin, d, W, oi);
       iters += so.iters;
-      for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) x64[q] += rn * (double)d[q];
+      refine_update(x64, d, rn, lo, hi);
       __syncthreads();
       if (so.status == 2 && so.iters <= 1) { status = 2; break; }
     }
