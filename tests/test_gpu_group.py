@@ -142,3 +142,35 @@ def test_column_tiles_within_mixed_bar(G, cols):
             print(fac, k, "picard/krylov/status", infos[k], "rel L2", err, flush=True)
             assert infos[k][2] == 0
             assert err <= 1e-6
+
+
+@pytest.mark.parametrize("G,cols", [(1, 1), (4, 1), (4, 3), (8, 1)])
+def test_reversed_columns_within_mixed_bar(G, cols):
+    """Block-Jacobi ILU(0) with the columns of every line block (or tile)
+    numbered in reverse (-ygpu_ilu_order reverse: a skew-1 wavefront), FP32
+    factors, one workgroup per member and member groups: every member meets
+    the refinement criterion within the mixed bar of the converged FP64
+    oracle.  With BF16 factors the option is ignored (natural order), so the
+    result equals the natural order's bit for bit."""
+    print(f"case reversed G={G} cols={cols}", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    sides = ((161, 8e3, 5), (113, 12e3, 8), (41, 32e3, 9))
+    probs = [generator.make_problem(n, dx, seed=s, dtype=np.float64) for n, dx, s in sides]
+    base = OPT + f" -ygpu_group {G} -ygpu_ilu_cols {cols}"
+    (states, infos), = _batch([p.copy() for p in probs], par,
+                              base + " -ygpu_factor fp32 -ygpu_ilu_order reverse")
+    (nat_states, nat_infos), = _batch([p.copy() for p in probs], par, base + " -ygpu_factor fp32")
+    for k, p in enumerate(probs):
+        ref_st, _ = oracle.calc_velocity_ssa(p.copy(), _converged(par), dp=True)
+        err = vel_rel_l2(states[k], ref_st)
+        print(k, "picard/krylov/status", infos[k], "natural", nat_infos[k], "rel L2", err,
+              flush=True)
+        assert infos[k][2] == 0
+        assert err <= 1e-6
+    (b_rev, i_rev), = _batch([p.copy() for p in probs[:2]], par,
+                             base + " -ygpu_factor bf16 -ygpu_ilu_order reverse")
+    (b_nat, i_nat), = _batch([p.copy() for p in probs[:2]], par, base + " -ygpu_factor bf16")
+    assert i_rev == i_nat
+    for k in range(2):
+        for n in STATE:
+            assert np.array_equal(b_rev[k][n], b_nat[k][n]), (k, n)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from tests._helpers import rel_l2, vel_rel_l2
+from tests._helpers import rel_l2, reverse_line_columns, vel_rel_l2
 from yelmo_amd import generator, velocity
 from yelmo_amd.params import SsaParams
 
@@ -118,6 +118,22 @@ def test_csr_solve_matches_oracle():
         assert lgs.status == ref["status"] == 0
         assert abs(lgs.lin_iter - ref["iters"]) <= 1
         assert err <= 1e-8
+    # block-Jacobi ILU(0) with every line's columns numbered in reverse
+    # (-ygpu_ilu_order reverse) is the oracle's block ILU(0) of the permuted
+    # system P A P^T, whose solution maps back through P
+    opts = base.format(p="ilu", m=1000, z="false")
+    pp, pi, pv, perm = reverse_line_columns(ap, ai, av, 81)
+    bp = np.empty_like(b); bp[perm] = b
+    x0p = np.empty_like(x0); x0p[perm] = x0
+    ref = oracle.csr_solve(pp, pi, pv, bp, x0p, opts + " -ilu_blocks 3")
+    lgs = velocity.Lgs(ap, ai, av, b, x0.copy())
+    velocity.linear_solver_matrix_solve(lgs, opts + " -ygpu_ilu_blocks 3 -ygpu_ilu_order reverse")
+    err = rel_l2(lgs.x_value, ref["x"][perm])
+    print("reversed iters", lgs.lin_iter, ref["iters"], "status", lgs.status, ref["status"],
+          "err", err)
+    assert lgs.status == ref["status"] == 0
+    assert abs(lgs.lin_iter - ref["iters"]) <= 1
+    assert err <= 1e-8
     # mixed precision (FP32 block ILU(0) inside FP64 refinement) meets the
     # same FP64 residual criterion.  Its solution is compared with the
     # oracle's solve converged to rtol 1e-12 (the oracle's own rtol 1e-8

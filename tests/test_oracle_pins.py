@@ -13,7 +13,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spl
 
 from oracle import oracle
-from tests._helpers import rel_l2
+from tests._helpers import rel_l2, reverse_line_columns
 from yelmo_amd import generator, inputs
 from yelmo_amd.params import SsaParams
 
@@ -182,6 +182,42 @@ def test_sparse_direct_and_krylov_agree():
     assert z["iters"] == 0 and z["status"] == 0 and not z["x"].any()
     m = oracle.csr_solve(ap, ai, av, b, x0, "-i bicgstab -p none -maxiter 2 -tol 1e-12")
     assert m["status"] == 4 and m["iters"] == 2
+
+
+def test_reverse_line_columns_is_a_symmetric_permutation():
+    """The reversed-column system (the block ILU(0) ordering of
+    -ygpu_ilu_order reverse) is P A P^T: (P A P^T)(P x) = P (A x) for a random
+    x, every line keeps its own rows, and Jacobi-preconditioned BiCGSTAB (an
+    ordering-invariant iteration) takes the same steps on it and returns the
+    permuted solution."""
+    par = SsaParams(visc_method=2, beta_method=4, ssa_iter_max=3)
+    p = generator.make_problem(21, 64e3, seed=4, dtype=np.float64)
+    st, _ = oracle.calc_velocity_ssa(p, par, dp=True)
+    q = p.copy()
+    q.state.update(ux_b=st["ux_b"], uy_b=st["uy_b"])
+    ap, ai, av, b, x0, _ = oracle.assemble(q, st["beta_acx"], st["beta_acy"], st["visc_eff_int"])
+    pp, pi, pv, perm = reverse_line_columns(ap, ai, av, 21)
+    n = len(b)
+    assert sorted(perm) == list(range(n))
+    # cell 0 of line 1 (unknowns 42, 43) takes position 20 of that line
+    assert perm[42] == 2 * (21 + 20) and perm[43] == 2 * (21 + 20) + 1
+    assert np.all(perm // 42 == np.arange(n) // 42)
+    A, Ap = _csr(ap, ai, av), _csr(pp, pi, pv)
+    x = np.random.default_rng(5).standard_normal(n)
+    xp = np.empty(n); xp[perm] = x
+    yp = np.empty(n); yp[perm] = A @ x
+    assert np.allclose(Ap @ xp, yp, rtol=0, atol=1e-12 * np.abs(yp).max())
+    for row in range(n):   # columns sorted within every row
+        c = pi[pp[row] - 1:pp[row + 1] - 1]
+        assert np.all(np.diff(c) > 0)
+    opts = "-i bicgstab -p jacobi -maxiter 5000 -tol 1.0e-10"
+    r = oracle.csr_solve(ap, ai, av, b, x0, opts)
+    bp = np.empty(n); bp[perm] = b
+    x0p = np.empty(n); x0p[perm] = x0
+    rp = oracle.csr_solve(pp, pi, pv, bp, x0p, opts)
+    assert r["status"] == rp["status"] == 0
+    assert abs(r["iters"] - rp["iters"]) <= 1
+    assert rel_l2(rp["x"][perm], r["x"]) < 1e-8
 
 
 def test_ilu0_pattern_identity_and_exactness():

@@ -30,6 +30,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.ir_max = 60; o.k.tol = 1e-12; o.k.inner_tol = 1e-3; o.k.ilu_blocks = -1; o.k.ir_target = 0.0;
   o.k.ilu_cols = 1;
   o.k.factor16 = 0; o.k.graph_chunk = 4;
+  o.k.ilu_rev = 0;
   o.substituted = nullptr;
   o.fields32 = 1;   // the mixed mode's field kernels compute in FP32 unless -ygpu_fields fp64
   o.pattern = 0;
@@ -63,6 +64,11 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k.is("-ir_max")) o.k.ir_max = v.to_int();
     else if (k.is("-ygpu_ilu_blocks")) o.k.ilu_blocks = v.to_int();
     else if (k.is("-ygpu_ilu_cols")) o.k.ilu_cols = std::max(1, v.to_int());
+    else if (k.is("-ygpu_ilu_order")) {
+      if (v.is("natural")) o.k.ilu_rev = 0;
+      else if (v.is("reverse")) o.k.ilu_rev = 1;
+      else { err = "-ygpu_ilu_order takes natural or reverse"; return YGPU_ERR_ARG; }
+    }
     else if (k.is("-ygpu_graph")) o.k.graph_chunk = std::max(0, v.to_int());
     else if (k.is("-ygpu_group")) o.group = v.to_int();
     else if (k.is("-ygpu_group_threads")) {

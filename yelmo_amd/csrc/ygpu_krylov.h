@@ -42,6 +42,9 @@ struct YgKrylovOpts {
                     // by their FP32 inverse diagonals); 2 = the U part in BF16 and
                     // the L part in FP32
   int graph_chunk;  // host side: Picard iterations per captured graph (0: no graph)
+  int ilu_rev;      // block-Jacobi ILU(0): 1 = columns of each line block numbered in
+                    // reverse (wavefront skew 1); used with FP32/FP64 factors when the
+                    // factorization's stage holds a step of the reversed wavefront
 };
 
 struct YgKrylovArgs {

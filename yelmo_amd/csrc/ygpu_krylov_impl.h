@@ -58,6 +58,12 @@ struct Ctx {
   // are dropped like those between line blocks, and each wave of a sweep
   // runs one tile (ncol = 1: whole lines)
   int ncol, mcol;
+  // blocked scheme: the columns of every line block (tile) are numbered in
+  // reverse (line i, column j before column j-1), which makes each line's
+  // lower couplings to the line before it reach only columns j and j+1, so
+  // line i can take column index t - i at step t (skew 1: ny + L - 1 steps
+  // instead of ny + 2(L-1)).  The natural order (rev = 0) has skew 2.
+  int rev;
   long long* tm;              // phase timers (thread 0), may be null
   // Offsets (values) from a factor pointer to the U part of its line-block
   // factors: 8 ncell for a whole member (FP32/FP64 and BF16 alike), and, for
@@ -251,8 +257,13 @@ template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
 
-// first line of step t that holds a cell: max(0, ceil((t - ny + 1) / 2))
-YG_DEV int glo_of(int t, int ny) { const int a = t - ny + 2; return a > 0 ? a >> 1 : 0; }
+// first line of step t that holds a cell: max(0, ceil((t - ny + 1) / s))
+// for the wavefront of skew s (2 natural, 1 reversed columns)
+YG_DEV int glo_of(const Ctx& c, int t) {
+  if (c.rev) { const int a = t - c.ny + 1; return a > 0 ? a : 0; }
+  const int a = t - c.ny + 2;
+  return a > 0 ? a >> 1 : 0;
+}
 
 // sum over s < t of floor(s / 2)
 YG_DEV int half_sum(int t) {
@@ -269,6 +280,15 @@ YG_DEV int blk_soff(int t, int L, int ny) {
   const int b = t > ny ? half_sum(t - ny) + (t - ny) : 0;
   return t + a - b;
 }
+// the same for skew 1 (line i takes column index t - i at step t): the sum
+// over s < t of min(L-1, s) - max(0, s-ny+1) + 1
+YG_DEV int blk_soff1(int t, int L, int ny) {
+  const int a = t <= L ? t * (t - 1) / 2 : L * (L - 1) / 2 + (t - L) * (L - 1);
+  const int b = t > ny ? (t - ny) * (t - ny + 1) / 2 : 0;
+  return t + a - b;
+}
+// column of line i at step t
+YG_DEV int col_of(const Ctx& c, int t, int i) { return c.rev ? c.ny - 1 - (t - i) : t - 2 * i; }
 
 // Cell of a wavefront chunk in stage slot tid: the chunk's steps t0..t0+C-1
 // hold at most C*maxcnt cells; slot tid takes step offset tid / maxcnt and
@@ -281,7 +301,7 @@ YG_DEV bool chunk_cell(const Ctx& c, int t0, int t1, int tid, int& q, int& nat, 
   if (t >= t1) return false;
   const int cnt = yg_soff[t + 1] - yg_soff[t];
   if (k >= cnt) return false;
-  const int i = glo_of(t, c.ny) + k, j = t - 2 * i;
+  const int i = glo_of(c, t) + k, j = col_of(c, t, i);
   q = yg_soff[t] + k;
   nat = i * c.ny + j;
   l = tid;
@@ -291,9 +311,9 @@ YG_DEV bool chunk_cell(const Ctx& c, int t0, int t1, int tid, int& q, int& nat, 
 // stage slot of this thread's line at step t of the chunk starting at t0,
 // or -1 if its line has no cell at step t
 YG_DEV int step_slot(const Ctx& c, int t, int t0) {
-  const int j = t - 2 * c.line;
-  if (c.line < 0 || j < 0 || j >= c.ny) return -1;
-  return (t - t0) * c.maxcnt + c.line - glo_of(t, c.ny);
+  const int jj = t - (c.rev ? 1 : 2) * c.line;   // the column's index in the line's order
+  if (c.line < 0 || jj < 0 || jj >= c.ny) return -1;
+  return (t - t0) * c.maxcnt + c.line - glo_of(c, t);
 }
 
 // ---- LDS staging of the wavefront sweeps ------------------------------------
@@ -343,40 +363,57 @@ YG_DEV double dpp_shl1(double v) {
 }
 
 // ---- ILU(0) factorization as a wavefront over grid lines -----------------
-// The line of a thread handles cell (i, j = t - 2 i) at step t.  The U rows
-// of the two most recent cells of each lane stay in registers; a lane reads
-// the line before it through wave shifts, and across waves (coupled scheme)
-// through an LDS ring.
+// The line of a thread handles cell (i, j = t - 2 i) at step t (reversed
+// columns: j = ny - 1 - (t - i)).  The U rows of the two most recent cells of
+// each lane stay in registers; a lane reads the line before it through wave
+// shifts, and across waves (coupled scheme) through an LDS ring.
+//
+// Stencil slots of a cell's two rows (Ctx::mask bit s of each row):
+//   ux row: 0 diag, 1 ux(i+1,j), 2 ux(i-1,j), 3 ux(i,j+1), 4 ux(i,j-1),
+//           5 uy(i,j), 6 uy(i+1,j), 7 uy(i+1,j-1), 8 uy(i,j-1)
+//   uy row: 0 diag, 1 uy(i,j+1), 2 uy(i,j-1), 3 uy(i+1,j), 4 uy(i-1,j),
+//           5 ux(i,j), 6 ux(i,j+1), 7 ux(i-1,j+1), 8 ux(i-1,j)
 template <typename CT>
-struct URow {   // U part of the two rows of one cell
+struct URow {   // U part of the two rows of one cell (natural column order)
   CT x5, x3, x7, x1, x6, xd;  // ux row: slots 5,3,7,1,6, 1/diag
   CT y6, y1, y3, yd;          // uy row: slots 6,1,3, 1/diag
 };
-
 template <typename CT>
-YG_DEV void urow_zero(URow<CT>& u) {
-  u.x5 = u.x3 = u.x7 = u.x1 = u.x6 = u.xd = CT(0);
-  u.y6 = u.y1 = u.y3 = u.yd = CT(0);
-}
+struct URowR {  // the same in the reversed column order
+  CT x5, x4, x8, x1, x6, x7, xd;  // ux row: slots 5,4,8,1,6,7, 1/diag
+  CT y2, y3, yd;                  // uy row: slots 2,3, 1/diag
+};
+template <typename TA, bool REV> struct URowOf { using type = URow<TA>; };
+template <typename TA> struct URowOf<TA, true> { using type = URowR<TA>; };
 
 template <typename CT>
 YG_DEV CT shup(CT v) { return dpp_shr1(v); }
 
+// cells of one stage buffer of the factorization (two buffers of 19 values
+// per cell in the dynamic LDS, at most two cells per thread)
+YG_DEV int factor_cap_of(int lds_bytes, int value_bytes) {
+  return min(2 * (int)blockDim.x, (lds_bytes / value_bytes) / (2 * 19));
+}
+template <typename TA>
+YG_DEV int factor_cap(const Ctx& c) { return factor_cap_of(c.lds, (int)sizeof(TA)); }
+
 // TA is the matrix and the factor type, and the arithmetic type of the
 // elimination (FP64 path: double; the FP32 inner solve of the mixed path:
-// float, whose factors are FP32 in any case)
-template <typename TA>
-YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restrict__ F,
-                        int* bad) {
+// float, whose factors are FP32 in any case).  REV: the reversed column order
+// of the blocked scheme (Ctx::rev).
+template <typename TA, bool REV>
+YG_NOINLINE void ilu0_factor_t(const Ctx& c, const TA* __restrict__ A, TA* __restrict__ F,
+                               int* bad) {
+  using UR = typename URowOf<TA, REV>::type;
   PhaseTimer ptm(c, 0);
   const YG_GLOBAL TA* __restrict__ Ag = gc(A);
   YG_GLOBAL TA* __restrict__ Fg = gm(F);
   const int g = threadIdx.x, lane = g & 63, w = g >> 6;
-  const bool ring_in = c.coupled && w > 0 && lane == 0;
-  const bool ring_out = c.coupled && w + 1 < c.nwl && lane == 63;
+  const bool ring_in = !REV && c.coupled && w > 0 && lane == 0;
+  const bool ring_out = !REV && c.coupled && w + 1 < c.nwl && lane == 63;
   const size_t nc = c.ncell;
   constexpr int NV = 19;  // 18 slot values + the presence mask
-  const int capc = min(2 * (int)blockDim.x, (int)(c.lds / sizeof(TA)) / (2 * NV));
+  const int capc = factor_cap<TA>(c);
   const int C = max(1, capc / c.maxcnt);
   const int stride = C * c.maxcnt;
   auto stage = [&](int sb, int t0) {   // the chunk starting at t0 -> stage at offset sb
@@ -407,9 +444,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
       }
     }
   };
-  URow<TA> o1, o2;  // own cells at steps t-1 and t-2
-  urow_zero(o1);
-  urow_zero(o2);
+  UR o1{}, o2{};  // own cells at steps t-1 and t-2
   const int nch = (c.T + C - 1) / C;
   stage(0, 0);
   __syncthreads();
@@ -417,26 +452,33 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
     const int t0 = ch * C, t1 = min(c.T, t0 + C);
     const int sb = (ch & 1) * NV * stride;
     for (int t = t0; t < t1; ++t) {
-      // P = line i-1 cell j (its step t-2), Q = line i-1 cell j+1 (its step t-1)
-      URow<TA> P, Q;
-      P.x5 = shup(o2.x5); P.x3 = shup(o2.x3); P.x7 = shup(o2.x7); P.x1 = shup(o2.x1);
-      P.x6 = shup(o2.x6); P.xd = shup(o2.xd); P.y6 = shup(o2.y6); P.y3 = shup(o2.y3);
-      P.yd = shup(o2.yd); P.y1 = TA(0);
-      Q.x7 = shup(o1.x7); Q.x1 = shup(o1.x1); Q.x6 = shup(o1.x6); Q.xd = shup(o1.xd);
-      Q.x5 = Q.x3 = Q.y6 = Q.y1 = Q.y3 = Q.yd = TA(0);
-      if (ring_in) {
-        const double* rp = yg_ring[w - 1][(t + 1) % 3];  // step t-2
-        const double* rq = yg_ring[w - 1][(t + 2) % 3];  // step t-1
-        P.x5 = rp[0]; P.x3 = rp[1]; P.x7 = rp[2]; P.x1 = rp[3]; P.x6 = rp[4]; P.xd = rp[5];
-        P.y6 = rp[6]; P.y1 = rp[7]; P.y3 = rp[8]; P.yd = rp[9];
-        Q.x5 = rq[0]; Q.x3 = rq[1]; Q.x7 = rq[2]; Q.x1 = rq[3]; Q.x6 = rq[4]; Q.xd = rq[5];
-        Q.y6 = rq[6]; Q.y1 = rq[7]; Q.y3 = rq[8]; Q.yd = rq[9];
+      // natural order: P = line i-1 cell j (its step t-2), Q = line i-1 cell
+      // j+1 (its step t-1); reversed: P = line i-1 cell j (its step t-1), Q =
+      // line i-1 cell j+1 (its step t-2)
+      UR P{}, Q{};
+      if constexpr (REV) {
+        P.x5 = shup(o1.x5); P.x1 = shup(o1.x1); P.x6 = shup(o1.x6); P.x7 = shup(o1.x7);
+        P.xd = shup(o1.xd); P.y3 = shup(o1.y3); P.yd = shup(o1.yd);
+        Q.x4 = shup(o2.x4); Q.x8 = shup(o2.x8); Q.x1 = shup(o2.x1); Q.x6 = shup(o2.x6);
+        Q.x7 = shup(o2.x7); Q.xd = shup(o2.xd);
+      } else {
+        P.x5 = shup(o2.x5); P.x3 = shup(o2.x3); P.x7 = shup(o2.x7); P.x1 = shup(o2.x1);
+        P.x6 = shup(o2.x6); P.xd = shup(o2.xd); P.y6 = shup(o2.y6); P.y3 = shup(o2.y3);
+        P.yd = shup(o2.yd);
+        Q.x7 = shup(o1.x7); Q.x1 = shup(o1.x1); Q.x6 = shup(o1.x6); Q.xd = shup(o1.xd);
+        if (ring_in) {
+          const double* rp = yg_ring[w - 1][(t + 1) % 3];  // step t-2
+          const double* rq = yg_ring[w - 1][(t + 2) % 3];  // step t-1
+          P.x5 = rp[0]; P.x3 = rp[1]; P.x7 = rp[2]; P.x1 = rp[3]; P.x6 = rp[4]; P.xd = rp[5];
+          P.y6 = rp[6]; P.y1 = rp[7]; P.y3 = rp[8]; P.yd = rp[9];
+          Q.x5 = rq[0]; Q.x3 = rq[1]; Q.x7 = rq[2]; Q.x1 = rq[3]; Q.x6 = rq[4]; Q.xd = rq[5];
+          Q.y6 = rq[6]; Q.y1 = rq[7]; Q.y3 = rq[8]; Q.yd = rq[9];
+        }
       }
       const int l = step_slot(c, t, t0);
-      URow<TA> X;
-      urow_zero(X);
+      UR X{};
       if (l >= 0) {
-        const int q = yg_soff[t] + c.line - glo_of(t, c.ny);
+        const int j = col_of(c, t, c.line);
         const uint32_t m = stage_to_mask<TA>(lds<TA>(sb + 18 * stride + l));
         TA a[9], b[9];
 #pragma unroll
@@ -450,7 +492,7 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
           // (ux row: ux(i,j-1), uy(i+1,j-1), uy(i,j-1); uy row: uy(i,j-1))
           // and after its last (ux row: ux(i,j+1); uy row: uy(i,j+1),
           // ux(i,j+1), ux(i-1,j+1)) are dropped
-          const int j = t - 2 * c.line, jt = j % c.mcol;
+          const int jt = j % c.mcol;
           if (jt == 0 && j > 0) {
             a[4] = a[7] = a[8] = 0.0; b[2] = 0.0;
             mx &= ~((1u << 4) | (1u << 7) | (1u << 8)); my &= ~(1u << 2);
@@ -462,73 +504,136 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
         }
 #define PX(s) ((mx >> (s)) & 1u)
 #define PY(s) ((my >> (s)) & 1u)
-        const URow<TA>& O = o1;
+        const UR& O = o1;
         TA lv;
-        // ux row, lower entries in column order: ux(i-1,j), ux(i,j-1), uy(i,j-1)
-        lv = a[2] * P.xd; a[2] = lv;
-        if (PX(8)) a[8] -= lv * P.x7;
-        if (PX(0)) a[0] -= lv * P.x1;
-        if (PX(5)) a[5] -= lv * P.x6;
-        lv = a[4] * O.xd; a[4] = lv;
-        if (PX(8)) a[8] -= lv * O.x5;
-        if (PX(0)) a[0] -= lv * O.x3;
-        if (PX(7)) a[7] -= lv * O.x6;
-        lv = a[8] * O.yd; a[8] = lv;
-        if (PX(0)) a[0] -= lv * O.y6;
-        if (PX(5)) a[5] -= lv * O.y1;
-        if (PX(7)) a[7] -= lv * O.y3;
-        X.xd = TA(1) / a[0];
-        X.x5 = a[5]; X.x3 = a[3]; X.x7 = a[7]; X.x1 = a[1]; X.x6 = a[6];
-        // uy row, lower entries: ux(i-1,j), uy(i-1,j), ux(i-1,j+1), uy(i,j-1), ux(i,j)
-        lv = b[8] * P.xd; b[8] = lv;
-        if (PY(4)) b[4] -= lv * P.x5;
-        if (PY(7)) b[7] -= lv * P.x3;
-        if (PY(2)) b[2] -= lv * P.x7;
-        if (PY(5)) b[5] -= lv * P.x1;
-        if (PY(0)) b[0] -= lv * P.x6;
-        lv = b[4] * P.yd; b[4] = lv;
-        if (PY(7)) b[7] -= lv * P.y6;
-        if (PY(0)) b[0] -= lv * P.y3;
-        lv = b[7] * Q.xd; b[7] = lv;
-        if (PY(0)) b[0] -= lv * Q.x7;
-        if (PY(6)) b[6] -= lv * Q.x1;
-        if (PY(1)) b[1] -= lv * Q.x6;
-        lv = b[2] * O.yd; b[2] = lv;
-        if (PY(5)) b[5] -= lv * O.y6;
-        if (PY(0)) b[0] -= lv * O.y1;
-        lv = b[5] * X.xd; b[5] = lv;
-        if (PY(0)) b[0] -= lv * X.x5;
-        if (PY(6)) b[6] -= lv * X.x3;
-        if (PY(3)) b[3] -= lv * X.x6;
-        X.yd = TA(1) / b[0];
-        X.y6 = b[6]; X.y1 = b[1]; X.y3 = b[3];
+        if constexpr (REV) {
+          // ux row, lower entries in column order: ux(i-1,j), ux(i,j+1)
+          lv = a[2] * P.xd; a[2] = lv;
+          if (PX(0)) a[0] -= lv * P.x1;
+          if (PX(5)) a[5] -= lv * P.x6;
+          if (PX(8)) a[8] -= lv * P.x7;
+          lv = a[3] * O.xd; a[3] = lv;
+          if (PX(0)) a[0] -= lv * O.x4;
+          if (PX(5)) a[5] -= lv * O.x8;
+          if (PX(6)) a[6] -= lv * O.x7;
+          X.xd = TA(1) / a[0];
+          X.x5 = a[5]; X.x4 = a[4]; X.x8 = a[8]; X.x1 = a[1]; X.x6 = a[6]; X.x7 = a[7];
+          // uy row, lower entries: ux(i-1,j+1), ux(i-1,j), uy(i-1,j), ux(i,j+1),
+          // uy(i,j+1), ux(i,j)
+          lv = b[7] * Q.xd; b[7] = lv;
+          if (PY(8)) b[8] -= lv * Q.x4;
+          if (PY(4)) b[4] -= lv * Q.x8;
+          if (PY(6)) b[6] -= lv * Q.x1;
+          if (PY(1)) b[1] -= lv * Q.x6;
+          if (PY(0)) b[0] -= lv * Q.x7;
+          lv = b[8] * P.xd; b[8] = lv;
+          if (PY(4)) b[4] -= lv * P.x5;
+          if (PY(5)) b[5] -= lv * P.x1;
+          if (PY(0)) b[0] -= lv * P.x6;
+          if (PY(2)) b[2] -= lv * P.x7;
+          lv = b[4] * P.yd; b[4] = lv;
+          if (PY(0)) b[0] -= lv * P.y3;
+          lv = b[6] * O.xd; b[6] = lv;
+          if (PY(1)) b[1] -= lv * O.x5;
+          if (PY(5)) b[5] -= lv * O.x4;
+          if (PY(0)) b[0] -= lv * O.x8;
+          if (PY(3)) b[3] -= lv * O.x7;
+          lv = b[1] * O.yd; b[1] = lv;
+          if (PY(0)) b[0] -= lv * O.y2;
+          lv = b[5] * X.xd; b[5] = lv;
+          if (PY(0)) b[0] -= lv * X.x5;
+          if (PY(2)) b[2] -= lv * X.x8;
+          if (PY(3)) b[3] -= lv * X.x6;
+          X.yd = TA(1) / b[0];
+          X.y2 = b[2]; X.y3 = b[3];
+        } else {
+          // ux row, lower entries in column order: ux(i-1,j), ux(i,j-1), uy(i,j-1)
+          lv = a[2] * P.xd; a[2] = lv;
+          if (PX(8)) a[8] -= lv * P.x7;
+          if (PX(0)) a[0] -= lv * P.x1;
+          if (PX(5)) a[5] -= lv * P.x6;
+          lv = a[4] * O.xd; a[4] = lv;
+          if (PX(8)) a[8] -= lv * O.x5;
+          if (PX(0)) a[0] -= lv * O.x3;
+          if (PX(7)) a[7] -= lv * O.x6;
+          lv = a[8] * O.yd; a[8] = lv;
+          if (PX(0)) a[0] -= lv * O.y6;
+          if (PX(5)) a[5] -= lv * O.y1;
+          if (PX(7)) a[7] -= lv * O.y3;
+          X.xd = TA(1) / a[0];
+          X.x5 = a[5]; X.x3 = a[3]; X.x7 = a[7]; X.x1 = a[1]; X.x6 = a[6];
+          // uy row, lower entries: ux(i-1,j), uy(i-1,j), ux(i-1,j+1), uy(i,j-1), ux(i,j)
+          lv = b[8] * P.xd; b[8] = lv;
+          if (PY(4)) b[4] -= lv * P.x5;
+          if (PY(7)) b[7] -= lv * P.x3;
+          if (PY(2)) b[2] -= lv * P.x7;
+          if (PY(5)) b[5] -= lv * P.x1;
+          if (PY(0)) b[0] -= lv * P.x6;
+          lv = b[4] * P.yd; b[4] = lv;
+          if (PY(7)) b[7] -= lv * P.y6;
+          if (PY(0)) b[0] -= lv * P.y3;
+          lv = b[7] * Q.xd; b[7] = lv;
+          if (PY(0)) b[0] -= lv * Q.x7;
+          if (PY(6)) b[6] -= lv * Q.x1;
+          if (PY(1)) b[1] -= lv * Q.x6;
+          lv = b[2] * O.yd; b[2] = lv;
+          if (PY(5)) b[5] -= lv * O.y6;
+          if (PY(0)) b[0] -= lv * O.y1;
+          lv = b[5] * X.xd; b[5] = lv;
+          if (PY(0)) b[0] -= lv * X.x5;
+          if (PY(6)) b[6] -= lv * X.x3;
+          if (PY(3)) b[3] -= lv * X.x6;
+          X.yd = TA(1) / b[0];
+          X.y6 = b[6]; X.y1 = b[1]; X.y3 = b[3];
+        }
 #undef PX
 #undef PY
         if (!(a[0] != 0.0) || !(b[0] != 0.0)) *bad = 1;
-        if (c.coupled) {
-          Fg[0 * nc + q] = (TA)a[2]; Fg[1 * nc + q] = (TA)a[4]; Fg[2 * nc + q] = (TA)a[8];
-          Fg[3 * nc + q] = (TA)b[8]; Fg[4 * nc + q] = (TA)b[4]; Fg[5 * nc + q] = (TA)b[7];
-          Fg[6 * nc + q] = (TA)b[2]; Fg[7 * nc + q] = (TA)b[5];
-          Fg[8 * nc + q] = (TA)X.x5; Fg[9 * nc + q] = (TA)X.x3; Fg[10 * nc + q] = (TA)X.x7;
-          Fg[11 * nc + q] = (TA)X.x1; Fg[12 * nc + q] = (TA)X.x6; Fg[13 * nc + q] = (TA)X.xd;
-          Fg[14 * nc + q] = (TA)X.y6; Fg[15 * nc + q] = (TA)X.y1; Fg[16 * nc + q] = (TA)X.y3;
-          Fg[17 * nc + q] = (TA)X.yd;
-        } else {
+        if constexpr (!REV) {
+          if (c.coupled) {
+            const int q = yg_soff[t] + c.line - glo_of(c, t);
+            Fg[0 * nc + q] = (TA)a[2]; Fg[1 * nc + q] = (TA)a[4]; Fg[2 * nc + q] = (TA)a[8];
+            Fg[3 * nc + q] = (TA)b[8]; Fg[4 * nc + q] = (TA)b[4]; Fg[5 * nc + q] = (TA)b[7];
+            Fg[6 * nc + q] = (TA)b[2]; Fg[7 * nc + q] = (TA)b[5];
+            Fg[8 * nc + q] = (TA)X.x5; Fg[9 * nc + q] = (TA)X.x3; Fg[10 * nc + q] = (TA)X.x7;
+            Fg[11 * nc + q] = (TA)X.x1; Fg[12 * nc + q] = (TA)X.x6; Fg[13 * nc + q] = (TA)X.xd;
+            Fg[14 * nc + q] = (TA)X.y6; Fg[15 * nc + q] = (TA)X.y1; Fg[16 * nc + q] = (TA)X.y3;
+            Fg[17 * nc + q] = (TA)X.yd;
+          }
+        }
+        if (!c.coupled) {
           // rotated tile order (the streamed sweeps): the tiles of a line
-          // block follow each other; cell (k, j) of a tile at slot
-          // ((j + 2k) mod M) L + k
+          // block follow each other; cell (k, jr) of a tile, jr its column's
+          // index in the tile's order (jt, or M - 1 - jt reversed), at slot
+          // ((jr + s k) mod M) L + k for skew s
           const int i0 = w * c.lpw, k = c.line - i0, Lb = min(c.lpw, c.nx - i0);
-          const int j = t - 2 * c.line, jt = c.ncol > 1 ? j % c.mcol : j;
+          const int jt = c.ncol > 1 ? j % c.mcol : j;
           const int j0 = j - jt, M = min(c.mcol, c.ny - j0);
-          // tb mod M with tb = jt + 2k < M + 2(Lb - 1): a few subtractions
-          int rq = jt + 2 * k;
+          // tb mod M with tb = jr + s k < M + 2(Lb - 1): a few subtractions
+          int rq = REV ? (M - 1 - jt) + k : jt + 2 * k;
           while (rq >= M) rq -= M;
           const size_t qb = (size_t)i0 * c.ny + (size_t)Lb * j0 + (size_t)rq * Lb + k;
           // 16-byte stores (the L and U records are 16-byte aligned)
-          const TA lv8[8] = {(TA)a[2], (TA)a[4], (TA)a[8], (TA)b[8],
-                             (TA)b[4], (TA)b[7], (TA)b[2], (TA)b[5]};
-          const TA uv12[12] = {(TA)X.x5, (TA)X.x3, (TA)X.x7, (TA)X.x1, (TA)X.x6, (TA)X.xd,
-                               (TA)X.y6, (TA)X.y1, (TA)X.y3, (TA)X.yd, (TA)0, (TA)0};
+          TA lv8[8], uv12[12];
+          if constexpr (REV) {
+            // L: ux row 2,3; uy row 7,8,4,6,1,5 (column order)
+            // U: ux row 5,4,8,1,6, 1/diag, 7; uy row 2,3, 1/diag
+            const TA l8[8] = {a[2], a[3], b[7], b[8], b[4], b[6], b[1], b[5]};
+            const TA u12[12] = {X.x5, X.x4, X.x8, X.x1, X.x6, X.xd,
+                                X.x7, X.y2, X.y3, X.yd, TA(0), TA(0)};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) lv8[e] = l8[e];
+#pragma unroll
+            for (int e = 0; e < 12; ++e) uv12[e] = u12[e];
+          } else {
+            const TA l8[8] = {a[2], a[4], a[8], b[8], b[4], b[7], b[2], b[5]};
+            const TA u12[12] = {X.x5, X.x3, X.x7, X.x1, X.x6, X.xd,
+                                X.y6, X.y1, X.y3, X.yd, TA(0), TA(0)};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) lv8[e] = l8[e];
+#pragma unroll
+            for (int e = 0; e < 12; ++e) uv12[e] = u12[e];
+          }
           constexpr int E = 16 / sizeof(TA);
           typedef TA V __attribute__((ext_vector_type(E)));
           YG_GLOBAL V* fl = reinterpret_cast<YG_GLOBAL V*>(Fg + 8 * qb);
@@ -551,11 +656,13 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
       }
       o2 = o1;
       o1 = X;
-      if (c.coupled) {
+      if (!REV && c.coupled) {
         if (ring_out) {
           double* rr = yg_ring[w][t % 3];
-          rr[0] = X.x5; rr[1] = X.x3; rr[2] = X.x7; rr[3] = X.x1; rr[4] = X.x6; rr[5] = X.xd;
-          rr[6] = X.y6; rr[7] = X.y1; rr[8] = X.y3; rr[9] = X.yd;
+          if constexpr (!REV) {
+            rr[0] = X.x5; rr[1] = X.x3; rr[2] = X.x7; rr[3] = X.x1; rr[4] = X.x6; rr[5] = X.xd;
+            rr[6] = X.y6; rr[7] = X.y1; rr[8] = X.y3; rr[9] = X.yd;
+          }
         }
         step_barrier();
       }
@@ -567,6 +674,12 @@ YG_NOINLINE void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restr
     if (ch + 1 < nch) stage(((ch + 1) & 1) * NV * stride, t1);
     __syncthreads();
   }
+}
+
+template <typename TA>
+YG_DEV void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restrict__ F, int* bad) {
+  if (c.rev) ilu0_factor_t<TA, true>(c, A, F, bad);
+  else ilu0_factor_t<TA, false>(c, A, F, bad);
 }
 
 // ---- triangular sweeps ---------------------------------------------------
@@ -894,13 +1007,15 @@ YG_DEV float bf_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
 YG_DEV float bf_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); }
 
 // Rotated tile order of the line-block factors.  A tile has L lines (one
-// lane each) and M columns and is swept in Tb = M + 2(L-1) wavefront steps:
-// line k handles column j = t - 2k at step t.  Cell (k, j) is stored at slot
-// ((j + 2k) mod M) L + k of the tile's range.  For each line, j -> (j + 2k)
-// mod M is a rotation, so the tile's M x L slots hold its L x M cells once
-// each, and the cells of step t are the contiguous slots (t mod M) L + [0, L):
-// the loads of a wave are coalesced, and the next step's slots are one add and
-// a wrap away.  Tiles follow each other by line block, then by column.
+// lane each) and M columns and is swept in Tb = M + s(L-1) wavefront steps
+// (skew s = 2, or 1 for reversed columns): line k handles the column of
+// index jr = t - s k in the line's order at step t (jr = j natural, M - 1 - j
+// reversed).  Cell (k, jr) is stored at slot ((jr + s k) mod M) L + k of the
+// tile's range.  For each line, jr -> (jr + s k) mod M is a rotation, so the
+// tile's M x L slots hold its L x M cells once each, and the cells of step t
+// are the contiguous slots (t mod M) L + [0, L): the loads of a wave are
+// coalesced, and the next step's slots are one add and a wrap away.  Tiles
+// follow each other by line block, then by column.
 //
 // A lane without a cell at step t (j outside [0, M), or a lane past the
 // tile's lines) passes an out-of-range offset, which loads zero factors and a
@@ -912,8 +1027,12 @@ YG_DEV int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 struct SweepTile {
   int L, M, Tb, LM;   // lines, columns, steps, cells (wave-uniform)
   int base;           // the tile's first slot in the member's factor order
-  int tl;             // -2 lane, or far out of range for a lane past the lines
-  int lane_n;         // the natural cell of this lane's line at step t is lane_n + t
+  int tl;             // -s lane, or far out of range for a lane past the lines
+  int rev;            // reversed columns (skew 1)
+  // the natural cell of this lane's line at step t: lane_n + t (natural
+  // order), or lane_n - 64 + (Tb - 1 - t) (reversed order, whose lines run
+  // backwards; see nat_rsrc)
+  int lane_n;
 };
 // the wave's tile: line block wb, columns [j0, j0 + M); false for a wave without one
 YG_DEV bool sweep_tile(const Ctx& c, SweepTile& g) {
@@ -922,13 +1041,17 @@ YG_DEV bool sweep_tile(const Ctx& c, SweepTile& g) {
   const int ncol = uni(c.ncol), mcol = uni(c.mcol);
   if (w >= nwl * ncol) return false;
   const int wb = w / ncol, j0 = (w - wb * ncol) * mcol, i0 = wb * lpw;
+  const int rev = uni(c.rev), sk = rev ? 1 : 2;
   g.L = min(lpw, nx - i0);
   g.M = min(mcol, ny - j0);
-  g.Tb = g.M + 2 * (g.L - 1);
+  g.Tb = g.M + sk * (g.L - 1);
   g.LM = g.L * g.M;
   g.base = i0 * ny + g.L * j0;
-  g.tl = lane < g.L ? -2 * lane : -(1 << 30);
-  g.lane_n = (i0 + lane) * ny + j0 - 2 * lane;
+  g.tl = lane < g.L ? -sk * lane : -(1 << 30);
+  g.rev = rev;
+  // reversed: column j0 + M - 1 - (t - k) of line i0 + k; at least 65 - L >= 1
+  g.lane_n = rev ? (i0 + lane) * ny + j0 + g.M - 1 + lane - (g.Tb - 1) + 64
+                 : (i0 + lane) * ny + j0 - 2 * lane;
   return true;
 }
 // does this lane have a cell at step t
@@ -941,9 +1064,20 @@ YG_DEV unsigned rot_off(const SweepTile& g, int t, int sq, unsigned bytes) {
   const unsigned o = (bytes & (bytes - 1)) ? __umul24(q, bytes) : q * bytes;
   return on_step(g, t) ? o : YG_OOB;
 }
-// natural-order byte offset of this lane's cell at step t, or out of range
-YG_DEV unsigned nat_off(const SweepTile& g, int t, unsigned bytes) {
-  return on_step(g, t) ? (unsigned)(g.lane_n + t) * bytes : YG_OOB;
+// The step's part of a natural-order offset (cells; the scalar offset of the
+// buffer instructions): t, or the steps left in the reversed order.
+template <bool REV>
+YG_DEV int nat_step(const SweepTile& g, int t) { return REV ? g.Tb - 1 - t : t; }
+// A natural-order vector (2 values per cell, nc cells) as a buffer resource
+// for nat_step offsets.  In the reversed order a lane's fixed offset is taken
+// from 64 cells before the vector (the lines run backwards, and the scalar
+// offset can only add), so the resource starts there; a lane with a cell at
+// a step always lands inside the vector.
+template <typename T>
+YG_DEV __amdgpu_buffer_rsrc_t nat_rsrc(const SweepTile& g, const T* v, int nc) {
+  const size_t pad = g.rev ? 2 * 64 : 0;
+  const T* b = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(v) - pad * sizeof(T));
+  return yg_rsrc(b, 2 * (size_t)nc + pad);
 }
 // (t mod M) L of the next and of the previous step
 YG_DEV int rot_next(const SweepTile& g, int sq) { sq += g.L; return sq == g.LM ? 0 : sq; }
@@ -1005,7 +1139,8 @@ YG_DEV uint4 load_raw16_so(__amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
   return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
 }
 
-template <typename T>
+// REV: the reversed column order (factor values in its own slot order)
+template <typename T, bool REV>
 YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
                              T* __restrict__ ys) {
   PhaseTimer ptm(c, 1);
@@ -1016,7 +1151,7 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   if (!sweep_tile(c, g)) return;
   const int nc = uni(c.ncell);
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + (size_t)8 * g.base, (size_t)8 * g.LM);
-  const __amdgpu_buffer_rsrc_t rR = yg_rsrc(rhs, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rR = nat_rsrc(g, rhs, nc);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
   const unsigned lane = threadIdx.x & 63;
   const unsigned voF = lane * SF, voY = lane * SV, voN = (unsigned)g.lane_n * SV;
@@ -1028,7 +1163,7 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   auto issue = [&](int s, int t) {
     const bool v = on_step(g, t);
     load_vals_so<8>(rF, v ? voF : YG_OOB, sq * SF, f[s]);
-    r[s] = pair_load_so(rR, v ? voN : YG_OOB, t * SV, T());
+    r[s] = pair_load_so(rR, v ? voN : YG_OOB, nat_step<REV>(g, t) * SV, T());
     so[s] = v ? voY : YG_OOB;
     ss[s] = sq * SV;
     sq = rot_next(g, sq);
@@ -1038,15 +1173,26 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
     issue(s, s);
     __builtin_amdgcn_sched_barrier(0);
   }
-  // o1: this lane's results one step back; p1: the line before, two steps back
+  // o1: this lane's results one step back; n1, p1: the line before, one and
+  // two steps back
   T o1x = 0, o1y = 0, p1x = 0, p1y = 0;
   for (int t0 = 0; t0 < g.Tb; t0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
       const T n1x = dpp_shr1(o1x), n1y = dpp_shr1(o1y);
-      const T ax = r[s].x - f[s][0] * p1x, ay = r[s].y - f[s][3] * p1x - f[s][4] * p1y;
-      T yx = ax - f[s][1] * o1x - f[s][2] * o1y;
-      T yy = ay - f[s][6] * o1y - f[s][5] * n1x - f[s][7] * yx;
+      T yx, yy;
+      if constexpr (REV) {
+        // ux row: ux(i-1,j) = n1, ux(i,j+1) = o1; uy row: ux(i-1,j+1) = p1,
+        // ux(i-1,j), uy(i-1,j) = n1, ux(i,j+1), uy(i,j+1) = o1, ux(i,j)
+        const T ax = r[s].x - f[s][1] * o1x;
+        const T ay = r[s].y - f[s][2] * p1x - f[s][5] * o1x - f[s][6] * o1y;
+        yx = ax - f[s][0] * n1x;
+        yy = ay - f[s][3] * n1x - f[s][4] * n1y - f[s][7] * yx;
+      } else {
+        const T ax = r[s].x - f[s][0] * p1x, ay = r[s].y - f[s][3] * p1x - f[s][4] * p1y;
+        yx = ax - f[s][1] * o1x - f[s][2] * o1y;
+        yy = ay - f[s][6] * o1y - f[s][5] * n1x - f[s][7] * yx;
+      }
       V2 o;
       o.x = yx;
       o.y = yy;
@@ -1058,7 +1204,22 @@ YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __r
   }
 }
 
+// The backward sweeps write z in the natural order, either straight to z
+// (ZL false: a step's cells lie on different lines, one cache line each) or
+// (ZL true) into the dynamic LDS, cell q of the block's part at value pair q,
+// which the block then copies to z with coalesced stores (zl_copy_out).  A
+// lane without a cell at a step writes a pair of its own past the part.
+// Staging needs (cells + 64) value pairs of LDS (zl_fits).
 template <typename T>
+YG_DEV bool zl_fits(const Ctx& c) { return (size_t)(c.ncell + 64) * 2 * sizeof(T) <= (size_t)c.lds; }
+template <typename T>
+YG_DEV void zl_copy_out(const Ctx& c, T* __restrict__ z) {
+  using V2 = typename Vec2<T>::type;
+  V2* __restrict__ zv = reinterpret_cast<V2*>(z);
+  for (int q = threadIdx.x; q < c.ncell; q += blockDim.x) zv[q] = lds<V2>(q);
+}
+
+template <typename T, bool REV, bool ZL>
 YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ ys,
                              T* __restrict__ z) {
   PhaseTimer ptm(c, 2);
@@ -1070,9 +1231,13 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
   const int nc = uni(c.ncell);
   const __amdgpu_buffer_rsrc_t rF = yg_rsrc(F + c.fu + (size_t)12 * g.base, (size_t)12 * g.LM);
   const __amdgpu_buffer_rsrc_t rY = yg_rsrc(ys + (size_t)2 * g.base, (size_t)2 * g.LM);
-  const __amdgpu_buffer_rsrc_t rZ = yg_rsrc(z, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rZ = nat_rsrc(g, z, nc);
   const unsigned lane = threadIdx.x & 63;
   const unsigned voF = lane * SF, voY = lane * SV, voN = (unsigned)g.lane_n * SV;
+  // LDS byte address of this lane's cell: zl0 + t SV (natural) or zl0 - t SV
+  // (reversed); its own pair past the part at a step without a cell
+  const unsigned zl0 = REV ? (unsigned)(g.lane_n - 64 + g.Tb - 1) * SV : voN;
+  const unsigned zlx = (unsigned)(nc + (int)lane) * SV;
   T f[D][12];
   V2 yv[D];
   unsigned so[D];
@@ -1083,8 +1248,16 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
     const bool v = on_step(g, t);
     load_vals_so<12>(rF, v ? voF : YG_OOB, sq * SF, f[s]);
     yv[s] = pair_load_so(rY, v ? voY : YG_OOB, sq * SV, T());
-    so[s] = v ? voN : YG_OOB;
-    ss[s] = t * SV;
+    if constexpr (ZL) {
+      // a bit mask rather than a select: the compiler turns the select into
+      // a branch around the step's loads
+      const unsigned za = REV ? zl0 - (unsigned)t * SV : zl0 + (unsigned)t * SV;
+      const unsigned m = 0u - (unsigned)v;
+      so[s] = (za & m) | (zlx & ~m);
+    } else {
+      so[s] = v ? voN : YG_OOB;
+      ss[s] = nat_step<REV>(g, t) * SV;
+    }
     sq = rot_prev(g, sq);
   };
 #pragma unroll
@@ -1092,20 +1265,31 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
     issue(s, tz - s);
     __builtin_amdgcn_sched_barrier(0);
   }
-  // o1: own results one step back; p1: the line after, two steps back
+  // o1: own results one step back; n1, p1: the line after, one and two steps back
   T o1x = 0, o1y = 0, p1x = 0, p1y = 0;
   for (int u0 = 0; u0 < g.Tb; u0 += D) {
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-      // line i+1: cell j-1 (one step back) and cell j (two steps back)
       const T n1y = dpp_shl1(o1y), n1x = dpp_shl1(o1x);
-      const T by = yv[s].y - f[s][8] * p1y, bx = yv[s].x - f[s][3] * p1x - f[s][4] * p1y;
-      T zy = (by - f[s][6] * o1x - f[s][7] * o1y) * f[s][9];
-      T zx = (bx - f[s][1] * o1x - f[s][2] * n1y - f[s][0] * zy) * f[s][5];
+      T zx, zy;
+      if constexpr (REV) {
+        // ux row: uy(i,j), ux(i,j-1), uy(i,j-1) = o1, ux(i+1,j), uy(i+1,j) = n1,
+        // uy(i+1,j-1) = p1; uy row: uy(i,j-1) = o1, uy(i+1,j) = n1
+        const T by = yv[s].y - f[s][7] * o1y;
+        const T bx = yv[s].x - f[s][6] * p1y - f[s][1] * o1x - f[s][2] * o1y;
+        zy = (by - f[s][8] * n1y) * f[s][9];
+        zx = (bx - f[s][3] * n1x - f[s][4] * n1y - f[s][0] * zy) * f[s][5];
+      } else {
+        // line i+1: cell j-1 (one step back) and cell j (two steps back)
+        const T by = yv[s].y - f[s][8] * p1y, bx = yv[s].x - f[s][3] * p1x - f[s][4] * p1y;
+        zy = (by - f[s][6] * o1x - f[s][7] * o1y) * f[s][9];
+        zx = (bx - f[s][1] * o1x - f[s][2] * n1y - f[s][0] * zy) * f[s][5];
+      }
       V2 o;
       o.x = zx;
       o.y = zy;
-      pair_store_so(o, rZ, so[s], ss[s]);
+      if constexpr (ZL) *reinterpret_cast<V2*>(yg_dyn_lds + so[s]) = o;
+      else pair_store_so(o, rZ, so[s], ss[s]);
       p1x = n1x; p1y = n1y; o1x = zx; o1y = zy;
       issue(s, tz - (u0 + s + D));
       __builtin_amdgcn_sched_barrier(0);
@@ -1114,8 +1298,9 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
 }
 
 // The streamed sweeps over BF16 factors (FP32 arithmetic): the same
-// recurrences as sweep_L_str / sweep_U_str<float>, with a factor cell of 16
-// (L) or 24 (U) bytes instead of 32 and 48.
+// recurrences as sweep_L_str / sweep_U_str<float, false>, with a factor cell
+// of 16 (L) or 24 (U) bytes instead of 32 and 48.  Natural column order only
+// (the kernels never set Ctx::rev with BF16 factors).
 YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
                                const float* __restrict__ rhs, float* __restrict__ ys) {
   PhaseTimer ptm(c, 1);
@@ -1279,17 +1464,34 @@ YG_DEV void precon_apply(const Ctx& c, const Precon<T>& P, const T* in, T* tmp, 
     if constexpr (sizeof(T) == 4) {
       if (pc.ncell > 0) {
         if (P.fb_l) sweep_L_str16(pc, P.FB, in + o, tmp + o);
-        else sweep_L_str<T>(pc, P.F, in + o, tmp + o);
+        else sweep_L_str<T, false>(pc, P.F, in + o, tmp + o);
       }
       __syncthreads();
       if (pc.ncell > 0) sweep_U_str16(pc, P.FB, tmp + o, out + o);
       __syncthreads();
     }
   } else if (P.kind == 2) {
-    if (pc.ncell > 0) sweep_L_str<T>(pc, P.F, in + o, tmp + o);
+    if (pc.ncell > 0) {
+      if (pc.rev) sweep_L_str<T, true>(pc, P.F, in + o, tmp + o);
+      else sweep_L_str<T, false>(pc, P.F, in + o, tmp + o);
+    }
     __syncthreads();
-    if (pc.ncell > 0) sweep_U_str<T>(pc, P.F, tmp + o, out + o);
+    // z staged in LDS when the part fits (coalesced stores afterwards)
+    const bool zl = zl_fits<T>(pc);
+    if (pc.ncell > 0) {
+      if (zl) {
+        if (pc.rev) sweep_U_str<T, true, true>(pc, P.F, tmp + o, out + o);
+        else sweep_U_str<T, false, true>(pc, P.F, tmp + o, out + o);
+      } else {
+        if (pc.rev) sweep_U_str<T, true, false>(pc, P.F, tmp + o, out + o);
+        else sweep_U_str<T, false, false>(pc, P.F, tmp + o, out + o);
+      }
+    }
     __syncthreads();
+    if (zl) {
+      zl_copy_out<T>(pc, out + o);
+      __syncthreads();
+    }
   } else {
     for (int q = 2 * c.cb + threadIdx.x; q < 2 * c.ce; q += blockDim.x)
       out[q] = P.kind == 1 ? in[q] * P.F[q] : in[q];
@@ -1800,6 +2002,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     if (nb < 0) nb = MODE == YG_MODE_MIXED ? (mb.nx + 63) / 64 : 0;
     c.ncol = 1;
     c.mcol = mb.ny;
+    c.rev = 0;
     if (nb == 0) {
       c.coupled = mb.nx > 64;
       c.nwl = (mb.nx + 63) / 64;
@@ -1815,6 +2018,18 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
       const int i = w * lpw + lane;
       c.line = (w < c.nwl && lane < lpw && i < mb.nx) ? i : -1;
       yg_tiles(mb.ncol != 0 ? mb.ncol : a.opts.ilu_cols, wmax / c.nwl, mb.ny, c);
+      // reversed columns: FP32/FP64 factors, and the widest step of the
+      // skew-1 wavefront (min(nx, ny) cells) fits the factorization's stage
+      const bool f16 = MODE == YG_MODE_MIXED && a.opts.factor16 && a.F16;
+      const int cap = factor_cap_of(YG_DYN_LDS, MODE == YG_MODE_FP64 ? 8 : 4);
+      if (a.opts.ilu_rev && !f16 && min(mb.nx, mb.ny) <= cap) {
+        c.rev = 1;
+        c.T = mb.ny + mb.nx - 1;
+        c.maxcnt = min(mb.nx, mb.ny);
+        __syncthreads();   // the host's (skew 2) step table is replaced
+        for (int k = threadIdx.x; k <= c.T; k += blockDim.x) yg_soff[k] = blk_soff1(k, mb.nx, mb.ny);
+        __syncthreads();
+      }
     }
   }
   __shared__ long long s_tm[8];
@@ -1999,9 +2214,14 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     const int nbr = (nwl + G - 1) / G;
     const int i0 = min(nx, rank * nbr * lpw), i1 = min(nx, (rank + 1) * nbr * lpw);
     const int Lg = i1 - i0;
-    const int T = Lg > 0 ? ny + 2 * (Lg - 1) : 0;
+    // reversed columns (skew 1): FP32 factors, and the widest step (min(Lg,
+    // ny) cells) fits the factorization's stage
+    const int rev = a.opts.ilu_rev && !(a.opts.factor16 && a.F16) && Lg > 0 &&
+                    min(Lg, ny) <= factor_cap_of(yg_group_lds<NT>(), 4);
+    const int T = Lg > 0 ? ny + (rev ? 1 : 2) * (Lg - 1) : 0;
     __syncthreads();   // the previous member's step table is no longer read
-    for (int k = threadIdx.x; k <= T; k += blockDim.x) yg_soff[k] = blk_soff(k, Lg, ny);
+    for (int k = threadIdx.x; k <= T; k += blockDim.x)
+      yg_soff[k] = rev ? blk_soff1(k, Lg, ny) : blk_soff(k, Lg, ny);
     if (threadIdx.x == 0) { s_bad = 0; yg_gwait = 0; }
     if (threadIdx.x < 8) s_tm[threadIdx.x] = 0;
     __syncthreads();
@@ -2014,6 +2234,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     cm.maxcnt = mb.maxcnt;
     cm.line = -1; cm.coupled = 0; cm.nwl = 0; cm.lpw = lpw;
     cm.ncol = 1; cm.mcol = ny;
+    cm.rev = 0;
     cm.tm = a.timers ? s_tm : nullptr;
     cm.fu = cm.fu16 = 8ll * ncell;
     cm.fd16 = 16ll * ncell;
@@ -2023,7 +2244,8 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     Ctx cr = cm;   // this rank's lines: factorization and sweeps
     cr.nx = Lg; cr.T = T; cr.ncell = Lg * ny;
     cr.mask = cm.mask + cm.cb;
-    cr.maxcnt = min(Lg, (ny + 1) / 2);
+    cr.maxcnt = rev ? min(Lg, ny) : min(Lg, (ny + 1) / 2);
+    cr.rev = rev;
     cr.nwl = (Lg + lpw - 1) / lpw;
     {
       const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = w * lpw + lane;
