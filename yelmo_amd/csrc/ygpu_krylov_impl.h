@@ -228,7 +228,7 @@ template <int K>
 YG_DEV void csum(const Ctx& c, double (&v)[K]) {
   yg_block_sum<K>(v, yg_red);
   if (!c.grp) return;
-  const Grp& g = *c.grp;
+  const Grp g = *c.grp;   // fields in registers, not reloaded after the atomics
   double* slot = g.part + (size_t)((yg_gbar + 1) & 1) * g.G * 4;
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -402,8 +402,9 @@ YG_DEV int factor_cap(const Ctx& c) { return factor_cap_of(c.lds, (int)sizeof(TA
 // float, whose factors are FP32 in any case).  REV: the reversed column order
 // of the blocked scheme (Ctx::rev).
 template <typename TA, bool REV>
-YG_NOINLINE void ilu0_factor_t(const Ctx& c, const TA* __restrict__ A, TA* __restrict__ F,
+YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __restrict__ F,
                                int* bad) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   using UR = typename URowOf<TA, REV>::type;
   PhaseTimer ptm(c, 0);
   const YG_GLOBAL TA* __restrict__ Ag = gc(A);
@@ -700,8 +701,9 @@ YG_DEV int sweep_cap_cells() {
 }
 
 template <typename T>
-YG_NOINLINE void sweep_L(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
+YG_NOINLINE void sweep_L(const Ctx& cin, const T* __restrict__ F, const T* __restrict__ rhs,
                     T* __restrict__ y) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   PhaseTimer ptm(c, 1);
   const YG_GLOBAL T* __restrict__ Fg = gc(F);
   const YG_GLOBAL T* __restrict__ rg = gc(rhs);
@@ -800,8 +802,9 @@ YG_NOINLINE void sweep_L(const Ctx& c, const T* __restrict__ F, const T* __restr
 }
 
 template <typename T>
-YG_NOINLINE void sweep_U(const Ctx& c, const T* __restrict__ F, const T* __restrict__ y,
+YG_NOINLINE void sweep_U(const Ctx& cin, const T* __restrict__ F, const T* __restrict__ y,
                     T* __restrict__ z) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   PhaseTimer ptm(c, 2);
   const YG_GLOBAL T* __restrict__ Fg = gc(F);
   const YG_GLOBAL T* __restrict__ yg = gc(y);
@@ -1141,8 +1144,9 @@ YG_DEV uint4 load_raw16_so(__amdgpu_buffer_rsrc_t r, unsigned off, int soff) {
 
 // REV: the reversed column order (factor values in its own slot order)
 template <typename T, bool REV>
-YG_NOINLINE void sweep_L_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ rhs,
+YG_NOINLINE void sweep_L_str(const Ctx& cin, const T* __restrict__ F, const T* __restrict__ rhs,
                              T* __restrict__ ys) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   PhaseTimer ptm(c, 1);
   constexpr int D = sizeof(T) == 4 ? 12 : 6;   // register slots (steps in flight)
   constexpr int SF = 8 * sizeof(T), SV = 2 * sizeof(T);   // bytes per cell
@@ -1220,8 +1224,9 @@ YG_DEV void zl_copy_out(const Ctx& c, T* __restrict__ z) {
 }
 
 template <typename T, bool REV, bool ZL>
-YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __restrict__ ys,
+YG_NOINLINE void sweep_U_str(const Ctx& cin, const T* __restrict__ F, const T* __restrict__ ys,
                              T* __restrict__ z) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   PhaseTimer ptm(c, 2);
   constexpr int D = sizeof(T) == 4 ? 10 : 6;
   constexpr int SF = 12 * sizeof(T), SV = 2 * sizeof(T);
@@ -1301,8 +1306,9 @@ YG_NOINLINE void sweep_U_str(const Ctx& c, const T* __restrict__ F, const T* __r
 // recurrences as sweep_L_str / sweep_U_str<float, false>, with a factor cell
 // of 16 (L) or 24 (U) bytes instead of 32 and 48.  Natural column order only
 // (the kernels never set Ctx::rev with BF16 factors).
-YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
+YG_NOINLINE void sweep_L_str16(const Ctx& cin, const uint16_t* __restrict__ FB,
                                const float* __restrict__ rhs, float* __restrict__ ys) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   PhaseTimer ptm(c, 1);
   constexpr int D = 20;   // steps in flight: 7 registers per step (4 packed factor words, 2 rhs, 1 offset)
   constexpr int SF = 16, SV = 8;
@@ -1355,8 +1361,9 @@ YG_NOINLINE void sweep_L_str16(const Ctx& c, const uint16_t* __restrict__ FB,
   }
 }
 
-YG_NOINLINE void sweep_U_str16(const Ctx& c, const uint16_t* __restrict__ FB,
+YG_NOINLINE void sweep_U_str16(const Ctx& cin, const uint16_t* __restrict__ FB,
                                const float* __restrict__ ys, float* __restrict__ z) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   PhaseTimer ptm(c, 2);
   constexpr int D = 18;   // 9 registers per step: 4 packed words, the 2 inverse diagonals, y, offset
   constexpr int SF = 16, SD = 8, SV = 8;
@@ -1568,8 +1575,9 @@ struct Sums2 {
 // belong to other workgroups).  A device function of its own (not inlined)
 // so that its addresses and resources get their own register allocation.
 template <int MODE, typename TA, typename TV, bool COH = false>
-YG_NOINLINE Sums2 spmv_dots(const Ctx& c, const TA* A, const TV* x, TV* y, const TV* u, TV* w1,
+YG_NOINLINE Sums2 spmv_dots(const Ctx& cin, const TA* A, const TV* x, TV* y, const TV* u, TV* w1,
                             TV* w2) {
+  const Ctx c = cin;   // fields in registers, not reloaded after stores
   PhaseTimer ptm(c, 3);
   using V2 = typename Vec2<TV>::type;
   const int nc = uni(c.ncell), B = blockDim.x, cb = uni(c.cb), ce = uni(c.ce);
