@@ -40,7 +40,13 @@ namespace {
 //            other inside a sweep.  A line block may further be cut into
 //            tiles of consecutive columns (one per wave in the sweeps, which
 //            then need mcol + 2(lpw-1) steps instead of ny + 2(lpw-1)).
-struct Grp;
+struct Grp {
+  int* cnt;       // arrivals at group barriers (monotonic over the launch)
+  int* err;       // set when a wait timed out
+  double* part;   // partial sums: [2 (barrier parity)][G][4]
+  int G, rank;
+};
+#define YG_GMAX 8   // largest member group
 struct Ctx {
   const YgMember* mb;
   int nx, ny, T;
@@ -73,9 +79,11 @@ struct Ctx {
   long long fu, fu16, fd16;
   // The cells [cb, ce) of the member whose rows this block computes in the
   // SpMV and vector passes (the whole member, or this block's share in a
-  // member group), and the group (null for one block per member).
+  // member group), and the group (null for one block per member) with a copy
+  // of it (gv), read without a pointer
   int cb, ce;
   const Grp* grp;
+  Grp gv;
   // the context of the preconditioner's part of the member (itself for one
   // block per member)
   const Ctx* pc;
@@ -143,12 +151,6 @@ YG_DEV void yg_tiles(int want, int fit, int ny, Ctx& c) {
 // several XCDs keeps those fences on every barrier (L2 write-back before the
 // arrival, invalidation after the wait: correct across L2s, slower).
 #define YG_GRP_SPIN 200000000ll   // wall-clock ticks (100 MHz): 2 s
-struct Grp {
-  int* cnt;       // arrivals at group barriers (monotonic over the launch)
-  int* err;       // set when a wait timed out
-  double* part;   // partial sums: [2 (barrier parity)][G][4]
-  int G, rank;
-};
 __shared__ int yg_gbar;    // group barriers this block has passed
 __shared__ int yg_gfail;   // a wait of this group timed out
 __shared__ int yg_gfence;  // the group spans several XCDs: barriers fence
@@ -224,21 +226,36 @@ YG_DEV void grp_start(const Grp& g, int* xcc_words) {
 // bits.  Partial sums alternate between two buffers by barrier parity: a
 // block can reach the next sum only after every block has passed this
 // barrier, and the sum after that only after every block has read this one.
+// The partial sums are loaded together (one round trip to the L2) and then
+// added in rank order.  The loads cover YG_GMAX ranks whatever G is: the
+// partial-sum buffer lies inside the group's synchronization block
+// (YG_GSYNC_BYTES from byte 64), which holds [2][YG_GMAX][4] doubles, so
+// every load stays inside it; values of ranks past G are not used.
+static_assert(64 + 2 * YG_GMAX * 4 * sizeof(double) <= YG_GSYNC_BYTES, "partial sums");
 template <int K>
 YG_DEV void csum(const Ctx& c, double (&v)[K]) {
+  static_assert(K <= 4, "four partial sums per rank");
   yg_block_sum<K>(v, yg_red);
   if (!c.grp) return;
-  const Grp g = *c.grp;   // fields in registers, not reloaded after the atomics
+  const Grp g = c.gv;
   double* slot = g.part + (size_t)((yg_gbar + 1) & 1) * g.G * 4;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int e = 0; e < K; ++e) slot[g.rank * 4 + e] = v[e];
   }
   grp_barrier(g);
+  double pv[YG_GMAX][K];
+#pragma unroll
+  for (int r = 0; r < YG_GMAX; ++r) {
+#pragma unroll
+    for (int e = 0; e < K; ++e) pv[r][e] = grp_loadd(slot + r * 4 + e);
+  }
 #pragma unroll
   for (int e = 0; e < K; ++e) {
     double t = 0.0;
-    for (int r = 0; r < g.G; ++r) t += grp_loadd(slot + r * 4 + e);
+#pragma unroll
+    for (int r = 0; r < YG_GMAX; ++r)
+      if (r < g.G) t += pv[r][e];
     v[e] = t;
   }
 }
@@ -1830,7 +1847,7 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     }
     __syncthreads();
     precon_apply<TV>(c, P, p, ytmp, ph);
-    if (GRP) grp_barrier(*c.grp);   // the halo lines of phat
+    if (GRP) grp_barrier(c.gv);   // the halo lines of phat
     // v = A phat, (rt, v)
     red[0] = spmv_dots<YG_SPMV_DOT, TA, TV, GRP>(c, A, ph, v, rt, nullptr, nullptr).s0;
     if (red[0] == 0.0) { out.status = 2; out.iters = it; return out; }
@@ -2248,6 +2265,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     cm.fd16 = 16ll * ncell;
     cm.cb = i0 * ny; cm.ce = i1 * ny;
     cm.grp = &grp;
+    cm.gv = grp;
     cm.lds = yg_group_lds<NT>();
     Ctx cr = cm;   // this rank's lines: factorization and sweeps
     cr.nx = Lg; cr.T = T; cr.ncell = Lg * ny;
