@@ -414,21 +414,28 @@ YG_DEV int factor_cap_of(int lds_bytes, int value_bytes) {
 template <typename TA>
 YG_DEV int factor_cap(const Ctx& c) { return factor_cap_of(c.lds, (int)sizeof(TA)); }
 
+// Pivot reciprocal: exact division in FP64 (the reference's ILU(0)); the
+// hardware reciprocal (1 ulp) for the FP32 factors of the mixed path's
+// inner preconditioner, where it shortens each wavefront step.
+YG_DEV double pivot_rcp(double v) { return 1.0 / v; }
+YG_DEV float pivot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
 // TA is the matrix and the factor type, and the arithmetic type of the
 // elimination (FP64 path: double; the FP32 inner solve of the mixed path:
 // float, whose factors are FP32 in any case).  REV: the reversed column order
-// of the blocked scheme (Ctx::rev).
-template <typename TA, bool REV>
+// of the blocked scheme (Ctx::rev).  CPL: the coupled scheme (Ctx::coupled).
+template <typename TA, bool REV, bool CPL>
 YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __restrict__ F,
                                int* bad) {
+  static_assert(!(REV && CPL), "the reversed order is a blocked-scheme option");
   const Ctx c = cin;   // fields in registers, not reloaded after stores
   using UR = typename URowOf<TA, REV>::type;
   PhaseTimer ptm(c, 0);
   const YG_GLOBAL TA* __restrict__ Ag = gc(A);
   YG_GLOBAL TA* __restrict__ Fg = gm(F);
   const int g = threadIdx.x, lane = g & 63, w = g >> 6;
-  const bool ring_in = !REV && c.coupled && w > 0 && lane == 0;
-  const bool ring_out = !REV && c.coupled && w + 1 < c.nwl && lane == 63;
+  const bool ring_in = CPL && w > 0 && lane == 0;
+  const bool ring_out = CPL && w + 1 < c.nwl && lane == 63;
   const size_t nc = c.ncell;
   constexpr int NV = 19;  // 18 slot values + the presence mask
   const int capc = factor_cap<TA>(c);
@@ -484,7 +491,7 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
         P.x6 = shup(o2.x6); P.xd = shup(o2.xd); P.y6 = shup(o2.y6); P.y3 = shup(o2.y3);
         P.yd = shup(o2.yd);
         Q.x7 = shup(o1.x7); Q.x1 = shup(o1.x1); Q.x6 = shup(o1.x6); Q.xd = shup(o1.xd);
-        if (ring_in) {
+        if (CPL && ring_in) {
           const double* rp = yg_ring[w - 1][(t + 1) % 3];  // step t-2
           const double* rq = yg_ring[w - 1][(t + 2) % 3];  // step t-1
           P.x5 = rp[0]; P.x3 = rp[1]; P.x7 = rp[2]; P.x1 = rp[3]; P.x6 = rp[4]; P.xd = rp[5];
@@ -505,7 +512,7 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
           b[s] = lds<TA>(sb + (2 * s + 1) * stride + l);
         }
         uint32_t mx = m & 0x1ffu, my = (m >> 16) & 0x1ffu;
-        if (!c.coupled && c.ncol > 1) {
+        if (!CPL && c.ncol > 1) {
           // tile edges: couplings to the column before the tile's first
           // (ux row: ux(i,j-1), uy(i+1,j-1), uy(i,j-1); uy row: uy(i,j-1))
           // and after its last (ux row: ux(i,j+1); uy row: uy(i,j+1),
@@ -520,6 +527,8 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
             mx &= ~(1u << 3); my &= ~((1u << 1) | (1u << 6) | (1u << 7));
           }
         }
+        // an update of slot s only where the row holds slot s (ILU(0) keeps
+        // the pattern): a select of the operand (or zero) rather than a branch
 #define PX(s) ((mx >> (s)) & 1u)
 #define PY(s) ((my >> (s)) & 1u)
         const UR& O = o1;
@@ -527,99 +536,96 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
         if constexpr (REV) {
           // ux row, lower entries in column order: ux(i-1,j), ux(i,j+1)
           lv = a[2] * P.xd; a[2] = lv;
-          if (PX(0)) a[0] -= lv * P.x1;
-          if (PX(5)) a[5] -= lv * P.x6;
-          if (PX(8)) a[8] -= lv * P.x7;
+          a[0] -= lv * (PX(0) ? P.x1 : TA(0));
+          a[5] -= lv * (PX(5) ? P.x6 : TA(0));
+          a[8] -= lv * (PX(8) ? P.x7 : TA(0));
           lv = a[3] * O.xd; a[3] = lv;
-          if (PX(0)) a[0] -= lv * O.x4;
-          if (PX(5)) a[5] -= lv * O.x8;
-          if (PX(6)) a[6] -= lv * O.x7;
-          X.xd = TA(1) / a[0];
+          a[0] -= lv * (PX(0) ? O.x4 : TA(0));
+          a[5] -= lv * (PX(5) ? O.x8 : TA(0));
+          a[6] -= lv * (PX(6) ? O.x7 : TA(0));
+          X.xd = pivot_rcp(a[0]);
           X.x5 = a[5]; X.x4 = a[4]; X.x8 = a[8]; X.x1 = a[1]; X.x6 = a[6]; X.x7 = a[7];
           // uy row, lower entries: ux(i-1,j+1), ux(i-1,j), uy(i-1,j), ux(i,j+1),
           // uy(i,j+1), ux(i,j)
           lv = b[7] * Q.xd; b[7] = lv;
-          if (PY(8)) b[8] -= lv * Q.x4;
-          if (PY(4)) b[4] -= lv * Q.x8;
-          if (PY(6)) b[6] -= lv * Q.x1;
-          if (PY(1)) b[1] -= lv * Q.x6;
-          if (PY(0)) b[0] -= lv * Q.x7;
+          b[8] -= lv * (PY(8) ? Q.x4 : TA(0));
+          b[4] -= lv * (PY(4) ? Q.x8 : TA(0));
+          b[6] -= lv * (PY(6) ? Q.x1 : TA(0));
+          b[1] -= lv * (PY(1) ? Q.x6 : TA(0));
+          b[0] -= lv * (PY(0) ? Q.x7 : TA(0));
           lv = b[8] * P.xd; b[8] = lv;
-          if (PY(4)) b[4] -= lv * P.x5;
-          if (PY(5)) b[5] -= lv * P.x1;
-          if (PY(0)) b[0] -= lv * P.x6;
-          if (PY(2)) b[2] -= lv * P.x7;
+          b[4] -= lv * (PY(4) ? P.x5 : TA(0));
+          b[5] -= lv * (PY(5) ? P.x1 : TA(0));
+          b[0] -= lv * (PY(0) ? P.x6 : TA(0));
+          b[2] -= lv * (PY(2) ? P.x7 : TA(0));
           lv = b[4] * P.yd; b[4] = lv;
-          if (PY(0)) b[0] -= lv * P.y3;
+          b[0] -= lv * (PY(0) ? P.y3 : TA(0));
           lv = b[6] * O.xd; b[6] = lv;
-          if (PY(1)) b[1] -= lv * O.x5;
-          if (PY(5)) b[5] -= lv * O.x4;
-          if (PY(0)) b[0] -= lv * O.x8;
-          if (PY(3)) b[3] -= lv * O.x7;
+          b[1] -= lv * (PY(1) ? O.x5 : TA(0));
+          b[5] -= lv * (PY(5) ? O.x4 : TA(0));
+          b[0] -= lv * (PY(0) ? O.x8 : TA(0));
+          b[3] -= lv * (PY(3) ? O.x7 : TA(0));
           lv = b[1] * O.yd; b[1] = lv;
-          if (PY(0)) b[0] -= lv * O.y2;
+          b[0] -= lv * (PY(0) ? O.y2 : TA(0));
           lv = b[5] * X.xd; b[5] = lv;
-          if (PY(0)) b[0] -= lv * X.x5;
-          if (PY(2)) b[2] -= lv * X.x8;
-          if (PY(3)) b[3] -= lv * X.x6;
-          X.yd = TA(1) / b[0];
+          b[0] -= lv * (PY(0) ? X.x5 : TA(0));
+          b[2] -= lv * (PY(2) ? X.x8 : TA(0));
+          b[3] -= lv * (PY(3) ? X.x6 : TA(0));
+          X.yd = pivot_rcp(b[0]);
           X.y2 = b[2]; X.y3 = b[3];
         } else {
           // ux row, lower entries in column order: ux(i-1,j), ux(i,j-1), uy(i,j-1)
           lv = a[2] * P.xd; a[2] = lv;
-          if (PX(8)) a[8] -= lv * P.x7;
-          if (PX(0)) a[0] -= lv * P.x1;
-          if (PX(5)) a[5] -= lv * P.x6;
+          a[8] -= lv * (PX(8) ? P.x7 : TA(0));
+          a[0] -= lv * (PX(0) ? P.x1 : TA(0));
+          a[5] -= lv * (PX(5) ? P.x6 : TA(0));
           lv = a[4] * O.xd; a[4] = lv;
-          if (PX(8)) a[8] -= lv * O.x5;
-          if (PX(0)) a[0] -= lv * O.x3;
-          if (PX(7)) a[7] -= lv * O.x6;
+          a[8] -= lv * (PX(8) ? O.x5 : TA(0));
+          a[0] -= lv * (PX(0) ? O.x3 : TA(0));
+          a[7] -= lv * (PX(7) ? O.x6 : TA(0));
           lv = a[8] * O.yd; a[8] = lv;
-          if (PX(0)) a[0] -= lv * O.y6;
-          if (PX(5)) a[5] -= lv * O.y1;
-          if (PX(7)) a[7] -= lv * O.y3;
-          X.xd = TA(1) / a[0];
+          a[0] -= lv * (PX(0) ? O.y6 : TA(0));
+          a[5] -= lv * (PX(5) ? O.y1 : TA(0));
+          a[7] -= lv * (PX(7) ? O.y3 : TA(0));
+          X.xd = pivot_rcp(a[0]);
           X.x5 = a[5]; X.x3 = a[3]; X.x7 = a[7]; X.x1 = a[1]; X.x6 = a[6];
           // uy row, lower entries: ux(i-1,j), uy(i-1,j), ux(i-1,j+1), uy(i,j-1), ux(i,j)
           lv = b[8] * P.xd; b[8] = lv;
-          if (PY(4)) b[4] -= lv * P.x5;
-          if (PY(7)) b[7] -= lv * P.x3;
-          if (PY(2)) b[2] -= lv * P.x7;
-          if (PY(5)) b[5] -= lv * P.x1;
-          if (PY(0)) b[0] -= lv * P.x6;
+          b[4] -= lv * (PY(4) ? P.x5 : TA(0));
+          b[7] -= lv * (PY(7) ? P.x3 : TA(0));
+          b[2] -= lv * (PY(2) ? P.x7 : TA(0));
+          b[5] -= lv * (PY(5) ? P.x1 : TA(0));
+          b[0] -= lv * (PY(0) ? P.x6 : TA(0));
           lv = b[4] * P.yd; b[4] = lv;
-          if (PY(7)) b[7] -= lv * P.y6;
-          if (PY(0)) b[0] -= lv * P.y3;
+          b[7] -= lv * (PY(7) ? P.y6 : TA(0));
+          b[0] -= lv * (PY(0) ? P.y3 : TA(0));
           lv = b[7] * Q.xd; b[7] = lv;
-          if (PY(0)) b[0] -= lv * Q.x7;
-          if (PY(6)) b[6] -= lv * Q.x1;
-          if (PY(1)) b[1] -= lv * Q.x6;
+          b[0] -= lv * (PY(0) ? Q.x7 : TA(0));
+          b[6] -= lv * (PY(6) ? Q.x1 : TA(0));
+          b[1] -= lv * (PY(1) ? Q.x6 : TA(0));
           lv = b[2] * O.yd; b[2] = lv;
-          if (PY(5)) b[5] -= lv * O.y6;
-          if (PY(0)) b[0] -= lv * O.y1;
+          b[5] -= lv * (PY(5) ? O.y6 : TA(0));
+          b[0] -= lv * (PY(0) ? O.y1 : TA(0));
           lv = b[5] * X.xd; b[5] = lv;
-          if (PY(0)) b[0] -= lv * X.x5;
-          if (PY(6)) b[6] -= lv * X.x3;
-          if (PY(3)) b[3] -= lv * X.x6;
-          X.yd = TA(1) / b[0];
+          b[0] -= lv * (PY(0) ? X.x5 : TA(0));
+          b[6] -= lv * (PY(6) ? X.x3 : TA(0));
+          b[3] -= lv * (PY(3) ? X.x6 : TA(0));
+          X.yd = pivot_rcp(b[0]);
           X.y6 = b[6]; X.y1 = b[1]; X.y3 = b[3];
         }
 #undef PX
 #undef PY
         if (!(a[0] != 0.0) || !(b[0] != 0.0)) *bad = 1;
-        if constexpr (!REV) {
-          if (c.coupled) {
-            const int q = yg_soff[t] + c.line - glo_of(c, t);
-            Fg[0 * nc + q] = (TA)a[2]; Fg[1 * nc + q] = (TA)a[4]; Fg[2 * nc + q] = (TA)a[8];
-            Fg[3 * nc + q] = (TA)b[8]; Fg[4 * nc + q] = (TA)b[4]; Fg[5 * nc + q] = (TA)b[7];
-            Fg[6 * nc + q] = (TA)b[2]; Fg[7 * nc + q] = (TA)b[5];
-            Fg[8 * nc + q] = (TA)X.x5; Fg[9 * nc + q] = (TA)X.x3; Fg[10 * nc + q] = (TA)X.x7;
-            Fg[11 * nc + q] = (TA)X.x1; Fg[12 * nc + q] = (TA)X.x6; Fg[13 * nc + q] = (TA)X.xd;
-            Fg[14 * nc + q] = (TA)X.y6; Fg[15 * nc + q] = (TA)X.y1; Fg[16 * nc + q] = (TA)X.y3;
-            Fg[17 * nc + q] = (TA)X.yd;
-          }
-        }
-        if (!c.coupled) {
+        if constexpr (CPL) {
+          const int q = yg_soff[t] + c.line - glo_of(c, t);
+          Fg[0 * nc + q] = (TA)a[2]; Fg[1 * nc + q] = (TA)a[4]; Fg[2 * nc + q] = (TA)a[8];
+          Fg[3 * nc + q] = (TA)b[8]; Fg[4 * nc + q] = (TA)b[4]; Fg[5 * nc + q] = (TA)b[7];
+          Fg[6 * nc + q] = (TA)b[2]; Fg[7 * nc + q] = (TA)b[5];
+          Fg[8 * nc + q] = (TA)X.x5; Fg[9 * nc + q] = (TA)X.x3; Fg[10 * nc + q] = (TA)X.x7;
+          Fg[11 * nc + q] = (TA)X.x1; Fg[12 * nc + q] = (TA)X.x6; Fg[13 * nc + q] = (TA)X.xd;
+          Fg[14 * nc + q] = (TA)X.y6; Fg[15 * nc + q] = (TA)X.y1; Fg[16 * nc + q] = (TA)X.y3;
+          Fg[17 * nc + q] = (TA)X.yd;
+        } else {
           // rotated tile order (the streamed sweeps): the tiles of a line
           // block follow each other; cell (k, jr) of a tile, jr its column's
           // index in the tile's order (jt, or M - 1 - jt reversed), at slot
@@ -674,13 +680,11 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
       }
       o2 = o1;
       o1 = X;
-      if (!REV && c.coupled) {
+      if constexpr (CPL) {
         if (ring_out) {
           double* rr = yg_ring[w][t % 3];
-          if constexpr (!REV) {
-            rr[0] = X.x5; rr[1] = X.x3; rr[2] = X.x7; rr[3] = X.x1; rr[4] = X.x6; rr[5] = X.xd;
-            rr[6] = X.y6; rr[7] = X.y1; rr[8] = X.y3; rr[9] = X.yd;
-          }
+          rr[0] = X.x5; rr[1] = X.x3; rr[2] = X.x7; rr[3] = X.x1; rr[4] = X.x6; rr[5] = X.xd;
+          rr[6] = X.y6; rr[7] = X.y1; rr[8] = X.y3; rr[9] = X.yd;
         }
         step_barrier();
       }
@@ -696,8 +700,9 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
 
 template <typename TA>
 YG_DEV void ilu0_factor(const Ctx& c, const TA* __restrict__ A, TA* __restrict__ F, int* bad) {
-  if (c.rev) ilu0_factor_t<TA, true>(c, A, F, bad);
-  else ilu0_factor_t<TA, false>(c, A, F, bad);
+  if (c.coupled) ilu0_factor_t<TA, false, true>(c, A, F, bad);
+  else if (c.rev) ilu0_factor_t<TA, true, false>(c, A, F, bad);
+  else ilu0_factor_t<TA, false, false>(c, A, F, bad);
 }
 
 // ---- triangular sweeps ---------------------------------------------------
