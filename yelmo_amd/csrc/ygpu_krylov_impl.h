@@ -1615,13 +1615,16 @@ YG_NOINLINE Sums2 spmv_dots(const Ctx& cin, const TA* A, const TV* x, TV* y, con
   // Four cells per thread per trip, with no branch around the memory
   // operations: a cell index past the member reads zeros (matrix record and
   // u out of range) and its stores are dropped, so every load of the trip
-  // is issued before the first is waited for.
+  // (u's too, ahead of the stores, which the compiler cannot move loads
+  // across) is issued before the first is waited for.
   for (int q0 = cb + threadIdx.x; q0 < ce; q0 += 4 * B) {
     double ax[4], ay[4];
+    V2 uqs[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       // a cell past the range reads as the zero record past the member
       const int q = q0 + k * B < ce ? q0 + k * B : nc;
+      uqs[k] = Pair16<TV>::load(rU, q0 + k * B < ce ? (unsigned)q * SV : YG_OOB);
       // only the first and last line of the range read lines that other
       // workgroups wrote; the other cells' x loads may go through the L1
       if (COH && (q - cb < ny || ce - q <= ny))
@@ -1638,7 +1641,7 @@ YG_NOINLINE Sums2 spmv_dots(const Ctx& cin, const TA* A, const TV* x, TV* y, con
       o.y = (TV)ay[k];
       Pair16<TV>::store(o, rY, off);
       const double yx = (double)o.x, yy = (double)o.y;
-      const V2 uq = Pair16<TV>::load(rU, off);
+      const V2 uq = uqs[k];
       if (MODE == YG_SPMV_RES) {
         V2 r;
         r.x = (TV)((double)uq.x - yx);
