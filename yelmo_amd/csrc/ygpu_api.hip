@@ -31,6 +31,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.ilu_cols = 1;
   o.k.factor16 = 0; o.k.graph_chunk = 4;
   o.k.ilu_rev = 0;
+  o.k.ir_delta = 0.0;
   o.substituted = nullptr;
   o.fields32 = 1;   // the mixed mode's field kernels compute in FP32 unless -ygpu_fields fp64
   o.pattern = 0;
@@ -62,6 +63,18 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k.is("-ir_inner_tol")) o.k.inner_tol = v.to_double();
     else if (k.is("-ygpu_ir_tol")) o.k.ir_target = v.to_double();
     else if (k.is("-ir_max")) o.k.ir_max = v.to_int();
+    else if (k.is("-ygpu_ir")) {
+      if (v.is("reliable")) { if (o.k.ir_delta <= 0.0) o.k.ir_delta = 1e-4; }
+      else if (v.is("restart")) o.k.ir_delta = 0.0;
+      else { err = "-ygpu_ir takes reliable or restart"; return YGPU_ERR_ARG; }
+    }
+    else if (k.is("-ygpu_ir_delta")) {
+      o.k.ir_delta = v.to_double();
+      if (!(o.k.ir_delta > 0.0 && o.k.ir_delta < 1.0)) {
+        err = "-ygpu_ir_delta takes a value in (0, 1)";
+        return YGPU_ERR_ARG;
+      }
+    }
     else if (k.is("-ygpu_ilu_blocks")) o.k.ilu_blocks = v.to_int();
     else if (k.is("-ygpu_ilu_cols")) o.k.ilu_cols = std::max(1, v.to_int());
     else if (k.is("-ygpu_ilu_order")) {

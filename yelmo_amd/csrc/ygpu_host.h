@@ -911,6 +911,18 @@ struct Batch {
               "vectors %.3f group-barrier waits %.3f total %.3f\n",
               tmh[0] * us * 1e-3, tmh[1] * us * 1e-3, tmh[2] * us * 1e-3, tmh[3] * us * 1e-3,
               tmh[4] * us * 1e-3, tmh[5] * us * 1e-3, tmh[7] * us * 1e-3);
+      double tsum = 0.0, tmax = 0.0;
+      for (int k = 0; k < nmem; ++k) {
+        tsum += tmh[8 * k + 7];
+        tmax = std::max(tmax, (double)tmh[8 * k + 7]);
+      }
+      fprintf(stderr, "ygpu phases: Krylov time per member (ms) mean %.3f max %.3f over %d members\n",
+              tsum / nmem * us * 1e-3, tmax * us * 1e-3, nmem);
+      if (getenv("YGPU_PHASE_TIMERS")[0] == '2') {
+        fprintf(stderr, "ygpu phases: per member (ms):");
+        for (int k = 0; k < nmem; ++k) fprintf(stderr, " %.2f", tmh[8 * k + 7] * us * 1e-3);
+        fprintf(stderr, "\n");
+      }
     }
     if (d_gfail) {
       int gf = 0;

@@ -45,6 +45,12 @@ struct YgKrylovOpts {
   int ilu_rev;      // block-Jacobi ILU(0): 1 = columns of each line block numbered in
                     // reverse (wavefront skew 1); used with FP32/FP64 factors when the
                     // factorization's stage holds a step of the reversed wavefront
+  double ir_delta;  // mixed mode, BiCGSTAB: > 0 one FP32 BiCGSTAB whose residual is
+                    // replaced by the FP64 one (b - A x in FP64, x accumulated in FP64)
+                    // whenever the iterated residual fell below ir_delta times its
+                    // value at the last replacement, the search direction kept
+                    // (reliable updates); 0 restarts a new inner solve at every FP64
+                    // residual instead (inner_tol)
 };
 
 struct YgKrylovArgs {

@@ -1663,6 +1663,58 @@ YG_NOINLINE Sums2 spmv_dots(const Ctx& cin, const TA* A, const TV* x, TV* y, con
   return Sums2{sums[0], sums[1]};
 }
 
+// The FP64 residual of the reliable-update solve over the rows of cells
+// [c.cb, c.ce): with e = b - A x (FP64 matrix records, x and b), stores
+// r = (float)(e sc) and returns the member sums (e, e) (FP64, unscaled) and
+// (rt, r).  The same trip structure as spmv_dots (four cells per thread, every
+// load of a trip in flight together; cells past the range read zeros and
+// their stores are dropped).  COH: a member group (x's halo lines were
+// written by other workgroups).
+template <bool COH>
+YG_NOINLINE Sums2 spmv_res32(const Ctx& cin, const double* A, const double* x, const double* b,
+                             const float* rt, float* r, double sc) {
+  const Ctx c = cin;
+  PhaseTimer ptm(c, 3);
+  const int nc = uni(c.ncell), B = blockDim.x, cb = uni(c.cb), ce = uni(c.ce);
+  const int nx = uni(c.nx), ny = uni(c.ny), wrap = uni((c.wrapx ? 1 : 0) | (c.wrapy ? 2 : 0));
+  const __amdgpu_buffer_rsrc_t rA = yg_rsrc(A, (size_t)YG_NA * nc);
+  const __amdgpu_buffer_rsrc_t rX = yg_rsrc(x, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rB = yg_rsrc(b, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rT = yg_rsrc(rt, (size_t)2 * nc);
+  const __amdgpu_buffer_rsrc_t rR = yg_rsrc(r, (size_t)2 * nc);
+  double s0 = 0.0, s1 = 0.0;
+  for (int q0 = cb + threadIdx.x; q0 < ce; q0 += 4 * B) {
+    double ax[4], ay[4];
+    double2 bq[4];
+    float2 tq[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool in = q0 + k * B < ce;
+      const int q = in ? q0 + k * B : nc;
+      bq[k] = Pair16<double>::load(rB, in ? (unsigned)q * 16u : YG_OOB);
+      tq[k] = Pair16<float>::load(rT, in ? (unsigned)q * 8u : YG_OOB);
+      if (COH && (q - cb < ny || ce - q <= ny))
+        spmv_cell<double, double, true>(nx, ny, wrap, rA, rX, q, ax[k], ay[k]);
+      else
+        spmv_cell<double, double, false>(nx, ny, wrap, rA, rX, q, ax[k], ay[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = q0 + k * B;
+      const double ex = bq[k].x - ax[k], ey = bq[k].y - ay[k];
+      float2 o;
+      o.x = (float)(ex * sc);
+      o.y = (float)(ey * sc);
+      Pair16<float>::store(o, rR, q < ce ? (unsigned)q * 8u : YG_OOB);
+      s0 += ex * ex + ey * ey;
+      s1 += (double)tq[k].x * (double)o.x + (double)tq[k].y * (double)o.y;
+    }
+  }
+  double sums[2] = {s0, s1};
+  csum<2>(c, sums);
+  return Sums2{sums[0], sums[1]};
+}
+
 // A 16-byte chunk of a vector (4 floats or 2 doubles).
 template <typename T>
 struct alignas(16) C16 {
@@ -1920,6 +1972,254 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
   return out;
 }
 
+// ---- mixed precision with reliable updates ----------------------------------
+// r = rt = (float)(r64 sc), d = 0 over the value pairs [lo, hi); returns this
+// thread's share of (r, r)
+YG_DEV double ir_start(const double* __restrict__ r64, float* __restrict__ r,
+                       float* __restrict__ rt, float* __restrict__ d, double sc, int lo, int hi) {
+  const int p0 = lo >> 1, p1 = hi >> 1, B = blockDim.x;
+  double rr = 0.0;
+  for (int k0 = p0 + (int)threadIdx.x; k0 < p1; k0 += YG_RU * B) {
+    double2 v[YG_RU];
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) v[u] = reinterpret_cast<const double2*>(r64)[k];
+    }
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) {
+        float2 o;
+        o.x = (float)(v[u].x * sc);
+        o.y = (float)(v[u].y * sc);
+        reinterpret_cast<float2*>(r)[k] = o;
+        reinterpret_cast<float2*>(rt)[k] = o;
+        reinterpret_cast<float2*>(d)[k] = make_float2(0.0f, 0.0f);
+        rr += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+      }
+    }
+  }
+  return rr;
+}
+
+// x64 += sb d, d = 0 over the value pairs [lo, hi)
+YG_DEV void ir_fold(double* __restrict__ x64, float* __restrict__ d, double sb, int lo, int hi) {
+  const int p0 = lo >> 1, p1 = hi >> 1, B = blockDim.x;
+  for (int k0 = p0 + (int)threadIdx.x; k0 < p1; k0 += YG_RU * B) {
+    double2 x[YG_RU];
+    float2 dv[YG_RU];
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) {
+        x[u] = reinterpret_cast<const double2*>(x64)[k];
+        dv[u] = reinterpret_cast<const float2*>(d)[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < YG_RU; ++u) {
+      const int k = k0 + u * B;
+      if (k < p1) {
+        double2 o;
+        o.x = x[u].x + sb * (double)dv[u].x;
+        o.y = x[u].y + sb * (double)dv[u].y;
+        reinterpret_cast<double2*>(x64)[k] = o;
+        reinterpret_cast<float2*>(d)[k] = make_float2(0.0f, 0.0f);
+      }
+    }
+  }
+}
+
+struct IrOut {
+  int iters, status, nres;
+  double res;
+};
+
+// The mixed-precision solve of A x = b as one FP32 BiCGSTAB with reliable
+// updates (residual replacement with group-wise updates of x, van der Vorst
+// and Ye, SIAM J. Sci. Comput. 22 (2000) 835-852): the recurrences run in
+// FP32 on the FP32 matrix records and factors, in units of the initial FP64
+// residual norm rn0; the correction d accumulates in FP32 and is folded into
+// the FP64 x (x += rn0 d) whenever the iterated residual has fallen below
+// ir_delta times its value at the last replacement (or below the target),
+// after which the iterated residual is replaced by the FP64 residual
+// b - A x (FP64 records) and the iteration continues with its search
+// direction.  A breakdown (rho, (rt, v), (t, t) or omega zero) or
+// convergence on s also folds d and replaces r, and then restarts the
+// recurrences from the true residual (rt = r, p = r); a breakdown right
+// after a (re)start ends the solve with status 2.  The solve stops when the
+// FP64 residual reaches the refinement target ||r|| <= ir_target ||r0||
+// (ir_target 0: a tenth of tol), or at maxiter iterations or ir_max FP64
+// residuals; status 0 when ||r|| <= tol ||r0||, else 4.  Every decision
+// rests on group sums, so every block of a member group takes the same
+// branches.  GRP: a member group (x's halo lines go through group barriers).
+// W holds the 9 FP32 Krylov vectors, d the FP32 correction; r64, y64 are
+// FP64 scratch vectors.
+template <bool GRP>
+YG_DEV IrOut ir_bicgstab(const Ctx& c, const double* A64, const float* A32,
+                         const Precon<float>& P, const double* b64, double* x64, double* r64,
+                         double* y64, float* W, float* d, const YgKrylovOpts& o) {
+  const int lo = 2 * c.cb, hi = 2 * c.ce;
+  const size_t vl = c.vlen;
+  float* __restrict__ r = W;
+  float* __restrict__ rt = W + 1 * vl;
+  float* __restrict__ p = W + 2 * vl;
+  float* __restrict__ v = W + 3 * vl;
+  float* __restrict__ ph = W + 4 * vl;
+  float* __restrict__ s = W + 5 * vl;
+  float* __restrict__ sh = W + 6 * vl;
+  float* __restrict__ tt = W + 7 * vl;
+  float* __restrict__ ytmp = W + 8 * vl;
+  IrOut out{0, 4, 0, 1.0};
+  const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
+  // the FP64 residual of x0
+  if (GRP) grp_barrier(c.gv);
+  double red[2];
+  {
+    const Sums2 z = spmv_dots<YG_SPMV_RES, double, double, GRP>(c, A64, x64, y64, b64, r64, nullptr);
+    red[0] = z.s0;
+    red[1] = z.s1;
+  }
+  out.nres = 1;
+  const double rn0 = sqrt(red[0]);
+  const double nrm0 = o.conv_cond == 1 ? sqrt(red[1]) : rn0;
+  if (nrm0 == 0.0 || rn0 == 0.0) { out.res = 0.0; out.status = 0; return out; }
+  out.res = rn0 / nrm0;
+  out.status = out.res <= o.tol ? 0 : 4;
+  if (out.res <= target || o.maxiter <= 0) return out;
+  const double sc = 1.0 / rn0;             // the FP32 vectors are residuals / rn0
+  const double tgt = target * nrm0 * sc;   // the iterated residual's target in those units
+  red[0] = ir_start(r64, r, rt, d, sc, lo, hi);
+  csum<1>(c, *reinterpret_cast<double(*)[1]>(red));
+  __syncthreads();
+  double rho = red[0], rho_old = 1.0, alpha = 1.0, omega = 1.0;
+  double rupd = sqrt(red[0]);   // the iterated residual at the last replacement
+  bool fresh = true;            // p starts from r (first iteration after a (re)start)
+  int since = 0;                // iterations since the last (re)start
+  for (;;) {
+    ++out.iters;
+    ++since;
+    bool brk = false, conv_s = false, upd = false;
+    if (rho == 0.0) {
+      brk = true;
+    } else {
+      // p = r + beta (p - omega v); phat = M^-1 p
+      {
+        PhaseTimer ptm(c, 4);
+        if (fresh) {
+          vmap<float, 1, 1>(lo, hi, {r}, {p}, [&](const double (&xi)[1], double (&xo)[1]) { xo[0] = xi[0]; });
+        } else {
+          const double beta = (rho / rho_old) * (alpha / omega);
+          vmap<float, 3, 1>(lo, hi, {r, p, v}, {p}, [&](const double (&xi)[3], double (&xo)[1]) {
+            xo[0] = xi[0] + beta * (xi[1] - omega * xi[2]);
+          });
+        }
+      }
+      fresh = false;
+      __syncthreads();
+      precon_apply<float>(c, P, p, ytmp, ph);
+      if (GRP) grp_barrier(c.gv);   // the halo lines of phat
+      // v = A phat, (rt, v)
+      const double rtv = spmv_dots<YG_SPMV_DOT, float, float, GRP>(c, A32, ph, v, rt, nullptr, nullptr).s0;
+      if (rtv == 0.0) {
+        brk = true;
+      } else {
+        alpha = rho / rtv;
+        // s = r - alpha v, ||s||
+        red[0] = 0.0;
+        {
+          PhaseTimer ptm(c, 4);
+          vmap<float, 2, 1>(lo, hi, {r, v}, {s}, [&](const double (&xi)[2], double (&xo)[1]) {
+            const double sq = (double)(float)(xi[0] - alpha * xi[1]);
+            xo[0] = sq;
+            red[0] += sq * sq;
+          });
+        }
+        if (GRP) {
+          // shat first: the sum of ||s|| doubles as the barrier before the
+          // SpMV on shat
+          __syncthreads();
+          precon_apply<float>(c, P, s, ytmp, sh);
+        }
+        csum<1>(c, *reinterpret_cast<double(*)[1]>(red));
+        if (sqrt(red[0]) <= tgt) {
+          // converged on s: d += alpha phat
+          PhaseTimer ptm(c, 4);
+          vmap<float, 2, 1>(lo, hi, {d, ph}, {d}, [&](const double (&xi)[2], double (&xo)[1]) {
+            xo[0] = xi[0] + alpha * xi[1];
+          });
+          conv_s = true;
+        } else {
+          if (!GRP) precon_apply<float>(c, P, s, ytmp, sh);
+          // t = A shat, (t, t), (t, s)
+          const Sums2 z = spmv_dots<YG_SPMV_DOT2, float, float, GRP>(c, A32, sh, tt, s, nullptr, nullptr);
+          omega = z.s0 != 0.0 ? z.s1 / z.s0 : 0.0;
+          if (omega == 0.0) {
+            brk = true;
+          } else {
+            // d += alpha phat + omega shat; r = s - omega t; (r, r), (rt, r)
+            red[0] = red[1] = 0.0;
+            {
+              PhaseTimer ptm(c, 4);
+              vmap<float, 6, 2>(lo, hi, {d, ph, sh, s, tt, rt}, {d, r},
+                                [&](const double (&xi)[6], double (&xo)[2]) {
+                xo[0] = xi[0] + alpha * xi[1] + omega * xi[2];
+                const double rq = (double)(float)(xi[3] - omega * xi[4]);
+                xo[1] = rq;
+                red[0] += rq * rq;
+                red[1] += xi[5] * rq;
+              });
+            }
+            csum<2>(c, red);
+            rho_old = rho;
+            rho = red[1];
+            const double rit = sqrt(red[0]);
+            upd = rit <= tgt || rit <= o.ir_delta * rupd;
+          }
+        }
+      }
+    }
+    const bool last = out.iters >= o.maxiter || out.nres >= o.ir_max;
+    if (!(brk || conv_s || upd || last)) continue;
+    // fold the correction into x and replace r by the FP64 residual
+    __syncthreads();
+    {
+      PhaseTimer ptm(c, 4);
+      ir_fold(x64, d, rn0, lo, hi);
+    }
+    if (GRP) grp_barrier(c.gv);   // x's halo lines
+    else __syncthreads();
+    const Sums2 z = spmv_res32<GRP>(c, A64, x64, b64, rt, r, sc);
+    ++out.nres;
+    out.res = sqrt(z.s0) / nrm0;
+    out.status = out.res <= o.tol ? 0 : 4;
+    if (out.res <= target) return out;
+    if (brk && since <= 1) { out.status = 2; return out; }
+    if (last) return out;
+    rupd = sqrt(z.s0) * sc;
+    if (brk || conv_s) {
+      // restart the recurrences from the true residual: rt = r, rho = (r, r)
+      red[0] = 0.0;
+      __syncthreads();
+      {
+        PhaseTimer ptm(c, 4);
+        vmap<float, 1, 1>(lo, hi, {r}, {rt}, [&](const double (&xi)[1], double (&xo)[1]) {
+          xo[0] = xi[0];
+          red[0] += xi[0] * xi[0];
+        });
+      }
+      csum<1>(c, *reinterpret_cast<double(*)[1]>(red));
+      __syncthreads();
+      rho = red[0];
+      fresh = true;
+      since = 0;
+    } else {
+      rho = z.s1;   // (rt, r) of the replaced residual
+    }
+  }
+}
+
 template <typename TA, typename TV>
 YG_DEV SolveOut cg(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x, TV* W,
                    const YgKrylovOpts& o) {
@@ -2136,7 +2436,12 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     // reported status still refers to the requested tolerance.
     const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
     int nres = 0;   // FP64 residual passes
-    for (int ir = 0; ir < o.ir_max; ++ir) {
+    const bool reliable = o.ir_delta > 0.0 && o.solver != 1;
+    if (reliable) {
+      const IrOut io = ir_bicgstab<false>(c, A64, A32, P, b64, x64, r64, y64, W, d, o);
+      iters = io.iters; status = io.status; res = io.res; nres = io.nres;
+    }
+    for (int ir = 0; !reliable && ir < o.ir_max; ++ir) {
       ++nres;
       double red[2];
       {
@@ -2343,7 +2648,11 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     int iters = 0, status = 4;
     const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
     int nres = 0;   // FP64 residual passes
-    for (int ir = 0; ir < o.ir_max; ++ir) {
+    if (o.ir_delta > 0.0) {
+      const IrOut io = ir_bicgstab<true>(cm, A64, A32, P, b64, x64, r64, y64, W, d, o);
+      iters = io.iters; status = io.status; res = io.res; nres = io.nres;
+    }
+    for (int ir = 0; o.ir_delta <= 0.0 && ir < o.ir_max; ++ir) {
       grp_barrier(grp);   // x's edge lines are current on every rank
       ++nres;
       double red[2];

@@ -40,9 +40,30 @@ __global__ __launch_bounds__(kSpmvBlock) void k_spmv_records(YgBatchGeom G, cons
   const int nchunk = (int)(nblk * YG_NA * sizeof(T) / 16);
   typedef unsigned V4u __attribute__((ext_vector_type(4)));
   const V4u* src = reinterpret_cast<const V4u*>(A + (size_t)YG_NA * g0);
-  for (int k = threadIdx.x; k < nchunk; k += kSpmvBlock) {
-    const V4u v = __builtin_nontemporal_load(src + k);
-    rec[k] = make_uint4(v.x, v.y, v.z, v.w);
+  // every chunk of this thread (10 FP64, 5 FP32) is requested before the
+  // first is waited for: one load in flight per thread would leave the CU's
+  // memory pipeline mostly idle
+  constexpr int kPer = YG_NA * (int)sizeof(T) / 16;
+  V4u v[kPer];
+  if (nblk == kSpmvBlock) {   // a whole block: no per-chunk bounds
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) v[u] = __builtin_nontemporal_load(src + threadIdx.x + u * kSpmvBlock);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int k = threadIdx.x + u * kSpmvBlock;
+      rec[k] = make_uint4(v[u].x, v[u].y, v[u].z, v[u].w);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int k = threadIdx.x + u * kSpmvBlock;
+      if (k < nchunk) v[u] = __builtin_nontemporal_load(src + k);
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int k = threadIdx.x + u * kSpmvBlock;
+      if (k < nchunk) rec[k] = make_uint4(v[u].x, v[u].y, v[u].z, v[u].w);
+    }
   }
   __syncthreads();
   const long long gq = g0 + threadIdx.x;
