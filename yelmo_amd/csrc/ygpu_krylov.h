@@ -92,6 +92,8 @@ void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, 
 // groups' synchronization words first.
 void yg_launch_krylov_group(const YgKrylovArgs& a, int ngroups, int threads, hipStream_t st);
 #define YG_GSYNC_BYTES 1024
-#define YG_GROUP_THREADS 512
+// threads per workgroup of the member groups that batches and servers pick
+// when all their groups fit the CUs (two workgroups per CU at most)
+#define YG_GROUP_THREADS 256
 // sets the kernels' launch attributes; call before capturing a launch in a graph
 void yg_krylov_prepare();

@@ -218,6 +218,7 @@ struct Server : YgServerBase {
   double n_glen = 0.0;
   bool have_glen = false;
   std::vector<Lane> lanes;
+  int gthreads = YG_GROUP_THREADS;   // threads per member-group workgroup (setup_groups)
   std::vector<Slot> slots;
   std::deque<std::unique_ptr<Req<R>>> pending;
   std::unordered_map<int64_t, std::unique_ptr<Req<R>>> running;  // ticket -> solve in a slot
@@ -620,6 +621,11 @@ struct Server : YgServerBase {
     }
     int ncu = 0;
     YG_CHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    // threads per group workgroup: what a batch of one such member takes
+    // (256 unless -ygpu_group_threads says otherwise), so a served sequence
+    // gives the bits of the same sequence solved alone in a session; the CU
+    // budget below still counts every workgroup as holding a whole CU
+    gthreads = B.opts.group_threads ? B.opts.group_threads : YG_GROUP_THREADS;
     auto groups8 = [](const Lane& L) { return ((L.ns + 7) / 8) * 8; };
     auto enable = [&](Lane& L, int G) -> int {
       L.group = G;
@@ -631,7 +637,7 @@ struct Server : YgServerBase {
       *L.h_gfail = 0;
       return YGPU_OK;
     };
-    auto fits_lines = [&](const Lane& L, int G) { return L.side <= 64 * G * (YG_GROUP_THREADS / 64); };
+    auto fits_lines = [&](const Lane& L, int G) { return L.side <= 64 * G * (gthreads / 64); };
     if (want != 0) {
       Lane& L = lanes.back();
       int G = want;
@@ -684,7 +690,7 @@ struct Server : YgServerBase {
                           B.kmode == YG_MODE_MIXED ? B.d_A32 : nullptr, B.d_b, B.d_x, B.d_mask, st);
     if (L.group > 1) {
       ka.group = L.group; ka.nmembers = L.ns; ka.gsync = L.d_gsync; ka.group_fail = L.d_gfail;
-      yg_launch_krylov_group(ka, L.ngroups, YG_GROUP_THREADS, st);
+      yg_launch_krylov_group(ka, L.ngroups, gthreads, st);
     } else {
       yg_launch_krylov(ka, B.kmode, L.ns, L.block, st);
     }
