@@ -84,6 +84,23 @@ def baseline_legs(args, base, h):
     print(f"# fp64 full device ({len(probs)} members): {len(probs) / t:.1f} solves/s",
           file=sys.stderr, flush=True)
     del probs
+    # the mixed path on the same 256 members: the like-for-like ratio
+    par_m = SsaParams(precision="mixed")
+    probs = [p.copy() for p in base for _ in range(4)]
+    rb = ResidentBatch(probs, par_m, handle=h)
+    rb.run()
+    _sync()
+    t0 = time.perf_counter()
+    rb.run()
+    _sync()
+    tm = time.perf_counter() - t0
+    rb.close()
+    out["mixed_full_device"] = {"members": len(probs), "solves_per_s": len(probs) / tm,
+                                "speedup_vs_fp64_full_device": t / tm,
+                                "note": "the same 256 members, mixed path (default schedule)"}
+    print(f"# mixed full device ({len(probs)} members): {len(probs) / tm:.1f} solves/s",
+          file=sys.stderr, flush=True)
+    del probs
     prod = {}
     for prec in ("fp64", "mixed"):
         par = SsaParams.yelmo_defaults(precision=prec)

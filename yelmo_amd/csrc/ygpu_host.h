@@ -301,12 +301,13 @@ struct Batch {
   // Workgroups per member: the member-group kernel covers the mixed mode
   // with BiCGSTAB and block-Jacobi ILU(0).  Automatic: the largest G of 8,
   // 4, 2 that leaves every workgroup at least 4096 cells of the largest
-  // member and whose groups (a multiple of 8, one per member) fit the CUs
-  // with 256-thread workgroups (up to two per CU), or else with 512-thread
-  // ones (one per CU).  With one workgroup per CU either way (64 x 161^2,
-  // G = 4), four waves run the SpMV 13 % faster than eight (301 against
-  // 292 solves/s): fewer records in flight per CU evict fewer of the
-  // neighbour velocity pairs that the stencil re-reads from the L1.
+  // member, with 256-thread workgroups (up to two per CU); the groups (a
+  // multiple of 8) are as many as the CUs hold at once, at most one per
+  // member, and take the members in turn.  With one workgroup per CU
+  // either way (64 x 161^2, G = 4), four waves run the SpMV 13 % faster
+  // than eight (301 against 292 solves/s): fewer records in flight per CU
+  // evict fewer of the neighbour velocity pairs that the stencil re-reads
+  // from the L1.
   int pick_group(int maxnx, long long maxcells) {
     group = 1;
     ngroups = 0;
@@ -330,11 +331,22 @@ struct Batch {
     auto all_fit = [&](int g, int thr) { return groups8 * g <= ncu0 * (512 / thr); };
     int G = want, thr = opts.group_threads;
     if (G == 0) {
+      // the largest G that leaves every workgroup 4096 cells; when the
+      // groups of all members do not fit the CUs at once, groups take
+      // members in turn (256 members of 161^2: G = 4 in two rounds, 373
+      // solves/s, against G = 2 in one, 339)
       G = 1;
       for (int g : {8, 4, 2}) {
         if (maxcells / g < 4096) continue;
-        if (thr != 512 && all_fit(g, YG_GROUP_THREADS)) { G = g; thr = YG_GROUP_THREADS; break; }
-        if (thr != 256 && all_fit(g, 512)) { G = g; thr = 512; break; }
+        // the group's waves sweep at most 64 lines each
+        int t = thr ? thr : YG_GROUP_THREADS;
+        if (maxnx > 64 * g * (t / 64)) {
+          if (thr == 0 && maxnx <= 64 * g * 8) t = 512;
+          else continue;
+        }
+        G = g;
+        thr = t;
+        break;
       }
       if (G == 1) return YGPU_OK;
     } else if (thr == 0) {
