@@ -303,7 +303,9 @@ typedef struct ygpu_stats {
   int64_t krylov_iters;         /* Krylov iterations summed over members        */
   double unknown_iters;         /* sum over members of N * iterations           */
   int64_t picard_iters;         /* Picard iterations summed over members        */
-  int32_t krylov_group;         /* workgroups per member of the last launch     */
+  int32_t krylov_group;         /* workgroups per member of the last launch (the
+                                   largest group when members of uneven sizes
+                                   have groups of their own sizes)              */
   int32_t krylov_block;         /* threads per workgroup of the last launch     */
   int64_t refine_sweeps;        /* mixed mode: FP64 residual passes summed over
                                    members (their bytes are in krylov_bytes)    */

@@ -174,3 +174,34 @@ def test_reversed_columns_within_mixed_bar(G, cols):
     for k in range(2):
         for n in STATE:
             assert np.array_equal(b_rev[k][n], b_nat[k][n]), (k, n)
+
+
+def test_uneven_batch_members_get_their_own_group_sizes():
+    """Automatic groups on a batch of uneven sizes: each member gets the
+    group size its own grid asks for (57^2: 1, 97^2: 2, 129^2 and 161^2: 4,
+    193^2: 8, 257^2: 16), the groups placed by the launch table.  Every
+    member is within the mixed bar of the converged FP64 GPU path (the FP64
+    path is held to the oracle elsewhere), and a member with a group of two
+    or more gives the bits of the same member solved alone (its group size,
+    line blocks and order of sums do not depend on the batch)."""
+    print("case group uneven", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    sides = ((57, 28e3, 11), (97, 14e3, 12), (129, 10e3, 13), (161, 8e3, 14), (193, 7e3, 15),
+             (257, 5e3, 16))
+    probs = [generator.make_problem(n, dx, seed=s, dtype=np.float64) for n, dx, s in sides]
+    opt = OPT + " -ygpu_factor fp32"
+    (mix, imix), = _batch([p.copy() for p in probs], par, opt)
+    (ref, iref), = _batch([p.copy() for p in probs], _converged(par),
+                          "-i bicgstab -p ilu -maxiter 3000 -tol 1.0e-12 -initx_zeros false")
+    for k, p in enumerate(probs):
+        err = vel_rel_l2(mix[k], ref[k])
+        print(sides[k][0], "picard/krylov/status", imix[k], "fp64", iref[k], "rel L2", err, flush=True)
+        assert imix[k][2] == 0
+        assert err <= 1e-6
+        assert abs(imix[k][0] - iref[k][0]) <= 1
+        if sides[k][0] == 57:
+            continue
+        (alone, ia), = _batch([p.copy()], par, opt)
+        assert ia[0] == imix[k], (sides[k][0], ia[0], imix[k])
+        for n in STATE:
+            assert np.array_equal(alone[0][n], mix[k][n]), (sides[k][0], n)

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <tuple>
 #include <string>
@@ -19,6 +20,7 @@
 #include "../../include/ygpu.h"
 #include "ygpu_diva.h"
 #include "ygpu_fields.h"
+#include "ygpu_gtable.h"
 #include "ygpu_krylov.h"
 #include "ygpu_spmv.h"
 
@@ -129,6 +131,13 @@ struct Batch {
   // member groups: workgroups per member (1: the one-block kernel) and groups
   int group = 1, ngroups = 0, gthreads = 512;
   int *d_gsync = nullptr, *d_gfail = nullptr;
+  // member groups of different sizes (automatic, members of uneven sizes):
+  // every member's group size and the launch table (ygpu_krylov.h); group
+  // reports the largest size and ngroups the groups
+  bool vargroup = false;
+  std::vector<int> mgroup, h_gtab, h_gmoff, h_gmem;
+  int *d_gtab = nullptr, *d_gmoff = nullptr, *d_gmem = nullptr;
+  int gnwg = 0;
   long long total_cells = 0, total_z = 0, total_v = 0;
   int zstride = 0;
   std::vector<YgMember> mem;
@@ -236,7 +245,7 @@ struct Batch {
     M.ncol = 0;
     tuned_block = 0;
     if (kmode == YG_MODE_MIXED && opts.k.precon == 2 && opts.k.ilu_blocks < 0) {
-      auto it = h->sched.find({P.nx, P.ny, group});
+      auto it = h->sched.find({P.nx, P.ny, vargroup ? mgroup[k] : group});
       if (it != h->sched.end()) {
         M.nb = it->second.nb;
         M.ncol = it->second.ncol;
@@ -270,7 +279,9 @@ struct Batch {
       maxnx = std::max(maxnx, probs[k].nx);
       maxcells = std::max(maxcells, (long long)probs[k].nx * probs[k].ny);
     }
-    if ((rc = pick_group(maxnx, maxcells))) return rc;   // before the schedules are looked up
+    std::vector<std::pair<int, int>> sizes(np);
+    for (int k = 0; k < np; ++k) sizes[k] = {probs[k].nx, probs[k].ny};
+    if ((rc = pick_group(maxnx, maxcells, sizes))) return rc;   // before the schedules are looked up
     for (int k = 0; k < np; ++k) {
       const ygpu_ssa_problem& P = probs[k];
       YgMember& M = mem[k];
@@ -299,24 +310,37 @@ struct Batch {
   }
 
   // Workgroups per member: the member-group kernel covers the mixed mode
-  // with BiCGSTAB and block-Jacobi ILU(0).  Automatic: the largest G of 8,
-  // 4, 2 that leaves every workgroup at least 4096 cells of the largest
-  // member, with 256-thread workgroups (up to two per CU); the groups (a
-  // multiple of 8) are as many as the CUs hold at once, at most one per
-  // member, and take the members in turn.  With one workgroup per CU
-  // either way (64 x 161^2, G = 4), four waves run the SpMV 13 % faster
-  // than eight (301 against 292 solves/s): fewer records in flight per CU
-  // evict fewer of the neighbour velocity pairs that the stencil re-reads
-  // from the L1.
-  int pick_group(int maxnx, long long maxcells) {
+  // with BiCGSTAB and block-Jacobi ILU(0).  Automatic: a member of nx x ny
+  // cells gets the largest G of 16, 8, 4, 2 that leaves every workgroup at
+  // least 4096 of its cells, with 256-thread workgroups (up to two per CU).
+  // When every member gets the same G, the groups (a multiple of 8) are as
+  // many as the CUs hold at once, at most one per member, and take the
+  // members in turn.  With one workgroup per CU either way (64 x 161^2,
+  // G = 4), four waves run the SpMV 13 % faster than eight (301 against
+  // 292 solves/s): fewer records in flight per CU evict fewer of the
+  // neighbour velocity pairs that the stencil re-reads from the L1.
+  // Members of uneven sizes get groups of their own sizes (a member's
+  // group size depends on its grid alone), placed by a table (vargroup).
+  static int auto_group(int nx, int ny, int thr) {
+    const long long cells = (long long)nx * ny;
+    const int waves = thr / 64;
+    int g = 1;
+    for (int c : {16, 8, 4, 2})
+      if (cells / c >= 4096 && c <= nx) { g = c; break; }
+    while (g < 16 && nx > 64 * g * waves) g *= 2;   // the group's waves sweep at most 64 lines each
+    return nx > 64 * g * waves ? 0 : g;            // 0: no group size covers the lines
+  }
+
+  int pick_group(int maxnx, long long maxcells, const std::vector<std::pair<int, int>>& sizes) {
     group = 1;
     ngroups = 0;
+    vargroup = false;
     const int want = opts.group;
     if (want == 1) return YGPU_OK;
     const bool fits = kmode == YG_MODE_MIXED && opts.k.solver == 4 && opts.k.precon == 2 &&
                       opts.k.ilu_blocks != 0;
-    if (want != 0 && want != 2 && want != 4 && want != 8) {
-      h->err = "-ygpu_group takes 0, 1, 2, 4 or 8";
+    if (want != 0 && want != 2 && want != 4 && want != 8 && want != 16) {
+      h->err = "-ygpu_group takes 0, 1, 2, 4, 8 or 16";
       return YGPU_ERR_ARG;
     }
     if (!fits) {
@@ -331,22 +355,30 @@ struct Batch {
     auto all_fit = [&](int g, int thr) { return groups8 * g <= ncu0 * (512 / thr); };
     int G = want, thr = opts.group_threads;
     if (G == 0) {
-      // the largest G that leaves every workgroup 4096 cells; when the
-      // groups of all members do not fit the CUs at once, groups take
-      // members in turn (256 members of 161^2: G = 4 in two rounds, 373
-      // solves/s, against G = 2 in one, 339)
-      G = 1;
-      for (int g : {8, 4, 2}) {
-        if (maxcells / g < 4096) continue;
-        // the group's waves sweep at most 64 lines each
-        int t = thr ? thr : YG_GROUP_THREADS;
-        if (maxnx > 64 * g * (t / 64)) {
-          if (thr == 0 && maxnx <= 64 * g * 8) t = 512;
-          else continue;
+      if (thr == 0) thr = YG_GROUP_THREADS;
+      mgroup.assign(nmem, 1);
+      bool same = true, covered = true;
+      for (int k = 0; k < nmem; ++k) {
+        mgroup[k] = auto_group(sizes[k].first, sizes[k].second, thr);
+        same &= mgroup[k] == mgroup[0];
+        covered &= mgroup[k] > 0;
+      }
+      if (!same && covered) return build_group_table(ncu0, thr);
+      G = covered ? mgroup[0] : 1;
+      if (G == 1) {
+        // the size rule of one group size for the largest member (members
+        // too narrow for their own groups)
+        for (int g : {8, 4, 2}) {
+          if (maxcells / g < 4096) continue;
+          int t = thr;
+          if (maxnx > 64 * g * (t / 64)) {
+            if (opts.group_threads == 0 && maxnx <= 64 * g * 8) t = 512;
+            else continue;
+          }
+          G = g;
+          thr = t;
+          break;
         }
-        G = g;
-        thr = t;
-        break;
       }
       if (G == 1) return YGPU_OK;
     } else if (thr == 0) {
@@ -361,6 +393,22 @@ struct Batch {
     }
     group = G;
     ngroups = std::min(groups8, (ncu / G) / 8 * 8);
+    return YGPU_OK;
+  }
+
+  // The launch table of groups of different sizes (mgroup): group_table
+  // (ygpu_gtable.h) with the XCDs' slots at `thr` threads per workgroup.
+  int build_group_table(int ncu0, int thr) {
+    const int per_xcd = (ncu0 / 8) * (512 / thr);
+    std::string e;
+    if (!group_table(mgroup, per_xcd, h_gtab, h_gmoff, h_gmem, gnwg, e)) {
+      h->err = e;
+      return YGPU_ERR_ARG;
+    }
+    vargroup = true;
+    group = *std::max_element(mgroup.begin(), mgroup.end());
+    ngroups = (int)h_gmoff.size() - 1;
+    gthreads = thr;
     return YGPU_OK;
   }
 
@@ -395,6 +443,14 @@ struct Batch {
     if (group > 1) {
       YG_CHECK(h, ar.alloc(&d_gsync, (size_t)ngroups * (YG_GSYNC_BYTES / 4)));
       YG_CHECK(h, ar.alloc(&d_gfail, 1));
+    }
+    if (vargroup) {
+      YG_CHECK(h, ar.alloc(&d_gtab, h_gtab.size()));
+      YG_CHECK(h, ar.alloc(&d_gmoff, h_gmoff.size()));
+      YG_CHECK(h, ar.alloc(&d_gmem, h_gmem.size()));
+      YG_CHECK(h, hipMemcpy(d_gtab, h_gtab.data(), sizeof(int) * h_gtab.size(), hipMemcpyHostToDevice));
+      YG_CHECK(h, hipMemcpy(d_gmoff, h_gmoff.data(), sizeof(int) * h_gmoff.size(), hipMemcpyHostToDevice));
+      YG_CHECK(h, hipMemcpy(d_gmem, h_gmem.data(), sizeof(int) * h_gmem.size(), hipMemcpyHostToDevice));
     }
     // fields
     R** r2d[] = {(R**)&f.c_bed, (R**)&f.f_slide, (R**)&f.taud_acx, (R**)&f.taud_acy,
@@ -846,6 +902,10 @@ struct Batch {
     ka.out_sweeps = d_ksw;
     ka.opts = opts.k;
     ka.group = group; ka.nmembers = nmem; ka.gsync = d_gsync; ka.group_fail = d_gfail;
+    if (vargroup) {
+      ka.gtab = d_gtab; ka.gmoff = d_gmoff; ka.gmem = d_gmem;
+      ka.nwg = gnwg; ka.ngrp = ngroups;
+    }
     if (d_gfail) YG_CHECK(h, hipMemsetAsync(d_gfail, 0, sizeof(int), st));
     long long* d_tm = nullptr;
     const bool phase = getenv("YGPU_PHASE_TIMERS") != nullptr;

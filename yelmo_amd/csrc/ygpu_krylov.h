@@ -82,6 +82,16 @@ struct YgKrylovArgs {
   int nmembers;
   int* gsync;
   int* group_fail;
+  // member groups of different sizes (uneven batches; null gtab: every
+  // group has a.group workgroups, workgroup b is rank (b/8) % G of group
+  // (b/8/G)*8 + b%8, and group g takes members g, g + ngroups, ...):
+  // gtab[3 b + {0, 1, 2}] = group, rank, group size of workgroup b (group
+  // -1: the workgroup has no part and returns); group g takes the members
+  // gmem[gmoff[g] .. gmoff[g+1]) in turn; nwg workgroups and ngrp groups
+  const int* gtab;
+  const int* gmoff;
+  const int* gmem;
+  int nwg, ngrp;
 };
 
 void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, hipStream_t st);
@@ -91,7 +101,7 @@ void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, 
 // in turn.  Mixed mode, BiCGSTAB, block-Jacobi ILU(0) only.  Clears the
 // groups' synchronization words first.
 void yg_launch_krylov_group(const YgKrylovArgs& a, int ngroups, int threads, hipStream_t st);
-#define YG_GSYNC_BYTES 1024
+#define YG_GSYNC_BYTES 2048
 // threads per workgroup of the member groups that batches and servers pick
 // when all their groups fit the CUs (two workgroups per CU at most)
 #define YG_GROUP_THREADS 256

@@ -46,7 +46,7 @@ struct Grp {
   double* part;   // partial sums: [2 (barrier parity)][G][4]
   int G, rank;
 };
-#define YG_GMAX 8   // largest member group
+#define YG_GMAX 16   // largest member group
 struct Ctx {
   const YgMember* mb;
   int nx, ny, T;
@@ -229,9 +229,9 @@ YG_DEV void grp_start(const Grp& g, int* xcc_words) {
 // The partial sums are loaded together (one round trip to the L2) and then
 // added in rank order.  The loads cover YG_GMAX ranks whatever G is: the
 // partial-sum buffer lies inside the group's synchronization block
-// (YG_GSYNC_BYTES from byte 64), which holds [2][YG_GMAX][4] doubles, so
+// (YG_GSYNC_BYTES from byte 128), which holds [2][YG_GMAX][4] doubles, so
 // every load stays inside it; values of ranks past G are not used.
-static_assert(64 + 2 * YG_GMAX * 4 * sizeof(double) <= YG_GSYNC_BYTES, "partial sums");
+static_assert(128 + 2 * YG_GMAX * 4 * sizeof(double) <= YG_GSYNC_BYTES, "partial sums");
 template <int K>
 YG_DEV void csum(const Ctx& c, double (&v)[K]) {
   static_assert(K <= 4, "four partial sums per rank");
@@ -2523,21 +2523,34 @@ template <int NT> constexpr int yg_group_lds() { return NT == 512 ? YG_DYN_LDS :
 
 template <int NT>
 __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) {
-  const int G = a.group;
-  const int blk = blockIdx.x, slot = blk >> 3;
-  const int gi = (slot / G) * 8 + (blk & 7), rank = slot % G;
-  const int ngroups = gridDim.x / G;
+  const int blk = blockIdx.x;
+  int G, gi, rank, k0, k1, ks;   // this workgroup's group, its members k0, k0 + ks, ... < k1
+  if (a.gtab) {
+    // groups of different sizes: this workgroup's place from the table
+    gi = a.gtab[3 * blk];
+    if (gi < 0) return;   // no part in any group
+    rank = a.gtab[3 * blk + 1];
+    G = a.gtab[3 * blk + 2];
+    k0 = a.gmoff[gi]; k1 = a.gmoff[gi + 1]; ks = 1;
+  } else {
+    G = a.group;
+    const int slot = blk >> 3;
+    gi = (slot / G) * 8 + (blk & 7);
+    rank = slot % G;
+    k0 = gi; k1 = a.nmembers; ks = gridDim.x / G;
+  }
   unsigned char* gs = reinterpret_cast<unsigned char*>(a.gsync) + (size_t)gi * YG_GSYNC_BYTES;
   Grp grp;
   grp.cnt = reinterpret_cast<int*>(gs);
   grp.err = reinterpret_cast<int*>(gs) + 1;
-  grp.part = reinterpret_cast<double*>(gs + 64);   // 2 x G x 4 doubles (ints 8..15: XCDs)
+  grp.part = reinterpret_cast<double*>(gs + 128);   // 2 x YG_GMAX x 4 doubles (ints 16..31: XCDs)
   grp.G = G;
   grp.rank = rank;
   __shared__ int s_bad;
   __shared__ long long s_tm[8];
-  grp_start(grp, reinterpret_cast<int*>(gs) + 8);
-  for (int m = gi; m < a.nmembers; m += ngroups) {
+  grp_start(grp, reinterpret_cast<int*>(gs) + 16);
+  for (int k = k0; k < k1; k += ks) {
+    const int m = a.gtab ? a.gmem[k] : k;
     if (a.skip && a.skip[m]) continue;
     const YgMember mb = a.members[m];
     const int nx = mb.nx, ny = mb.ny, ncell = nx * ny;
@@ -2714,8 +2727,9 @@ void yg_krylov_group_attrs_nt() {
 template <int NT>
 void yg_launch_krylov_group_nt(const YgKrylovArgs& a, int ngroups, hipStream_t st) {
   yg_krylov_group_attrs_nt<NT>();
-  const int nwords = ngroups * (YG_GSYNC_BYTES / 4);
+  const int ng = a.gtab ? a.ngrp : ngroups;
+  const int nwords = ng * (YG_GSYNC_BYTES / 4);
   hipLaunchKernelGGL(k_clear_gsync, dim3((nwords + 255) / 256), dim3(256), 0, st, a.gsync, nwords);
-  hipLaunchKernelGGL(yg_krylov_group_kernel<NT>, dim3(ngroups * a.group), dim3(NT),
+  hipLaunchKernelGGL(yg_krylov_group_kernel<NT>, dim3(a.gtab ? a.nwg : ngroups * a.group), dim3(NT),
                      yg_group_lds<NT>(), st, a);
 }

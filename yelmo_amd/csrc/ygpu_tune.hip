@@ -138,7 +138,7 @@ int tune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_problem* pro
   const bool group_ok = o.k.solver == 4 && o.k.precon == 2 && o.k.ilu_blocks != 0;
   Candidate win1{};
   win1.status = -1;
-  for (int G : {1, 2, 4, 8}) {
+  for (int G : {1, 2, 4, 8, 16}) {
     // the group sizes pick_group may choose for this domain
     if (G > 1 && (!group_ok || cells / G < 4096 || 8 * G > ncu)) continue;
     const int copies = (int)std::min<long long>({(long long)ncu / G, by_mem, 64LL});
@@ -177,7 +177,7 @@ int ygpu_autotune(ygpu_handle* h, const ygpu_ssa_params* par, const ygpu_ssa_pro
 int ygpu_set_schedule(ygpu_handle* h, int32_t nx, int32_t ny, int32_t ilu_blocks, int32_t block) {
   if (!h || nx < 3 || ny < 3) return YGPU_ERR_ARG;
   if (block != 0 && block != 512 && block != 1024) return YGPU_ERR_ARG;
-  for (int G : {1, 2, 4, 8}) {
+  for (int G : {1, 2, 4, 8, 16}) {
     if (ilu_blocks <= 0) h->sched.erase({nx, ny, G});
     else h->sched[{nx, ny, G}] = {ilu_blocks, G > 1 ? YG_GROUP_THREADS : block, 0};
   }
