@@ -32,6 +32,8 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.factor16 = 0; o.k.graph_chunk = 4;
   o.k.ilu_rev = 0;
   o.k.ir_delta = 0.0;
+  o.k.ir_loose = -1.0;   // automatic: 10 ssa_iter_conv where a Picard loop runs
+  o.k.loose_inner = 1e-2;
   o.substituted = nullptr;
   o.fields32 = 1;   // the mixed mode's field kernels compute in FP32 unless -ygpu_fields fp64
   o.pattern = 0;
@@ -67,6 +69,17 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
       if (v.is("reliable")) { if (o.k.ir_delta <= 0.0) o.k.ir_delta = 1e-4; }
       else if (v.is("restart")) o.k.ir_delta = 0.0;
       else { err = "-ygpu_ir takes reliable or restart"; return YGPU_ERR_ARG; }
+    }
+    else if (k.is("-ygpu_ir_loose")) {
+      o.k.ir_loose = v.to_double();
+      if (!(o.k.ir_loose >= 0.0)) { err = "-ygpu_ir_loose takes a value >= 0 (0: off)"; return YGPU_ERR_ARG; }
+    }
+    else if (k.is("-ygpu_ir_loose_inner")) {
+      o.k.loose_inner = v.to_double();
+      if (!(o.k.loose_inner > 0.0 && o.k.loose_inner < 1.0)) {
+        err = "-ygpu_ir_loose_inner takes a value in (0, 1)";
+        return YGPU_ERR_ARG;
+      }
     }
     else if (k.is("-ygpu_ir_delta")) {
       o.k.ir_delta = v.to_double();

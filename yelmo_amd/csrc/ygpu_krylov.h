@@ -45,6 +45,11 @@ struct YgKrylovOpts {
   int ilu_rev;      // block-Jacobi ILU(0): 1 = columns of each line block numbered in
                     // reverse (wavefront skew 1); used with FP32/FP64 factors when the
                     // factorization's stage holds a step of the reversed wavefront
+  double ir_loose;  // mixed mode: a member whose last Picard residual (picard_resid)
+                    // is above ir_loose solves to the tolerance itself with inner
+                    // solves to loose_inner; the others (and every member when 0) to
+                    // ir_target with inner solves to inner_tol
+  double loose_inner;
   double ir_delta;  // mixed mode, BiCGSTAB: > 0 one FP32 BiCGSTAB whose residual is
                     // replaced by the FP64 one (b - A x in FP64, x accumulated in FP64)
                     // whenever the iterated residual fell below ir_delta times its
@@ -73,6 +78,8 @@ struct YgKrylovArgs {
   int* out_status;
   double* out_resid;
   int* out_sweeps;          // optional (mixed mode): FP64 residual passes of the refinement
+  const double* picard_resid;   // optional (mixed mode): per member Picard residual of the
+                                // last iteration (opts.ir_loose)
   long long* timers;        // optional: per member 8 phase totals (wall clock ticks)
   YgKrylovOpts opts;
   // member groups (yg_launch_krylov_group): workgroups per member, members,

@@ -2423,22 +2423,28 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
       P.FB = FB;
       P.fb_l = o.factor16 == 1;
     }
+    // Refinement continues past the requested tolerance to ir_target (by
+    // default a tenth of it): the FP32 inner solves then never leave the FP64
+    // residual just under the tolerance, which would carry the
+    // solver-tolerance error of a plain solve into the Picard iterate.  A
+    // member far from Picard convergence (its last Picard residual above
+    // ir_loose) solves to the tolerance itself, with looser inner solves: the
+    // Picard iteration contracts that error away before it converges.  The
+    // reported status refers to the requested tolerance.
+    const bool loose = o.ir_loose > 0.0 && a.picard_resid && a.picard_resid[m] > o.ir_loose;
     YgKrylovOpts oi = o;
-    oi.tol = o.inner_tol;
+    oi.tol = loose ? o.loose_inner : o.inner_tol;
     oi.conv_cond = 0;
     oi.initx_zeros = 1;
     double nrm0 = -1.0, res = 1.0;
     int iters = 0, status = 4;
-    // Refinement continues past the requested tolerance to ir_target (by
-    // default a tenth of it): the FP32 inner solves then never leave the FP64
-    // residual just under the tolerance, which would carry the
-    // solver-tolerance error of a plain solve into the Picard iterate.  The
-    // reported status still refers to the requested tolerance.
-    const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
+    const double target = loose ? o.tol : o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
     int nres = 0;   // FP64 residual passes
     const bool reliable = o.ir_delta > 0.0 && o.solver != 1;
     if (reliable) {
-      const IrOut io = ir_bicgstab<false>(c, A64, A32, P, b64, x64, r64, y64, W, d, o);
+      YgKrylovOpts orl = o;
+      orl.ir_target = target;
+      const IrOut io = ir_bicgstab<false>(c, A64, A32, P, b64, x64, r64, y64, W, d, orl);
       iters = io.iters; status = io.status; res = io.res; nres = io.nres;
     }
     for (int ir = 0; !reliable && ir < o.ir_max; ++ir) {
@@ -2653,16 +2659,20 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     }
     double bad[1] = {(double)s_bad};
     csum<1>(cm, bad);   // a zero pivot on any rank
+    // a member far from Picard convergence solves to the tolerance itself
+    const bool loose = o.ir_loose > 0.0 && a.picard_resid && a.picard_resid[m] > o.ir_loose;
     YgKrylovOpts oi = o;
-    oi.tol = o.inner_tol;
+    oi.tol = loose ? o.loose_inner : o.inner_tol;
     oi.conv_cond = 0;
     oi.initx_zeros = 1;
     double nrm0 = -1.0, res = 1.0;
     int iters = 0, status = 4;
-    const double target = o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
+    const double target = loose ? o.tol : o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
     int nres = 0;   // FP64 residual passes
     if (o.ir_delta > 0.0) {
-      const IrOut io = ir_bicgstab<true>(cm, A64, A32, P, b64, x64, r64, y64, W, d, o);
+      YgKrylovOpts orl = o;
+      orl.ir_target = target;
+      const IrOut io = ir_bicgstab<true>(cm, A64, A32, P, b64, x64, r64, y64, W, d, orl);
       iters = io.iters; status = io.status; res = io.res; nres = io.nres;
     }
     for (int ir = 0; o.ir_delta <= 0.0 && ir < o.ir_max; ++ir) {
