@@ -1,15 +1,19 @@
-"""Mixed precision with reliable updates (-ygpu_ir reliable): one FP32
-BiCGSTAB per linear solve whose iterated residual is replaced by the FP64
-residual b - A x (x accumulated in FP64) each time it has fallen by the
-factor -ygpu_ir_delta, the search direction kept, instead of a new inner
-solve per FP64 residual.  The stopping rule is the refinement's (FP64
-residual below a tenth of -tol), so the bars are the mixed path's:
+"""Refinement variants of the mixed precision.
+
+Reliable updates (-ygpu_ir reliable): one FP32 BiCGSTAB per linear solve
+whose iterated residual is replaced by the FP64 residual b - A x (x
+accumulated in FP64) each time it has fallen by the factor -ygpu_ir_delta,
+the search direction kept, instead of a new inner solve per FP64 residual.
+Loose solves far from Picard convergence (-ygpu_ir_loose): refinement to
+-tol itself while a member's Picard residual is large.  The stopping rules
+are the refinement's, so the bars are the mixed path's:
 
   velocity vs converged FP64 oracle      1e-6 rel L2, Picard count +-1
   CSR solve vs converged FP64 oracle      1e-6 rel L2, status 0, FP64 residual <= tol
   graph replays vs eager loop             bitwise (group sums in a fixed order)
   iteration limit                         status 4 after exactly maxiter iterations,
                                           x finite (the FP32 correction folded in)
+  loose rule off / never triggered        bitwise equal
 """
 import numpy as np
 import pytest
@@ -120,6 +124,44 @@ def test_reliable_option_values():
     h = _lib.default_handle()
     ap, ai, av, b, x0, _ = _mid_picard_system(21, 60e3)
     for bad in ("-ygpu_ir sometimes", "-ygpu_ir_delta 1.5", "-ygpu_ir_delta 0"):
+        lgs = velocity.Lgs(ap, ai, av, b, x0.copy())
+        with pytest.raises(Exception):
+            velocity.linear_solver_matrix_solve(lgs, "-i bicgstab -p ilu -ygpu_precision mixed " + bad,
+                                                handle=h)
+
+
+def test_loose_refinement_far_from_picard_convergence():
+    """-ygpu_ir_loose R: members whose last Picard residual is above R stop
+    refining at -tol with looser inner solves (automatic R: 10
+    ssa_iter_conv).  Off (0) and a threshold no residual reaches give the
+    same bits; the automatic rule stays within the mixed bar of the
+    converged oracle, Picard count +-1, with no more Krylov iterations
+    over the members."""
+    print("case loose refinement", flush=True)
+    par = SsaParams(visc_method=1, beta_method=2, precision="mixed")
+    probs = [generator.make_problem(97, 14e3, seed=21, dtype=np.float64),
+             generator.make_problem(129, 10e3, seed=22, dtype=np.float64)]
+    base = "-i bicgstab -p ilu -maxiter 1000 -tol 1.0e-8 -initx_zeros false -ygpu_factor fp32"
+    (s_auto, i_auto), = _batch([p.copy() for p in probs], par, base)
+    (s_off, i_off), = _batch([p.copy() for p in probs], par, base + " -ygpu_ir_loose 0")
+    (s_never, i_never), = _batch([p.copy() for p in probs], par, base + " -ygpu_ir_loose 1e30")
+    assert i_off == i_never
+    for k in range(len(probs)):
+        for n in STATE:
+            assert np.array_equal(s_off[k][n], s_never[k][n])
+    for k, p in enumerate(probs):
+        ref_st, ref = oracle.calc_velocity_ssa(p.copy(), _converged(par), dp=True)
+        e_auto, e_off = vel_rel_l2(s_auto[k], ref_st), vel_rel_l2(s_off[k], ref_st)
+        print(k, "auto", i_auto[k], e_auto, "off", i_off[k], e_off, "oracle picard",
+              ref["ssa_iter_now"], flush=True)
+        assert i_auto[k][2] == 0 and e_auto <= 1e-6 and e_off <= 1e-6
+        assert abs(i_auto[k][0] - ref["ssa_iter_now"]) <= 1
+    # fewer Krylov iterations over the members (a member's own count can
+    # come out a few iterations either way)
+    assert sum(i[1] for i in i_auto) <= sum(i[1] for i in i_off)
+    h = _lib.default_handle()
+    ap, ai, av, b, x0, _ = _mid_picard_system(21, 60e3)
+    for bad in ("-ygpu_ir_loose -1", "-ygpu_ir_loose_inner 2"):
         lgs = velocity.Lgs(ap, ai, av, b, x0.copy())
         with pytest.raises(Exception):
             velocity.linear_solver_matrix_solve(lgs, "-i bicgstab -p ilu -ygpu_precision mixed " + bad,
