@@ -31,7 +31,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.ilu_cols = 1;
   o.k.factor16 = 0; o.k.graph_chunk = 4;
   o.k.ilu_rev = 0;
-  o.k.ir_delta = 0.0;
+  o.k.ir_delta = 1e-3;   // reliable updates (-ygpu_ir restart: 0)
   o.k.ir_loose = -1.0;   // automatic: 10 ssa_iter_conv where a Picard loop runs
   o.k.loose_inner = 1e-2;
   o.substituted = nullptr;
@@ -66,7 +66,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
     else if (k.is("-ygpu_ir_tol")) o.k.ir_target = v.to_double();
     else if (k.is("-ir_max")) o.k.ir_max = v.to_int();
     else if (k.is("-ygpu_ir")) {
-      if (v.is("reliable")) { if (o.k.ir_delta <= 0.0) o.k.ir_delta = 1e-4; }
+      if (v.is("reliable")) { if (o.k.ir_delta <= 0.0) o.k.ir_delta = 1e-3; }
       else if (v.is("restart")) o.k.ir_delta = 0.0;
       else { err = "-ygpu_ir takes reliable or restart"; return YGPU_ERR_ARG; }
     }

@@ -50,12 +50,12 @@ struct YgKrylovOpts {
                     // solves to loose_inner; the others (and every member when 0) to
                     // ir_target with inner solves to inner_tol
   double loose_inner;
-  double ir_delta;  // mixed mode, BiCGSTAB: > 0 one FP32 BiCGSTAB whose residual is
-                    // replaced by the FP64 one (b - A x in FP64, x accumulated in FP64)
-                    // whenever the iterated residual fell below ir_delta times its
-                    // value at the last replacement, the search direction kept
-                    // (reliable updates); 0 restarts a new inner solve at every FP64
-                    // residual instead (inner_tol)
+  double ir_delta;  // mixed mode, BiCGSTAB: > 0 (1e-3 by default) one FP32 BiCGSTAB
+                    // whose residual is replaced by the FP64 one (b - A x in FP64, x
+                    // accumulated in FP64) whenever the iterated residual fell below
+                    // ir_delta times its value at the last replacement, the search
+                    // direction kept (reliable updates); 0 restarts a new inner solve
+                    // at every FP64 residual instead (inner_tol)
 };
 
 struct YgKrylovArgs {
