@@ -99,3 +99,40 @@ def test_energy_diva_fp64_matches_oracle():
     assert info.status == 0
     assert abs(info.ssa_iter_now - ref["ssa_iter_now"]) <= 1
     assert err <= 1e-8
+
+
+BIG = ((97, 12e3, "infinite", 31), (129, 10e3, "zeros", 32))
+SMALL = ((41, 28e3, "infinite", 33),)
+
+
+@pytest.mark.parametrize("form,opt,G,sides", [
+    ("energy", "-i cg -p jacobi -maxiter 4000 -tol 1.0e-8 -initx_zeros false", 4, BIG),
+    ("energy", "-i cg -p jacobi -maxiter 4000 -tol 1.0e-8 -initx_zeros false", 2, BIG),
+    ("energy", "-i cg -p none -maxiter 8000 -tol 1.0e-8 -initx_zeros false", 2, SMALL),
+    ("residual", "-i bicgstab -p jacobi -maxiter 3000 -tol 1.0e-8 -initx_zeros false", 4, BIG),
+])
+def test_member_groups_cg_and_jacobi_within_mixed_bar(form, opt, G, sides):
+    """Member groups (G workgroups per member) with CG or Jacobi: the energy
+    form with CG + Jacobi (Yelmo's defaults) or no preconditioner, and the
+    residual form with BiCGSTAB + Jacobi, in mixed precision (97^2 with ice
+    to the edge and 129^2; 41^2 without a preconditioner, whose converged
+    oracle solve is long): within the mixed bar of the oracle converged at
+    rtol 1e-12, Picard counts +-1."""
+    print(f"case groups {form} {opt.split()[1]}+{opt.split()[3]} G={G}", flush=True)
+    for n, dx, bnd, seed in sides:
+        par = SsaParams(ssa_solver=form, boundaries=bnd, visc_method=1, beta_method=2,
+                        precision="mixed", ssa_lis_opt=opt + f" -ygpu_group {G}")
+        p = generator.make_problem(n, dx, seed=seed, dtype=np.float64, boundaries=bnd)
+        conv = par.with_(precision="fp64",
+                         ssa_lis_opt=opt.replace("1.0e-8", "1.0e-12").replace("maxiter 4000", "maxiter 20000")
+                         .replace("maxiter 8000", "maxiter 40000").replace("maxiter 3000", "maxiter 20000"))
+        ref_st, ref = oracle.calc_velocity_ssa(p.copy(), conv, dp=True)
+        assert ref["rc"] == 0
+        g = p.copy()
+        info = velocity.calc_velocity_ssa_batch([g], par)[0]
+        err = vel_rel_l2(g.state, ref_st)
+        print(n, bnd, "picard", info.ssa_iter_now, "oracle", ref["ssa_iter_now"], "lin",
+              info.ssa_lin_iter, "rel L2", err, flush=True)
+        assert info.status == 0
+        assert abs(info.ssa_iter_now - ref["ssa_iter_now"]) <= 1
+        assert err <= 1e-6

@@ -328,6 +328,14 @@ struct Batch {
   // neighbour velocity pairs that the stencil re-reads from the L1.
   // Members of uneven sizes get groups of their own sizes (a member's
   // group size depends on its grid alone), placed by a table (vargroup).
+  // The solves the member-group kernel runs: mixed precision, BiCGSTAB with
+  // line-block ILU(0), Jacobi or no preconditioner, or CG with Jacobi or none.
+  static bool group_solver_fits(int kmode, const YgKrylovOpts& k) {
+    if (kmode != YG_MODE_MIXED) return false;
+    if (k.solver == 4) return k.precon != 2 || k.ilu_blocks != 0;
+    return k.solver == 1 && k.precon != 2;
+  }
+
   static int auto_group(int nx, int ny, int thr) {
     const long long cells = (long long)nx * ny;
     const int waves = thr / 64;
@@ -344,15 +352,15 @@ struct Batch {
     vargroup = false;
     const int want = opts.group;
     if (want == 1) return YGPU_OK;
-    const bool fits = kmode == YG_MODE_MIXED && opts.k.solver == 4 && opts.k.precon == 2 &&
-                      opts.k.ilu_blocks != 0;
+    const bool fits = group_solver_fits(kmode, opts.k);
     if (want != 0 && want != 2 && want != 4 && want != 8 && want != 16) {
       h->err = "-ygpu_group takes 0, 1, 2, 4, 8 or 16";
       return YGPU_ERR_ARG;
     }
     if (!fits) {
       if (want == 0) return YGPU_OK;
-      h->err = "-ygpu_group: member groups run the mixed precision, BiCGSTAB, block ILU(0) solve";
+      h->err = "-ygpu_group: member groups run the mixed precision with BiCGSTAB (block ILU(0), "
+               "Jacobi or none) or CG (Jacobi or none)";
       return YGPU_ERR_UNSUPPORTED;
     }
     int ncu0 = 0;

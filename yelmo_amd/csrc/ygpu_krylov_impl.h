@@ -2220,6 +2220,95 @@ YG_DEV IrOut ir_bicgstab(const Ctx& c, const double* A64, const float* A32,
   }
 }
 
+// CG over the rows [c.cb, c.ce) of a member group's member, with Jacobi
+// (P.kind 1: z = r * 1/diag, P.F the inverse diagonal per value) or no
+// preconditioner, fused into the update pass; x = 0 on entry (the
+// refinement's inner solves), so r0 = b and conv_cond 0 and 1 agree.  Per
+// iteration a group barrier (the halo lines of p), the sum (p, A p) and the
+// sums (r, r), (r, z): every block takes the same branches.
+template <typename TA, typename TV>
+YG_DEV SolveOut cg_grp(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x, TV* W,
+                       const YgKrylovOpts& o) {
+  const int lo = 2 * c.cb, hi = 2 * c.ce;
+  const size_t vl = c.vlen;
+  TV* __restrict__ r = W;
+  TV* __restrict__ z = W + 1 * vl;
+  TV* __restrict__ p = W + 2 * vl;
+  TV* __restrict__ q = W + 3 * vl;
+  const bool jac = P.kind == 1;
+  const TV* dinv = P.F;
+  SolveOut out{0, 4, 0.0};
+  double red[2] = {0.0, 0.0};
+  // r = b, z = M^-1 r, p = z; (r, r), (r, z)
+  {
+    PhaseTimer ptm(c, 4);
+    if (jac) {
+      vmap<TV, 2, 3>(lo, hi, {b, dinv}, {r, z, p}, [&](const double (&xi)[2], double (&xo)[3]) {
+        const double zz = (double)(TV)(xi[0] * xi[1]);
+        xo[0] = xi[0]; xo[1] = zz; xo[2] = zz;
+        red[0] += xi[0] * xi[0];
+        red[1] += xi[0] * zz;
+      });
+    } else {
+      vmap<TV, 1, 3>(lo, hi, {b}, {r, z, p}, [&](const double (&xi)[1], double (&xo)[3]) {
+        xo[0] = xo[1] = xo[2] = xi[0];
+        red[0] += xi[0] * xi[0];
+        red[1] += xi[0] * xi[0];
+      });
+    }
+  }
+  csum<2>(c, red);
+  const double nrm0 = sqrt(red[0]);
+  if (nrm0 == 0.0) { out.status = 0; return out; }
+  const double bnrm = 1.0 / nrm0;
+  double rho = red[1];
+  for (int it = 1; it <= o.maxiter; ++it) {
+    if (rho == 0.0) { out.status = 2; out.iters = it; return out; }
+    grp_barrier(c.gv);   // the halo lines of p
+    const double pq = spmv_dots<YG_SPMV_DOT, TA, TV, true>(c, A, p, q, p, nullptr, nullptr).s0;
+    if (pq == 0.0) { out.status = 2; out.iters = it; return out; }
+    const double alpha = rho / pq;
+    // x += alpha p, r -= alpha q, z = M^-1 r; (r, r), (r, z)
+    red[0] = red[1] = 0.0;
+    {
+      PhaseTimer ptm(c, 4);
+      if (jac) {
+        vmap<TV, 5, 3>(lo, hi, {x, p, r, q, dinv}, {x, r, z},
+                       [&](const double (&xi)[5], double (&xo)[3]) {
+          xo[0] = xi[0] + alpha * xi[1];
+          const double rq = (double)(TV)(xi[2] - alpha * xi[3]);
+          const double zq = (double)(TV)(rq * xi[4]);
+          xo[1] = rq; xo[2] = zq;
+          red[0] += rq * rq;
+          red[1] += rq * zq;
+        });
+      } else {
+        vmap<TV, 4, 3>(lo, hi, {x, p, r, q}, {x, r, z}, [&](const double (&xi)[4], double (&xo)[3]) {
+          xo[0] = xi[0] + alpha * xi[1];
+          const double rq = (double)(TV)(xi[2] - alpha * xi[3]);
+          xo[1] = xo[2] = rq;
+          red[0] += rq * rq;
+          red[1] += rq * rq;
+        });
+      }
+    }
+    csum<2>(c, red);
+    out.resid = sqrt(red[0]) * bnrm;
+    out.iters = it;
+    if (out.resid <= o.tol) { out.status = 0; return out; }
+    const double beta = red[1] / rho;
+    rho = red[1];
+    {
+      PhaseTimer ptm(c, 4);
+      vmap<TV, 2, 1>(lo, hi, {z, p}, {p}, [&](const double (&xi)[2], double (&xo)[1]) {
+        xo[0] = xi[0] + beta * xi[1];
+      });
+    }
+  }
+  out.status = 4;
+  return out;
+}
+
 template <typename TA, typename TV>
 YG_DEV SolveOut cg(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x, TV* W,
                    const YgKrylovOpts& o) {
@@ -2648,14 +2737,28 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
       for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) x64[q] = 0.0;
     }
     Precon<float> P;
-    P.kind = 2;
-    P.F = F + (size_t)8 * cm.cb;   // this part's L factors (rotated tile order)
-    if (Lg > 0) ilu0_factor<float>(cr, A32 + (size_t)YG_NA * cm.cb, F + (size_t)8 * cm.cb, &s_bad);
-    if (o.factor16 && a.F16) {
-      uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff + (size_t)8 * cm.cb;
-      if (Lg > 0) factors_to_bf16(cr, P.F, FB);
-      P.FB = FB;
-      P.fb_l = o.factor16 == 1;
+    if (o.precon == 2) {
+      P.kind = 2;
+      P.F = F + (size_t)8 * cm.cb;   // this part's L factors (rotated tile order)
+      if (Lg > 0) ilu0_factor<float>(cr, A32 + (size_t)YG_NA * cm.cb, F + (size_t)8 * cm.cb, &s_bad);
+      if (o.factor16 && a.F16) {
+        uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff + (size_t)8 * cm.cb;
+        if (Lg > 0) factors_to_bf16(cr, P.F, FB);
+        P.FB = FB;
+        P.fb_l = o.factor16 == 1;
+      }
+    } else {
+      // Jacobi (the inverse diagonal of this rank's rows, by value in the
+      // member's row order) or none
+      P.kind = o.precon;
+      P.F = F;
+      if (o.precon == 1) {
+        for (int q = cm.cb + threadIdx.x; q < cm.ce; q += blockDim.x) {
+          F[2 * q] = (float)(1.0 / (double)A32[(size_t)q * YG_NA + 0]);
+          F[2 * q + 1] = (float)(1.0 / (double)A32[(size_t)q * YG_NA + 1]);
+        }
+        __syncthreads();
+      }
     }
     double bad[1] = {(double)s_bad};
     csum<1>(cm, bad);   // a zero pivot on any rank
@@ -2669,13 +2772,14 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     int iters = 0, status = 4;
     const double target = loose ? o.tol : o.ir_target > 0.0 ? o.ir_target : 1e-1 * o.tol;
     int nres = 0;   // FP64 residual passes
-    if (o.ir_delta > 0.0) {
+    const bool reliable = o.ir_delta > 0.0 && o.solver != 1;
+    if (reliable) {
       YgKrylovOpts orl = o;
       orl.ir_target = target;
       const IrOut io = ir_bicgstab<true>(cm, A64, A32, P, b64, x64, r64, y64, W, d, orl);
       iters = io.iters; status = io.status; res = io.res; nres = io.nres;
     }
-    for (int ir = 0; o.ir_delta <= 0.0 && ir < o.ir_max; ++ir) {
+    for (int ir = 0; !reliable && ir < o.ir_max; ++ir) {
       grp_barrier(grp);   // x's edge lines are current on every rank
       ++nres;
       double red[2];
@@ -2695,7 +2799,8 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
       refine_scale(r64, rin, d, sc, lo, hi);
       __syncthreads();
       oi.maxiter = o.maxiter - iters;
-      SolveOut so = bicgstab<float, float, true>(cm, A32, P, rin, d, W, oi);
+      SolveOut so = o.solver == 1 ? cg_grp<float, float>(cm, A32, P, rin, d, W, oi)
+                                  : bicgstab<float, float, true>(cm, A32, P, rin, d, W, oi);
       iters += so.iters;
       refine_update(x64, d, rn, lo, hi);
       __syncthreads();

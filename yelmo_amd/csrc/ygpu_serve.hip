@@ -612,8 +612,7 @@ struct Server : YgServerBase {
   // waiting for anything.
   int setup_groups() {
     const int want = B.opts.group;
-    const bool fits = B.kmode == YG_MODE_MIXED && B.opts.k.solver == 4 && B.opts.k.precon == 2 &&
-                      B.opts.k.ilu_blocks != 0;
+    const bool fits = Batch<R>::group_solver_fits(B.kmode, B.opts.k);
     if (want == 1 || !fits) return YGPU_OK;
     if (want != 0 && want != 2 && want != 4 && want != 8) {
       h->err = "-ygpu_group takes 0, 1, 2, 4 or 8";
