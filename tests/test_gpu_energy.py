@@ -136,3 +136,31 @@ def test_member_groups_cg_and_jacobi_within_mixed_bar(form, opt, G, sides):
         assert info.status == 0
         assert abs(info.ssa_iter_now - ref["ssa_iter_now"]) <= 1
         assert err <= 1e-6
+
+
+@pytest.mark.parametrize("form,opt,n,dx,bnd", [
+    ("energy", "-i cg -p jacobi -maxiter 4000 -tol 1.0e-8 -initx_zeros false", 97, 12e3, "infinite"),
+    ("energy", "-i cg -p jacobi -maxiter 4000 -tol 1.0e-8 -initx_zeros false", 129, 10e3, "zeros"),
+    ("residual", "-i bicgstab -p jacobi -maxiter 3000 -tol 1.0e-8 -initx_zeros false", 129, 10e3, "zeros"),
+])
+def test_fp64_member_groups_match_fp64_oracle(form, opt, n, dx, bnd):
+    """The plain FP64 path with Jacobi split over a member group (four
+    workgroups): the reference's Krylov solve from x0, so it matches the
+    FP64 oracle at the reference's tolerance like the one-workgroup kernel:
+    1e-8 rel L2, Picard +-1, Krylov iterations within 2 % (sums in another
+    order)."""
+    print(f"case fp64 groups {form} n={n}", flush=True)
+    par = SsaParams(ssa_solver=form, boundaries=bnd, visc_method=1, beta_method=2,
+                    precision="fp64", ssa_lis_opt=opt)
+    p = generator.make_problem(n, dx, seed=41, dtype=np.float64, boundaries=bnd)
+    ref_st, ref = oracle.calc_velocity_ssa(p.copy(), par, dp=True)
+    assert ref["rc"] == 0
+    g = p.copy()
+    info = velocity.calc_velocity_ssa_batch([g], par.with_(ssa_lis_opt=opt + " -ygpu_group 4"))[0]
+    err = vel_rel_l2(g.state, ref_st)
+    print("picard", info.ssa_iter_now, ref["ssa_iter_now"], "lin", info.ssa_lin_iter,
+          ref["ssa_lin_iter"], "rel L2", err, flush=True)
+    assert info.status == 0
+    assert abs(info.ssa_iter_now - ref["ssa_iter_now"]) <= 1
+    assert abs(info.ssa_lin_iter - ref["ssa_lin_iter"]) <= 0.02 * ref["ssa_lin_iter"] + 1
+    assert err <= 1e-8

@@ -329,11 +329,14 @@ struct Batch {
   // Members of uneven sizes get groups of their own sizes (a member's
   // group size depends on its grid alone), placed by a table (vargroup).
   // The solves the member-group kernel runs: mixed precision, BiCGSTAB with
-  // line-block ILU(0), Jacobi or no preconditioner, or CG with Jacobi or none.
+  // line-block ILU(0), Jacobi or no preconditioner, or CG with Jacobi or
+  // none; plain FP64, BiCGSTAB or CG with Jacobi or none (the FP64 path's
+  // ILU(0) is the exact one, a wavefront over all lines of a member).
   static bool group_solver_fits(int kmode, const YgKrylovOpts& k) {
-    if (kmode != YG_MODE_MIXED) return false;
+    if (k.solver != 1 && k.solver != 4) return false;
+    if (kmode == YG_MODE_FP64) return k.precon != 2;
     if (k.solver == 4) return k.precon != 2 || k.ilu_blocks != 0;
-    return k.solver == 1 && k.precon != 2;
+    return k.precon != 2;
   }
 
   static int auto_group(int nx, int ny, int thr) {
@@ -359,8 +362,8 @@ struct Batch {
     }
     if (!fits) {
       if (want == 0) return YGPU_OK;
-      h->err = "-ygpu_group: member groups run the mixed precision with BiCGSTAB (block ILU(0), "
-               "Jacobi or none) or CG (Jacobi or none)";
+      h->err = "-ygpu_group: member groups run BiCGSTAB or CG with Jacobi or no preconditioner, "
+               "and in mixed precision BiCGSTAB with line-block ILU(0)";
       return YGPU_ERR_UNSUPPORTED;
     }
     int ncu0 = 0;
@@ -916,6 +919,7 @@ struct Batch {
     ka.skip = d_done; ka.out_iters = d_kit; ka.out_status = d_kst; ka.out_resid = d_kres;
     ka.out_sweeps = d_ksw;
     ka.picard_resid = d_resid;
+    ka.plain64 = kmode == YG_MODE_FP64;
     ka.opts = opts.k;
     ka.group = group; ka.nmembers = nmem; ka.gsync = d_gsync; ka.group_fail = d_gfail;
     if (vargroup) {

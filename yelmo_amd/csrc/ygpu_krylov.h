@@ -99,14 +99,18 @@ struct YgKrylovArgs {
   const int* gmoff;
   const int* gmem;
   int nwg, ngrp;
+  // member groups in the plain FP64 mode (Jacobi or no preconditioner; W64's
+  // nine vectors hold the Krylov vectors); 0: the mixed mode
+  int plain64;
 };
 
 void yg_launch_krylov(const YgKrylovArgs& a, int mode, int nmembers, int block, hipStream_t st);
 // The member-group kernel: ngroups groups of a.group workgroups (threads 512,
 // one workgroup per CU, or 256, two per CU; ngroups a multiple of 8, the
 // workgroups at most what the CUs hold at once) take the a.nmembers members
-// in turn.  Mixed mode, BiCGSTAB, block-Jacobi ILU(0) only.  Clears the
-// groups' synchronization words first.
+// in turn: the mixed mode (BiCGSTAB with line-block ILU(0), Jacobi or none;
+// CG with Jacobi or none) or, with a.plain64, the FP64 mode with Jacobi or
+// no preconditioner.  Clears the groups' synchronization words first.
 void yg_launch_krylov_group(const YgKrylovArgs& a, int ngroups, int threads, hipStream_t st);
 #define YG_GSYNC_BYTES 2048
 // threads per workgroup of the member groups that batches and servers pick

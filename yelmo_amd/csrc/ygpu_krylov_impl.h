@@ -1854,11 +1854,12 @@ YG_DEV double count_nonzero(const double* __restrict__ b64, int lo, int hi) {
 
 // BiCGSTAB over the rows [c.cb, c.ce) of the member.  GRP: a member group
 // (the rows are this block's share; every sum is a group sum and a group
-// barrier precedes each SpMV whose x other blocks wrote).  GRP requires
-// x = 0 on entry (the refinement's inner solves), where r0 = b needs no SpMV.
+// barrier precedes each SpMV whose x other blocks wrote).  With x0zero
+// (GRP only: the refinement's inner solves) x = 0 on entry and r0 = b needs
+// no SpMV.
 template <typename TA, typename TV, bool GRP = false>
 YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x,
-                         TV* W, const YgKrylovOpts& o) {
+                         TV* W, const YgKrylovOpts& o, bool x0zero = GRP) {
   const int lo = 2 * c.cb, hi = 2 * c.ce;
   const size_t vl = c.vlen;   // vector stride in W (16-byte aligned)
   TV* __restrict__ r = W;
@@ -1873,7 +1874,7 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
   SolveOut out{0, 4, 0.0};
   double red[2];
   // r = b - A x, rt = r, (r,r), (b,b)
-  if (GRP) {
+  if (GRP && x0zero) {
     red[0] = 0.0;
     vmap<TV, 1, 2>(lo, hi, {b}, {r, rt}, [&](const double (&xi)[1], double (&xo)[2]) {
       xo[0] = xo[1] = xi[0];
@@ -1882,7 +1883,8 @@ YG_DEV SolveOut bicgstab(const Ctx& c, const TA* A, const Precon<TV>& P, const T
     csum<1>(c, *reinterpret_cast<double(*)[1]>(red));
     red[1] = red[0];
   } else {
-    const Sums2 z = spmv_dots<YG_SPMV_RES, TA, TV>(c, A, x, tt, b, r, rt);
+    if (GRP) grp_barrier(c.gv);   // the halo lines of x
+    const Sums2 z = spmv_dots<YG_SPMV_RES, TA, TV, GRP>(c, A, x, tt, b, r, rt);
     red[0] = z.s0;
     red[1] = z.s1;
   }
@@ -2222,13 +2224,14 @@ YG_DEV IrOut ir_bicgstab(const Ctx& c, const double* A64, const float* A32,
 
 // CG over the rows [c.cb, c.ce) of a member group's member, with Jacobi
 // (P.kind 1: z = r * 1/diag, P.F the inverse diagonal per value) or no
-// preconditioner, fused into the update pass; x = 0 on entry (the
-// refinement's inner solves), so r0 = b and conv_cond 0 and 1 agree.  Per
-// iteration a group barrier (the halo lines of p), the sum (p, A p) and the
-// sums (r, r), (r, z): every block takes the same branches.
+// preconditioner, fused into the update pass.  With x0zero (the
+// refinement's inner solves) x = 0 on entry and r0 = b; else r0 = b - A x0
+// (conv_cond 0: relative to ||r0||, 1: to ||b||).  Per iteration a group
+// barrier (the halo lines of p), the sum (p, A p) and the sums (r, r),
+// (r, z): every block takes the same branches.
 template <typename TA, typename TV>
 YG_DEV SolveOut cg_grp(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, TV* x, TV* W,
-                       const YgKrylovOpts& o) {
+                       const YgKrylovOpts& o, bool x0zero = true) {
   const int lo = 2 * c.cb, hi = 2 * c.ce;
   const size_t vl = c.vlen;
   TV* __restrict__ r = W;
@@ -2239,6 +2242,14 @@ YG_DEV SolveOut cg_grp(const Ctx& c, const TA* A, const Precon<TV>& P, const TV*
   const TV* dinv = P.F;
   SolveOut out{0, 4, 0.0};
   double red[2] = {0.0, 0.0};
+  double bb = 0.0;   // (b, b) when x0 is not zero
+  if (!x0zero) {
+    // r = b - A x0 in place of b below
+    grp_barrier(c.gv);   // the halo lines of x
+    const Sums2 zs = spmv_dots<YG_SPMV_RES, TA, TV, true>(c, A, x, q, b, r, nullptr);
+    bb = zs.s1;
+    b = r;
+  }
   // r = b, z = M^-1 r, p = z; (r, r), (r, z)
   {
     PhaseTimer ptm(c, 4);
@@ -2258,9 +2269,11 @@ YG_DEV SolveOut cg_grp(const Ctx& c, const TA* A, const Precon<TV>& P, const TV*
     }
   }
   csum<2>(c, red);
-  const double nrm0 = sqrt(red[0]);
+  const double nrm0 = !x0zero && o.conv_cond == 1 ? sqrt(bb) : sqrt(red[0]);
   if (nrm0 == 0.0) { out.status = 0; return out; }
   const double bnrm = 1.0 / nrm0;
+  out.resid = sqrt(red[0]) * bnrm;
+  if (out.resid <= o.tol) { out.status = 0; return out; }
   double rho = red[1];
   for (int it = 1; it <= o.maxiter; ++it) {
     if (rho == 0.0) { out.status = 2; out.iters = it; return out; }
@@ -2735,6 +2748,37 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     double* y64 = r64 + mb.vlen;                      // ... and A x
     if (o.initx_zeros) {
       for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) x64[q] = 0.0;
+    }
+    if (a.plain64) {
+      // the plain FP64 path with Jacobi or no preconditioner: the
+      // reference's Krylov solve from x0 (LIS semantics), over the group
+      double* F64 = a.F64 + (size_t)YG_NF * mb.coff;
+      Precon<double> P64;
+      P64.kind = o.precon;
+      P64.F = F64;
+      if (o.precon == 1) {
+        for (int q = cm.cb + threadIdx.x; q < cm.ce; q += blockDim.x) {
+          F64[2 * q] = 1.0 / A64[(size_t)q * YG_NA + 0];
+          F64[2 * q + 1] = 1.0 / A64[(size_t)q * YG_NA + 1];
+        }
+        __syncthreads();
+      }
+      const SolveOut so = o.solver == 1 ? cg_grp<double, double>(cm, A64, P64, b64, x64, r64, o, false)
+                                        : bicgstab<double, double, true>(cm, A64, P64, b64, x64, r64, o, false);
+      int st = so.status;
+      if (yg_gfail) {
+        st = 2;
+        if (threadIdx.x == 0) __hip_atomic_store(a.group_fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (rank == 0 && threadIdx.x == 0) {
+        a.out_iters[m] = so.iters; a.out_status[m] = st; a.out_resid[m] = so.resid;
+        if (a.timers) {
+          s_tm[5] = yg_gwait;
+          s_tm[7] = wall_clock64() - t_kernel0;
+          for (int k = 0; k < 8; ++k) a.timers[8 * m + k] += s_tm[k];
+        }
+      }
+      continue;
     }
     Precon<float> P;
     if (o.precon == 2) {

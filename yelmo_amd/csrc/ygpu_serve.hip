@@ -683,6 +683,7 @@ struct Server : YgServerBase {
     ka.skip = ps.done; ka.out_iters = B.d_kit + L.s0; ka.out_status = B.d_kst + L.s0;
     ka.out_resid = B.d_kres + L.s0;
     ka.picard_resid = ps.resid;
+    ka.plain64 = B.kmode == YG_MODE_FP64;
     ka.opts = B.opts.k;
     if (B.diva) yg_launch_diva_pre<R>(G, f, B.dv, B.fp, ps.done, st);
     yg_launch_fields_pre<R>(G, f, B.fp, ps.done, st);
