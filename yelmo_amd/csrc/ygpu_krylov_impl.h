@@ -441,41 +441,59 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
   const int capc = factor_cap<TA>(c);
   const int C = max(1, capc / c.maxcnt);
   const int stride = C * c.maxcnt;
-  auto stage = [&](int sb, int t0) {   // the chunk starting at t0 -> stage at offset sb
-    for (int r = 0; r < 2; ++r) {
-      int q, nat, l;
-      if (chunk_cell(c, t0, min(c.T, t0 + C), g + r * (int)blockDim.x, q, nat, l)) {
-        // the record's 18 slot values in 16-byte loads (records are 16-byte aligned)
-        const YG_GLOBAL TA* rp = Ag + (size_t)nat * YG_NA;
-        if constexpr (sizeof(TA) == 4) {
-          typedef float F4 __attribute__((ext_vector_type(4)));
+  // The chunk starting at t0 -> stage at offset sb: the threads s0, s0 + ds,
+  // ... take its slots, two at a time, both slots' loads issued before the
+  // first LDS store waits for them.
+  constexpr int NL = sizeof(TA) == 4 ? 5 : 9;   // 16-byte chunks of a record's 18 values
+  typedef TA V16 __attribute__((ext_vector_type(16 / sizeof(TA))));
+  auto stage = [&](int sb, int t0, int s0, int ds) {
+    const int t1s = min(c.T, t0 + C);
+    for (int base = s0; base < stride; base += 2 * ds) {
+      int q[2], nat[2], l[2];
+      bool ok[2];
+      V16 v[2][NL];
+      uint32_t mk[2];
 #pragma unroll
-          for (int e = 0; e < 5; ++e) {
-            const F4 v4 = reinterpret_cast<const YG_GLOBAL F4*>(rp)[e];
+      for (int k = 0; k < 2; ++k) {
+        ok[k] = chunk_cell(c, t0, t1s, base + k * ds, q[k], nat[k], l[k]);
+        if (ok[k]) {
+          // the record's 18 slot values in 16-byte loads (records are 16-byte aligned)
+          const YG_GLOBAL V16* rp = reinterpret_cast<const YG_GLOBAL V16*>(Ag + (size_t)nat[k] * YG_NA);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-              if (4 * e + u < 18) lds<TA>(sb + (4 * e + u) * stride + l) = v4[u];
-          }
-        } else {
-          typedef double D2 __attribute__((ext_vector_type(2)));
-#pragma unroll
-          for (int e = 0; e < 9; ++e) {
-            const D2 v2 = reinterpret_cast<const YG_GLOBAL D2*>(rp)[e];
-            lds<TA>(sb + (2 * e) * stride + l) = v2[0];
-            lds<TA>(sb + (2 * e + 1) * stride + l) = v2[1];
-          }
+          for (int e = 0; e < NL; ++e) v[k][e] = rp[e];
+          mk[k] = gc(c.mask)[nat[k]];
         }
-        lds<TA>(sb + 18 * stride + l) = mask_to_stage<TA>(gc(c.mask)[nat]);
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!ok[k]) continue;
+        constexpr int E = 16 / sizeof(TA);
+#pragma unroll
+        for (int e = 0; e < NL; ++e)
+#pragma unroll
+          for (int u = 0; u < E; ++u)
+            if (E * e + u < 18) lds<TA>(sb + (E * e + u) * stride + l[k]) = v[k][e][u];
+        lds<TA>(sb + 18 * stride + l[k]) = mask_to_stage<TA>(mk[k]);
       }
     }
   };
   UR o1{}, o2{};  // own cells at steps t-1 and t-2
   const int nch = (c.T + C - 1) / C;
-  stage(0, 0);
+  // blocked scheme with waves that own no lines: those waves stage the next
+  // chunk while the others eliminate this one (no barrier inside a chunk)
+  const int nw = (int)(blockDim.x >> 6);
+  const bool split = !CPL && c.nwl < nw;
+  const bool stager = split && w >= c.nwl;
+  stage(0, 0, g, (int)blockDim.x);
   __syncthreads();
   for (int ch = 0; ch < nch; ++ch) {
     const int t0 = ch * C, t1 = min(c.T, t0 + C);
     const int sb = (ch & 1) * NV * stride;
+    if (stager) {
+      if (ch + 1 < nch) stage(((ch + 1) & 1) * NV * stride, t1, g - c.nwl * 64, (nw - c.nwl) * 64);
+      __syncthreads();
+      continue;
+    }
     for (int t = t0; t < t1; ++t) {
       // natural order: P = line i-1 cell j (its step t-2), Q = line i-1 cell
       // j+1 (its step t-1); reversed: P = line i-1 cell j (its step t-1), Q =
@@ -689,11 +707,14 @@ YG_NOINLINE void ilu0_factor_t(const Ctx& cin, const TA* __restrict__ A, TA* __r
         step_barrier();
       }
     }
-    // the factorization runs once per linear solve: the next chunk is loaded
-    // after the steps, which keeps its 19 values per cell out of the registers
-    // that the recurrence needs
+    if (split) {   // the stagers have loaded the next chunk meanwhile
+      __syncthreads();
+      continue;
+    }
+    // every wave owns lines: the next chunk is loaded after the steps, which
+    // keeps its 19 values per cell out of the registers the recurrence needs
     __syncthreads();
-    if (ch + 1 < nch) stage(((ch + 1) & 1) * NV * stride, t1);
+    if (ch + 1 < nch) stage(((ch + 1) & 1) * NV * stride, t1, g, (int)blockDim.x);
     __syncthreads();
   }
 }
