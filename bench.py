@@ -448,15 +448,30 @@ def run_serve(args):
     budget = int(args.budget_gb * (1 << 30))
     lanes = [tuple(int(v) for v in part.split(":")) for part in args.lanes.split(",")]
     out, res = {}, {}
-    for prec in ("mixed", "fp64"):
+    keep = f" -ygpu_keep_factors {args.keep_factors}"
+    for prec in (("mixed",) if args.serve_mixed_only else ("mixed", "fp64")):
         # the plain FP64 server's Poisson pass (its 641^2 solves on the FP64
         # path take minutes) only on request
         load = args.load if prec == "mixed" or args.serve_fp64_poisson else None
-        out[prec], res[prec] = serve_stream(seqs, SsaParams(precision=prec), lanes, budget,
-                                            load=load)
+        par = SsaParams(precision=prec)
+        if prec == "mixed":
+            par = par.with_(ssa_lis_opt=par.ssa_lis_opt + keep)
+        out[prec], res[prec] = serve_stream(seqs, par, lanes, budget, load=load)
         p99 = out[prec]["poisson"]["request_latency_s"]["p99"] if out[prec].get("poisson") else None
         print(f"# {prec}: capacity {out[prec]['capacity_solves_per_s']:.1f} solves/s, p99 "
               f"{'-' if p99 is None else f'{p99:.2f}s'}", file=sys.stderr, flush=True)
+    if args.serve_mixed_only:
+        mixed = out["mixed"]
+        print(json.dumps(_finite({
+            "metric": "served SSA solves/s and p50/p99 request latency, continuous batching "
+                      "(mixed server only: no FP64 passes, no error reference)",
+            "value": mixed["poisson"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+            "keep_factors": args.keep_factors, "requests": nreq,
+            "capacity_solves_per_s": mixed["capacity_solves_per_s"],
+            "p50_latency_s": mixed["poisson"]["request_latency_s"]["p50"],
+            "p99_latency_s": mixed["poisson"]["request_latency_s"]["p99"], "mixed": mixed})),
+            flush=True)
+        return
     conv = SsaParams(precision="fp64")
     conv = conv.with_(ssa_lis_opt=conv.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
     _, ref = serve_stream(seqs, conv, lanes, budget, load=None)
@@ -644,10 +659,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-baselines", action="store_true",
                     help="skip the full-device FP64 and production-default legs")
-    ap.add_argument("--factor", choices=("fp32", "bf16", "bf16u"), default="fp32",
+    ap.add_argument("--factor", choices=("fp32", "bf16", "bf16u"), default=None,
                     help="storage of the mixed path's ILU(0) factors: bf16 (with the U rows "
                          "scaled by their FP32 inverse diagonals), bf16u (the U part in BF16, "
-                         "the L part in FP32) or fp32")
+                         "the L part in FP32) or fp32; default bf16 for the DIVA batch (its "
+                         "workload names BF16 factors), fp32 otherwise")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the schedule autotuner (use the size rule: ceil(nx/64) line "
                          "blocks)")
@@ -659,6 +675,12 @@ def main():
                          "diva: 48 DIVA domains of uneven size (96..384)^2, nz = 21; "
                          "serve: a served stream of warm-started solve sequences")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
+    ap.add_argument("--keep-factors", type=int, choices=(0, 1), default=0,
+                    help="serve: 1 = a continuation solve starts on the ILU(0) factors its "
+                         "predecessor left on the device (-ygpu_keep_factors 1)")
+    ap.add_argument("--serve-mixed-only", action="store_true",
+                    help="serve: the mixed server's passes only (no FP64 server, no error "
+                         "reference)")
     ap.add_argument("--serve-fp64-poisson", action="store_true",
                     help="serve: also time the plain FP64 server under Poisson arrivals")
     ap.add_argument("--lanes", default="161:48,321:24,641:8",
@@ -670,6 +692,8 @@ def main():
                     help="instead of the benchmark, run a plain device copy of MIB MiB "
                          "five times (the reference kernel for scaling HBM byte counters)")
     args = ap.parse_args()
+    if args.factor is None:
+        args.factor = "bf16" if args.workload == "diva" else "fp32"
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the ensemble members are independent and a "
               f"multi-GPU run would only split them (--gpus {args.gpus} not supported)",

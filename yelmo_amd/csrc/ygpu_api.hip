@@ -34,6 +34,7 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
   o.k.ir_delta = 1e-3;   // reliable updates (-ygpu_ir restart: 0)
   o.k.ir_loose = -1.0;   // automatic: 10 ssa_iter_conv where a Picard loop runs
   o.k.loose_inner = 1e-2;
+  o.k.keep_factors = 0;  // 1: warm-started solves begin on their previous solve's factors
   o.substituted = nullptr;
   o.fields32 = 1;   // the mixed mode's field kernels compute in FP32 unless -ygpu_fields fp64
   o.pattern = 0;
@@ -87,6 +88,11 @@ int parse_opts(const char* s, Opts& o, std::string& err) {
         err = "-ygpu_ir_delta takes a value in (0, 1)";
         return YGPU_ERR_ARG;
       }
+    }
+    else if (k.is("-ygpu_keep_factors")) {
+      if (v.is("1") || v.is("true")) o.k.keep_factors = 1;
+      else if (v.is("0") || v.is("false")) o.k.keep_factors = 0;
+      else { err = "-ygpu_keep_factors takes 1 or 0"; return YGPU_ERR_ARG; }
     }
     else if (k.is("-ygpu_ilu_blocks")) o.k.ilu_blocks = v.to_int();
     else if (k.is("-ygpu_ilu_cols")) o.k.ilu_cols = std::max(1, v.to_int());
@@ -638,8 +644,10 @@ struct YgSessionT : YgSessionBase {
   int solve(ygpu_ssa_problem* P) override {
     int rc;
     if ((rc = B.upload_inputs(P))) return rc;
-    // the device-resident velocity, viscosity and friction are the warm start
+    // the device-resident velocity, viscosity, friction and (mixed mode)
+    // ILU(0) factors are the warm start
     if (first && (rc = B.upload_state(P))) return rc;
+    if ((rc = B.set_warm(first ? 0 : 1))) return rc;
     double secs = 0.0;
     if ((rc = B.run(&secs))) return rc;
     if ((rc = B.download(P))) return rc;

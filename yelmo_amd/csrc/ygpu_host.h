@@ -157,6 +157,7 @@ struct Batch {
   uint16_t* d_F16 = nullptr;   // BF16 line-block factors (-ygpu_factor bf16)
   int *d_done = nullptr, *d_iter_now = nullptr, *d_lin_iter = nullptr, *d_lin_fail = nullptr,
       *d_lim_n = nullptr, *d_kit = nullptr, *d_kst = nullptr, *d_ksw = nullptr;
+  int* d_warm = nullptr;   // per member: factors of the previous solve resident (keep_factors)
   double *d_resid = nullptr, *d_kres = nullptr;
   double* d_epart = nullptr;   // epilogue partial sums, YG_EPI_BLOCKS x 4 per member
   int* d_eticket = nullptr;    // epilogue arrival counters
@@ -449,10 +450,12 @@ struct Batch {
     } else {
       YG_CHECK(h, ar.alloc(&d_F64, (size_t)YG_NF * C));
     }
-    int* ints[8];
-    for (int q = 0; q < 8; ++q) YG_CHECK(h, ar.alloc(&ints[q], nmem));
+    int* ints[9];
+    for (int q = 0; q < 9; ++q) YG_CHECK(h, ar.alloc(&ints[q], nmem));
     d_done = ints[0]; d_iter_now = ints[1]; d_lin_iter = ints[2]; d_lin_fail = ints[3];
     d_lim_n = ints[4]; d_kit = ints[5]; d_kst = ints[6]; d_ksw = ints[7];
+    d_warm = ints[8];
+    YG_CHECK(h, hipMemsetAsync(d_warm, 0, sizeof(int) * nmem, h->stream));
     YG_CHECK(h, ar.alloc(&d_resid, nmem));
     YG_CHECK(h, ar.alloc(&d_kres, nmem));
     YG_CHECK(h, ar.alloc(&d_epart, (size_t)nmem * YG_EPI_BLOCKS * 4));
@@ -698,6 +701,15 @@ struct Batch {
   // device copy of the initial INOUT state, so a resident batch can be
   // solved again from the same start without touching the host
   std::vector<std::pair<R*, R*>> backup;  // (live, saved)
+  // every member's warm flag (keep_factors): 1 when its next solve continues
+  // the previous one on the same cells, so its factors are still in place
+  int set_warm(int v) {
+    std::vector<int> w(nmem, v);
+    YG_CHECK(h, hipMemcpyAsync(d_warm, w.data(), sizeof(int) * nmem, hipMemcpyHostToDevice, h->stream));
+    YG_CHECK(h, hipStreamSynchronize(h->stream));
+    return YGPU_OK;
+  }
+
   int save_initial() {
     std::vector<std::pair<R*, long long>> st = {
         {f.ux, total_cells}, {f.uy, total_cells}, {f.visc, total_z}, {f.beta, total_cells},
@@ -919,6 +931,7 @@ struct Batch {
     ka.skip = d_done; ka.out_iters = d_kit; ka.out_status = d_kst; ka.out_resid = d_kres;
     ka.out_sweeps = d_ksw;
     ka.picard_resid = d_resid;
+    ka.warm = d_warm; ka.iter_now = d_iter_now;
     ka.plain64 = kmode == YG_MODE_FP64;
     ka.opts = opts.k;
     ka.group = group; ka.nmembers = nmem; ka.gsync = d_gsync; ka.group_fail = d_gfail;

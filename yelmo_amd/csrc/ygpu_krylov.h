@@ -50,6 +50,9 @@ struct YgKrylovOpts {
                     // solves to loose_inner; the others (and every member when 0) to
                     // ir_target with inner solves to inner_tol
   double loose_inner;
+  int keep_factors; // mixed mode, ILU(0): 1 = a member flagged in YgKrylovArgs::warm uses
+                    // the line-block factors left by its previous solve in the first
+                    // Picard iteration of the next one instead of factorizing anew
   double ir_delta;  // mixed mode, BiCGSTAB: > 0 (1e-3 by default) one FP32 BiCGSTAB
                     // whose residual is replaced by the FP64 one (b - A x in FP64, x
                     // accumulated in FP64) whenever the iterated residual fell below
@@ -80,6 +83,10 @@ struct YgKrylovArgs {
   int* out_sweeps;          // optional (mixed mode): FP64 residual passes of the refinement
   const double* picard_resid;   // optional (mixed mode): per member Picard residual of the
                                 // last iteration (opts.ir_loose)
+  const int* warm;          // optional (mixed mode, opts.keep_factors): per member, nonzero
+                            // when its factors from the previous solve of its sequence are
+                            // resident; they stand in while iter_now of the member is 0
+  const int* iter_now;      // per member Picard iterations finished in the current solve
   long long* timers;        // optional: per member 8 phase totals (wall clock ticks)
   YgKrylovOpts opts;
   // member groups (yg_launch_krylov_group): workgroups per member, members,

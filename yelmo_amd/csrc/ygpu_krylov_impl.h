@@ -2397,6 +2397,17 @@ YG_DEV SolveOut cg(const Ctx& c, const TA* A, const Precon<TV>& P, const TV* b, 
   return out;
 }
 
+// Preconditioner residency: a member whose previous solve of the same
+// sequence left its ILU(0) factors in place (warm flag) uses them in the
+// first Picard iteration of the next solve, whose matrix differs from the
+// last one factorized by one thickness update; later iterations factorize
+// their own matrices.  Mixed mode only (the FP64 path factorizes every call,
+// as the reference's solver does).
+YG_DEV bool yg_keep_factors(const YgKrylovArgs& a, int m) {
+  return a.opts.keep_factors && a.opts.precon == 2 && a.warm && a.iter_now && a.warm[m] != 0 &&
+         a.iter_now[m] == 0;
+}
+
 template <typename T>
 YG_DEV void build_precon(const Ctx& c, const T* A, T* F, Precon<T>& P, int kind, int* bad) {
   P.kind = kind;
@@ -2539,10 +2550,12 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
       __syncthreads();
     }
     Precon<float> P;
-    build_precon<float>(c, A32, F, P, o.precon, &s_bad);
+    const bool keepf = yg_keep_factors(a, m);
+    if (keepf) { P.kind = 2; P.F = F; }   // the factors of the sequence's previous solve
+    else build_precon<float>(c, A32, F, P, o.precon, &s_bad);
     if (o.factor16 && P.kind == 2 && !c.coupled && a.F16) {
       uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff;
-      factors_to_bf16(c, F, FB);
+      if (!keepf) factors_to_bf16(c, F, FB);
       P.FB = FB;
       P.fb_l = o.factor16 == 1;
     }
@@ -2803,12 +2816,15 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     }
     Precon<float> P;
     if (o.precon == 2) {
+      // a warm-started member's first Picard iteration keeps the factors of
+      // its sequence's previous solve (same schedule, same cells)
+      const bool keepf = yg_keep_factors(a, m);
       P.kind = 2;
       P.F = F + (size_t)8 * cm.cb;   // this part's L factors (rotated tile order)
-      if (Lg > 0) ilu0_factor<float>(cr, A32 + (size_t)YG_NA * cm.cb, F + (size_t)8 * cm.cb, &s_bad);
+      if (Lg > 0 && !keepf) ilu0_factor<float>(cr, A32 + (size_t)YG_NA * cm.cb, F + (size_t)8 * cm.cb, &s_bad);
       if (o.factor16 && a.F16) {
         uint16_t* FB = a.F16 + (size_t)YG_NF16 * mb.coff + (size_t)8 * cm.cb;
-        if (Lg > 0) factors_to_bf16(cr, P.F, FB);
+        if (Lg > 0 && !keepf) factors_to_bf16(cr, P.F, FB);
         P.FB = FB;
         P.fb_l = o.factor16 == 1;
       }

@@ -232,7 +232,7 @@ struct Server : YgServerBase {
   int* d_sel = nullptr;
   long long* d_seg = nullptr;
   int* d_segm = nullptr;
-  std::vector<int> h_zero_i, h_idle;
+  std::vector<int> h_zero_i, h_idle, h_warm;
   std::vector<double> h_zero_d;
 
   int64_t device_bytes() const override { return (int64_t)B.ar.bytes; }
@@ -334,6 +334,7 @@ struct Server : YgServerBase {
       for (int k = 0; k < L.ns; ++k) slots[L.s0 + k].lane = (int)(&L - lanes.data());
     h_idle.assign(S, 3);
     h_zero_i.assign(S, 0);
+    h_warm.assign(S, 0);
     h_zero_d.assign(S, 0.0);
     // every slot starts idle: skipped by every kernel
     YG_CHECK(h, hipMemcpyAsync(B.d_done, h_idle.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
@@ -506,6 +507,9 @@ struct Server : YgServerBase {
     // fresh Picard counters
     YG_CHECK(h, hipMemcpyAsync(B.d_done + s, &h_zero_i[s], sizeof(int), hipMemcpyHostToDevice, st));
     YG_CHECK(h, hipMemcpyAsync(B.d_iter_now + s, &h_zero_i[s], sizeof(int), hipMemcpyHostToDevice, st));
+    // a continuation finds its previous solve's factors in the slot
+    h_warm[s] = r.first ? 0 : 1;
+    YG_CHECK(h, hipMemcpyAsync(B.d_warm + s, &h_warm[s], sizeof(int), hipMemcpyHostToDevice, st));
     YG_CHECK(h, hipMemcpyAsync(B.d_lin_iter + s, &h_zero_i[s], sizeof(int), hipMemcpyHostToDevice, st));
     YG_CHECK(h, hipMemcpyAsync(B.d_lin_fail + s, &h_zero_i[s], sizeof(int), hipMemcpyHostToDevice, st));
     YG_CHECK(h, hipMemcpyAsync(B.d_resid + s, &h_zero_d[s], sizeof(double), hipMemcpyHostToDevice, st));
@@ -683,6 +687,7 @@ struct Server : YgServerBase {
     ka.skip = ps.done; ka.out_iters = B.d_kit + L.s0; ka.out_status = B.d_kst + L.s0;
     ka.out_resid = B.d_kres + L.s0;
     ka.picard_resid = ps.resid;
+    ka.warm = B.d_warm + L.s0; ka.iter_now = ps.iter_now;
     ka.plain64 = B.kmode == YG_MODE_FP64;
     ka.opts = B.opts.k;
     if (B.diva) yg_launch_diva_pre<R>(G, f, B.dv, B.fp, ps.done, st);
