@@ -336,8 +336,12 @@ def _serve_run(srv, seqs, arrivals, record):
             tn = time.perf_counter() - t0
             solve_lat.append(tn - t_submit.pop(t))
             t_done.append(tn)
-            info = srv.fetch(t, seqs[k][j])
-            record(k, j, seqs[k][j], info)
+            # results land in a scratch copy: the request objects stay as
+            # submitted (cold), so passes and repeated requests can share them
+            q = seqs[k][j].copy()
+            info = srv.fetch(t, q)
+            record(k, j, q, info)
+            del q
             if j + 1 < len(seqs[k]):
                 t2 = srv.submit(seqs[k][j + 1], k, cont=True, last=j + 2 == len(seqs[k]))
                 owner[t2] = (k, j + 1)
@@ -349,23 +353,21 @@ def _serve_run(srv, seqs, arrivals, record):
     return req_lat, np.array(solve_lat), time.perf_counter() - t0, peak, steps, t90
 
 
-def _fresh(seqs):
-    """Copies of the request sequences: every pass starts from the
-    requests' own cold initial state (fetched results land in the copies)."""
-    return [[p.copy() for p in s] for s in seqs]
-
-
 def _vel(p):
     return np.concatenate([p.state["ux_b"].ravel(), p.state["uy_b"].ravel()]).astype(np.float64)
 
 
-def serve_stream(seqs, par, lanes, budget, load=None, timed=True):
+def serve_stream(seqs, par, lanes, budget, load=None, ref=None, key=None):
     """Serves the request sequences with continuous batching.  A closed-loop
     pass (every request queued at the start) gives the capacity in solves/s
     (completion rate up to 90 % of the solves); with ``load``, a second pass
-    on fresh copies has requests arrive Poisson at that fraction of the
-    capacity and gives the latencies.  Returns (figures, results) with
-    results[(k, j)] = (velocity, Picard iterations, status) of every solve."""
+    has requests arrive Poisson at that fraction of the capacity and gives
+    the latencies.  Every pass submits the requests' own cold inputs.
+    Returns (figures, results) with results[(k, j)] = (velocity, Picard
+    iterations, status) of every solve; with ``ref`` (reference results of
+    the distinct sequences, request k being sequence key[k]) each solve is
+    compared as it finishes and results[(k, j)] = (rel L2 from the
+    reference, Picard iterations, status, reference Picard iterations)."""
     from yelmo_amd.serve import Server
     nreq = len(seqs)
     nsolves = sum(len(s) for s in seqs)
@@ -378,10 +380,19 @@ def serve_stream(seqs, par, lanes, budget, load=None, timed=True):
 
     def recorder(store):
         def rec(k, j, p, info):
-            store[(k, j)] = (_vel(p), info.ssa_iter_now, info.status)
+            if ref is None:
+                store[(k, j)] = (_vel(p), info.ssa_iter_now, info.status)
+            else:
+                r = ref[(key[k], j)]
+                v = _vel(p)
+                store[(k, j)] = (rel_l2(v, r[0]) if np.isfinite(v).all() else np.inf,
+                                 info.ssa_iter_now, info.status, r[1])
         return rec
 
-    lat_c, slat_c, span_c, peak_c, steps_c, t90_c = _serve_run(srv, _fresh(seqs), np.zeros(nreq),
+    def failed(v):
+        return v[2] != 0 or not (np.isfinite(v[0]).all() if ref is None else np.isfinite(v[0]))
+
+    lat_c, slat_c, span_c, peak_c, steps_c, t90_c = _serve_run(srv, seqs, np.zeros(nreq),
                                                                recorder(res_closed))
     capacity = 0.9 * nsolves / t90_c
     out = {"capacity_solves_per_s": capacity,
@@ -390,15 +401,13 @@ def serve_stream(seqs, par, lanes, budget, load=None, timed=True):
                            "request_latency_s": {"p50": float(np.percentile(lat_c, 50)),
                                                  "p99": float(np.percentile(lat_c, 99))}},
            "server_device_bytes": srv.device_bytes,
-           "failed_solves": sum(1 for v in res_closed.values()
-                                if v[2] != 0 or not np.isfinite(v[0]).all())}
+           "failed_solves": sum(1 for v in res_closed.values() if failed(v))}
     if load is not None:
         from yelmo_amd import generator as g
         rate = load * capacity / g.SERVE_SOLVES
         rng = np.random.default_rng(50_000)
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-        lat, slat, span, peak, steps, _ = _serve_run(srv, _fresh(seqs), arrivals,
-                                                     recorder(res_poisson))
+        lat, slat, span, peak, steps, _ = _serve_run(srv, seqs, arrivals, recorder(res_poisson))
         out["poisson"] = {"load": load, "arrival_rate_req_per_s": rate,
                           "solves_per_s": nsolves / span, "makespan_s": span,
                           "peak_running": peak, "steps": steps,
@@ -409,16 +418,24 @@ def serve_stream(seqs, par, lanes, budget, load=None, timed=True):
         # the same requests served under another arrival pattern: a solve
         # lands at another place of the device arena, which changes the
         # chunking of its dot products (rounding), nothing else
-        out["max_rel_l2_poisson_vs_closed_loop"] = max(
-            rel_l2(res_poisson[key][0], res_closed[key][0]) for key in res_closed)
+        if ref is None:
+            out["max_rel_l2_poisson_vs_closed_loop"] = max(
+                rel_l2(res_poisson[kj][0], res_closed[kj][0]) for kj in res_closed)
+        else:
+            out["poisson"]["vs_reference"] = serve_errors(res_poisson, None)
+            out["poisson"]["failed_solves"] = sum(1 for v in res_poisson.values() if failed(v))
     srv.close()
     return out, res_closed
 
 
 def serve_errors(res, ref):
     """Per-solve relative L2 distance of served velocities from the
-    reference run's, the worst solve and its Picard counts."""
-    errs = [(rel_l2(v, ref[key][0]), key, it, ref[key][1]) for key, (v, it, _) in res.items()]
+    reference run's, the worst solve and its Picard counts (ref None: the
+    results already hold the distances and reference counts)."""
+    if ref is None:
+        errs = [(e, kj, it, rit) for kj, (e, it, _, rit) in res.items()]
+    else:
+        errs = [(rel_l2(v, ref[kj][0]), kj, it, ref[kj][1]) for kj, (v, it, _) in res.items()]
     w = max(errs, key=lambda a: a[0])
     return {"max_rel_l2": w[0], "median_rel_l2": float(np.median([e[0] for e in errs])),
             "worst_solve": {"request": w[1][0], "solve": w[1][1], "picard": w[2],
@@ -432,52 +449,76 @@ def run_serve(args):
     between them, warm-started from the previous solve's device-resident
     state.  Per precision (mixed, plain FP64 at the reference's rtol 1e-8):
     the capacity from a closed-loop pass, then p50/p99 latency with Poisson
-    arrivals at --load of that capacity; every pass serves fresh copies of
-    the requests, so every first solve starts cold.  The error reference is
-    the FP64 server with the linear solves converged to rtol 1e-12."""
+    arrivals at --load of that capacity; every pass submits the requests'
+    cold inputs.  The error reference is the FP64 server with the linear
+    solves converged to rtol 1e-12, run first over the distinct sequences.
+    With --distinct D (fewer than --requests) the requests are drawn
+    (seeded, uniformly) from a pool of D generated sequences, so that a
+    stream of thousands of requests needs the host memory of D."""
     from yelmo_amd import generator
     from yelmo_amd.params import SsaParams
     nreq = args.requests
-    sides = generator.serve_request_sides(nreq, seed=0)
+    ndist = args.distinct if 0 < args.distinct < nreq else nreq
+    sides = generator.serve_request_sides(ndist, seed=0)
     t_gen = time.perf_counter()
-    seqs = [generator.served_sequence(int(s), 1000 + k, generator.SERVE_SOLVES)
+    pool = [generator.served_sequence(int(s), 1000 + k, generator.SERVE_SOLVES)
             for k, s in enumerate(sides)]
-    print(f"# generated {nreq} requests x {generator.SERVE_SOLVES} solves in "
-          f"{time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    key = (np.arange(nreq) if ndist == nreq
+           else np.random.default_rng(4000).integers(0, ndist, size=nreq))
+    seqs = [pool[int(k)] for k in key]
+    print(f"# generated {ndist} distinct sequences x {generator.SERVE_SOLVES} solves in "
+          f"{time.perf_counter() - t_gen:.1f}s; {nreq} requests", file=sys.stderr, flush=True)
     nsolves = sum(len(s) for s in seqs)
     budget = int(args.budget_gb * (1 << 30))
     lanes = [tuple(int(v) for v in part.split(":")) for part in args.lanes.split(",")]
+    ref = None
+    if not args.serve_mixed_only:
+        conv = SsaParams(precision="fp64")
+        conv = conv.with_(ssa_lis_opt=conv.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
+        t_ref = time.perf_counter()
+        _, ref = serve_stream(pool, conv, lanes, budget, load=None)
+        print(f"# converged FP64 reference of {ndist} sequences in "
+              f"{time.perf_counter() - t_ref:.1f}s", file=sys.stderr, flush=True)
     out, res = {}, {}
     keep = f" -ygpu_keep_factors {args.keep_factors}"
-    for prec in (("mixed",) if args.serve_mixed_only else ("mixed", "fp64")):
+    fp64_pool = args.serve_fp64 == "pool" and ndist < nreq
+    precs = ("mixed",) if args.serve_mixed_only or args.serve_fp64 == "none" else ("mixed", "fp64")
+    for prec in precs:
         # the plain FP64 server's Poisson pass (its 641^2 solves on the FP64
-        # path take minutes) only on request
+        # path take minutes) only on request; with --serve-fp64 pool its
+        # closed-loop pass serves the distinct sequences once each
         load = args.load if prec == "mixed" or args.serve_fp64_poisson else None
         par = SsaParams(precision=prec)
         if prec == "mixed":
             par = par.with_(ssa_lis_opt=par.ssa_lis_opt + keep)
-        out[prec], res[prec] = serve_stream(seqs, par, lanes, budget, load=load)
+        if prec == "fp64" and fp64_pool:
+            out[prec], res[prec] = serve_stream(pool, par, lanes, budget, load=load, ref=ref,
+                                                key=np.arange(ndist))
+            out[prec]["requests_served"] = f"the {ndist} distinct sequences once each"
+        else:
+            out[prec], res[prec] = serve_stream(seqs, par, lanes, budget, load=load, ref=ref,
+                                                key=key)
+        if ref is not None:
+            out[prec]["vs_fp64_converged"] = serve_errors(res[prec], None)
         p99 = out[prec]["poisson"]["request_latency_s"]["p99"] if out[prec].get("poisson") else None
         print(f"# {prec}: capacity {out[prec]['capacity_solves_per_s']:.1f} solves/s, p99 "
               f"{'-' if p99 is None else f'{p99:.2f}s'}", file=sys.stderr, flush=True)
+        # the pass's figures at once, should a later pass not finish
+        print(f"# {prec} figures: {json.dumps(_finite(out[prec]))}", file=sys.stderr, flush=True)
     if args.serve_mixed_only:
         mixed = out["mixed"]
         print(json.dumps(_finite({
             "metric": "served SSA solves/s and p50/p99 request latency, continuous batching "
                       "(mixed server only: no FP64 passes, no error reference)",
             "value": mixed["poisson"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
-            "keep_factors": args.keep_factors, "requests": nreq,
+            "keep_factors": args.keep_factors, "requests": nreq, "distinct": ndist,
             "capacity_solves_per_s": mixed["capacity_solves_per_s"],
             "p50_latency_s": mixed["poisson"]["request_latency_s"]["p50"],
             "p99_latency_s": mixed["poisson"]["request_latency_s"]["p99"], "mixed": mixed})),
             flush=True)
         return
-    conv = SsaParams(precision="fp64")
-    conv = conv.with_(ssa_lis_opt=conv.ssa_lis_opt.replace("1.0e-8", "1.0e-12"))
-    _, ref = serve_stream(seqs, conv, lanes, budget, load=None)
-    out["mixed"]["vs_fp64_converged"] = serve_errors(res["mixed"], ref)
-    out["fp64"]["vs_fp64_converged"] = serve_errors(res["fp64"], ref)
     mixed = out["mixed"]
+    fp64 = out.get("fp64", {"capacity_solves_per_s": float("nan"), "failed_solves": 0})
     line = {
         "metric": "served SSA solves/s and p50/p99 request latency, continuous batching",
         "value": mixed["poisson"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
@@ -488,17 +529,20 @@ def run_serve(args):
         "config": {"workload": f"{nreq} requests x {generator.SERVE_SOLVES} warm-started SSA "
                                f"solves, side lognormal (median 161) in [81, 641], 1536 km "
                                f"across, Poisson arrivals at {args.load:.0%} of the measured "
-                               f"capacity, mixed FP32/FP64",
-                   "requests": nreq, "sides": [int(s) for s in sides], "lanes": args.lanes,
+                               f"capacity, mixed FP32/FP64" +
+                               ("" if ndist == nreq else
+                                f"; requests drawn from {ndist} distinct sequences"),
+                   "requests": nreq, "distinct_sequences": ndist,
+                   "sides": [int(s) for s in sides], "lanes": args.lanes,
                    "hbm_budget_gb": args.budget_gb},
         "p50_latency_s": mixed["poisson"]["request_latency_s"]["p50"],
         "p99_latency_s": mixed["poisson"]["request_latency_s"]["p99"],
         # against the FP64 server with linear solves converged to rtol 1e-12
         "max_rel_l2_vs_fp64": mixed["vs_fp64_converged"]["max_rel_l2"],
         "error_ok": mixed["vs_fp64_converged"]["max_rel_l2"] <= 1e-6,
-        "failed_solves": mixed["failed_solves"] + out["fp64"]["failed_solves"],
-        "mixed": mixed, "fp64": out["fp64"],
-        "speedup_capacity_vs_fp64": mixed["capacity_solves_per_s"] / out["fp64"]["capacity_solves_per_s"],
+        "failed_solves": mixed["failed_solves"] + fp64["failed_solves"],
+        "mixed": mixed, "fp64": out.get("fp64"),
+        "speedup_capacity_vs_fp64": mixed["capacity_solves_per_s"] / fp64["capacity_solves_per_s"],
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(_finite(line)), flush=True)
@@ -675,6 +719,12 @@ def main():
                          "diva: 48 DIVA domains of uneven size (96..384)^2, nz = 21; "
                          "serve: a served stream of warm-started solve sequences")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
+    ap.add_argument("--serve-fp64", choices=("full", "pool", "none"), default="full",
+                    help="serve: the plain FP64 server's closed-loop pass over every request "
+                         "(full), over the distinct sequences once each (pool) or not at all")
+    ap.add_argument("--distinct", type=int, default=0,
+                    help="serve: draw the requests from this many distinct generated sequences "
+                         "(0: every request its own)")
     ap.add_argument("--keep-factors", type=int, choices=(0, 1), default=0,
                     help="serve: 1 = a continuation solve starts on the ILU(0) factors its "
                          "predecessor left on the device (-ygpu_keep_factors 1)")
