@@ -438,6 +438,8 @@ def serve_errors(res, ref):
         errs = [(rel_l2(v, ref[kj][0]), kj, it, ref[kj][1]) for kj, (v, it, _) in res.items()]
     w = max(errs, key=lambda a: a[0])
     return {"max_rel_l2": w[0], "median_rel_l2": float(np.median([e[0] for e in errs])),
+            "p99_rel_l2": float(np.percentile([e[0] for e in errs], 99)),
+            "solves_above_1e-6": sum(1 for e in errs if e[0] > 1e-6), "solves": len(errs),
             "worst_solve": {"request": w[1][0], "solve": w[1][1], "picard": w[2],
                             "picard_reference": w[3], "rel_l2": w[0]}}
 
@@ -505,6 +507,16 @@ def run_serve(args):
               f"{'-' if p99 is None else f'{p99:.2f}s'}", file=sys.stderr, flush=True)
         # the pass's figures at once, should a later pass not finish
         print(f"# {prec} figures: {json.dumps(_finite(out[prec]))}", file=sys.stderr, flush=True)
+    for extra in args.serve_extra:
+        # further mixed-precision servers (closed loop) with extra options,
+        # against the same reference
+        par = SsaParams(precision="mixed")
+        par = par.with_(ssa_lis_opt=par.ssa_lis_opt + keep + " " + extra)
+        o, _ = serve_stream(seqs, par, lanes, budget, load=None, ref=ref, key=key)
+        if ref is not None:
+            o["vs_fp64_converged"] = serve_errors(_, None)
+        out.setdefault("mixed_variants", {})[extra] = o
+        print(f"# mixed {extra!r} figures: {json.dumps(_finite(o))}", file=sys.stderr, flush=True)
     if args.serve_mixed_only:
         mixed = out["mixed"]
         print(json.dumps(_finite({
@@ -541,7 +553,7 @@ def run_serve(args):
         "max_rel_l2_vs_fp64": mixed["vs_fp64_converged"]["max_rel_l2"],
         "error_ok": mixed["vs_fp64_converged"]["max_rel_l2"] <= 1e-6,
         "failed_solves": mixed["failed_solves"] + fp64["failed_solves"],
-        "mixed": mixed, "fp64": out.get("fp64"),
+        "mixed": mixed, "fp64": out.get("fp64"), "mixed_variants": out.get("mixed_variants"),
         "speedup_capacity_vs_fp64": mixed["capacity_solves_per_s"] / fp64["capacity_solves_per_s"],
         "roofline": None, "cpu_baseline": None,
     }
@@ -719,6 +731,9 @@ def main():
                          "diva: 48 DIVA domains of uneven size (96..384)^2, nz = 21; "
                          "serve: a served stream of warm-started solve sequences")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
+    ap.add_argument("--serve-extra", action="append", default=[],
+                    help="serve: an extra closed-loop pass of the mixed server with these "
+                         "options appended (repeatable), against the same reference")
     ap.add_argument("--serve-fp64", choices=("full", "pool", "none"), default="full",
                     help="serve: the plain FP64 server's closed-loop pass over every request "
                          "(full), over the distinct sequences once each (pool) or not at all")

@@ -84,8 +84,8 @@ struct YgKrylovArgs {
   const double* picard_resid;   // optional (mixed mode): per member Picard residual of the
                                 // last iteration (opts.ir_loose)
   const int* warm;          // optional (mixed mode, opts.keep_factors): per member, nonzero
-                            // when its factors from the previous solve of its sequence are
-                            // resident; they stand in while iter_now of the member is 0
+                            // when its solve continues a sequence, whose previous solve's
+                            // factors then stand in while iter_now of the member is 0
   const int* iter_now;      // per member Picard iterations finished in the current solve
   long long* timers;        // optional: per member 8 phase totals (wall clock ticks)
   YgKrylovOpts opts;

@@ -2408,6 +2408,12 @@ YG_DEV bool yg_keep_factors(const YgKrylovArgs& a, int m) {
          a.iter_now[m] == 0;
 }
 
+// Loose refinement (opts.ir_loose): a member whose last Picard residual is
+// above ir_loose refines only to the requested tolerance.
+YG_DEV bool yg_loose(const YgKrylovArgs& a, int m) {
+  return a.opts.ir_loose > 0.0 && a.picard_resid && a.picard_resid[m] > a.opts.ir_loose;
+}
+
 template <typename T>
 YG_DEV void build_precon(const Ctx& c, const T* A, T* F, Precon<T>& P, int kind, int* bad) {
   P.kind = kind;
@@ -2567,7 +2573,7 @@ __global__ __launch_bounds__(NT) void yg_krylov_kernel(YgKrylovArgs a) {
     // ir_loose) solves to the tolerance itself, with looser inner solves: the
     // Picard iteration contracts that error away before it converges.  The
     // reported status refers to the requested tolerance.
-    const bool loose = o.ir_loose > 0.0 && a.picard_resid && a.picard_resid[m] > o.ir_loose;
+    const bool loose = yg_loose(a, m);
     YgKrylovOpts oi = o;
     oi.tol = loose ? o.loose_inner : o.inner_tol;
     oi.conv_cond = 0;
@@ -2844,7 +2850,7 @@ __global__ __launch_bounds__(NT, 2) void yg_krylov_group_kernel(YgKrylovArgs a) 
     double bad[1] = {(double)s_bad};
     csum<1>(cm, bad);   // a zero pivot on any rank
     // a member far from Picard convergence solves to the tolerance itself
-    const bool loose = o.ir_loose > 0.0 && a.picard_resid && a.picard_resid[m] > o.ir_loose;
+    const bool loose = yg_loose(a, m);
     YgKrylovOpts oi = o;
     oi.tol = loose ? o.loose_inner : o.inner_tol;
     oi.conv_cond = 0;
